@@ -1,0 +1,185 @@
+/*
+ * jfem_hip.h -- C ABI of libjfem_hip.so, the MI355X (gfx950) implementation of
+ * JFEM's solve path for tensor-product Qk spaces: structured space and DOF
+ * numbering, TPCellIntegral assembly into CSR, Dirichlet elimination, the
+ * sparse matrix-vector product, CG / BiCGStab / GMRES and the geometric
+ * multigrid preconditioner.
+ *
+ * Conventions (mirroring the reference's Java API and its JNI sparse solver,
+ * JSparse/src/com/simonschmidt/SparseSolver.java:22-46):
+ *  - every entry point returns a jfem_status (0 = success); failures never
+ *    throw across the ABI and jfem_last_error() returns the message;
+ *  - host arrays are caller-owned and only read/written during the call;
+ *  - jfem_vec / jfem_mat / jfem_mg handles own device (HBM) memory, so solves
+ *    run with operands resident on the GPU;
+ *  - one context per host thread; calls on one context are not re-entrant
+ *    (as the reference's IterativeSolver instances are not).
+ * Reference paths are relative to the JFEM repository, M/ =
+ * Core/src/main/java/.
+ */
+#ifndef JFEM_HIP_H
+#define JFEM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    JFEM_OK = 0,
+    JFEM_ERR_ARG = 1,          /* IllegalArgumentException in the reference */
+    JFEM_ERR_STATE = 2,        /* IllegalStateException */
+    JFEM_ERR_UNSUPPORTED = 3,  /* UnsupportedOperationException */
+    JFEM_ERR_DEVICE = 4,       /* HIP runtime failure */
+    JFEM_ERR_NOMEM = 5
+} jfem_status;
+
+/* integral kinds: TPCellIntegral names (M/tensorproduct/TPCellIntegral.java:20-24) */
+enum { JFEM_GRAD_GRAD = 0, JFEM_VALUE_VALUE = 1, JFEM_H1 = 2 };
+
+/* quadrature rules: QuadratureRule1D (M/tensorproduct/QuadratureRule1D.java:11-64);
+ * 1..5 = Gauss1..Gauss5 (Gauss5 is TPCellIntegral's default), 12..16 = GaussLobatto2..6 */
+enum { JFEM_GAUSS5 = 5 };
+
+/* smoothers */
+enum { JFEM_SM_RICHARDSON = 0, JFEM_SM_JACOBI = 1, JFEM_SM_CHEBYSHEV = 2 };
+
+/* precision of the multigrid preconditioner */
+enum { JFEM_FP64 = 0, JFEM_FP32 = 1 };
+
+typedef struct jfem_ctx jfem_ctx;
+typedef struct jfem_space jfem_space;
+typedef struct jfem_vec jfem_vec;
+typedef struct jfem_mat jfem_mat;
+typedef struct jfem_mg jfem_mg;
+
+const char *jfem_last_error(void);
+
+/* ---- context ------------------------------------------------------------ */
+/* no reference counterpart: owns the HIP stream and reduction scratch */
+int jfem_ctx_create(int device, jfem_ctx **out);
+int jfem_ctx_destroy(jfem_ctx *ctx);
+int jfem_ctx_sync(jfem_ctx *ctx);
+/* event timer on the context's stream: start, then stop returns milliseconds */
+int jfem_timer_start(jfem_ctx *ctx);
+int jfem_timer_stop(jfem_ctx *ctx, double *ms);
+
+/* ---- space: new ContinuousTPFESpace(start, end, cells) + assembleFunctions(k)
+ *      (M/tensorproduct/ContinuousTPFESpace.java:17-50) -------------------- */
+int jfem_space_create(jfem_ctx *ctx, int dim, const int *cells, const double *start, const double *end,
+                      int degree, jfem_space **out);
+int jfem_space_destroy(jfem_space *s);
+long jfem_space_ndofs(const jfem_space *s);
+/* global index of every (cell, local function): cells in IntCoordinates order,
+ * local index with axis 0 fastest; out has ncells*(k+1)^dim ints */
+int jfem_space_cell_dofs(jfem_space *s, int *out);
+/* Dirichlet nodes of the whole boundary, ascending (getBoundaryNodes,
+ * M/basic/AcceptsMatrixBoundaryValues.java:46-60); out has ndofs capacity */
+int jfem_space_boundary_nodes(jfem_space *s, int *out, long *count);
+/* node point of every DOF (LagrangeNodeFunctional point), out[dof*dim + a] */
+int jfem_space_node_coords(jfem_space *s, double *out);
+
+/* ---- vectors (DenseVector) ---------------------------------------------- */
+int jfem_vec_create(jfem_ctx *ctx, long n, jfem_vec **out);
+int jfem_vec_destroy(jfem_vec *v);
+int jfem_vec_upload(jfem_vec *v, const double *host);
+int jfem_vec_download(const jfem_vec *v, double *host);
+int jfem_vec_fill(jfem_vec *v, double value);
+/* DenseVector.inner (M/linalg/DenseVector.java:103-121) */
+int jfem_vec_dot(const jfem_vec *x, const jfem_vec *y, double *out);
+/* y = a*x + b*y */
+int jfem_vec_axpby(double a, const jfem_vec *x, double b, jfem_vec *y);
+
+/* ---- assembly: MatrixFESpace.evaluateCellIntegrals / writeCellIntegralsToMatrix
+ *      (M/basic/MatrixFESpace.java:22-42,115-139) with TPCellIntegral
+ *      (M/tensorproduct/TPCellIntegral.java:111-159) into a CSR whose pattern is
+ *      the key set of SparseMatrix.getCoordinateEntryList()
+ *      (M/linalg/SparseMatrix.java:402-418).  kinds/weights: nint integrals
+ *      summed per entry; cell_coeff (nullable, ncells host values) multiplies
+ *      every integral's weight on that cell. ---------------------------------- */
+int jfem_assemble(jfem_space *s, int nint, const int *kinds, const double *weights, const double *cell_coeff,
+                  int quad_rule, jfem_mat **out);
+/* TPRightHandSideIntegral VALUE (M/tensorproduct/TPRightHandSideIntegral.java:22-31)
+ * of f(x) = sum_j amp[j] prod_a sin(kw[j*dim+a] pi x_a), into rhs (overwritten) */
+int jfem_assemble_rhs_sines(jfem_space *s, int nterms, const int *kw, const double *amp, int quad_rule,
+                            jfem_vec *rhs);
+
+/* ---- matrices (SparseMatrix / VectorMultiplyable) ----------------------- */
+int jfem_mat_from_csr(jfem_ctx *ctx, long rows, long cols, const long *rowptr, const int *colidx,
+                      const double *vals, jfem_mat **out);
+int jfem_mat_destroy(jfem_mat *A);
+int jfem_mat_info(const jfem_mat *A, long *rows, long *cols, long *nnz);
+int jfem_mat_export_csr(const jfem_mat *A, long *rowptr, int *colidx, double *vals);
+/* setBoundaryValues -> overWriteValue (M/basic/AcceptsMatrixBoundaryValues.java:18-44):
+ * rhs_j -= A_ji g_i for free j, row/column i zeroed (entries kept), A_ii = 1,
+ * rhs_i = g_i.  values nullable (zero Dirichlet data). */
+int jfem_apply_dirichlet(jfem_mat *A, jfem_vec *rhs, long nb, const int *nodes, const double *values);
+/* VectorMultiplyable.mvMul (SparseMatrix.mvMul, M/linalg/SparseMatrix.java:479-489) */
+int jfem_spmv(jfem_mat *A, const jfem_vec *x, jfem_vec *y);
+/* y = A x with FP32 matrix values and FP32 vectors (device pointers to float) */
+int jfem_spmv_f32(jfem_mat *A, const float *x, float *y);
+/* repeat SpMV `reps` times on the context stream and return the mean kernel time (ms) */
+int jfem_spmv_bench(jfem_mat *A, const jfem_vec *x, jfem_vec *y, int reps, int fp32, double *ms);
+
+/* ---- Krylov solvers (IterativeSolver, M/linalg/IterativeSolver.java) ---- */
+typedef struct {
+    int iterations;   /* IterativeSolver.iterations */
+    int fell_back;    /* BiCGStab breakdown fell back to (P)GMRES */
+    double resnorm;   /* final ||b - A x||_2 (true residual) */
+    double seconds;   /* wall time of the solve */
+} jfem_solve_info;
+
+/* solveCG / solvePCG (:48-162): stop when ||r||_2 <= tol (absolute), at most
+ * min(n, maxit) iterations; x holds the start iterate on entry (the reference
+ * default is e_0) and the solution on return.  M nullable (NULL = CG).
+ * res_hist (nullable, maxit entries) receives ||r|| per iteration. */
+int jfem_solve_cg(jfem_mat *A, jfem_mg *M, const jfem_vec *b, jfem_vec *x, double tol, long maxit,
+                  jfem_solve_info *info, double *res_hist);
+/* solveBiCGStab / solvePBiCGStab (:234-402) */
+int jfem_solve_bicgstab(jfem_mat *A, jfem_mg *M, const jfem_vec *b, jfem_vec *x, double tol, long maxit,
+                        jfem_solve_info *info);
+/* GMRES.solve (M/linalg/GMRES.java:40-176): restarted, left preconditioned,
+ * classical Gram-Schmidt, Givens; restart = ITERATIONS_BEFORE_RESTART (150) */
+int jfem_solve_gmres(jfem_mat *A, jfem_mg *M, const jfem_vec *b, jfem_vec *x, double tol, int restart,
+                     int max_cycles, jfem_solve_info *info);
+/* flexible CG (Notay 2000, Polak-Ribiere beta) in FP64 around a multigrid
+ * preconditioner of any precision; not in the reference. */
+int jfem_solve_fcg(jfem_mat *A, jfem_mg *M, const jfem_vec *b, jfem_vec *x, double tol, long maxit,
+                   jfem_solve_info *info, double *res_hist);
+
+/* ---- geometric multigrid (MGPreconditionerInterface / MGPreconditionerSpace,
+ *      M/multigrid/MGPreconditionerInterface.java:125-379) -------------------- */
+typedef struct {
+    int smoother;      /* JFEM_SM_* */
+    int runs;          /* Richardson/Jacobi runs, or Chebyshev polynomial applications */
+    double omega;      /* Richardson/Jacobi relaxation */
+    int degree;        /* Chebyshev degree */
+    double cheb_ratio; /* Chebyshev interval [lmax/ratio, lmax] on D^-1 A */
+    double cheb_safety;/* lmax = safety * (power-iteration estimate) */
+    int power_its;     /* power iterations for the estimate */
+    int cycles;        /* V-cycles per application (reference: 3) */
+    int fmg;           /* 1: reference mvMul (copy BC, full-V correction, cycles V) */
+    int precision;     /* JFEM_FP64 or JFEM_FP32 */
+} jfem_mg_opts;
+
+/* levels: nlev spaces, coarsest first, each with twice the cells of the
+ * previous one; every level is assembled with the same integrals and zero
+ * Dirichlet data on the whole boundary (createSystem per level).  cell_coeff
+ * (nullable) is the finest level's per-cell coefficient; coarser levels use
+ * the mean of their 2^dim children. */
+int jfem_mg_create(jfem_space *const *spaces, int nlev, int nint, const int *kinds, const double *weights,
+                   const double *cell_coeff, int quad_rule, const jfem_mg_opts *opts, jfem_mg **out);
+int jfem_mg_destroy(jfem_mg *mg);
+/* the finest level's (Dirichlet-eliminated) system matrix, owned by the hierarchy */
+int jfem_mg_level_matrix(jfem_mg *mg, int level, jfem_mat **out);
+/* MGPreconditionerInterface.mvMul: z = M r */
+int jfem_mg_apply(jfem_mg *mg, const jfem_vec *r, jfem_vec *z);
+/* one V-cycle mgStepV(finest, x, b) updating x in place */
+int jfem_mg_vcycle(jfem_mg *mg, jfem_vec *x, const jfem_vec *b);
+/* Chebyshev interval upper bound used on each level (after safety factor) */
+int jfem_mg_lmax(jfem_mg *mg, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,901 @@
+// Structured Qk space (ContinuousTPFESpace) and CSR assembly of TPCellIntegral
+// GRAD_GRAD / VALUE_VALUE / H1 on gfx950.
+//
+// Element matrices: the reference evaluates every element entry by tensor
+// Gauss quadrature in a fixed floating-point order
+// (TPCellIntegral.integrateNonTensorProduct, M/tensorproduct/TPCellIntegral.java:78-102,
+// with ContinuousTPShapeFunction.fastGradientInCell :173-198).  That value only
+// depends, per axis, on the 1D cell's quadrature weights and basis tables, so
+// 1D cells with bit-identical tables form a class and one element matrix per
+// class tuple reproduces the reference's arithmetic for every cell of the tuple
+// (computed on the GPU with contraction disabled, operation for operation).
+//
+// CSR: one group of lanes per row (row-owner gather, deterministic, no
+// atomics).  The columns of row i are the lattice points of the cells around
+// node i; in the reference's first-appearance numbering their ascending order
+// is: owner cell in lexicographic order, then rank inside the owner cell, so
+// the sorted column list is enumerated directly without sorting.  An entry
+// exists iff some cell contribution is nonzero (MatrixFESpace.addToMatrix
+// skips exact zeros, M/basic/MatrixFESpace.java:115-122) or it is the dummy
+// (0,0) entry of a new SparseMatrix (M/linalg/SparseMatrix.java:95-102);
+// values are summed over cells in lexicographic cell order.
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+
+namespace jfem {
+
+// ------------------------------- host tables -------------------------------
+
+// QuadratureRule1D (M/tensorproduct/QuadratureRule1D.java:11-92)
+#pragma clang fp contract(off)
+static int quad_rule(int rule, double* pts, double* wts)
+{
+    double p[6], w[6];
+    int n = 0;
+    switch (rule) {
+    case 1: n = 1; w[0] = 2; p[0] = 0; break;
+    case 2: n = 2; w[0] = 1; w[1] = 1; p[0] = -std::sqrt(1. / 3); p[1] = std::sqrt(1. / 3); break;
+    case 3: n = 3; w[0] = 5. / 9; w[1] = 8. / 9; w[2] = 5. / 9;
+        p[0] = -std::sqrt(3. / 5); p[1] = 0; p[2] = std::sqrt(3. / 5); break;
+    case 4: n = 4;
+        w[0] = (18. - std::sqrt(30.)) / 36; w[1] = (18. + std::sqrt(30.)) / 36;
+        w[2] = (18. + std::sqrt(30.)) / 36; w[3] = (18. - std::sqrt(30.)) / 36;
+        p[0] = -std::sqrt(3. / 7 + 2. / 7 * std::sqrt(6. / 5)); p[1] = -std::sqrt(3. / 7 - 2. / 7 * std::sqrt(6. / 5));
+        p[2] = std::sqrt(3. / 7 - 2. / 7 * std::sqrt(6. / 5));  p[3] = std::sqrt(3. / 7 + 2. / 7 * std::sqrt(6. / 5));
+        break;
+    case 5: n = 5;
+        w[0] = (322.0 - 13 * std::sqrt(70.)) / 900; w[1] = (322.0 + 13 * std::sqrt(70.)) / 900;
+        w[2] = 128.0 / 225;
+        w[3] = (322.0 + 13 * std::sqrt(70.)) / 900; w[4] = (322.0 - 13 * std::sqrt(70.)) / 900;
+        p[0] = -std::sqrt(5.0 + 2 * std::sqrt(10.0 / 7)) / 3; p[1] = -std::sqrt(5.0 - 2 * std::sqrt(10.0 / 7)) / 3;
+        p[2] = 0;
+        p[3] = std::sqrt(5.0 - 2 * std::sqrt(10.0 / 7)) / 3;  p[4] = std::sqrt(5.0 + 2 * std::sqrt(10.0 / 7)) / 3;
+        break;
+    case 12: n = 2; w[0] = 1. / 2; w[1] = 1. / 2; p[0] = -1; p[1] = 1; break;
+    case 13: n = 3; w[0] = 1. / 3; w[1] = 4. / 3; w[2] = 1. / 3; p[0] = -1; p[1] = 0; p[2] = 1; break;
+    case 14: n = 4; w[0] = 1. / 6; w[1] = 5. / 6; w[2] = 5. / 6; w[3] = 1. / 6;
+        p[0] = -1; p[1] = -std::sqrt(1. / 5); p[2] = std::sqrt(1. / 5); p[3] = 1; break;
+    case 15: n = 5; w[0] = 1. / 10; w[1] = 49. / 90; w[2] = 32. / 45; w[3] = 49. / 90; w[4] = 1. / 10;
+        p[0] = -1; p[1] = -std::sqrt(3. / 7); p[2] = 0; p[3] = std::sqrt(3. / 7); p[4] = 1; break;
+    case 16: n = 6;
+        w[0] = 1. / 15; w[1] = (14. - std::sqrt(7.)) / 30; w[2] = (14. + std::sqrt(7.)) / 30;
+        w[3] = (14. + std::sqrt(7.)) / 30; w[4] = (14. - std::sqrt(7.)) / 30; w[5] = 1. / 15;
+        p[0] = -1; p[1] = -std::sqrt(1. / 3 + 2 * std::sqrt(7.) / 21); p[2] = -std::sqrt(1. / 3 - 2 * std::sqrt(7.) / 21);
+        p[3] = +std::sqrt(1. / 3 - 2 * std::sqrt(7.) / 21); p[4] = +std::sqrt(1. / 3 + 2 * std::sqrt(7.) / 21); p[5] = 1;
+        break;
+    default: return -1;
+    }
+    for (int i = 0; i < n; i++) { pts[i] = p[i] / 2 + 0.5; wts[i] = w[i] / 2; }
+    return n;
+}
+
+// LagrangeBasisFunction1D.valueOnReferenceCell / derivativeOnReferenceCell
+// (M/tensorproduct/LagrangeBasisFunction1D.java:55-154)
+static double lag_value(double pos, int l, int k)
+{
+    if (k == 1) return l == 0 ? 1.0 - pos : pos;
+    if (k == 2) {
+        if (l == 0) return 1.0 - 3.0 * pos + 2.0 * pos * pos;
+        if (l == 1) return (pos - pos * pos) * 4.0;
+        return 2.0 * pos * pos - pos;
+    }
+    double ret = 1;
+    for (int i = 0; i <= k; i++)
+        if (i != l) ret *= (pos - 1. * i / k) / (1. * l / k - 1. * i / k);
+    return ret;
+}
+
+static double lag_deriv(double pos, int l, int k)
+{
+    if (k == 1) return l == 0 ? -1.0 : 1.0;
+    if (k == 2) {
+        if (l == 0) return -3.0 + 4.0 * pos;
+        if (l == 1) return (1.0 - 2.0 * pos) * 4.0;
+        return 4.0 * pos - 1.0;
+    }
+    double derret = 0;
+    for (int j = 0; j <= k; j++) {
+        if (j == l) continue;
+        double ret = 1. / (1. * l / k - 1. * j / k);
+        for (int i = 0; i <= k; i++)
+            if (i != l && i != j) ret *= (pos - 1. * i / k) / (1. * l / k - 1. * i / k);
+        derret += ret;
+    }
+    return derret;
+}
+
+// Cell1D.isInCell with PerformanceArguments.doubleTolerance = 1e-10 (Cell1D.java:55-59)
+static bool in_cell(double s, double e, double pos) { return pos >= s - 100 * 1e-10 && pos <= e + 100 * 1e-10; }
+#pragma clang fp contract(on)
+
+// 1D table of one cell: weights w[q] = refw*len, value[l][q], deriv[l][q]
+constexpr int MAXQ = 6;
+constexpr int MAXK1 = 5;   // k <= 4
+struct Tab1D {
+    double w[MAXQ];
+    double val[MAXK1][MAXQ];
+    double der[MAXK1][MAXQ];
+};
+
+#pragma clang fp contract(off)
+static Tab1D make_tab(double s, double e, int k, const double* qp, const double* qw, int Q)
+{
+    Tab1D t;
+    std::memset(&t, 0, sizeof(t));
+    const double len = e - s;
+    for (int q = 0; q < Q; q++) {
+        const double pos = qp[q] * len + s;   // Cell1D.distributeQuadrature :29-30
+        t.w[q] = qw[q] * len;
+        for (int l = 0; l <= k; l++) {
+            t.val[l][q] = in_cell(s, e, pos) ? lag_value((pos - s) / len, l, k) : 0;
+            t.der[l][q] = in_cell(s, e, pos) ? lag_deriv((pos - s) / len, l, k) / len : 0;
+        }
+    }
+    return t;
+}
+#pragma clang fp contract(on)
+
+// ------------------------------- kernels ----------------------------------
+
+struct ElemParams {
+    int d, k, Q, nloc, nint;
+    int kinds[4];
+    double wconst[4];
+    int ncls[MAXD];
+    long ntuples;
+};
+
+// E[(i*ntuples + t)*nloc*nloc + iv*nloc + iu] = integral i on class tuple t
+// between u = shapeFunction1 (local iu) and v = shapeFunction2 (local iv).
+// Operation order follows integrateNonTensorProduct exactly (no contraction).
+__global__ void k_element_tables(ElemParams P, const Tab1D* __restrict__ tabs0, const Tab1D* __restrict__ tabs1,
+                                 const Tab1D* __restrict__ tabs2, double* __restrict__ E)
+{
+#pragma clang fp contract(off)
+    const long nent = (long)P.nloc * P.nloc;
+    const long total = nent * P.ntuples * P.nint;
+    for (long id = blockIdx.x * (long)blockDim.x + threadIdx.x; id < total; id += (long)gridDim.x * blockDim.x) {
+        const int iu = (int)(id % P.nloc);
+        const int iv = (int)((id / P.nloc) % P.nloc);
+        const long t = (id / nent) % P.ntuples;
+        const int ii = (int)(id / (nent * P.ntuples));
+        int cls[MAXD] = {0, 0, 0};
+        long r = t;
+        for (int a = P.d - 1; a >= 0; a--) { cls[a] = (int)(r % P.ncls[a]); r /= P.ncls[a]; }
+        const Tab1D* tb[MAXD] = {tabs0 + cls[0], P.d > 1 ? tabs1 + cls[1] : nullptr, P.d > 2 ? tabs2 + cls[2] : nullptr};
+        int lu[MAXD], lv[MAXD];
+        int ru = iu, rv = iv;
+        for (int a = 0; a < P.d; a++) { lu[a] = ru % (P.k + 1); ru /= P.k + 1; lv[a] = rv % (P.k + 1); rv /= P.k + 1; }
+        const int kind = P.kinds[ii];
+        const double wc = P.wconst[ii];
+        int nq = 1;
+        for (int a = 0; a < P.d; a++) nq *= P.Q;
+        double ret = 0;
+        for (int q = 0; q < nq; q++) {
+            int c[MAXD];
+            int qq = q;
+            for (int a = P.d - 1; a >= 0; a--) { c[a] = qq % P.Q; qq /= P.Q; }   // last axis fastest
+            double val;
+            double vu = 1, vv = 1;
+            if (kind != 0) {
+                for (int a = 0; a < P.d; a++) { vu *= tb[a]->val[lu[a]][c[a]]; vv *= tb[a]->val[lv[a]][c[a]]; }
+            }
+            if (kind == 1) {
+                val = vu * vv * wc;
+            } else {
+                double inner = 0;
+                for (int i = 0; i < P.d; i++) {
+                    double g1 = 1, g2 = 1;
+                    for (int j = 0; j < P.d; j++) {
+                        if (i == j) { g1 *= tb[j]->der[lu[j]][c[j]]; g2 *= tb[j]->der[lv[j]][c[j]]; }
+                        else { g1 *= tb[j]->val[lu[j]][c[j]]; g2 *= tb[j]->val[lv[j]][c[j]]; }
+                    }
+                    inner += g1 * g2;
+                }
+                val = kind == 0 ? inner * wc : (vu * vv + inner) * wc;
+            }
+            for (int a = 0; a < P.d; a++) val *= tb[a]->w[c[a]];
+            ret += val;
+        }
+        E[id] = ret;
+    }
+}
+
+struct RowParams {
+    GridDev g;
+    int nloc, nint;
+    long ntuples;
+    const int* cls[MAXD];    // class id of each 1D cell per axis
+    int ncls[MAXD];
+    const double* E;         // element tables
+    const double* coeff;     // per-cell coefficient or null
+};
+
+// per-axis candidate segments of a row: owner cell, first lattice point, count
+struct AxisSegs {
+    int nseg;
+    int owner[3], first[3], cnt[3];
+    int W;              // total lattice points
+    int clo, chi;       // cells containing the row's node
+};
+
+__device__ inline void axis_segments(int p, int k, int n, AxisSegs& s)
+{
+    int lo, hi;
+    if (p % k == 0) { lo = p / k - 1; hi = p / k; } else { lo = hi = p / k; }
+    if (lo < 0) lo = 0;
+    if (hi > n - 1) hi = n - 1;
+    s.clo = lo; s.chi = hi;
+    s.nseg = 0;
+    if (lo > 0) { s.owner[s.nseg] = lo - 1; s.first[s.nseg] = k * lo; s.cnt[s.nseg] = 1; s.nseg++;
+                  s.owner[s.nseg] = lo; s.first[s.nseg] = k * lo + 1; s.cnt[s.nseg] = k; s.nseg++; }
+    else { s.owner[s.nseg] = 0; s.first[s.nseg] = 0; s.cnt[s.nseg] = k + 1; s.nseg++; }
+    if (hi > lo) { s.owner[s.nseg] = hi; s.first[s.nseg] = k * hi + 1; s.cnt[s.nseg] = k; s.nseg++; }
+    s.W = k * (hi - lo + 1) + 1;
+}
+
+// decode candidate index t of a row into its column lattice point q
+__device__ inline void decode_candidate(int d, const AxisSegs* S, int t, int* q)
+{
+    int tailW[MAXD];
+    int tw = 1;
+    for (int a = d - 1; a >= 0; a--) { tailW[a] = tw; tw *= S[a].W; }
+    int rem = t, M = 1, seg[MAXD];
+    for (int a = 0; a < d; a++) {
+        int s = 0;
+        for (; s < S[a].nseg - 1; s++) {
+            const int block = M * S[a].cnt[s] * tailW[a];
+            if (rem < block) break;
+            rem -= block;
+        }
+        seg[a] = s;
+        M *= S[a].cnt[s];
+    }
+    // rem indexes inside the owner box, axis 0 fastest; strip the tail factor
+    // (all later-axis segments were already resolved, so tailW contributions are 1)
+    int stride = 1;
+    for (int a = 0; a < d; a++) {
+        const int c = S[a].cnt[seg[a]];
+        q[a] = S[a].first[seg[a]] + (rem / stride) % c;
+        stride *= c;
+    }
+}
+
+// value of entry (row node p, column node q) summed over the cells containing both
+__device__ inline double entry_value(const RowParams& P, const int* p, const int* q, bool& present)
+{
+    const int d = P.g.d, k = P.g.k;
+    int lo[MAXD], cnt[MAXD];
+    for (int a = 0; a < d; a++) {
+        // cells containing p_a and q_a
+        int plo, phi, qlo, qhi;
+        if (p[a] % k == 0) { plo = p[a] / k - 1; phi = p[a] / k; } else { plo = phi = p[a] / k; }
+        if (q[a] % k == 0) { qlo = q[a] / k - 1; qhi = q[a] / k; } else { qlo = qhi = q[a] / k; }
+        int l0 = max(max(plo, qlo), 0), h0 = min(min(phi, qhi), P.g.n[a] - 1);
+        lo[a] = l0;
+        cnt[a] = h0 - l0 + 1;
+    }
+    int ncell = 1;
+    for (int a = 0; a < d; a++) ncell *= cnt[a] > 0 ? cnt[a] : 0;
+    double sum = 0;
+    present = false;
+    const long nent = (long)P.nloc * P.nloc;
+    for (int ci = 0; ci < ncell; ci++) {
+        int c[MAXD];
+        int r = ci;
+        for (int a = d - 1; a >= 0; a--) { c[a] = lo[a] + r % cnt[a]; r /= cnt[a]; }   // lexicographic
+        long tup = 0, cell = 0;
+        int iv = 0, iu = 0, mul = 1;
+        for (int a = 0; a < d; a++) {
+            tup = tup * P.ncls[a] + P.cls[a][c[a]];
+            cell = cell * P.g.n[a] + c[a];
+            iv += (p[a] - k * c[a]) * mul;
+            iu += (q[a] - k * c[a]) * mul;
+            mul *= k + 1;
+        }
+        double integral = 0;
+        for (int i = 0; i < P.nint; i++) {
+            const double e = P.E[(i * P.ntuples + tup) * nent + (long)iv * P.nloc + iu];
+            integral += P.coeff ? P.coeff[cell] * e : e;
+        }
+        if (integral != 0) {
+            sum = integral + sum;
+            present = true;
+        }
+    }
+    return sum;
+}
+
+// G lanes per row; FILL = false counts, FILL = true writes cols/vals
+template <int G, bool FILL>
+__global__ __launch_bounds__(256) void k_assemble_rows(RowParams P, long nrows, int* __restrict__ counts,
+                                                      const int* __restrict__ rowptr, int* __restrict__ cols,
+                                                      double* __restrict__ vals)
+{
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x % G;
+    const int wlane = threadIdx.x & 63;
+    const int gbase = wlane - lane;   // first lane of the group inside the wavefront
+    const long groups_total = (long)gridDim.x * (blockDim.x / G);
+    const long gi = (blockIdx.x * (long)blockDim.x + threadIdx.x) / G;
+    const long wave_first = gi - wlane / G;   // first group of this wavefront
+    for (long it = 0; wave_first + it < nrows; it += groups_total) {
+        const long row = gi + it;
+        // all lanes of a wavefront iterate together so that ballots stay uniform
+        const bool active = row < nrows;
+        AxisSegs S[MAXD];
+        int p[MAXD] = {0, 0, 0};
+        int total = 0;
+        if (active) {
+            lattice_of_dof(P.g, row, p);
+            total = 1;
+            for (int a = 0; a < P.g.d; a++) { axis_segments(p[a], P.g.k, P.g.n[a], S[a]); total *= S[a].W; }
+        }
+        // uniform trip count across the wavefront
+        int my_chunks = (total + G - 1) / G;
+        int chunks = my_chunks;
+        for (int o = 32; o > 0; o >>= 1) chunks = max(chunks, __shfl_xor(chunks, o, 64));
+        int written = 0;
+        const int base = (FILL && active) ? rowptr[row] : 0;
+        for (int ch = 0; ch < chunks; ch++) {
+            const int t = ch * G + lane;
+            bool present = false;
+            double v = 0;
+            int colv = 0;
+            if (active && t < total) {
+                int q[MAXD] = {0, 0, 0};
+                decode_candidate(P.g.d, S, t, q);
+                v = entry_value(P, p, q, present);
+                colv = (int)dof_of_lattice(P.g, q);
+                if (row == 0 && colv == 0) present = true;   // dummy (0,0) entry
+            }
+            const unsigned long long bal = __ballot(present);
+            const unsigned long long gmask = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << gbase);
+            const unsigned long long mine = bal & gmask;
+            if (FILL && present) {
+                const unsigned long long below = mine & ((1ull << wlane) - 1ull);
+                const int pos = base + written + __popcll(below);
+                cols[pos] = colv;
+                vals[pos] = v;
+            }
+            written += __popcll(mine);
+        }
+        if (!FILL && active && lane == 0) counts[row] = written;
+    }
+}
+
+// --------------------------- scan (int64 totals) ---------------------------
+constexpr int SCAN_BS = 256;
+constexpr int SCAN_PER = 8;
+constexpr int SCAN_TILE = SCAN_BS * SCAN_PER;
+
+__device__ inline long block_excl_scan(long v, long* sh, long& total)
+{
+    // inclusive scan in wavefront, then across wavefronts
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    long x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const long y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long acc = 0;
+        for (int i = 0; i < SCAN_BS / 64; i++) { const long t = sh[i]; sh[i] = acc; acc += t; }
+        sh[SCAN_BS / 64] = acc;
+    }
+    __syncthreads();
+    total = sh[SCAN_BS / 64];
+    const long r = x - v + sh[w];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(SCAN_BS) void k_scan_tiles(const int* __restrict__ in, long n, long* __restrict__ out,
+                                                        long* __restrict__ tile_sums)
+{
+    __shared__ long sh[SCAN_BS / 64 + 1];
+    const long base = (long)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER;
+    long loc[SCAN_PER];
+    long s = 0;
+    for (int i = 0; i < SCAN_PER; i++) { loc[i] = (base + i < n) ? in[base + i] : 0; s += loc[i]; }
+    long total;
+    long off = block_excl_scan(s, sh, total);
+    for (int i = 0; i < SCAN_PER; i++) { if (base + i < n) out[base + i] = off; off += loc[i]; }
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(SCAN_BS) void k_scan_sums(long* __restrict__ sums, long m, long* __restrict__ grand)
+{
+    __shared__ long sh[SCAN_BS / 64 + 1];
+    long carry = 0;
+    for (long b = 0; b < m; b += SCAN_BS) {
+        const long i = b + threadIdx.x;
+        const long v = i < m ? sums[i] : 0;
+        long total;
+        const long e = block_excl_scan(v, sh, total);
+        if (i < m) sums[i] = e + carry;
+        carry += total;
+    }
+    if (threadIdx.x == 0) *grand = carry;
+}
+
+__global__ void k_scan_finish(const long* __restrict__ tmp, const long* __restrict__ sums, long n, int* __restrict__ rowptr)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        rowptr[i] = (int)(tmp[i] + sums[i / SCAN_TILE]);
+}
+
+// rowptr[0..n] from counts[0..n-1]; returns total (host)
+static long scan_counts(jfem_ctx* c, const int* counts, long n, int* rowptr)
+{
+    const long tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    long *tmp = nullptr, *sums = nullptr;
+    JFEM_HIP(hipMalloc(&tmp, sizeof(long) * (n > 0 ? n : 1)));
+    JFEM_HIP(hipMalloc(&sums, sizeof(long) * (tiles + 2)));
+    if (n > 0) k_scan_tiles<<<tiles, SCAN_BS, 0, c->stream>>>(counts, n, tmp, sums);
+    k_scan_sums<<<1, SCAN_BS, 0, c->stream>>>(sums, tiles, sums + tiles + 1);
+    long total = 0;
+    JFEM_HIP(hipMemcpyAsync(&total, sums + tiles + 1, sizeof(long), hipMemcpyDeviceToHost, c->stream));
+    JFEM_HIP(hipStreamSynchronize(c->stream));
+    if (total < 0 || total >= (1l << 31) - 1) {
+        (void)hipFree(tmp); (void)hipFree(sums);
+        throw Error{JFEM_ERR_UNSUPPORTED, "matrix has 2^31 or more entries (int32 CSR indices)"};
+    }
+    if (n > 0) k_scan_finish<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(tmp, sums, n, rowptr);
+    const int tot = (int)total;
+    JFEM_HIP(hipMemcpyAsync(rowptr + n, &tot, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    JFEM_HIP(hipStreamSynchronize(c->stream));
+    (void)hipFree(tmp);
+    (void)hipFree(sums);
+    return total;
+}
+
+// ------------------------- space helper kernels ----------------------------
+
+__global__ void k_cell_dofs(GridDev g, long ncells, int nloc, int* __restrict__ out)
+{
+    for (long id = blockIdx.x * (long)blockDim.x + threadIdx.x; id < ncells * nloc; id += (long)gridDim.x * blockDim.x) {
+        const long cell = id / nloc;
+        int li = (int)(id % nloc);
+        int c[MAXD], p[MAXD];
+        long r = cell;
+        for (int a = g.d - 1; a >= 0; a--) { c[a] = (int)(r % g.n[a]); r /= g.n[a]; }
+        for (int a = 0; a < g.d; a++) { p[a] = c[a] * g.k + li % (g.k + 1); li /= g.k + 1; }
+        out[id] = (int)dof_of_lattice(g, p);
+    }
+}
+
+__global__ void k_boundary_mask(GridDev g, long n, unsigned char* __restrict__ mask)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        int p[MAXD];
+        lattice_of_dof(g, i, p);
+        bool b = false;
+        for (int a = 0; a < g.d; a++) b = b || p[a] == 0 || p[a] == g.k * g.n[a];
+        mask[i] = b ? 1 : 0;
+    }
+}
+
+struct Coord1D { double s[MAXD]; double e[MAXD]; };
+
+// node coordinates: (1.0/k * l) * len + start of the owner cell (LagrangeBasisFunction1D ctor :19)
+__global__ void k_node_coords(GridDev g, long n, const double* __restrict__ s0, const double* __restrict__ s1,
+                              const double* __restrict__ s2, const double* __restrict__ e0,
+                              const double* __restrict__ e1, const double* __restrict__ e2, double* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    const double* S[MAXD] = {s0, s1, s2};
+    const double* Ee[MAXD] = {e0, e1, e2};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        int p[MAXD];
+        lattice_of_dof(g, i, p);
+        for (int a = 0; a < g.d; a++) {
+            const int c = p[a] == 0 ? 0 : (p[a] - 1) / g.k;
+            const int l = p[a] - c * g.k;
+            out[i * g.d + a] = 1.0 / g.k * l * (Ee[a][c] - S[a][c]) + S[a][c];
+        }
+    }
+}
+
+// ------------------------------ RHS (sines) --------------------------------
+// TPRightHandSideIntegral VALUE: per cell, ret_v = sum_q (phi_v(x_q) f(x_q)) w_0 .. w_{d-1};
+// rhs[dof] += ret_v.  Cells of one parity colour share no node, so 2^d passes
+// accumulate without atomics (summation order per DOF: colour order).
+struct RhsParams {
+    GridDev g;
+    int Q, nloc, nterms;
+    const int* cls[MAXD];
+    int ncls[MAXD];
+    const double* sinq[MAXD];   // sin(kw * pi * x_q) per axis: [term][cell][q]
+};
+
+__global__ __launch_bounds__(128) void k_rhs_color(RhsParams P, const Tab1D* __restrict__ t0, const Tab1D* __restrict__ t1,
+                                                   const Tab1D* __restrict__ t2, int color, long ncolor_cells,
+                                                   double* __restrict__ rhs)
+{
+#pragma clang fp contract(off)
+    __shared__ double fq[MAXQ * MAXQ * MAXQ];
+    const int d = P.g.d;
+    int half[MAXD];
+    for (int a = 0; a < d; a++) half[a] = (P.g.n[a] - ((color >> a) & 1) + 1) / 2;
+    for (long cid = blockIdx.x; cid < ncolor_cells; cid += gridDim.x) {
+        int c[MAXD];
+        long r = cid;
+        for (int a = d - 1; a >= 0; a--) { c[a] = 2 * (int)(r % half[a]) + ((color >> a) & 1); r /= half[a]; }
+        int nq = 1;
+        for (int a = 0; a < d; a++) nq *= P.Q;
+        __syncthreads();
+        for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+            int qa[MAXD];
+            int qq = q;
+            for (int a = d - 1; a >= 0; a--) { qa[a] = qq % P.Q; qq /= P.Q; }
+            double s = 0;
+            for (int m = 0; m < P.nterms; m++) {
+                double t = 1;   // amplitude folded into the axis-0 table
+                for (int a = 0; a < d; a++) t *= P.sinq[a][((long)m * P.g.n[a] + c[a]) * P.Q + qa[a]];
+                s += t;
+            }
+            fq[q] = s;
+        }
+        __syncthreads();
+        const Tab1D* tb[MAXD] = {t0 + P.cls[0][c[0]], d > 1 ? t1 + P.cls[1][c[1]] : nullptr,
+                                 d > 2 ? t2 + P.cls[2][c[2]] : nullptr};
+        for (int iv = threadIdx.x; iv < P.nloc; iv += blockDim.x) {
+            int lv[MAXD], rr = iv, p[MAXD];
+            for (int a = 0; a < d; a++) { lv[a] = rr % (P.g.k + 1); rr /= P.g.k + 1; p[a] = c[a] * P.g.k + lv[a]; }
+            double ret = 0;
+            for (int q = 0; q < nq; q++) {
+                int qa[MAXD];
+                int qq = q;
+                for (int a = d - 1; a >= 0; a--) { qa[a] = qq % P.Q; qq /= P.Q; }
+                double v = 1;
+                for (int a = 0; a < d; a++) v *= tb[a]->val[lv[a]][qa[a]];
+                double val = v * fq[q];
+                for (int a = 0; a < d; a++) val *= tb[a]->w[qa[a]];
+                ret += val;
+            }
+            const long dof = dof_of_lattice(P.g, p);
+            if (ret != 0) rhs[dof] += ret;
+        }
+    }
+}
+
+// ------------------------------ host driver --------------------------------
+
+struct ClassTables {
+    int Q = 0;
+    int ncls[MAXD] = {1, 1, 1};
+    std::vector<int> cls[MAXD];        // per 1D cell
+    std::vector<Tab1D> tabs[MAXD];     // per class
+    std::vector<double> s1d[MAXD], e1d[MAXD];
+};
+
+// CartesianGrid.createCells1D (M/tensorproduct/geometry/CartesianGrid.java:26-49)
+#pragma clang fp contract(off)
+static void cells_1d(double start, double end, int n, std::vector<double>& s, std::vector<double>& e)
+{
+    const double width = (end - start) / n;
+    s.resize(n); e.resize(n);
+    for (int j = 0; j < n; j++) { s[j] = j * width + start; e[j] = (j + 1) * width + start; }
+}
+#pragma clang fp contract(on)
+
+static ClassTables build_classes(const Grid& g, int rule)
+{
+    ClassTables C;
+    double qp[8], qw[8];
+    C.Q = quad_rule(rule, qp, qw);
+    JFEM_CHECK(C.Q > 0, JFEM_ERR_ARG, "unknown quadrature rule");
+    JFEM_CHECK(g.k >= 1 && g.k < MAXK1, JFEM_ERR_UNSUPPORTED, "polynomial degree must be 1..4");
+    for (int a = 0; a < g.d; a++) {
+        cells_1d(g.start[a], g.end[a], g.n[a], C.s1d[a], C.e1d[a]);
+        std::map<std::string, int> seen;
+        C.cls[a].resize(g.n[a]);
+        for (int j = 0; j < g.n[a]; j++) {
+            Tab1D t = make_tab(C.s1d[a][j], C.e1d[a][j], g.k, qp, qw, C.Q);
+            std::string key(reinterpret_cast<const char*>(&t), sizeof(t));
+            auto it = seen.find(key);
+            if (it == seen.end()) {
+                const int id = (int)C.tabs[a].size();
+                seen.emplace(key, id);
+                C.tabs[a].push_back(t);
+                C.cls[a][j] = id;
+            } else {
+                C.cls[a][j] = it->second;
+            }
+        }
+        C.ncls[a] = (int)C.tabs[a].size();
+    }
+    return C;
+}
+
+struct DevTables {
+    Tab1D* tabs[MAXD] = {nullptr, nullptr, nullptr};
+    int* cls[MAXD] = {nullptr, nullptr, nullptr};
+    ~DevTables()
+    {
+        for (int a = 0; a < MAXD; a++) { (void)hipFree(tabs[a]); (void)hipFree(cls[a]); }
+    }
+};
+
+static void upload_classes(jfem_ctx* c, const Grid& g, const ClassTables& C, DevTables& D)
+{
+    for (int a = 0; a < g.d; a++) {
+        JFEM_HIP(hipMalloc(&D.tabs[a], sizeof(Tab1D) * C.tabs[a].size()));
+        JFEM_HIP(hipMemcpyAsync(D.tabs[a], C.tabs[a].data(), sizeof(Tab1D) * C.tabs[a].size(), hipMemcpyHostToDevice, c->stream));
+        JFEM_HIP(hipMalloc(&D.cls[a], sizeof(int) * g.n[a]));
+        JFEM_HIP(hipMemcpyAsync(D.cls[a], C.cls[a].data(), sizeof(int) * g.n[a], hipMemcpyHostToDevice, c->stream));
+    }
+}
+
+int choose_lanes(double mean_row)
+{
+    if (mean_row <= 6) return 4;
+    if (mean_row <= 12) return 8;
+    if (mean_row <= 24) return 16;
+    if (mean_row <= 48) return 32;
+    return 64;
+}
+
+jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const double* weights, const double* coeff_host,
+                          int rule)
+{
+    jfem_ctx* c = s->ctx;
+    const Grid& g = s->grid;
+    JFEM_CHECK(nint >= 1 && nint <= 4, JFEM_ERR_ARG, "1..4 integrals supported");
+    for (int i = 0; i < nint; i++)
+        JFEM_CHECK(kinds[i] >= 0 && kinds[i] <= 2, JFEM_ERR_UNSUPPORTED, "unknown integral name");
+    ClassTables C = build_classes(g, rule);
+    DevTables D;
+    upload_classes(c, g, C, D);
+    ElemParams EP;
+    std::memset(&EP, 0, sizeof(EP));
+    EP.d = g.d; EP.k = g.k; EP.Q = C.Q; EP.nloc = g.nloc(); EP.nint = nint;
+    EP.ntuples = 1;
+    for (int a = 0; a < g.d; a++) { EP.ncls[a] = C.ncls[a]; EP.ntuples *= C.ncls[a]; }
+    for (int i = 0; i < nint; i++) { EP.kinds[i] = kinds[i]; EP.wconst[i] = weights[i]; }
+    const long Esize = (long)nint * EP.ntuples * EP.nloc * EP.nloc;
+    JFEM_CHECK(Esize < (1l << 30), JFEM_ERR_UNSUPPORTED, "too many distinct cell geometries");
+    double* E = nullptr;
+    JFEM_HIP(hipMalloc(&E, sizeof(double) * Esize));
+    k_element_tables<<<grid_for(Esize, 256, 16384), 256, 0, c->stream>>>(EP, D.tabs[0], D.tabs[1], D.tabs[2], E);
+    JFEM_HIP(hipGetLastError());
+
+    double* coeff = nullptr;
+    if (coeff_host) {
+        JFEM_HIP(hipMalloc(&coeff, sizeof(double) * g.ncells()));
+        JFEM_HIP(hipMemcpyAsync(coeff, coeff_host, sizeof(double) * g.ncells(), hipMemcpyHostToDevice, c->stream));
+    }
+    RowParams RP;
+    std::memset(&RP, 0, sizeof(RP));
+    RP.g = to_dev(g);
+    RP.nloc = EP.nloc; RP.nint = nint; RP.ntuples = EP.ntuples; RP.E = E; RP.coeff = coeff;
+    for (int a = 0; a < MAXD; a++) { RP.cls[a] = D.cls[a]; RP.ncls[a] = a < g.d ? C.ncls[a] : 1; }
+
+    const long n = g.ndofs();
+    auto* A = new jfem_mat();
+    A->ctx = c;
+    A->rows = A->cols = n;
+    int* counts = nullptr;
+    JFEM_HIP(hipMalloc(&counts, sizeof(int) * n));
+    JFEM_HIP(hipMalloc(&A->rowptr, sizeof(int) * (n + 1)));
+    int maxW = 1;
+    for (int a = 0; a < g.d; a++) maxW *= 2 * g.k + 1;
+    const int G = maxW <= 16 ? 16 : (maxW <= 32 ? 32 : 64);
+    const int groups_per_block = 256 / G;
+    const int blocks = grid_for(n, groups_per_block, 65536);
+    auto launch = [&](bool fill) {
+        if (G == 16) {
+            if (fill) k_assemble_rows<16, true><<<blocks, 256, 0, c->stream>>>(RP, n, counts, A->rowptr, A->colidx, A->vals);
+            else k_assemble_rows<16, false><<<blocks, 256, 0, c->stream>>>(RP, n, counts, A->rowptr, A->colidx, A->vals);
+        } else if (G == 32) {
+            if (fill) k_assemble_rows<32, true><<<blocks, 256, 0, c->stream>>>(RP, n, counts, A->rowptr, A->colidx, A->vals);
+            else k_assemble_rows<32, false><<<blocks, 256, 0, c->stream>>>(RP, n, counts, A->rowptr, A->colidx, A->vals);
+        } else {
+            if (fill) k_assemble_rows<64, true><<<blocks, 256, 0, c->stream>>>(RP, n, counts, A->rowptr, A->colidx, A->vals);
+            else k_assemble_rows<64, false><<<blocks, 256, 0, c->stream>>>(RP, n, counts, A->rowptr, A->colidx, A->vals);
+        }
+        JFEM_HIP(hipGetLastError());
+    };
+    launch(false);
+    const long nnz = scan_counts(c, counts, n, A->rowptr);
+    A->nnz = nnz;
+    JFEM_HIP(hipMalloc(&A->colidx, sizeof(int) * (nnz > 0 ? nnz : 1)));
+    JFEM_HIP(hipMalloc(&A->vals, sizeof(double) * (nnz > 0 ? nnz : 1)));
+    launch(true);
+    JFEM_HIP(hipStreamSynchronize(c->stream));
+    (void)hipFree(counts);
+    (void)hipFree(E);
+    (void)hipFree(coeff);
+    A->lanes = choose_lanes(n > 0 ? (double)nnz / n : 1.0);
+    return A;
+}
+
+void assemble_rhs_sines(jfem_space* s, int nterms, const int* kw, const double* amp, int rule, double* rhs)
+{
+    jfem_ctx* c = s->ctx;
+    const Grid& g = s->grid;
+    JFEM_CHECK(nterms >= 1, JFEM_ERR_ARG, "no source terms");
+    ClassTables C = build_classes(g, rule);
+    DevTables D;
+    upload_classes(c, g, C, D);
+    double qp[8], qw[8];
+    quad_rule(rule, qp, qw);
+    // per-axis tables sin(kw * pi * x_q); the amplitude is folded into axis 0
+    std::vector<double*> dsin(g.d, nullptr);
+    RhsParams P;
+    std::memset(&P, 0, sizeof(P));
+    P.g = to_dev(g);
+    P.Q = C.Q; P.nloc = g.nloc(); P.nterms = nterms;
+    for (int a = 0; a < g.d; a++) {
+        std::vector<double> t((size_t)nterms * g.n[a] * C.Q);
+        for (int m = 0; m < nterms; m++)
+            for (int j = 0; j < g.n[a]; j++)
+                for (int q = 0; q < C.Q; q++) {
+#pragma clang fp contract(off)
+                    const double len = C.e1d[a][j] - C.s1d[a][j];
+                    const double x = qp[q] * len + C.s1d[a][j];
+                    double v = std::sin(kw[m * g.d + a] * M_PI * x);
+                    if (a == 0) v = amp[m] * v;
+                    t[((size_t)m * g.n[a] + j) * C.Q + q] = v;
+                }
+        JFEM_HIP(hipMalloc(&dsin[a], sizeof(double) * t.size()));
+        JFEM_HIP(hipMemcpyAsync(dsin[a], t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, c->stream));
+        P.sinq[a] = dsin[a];
+        P.cls[a] = D.cls[a];
+        P.ncls[a] = C.ncls[a];
+    }
+    vec_fill(c, rhs, 0.0, g.ndofs());
+    for (int color = 0; color < (1 << g.d); color++) {
+        long ncc = 1;
+        bool empty = false;
+        for (int a = 0; a < g.d; a++) {
+            const long h = (g.n[a] - ((color >> a) & 1) + 1) / 2;
+            if (h <= 0) empty = true;
+            ncc *= h;
+        }
+        if (empty) continue;
+        k_rhs_color<<<grid_for(ncc, 1, 65536), 128, 0, c->stream>>>(P, D.tabs[0], D.tabs[1], D.tabs[2], color, ncc, rhs);
+        JFEM_HIP(hipGetLastError());
+    }
+    JFEM_HIP(hipStreamSynchronize(c->stream));
+    for (auto p : dsin) (void)hipFree(p);
+}
+
+void space_boundary_mask(jfem_space* s, unsigned char* mask_dev)
+{
+    const long n = s->grid.ndofs();
+    k_boundary_mask<<<grid_for(n, 256, 16384), 256, 0, s->ctx->stream>>>(to_dev(s->grid), n, mask_dev);
+    JFEM_HIP(hipGetLastError());
+}
+
+}  // namespace jfem
+
+using namespace jfem;
+
+extern "C" {
+
+int jfem_space_create(jfem_ctx* ctx, int dim, const int* cells, const double* start, const double* end, int degree,
+                      jfem_space** out)
+{
+    API_BEGIN
+    JFEM_CHECK(ctx && cells && start && end && out, JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(dim >= 1 && dim <= MAXD, JFEM_ERR_ARG, "Dimensions of coordinates or cell counts do not match");
+    JFEM_CHECK(degree >= 1 && degree <= 4, JFEM_ERR_UNSUPPORTED, "polynomial degree must be 1..4");
+    Grid g;
+    g.d = dim;
+    g.k = degree;
+    for (int a = 0; a < dim; a++) {
+        JFEM_CHECK(cells[a] >= 1, JFEM_ERR_ARG, "Range: lower index higher than upper");
+        JFEM_CHECK(cells[a] <= (1 << 20), JFEM_ERR_ARG, "too many cells along an axis");
+        g.n[a] = cells[a];
+        g.start[a] = start[a];
+        g.end[a] = end[a];
+    }
+    JFEM_CHECK(g.ndofs() < (1l << 31) - 1, JFEM_ERR_UNSUPPORTED, "2^31 or more degrees of freedom");
+    auto* s = new jfem_space();
+    s->ctx = ctx;
+    s->grid = g;
+    *out = s;
+    API_END
+}
+
+int jfem_space_destroy(jfem_space* s)
+{
+    API_BEGIN
+    delete s;
+    API_END
+}
+
+long jfem_space_ndofs(const jfem_space* s) { return s ? s->grid.ndofs() : -1; }
+
+int jfem_space_cell_dofs(jfem_space* s, int* out)
+{
+    API_BEGIN
+    JFEM_CHECK(s && out, JFEM_ERR_ARG, "null argument");
+    const long nc = s->grid.ncells();
+    const int nloc = s->grid.nloc();
+    int* d = nullptr;
+    JFEM_HIP(hipMalloc(&d, sizeof(int) * nc * nloc));
+    k_cell_dofs<<<grid_for(nc * nloc, 256, 16384), 256, 0, s->ctx->stream>>>(to_dev(s->grid), nc, nloc, d);
+    hipError_t e = hipMemcpyAsync(out, d, sizeof(int) * nc * nloc, hipMemcpyDeviceToHost, s->ctx->stream);
+    hipError_t e2 = hipStreamSynchronize(s->ctx->stream);
+    (void)hipFree(d);
+    JFEM_HIP(e);
+    JFEM_HIP(e2);
+    API_END
+}
+
+int jfem_space_boundary_nodes(jfem_space* s, int* out, long* count)
+{
+    API_BEGIN
+    JFEM_CHECK(s && out && count, JFEM_ERR_ARG, "null argument");
+    const long n = s->grid.ndofs();
+    unsigned char* m = nullptr;
+    JFEM_HIP(hipMalloc(&m, n));
+    space_boundary_mask(s, m);
+    std::vector<unsigned char> h(n);
+    hipError_t e = hipMemcpyAsync(h.data(), m, n, hipMemcpyDeviceToHost, s->ctx->stream);
+    hipError_t e2 = hipStreamSynchronize(s->ctx->stream);
+    (void)hipFree(m);
+    JFEM_HIP(e);
+    JFEM_HIP(e2);
+    long nb = 0;
+    for (long i = 0; i < n; i++) if (h[i]) out[nb++] = (int)i;
+    *count = nb;
+    API_END
+}
+
+int jfem_space_node_coords(jfem_space* s, double* out)
+{
+    API_BEGIN
+    JFEM_CHECK(s && out, JFEM_ERR_ARG, "null argument");
+    const Grid& g = s->grid;
+    const long n = g.ndofs();
+    double* d = nullptr;
+    double* se[2 * MAXD] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    JFEM_HIP(hipMalloc(&d, sizeof(double) * n * g.d));
+    for (int a = 0; a < g.d; a++) {
+        std::vector<double> s1, e1;
+        cells_1d(g.start[a], g.end[a], g.n[a], s1, e1);
+        JFEM_HIP(hipMalloc(&se[a], sizeof(double) * g.n[a]));
+        JFEM_HIP(hipMalloc(&se[MAXD + a], sizeof(double) * g.n[a]));
+        JFEM_HIP(hipMemcpy(se[a], s1.data(), sizeof(double) * g.n[a], hipMemcpyHostToDevice));
+        JFEM_HIP(hipMemcpy(se[MAXD + a], e1.data(), sizeof(double) * g.n[a], hipMemcpyHostToDevice));
+    }
+    k_node_coords<<<grid_for(n, 256, 16384), 256, 0, s->ctx->stream>>>(to_dev(g), n, se[0], se[1], se[2], se[3], se[4],
+                                                                      se[5], d);
+    hipError_t e = hipMemcpyAsync(out, d, sizeof(double) * n * g.d, hipMemcpyDeviceToHost, s->ctx->stream);
+    hipError_t e2 = hipStreamSynchronize(s->ctx->stream);
+    (void)hipFree(d);
+    for (auto p : se) (void)hipFree(p);
+    JFEM_HIP(e);
+    JFEM_HIP(e2);
+    API_END
+}
+
+int jfem_assemble(jfem_space* s, int nint, const int* kinds, const double* weights, const double* cell_coeff,
+                  int quad_rule, jfem_mat** out)
+{
+    API_BEGIN
+    JFEM_CHECK(s && kinds && weights && out, JFEM_ERR_ARG, "null argument");
+    *out = assemble_matrix(s, nint, kinds, weights, cell_coeff, quad_rule);
+    API_END
+}
+
+int jfem_assemble_rhs_sines(jfem_space* s, int nterms, const int* kw, const double* amp, int quad_rule, jfem_vec* rhs)
+{
+    API_BEGIN
+    JFEM_CHECK(s && kw && amp && rhs, JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(rhs->n == s->grid.ndofs(), JFEM_ERR_ARG, "Incompatible sizes");
+    assemble_rhs_sines(s, nterms, kw, amp, quad_rule, rhs->d);
+    API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,220 @@
+// Shared definitions of libjfem_hip: context, handles, error reporting,
+// launch helpers and the structured-grid DOF arithmetic used by every kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/jfem_hip.h"
+
+namespace jfem {
+
+// ---------------------------------------------------------------------------
+// errors: every C-ABI call returns a jfem_status; the message of the last
+// failure on the calling thread is kept for jfem_last_error().
+// ---------------------------------------------------------------------------
+void set_error(const std::string& msg);
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define JFEM_HIP(call)                                                                       \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            throw ::jfem::Error{JFEM_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+#define API_BEGIN try {
+#define API_END                                                   \
+    }                                                             \
+    catch (const ::jfem::Error& e) {                              \
+        ::jfem::set_error(e.msg);                                 \
+        return e.code;                                            \
+    }                                                             \
+    catch (const std::bad_alloc&) {                               \
+        ::jfem::set_error("host allocation failed");              \
+        return JFEM_ERR_NOMEM;                                    \
+    }                                                             \
+    catch (...) {                                                 \
+        ::jfem::set_error("unknown failure");                     \
+        return JFEM_ERR_STATE;                                    \
+    }                                                             \
+    return JFEM_OK;
+
+#define JFEM_CHECK(cond, code, msg)                                 \
+    do {                                                            \
+        if (!(cond)) throw ::jfem::Error{(code), std::string(msg)}; \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// structured Qk space on a Cartesian box (ContinuousTPFESpace)
+// ---------------------------------------------------------------------------
+constexpr int MAXD = 3;
+
+struct Grid {
+    int d = 0, k = 0;
+    int n[MAXD] = {1, 1, 1};   // cells per axis
+    double start[MAXD] = {0, 0, 0}, end[MAXD] = {0, 0, 0};
+    long ncells() const { long c = 1; for (int a = 0; a < d; a++) c *= n[a]; return c; }
+    long nodes_axis(int a) const { return (long)k * n[a] + 1; }
+    long ndofs() const { long c = 1; for (int a = 0; a < d; a++) c *= nodes_axis(a); return c; }
+    int nloc() const { int c = 1; for (int a = 0; a < d; a++) c *= k + 1; return c; }
+};
+
+// Closed form of the reference's first-appearance DOF numbering
+// (ContinuousTPFESpace.assembleFunctions, ContinuousTPFESpace.java:29-50):
+// cells are visited in IntCoordinates order (last axis fastest), the local
+// nodes of a cell with axis 0 fastest, and a node is new in cell c iff l_a >= 1
+// on every axis with c_a > 0.  Hence with m_a(c) = (c > 0 ? k : k + 1),
+// S_a(c) = (c > 0 ? c*k + 1 : 0) and T_a = k*n_a + 1:
+//   prefix(c) = sum_a [prod_{b<a} m_b(c_b)] * S_a(c_a) * [prod_{b>a} T_b]
+//   rank(l|c) = sum_a (l_a - [c_a > 0]) * prod_{b<a} m_b(c_b)
+// and the node at lattice point p is owned by c_a = (p_a == 0 ? 0 : (p_a-1)/k).
+struct GridDev {
+    int d, k;
+    int n[MAXD];
+};
+
+__host__ __device__ inline long dof_of_lattice(const GridDev& g, const int* p)
+{
+    long pre = 0, mprod = 1, rank = 0;
+    // tail products T_{b>a}
+    long tail[MAXD];
+    long t = 1;
+    for (int a = g.d - 1; a >= 0; a--) { tail[a] = t; t *= (long)g.k * g.n[a] + 1; }
+    for (int a = 0; a < g.d; a++) {
+        const int c = p[a] == 0 ? 0 : (p[a] - 1) / g.k;
+        const int l = p[a] - c * g.k;
+        const long m = c > 0 ? g.k : g.k + 1;
+        const long S = c > 0 ? (long)c * g.k + 1 : 0;
+        pre += mprod * S * tail[a];
+        rank += (long)(l - (c > 0 ? 1 : 0)) * mprod;
+        mprod *= m;
+    }
+    return pre + rank;
+}
+
+// inverse of dof_of_lattice
+__host__ __device__ inline void lattice_of_dof(const GridDev& g, long dof, int* p)
+{
+    long tail[MAXD];
+    long t = 1;
+    for (int a = g.d - 1; a >= 0; a--) { tail[a] = t; t *= (long)g.k * g.n[a] + 1; }
+    long rem = dof, mprod = 1;
+    int c[MAXD];
+    for (int a = 0; a < g.d; a++) {
+        const long unit = mprod * tail[a];
+        const long q = rem / unit;
+        const int ca = q < g.k + 1 ? 0 : (int)((q - 1) / g.k);
+        c[a] = ca;
+        const long S = ca > 0 ? (long)ca * g.k + 1 : 0;
+        rem -= S * unit;
+        mprod *= ca > 0 ? g.k : g.k + 1;
+    }
+    // rem = rank inside the owner cell, axis 0 fastest
+    for (int a = 0; a < g.d; a++) {
+        const long m = c[a] > 0 ? g.k : g.k + 1;
+        const int la = (int)(rem % m) + (c[a] > 0 ? 1 : 0);
+        rem /= m;
+        p[a] = c[a] * g.k + la;
+    }
+}
+
+inline GridDev to_dev(const Grid& g)
+{
+    GridDev r;
+    r.d = g.d; r.k = g.k;
+    for (int a = 0; a < MAXD; a++) r.n[a] = a < g.d ? g.n[a] : 1;
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// handles
+// ---------------------------------------------------------------------------
+}  // namespace jfem
+
+struct jfem_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // reduction scratch: partial sums and a small result area (device)
+    double* red_partials = nullptr;   // RED_BLOCKS * RED_SLOTS doubles
+    double* red_result = nullptr;     // RED_SLOTS doubles (device)
+    double* red_host = nullptr;       // pinned mirror of red_result
+    double* scal = nullptr;           // device scalars used by fused solver kernels
+    int cu_count = 256;
+};
+
+struct jfem_space {
+    jfem_ctx* ctx = nullptr;
+    jfem::Grid grid;
+};
+
+struct jfem_vec {
+    jfem_ctx* ctx = nullptr;
+    long n = 0;
+    double* d = nullptr;     // device storage (float64)
+};
+
+struct jfem_mat {
+    jfem_ctx* ctx = nullptr;
+    long rows = 0, cols = 0, nnz = 0;
+    int* rowptr = nullptr;   // device, rows+1 (int32: nnz < 2^31 enforced)
+    int* colidx = nullptr;   // device, nnz
+    double* vals = nullptr;  // device, nnz
+    float* vals32 = nullptr; // optional FP32 copy of the values
+    double* diag = nullptr;  // optional: diagonal (device) for Jacobi-type smoothers
+    int lanes = 8;           // lanes per row for the SpMV, from the mean row length
+};
+
+namespace jfem {
+
+constexpr int RED_BLOCKS = 2048;  // fixed reduction grid: deterministic partial-sum order
+constexpr int RED_SLOTS = 8;      // simultaneous reductions per launch
+
+inline int grid_for(long n, int block, int max_blocks = 1 << 20)
+{
+    long g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > max_blocks) g = max_blocks;
+    return (int)g;
+}
+
+// ----- device helpers shared by several translation units -----
+void vec_fill(jfem_ctx* c, double* x, double v, long n);
+void vec_copy(jfem_ctx* c, double* dst, const double* src, long n);
+// y = a*x + b*y
+void vec_axpby(jfem_ctx* c, long n, double a, const double* x, double b, double* y);
+// dots: computes up to RED_SLOTS dot products (xs[i], ys[i]) in one pass; result on host
+void vec_dots(jfem_ctx* c, long n, int m, const double* const* xs, const double* const* ys, double* out_host);
+double vec_dot(jfem_ctx* c, long n, const double* x, const double* y);
+// deterministic two-stage reduction of partials already written into ctx->red_partials
+void reduce_partials(jfem_ctx* c, int slots, int blocks, bool to_host, double* out_host);
+
+// SpMV y = A x (FP64)
+void spmv(jfem_mat* A, const double* x, double* y);
+// SpMV y = A x with FP32 values and FP32 vectors
+void spmv32(jfem_mat* A, const float* x, float* y);
+// SpMV y = A x and partial dot(x, y) into reduction slot 0 (not finalised)
+int spmv_dot(jfem_mat* A, const double* x, double* y);   // returns the number of partials
+int choose_lanes(double mean_row);
+// reduce partials into a device scalar (no host sync)
+void reduce_partials_dev(jfem_ctx* c, int slots, int blocks, double* dst_dev);
+void mat_make_f32(jfem_mat* A);
+void mat_diag(jfem_mat* A, double* out);
+void mat_free(jfem_mat* A);
+void apply_dirichlet_mask(jfem_mat* A, double* rhs, const unsigned char* fixed_dev, const double* g_dev);
+void space_boundary_mask(jfem_space* s, unsigned char* mask_dev);
+jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const double* weights, const double* coeff_host,
+                          int rule);
+// multigrid preconditioner application z = M r (FP64 vectors on the context stream)
+void mg_apply(jfem_mg* mg, const double* r, double* z);
+long mg_size(const jfem_mg* mg);
+
+}  // namespace jfem

@@ -1,0 +1,563 @@
+// Krylov solvers of IterativeSolver (M/linalg/IterativeSolver.java) and GMRES
+// (M/linalg/GMRES.java) on device-resident vectors.  The host keeps the
+// scalar control flow; alpha/beta live on the device and are consumed by the
+// fused update kernels, so each CG iteration costs one scalar read-back (the
+// convergence test ||r|| > tol).  Vector arithmetic keeps the reference's
+// operation order per element (x + p*alpha, r - q*alpha, r + p*beta, ...).
+#include "common.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace jfem {
+
+constexpr int KB = 256;
+
+template <int NT>
+__device__ inline void block_partial(double v, double* sh, double* out)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0;
+        for (int i = 0; i < NT / 64; i++) r += sh[i];
+        *out = r;
+    }
+}
+
+// x += alpha p ; r -= alpha q ; partial ||r||^2.  alpha = s[num] / s[den].
+__global__ __launch_bounds__(KB) void k_cg_update(long n, const double* __restrict__ sc, int num, int den,
+                                                  const double* __restrict__ p, const double* __restrict__ q,
+                                                  double* __restrict__ x, double* __restrict__ r,
+                                                  double* __restrict__ partials)
+{
+#pragma clang fp contract(off)
+    __shared__ double sh[KB / 64];
+    const double alpha = sc[num] / sc[den];
+    double acc = 0;
+    for (long i = blockIdx.x * (long)KB + threadIdx.x; i < n; i += (long)gridDim.x * KB) {
+        x[i] = x[i] + p[i] * alpha;
+        const double rn = r[i] - q[i] * alpha;
+        r[i] = rn;
+        acc += rn * rn;
+    }
+    block_partial<KB>(acc, sh, &partials[blockIdx.x]);
+}
+
+// p = z + p * beta, beta = s[num] / s[den]   (newResiduum.add(defect.mul(beta)))
+__global__ __launch_bounds__(KB) void k_xpby(long n, const double* __restrict__ sc, int num, int den,
+                                             const double* __restrict__ z, double* __restrict__ p)
+{
+#pragma clang fp contract(off)
+    const double beta = sc[num] / sc[den];
+    for (long i = blockIdx.x * (long)KB + threadIdx.x; i < n; i += (long)gridDim.x * KB) p[i] = z[i] + p[i] * beta;
+}
+
+// p = z + p * beta with beta = -s[num]/s[den]  (flexible CG, Notay's FCG(1))
+__global__ __launch_bounds__(KB) void k_xpby_neg(long n, const double* __restrict__ sc, int num, int den,
+                                                 const double* __restrict__ z, double* __restrict__ p)
+{
+    const double beta = -sc[num] / sc[den];
+    for (long i = blockIdx.x * (long)KB + threadIdx.x; i < n; i += (long)gridDim.x * KB) p[i] = z[i] + p[i] * beta;
+}
+
+// r = b - t
+__global__ void k_sub(long n, const double* __restrict__ b, const double* __restrict__ t, double* __restrict__ r)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) r[i] = b[i] - t[i];
+}
+
+// up to RED_SLOTS dots into partials (slot-major), one pass
+struct Dots {
+    const double* x[RED_SLOTS];
+    const double* y[RED_SLOTS];
+};
+__global__ __launch_bounds__(KB) void k_dots_dev(long n, int m, Dots D, double* __restrict__ partials)
+{
+    __shared__ double sh[KB / 64];
+    for (int s = 0; s < m; s++) {
+        double acc = 0;
+        for (long i = blockIdx.x * (long)KB + threadIdx.x; i < n; i += (long)gridDim.x * KB) acc += D.x[s][i] * D.y[s][i];
+        block_partial<KB>(acc, sh, &partials[s * RED_BLOCKS + blockIdx.x]);
+        __syncthreads();
+    }
+}
+
+static int dots_dev(jfem_ctx* c, long n, int m, const double* const* xs, const double* const* ys, double* dst_dev)
+{
+    Dots D;
+    for (int s = 0; s < m; s++) { D.x[s] = xs[s]; D.y[s] = ys[s]; }
+    const int blocks = grid_for(n, KB, RED_BLOCKS);
+    k_dots_dev<<<blocks, KB, 0, c->stream>>>(n, m, D, c->red_partials);
+    JFEM_HIP(hipGetLastError());
+    reduce_partials_dev(c, m, blocks, dst_dev);
+    return blocks;
+}
+
+static double read_scalar(jfem_ctx* c, const double* dev)
+{
+    JFEM_HIP(hipMemcpyAsync(c->red_host, dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    JFEM_HIP(hipStreamSynchronize(c->stream));
+    return c->red_host[0];
+}
+
+static void read_scalars(jfem_ctx* c, const double* dev, int m, double* out)
+{
+    JFEM_HIP(hipMemcpyAsync(c->red_host, dev, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+    JFEM_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < m; i++) out[i] = c->red_host[i];
+}
+
+struct DevBuf {
+    double* p = nullptr;
+    DevBuf(long n) { JFEM_HIP(hipMalloc(&p, sizeof(double) * (n > 0 ? n : 1))); }
+    ~DevBuf() { (void)hipFree(p); }
+};
+
+static int kgrid(long n) { return grid_for(n, KB, RED_BLOCKS); }
+
+static double true_resnorm(jfem_mat* A, const double* b, const double* x, double* tmp)
+{
+    jfem_ctx* c = A->ctx;
+    spmv(A, x, tmp);
+    k_sub<<<kgrid(A->rows), KB, 0, c->stream>>>(A->rows, b, tmp, tmp);
+    return std::sqrt(vec_dot(c, A->rows, tmp, tmp));
+}
+
+// ------------------------------------------------------------------ CG / PCG
+// scalar slots in ctx->scal: 0 = r.r (or r.z), 1 = p.Ap, 2 = new r.r, 3 = new r.z
+static void solve_cg(jfem_mat* A, jfem_mg* M, const double* b, double* x, double tol, long maxit, jfem_solve_info* info,
+                     double* hist)
+{
+    jfem_ctx* c = A->ctx;
+    const long n = A->rows;
+    double* sc = c->scal;
+    DevBuf r(n), z(n), p(n), q(n);
+    // r = b - A x
+    spmv(A, x, q.p);
+    k_sub<<<kgrid(n), KB, 0, c->stream>>>(n, b, q.p, r.p);
+    if (M) mg_apply(M, r.p, z.p);
+    vec_copy(c, p.p, M ? z.p : r.p, n);
+    {
+        const double* xs[2] = {r.p, r.p};
+        const double* ys[2] = {M ? z.p : r.p, r.p};
+        dots_dev(c, n, 2, xs, ys, sc);   // sc[0] = r.z, sc[1] = r.r
+    }
+    double hv[2];
+    read_scalars(c, sc, 2, hv);
+    double rr = hv[1];
+    // the reference's start: sc[0] holds r.z (PCG) or r.r (CG)
+    int it = 0;
+    const long cap = std::min(n, maxit);
+    for (long iter = 0; iter < cap && std::sqrt(rr) > tol; iter++) {
+        const int blocks = spmv_dot(A, p.p, q.p);           // q = A p, partial p.q
+        reduce_partials_dev(c, 1, blocks, sc + 1);           // sc[1] = p.Ap
+        k_cg_update<<<kgrid(n), KB, 0, c->stream>>>(n, sc, 0, 1, p.p, q.p, x, r.p, c->red_partials);
+        reduce_partials_dev(c, 1, kgrid(n), sc + 2);         // sc[2] = r.r (new)
+        if (M) {
+            mg_apply(M, r.p, z.p);
+            const double* xs[1] = {r.p};
+            const double* ys[1] = {z.p};
+            dots_dev(c, n, 1, xs, ys, sc + 3);               // sc[3] = r.z (new)
+            k_xpby<<<kgrid(n), KB, 0, c->stream>>>(n, sc, 3, 0, z.p, p.p);
+            JFEM_HIP(hipMemcpyAsync(sc, sc + 3, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        } else {
+            k_xpby<<<kgrid(n), KB, 0, c->stream>>>(n, sc, 2, 0, r.p, p.p);
+            JFEM_HIP(hipMemcpyAsync(sc, sc + 2, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        }
+        rr = read_scalar(c, sc + 2);
+        if (hist) hist[it] = std::sqrt(rr);
+        it++;
+        JFEM_CHECK(std::isfinite(rr), JFEM_ERR_STATE, "CG diverged (non-finite residual)");
+    }
+    info->iterations = it;
+    info->resnorm = std::sqrt(rr);
+}
+
+// ------------------------------------------------------------ flexible CG
+// Notay, "Flexible conjugate gradients", SIAM J. Sci. Comput. 22 (2000), FCG(1):
+// beta = -(z_{k+1} . q_k) / (p_k . q_k), q_k = A p_k.  Outer arithmetic in FP64;
+// the preconditioner may run in FP32.
+static void solve_fcg(jfem_mat* A, jfem_mg* M, const double* b, double* x, double tol, long maxit,
+                      jfem_solve_info* info, double* hist)
+{
+    jfem_ctx* c = A->ctx;
+    const long n = A->rows;
+    double* sc = c->scal;
+    DevBuf r(n), z(n), p(n), q(n);
+    spmv(A, x, q.p);
+    k_sub<<<kgrid(n), KB, 0, c->stream>>>(n, b, q.p, r.p);
+    mg_apply(M, r.p, z.p);
+    vec_copy(c, p.p, z.p, n);
+    {
+        const double* xs[2] = {r.p, r.p};
+        const double* ys[2] = {z.p, r.p};
+        dots_dev(c, n, 2, xs, ys, sc);   // sc[0] = r.z, sc[1] = r.r
+    }
+    double hv[2];
+    read_scalars(c, sc, 2, hv);
+    double rr = hv[1];
+    int it = 0;
+    const long cap = std::min(n, maxit);
+    for (long iter = 0; iter < cap && std::sqrt(rr) > tol; iter++) {
+        // alpha = (p . r) / (p . Ap); for FCG(1) p.r = z.r of the current step
+        {
+            const double* xs[1] = {p.p};
+            const double* ys[1] = {r.p};
+            dots_dev(c, n, 1, xs, ys, sc + 0);
+        }
+        const int blocks = spmv_dot(A, p.p, q.p);
+        reduce_partials_dev(c, 1, blocks, sc + 1);
+        k_cg_update<<<kgrid(n), KB, 0, c->stream>>>(n, sc, 0, 1, p.p, q.p, x, r.p, c->red_partials);
+        reduce_partials_dev(c, 1, kgrid(n), sc + 2);
+        mg_apply(M, r.p, z.p);
+        {
+            const double* xs[1] = {z.p};
+            const double* ys[1] = {q.p};
+            dots_dev(c, n, 1, xs, ys, sc + 3);   // sc[3] = z.q
+        }
+        k_xpby_neg<<<kgrid(n), KB, 0, c->stream>>>(n, sc, 3, 1, z.p, p.p);
+        rr = read_scalar(c, sc + 2);
+        if (hist) hist[it] = std::sqrt(rr);
+        it++;
+        JFEM_CHECK(std::isfinite(rr), JFEM_ERR_STATE, "FCG diverged (non-finite residual)");
+    }
+    info->iterations = it;
+    info->resnorm = std::sqrt(rr);
+}
+
+// --------------------------------------------------------------- BiCGStab
+// generic element kernels for BiCGStab
+// p = (r + p*beta) - v*(omega*beta)
+__global__ void k_bicg_p(long n, double beta, double ob, const double* __restrict__ r, const double* __restrict__ v,
+                         double* __restrict__ p)
+{
+#pragma clang fp contract(off)
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        p[i] = (r[i] + p[i] * beta) - v[i] * ob;
+}
+// s = r - v*alpha
+__global__ void k_axmy(long n, double alpha, const double* __restrict__ r, const double* __restrict__ v,
+                       double* __restrict__ s)
+{
+#pragma clang fp contract(off)
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        s[i] = r[i] - v[i] * alpha;
+}
+// x = (x + a*alpha) + b*omega
+__global__ void k_x2(long n, double alpha, double omega, const double* __restrict__ a, const double* __restrict__ bb,
+                     double* __restrict__ x)
+{
+#pragma clang fp contract(off)
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        x[i] = (x[i] + a[i] * alpha) + bb[i] * omega;
+}
+
+static void solve_gmres(jfem_mat* A, jfem_mg* M, const double* b, double* x, double tol, int restart, int max_cycles,
+                        jfem_solve_info* info);
+
+static void solve_bicgstab(jfem_mat* A, jfem_mg* M, const double* b, double* x, double tol, long maxit,
+                           jfem_solve_info* info)
+{
+    jfem_ctx* c = A->ctx;
+    const long n = A->rows;
+    DevBuf v(n), s(n), t(n), r0(n), r(n), p(n), y(n), z(n), pt(n), tmp(n);
+    vec_fill(c, v.p, 0, n);
+    vec_fill(c, p.p, 0, n);
+    spmv(A, x, tmp.p);
+    k_sub<<<kgrid(n), KB, 0, c->stream>>>(n, b, tmp.p, r0.p);
+    vec_copy(c, r.p, r0.p, n);
+    double alpha = 1, omega = 1, rho = 1, rhoLast = 1, beta = 1;
+    int it = 0;
+    info->fell_back = 0;
+    const long cap = std::min(M ? n * 10 : n, maxit);
+    const int g = kgrid(n);
+    double rnorm = std::sqrt(vec_dot(c, n, r.p, r.p));
+    for (long iter = 0; iter < cap && rnorm > tol; iter++) {
+        rho = vec_dot(c, n, r.p, r0.p);
+        if (std::fabs(rhoLast) < tol) {
+            info->fell_back = 1;
+            // the reference restarts from its default start iterate: e_0 (GMRES) or 0 (PGMRES)
+            vec_fill(c, x, 0.0, n);
+            if (!M) {
+                const double one = 1.0;
+                JFEM_HIP(hipMemcpyAsync(x, &one, sizeof(double), hipMemcpyHostToDevice, c->stream));
+            }
+            solve_gmres(A, M, b, x, tol, 150, 1000, info);
+            info->fell_back = 1;
+            return;
+        }
+        beta = alpha / omega * rho / rhoLast;
+        k_bicg_p<<<g, KB, 0, c->stream>>>(n, beta, omega * beta, r.p, v.p, p.p);
+        if (M) { mg_apply(M, p.p, y.p); spmv(A, y.p, v.p); } else spmv(A, p.p, v.p);
+        alpha = rho / vec_dot(c, n, v.p, r0.p);
+        k_axmy<<<g, KB, 0, c->stream>>>(n, alpha, r.p, v.p, s.p);
+        if (std::sqrt(vec_dot(c, n, s.p, s.p)) < tol) {
+            k_x2<<<g, KB, 0, c->stream>>>(n, alpha, 0.0, M ? y.p : p.p, M ? y.p : p.p, x);
+            break;
+        }
+        if (M) {
+            mg_apply(M, s.p, z.p);
+            spmv(A, z.p, t.p);
+            mg_apply(M, t.p, pt.p);
+            double d[2];
+            const double* xs[2] = {pt.p, pt.p};
+            const double* ys[2] = {z.p, pt.p};
+            vec_dots(c, n, 2, xs, ys, d);
+            omega = d[0] / d[1];
+            k_x2<<<g, KB, 0, c->stream>>>(n, alpha, omega, y.p, z.p, x);
+        } else {
+            spmv(A, s.p, t.p);
+            double d[2];
+            const double* xs[2] = {t.p, t.p};
+            const double* ys[2] = {s.p, t.p};
+            vec_dots(c, n, 2, xs, ys, d);
+            omega = d[0] / d[1];
+            k_x2<<<g, KB, 0, c->stream>>>(n, alpha, omega, p.p, s.p, x);
+        }
+        k_axmy<<<g, KB, 0, c->stream>>>(n, omega, s.p, t.p, r.p);
+        rhoLast = rho;
+        it++;
+        rnorm = std::sqrt(vec_dot(c, n, r.p, r.p));
+        JFEM_CHECK(std::isfinite(rnorm), JFEM_ERR_STATE, "BiCGStab diverged (non-finite residual)");
+    }
+    info->iterations = it;
+    info->resnorm = true_resnorm(A, b, x, tmp.p);
+}
+
+// ------------------------------------------------------------------- GMRES
+// Java 11 DoubleStream.sum() (compensated; final sum + compensation)
+struct JSum {
+    double s = 0, c = 0, simple = 0;
+    void add(double v)
+    {
+#pragma clang fp contract(off)
+        const double tmp = v - c;
+        const double sum = s;
+        const double velvel = sum + tmp;
+        c = (velvel - sum) - tmp;
+        s = velvel;
+        simple += v;
+    }
+    double final_() const
+    {
+        const double tmp = s + c;
+        if (std::isnan(tmp) && std::isinf(simple)) return simple;
+        return tmp;
+    }
+};
+
+// multi-dot f[j] = V_j . q for j < nv (partials [j][block])
+__global__ __launch_bounds__(KB) void k_multidot(long n, int nv, const double* __restrict__ V, long ldv,
+                                                 const double* __restrict__ q, double* __restrict__ partials, int nb)
+{
+    __shared__ double sh[KB / 64];
+    for (int j = 0; j < nv; j++) {
+        const double* __restrict__ vj = V + (long)j * ldv;
+        double acc = 0;
+        for (long i = blockIdx.x * (long)KB + threadIdx.x; i < n; i += (long)gridDim.x * KB) acc += vj[i] * q[i];
+        block_partial<KB>(acc, sh, &partials[(long)j * nb + blockIdx.x]);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(KB) void k_sum_partials(int nv, int nb, const double* __restrict__ partials,
+                                                     double* __restrict__ out)
+{
+    __shared__ double sh[KB / 64];
+    for (int j = 0; j < nv; j++) {
+        double acc = 0;
+        for (int i = threadIdx.x; i < nb; i += KB) acc += partials[(long)j * nb + i];
+        block_partial<KB>(acc, sh, &out[j]);
+        __syncthreads();
+    }
+}
+
+// w = q - sum_j V_j f_j   (orthogonalPartInSubspace summed left to right from 0)
+__global__ void k_gs_sub(long n, int nv, const double* __restrict__ V, long ldv, const double* __restrict__ f,
+                         const double* __restrict__ q, double* __restrict__ w)
+{
+#pragma clang fp contract(off)
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        double acc = 0;
+        for (int j = 0; j < nv; j++) acc = acc + V[(long)j * ldv + i] * f[j];
+        w[i] = q[i] - acc;
+    }
+}
+
+// out = base + sum_j V_j a_j
+__global__ void k_lincomb(long n, int nv, const double* __restrict__ V, long ldv, const double* __restrict__ a,
+                          double* __restrict__ x)
+{
+#pragma clang fp contract(off)
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        double acc = 0;
+        for (int j = 0; j < nv; j++) acc = acc + V[(long)j * ldv + i] * a[j];
+        x[i] = x[i] + acc;
+    }
+}
+
+__global__ void k_scale(long n, double s, const double* __restrict__ a, double* __restrict__ b)
+{
+#pragma clang fp contract(off)
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i] * s;
+}
+
+static void solve_gmres(jfem_mat* A, jfem_mg* M, const double* b, double* x, double tol, int restart, int max_cycles,
+                        jfem_solve_info* info)
+{
+#pragma clang fp contract(off)
+    jfem_ctx* c = A->ctx;
+    const long n = A->rows;
+    JFEM_CHECK(restart >= 1 && restart < 100000, JFEM_ERR_ARG, "bad restart length");
+    const int ld = restart + 10;
+    const int maxv = restart + 3;
+    const long ldv = n;
+    double* V = nullptr;
+    JFEM_CHECK(maxv * (double)n * 8 < 200e9, JFEM_ERR_NOMEM, "Krylov basis exceeds the device memory budget");
+    JFEM_HIP(hipMalloc(&V, sizeof(double) * (size_t)maxv * n));
+    DevBuf t(n), r0(n), q(n), tq(n), w(n), fdev(maxv), part((long)maxv * RED_BLOCKS);
+    std::vector<double> f(maxv), H((size_t)ld * ld), gamma(ld), cs(ld), sn(ld), alpha(ld);
+    int restarts = 0, iterations = 0, cycles = 0;
+    const int nb = kgrid(n);
+    double residual = 0;
+    try {
+        for (;;) {
+            // r = M (b - A x)
+            spmv(A, x, t.p);
+            k_sub<<<nb, KB, 0, c->stream>>>(n, b, t.p, r0.p);
+            double* r = V;   // v_0 slot holds r, normalised in place
+            if (M) mg_apply(M, r0.p, r); else vec_copy(c, r, r0.p, n);
+            const double rn = std::sqrt(vec_dot(c, n, r, r));
+            if (rn <= tol) { residual = -1; break; }
+            std::fill(H.begin(), H.end(), 0.0);
+            std::fill(gamma.begin(), gamma.end(), 0.0);
+            gamma[0] = rn;
+            k_scale<<<nb, KB, 0, c->stream>>>(n, 1. / gamma[0], r, V);
+            int nv = 1, j;
+            for (j = 0; j < n; j++) {
+                const double* vj = V + (long)j * ldv;
+                if (M) { spmv(A, vj, tq.p); mg_apply(M, tq.p, q.p); } else spmv(A, vj, q.p);
+                k_multidot<<<nb, KB, 0, c->stream>>>(n, nv, V, ldv, q.p, part.p, nb);
+                k_sum_partials<<<1, KB, 0, c->stream>>>(nv, nb, part.p, fdev.p);
+                k_gs_sub<<<nb, KB, 0, c->stream>>>(n, nv, V, ldv, fdev.p, q.p, w.p);
+                JFEM_HIP(hipMemcpyAsync(f.data(), fdev.p, sizeof(double) * nv, hipMemcpyDeviceToHost, c->stream));
+                const double wn = std::sqrt(vec_dot(c, n, w.p, w.p));   // synchronises
+                for (int i = 0; i < nv; i++) if (f[i] != 0) H[(size_t)i * ld + j] = H[(size_t)i * ld + j] + f[i];
+                if (wn != 0) H[(size_t)(j + 1) * ld + j] = H[(size_t)(j + 1) * ld + j] + wn;
+                for (int i = 0; i < j; i++) {   // calculateGivens with set() = append(value - at)
+                    const double hij = H[(size_t)i * ld + j], hipj = H[(size_t)(i + 1) * ld + j];
+                    const double a1 = hij * cs[i] + hipj * sn[i];
+                    H[(size_t)i * ld + j] = H[(size_t)i * ld + j] + (a1 - H[(size_t)i * ld + j]);
+                    const double a2 = -hij * sn[i] + hipj * cs[i];
+                    H[(size_t)(i + 1) * ld + j] = H[(size_t)(i + 1) * ld + j] + (a2 - H[(size_t)(i + 1) * ld + j]);
+                }
+                const double hjj = H[(size_t)j * ld + j], hj1 = H[(size_t)(j + 1) * ld + j];
+                const double beta = std::sqrt(std::pow(hjj, 2) + std::pow(hj1, 2));
+                sn[j] = hj1 / beta;
+                cs[j] = hjj / beta;
+                H[(size_t)j * ld + j] = H[(size_t)j * ld + j] + (beta - H[(size_t)j * ld + j]);
+                gamma[j + 1] = -sn[j] * gamma[j];
+                gamma[j] = cs[j] * gamma[j];
+                if (std::fabs(gamma[j + 1]) < tol || j > restart) break;
+                JFEM_CHECK(nv < maxv, JFEM_ERR_STATE, "Krylov basis overflow");
+                k_scale<<<nb, KB, 0, c->stream>>>(n, 1. / H[(size_t)(j + 1) * ld + j], w.p, V + (long)nv * ldv);
+                nv++;
+                iterations++;
+            }
+            if (j == n) j--;
+            for (int i = j; i >= 0; i--) {   // calculateAlpha
+                JSum s;
+                for (int k2 = i + 1; k2 <= j; k2++) s.add(H[(size_t)i * ld + k2] * alpha[k2]);
+                alpha[i] = 1. / H[(size_t)i * ld + i] * (gamma[i] - s.final_());
+            }
+            JFEM_HIP(hipMemcpyAsync(fdev.p, alpha.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, c->stream));
+            k_lincomb<<<nb, KB, 0, c->stream>>>(n, j + 1, V, ldv, fdev.p, x);
+            residual = true_resnorm(A, b, x, t.p);
+            cycles++;
+            const bool again = (j > restart && restarts < 200) || residual > tol;
+            if (!again || cycles >= max_cycles) break;
+            restarts++;
+        }
+    } catch (...) {
+        (void)hipFree(V);
+        throw;
+    }
+    hipFree(V);
+    info->iterations = iterations;
+    info->resnorm = residual >= 0 ? residual : true_resnorm(A, b, x, t.p);
+}
+
+}  // namespace jfem
+
+using namespace jfem;
+
+static void check_solve_args(jfem_mat* A, jfem_mg* M, const jfem_vec* b, jfem_vec* x)
+{
+    JFEM_CHECK(A && b && x, JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(A->rows == A->cols, JFEM_ERR_ARG, "Object is not square");
+    JFEM_CHECK(b->n == A->rows && x->n == A->rows, JFEM_ERR_ARG, "Incompatible sizes");
+    JFEM_CHECK(!M || mg_size(M) == A->rows, JFEM_ERR_ARG, "preconditioner has the wrong size");
+}
+
+template <typename F>
+static void timed(jfem_ctx* c, jfem_solve_info* info, F&& f)
+{
+    std::memset(info, 0, sizeof(*info));
+    JFEM_HIP(hipStreamSynchronize(c->stream));
+    auto t0 = std::chrono::steady_clock::now();
+    f();
+    JFEM_HIP(hipStreamSynchronize(c->stream));
+    info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+extern "C" {
+
+int jfem_solve_cg(jfem_mat* A, jfem_mg* M, const jfem_vec* b, jfem_vec* x, double tol, long maxit,
+                  jfem_solve_info* info, double* res_hist)
+{
+    API_BEGIN
+    check_solve_args(A, M, b, x);
+    jfem_solve_info tmp;
+    jfem_solve_info* inf = info ? info : &tmp;
+    timed(A->ctx, inf, [&] { solve_cg(A, M, b->d, x->d, tol, maxit, inf, res_hist); });
+    API_END
+}
+
+int jfem_solve_fcg(jfem_mat* A, jfem_mg* M, const jfem_vec* b, jfem_vec* x, double tol, long maxit,
+                   jfem_solve_info* info, double* res_hist)
+{
+    API_BEGIN
+    check_solve_args(A, M, b, x);
+    JFEM_CHECK(M, JFEM_ERR_ARG, "flexible CG needs a preconditioner");
+    jfem_solve_info tmp;
+    jfem_solve_info* inf = info ? info : &tmp;
+    timed(A->ctx, inf, [&] { solve_fcg(A, M, b->d, x->d, tol, maxit, inf, res_hist); });
+    API_END
+}
+
+int jfem_solve_bicgstab(jfem_mat* A, jfem_mg* M, const jfem_vec* b, jfem_vec* x, double tol, long maxit,
+                        jfem_solve_info* info)
+{
+    API_BEGIN
+    check_solve_args(A, M, b, x);
+    jfem_solve_info tmp;
+    jfem_solve_info* inf = info ? info : &tmp;
+    timed(A->ctx, inf, [&] { solve_bicgstab(A, M, b->d, x->d, tol, maxit, inf); });
+    API_END
+}
+
+int jfem_solve_gmres(jfem_mat* A, jfem_mg* M, const jfem_vec* b, jfem_vec* x, double tol, int restart, int max_cycles,
+                     jfem_solve_info* info)
+{
+    API_BEGIN
+    check_solve_args(A, M, b, x);
+    jfem_solve_info tmp;
+    jfem_solve_info* inf = info ? info : &tmp;
+    timed(A->ctx, inf, [&] { solve_gmres(A, M, b->d, x->d, tol, restart, max_cycles, inf); });
+    API_END
+}
+
+}  // extern "C"

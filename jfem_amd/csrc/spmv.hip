@@ -1,0 +1,369 @@
+// CSR sparse matrix-vector product and matrix handles (SparseMatrix.mvMul,
+// M/linalg/SparseMatrix.java:479-489; SparseMvMul.mvMul, M/linalg/SparseMvMul.java:75-94),
+// plus Dirichlet elimination (AcceptsMatrixBoundaryValues.overWriteValue,
+// M/basic/AcceptsMatrixBoundaryValues.java:18-44).
+//
+// SpMV layout: L lanes per row (L from the mean row length), each lane
+// strides the row's entries with stride L so a group reads a contiguous
+// segment of values/columns; group partial sums are combined with wavefront
+// shuffles.  The kernel is HBM-bound: 12 bytes per entry (8-byte value,
+// 4-byte column) plus the row pointer, x (once, through L2/MALL) and y.
+#include "common.hpp"
+
+#include <cstring>
+
+namespace jfem {
+
+constexpr int SPMV_BS = 256;
+
+template <int L, typename VT, typename XT>
+__global__ __launch_bounds__(SPMV_BS) void k_spmv(long nrows, const int* __restrict__ rowptr,
+                                                  const int* __restrict__ cols, const VT* __restrict__ vals,
+                                                  const XT* __restrict__ x, XT* __restrict__ y)
+{
+    const int lane = threadIdx.x % L;
+    const long groups_total = (long)gridDim.x * (SPMV_BS / L);
+    for (long row = (blockIdx.x * (long)SPMV_BS + threadIdx.x) / L; row < nrows; row += groups_total) {
+        const int s = rowptr[row], e = rowptr[row + 1];
+        XT acc = 0;
+        for (int p = s + lane; p < e; p += L) acc += (XT)vals[p] * x[cols[p]];
+        for (int o = L / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, L);
+        if (lane == 0) y[row] = acc;
+    }
+}
+
+// y = A x and per-block partial of dot(x, y) (reduction slot 0)
+template <int L>
+__global__ __launch_bounds__(SPMV_BS) void k_spmv_dot(long nrows, const int* __restrict__ rowptr,
+                                                      const int* __restrict__ cols, const double* __restrict__ vals,
+                                                      const double* __restrict__ x, double* __restrict__ y,
+                                                      double* __restrict__ partials)
+{
+    __shared__ double sh[SPMV_BS / 64];
+    const int lane = threadIdx.x % L;
+    const long groups_total = (long)gridDim.x * (SPMV_BS / L);
+    double dacc = 0;
+    for (long row = (blockIdx.x * (long)SPMV_BS + threadIdx.x) / L; row < nrows; row += groups_total) {
+        const int s = rowptr[row], e = rowptr[row + 1];
+        double acc = 0;
+        for (int p = s + lane; p < e; p += L) acc += vals[p] * x[cols[p]];
+        for (int o = L / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, L);
+        if (lane == 0) { y[row] = acc; dacc += x[row] * acc; }
+    }
+    for (int o = 32; o > 0; o >>= 1) dacc += __shfl_xor(dacc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = dacc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0;
+        for (int i = 0; i < SPMV_BS / 64; i++) r += sh[i];
+        partials[blockIdx.x] = r;
+    }
+}
+
+static int spmv_grid(const jfem_mat* A, int L)
+{
+    // enough groups for one pass over the rows, capped at 16 waves per SIMD
+    const long need = (A->rows * L + SPMV_BS - 1) / SPMV_BS;
+    const long cap = (long)A->ctx->cu_count * 32;
+    return (int)(need < 1 ? 1 : (need > cap ? cap : need));
+}
+
+template <typename VT, typename XT>
+static void launch_spmv(jfem_mat* A, const VT* vals, const XT* x, XT* y)
+{
+    const int L = A->lanes;
+    const int grid = spmv_grid(A, L);
+    hipStream_t st = A->ctx->stream;
+    switch (L) {
+    case 4: k_spmv<4, VT, XT><<<grid, SPMV_BS, 0, st>>>(A->rows, A->rowptr, A->colidx, vals, x, y); break;
+    case 8: k_spmv<8, VT, XT><<<grid, SPMV_BS, 0, st>>>(A->rows, A->rowptr, A->colidx, vals, x, y); break;
+    case 16: k_spmv<16, VT, XT><<<grid, SPMV_BS, 0, st>>>(A->rows, A->rowptr, A->colidx, vals, x, y); break;
+    case 32: k_spmv<32, VT, XT><<<grid, SPMV_BS, 0, st>>>(A->rows, A->rowptr, A->colidx, vals, x, y); break;
+    default: k_spmv<64, VT, XT><<<grid, SPMV_BS, 0, st>>>(A->rows, A->rowptr, A->colidx, vals, x, y); break;
+    }
+    JFEM_HIP(hipGetLastError());
+}
+
+void spmv(jfem_mat* A, const double* x, double* y) { launch_spmv<double, double>(A, A->vals, x, y); }
+
+void spmv32(jfem_mat* A, const float* x, float* y)
+{
+    JFEM_CHECK(A->vals32, JFEM_ERR_STATE, "matrix has no FP32 values");
+    launch_spmv<float, float>(A, A->vals32, x, y);
+}
+
+int spmv_dot(jfem_mat* A, const double* x, double* y)
+{
+    const int L = A->lanes;
+    int grid = spmv_grid(A, L);
+    if (grid > RED_BLOCKS) grid = RED_BLOCKS;
+    hipStream_t st = A->ctx->stream;
+    double* part = A->ctx->red_partials;
+    switch (L) {
+    case 4: k_spmv_dot<4><<<grid, SPMV_BS, 0, st>>>(A->rows, A->rowptr, A->colidx, A->vals, x, y, part); break;
+    case 8: k_spmv_dot<8><<<grid, SPMV_BS, 0, st>>>(A->rows, A->rowptr, A->colidx, A->vals, x, y, part); break;
+    case 16: k_spmv_dot<16><<<grid, SPMV_BS, 0, st>>>(A->rows, A->rowptr, A->colidx, A->vals, x, y, part); break;
+    case 32: k_spmv_dot<32><<<grid, SPMV_BS, 0, st>>>(A->rows, A->rowptr, A->colidx, A->vals, x, y, part); break;
+    default: k_spmv_dot<64><<<grid, SPMV_BS, 0, st>>>(A->rows, A->rowptr, A->colidx, A->vals, x, y, part); break;
+    }
+    JFEM_HIP(hipGetLastError());
+    return grid;
+}
+
+// ---- Dirichlet elimination, one thread group per row ----
+// fixed rows: zeroed, diagonal set to 0 + (1 - 0) = 1, rhs = g.
+// free rows: rhs_j += -(A_ji) * g_i for fixed columns i in ascending order,
+// then those entries are zeroed (entries stay in the pattern).
+__global__ void k_dirichlet(long n, const int* __restrict__ rowptr, const int* __restrict__ cols,
+                            double* __restrict__ vals, const unsigned char* __restrict__ fixed,
+                            const double* __restrict__ g, double* __restrict__ rhs, int* __restrict__ missing_diag)
+{
+#pragma clang fp contract(off)
+    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) {
+        const int s = rowptr[r], e = rowptr[r + 1];
+        if (fixed[r]) {
+            bool diag = false;
+            for (int p = s; p < e; p++) {
+                if (cols[p] == r) { vals[p] = 1.0; diag = true; }
+                else vals[p] = 0;
+            }
+            if (!diag) *missing_diag = 1;
+            if (rhs) rhs[r] = g ? g[r] : 0.0;
+        } else {
+            double acc = rhs ? rhs[r] : 0.0;
+            for (int p = s; p < e; p++) {
+                const int c = cols[p];
+                if (fixed[c]) {
+                    if (g) acc += -vals[p] * g[c];
+                    vals[p] = 0;
+                }
+            }
+            if (rhs && g) rhs[r] = acc;
+        }
+    }
+}
+
+__global__ void k_to_f32(long n, const double* __restrict__ a, float* __restrict__ b)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = (float)a[i];
+}
+
+__global__ void k_diag(long n, const int* __restrict__ rowptr, const int* __restrict__ cols,
+                       const double* __restrict__ vals, double* __restrict__ diag)
+{
+    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) {
+        double d = 0;
+        for (int p = rowptr[r]; p < rowptr[r + 1]; p++) if (cols[p] == r) d += vals[p];
+        diag[r] = d;
+    }
+}
+
+void mat_make_f32(jfem_mat* A)
+{
+    if (A->vals32) return;
+    JFEM_HIP(hipMalloc(&A->vals32, sizeof(float) * (A->nnz > 0 ? A->nnz : 1)));
+    if (A->nnz) k_to_f32<<<grid_for(A->nnz, 256, 16384), 256, 0, A->ctx->stream>>>(A->nnz, A->vals, A->vals32);
+    JFEM_HIP(hipGetLastError());
+}
+
+void mat_diag(jfem_mat* A, double* out)
+{
+    k_diag<<<grid_for(A->rows, 256, 16384), 256, 0, A->ctx->stream>>>(A->rows, A->rowptr, A->colidx, A->vals, out);
+    JFEM_HIP(hipGetLastError());
+}
+
+void apply_dirichlet_mask(jfem_mat* A, double* rhs, const unsigned char* fixed_dev, const double* g_dev)
+{
+    jfem_ctx* c = A->ctx;
+    int* flag = nullptr;
+    JFEM_HIP(hipMalloc(&flag, sizeof(int)));
+    JFEM_HIP(hipMemsetAsync(flag, 0, sizeof(int), c->stream));
+    k_dirichlet<<<grid_for(A->rows, 256, 16384), 256, 0, c->stream>>>(A->rows, A->rowptr, A->colidx, A->vals,
+                                                                        fixed_dev, g_dev, rhs, flag);
+    int h = 0;
+    hipError_t e = hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    hipError_t e2 = hipStreamSynchronize(c->stream);
+    (void)hipFree(flag);
+    JFEM_HIP(e);
+    JFEM_HIP(e2);
+    JFEM_CHECK(!h, JFEM_ERR_STATE, "Dirichlet node without a diagonal entry in the pattern");
+    if (A->vals32) {
+        (void)hipFree(A->vals32);
+        A->vals32 = nullptr;
+        mat_make_f32(A);
+    }
+}
+
+void mat_free(jfem_mat* A)
+{
+    if (!A) return;
+    (void)hipStreamSynchronize(A->ctx->stream);
+    (void)hipFree(A->rowptr);
+    (void)hipFree(A->colidx);
+    (void)hipFree(A->vals);
+    (void)hipFree(A->vals32);
+    (void)hipFree(A->diag);
+    delete A;
+}
+
+}  // namespace jfem
+
+using namespace jfem;
+
+extern "C" {
+
+int jfem_mat_from_csr(jfem_ctx* ctx, long rows, long cols, const long* rowptr, const int* colidx, const double* vals,
+                      jfem_mat** out)
+{
+    API_BEGIN
+    JFEM_CHECK(ctx && rowptr && out && rows >= 0 && cols >= 0, JFEM_ERR_ARG, "null argument");
+    const long nnz = rowptr[rows];
+    JFEM_CHECK(nnz >= 0 && nnz < (1l << 31) - 1, JFEM_ERR_UNSUPPORTED, "2^31 or more entries");
+    std::vector<int> rp(rows + 1);
+    for (long i = 0; i <= rows; i++) {
+        JFEM_CHECK(i == 0 || rowptr[i] >= rowptr[i - 1], JFEM_ERR_ARG, "row pointer not monotone");
+        rp[i] = (int)rowptr[i];
+    }
+    for (long p = 0; p < nnz; p++) JFEM_CHECK(colidx[p] >= 0 && colidx[p] < cols, JFEM_ERR_ARG, "x coordinate out of bounds");
+    auto* A = new jfem_mat();
+    A->ctx = ctx;
+    A->rows = rows;
+    A->cols = cols;
+    A->nnz = nnz;
+    JFEM_HIP(hipMalloc(&A->rowptr, sizeof(int) * (rows + 1)));
+    JFEM_HIP(hipMalloc(&A->colidx, sizeof(int) * (nnz > 0 ? nnz : 1)));
+    JFEM_HIP(hipMalloc(&A->vals, sizeof(double) * (nnz > 0 ? nnz : 1)));
+    JFEM_HIP(hipMemcpy(A->rowptr, rp.data(), sizeof(int) * (rows + 1), hipMemcpyHostToDevice));
+    if (nnz) {
+        JFEM_HIP(hipMemcpy(A->colidx, colidx, sizeof(int) * nnz, hipMemcpyHostToDevice));
+        JFEM_HIP(hipMemcpy(A->vals, vals, sizeof(double) * nnz, hipMemcpyHostToDevice));
+    }
+    A->lanes = choose_lanes(rows > 0 ? (double)nnz / rows : 1.0);
+    *out = A;
+    API_END
+}
+
+int jfem_mat_destroy(jfem_mat* A)
+{
+    API_BEGIN
+    mat_free(A);
+    API_END
+}
+
+int jfem_mat_info(const jfem_mat* A, long* rows, long* cols, long* nnz)
+{
+    API_BEGIN
+    JFEM_CHECK(A, JFEM_ERR_ARG, "null matrix");
+    if (rows) *rows = A->rows;
+    if (cols) *cols = A->cols;
+    if (nnz) *nnz = A->nnz;
+    API_END
+}
+
+int jfem_mat_export_csr(const jfem_mat* A, long* rowptr, int* colidx, double* vals)
+{
+    API_BEGIN
+    JFEM_CHECK(A && rowptr && colidx && vals, JFEM_ERR_ARG, "null argument");
+    std::vector<int> rp(A->rows + 1);
+    hipStream_t st = A->ctx->stream;
+    JFEM_HIP(hipMemcpyAsync(rp.data(), A->rowptr, sizeof(int) * (A->rows + 1), hipMemcpyDeviceToHost, st));
+    if (A->nnz) {
+        JFEM_HIP(hipMemcpyAsync(colidx, A->colidx, sizeof(int) * A->nnz, hipMemcpyDeviceToHost, st));
+        JFEM_HIP(hipMemcpyAsync(vals, A->vals, sizeof(double) * A->nnz, hipMemcpyDeviceToHost, st));
+    }
+    JFEM_HIP(hipStreamSynchronize(st));
+    for (long i = 0; i <= A->rows; i++) rowptr[i] = rp[i];
+    API_END
+}
+
+int jfem_apply_dirichlet(jfem_mat* A, jfem_vec* rhs, long nb, const int* nodes, const double* values)
+{
+    API_BEGIN
+    JFEM_CHECK(A && (nodes || nb == 0), JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(A->rows == A->cols, JFEM_ERR_ARG, "Dirichlet elimination needs a square matrix");
+    JFEM_CHECK(!rhs || rhs->n == A->rows, JFEM_ERR_ARG, "Incompatible sizes");
+    const long n = A->rows;
+    std::vector<unsigned char> mask(n, 0);
+    std::vector<double> g;
+    if (values) g.assign(n, 0.0);
+    for (long b = 0; b < nb; b++) {
+        JFEM_CHECK(nodes[b] >= 0 && nodes[b] < n, JFEM_ERR_ARG, "boundary node out of range");
+        mask[nodes[b]] = 1;
+        if (values) g[nodes[b]] = values[b];
+    }
+    unsigned char* dm = nullptr;
+    double* dg = nullptr;
+    JFEM_HIP(hipMalloc(&dm, n > 0 ? n : 1));
+    if (n) JFEM_HIP(hipMemcpy(dm, mask.data(), n, hipMemcpyHostToDevice));
+    if (values) {
+        JFEM_HIP(hipMalloc(&dg, sizeof(double) * n));
+        JFEM_HIP(hipMemcpy(dg, g.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    try {
+        apply_dirichlet_mask(A, rhs ? rhs->d : nullptr, dm, dg);
+    } catch (...) {
+        (void)hipFree(dm);
+        (void)hipFree(dg);
+        throw;
+    }
+    hipFree(dm);
+    (void)hipFree(dg);
+    API_END
+}
+
+int jfem_spmv(jfem_mat* A, const jfem_vec* x, jfem_vec* y)
+{
+    API_BEGIN
+    JFEM_CHECK(A && x && y, JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(x->n == A->cols && y->n == A->rows, JFEM_ERR_ARG, "Incompatible sizes");
+    JFEM_CHECK(x->d != y->d || A->rows == 0, JFEM_ERR_ARG, "SpMV output aliases its input");
+    spmv(A, x->d, y->d);
+    JFEM_HIP(hipStreamSynchronize(A->ctx->stream));
+    API_END
+}
+
+int jfem_spmv_f32(jfem_mat* A, const float* x, float* y)
+{
+    API_BEGIN
+    JFEM_CHECK(A && x && y, JFEM_ERR_ARG, "null argument");
+    mat_make_f32(A);
+    spmv32(A, x, y);
+    JFEM_HIP(hipStreamSynchronize(A->ctx->stream));
+    API_END
+}
+
+int jfem_spmv_bench(jfem_mat* A, const jfem_vec* x, jfem_vec* y, int reps, int fp32, double* ms)
+{
+    API_BEGIN
+    JFEM_CHECK(A && x && y && ms && reps > 0, JFEM_ERR_ARG, "bad argument");
+    JFEM_CHECK(x->n == A->cols && y->n == A->rows, JFEM_ERR_ARG, "Incompatible sizes");
+    hipStream_t st = A->ctx->stream;
+    float *x32 = nullptr, *y32 = nullptr;
+    if (fp32) {
+        mat_make_f32(A);
+        JFEM_HIP(hipMalloc(&x32, sizeof(float) * A->cols));
+        JFEM_HIP(hipMalloc(&y32, sizeof(float) * A->rows));
+        k_to_f32<<<grid_for(A->cols, 256, 16384), 256, 0, st>>>(A->cols, x->d, x32);
+    }
+    hipEvent_t e0, e1;
+    JFEM_HIP(hipEventCreate(&e0));
+    JFEM_HIP(hipEventCreate(&e1));
+    // one warm-up launch
+    if (fp32) spmv32(A, x32, y32); else spmv(A, x->d, y->d);
+    JFEM_HIP(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; r++) {
+        if (fp32) spmv32(A, x32, y32); else spmv(A, x->d, y->d);
+    }
+    JFEM_HIP(hipEventRecord(e1, st));
+    JFEM_HIP(hipEventSynchronize(e1));
+    float f = 0;
+    JFEM_HIP(hipEventElapsedTime(&f, e0, e1));
+    *ms = f / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(x32);
+    (void)hipFree(y32);
+    API_END
+}
+
+}  // extern "C"
