@@ -1,0 +1,216 @@
+"""Benchmark of the JFEM Qk solve path on one MI355X.
+
+Workload (N=1): 3D Poisson -div grad u = f on the unit cube, Q2 ContinuousTPFESpace
+on 96^3 cells (7,189,057 DOFs), f = seeded sum of 16 3D sinusoids, zero Dirichlet,
+FP64 geometric multigrid (levels 3..96 cells, V(2,2) with Chebyshev-Jacobi degree 4,
+coarsest 3^3 solved directly) preconditioning CG to ||r|| <= 1e-8 ||b||.
+
+A step is one solve from x0 = 0 with the matrices, hierarchy and right-hand side
+already resident in HBM.  The line also reports assembly time, the FP64 SpMV's
+roofline, a flexible-CG solve around an FP32 V-cycle, and the CPU oracle timed
+on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_problem(jf, base, nlev, degree, smoother, precision, seed, nterms):
+    d = len(base)
+    spaces = []
+    for l in range(nlev):
+        s = jf.ContinuousTPFESpace([0.0] * d, [1.0] * d, [c << l for c in base])
+        s.assembleCells()
+        s.assembleFunctions(degree)
+        spaces.append(s)
+    ints = [jf.TPCellIntegral("GradGrad")]
+    t0 = time.perf_counter()
+    G = jf.MGPreconditioner(spaces, ints, smoother, cycles=1, fmg=False, precision=precision)
+    t_mg = time.perf_counter() - t0
+    return spaces, G, t_mg
+
+
+def rhs_for(jf, space, seed, nterms):
+    from jfem_amd.workloads import sinusoid_terms
+    kw, amp = sinusoid_terms(seed, nterms, space.getDimension())
+    space.initializeRhs()
+    space.writeCellIntegralsToRhs([jf.TPRightHandSideIntegral(jf.SinusoidSource(kw, amp))], space.getRhs())
+    r = space.getRhs().numpy()
+    r[space.boundary_nodes()] = 0.0
+    return jf.DenseVector(r), r
+
+
+def cpu_baseline(seconds_budget=30.0):
+    """Oracle (single-threaded C restatement) MG-PCG on a bounded sample: 3D Q2
+    16^3 cells (35,937 DOFs), same smoother and cycle; time scaled by the DOF ratio."""
+    from oracle import oracle as O
+    from jfem_amd.workloads import sinusoid_terms
+    base, nlev, k = [2, 2, 2], 4, 2
+    cells = [16, 16, 16]
+    t0 = time.perf_counter()
+    G = O.MG(base, k, nlev, [0, 0, 0], [1, 1, 1], [O.GRAD_GRAD], [1.0], smoother=O.SM_CHEBYSHEV, runs=2,
+             degree=4, lmin=[0.0] * nlev, lmax=[0.0] * nlev, cycles=1, fmg=False)
+    for l in range(1, nlev):
+        lmax = 1.1 * G.estimate_lmax(l)
+        G.set_interval(l, lmax / 20.0, lmax)
+    A = G.level_matrix(nlev - 1)
+    kw, amp = sinusoid_terms(2, 16, 3)
+    rhs = O.rhs_sines(cells, k, [0, 0, 0], [1, 1, 1], kw, amp)
+    rhs[O.boundary_nodes(cells, k)] = 0
+    setup = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    x, it, res, _ = O.solve("cg", A, rhs, np.zeros(A.n), 1e-8 * np.linalg.norm(rhs), precond=G)
+    solve = time.perf_counter() - t1
+    # SpMV throughput of the oracle on the sample matrix
+    xx = np.random.default_rng(0).standard_normal(A.n)
+    reps = 20
+    t2 = time.perf_counter()
+    for _ in range(reps):
+        A.matvec(xx)
+    t_spmv = (time.perf_counter() - t2) / reps
+    bytes_spmv = 12 * A.nnz + 4 * (A.n + 1) + 16 * A.n
+    return dict(solve_s=solve, iterations=it, n=A.n, setup_s=setup, spmv_gbs=bytes_spmv / t_spmv / 1e9)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", type=int, default=96, help="cells per axis of the finest grid (3 * 2^m)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-fp32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the solve path is single-GPU in this build "
+              "(multi-GPU slab decomposition is a design note only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import jfem_amd as jf
+
+    nlev = 1
+    c = args.cells
+    while c % 2 == 0 and c > 3:
+        c //= 2
+        nlev += 1
+    base = [c, c, c]
+    log(f"building hierarchy {base} x 2^{nlev - 1}")
+    smoother = jf.ChebyshevSmoother(degree=4, runs=2, ratio=20.0)
+    spaces, G, t_mg = build_problem(jf, base, nlev, 2, smoother, jf.FP64, 2, 16)
+    fine = spaces[-1]
+    A = G.getFinestSystem()
+    n, _, nnz = A.info()
+    log(f"n={n} nnz={nnz} hierarchy setup {t_mg:.2f}s")
+
+    # assembly of the finest matrix alone (jfem_assemble), timed
+    ctx = jf.context()
+    ctx.sync()
+    t0 = time.perf_counter()
+    A_asm = fine.assemble_matrix([jf.TPCellIntegral("GradGrad")])
+    ctx.sync()
+    t_asm = time.perf_counter() - t0
+    del A_asm
+    log(f"finest assembly {t_asm * 1e3:.1f} ms")
+
+    b, rhs = rhs_for(jf, fine, 2, 16)
+    bnorm = float(np.linalg.norm(rhs))
+    tol = 1e-8 * bnorm
+
+    def solve_fp64():
+        x = jf.DenseVector(n)
+        it = jf.IterativeSolver()
+        it.solvePCG(A, G, b, tol, x0=x)
+        return it, x
+
+    for w in range(args.warmup):
+        it, x = solve_fp64()
+        log(f"warmup {w}: {it.iterations} its, {it.info.seconds:.3f}s")
+    times, iters = [], []
+    ctx.sync()
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        it, x = solve_fp64()
+        times.append(it.info.seconds)
+        iters.append(it.iterations)
+        log(f"step {s}: {it.iterations} its, {it.info.seconds:.3f}s, resnorm/b {it.info.resnorm / bnorm:.2e}")
+    ctx.sync()
+    wall = time.perf_counter() - t_start
+    ms_per_step = wall / args.steps * 1e3
+    # true residual of the last solution
+    r = A.mvMul(x)
+    true_rel = float(np.linalg.norm(rhs - r.numpy()) / bnorm)
+    x64 = x.numpy()
+
+    # roofline of the FP64 SpMV on the finest level
+    xs = jf.DenseVector(np.random.default_rng(0).standard_normal(n))
+    ys = jf.DenseVector(n)
+    spmv_ms = A.spmv_time_ms(xs, ys, reps=50)
+    spmv_bytes = 12 * nnz + 4 * (n + 1) + 16 * n
+    spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    spmv32_ms = A.spmv_time_ms(xs, ys, reps=50, fp32=True)
+    spmv32_gbs = (8 * nnz + 4 * (n + 1) + 8 * n) / (spmv32_ms * 1e-3) / 1e9
+    log(f"SpMV fp64 {spmv_ms:.3f} ms {spmv_gbs:.0f} GB/s; fp32 {spmv32_ms:.3f} ms {spmv32_gbs:.0f} GB/s")
+
+    extra = {}
+    if not args.no_fp32:
+        del G
+        _, G32, t_mg32 = build_problem(jf, base, nlev, 2, smoother, jf.FP32, 2, 16)
+        A32 = G32.getFinestSystem()
+        t32 = []
+        for s in range(max(1, args.steps)):
+            it = jf.IterativeSolver()
+            x32 = it.solveFCG(A32, G32, b, tol)
+            t32.append(it.info.seconds)
+            log(f"fp32-MG FCG: {it.iterations} its, {it.info.seconds:.3f}s")
+        err = float(np.linalg.norm(x32.numpy() - x64) / np.linalg.norm(x64))
+        extra = {"fp32mg_fcg_s": float(np.median(t32)), "fp32mg_fcg_iterations": it.iterations,
+                 "fp32mg_vs_fp64_rel_l2": err, "speedup_fp32mg_vs_fp64": float(np.median(times) / np.median(t32))}
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU oracle baseline (bounded sample)")
+        cb = cpu_baseline()
+        scale = n / cb["n"]
+        cpu = {"value": cb["solve_s"] * scale, "unit": "s", "cores": 1, "kind": "port",
+               "sample": f"oracle MG-PCG (Chebyshev V(2,2) deg 4) on 3D Q2 16^3 cells ({cb['n']} DOFs), "
+                         f"{cb['iterations']} its in {cb['solve_s']:.3f}s, scaled by DOF ratio {scale:.1f}; "
+                         f"oracle SpMV {cb['spmv_gbs']:.2f} GB/s"}
+
+    value = float(np.median(times))
+    line = {
+        "metric": "time-to-1e-8 residual (s): FP64 MG-PCG on 3D Q2 96^3 (7.19M DOFs)",
+        "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3D Poisson Q2 {args.cells}^3 cells, FP64 GMG V(2,2) Chebyshev-Jacobi deg 4, CG to 1e-8",
+                   "ndofs": n, "nnz": nnz, "levels": nlev, "iterations": int(np.median(iters))},
+        "true_rel_residual": true_rel,
+        "assembly_s": t_asm, "hierarchy_setup_s": t_mg,
+        "spmv_fp64_ms": spmv_ms, "spmv_fp64_gbs": spmv_gbs, "spmv_fp32_ms": spmv32_ms, "spmv_fp32_gbs": spmv32_gbs,
+        "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_spmv<64,double,double> (finest-level CSR SpMV)",
+                     "bytes_per_launch": spmv_bytes},
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,16 @@
+"""MI355X (gfx950) implementation of JFEM's solve path for tensor-product Qk
+spaces: structured space and DOF numbering, TPCellIntegral assembly into CSR,
+Dirichlet elimination, SpMV, CG/BiCGStab/GMRES and geometric multigrid, all in
+libjfem_hip.so behind the C ABI of include/jfem_hip.h."""
+from ._native import (IllegalArgumentException, IllegalStateException, JfemError, NativeLibraryError,
+                      UnsupportedOperationException, lib)
+from .api import (FP32, FP64, ChebyshevSmoother, ContinuousTPFESpace, Context, DenseVector, IterativeSolver,
+                  JacobiSmoother, MGPreconditioner, QuadratureRule1D, RichardsonSmoother, SinusoidSource,
+                  SparseMatrix, TPCellIntegral, TPRightHandSideIntegral, context)
+
+__all__ = [
+    "IllegalArgumentException", "IllegalStateException", "JfemError", "NativeLibraryError",
+    "UnsupportedOperationException", "lib", "FP32", "FP64", "ChebyshevSmoother", "ContinuousTPFESpace", "Context",
+    "DenseVector", "IterativeSolver", "JacobiSmoother", "MGPreconditioner", "QuadratureRule1D",
+    "RichardsonSmoother", "SinusoidSource", "SparseMatrix", "TPCellIntegral", "TPRightHandSideIntegral", "context",
+]

@@ -1,0 +1,555 @@
+"""Host-side mirror of the reference's Java API for the Qk solve path.
+
+Class and method names follow the reference (ContinuousTPFESpace,
+TPCellIntegral, SparseMatrix, DenseVector, IterativeSolver, the multigrid
+preconditioner and its smoothers); every operation runs in libjfem_hip.so on
+the GPU through the C ABI.  Reference paths are relative to the JFEM
+repository, M/ = Core/src/main/java/.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Callable, Optional, Sequence
+
+import numpy as np
+
+from . import _native as N
+from ._native import check, dptr, f64, i32, iptr, lptr
+
+# --------------------------------------------------------------------------
+# context
+# --------------------------------------------------------------------------
+_ctx = None
+
+
+class Context:
+    """One HIP device and stream (jfem_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(N.lib().jfem_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def sync(self):
+        check(N.lib().jfem_ctx_sync(self.h))
+
+    def timer_start(self):
+        check(N.lib().jfem_timer_start(self.h))
+
+    def timer_stop(self) -> float:
+        ms = C.c_double()
+        check(N.lib().jfem_timer_stop(self.h, C.byref(ms)))
+        return ms.value
+
+
+def context() -> Context:
+    global _ctx
+    if _ctx is None:
+        _ctx = Context(0)
+    return _ctx
+
+
+# --------------------------------------------------------------------------
+# quadrature rules and integrals
+# --------------------------------------------------------------------------
+class QuadratureRule1D:
+    """QuadratureRule1D (M/tensorproduct/QuadratureRule1D.java:9-64) by id."""
+    Gauss1, Gauss2, Gauss3, Gauss4, Gauss5 = 1, 2, 3, 4, 5
+    GaussLobatto2, GaussLobatto3, GaussLobatto4, GaussLobatto5, GaussLobatto6 = 12, 13, 14, 15, 16
+
+
+class TPCellIntegral:
+    """TPCellIntegral (M/tensorproduct/TPCellIntegral.java:17-76) with a constant
+    weight; default quadrature Gauss5 as in the reference (:36-39, :58-61)."""
+    GRAD_GRAD = "GradGrad"
+    VALUE_VALUE = "ValueValue"
+    H1 = "H1"
+    _KIND = {GRAD_GRAD: 0, VALUE_VALUE: 1, H1: 2}
+
+    def __init__(self, *args, quadratureRule1D: int = QuadratureRule1D.Gauss5):
+        # TPCellIntegral(name) | TPCellIntegral(weight, name) | (weight, name, rule)
+        if len(args) == 1:
+            weight, name = 1.0, args[0]
+        elif len(args) == 2:
+            weight, name = args
+        elif len(args) == 3:
+            weight, name, quadratureRule1D = args
+        else:
+            raise TypeError("TPCellIntegral(name) or TPCellIntegral(weight, name[, rule])")
+        if name not in self._KIND:
+            raise N.UnsupportedOperationException(3, "unknown integral name")
+        if not isinstance(weight, (int, float)):
+            raise N.IllegalArgumentException(1, f"{name} needs a Double weight")
+        self.weight = float(weight)
+        self.name = name
+        self.quadratureRule1D = quadratureRule1D
+
+    @property
+    def kind(self):
+        return self._KIND[self.name]
+
+
+class SinusoidSource:
+    """f(x) = sum_j amp[j] prod_a sin(kw[j][a] pi x_a)."""
+
+    def __init__(self, kw, amp):
+        self.kw = i32(kw)
+        self.amp = f64(amp)
+        if self.kw.ndim != 2 or self.kw.shape[0] != self.amp.shape[0]:
+            raise ValueError("kw must be (nterms, dim) and amp (nterms,)")
+
+    def __call__(self, x):
+        x = np.asarray(x, float)
+        s = 0.0
+        for j in range(len(self.amp)):
+            t = self.amp[j]
+            for a in range(x.shape[-1]):
+                t = t * np.sin(self.kw[j, a] * np.pi * x[..., a])
+            s = s + t
+        return s
+
+
+class TPRightHandSideIntegral:
+    """TPRightHandSideIntegral VALUE (M/tensorproduct/TPRightHandSideIntegral.java:22-31)."""
+    VALUE = "Value"
+
+    def __init__(self, rightHandSide: SinusoidSource, name: str = "Value",
+                 quadratureRule1D: int = QuadratureRule1D.Gauss5):
+        if name != self.VALUE:
+            raise N.UnsupportedOperationException(3, "unknown rhs integral")
+        if not isinstance(rightHandSide, SinusoidSource):
+            raise N.UnsupportedOperationException(3, "GPU right-hand sides are sinusoid sums")
+        self.rightHandSide = rightHandSide
+        self.quadratureRule1D = quadratureRule1D
+
+
+# --------------------------------------------------------------------------
+# vectors and matrices
+# --------------------------------------------------------------------------
+class DenseVector:
+    """Device-resident FP64 vector (DenseVector, M/linalg/DenseVector.java)."""
+
+    def __init__(self, n_or_values, ctx: Optional[Context] = None):
+        self.ctx = ctx or context()
+        if np.isscalar(n_or_values):
+            n, values = int(n_or_values), None
+        else:
+            values = f64(n_or_values)
+            n = values.shape[0]
+        h = C.c_void_p()
+        check(N.lib().jfem_vec_create(self.ctx.h, n, C.byref(h)))
+        self.h = h
+        self.n = n
+        if values is not None and n:
+            check(N.lib().jfem_vec_upload(h, dptr(values)))
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and N._lib is not None:
+            N.lib().jfem_vec_destroy(h)
+            self.h = None
+
+    def getLength(self):
+        return self.n
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.n)
+        if self.n:
+            check(N.lib().jfem_vec_download(self.h, dptr(out)))
+        return out
+
+    def set_values(self, values):
+        v = f64(values)
+        if v.shape[0] != self.n:
+            raise N.IllegalArgumentException(1, "Vectors have different size")
+        check(N.lib().jfem_vec_upload(self.h, dptr(v)))
+
+    def fill(self, value: float):
+        check(N.lib().jfem_vec_fill(self.h, float(value)))
+
+    def inner(self, other: "DenseVector") -> float:
+        r = C.c_double()
+        check(N.lib().jfem_vec_dot(self.h, other.h, C.byref(r)))
+        return r.value
+
+    def euclidianNorm(self) -> float:
+        return float(np.sqrt(self.inner(self)))
+
+    def copy(self) -> "DenseVector":
+        v = DenseVector(self.n, self.ctx)
+        check(N.lib().jfem_vec_axpby(1.0, self.h, 0.0, v.h))
+        return v
+
+
+class SparseMatrix:
+    """Device CSR matrix behind the SparseMatrix / VectorMultiplyable interface."""
+
+    def __init__(self, handle, ctx: Context, owned: bool = True, keepalive=None):
+        self.h = handle
+        self.ctx = ctx
+        self.owned = owned
+        self._keepalive = keepalive
+
+    @classmethod
+    def from_csr(cls, rowptr, cols, vals, shape=None, ctx: Optional[Context] = None):
+        ctx = ctx or context()
+        rp = np.ascontiguousarray(rowptr, dtype=np.int64)
+        n = rp.shape[0] - 1
+        ncols = n if shape is None else shape[1]
+        h = C.c_void_p()
+        check(N.lib().jfem_mat_from_csr(ctx.h, n, ncols, lptr(rp), iptr(i32(cols)), dptr(f64(vals)), C.byref(h)))
+        return cls(h, ctx)
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and self.owned and N._lib is not None:
+            N.lib().jfem_mat_destroy(h)
+            self.h = None
+
+    def info(self):
+        r, c, z = C.c_long(), C.c_long(), C.c_long()
+        check(N.lib().jfem_mat_info(self.h, C.byref(r), C.byref(c), C.byref(z)))
+        return r.value, c.value, z.value
+
+    def getRows(self):
+        return self.info()[0]
+
+    def getCols(self):
+        return self.info()[1]
+
+    def getVectorSize(self):
+        return self.getCols()
+
+    def getTVectorSize(self):
+        return self.getRows()
+
+    def nnz(self):
+        return self.info()[2]
+
+    def to_csr(self):
+        n, _, nnz = self.info()
+        rp = np.zeros(n + 1, dtype=np.int64)
+        cols = np.zeros(max(nnz, 1), dtype=np.int32)
+        vals = np.zeros(max(nnz, 1))
+        check(N.lib().jfem_mat_export_csr(self.h, lptr(rp), iptr(cols), dptr(vals)))
+        return rp, cols[:nnz], vals[:nnz]
+
+    def getCoordinateEntryList(self) -> dict:
+        """SparseMatrix.getCoordinateEntryList (M/linalg/SparseMatrix.java:402-418)."""
+        rp, cols, vals = self.to_csr()
+        out = {}
+        for r in range(len(rp) - 1):
+            for p in range(rp[r], rp[r + 1]):
+                out[(r, int(cols[p]))] = float(vals[p])
+        return out
+
+    def mvMul(self, v: DenseVector, out: Optional[DenseVector] = None) -> DenseVector:
+        """SparseMatrix.mvMul (M/linalg/SparseMatrix.java:479-489)."""
+        y = out if out is not None else DenseVector(self.getRows(), self.ctx)
+        check(N.lib().jfem_spmv(self.h, v.h, y.h))
+        return y
+
+    def spmv_time_ms(self, x: DenseVector, y: DenseVector, reps: int = 20, fp32: bool = False) -> float:
+        ms = C.c_double()
+        check(N.lib().jfem_spmv_bench(self.h, x.h, y.h, reps, 1 if fp32 else 0, C.byref(ms)))
+        return ms.value
+
+
+# --------------------------------------------------------------------------
+# space
+# --------------------------------------------------------------------------
+class ContinuousTPFESpace:
+    """ContinuousTPFESpace (M/tensorproduct/ContinuousTPFESpace.java:13-50) on the GPU."""
+
+    def __init__(self, startCoordinates: Sequence[float], endCoordinates: Sequence[float],
+                 cellsPerDimension: Sequence[int], ctx: Optional[Context] = None):
+        if len(startCoordinates) != len(endCoordinates) or len(startCoordinates) != len(cellsPerDimension):
+            raise N.IllegalArgumentException(1, "Dimensions of coordinates or cell counts do not match")
+        self.ctx = ctx or context()
+        self.start = f64(startCoordinates)
+        self.end = f64(endCoordinates)
+        self.cells = i32(cellsPerDimension)
+        self.dim = len(self.cells)
+        self.h = None
+        self.degree = None
+        self.systemMatrix: Optional[SparseMatrix] = None
+        self.rhs: Optional[DenseVector] = None
+        self.fixedNodes: set = set()
+
+    def __del__(self):
+        if getattr(self, "h", None) and N._lib is not None:
+            N.lib().jfem_space_destroy(self.h)
+            self.h = None
+
+    def assembleCells(self):
+        pass
+
+    def assembleFunctions(self, polynomialDegree: int):
+        h = C.c_void_p()
+        check(N.lib().jfem_space_create(self.ctx.h, self.dim, iptr(self.cells), dptr(self.start), dptr(self.end),
+                                        polynomialDegree, C.byref(h)))
+        self.h = h
+        self.degree = polynomialDegree
+
+    def _need(self):
+        if self.h is None:
+            raise N.IllegalStateException(2, "assembleFunctions has not been called")
+
+    def getDimension(self):
+        return self.dim
+
+    def ndofs(self) -> int:
+        self._need()
+        return N.lib().jfem_space_ndofs(self.h)
+
+    def cell_dofs(self) -> np.ndarray:
+        self._need()
+        nc = int(np.prod(self.cells.astype(np.int64)))
+        out = np.zeros(nc * (self.degree + 1) ** self.dim, dtype=np.int32)
+        check(N.lib().jfem_space_cell_dofs(self.h, iptr(out)))
+        return out.reshape(nc, -1)
+
+    def boundary_nodes(self) -> np.ndarray:
+        self._need()
+        out = np.zeros(self.ndofs(), dtype=np.int32)
+        cnt = C.c_long()
+        check(N.lib().jfem_space_boundary_nodes(self.h, iptr(out), C.byref(cnt)))
+        return out[:cnt.value].copy()
+
+    def node_coords(self) -> np.ndarray:
+        self._need()
+        out = np.zeros(self.ndofs() * self.dim)
+        check(N.lib().jfem_space_node_coords(self.h, dptr(out)))
+        return out.reshape(-1, self.dim)
+
+    def initializeSystemMatrix(self):
+        self._need()
+        self.systemMatrix = None
+
+    def initializeRhs(self):
+        self._need()
+        self.rhs = DenseVector(self.ndofs(), self.ctx)
+
+    def getSystemMatrix(self) -> SparseMatrix:
+        return self.systemMatrix
+
+    def getRhs(self) -> DenseVector:
+        return self.rhs
+
+    def getFixedNodeIndices(self) -> set:
+        return self.fixedNodes
+
+    def assemble_matrix(self, cellIntegrals: Sequence[TPCellIntegral], cell_coeff=None) -> SparseMatrix:
+        self._need()
+        if not cellIntegrals:
+            raise N.IllegalArgumentException(1, "no cell integrals")
+        rules = {ci.quadratureRule1D for ci in cellIntegrals}
+        if len(rules) != 1:
+            raise N.UnsupportedOperationException(3, "all integrals must share one quadrature rule")
+        kinds = i32([ci.kind for ci in cellIntegrals])
+        w = f64([ci.weight for ci in cellIntegrals])
+        cc = None if cell_coeff is None else f64(cell_coeff)
+        h = C.c_void_p()
+        check(N.lib().jfem_assemble(self.h, len(kinds), iptr(kinds), dptr(w), dptr(cc) if cc is not None else None,
+                                    rules.pop(), C.byref(h)))
+        return SparseMatrix(h, self.ctx)
+
+    def writeCellIntegralsToRhs(self, rhsIntegrals: Sequence[TPRightHandSideIntegral], d: DenseVector):
+        for ri in rhsIntegrals:
+            src = ri.rightHandSide
+            tmp = DenseVector(self.ndofs(), self.ctx)
+            check(N.lib().jfem_assemble_rhs_sines(self.h, len(src.amp), iptr(src.kw), dptr(src.amp),
+                                                  ri.quadratureRule1D, tmp.h))
+            check(N.lib().jfem_vec_axpby(1.0, tmp.h, 1.0, d.h))
+
+    def evaluateCellIntegrals(self, cellIntegrals, rightHandSideIntegrals, cell_coeff=None):
+        """MatrixFESpace.evaluateCellIntegrals (M/basic/MatrixFESpace.java:22-27)."""
+        if self.rhs is None:
+            self.initializeRhs()
+        self.systemMatrix = self.assemble_matrix(cellIntegrals, cell_coeff)
+        if rightHandSideIntegrals:
+            self.writeCellIntegralsToRhs(rightHandSideIntegrals, self.rhs)
+
+    def setBoundaryValues(self, boundaryValues: Optional[Callable] = None):
+        """AcceptsBoundaryValues.setBoundaryValues (M/basic/AcceptsBoundaryValues.java:20-51)
+        over the whole boundary; boundaryValues(x) -> float, None = zero data."""
+        nodes = self.boundary_nodes()
+        vals = None
+        if boundaryValues is not None:
+            xyz = self.node_coords()
+            vals = f64([boundaryValues(xyz[i]) for i in nodes])
+        check(N.lib().jfem_apply_dirichlet(self.systemMatrix.h, self.rhs.h, len(nodes), iptr(nodes),
+                                           dptr(vals) if vals is not None else None))
+        self.fixedNodes.update(int(i) for i in nodes)
+
+
+# --------------------------------------------------------------------------
+# multigrid
+# --------------------------------------------------------------------------
+class RichardsonSmoother:
+    """RichardsonSmoother(omega, runs) (M/multigrid/RichardsonSmoother.java:13-50)."""
+    kind = 0
+
+    def __init__(self, omega: float, runs: int):
+        self.omega, self.runs = omega, runs
+
+
+class JacobiSmoother:
+    """JacobiSmoother(runs, omega) (M/multigrid/JacobiSmoother.java:12-50)."""
+    kind = 1
+
+    def __init__(self, runs: int, omega: float):
+        self.runs, self.omega = runs, omega
+
+
+class ChebyshevSmoother:
+    """Chebyshev-Jacobi smoother of the given degree, applied `runs` times."""
+    kind = 2
+
+    def __init__(self, degree: int = 4, runs: int = 2, ratio: float = 20.0, safety: float = 1.1, power_its: int = 30):
+        self.degree, self.runs, self.ratio, self.safety, self.power_its = degree, runs, ratio, safety, power_its
+        self.omega = 1.0
+
+
+FP64, FP32 = 0, 1
+
+
+class MGPreconditioner:
+    """Geometric multigrid preconditioner over nested ContinuousTPFESpaces
+    (MGPreconditionerSpace + MGPreconditionerInterface.mvMul,
+    M/multigrid/MGPreconditionerInterface.java:352-379).  Levels: coarsest first."""
+
+    def __init__(self, spaces: Sequence[ContinuousTPFESpace], cellIntegrals: Sequence[TPCellIntegral], smoother,
+                 cycles: int = 3, fmg: bool = True, precision: int = FP64, cell_coeff=None):
+        self.ctx = spaces[0].ctx
+        self.spaces = list(spaces)
+        opts = N.MgOpts()
+        opts.smoother = smoother.kind
+        opts.runs = smoother.runs
+        opts.omega = getattr(smoother, "omega", 1.0)
+        opts.degree = getattr(smoother, "degree", 0)
+        opts.cheb_ratio = getattr(smoother, "ratio", 0.0)
+        opts.cheb_safety = getattr(smoother, "safety", 1.1)
+        opts.power_its = getattr(smoother, "power_its", 30)
+        opts.cycles = cycles
+        opts.fmg = 1 if fmg else 0
+        opts.precision = precision
+        kinds = i32([ci.kind for ci in cellIntegrals])
+        w = f64([ci.weight for ci in cellIntegrals])
+        arr = (C.c_void_p * len(spaces))(*[s.h for s in spaces])
+        cc = None if cell_coeff is None else f64(cell_coeff)
+        h = C.c_void_p()
+        check(N.lib().jfem_mg_create(arr, len(spaces), len(kinds), iptr(kinds), dptr(w),
+                                     dptr(cc) if cc is not None else None, cellIntegrals[0].quadratureRule1D,
+                                     C.byref(opts), C.byref(h)))
+        self.h = h
+        self.nlev = len(spaces)
+
+    def __del__(self):
+        if getattr(self, "h", None) and N._lib is not None:
+            N.lib().jfem_mg_destroy(self.h)
+            self.h = None
+
+    def getSystem(self, level: int) -> SparseMatrix:
+        h = C.c_void_p()
+        check(N.lib().jfem_mg_level_matrix(self.h, level, C.byref(h)))
+        return SparseMatrix(h, self.ctx, owned=False, keepalive=self)
+
+    def getFinestSystem(self) -> SparseMatrix:
+        return self.getSystem(self.nlev - 1)
+
+    def getVectorSize(self):
+        return self.spaces[-1].ndofs()
+
+    def mvMul(self, r: DenseVector, out: Optional[DenseVector] = None) -> DenseVector:
+        z = out if out is not None else DenseVector(r.n, self.ctx)
+        check(N.lib().jfem_mg_apply(self.h, r.h, z.h))
+        return z
+
+    def vCycle(self, x: DenseVector, b: DenseVector) -> DenseVector:
+        check(N.lib().jfem_mg_vcycle(self.h, x.h, b.h))
+        return x
+
+    def lmax(self) -> np.ndarray:
+        out = np.zeros(self.nlev)
+        check(N.lib().jfem_mg_lmax(self.h, dptr(out)))
+        return out
+
+    def tvMul(self, v):
+        raise N.UnsupportedOperationException(3, "not implemented yet")
+
+
+# --------------------------------------------------------------------------
+# iterative solvers
+# --------------------------------------------------------------------------
+class IterativeSolver:
+    """IterativeSolver (M/linalg/IterativeSolver.java:11-402): stateful, keeps
+    `iterations` and `lastSolution` (next start iterate) like the reference."""
+
+    def __init__(self, showProgress: bool = False):
+        self.iterations = 0
+        self.lastSolution: Optional[np.ndarray] = None
+        self.info = N.SolveInfo()
+        self.residual_history: Optional[np.ndarray] = None
+
+    def _start(self, n, ctx, default_e0=True):
+        if self.lastSolution is not None:
+            return DenseVector(self.lastSolution, ctx)
+        x0 = np.zeros(n)
+        if default_e0 and n:
+            x0[0] = 1.0
+        return DenseVector(x0, ctx)
+
+    def _finish(self, x: DenseVector):
+        self.iterations = self.info.iterations
+        self.lastSolution = x.numpy()
+        return x
+
+    def solveCG(self, operator: SparseMatrix, rhs: DenseVector, tol: float, maxit: int = 2 ** 31 - 1,
+                x0: Optional[DenseVector] = None) -> DenseVector:
+        x = x0 if x0 is not None else self._start(rhs.n, operator.ctx)
+        hist = np.zeros(min(rhs.n, maxit) + 1)
+        check(N.lib().jfem_solve_cg(operator.h, None, rhs.h, x.h, tol, maxit, C.byref(self.info), dptr(hist)))
+        self.residual_history = hist[:self.info.iterations].copy()
+        return self._finish(x)
+
+    def solvePCG(self, operator: SparseMatrix, preconditioner: MGPreconditioner, rhs: DenseVector, tol: float,
+                 maxit: int = 2 ** 31 - 1, x0: Optional[DenseVector] = None) -> DenseVector:
+        x = x0 if x0 is not None else self._start(rhs.n, operator.ctx)
+        hist = np.zeros(min(rhs.n, maxit) + 1)
+        check(N.lib().jfem_solve_cg(operator.h, preconditioner.h, rhs.h, x.h, tol, maxit, C.byref(self.info),
+                                    dptr(hist)))
+        self.residual_history = hist[:self.info.iterations].copy()
+        return self._finish(x)
+
+    def solveFCG(self, operator: SparseMatrix, preconditioner: MGPreconditioner, rhs: DenseVector, tol: float,
+                 maxit: int = 100000, x0: Optional[DenseVector] = None) -> DenseVector:
+        """Flexible CG (FP64) around a multigrid preconditioner of any precision (not in the reference)."""
+        x = x0 if x0 is not None else DenseVector(rhs.n, operator.ctx)
+        hist = np.zeros(min(rhs.n, maxit) + 1)
+        check(N.lib().jfem_solve_fcg(operator.h, preconditioner.h, rhs.h, x.h, tol, maxit, C.byref(self.info),
+                                     dptr(hist)))
+        self.residual_history = hist[:self.info.iterations].copy()
+        return self._finish(x)
+
+    def solveBiCGStab(self, operator, rhs, tol, maxit: int = 2 ** 31 - 1, x0=None):
+        x = x0 if x0 is not None else self._start(rhs.n, operator.ctx)
+        check(N.lib().jfem_solve_bicgstab(operator.h, None, rhs.h, x.h, tol, maxit, C.byref(self.info)))
+        return self._finish(x)
+
+    def solvePBiCGStab(self, operator, preconditioner, rhs, tol, maxit: int = 2 ** 31 - 1, x0=None):
+        x = x0 if x0 is not None else self._start(rhs.n, operator.ctx)
+        check(N.lib().jfem_solve_bicgstab(operator.h, preconditioner.h, rhs.h, x.h, tol, maxit,
+                                          C.byref(self.info)))
+        return self._finish(x)
+
+    def solveGMRES(self, operator, rhs, tol, restart: int = 150, max_cycles: int = 200, x0=None):
+        x = x0 if x0 is not None else self._start(rhs.n, operator.ctx)
+        check(N.lib().jfem_solve_gmres(operator.h, None, rhs.h, x.h, tol, restart, max_cycles, C.byref(self.info)))
+        return self._finish(x)
+
+    def solvePGMRES(self, operator, preconditioner, rhs, tol, restart: int = 150, max_cycles: int = 200, x0=None):
+        x = x0 if x0 is not None else self._start(rhs.n, operator.ctx, default_e0=False)
+        check(N.lib().jfem_solve_gmres(operator.h, preconditioner.h, rhs.h, x.h, tol, restart, max_cycles,
+                                       C.byref(self.info)))
+        return self._finish(x)

@@ -1,0 +1,57 @@
+"""Seeded synthetic inputs of the benchmark workloads.
+
+SplitMix64 (Steele, Lea, Flood, OOPSLA 2014) drives every random choice so the
+same seed gives the same problem on any machine; normals use Box-Muller.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+MASK = (1 << 64) - 1
+
+
+class SplitMix64:
+    def __init__(self, seed: int):
+        self.state = seed & MASK
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & MASK
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK
+        return z ^ (z >> 31)
+
+    def uniform(self) -> float:
+        """uniform in [0, 1) with 53 random bits"""
+        return (self.next_u64() >> 11) * (1.0 / (1 << 53))
+
+    def randint(self, lo: int, hi: int) -> int:
+        """uniform integer in [lo, hi]"""
+        return lo + self.next_u64() % (hi - lo + 1)
+
+    def normal(self) -> float:
+        u1 = max(self.uniform(), 1e-300)
+        u2 = self.uniform()
+        return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+
+
+def sinusoid_terms(seed: int, nterms: int, dim: int, kmax: int = 8):
+    """(kw[nterms, dim], amp[nterms]): integer wavenumbers 1..kmax, N(0,1) amplitudes."""
+    rng = SplitMix64(seed)
+    kw = np.zeros((nterms, dim), dtype=np.int32)
+    amp = np.zeros(nterms)
+    for j in range(nterms):
+        for a in range(dim):
+            kw[j, a] = rng.randint(1, kmax)
+        amp[j] = rng.normal()
+    return kw, amp
+
+
+# the five workloads: name -> (dim, degree, cells, start, end, seed, nterms)
+CONFIGS = {
+    "c1_2d_q1_64": dict(dim=2, degree=1, cells=(64, 64), seed=1, nterms=8),
+    "c2_3d_q2_96": dict(dim=3, degree=2, cells=(96, 96, 96), seed=2, nterms=16),
+    "c4_2d_q1_1024": dict(dim=2, degree=1, cells=(1024, 1024), seed=4, nterms=8),
+}

@@ -234,55 +234,76 @@ static double inner(const double *a, const double *b, int n)
     return r;
 }
 
+/* Per-cell 1D tables: the values the reference's basis evaluations produce at
+ * the cell's quadrature points (computed once per cell; the integral below
+ * consumes exactly these numbers in the reference's order). */
+typedef struct {
+    int Q;
+    double pw[ORC_MAXD][2][8];
+    double val[ORC_MAXD][8][8];   /* [axis][l][q] */
+    double der[ORC_MAXD][8][8];
+} orc_tabs;
+
+static void cell_tabs(const orc_cell *K, int rule, orc_tabs *T)
+{
+    double qp[8], qw[8];
+    T->Q = orc_quad(rule, qp, qw);
+    for (int j = 0; j < K->d; j++)                /* Cell1D.distributeQuadrature :24-33 */
+        for (int i = 0; i < T->Q; i++) {
+            T->pw[j][0][i] = qp[i] * (K->e[j] - K->s[j]) + K->s[j];
+            T->pw[j][1][i] = qw[i] * (K->e[j] - K->s[j]);
+            for (int l = 0; l <= K->k; l++) {
+                T->val[j][l][i] = lag_val_cell(K->s[j], K->e[j], l, K->k, T->pw[j][0][i]);
+                T->der[j][l][i] = lag_der_cell(K->s[j], K->e[j], l, K->k, T->pw[j][0][i]);
+            }
+        }
+}
+
 /* One cell integral between local functions u (shapeFunction1) and v
  * (shapeFunction2), local multi-indices lu/lv.  Weight: wfun(x) if given, else
  * the constant wconst; vector weight vfun for GRAD_VALUE/VALUE_GRAD. */
-double orc_cell_integral_ex(const orc_cell *K, int kind, int rule, const int *lu, const int *lv,
+static double integral_tabs(const orc_cell *K, const orc_tabs *T, int kind, const int *lu, const int *lv,
                             double wconst, orc_sfun wfun, orc_vfun vfun, void *ctx)
 {
-    double qp[8], qw[8];
-    const int Q = orc_quad(rule, qp, qw);
-    const int d = K->d;
-    double pw[ORC_MAXD][2][8];
-    for (int j = 0; j < d; j++)                /* Cell1D.distributeQuadrature :24-33 */
-        for (int i = 0; i < Q; i++) {
-            pw[j][0][i] = qp[i] * (K->e[j] - K->s[j]) + K->s[j];
-            pw[j][1][i] = qw[i] * (K->e[j] - K->s[j]);
-        }
+    const int d = K->d, Q = T->Q;
     int nq = 1;
     for (int j = 0; j < d; j++) nq *= Q;
     double ret = 0, x[ORC_MAXD];
     for (int q = 0; q < nq; q++) {
         int c[ORC_MAXD], t = q;
         for (int j = d - 1; j >= 0; j--) { c[j] = t % Q; t /= Q; }   /* last axis fastest */
-        for (int j = 0; j < d; j++) x[j] = pw[j][0][c[j]];
+        for (int j = 0; j < d; j++) x[j] = T->pw[j][0][c[j]];
         double val, g1[ORC_MAXD], g2[ORC_MAXD], wv[ORC_MAXD];
+        double v1 = 1, v2 = 1;                   /* fastValueInCell :154-171 */
+        for (int i = 0; i < d; i++) { v1 *= T->val[i][lu[i]][c[i]]; v2 *= T->val[i][lv[i]][c[i]]; }
+        for (int i = 0; i < d; i++) {            /* fastGradientInCell :173-198 */
+            double c1 = 1, c2 = 1;
+            for (int j = 0; j < d; j++) {
+                if (i == j) { c1 *= T->der[j][lu[j]][c[j]]; c2 *= T->der[j][lv[j]][c[j]]; }
+                else { c1 *= T->val[j][lu[j]][c[j]]; c2 *= T->val[j][lv[j]][c[j]]; }
+            }
+            g1[i] = c1; g2[i] = c2;
+        }
         switch (kind) {
-        case ORC_GRAD_GRAD:
-            sf_grad(K, lu, x, g1); sf_grad(K, lv, x, g2);
-            val = inner(g1, g2, d) * (wfun ? wfun(x, d, ctx) : wconst);
-            break;
-        case ORC_VALUE_VALUE:
-            val = sf_value(K, lu, x) * sf_value(K, lv, x) * (wfun ? wfun(x, d, ctx) : wconst);
-            break;
-        case ORC_H1:
-            sf_grad(K, lu, x, g1); sf_grad(K, lv, x, g2);
-            val = (sf_value(K, lu, x) * sf_value(K, lv, x) + inner(g1, g2, d)) * (wfun ? wfun(x, d, ctx) : wconst);
-            break;
-        case ORC_GRAD_VALUE:
-            sf_grad(K, lu, x, g1); vfun(x, d, wv, ctx);
-            val = inner(g1, wv, d) * sf_value(K, lv, x);
-            break;
-        case ORC_VALUE_GRAD:
-            sf_grad(K, lv, x, g2); vfun(x, d, wv, ctx);
-            val = inner(g2, wv, d) * sf_value(K, lu, x);
-            break;
+        case ORC_GRAD_GRAD: val = inner(g1, g2, d) * (wfun ? wfun(x, d, ctx) : wconst); break;
+        case ORC_VALUE_VALUE: val = v1 * v2 * (wfun ? wfun(x, d, ctx) : wconst); break;
+        case ORC_H1: val = (v1 * v2 + inner(g1, g2, d)) * (wfun ? wfun(x, d, ctx) : wconst); break;
+        case ORC_GRAD_VALUE: vfun(x, d, wv, ctx); val = inner(g1, wv, d) * v2; break;
+        case ORC_VALUE_GRAD: vfun(x, d, wv, ctx); val = inner(g2, wv, d) * v1; break;
         default: return NAN;
         }
-        for (int j = 0; j < d; j++) val *= pw[j][1][c[j]];
+        for (int j = 0; j < d; j++) val *= T->pw[j][1][c[j]];
         ret += val;
     }
     return ret;
+}
+
+double orc_cell_integral_ex(const orc_cell *K, int kind, int rule, const int *lu, const int *lv,
+                            double wconst, orc_sfun wfun, orc_vfun vfun, void *ctx)
+{
+    orc_tabs T;
+    cell_tabs(K, rule, &T);
+    return integral_tabs(K, &T, kind, lu, lv, wconst, wfun, vfun, ctx);
 }
 
 /* Convenience entry for ctypes: cell given by starts/ends arrays. */
@@ -429,6 +450,8 @@ orc_csr *orc_assemble(int d, int k, const int *cells, const double *start, const
         int cc[ORC_MAXD];
         orc_cell K;
         cell_geometry(&D, c, cc, &K, g_s1d, g_e1d);
+        orc_tabs T;
+        cell_tabs(&K, rule, &T);
         for (int iu = 0; iu < nloc; iu++) {          /* function1 = u (column) */
             int r = iu;
             for (int a = 0; a < d; a++) { lu[a] = r % (k + 1); r /= (k + 1); }
@@ -438,7 +461,7 @@ orc_csr *orc_assemble(int d, int k, const int *cells, const double *start, const
                 double integral = 0;
                 for (int i = 0; i < nint; i++) {
                     const double w = cell_coeff ? cell_coeff[c] * wconst[i] : wconst[i];
-                    integral += orc_cell_integral_ex(&K, kinds[i], rule, lu, lv, w, NULL, NULL, NULL);
+                    integral += integral_tabs(&K, &T, kinds[i], lu, lv, w, NULL, NULL, NULL);
                 }
                 if (integral != 0) {
                     coo[m].row = cd[c * nloc + iv];
@@ -1323,6 +1346,7 @@ void orc_mg_prol_csr(const orc_mg *G, int l, long *rowptr, int *cols, double *va
     memcpy(vals, L->pval, sizeof(double) * L->prow[L->pn]);
 }
 void orc_mg_smooth(const orc_mg *G, int l, const double *b, double *x) { smooth(&G->lev[l], b, x); }
+void orc_mg_set_interval(orc_mg *G, int l, double lmin, double lmax) { G->lev[l].sm.lmin = lmin; G->lev[l].sm.lmax = lmax; }
 
 /* ------------------------------------------------------------------------- */
 /* Solver entry points over CSR / dense / MG operators, for ctypes.           */

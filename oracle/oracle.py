@@ -89,6 +89,7 @@ def lib():
         L.orc_mg_apply.argtypes = [C.c_void_p, dp, dp]
         L.orc_mg_vcycle.argtypes = [C.c_void_p, dp, dp]
         L.orc_mg_smooth.argtypes = [C.c_void_p, C.c_int, dp, dp]
+        L.orc_mg_set_interval.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
         L.orc_solve.argtypes = [C.c_int, C.c_long, lp, ip, dp, C.c_int, C.c_void_p, lp, ip, dp,
                                 dp, dp, C.c_double, C.c_long, C.c_int, C.POINTER(_SolveOut), dp]
         L.orc_splitmix64.restype = C.c_uint64
@@ -347,6 +348,31 @@ class MG:
         v = np.zeros(nnz)
         lib().orc_mg_prol_csr(self.ptr, l, _l(rp), _i(c), _d(v))
         return CSR(nf, rp, c, v, self.level_n(l))
+
+    def set_interval(self, l, lmin, lmax):
+        lib().orc_mg_set_interval(self.ptr, l, lmin, lmax)
+
+    def estimate_lmax(self, l, its=30):
+        """power iteration on D^-1 A of level l (same recurrence as the device estimate)"""
+        A = self.level_matrix(l).to_scipy()
+        dinv = self.level_dinv(l)
+        n = A.shape[0]
+        i = np.arange(n, dtype=np.uint64)
+        with np.errstate(over="ignore"):
+            z = i * np.uint64(0x9E3779B97F4A7C15) + np.uint64(0x632BE59BD9B4E019)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        v = 0.5 + (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+        v /= np.linalg.norm(v)
+        lam = 0.0
+        for _ in range(its):
+            w = dinv * (A @ v)
+            lam = np.linalg.norm(w)
+            if lam == 0:
+                break
+            v = w / lam
+        return lam
 
     def apply(self, v):
         v = np.ascontiguousarray(v, dtype=np.float64)

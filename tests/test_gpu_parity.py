@@ -1,0 +1,283 @@
+"""Parity of the HIP path (libjfem_hip.so through the C ABI) with the CPU
+oracle on seeded inputs.  Integer and index results (DOF numbering, boundary
+nodes, CSR pattern) must be bit-exact; assembled values with constant weights
+are bit-exact too (same operation sequence); floating-point results of
+reductions are compared with the tolerance written in each test."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GRIDS = [([1, 1], 1), ([5, 3], 1), ([4, 3], 2), ([3, 2, 2], 2), ([2, 3, 2], 1), ([4, 3], 3), ([2, 2, 1], 3),
+         ([7], 2), ([2, 2], 4)]
+
+
+def _space(jf, cells, k, start=None, end=None):
+    d = len(cells)
+    start = start or [0.0, -1.0, 0.25][:d]
+    end = end or [1.0, 1.0, 2.0][:d]
+    s = jf.ContinuousTPFESpace(start, end, cells)
+    s.assembleCells()
+    s.assembleFunctions(k)
+    return s, start, end
+
+
+@pytest.mark.parametrize("cells,k", GRIDS)
+def test_dof_numbering_bit_exact(jf, oracle, cells, k):
+    s, start, end = _space(jf, cells, k)
+    n, table = oracle.dof_table(cells, k)
+    assert s.ndofs() == n
+    assert np.array_equal(s.cell_dofs(), table)
+
+
+@pytest.mark.parametrize("cells,k", GRIDS)
+def test_boundary_nodes_and_coords(jf, oracle, cells, k):
+    s, start, end = _space(jf, cells, k)
+    assert np.array_equal(s.boundary_nodes(), oracle.boundary_nodes(cells, k))
+    assert np.array_equal(s.node_coords(), oracle.node_coords(cells, k, start, end))
+
+
+ASM_CASES = [
+    ([64, 64], 1, [0, 1], [1.0, 5.0]),          # the 2D Q1 -Laplace+5u workload
+    ([6, 5], 2, [0], [1.0]),                     # 2D Q2: rounding-level entries stay
+    ([4, 4, 3], 2, [0], [1.0]),                  # 3D Q2
+    ([5, 4, 3], 1, [0, 1], [2.5, 0.75]),
+    ([4, 3], 3, [0, 1], [1.0, 1.0]),
+    ([3, 3], 2, [2], [1.5]),                     # H1
+    ([9], 2, [1], [1.0]),
+]
+
+
+@pytest.mark.parametrize("cells,k,kinds,weights", ASM_CASES)
+def test_assembly_bit_exact(jf, oracle, cells, k, kinds, weights):
+    s, start, end = _space(jf, cells, k)
+    ints = [jf.TPCellIntegral(w, ["GradGrad", "ValueValue", "H1"][kd]) for kd, w in zip(kinds, weights)]
+    A = s.assemble_matrix(ints)
+    rp, cols, vals = A.to_csr()
+    ref = oracle.assemble(cells, k, start, end, kinds, weights)
+    assert np.array_equal(rp, ref.rowptr)
+    assert np.array_equal(cols, ref.cols)
+    assert np.array_equal(vals, ref.vals), np.max(np.abs(vals - ref.vals))
+
+
+def test_assembly_with_cell_coefficient(jf, oracle):
+    cells, k = [6, 5, 4], 2
+    s, start, end = _space(jf, cells, k)
+    rng = np.random.default_rng(7)
+    coeff = np.exp(rng.standard_normal(int(np.prod(cells))))
+    A = s.assemble_matrix([jf.TPCellIntegral("GradGrad")], cell_coeff=coeff)
+    rp, cols, vals = A.to_csr()
+    ref = oracle.assemble(cells, k, start, end, [0], [1.0], cell_coeff=coeff)
+    assert np.array_equal(rp, ref.rowptr) and np.array_equal(cols, ref.cols)
+    # the coefficient multiplies a class element matrix instead of entering the
+    # quadrature loop: agreement to a few ulps, tolerance 1e-14 relative to the row scale
+    scale = np.repeat(np.maximum.reduceat(np.abs(ref.vals), ref.rowptr[:-1]), np.diff(ref.rowptr))
+    assert np.max(np.abs(vals - ref.vals) / scale) < 1e-14
+
+
+def test_rhs_sines(jf, oracle):
+    cells, k = [6, 4, 5], 2
+    s, start, end = _space(jf, cells, k)
+    kw = np.array([[1, 2, 3], [4, 1, 2], [8, 8, 1]], dtype=np.int32)
+    amp = np.array([0.7, -1.3, 0.2])
+    s.initializeRhs()
+    s.writeCellIntegralsToRhs([jf.TPRightHandSideIntegral(jf.SinusoidSource(kw, amp))], s.getRhs())
+    got = s.getRhs().numpy()
+    ref = oracle.rhs_sines(cells, k, start, end, kw, amp)
+    # per-cell integrals are bit-identical; DOFs shared by several cells sum in
+    # another order (and the device sin may differ by an ulp): 1e-13 of max |rhs|
+    assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("cells,k", [([8, 6], 1), ([4, 3, 3], 2)])
+def test_dirichlet_bit_exact(jf, oracle, cells, k):
+    s, start, end = _space(jf, cells, k)
+    s.initializeRhs()
+    rng = np.random.default_rng(3)
+    b0 = rng.standard_normal(s.ndofs())
+    s.getRhs().set_values(b0)
+    s.evaluateCellIntegrals([jf.TPCellIntegral("GradGrad"), jf.TPCellIntegral(5.0, "ValueValue")], [])
+    g = lambda x: float(np.sin(3 * x[0]) + x[-1])
+    s.setBoundaryValues(g)
+    rp, cols, vals = s.getSystemMatrix().to_csr()
+    ref = oracle.assemble(cells, k, start, end, [0, 1], [1.0, 5.0])
+    nodes = oracle.boundary_nodes(cells, k)
+    xyz = oracle.node_coords(cells, k, start, end)
+    rhs = b0.copy()
+    oracle.dirichlet(ref, rhs, nodes, np.array([g(xyz[i]) for i in nodes]))
+    assert np.array_equal(cols, ref.cols)
+    assert np.array_equal(vals, ref.vals)
+    assert np.array_equal(s.getRhs().numpy(), rhs)
+
+
+def test_spmv(jf, oracle):
+    cells, k = [10, 9, 8], 2
+    s, start, end = _space(jf, cells, k)
+    A = s.assemble_matrix([jf.TPCellIntegral("GradGrad"), jf.TPCellIntegral(5.0, "ValueValue")])
+    ref = oracle.assemble(cells, k, start, end, [0, 1], [1.0, 5.0])
+    x = np.random.default_rng(11).standard_normal(ref.n)
+    y = A.mvMul(jf.DenseVector(x)).numpy()
+    yr = ref.matvec(x)
+    bound = np.abs(ref.to_scipy()) @ np.abs(x)
+    assert np.all(np.abs(y - yr) <= 1e-14 * bound + 1e-300)
+
+
+def test_spmv_fp32_and_from_csr(jf, oracle):
+    ref = oracle.assemble([30, 20], 1, [0, 0], [1, 1], [0, 1], [1.0, 5.0])
+    A = jf.SparseMatrix.from_csr(ref.rowptr, ref.cols, ref.vals)
+    x = np.random.default_rng(2).standard_normal(ref.n)
+    y = A.mvMul(jf.DenseVector(x)).numpy()
+    assert np.max(np.abs(y - ref.matvec(x))) <= 1e-13 * np.max(np.abs(y))
+
+
+# ---------------------------------------------------------------- solvers
+def _c1_system(jf, oracle, cells=(64, 64)):
+    """the 2D Q1 -Laplace u + 5u = f workload with seeded sinusoids, zero Dirichlet"""
+    from jfem_amd.workloads import sinusoid_terms
+    kw, amp = sinusoid_terms(1, 8, 2)
+    s, start, end = _space(jf, list(cells), 1, [0.0, 0.0], [1.0, 1.0])
+    s.initializeSystemMatrix()
+    s.initializeRhs()
+    s.evaluateCellIntegrals([jf.TPCellIntegral("GradGrad"), jf.TPCellIntegral(5.0, "ValueValue")],
+                            [jf.TPRightHandSideIntegral(jf.SinusoidSource(kw, amp))])
+    s.setBoundaryValues(None)
+    ref = oracle.assemble(list(cells), 1, start, end, [0, 1], [1.0, 5.0])
+    rhs = oracle.rhs_sines(list(cells), 1, start, end, kw, amp)
+    oracle.dirichlet(ref, rhs, oracle.boundary_nodes(list(cells), 1))
+    return s, ref, rhs
+
+
+def test_cg_c1_workload(jf, oracle):
+    s, ref, rhs = _c1_system(jf, oracle)
+    b = s.getRhs()
+    assert np.max(np.abs(b.numpy() - rhs)) <= 1e-13 * np.max(np.abs(rhs))
+    tol = 1e-10 * np.linalg.norm(rhs)
+    it = jf.IterativeSolver()
+    x = it.solveCG(s.getSystemMatrix(), b, tol).numpy()
+    x0 = np.zeros(ref.n)
+    x0[0] = 1.0
+    xr, itr, resr, _ = oracle.solve("cg", ref, rhs, x0, tol)
+    assert np.linalg.norm(x - xr) <= 1e-10 * np.linalg.norm(xr)
+    assert abs(it.iterations - itr) <= 3
+    assert np.linalg.norm(rhs - ref.matvec(x)) <= 1.01 * tol
+
+
+@pytest.mark.parametrize("method", ["bicgstab", "gmres"])
+def test_krylov_against_oracle(jf, oracle, method):
+    s, ref, rhs = _c1_system(jf, oracle, (24, 20))
+    tol = 1e-10 * np.linalg.norm(rhs)
+    it = jf.IterativeSolver()
+    if method == "bicgstab":
+        x = it.solveBiCGStab(s.getSystemMatrix(), s.getRhs(), tol).numpy()
+    else:
+        x = it.solveGMRES(s.getSystemMatrix(), s.getRhs(), tol).numpy()
+    x0 = np.zeros(ref.n)
+    x0[0] = 1
+    xr, itr, resr, _ = oracle.solve(method, ref, rhs, x0, tol)
+    assert np.linalg.norm(x - xr) <= 1e-8 * np.linalg.norm(xr)
+    assert np.linalg.norm(rhs - ref.matvec(x)) <= 1.01 * tol
+
+
+@pytest.mark.parametrize("mat", [[[0, 1, 2], [1, 0, 3], [2, 3, 0]], [[3, 1, 7], [2, 0, 3], [2, 3, 0]],
+                                 [[3, 1, 7], [2, 4, 3], [2, 3, 7]]])
+def test_small_known_answers(jf, mat):
+    """BiCGStabTest / GMResTest systems on the GPU path."""
+    import scipy.sparse as sp
+    S = sp.csr_matrix(np.array(mat, float))
+    A = jf.SparseMatrix.from_csr(S.indptr, S.indices, S.data)
+    b = jf.DenseVector([3.0, 4.0, 5.0])
+    exact = np.linalg.solve(mat, [3.0, 4.0, 5.0])
+    for solve in ("solveBiCGStab", "solveGMRES"):
+        x = getattr(jf.IterativeSolver(), solve)(A, b, 1e-12).numpy()
+        assert np.allclose(x, exact, atol=1e-9), solve
+
+
+# ---------------------------------------------------------------- multigrid
+def _mg_pair(jf, oracle, base, k, nlev, smoother, cycles, fmg, precision=0):
+    d = len(base)
+    start, end = [0.0] * d, [1.0] * d
+    spaces = []
+    for l in range(nlev):
+        s, _, _ = _space(jf, [c << l for c in base], k, start, end)
+        spaces.append(s)
+    ints = [jf.TPCellIntegral("GradGrad")]
+    G = jf.MGPreconditioner(spaces, ints, smoother, cycles=cycles, fmg=fmg, precision=precision)
+    kind = smoother.kind
+    lmax = G.lmax()
+    degree = getattr(smoother, "degree", 0)
+    ratio = getattr(smoother, "ratio", 1.0)
+    O = oracle.MG(base, k, nlev, start, end, [0], [1.0], smoother=kind, runs=smoother.runs,
+                  omega=getattr(smoother, "omega", 1.0), degree=degree, lmin=lmax / ratio if kind == 2 else None,
+                  lmax=lmax if kind == 2 else None, cycles=cycles, fmg=fmg)
+    return G, O, spaces
+
+
+@pytest.mark.parametrize("which", ["jacobi", "chebyshev", "richardson"])
+def test_mg_level_matrices_and_vcycle(jf, oracle, which):
+    sm = {"jacobi": jf.JacobiSmoother(2, 0.6), "chebyshev": jf.ChebyshevSmoother(4, 2),
+          "richardson": jf.RichardsonSmoother(0.1, 4)}[which]
+    G, O, spaces = _mg_pair(jf, oracle, [2, 3], 2, 3, sm, cycles=1, fmg=False)
+    for l in range(3):
+        rp, cols, vals = G.getSystem(l).to_csr()
+        ref = O.level_matrix(l)
+        assert np.array_equal(cols, ref.cols) and np.array_equal(vals, ref.vals)
+    n = O.level_n(2)
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal(n)
+    x = np.zeros(n)
+    got = G.vCycle(jf.DenseVector(x), jf.DenseVector(b)).numpy()
+    exp = O.vcycle(x, b)
+    # transfer weights are exact dyadic numbers here and evaluated Lagrange
+    # values in the oracle; summation orders differ: 1e-11 relative
+    assert np.linalg.norm(got - exp) <= 1e-11 * np.linalg.norm(exp)
+
+
+def test_mg_reference_apply_fmg(jf, oracle):
+    """MGPreconditionerInterface.mvMul: copy BC, full-V correction, 3 V-cycles."""
+    G, O, spaces = _mg_pair(jf, oracle, [2, 2], 1, 4, jf.RichardsonSmoother(0.1, 8), cycles=3, fmg=True)
+    n = O.level_n(3)
+    v = np.random.default_rng(9).standard_normal(n)
+    got = G.mvMul(jf.DenseVector(v)).numpy()
+    exp = O.apply(v)
+    assert np.linalg.norm(got - exp) <= 1e-11 * np.linalg.norm(exp)
+
+
+def test_laplace_mg_pgmres(jf, oracle):
+    """LaplaceMG.testMG: MG-preconditioned GMRES to 1e-8 in fewer than 10 iterations."""
+    for refinements in (2, 3, 4):
+        G, O, spaces = _mg_pair(jf, oracle, [4, 4], 1, refinements + 1, jf.RichardsonSmoother(0.1, 8),
+                                cycles=3, fmg=True)
+        cells = [4 << refinements] * 2
+        rhs = oracle.rhs_sines(cells, 1, [0, 0], [1, 1], [[1, 1]], [1.0])
+        rhs[oracle.boundary_nodes(cells, 1)] = 0
+        it = jf.IterativeSolver()
+        x = it.solvePGMRES(G.getFinestSystem(), G, jf.DenseVector(rhs), 1e-8).numpy()
+        assert it.iterations < 10
+        A = O.level_matrix(refinements)
+        assert np.linalg.norm(rhs - A.matvec(x)) <= 1e-8
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_mg_pcg_fcg_converge(jf, oracle, precision):
+    """Chebyshev V(2,2)-preconditioned CG (FP64) and flexible CG around an FP32
+    V-cycle reach 1e-8 and agree with an LU solve."""
+    import scipy.sparse.linalg as spla
+    from jfem_amd.workloads import sinusoid_terms
+    G, O, spaces = _mg_pair(jf, oracle, [3, 3, 3], 2, 3, jf.ChebyshevSmoother(4, 2), cycles=1, fmg=False,
+                            precision=precision)
+    fine = spaces[-1]
+    kw, amp = sinusoid_terms(2, 16, 3)
+    fine.initializeRhs()
+    fine.writeCellIntegralsToRhs([jf.TPRightHandSideIntegral(jf.SinusoidSource(kw, amp))], fine.getRhs())
+    rhs = fine.getRhs().numpy()
+    rhs[fine.boundary_nodes()] = 0
+    b = jf.DenseVector(rhs)
+    tol = 1e-8 * np.linalg.norm(rhs)
+    it = jf.IterativeSolver()
+    A = G.getFinestSystem()
+    x = (it.solveFCG(A, G, b, tol) if precision else it.solvePCG(A, G, b, tol, x0=jf.DenseVector(np.zeros(b.n)))).numpy()
+    ref = O.level_matrix(2).to_scipy()
+    exact = spla.spsolve(ref.tocsc(), rhs)
+    assert np.linalg.norm(rhs - ref @ x) <= 1.01 * tol
+    assert np.linalg.norm(x - exact) <= 1e-6 * np.linalg.norm(exact)
+    assert it.iterations < 30

@@ -1,0 +1,396 @@
+"""Pins the CPU oracle (oracle/jfem_oracle.c) against the reference's golden
+vectors, closed forms and brute force.  CPU only."""
+import itertools
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sp
+
+# ---------------------------------------------------------------- golden arrays
+# ContinuousIntegralTest.testTPCellIntegralDeg1G2 (Core/src/test/java/tensorproduct/
+# ContinuousIntegralTest.java:23-40): Q1 cell [0,1]x[2.3,3.1], Gauss2, 4 decimals.
+GG = [0.6833, -0.0583, -0.2833, -0.3417, -0.0583, 0.6833, -0.3417, -0.2833, -0.2833, -0.3417,
+      0.6833, -0.0583, -0.3417, -0.2833, -0.0583, 0.6833]
+GGW = [7.7812, -0.4688, -3.1612, -4.1513, -0.4688, 8.3437, -4.1513, -3.7237, -3.1612, -4.1513,
+       8.2612, -0.9488, -4.1513, -3.7237, -0.9488, 8.8238]
+VV = [0.0889, 0.0444, 0.0444, 0.0222, 0.0444, 0.0889, 0.0222, 0.0444,
+      0.0444, 0.0222, 0.0889, 0.0444, 0.0222, 0.0444, 0.0444, 0.0889]
+VVW = [0.9444, 0.5000, 0.5100, 0.2700, 0.5000, 1.0556, 0.2700, 0.5700, 0.5100, 0.2700, 1.0956,
+       0.5800, 0.2700, 0.5700, 0.5800, 1.2244]
+VG = [-0.3625, 0.1486, -0.0222, 0.2361, -0.3181, -0.1097, 0.0001, 0.4277, -0.2472, 0.0111, -0.1428,
+      0.3789, -0.2249, -0.2473, -0.0984, 0.5706]
+GV = [-0.3625, -0.3181, -0.2472, -0.2249, 0.1486, -0.1097, 0.0111, -0.2473, -0.0222, 0.0001,
+      -0.1428, -0.0984, 0.2361, 0.4277, 0.3789, 0.5706]
+
+
+def test_golden_cell_integrals(oracle):
+    O = oracle
+    s, e = [0, 2.3], [1, 3.1]
+
+    def loc(i):
+        return [i % 2, i // 2]
+
+    def w(x):
+        return x[1] * (x[0] + 4)
+
+    def vw(x):
+        return [2 - x[0] + x[1] * 0.1, x[0] * x[1]]
+
+    k = 0
+    for i in range(4):
+        for j in range(4):
+            assert abs(O.cell_integral(s, e, 1, O.GRAD_GRAD, 2, loc(i), loc(j)) - GG[k]) <= 6e-5
+            assert abs(O.cell_integral(s, e, 1, O.GRAD_GRAD, 2, loc(i), loc(j), wfun=w) - GGW[k]) <= 6e-5
+            assert abs(O.cell_integral(s, e, 1, O.VALUE_VALUE, 2, loc(i), loc(j)) - VV[k]) <= 6e-5
+            assert abs(O.cell_integral(s, e, 1, O.VALUE_VALUE, 2, loc(i), loc(j), wfun=w) - VVW[k]) <= 6e-5
+            assert abs(O.cell_integral(s, e, 1, O.VALUE_GRAD, 2, loc(i), loc(j), vfun=vw) - VG[k]) <= 6e-5
+            assert abs(O.cell_integral(s, e, 1, O.GRAD_VALUE, 2, loc(i), loc(j), vfun=vw) - GV[k]) <= 6e-5
+            k += 1
+
+
+def test_golden_detects_transposed_operands(oracle):
+    """GRAD_VALUE is not symmetric: swapping u and v must break the golden array."""
+    O = oracle
+    vw = lambda x: [2 - x[0] + x[1] * 0.1, x[0] * x[1]]
+    dev = max(abs(O.cell_integral([0, 2.3], [1, 3.1], 1, O.GRAD_VALUE, 2, [j % 2, j // 2], [i % 2, i // 2],
+                                  vfun=vw) - GV[i * 4 + j]) for i in range(4) for j in range(4))
+    assert dev > 1e-2
+
+
+# ---------------------------------------------------------------- quadrature / basis
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5])
+def test_gauss_exactness(oracle, n):
+    p, w = oracle.quad(n)
+    assert len(p) == n
+    for deg in range(2 * n):
+        assert abs(np.dot(w, p ** deg) - 1.0 / (deg + 1)) < 1e-14
+    assert abs(np.dot(w, p ** (2 * n)) - 1.0 / (2 * n + 1)) > 1e-6
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 6])
+def test_gauss_lobatto(oracle, n):
+    p, w = oracle.quad(10 + n)
+    assert p[0] == 0.0 and p[-1] == 1.0
+    # the reference lists GaussLobatto2's weights as {1/2, 1/2} on [-1,1]
+    # (QuadratureRule1D.java:42) and halves them again in the constructor, so
+    # that rule integrates to 1/2 of the true value; the oracle keeps the quirk.
+    scale = 0.5 if n == 2 else 1.0
+    for deg in range(2 * n - 2):
+        assert abs(np.dot(w, p ** deg) - scale / (deg + 1)) < 1e-14
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+def test_lagrange_basis(oracle, k):
+    O = oracle
+    for t in np.linspace(0, 1, 13):
+        assert abs(sum(O.lagrange_value(t, l, k) for l in range(k + 1)) - 1) < 1e-13
+        assert abs(sum(O.lagrange_deriv(t, l, k) for l in range(k + 1))) < 1e-11
+    for l in range(k + 1):
+        for m in range(k + 1):
+            assert abs(O.lagrange_value(m / k, l, k) - (1.0 if l == m else 0.0)) < 1e-14
+        # derivative against a central difference of the value
+        for t in (0.1, 0.37, 0.8):
+            h = 1e-6
+            fd = (O.lagrange_value(t + h, l, k) - O.lagrange_value(t - h, l, k)) / (2 * h)
+            assert abs(fd - O.lagrange_deriv(t, l, k)) < 1e-6
+
+
+# ---------------------------------------------------------------- element matrices
+def k1d(k, h):
+    if k == 1:
+        return np.array([[1, -1], [-1, 1]]) / h, np.array([[2, 1], [1, 2]]) * h / 6
+    return (np.array([[7, -8, 1], [-8, 16, -8], [1, -8, 7]]) / (3 * h),
+            np.array([[4, 2, -1], [2, 16, 2], [-1, 2, 4]]) * h / 30)
+
+
+def tensor_element(k, hs, kinds_weights):
+    """sum over integrals of the Kronecker form; local index with axis 0 fastest."""
+    d = len(hs)
+    KM = [k1d(k, h) for h in hs]
+    out = 0
+    for kind, w in kinds_weights:
+        if kind == 0:      # GRAD_GRAD = sum_a (K on axis a, M elsewhere)
+            for a in range(d):
+                mats = [KM[b][0] if b == a else KM[b][1] for b in range(d)]
+                m = mats[d - 1]
+                for b in range(d - 2, -1, -1):
+                    m = np.kron(m, mats[b])
+                out = out + w * m
+        else:
+            mats = [KM[b][1] for b in range(d)]
+            m = mats[d - 1]
+            for b in range(d - 2, -1, -1):
+                m = np.kron(m, mats[b])
+            out = out + w * m
+    return out
+
+
+@pytest.mark.parametrize("d,k", [(2, 1), (2, 2), (3, 1), (3, 2)])
+def test_element_matrix_closed_form(oracle, d, k):
+    cells = [3, 2, 2][:d]
+    start = [0.0, -1.0, 0.5][:d]
+    end = [1.5, 1.0, 2.0][:d]
+    hs = [(e - s) / c for s, e, c in zip(start, end, cells)]
+    E = oracle.element_matrix(cells, k, start, end, 1, [0, 1], [1.0, 5.0])
+    ref = tensor_element(k, hs, [(0, 1.0), (1, 5.0)])
+    assert np.max(np.abs(E - ref)) < 1e-13 * np.max(np.abs(ref))
+
+
+# ---------------------------------------------------------------- DOF numbering
+def closed_form_dof(cells, k, p):
+    """prefix(owner cell) + rank, written from the first-appearance rule."""
+    d = len(cells)
+    T = [k * n + 1 for n in cells]
+    pre, mprod, rank = 0, 1, 0
+    for a in range(d):
+        c = 0 if p[a] == 0 else (p[a] - 1) // k
+        l = p[a] - c * k
+        tail = int(np.prod(T[a + 1:])) if a + 1 < d else 1
+        S = c * k + 1 if c > 0 else 0
+        pre += mprod * S * tail
+        rank += (l - (1 if c > 0 else 0)) * mprod
+        mprod *= k if c > 0 else k + 1
+    return pre + rank
+
+
+@pytest.mark.parametrize("cells,k", [([1, 1], 1), ([2, 3], 1), ([3, 2], 2), ([3, 2, 2], 2), ([2, 2, 3], 1),
+                                     ([4], 3), ([2, 3], 3), ([2, 1, 2], 4)])
+def test_dof_numbering_closed_form(oracle, cells, k):
+    n, table = oracle.dof_table(cells, k)
+    assert n == int(np.prod([k * c + 1 for c in cells]))
+    d = len(cells)
+    for cell in range(table.shape[0]):
+        cc = np.unravel_index(cell, cells)          # last axis fastest
+        for li in range(table.shape[1]):
+            l = [(li // (k + 1) ** a) % (k + 1) for a in range(d)]
+            p = [cc[a] * k + l[a] for a in range(d)]
+            assert table[cell, li] == closed_form_dof(cells, k, p)
+
+
+def test_dof_numbering_first_cell_is_lexicographic(oracle):
+    n, t = oracle.dof_table([3, 2], 2)
+    assert list(t[0]) == list(range(9))
+    assert list(t[1][:3]) == [6, 7, 8]      # shared face of cells (0,0) and (0,1)
+    assert t[2][0] == 2                      # cell (1,0) starts on the shared face
+
+
+# ---------------------------------------------------------------- assembly
+def structural_nnz(cells, k):
+    per_axis = []
+    for n in cells:
+        cnt = 0
+        for p in range(k * n + 1):
+            if p % k == 0:
+                lo, hi = max(p // k - 1, 0), min(p // k, n - 1)
+            else:
+                lo = hi = p // k
+            cnt += k * (hi - lo + 1) + 1
+        per_axis.append(cnt)
+    return int(np.prod(per_axis))
+
+
+@pytest.mark.parametrize("cells,k", [([8, 8], 1), ([5, 4], 2), ([3, 3, 3], 1), ([3, 2, 3], 2)])
+def test_assembly_invariants(oracle, cells, k):
+    O = oracle
+    d = len(cells)
+    start, end = [0.0] * d, [1.0, 2.0, 0.5][:d]
+    A = O.assemble(cells, k, start, end, [O.GRAD_GRAD], [1.0])
+    M = O.assemble(cells, k, start, end, [O.VALUE_VALUE], [1.0])
+    S = A.to_scipy()
+    assert A.nnz == structural_nnz(cells, k)
+    assert abs(S - S.T).max() == 0.0                              # bit-symmetric
+    assert np.max(np.abs(S.sum(axis=1))) < 1e-12 * abs(S).max()   # constants are in the kernel
+    vol = np.prod(np.array(end) - np.array(start))
+    assert abs(M.to_scipy().sum() - vol) < 1e-12 * vol            # partition of unity integrates to |Omega|
+    # columns sorted within rows
+    for r in range(A.n):
+        c = A.cols[A.rowptr[r]:A.rowptr[r + 1]]
+        assert np.all(np.diff(c) > 0)
+
+
+def test_assembly_2d_q2_keeps_rounding_level_entries(oracle):
+    """2D Q2 GRAD_GRAD has element entries that vanish mathematically
+    (K02*M11 + M02*K11 = 16/90 - 16/90).  Evaluated by the reference's
+    quadrature they come out at rounding level, not exactly 0.0, so
+    addToMatrix keeps them and the pattern stays structural."""
+    O = oracle
+    E = O.element_matrix([4, 4], 2, [0, 0], [1, 1], 0, [O.GRAD_GRAD], [1.0])
+    tiny = np.abs(E)[np.abs(E) < 1e-10]
+    assert len(tiny) == 4 and np.all(tiny > 0)
+    A = O.assemble([4, 4], 2, [0, 0], [1, 1], [O.GRAD_GRAD], [1.0])
+    assert A.nnz == structural_nnz([4, 4], 2)
+
+
+def test_assembly_matches_global_quadrature(oracle):
+    """u^T A u equals the exact energy of a smooth function's interpolant."""
+    O = oracle
+    cells, k = [6, 5], 2
+    A = O.assemble(cells, k, [0, 0], [1, 1], [O.GRAD_GRAD], [1.0])
+    xyz = O.node_coords(cells, k, [0, 0], [1, 1])
+    u = xyz[:, 0] ** 2 + xyz[:, 0] * xyz[:, 1]     # in Q2, so interpolation is exact
+    energy = u @ A.matvec(u)
+    # int_0^1 int_0^1 |grad u|^2 = int (2x+y)^2 + x^2 = 4/3 + 1 + 1/3 + 1/3 = 3
+    assert abs(energy - 3.0) < 1e-12
+
+
+# ---------------------------------------------------------------- Dirichlet
+def test_dirichlet_matches_dense_elimination(oracle):
+    O = oracle
+    cells, k = [4, 3], 2
+    A = O.assemble(cells, k, [0, 0], [1, 1], [O.GRAD_GRAD, O.VALUE_VALUE], [1.0, 5.0])
+    dense = A.to_scipy().toarray()
+    rng = np.random.default_rng(1)
+    rhs = rng.standard_normal(A.n)
+    nodes = O.boundary_nodes(cells, k)
+    g = rng.standard_normal(len(nodes))
+    # symmetric elimination written densely
+    free = np.setdiff1d(np.arange(A.n), nodes)
+    exp_rhs = rhs.copy()
+    exp_rhs[free] -= dense[np.ix_(free, nodes)] @ g
+    exp_rhs[nodes] = g
+    exp = dense.copy()
+    exp[nodes, :] = 0
+    exp[:, nodes] = 0
+    exp[nodes, nodes] = 1
+    r2 = rhs.copy()
+    O.dirichlet(A, r2, nodes, g)
+    assert np.allclose(A.to_scipy().toarray(), exp, rtol=0, atol=0)
+    assert np.max(np.abs(r2 - exp_rhs)) < 1e-12 * np.max(np.abs(exp_rhs))
+    assert A.nnz == int(np.count_nonzero(dense) + 0)   # explicit zeros stay in the pattern
+
+
+def test_boundary_nodes(oracle):
+    O = oracle
+    cells, k = [3, 4], 2
+    xyz = O.node_coords(cells, k, [0, 0], [1, 1])
+    nodes = set(O.boundary_nodes(cells, k).tolist())
+    on = {i for i in range(len(xyz)) if np.any(np.isclose(xyz[i], 0)) or np.any(np.isclose(xyz[i], 1))}
+    assert nodes == on
+
+
+# ---------------------------------------------------------------- solvers
+def csr_of(dense):
+    S = sp.csr_matrix(np.asarray(dense, float))
+    from oracle.oracle import CSR
+    return CSR(S.shape[0], S.indptr.astype(np.int64), S.indices.astype(np.int32), S.data.astype(float))
+
+
+SYMM = [[0, 1, 2], [1, 0, 3], [2, 3, 0]]
+NONSYMM = [[3, 1, 7], [2, 0, 3], [2, 3, 0]]
+GOOD = [[3, 1, 7], [2, 4, 3], [2, 3, 7]]
+
+
+@pytest.mark.parametrize("mat", [SYMM, NONSYMM, GOOD])
+def test_bicgstab_known_answers(oracle, mat):
+    """BiCGStabTest.java:19-40: A sol = b and sol = A^-1 b."""
+    A = csr_of(mat)
+    b = np.array([3.0, 4.0, 5.0])
+    x0 = np.array([1.0, 0, 0])
+    x, it, res, fb = oracle.solve("bicgstab", A, b, x0, 1e-12)
+    assert np.allclose(np.array(mat) @ x, b, atol=1e-9)
+    assert np.allclose(x, np.linalg.solve(mat, b), atol=1e-9)
+
+
+@pytest.mark.parametrize("mat", [SYMM, NONSYMM, GOOD])
+def test_gmres_known_answers(oracle, mat):
+    A = csr_of(mat)
+    b = np.array([3.0, 4.0, 5.0])
+    x, it, res, fb = oracle.solve("gmres", A, b, np.array([1.0, 0, 0]), 1e-12)
+    assert np.allclose(x, np.linalg.solve(mat, b), atol=1e-9)
+
+
+@pytest.mark.parametrize("n", [5, 15])
+def test_identity_solves(oracle, n):
+    A = csr_of(np.eye(n))
+    b = np.arange(1.0, n + 1)
+    for m in ("cg", "bicgstab", "gmres"):
+        x, *_ = oracle.solve(m, A, b, np.eye(n)[0], 1e-12)
+        assert np.allclose(x, b, atol=1e-12)
+
+
+def test_cg_on_fe_system_matches_lu(oracle):
+    O = oracle
+    cells, k = [12, 10], 1
+    A = O.assemble(cells, k, [0, 0], [1, 1], [O.GRAD_GRAD, O.VALUE_VALUE], [1.0, 5.0])
+    rhs = O.rhs_sines(cells, k, [0, 0], [1, 1], [[1, 2], [3, 1]], [1.0, -0.5])
+    O.dirichlet(A, rhs, O.boundary_nodes(cells, k))
+    x0 = np.zeros(A.n)
+    x0[0] = 1
+    x, it, res, _ = O.solve("cg", A, rhs, x0, 1e-12 * np.linalg.norm(rhs))
+    exact = sla.lu_solve(sla.lu_factor(A.to_scipy().toarray()), rhs)
+    assert np.linalg.norm(x - exact) <= 1e-10 * np.linalg.norm(exact)
+    assert res <= 1e-12 * np.linalg.norm(rhs) * 1.0001
+
+
+def test_lu_solve(oracle):
+    rng = np.random.default_rng(3)
+    M = rng.standard_normal((40, 40))
+    b = rng.standard_normal(40)
+    assert np.allclose(oracle.lu_solve(M, b), np.linalg.solve(M, b), atol=1e-10)
+
+
+# ---------------------------------------------------------------- multigrid
+def test_prolongation_reproduces_linear_functions(oracle):
+    """MultiGridSpaceTest.testProlongateInterpolate: P v of a linear v equals v."""
+    O = oracle
+    for k, cc in ((1, [4, 4]), (2, [3, 2]), (2, [2, 2, 2])):
+        d = len(cc)
+        start, end = [100.7, 0.2, 0.0][:d], [1000.0, 0.21, 1.0][:d]
+        P = O.prolongation(cc, k, start, end)
+        xc = O.node_coords(cc, k, start, end)
+        xf = O.node_coords([2 * c for c in cc], k, start, end)
+        coef = np.array([0.3, 2.0, -1.0][:d])
+        vc = xc @ coef + 1.5
+        vf = xf @ coef + 1.5
+        assert np.max(np.abs(P.matvec(vc) - vf)) <= 1e-8 * np.max(np.abs(vf))
+        quad = lambda x: x[:, 0] ** 2 if k == 2 else x[:, 0]
+        assert np.max(np.abs(P.matvec(quad(xc)) - quad(xf))) <= 1e-8 * np.max(np.abs(quad(xf)))
+
+
+def test_galerkin_identity(oracle):
+    """MultiGridSpaceTest: P^T A_fine P equals A_coarse for nested spaces."""
+    O = oracle
+    start, end = [100.7, 0.2], [1000.0, 0.21]
+    for k in (1, 2):
+        Ac = O.assemble([4, 4], k, start, end, [O.GRAD_GRAD], [1.0]).to_scipy()
+        Af = O.assemble([8, 8], k, start, end, [O.GRAD_GRAD], [1.0]).to_scipy()
+        P = O.prolongation([4, 4], k, start, end).to_scipy()
+        G = (P.T @ Af @ P).toarray()
+        assert np.max(np.abs(G - Ac.toarray())) <= 1e-9 * np.max(np.abs(Ac.toarray()))
+
+
+def test_mg_preconditioned_gmres_laplace(oracle):
+    """LaplaceMG.testMG (Core/src/test/java/examples/LaplaceMG.java:23-101):
+    Q1 on [0,1]^2, Richardson(0.1, 8) smoothing, MG-preconditioned GMRES to
+    1e-8 in fewer than 10 iterations."""
+    O = oracle
+    for refinements in (2, 3):
+        nlev = refinements + 1
+        G = O.MG([4, 4], 1, nlev, [0, 0], [1, 1], [O.GRAD_GRAD], [1.0], smoother=O.SM_RICHARDSON, runs=8,
+                 omega=0.1, cycles=3, fmg=True)
+        cells = [4 << refinements] * 2
+        A = G.level_matrix(nlev - 1)
+        rhs = O.rhs_sines(cells, 1, [0, 0], [1, 1], [[1, 1]], [1.0])
+        rhs[O.boundary_nodes(cells, 1)] = 0
+        x, it, res, _ = O.solve("gmres", A, rhs, np.zeros(A.n), 1e-8, precond=G)
+        assert it < 10
+        assert res <= 1e-8
+
+
+@pytest.mark.parametrize("sm", ["jacobi", "chebyshev"])
+def test_mg_vcycle_contracts(oracle, sm):
+    O = oracle
+    kind = O.SM_JACOBI if sm == "jacobi" else O.SM_CHEBYSHEV
+    nlev = 4
+    G = O.MG([2, 2], 2, nlev, [0, 0], [1, 1], [O.GRAD_GRAD], [1.0], smoother=kind, runs=2, omega=0.6,
+             degree=4, lmin=[0.05] * nlev, lmax=[2.0] * nlev, cycles=1, fmg=False)
+    A = G.level_matrix(nlev - 1).to_scipy()
+    rng = np.random.default_rng(0)
+    b = rng.standard_normal(A.shape[0])
+    b[O.boundary_nodes([16, 16], 2)] = 0
+    x = np.zeros_like(b)
+    r0 = np.linalg.norm(b)
+    for _ in range(5):
+        x = G.vcycle(x, b)
+    assert np.linalg.norm(b - A @ x) < 1e-3 * r0
