@@ -714,6 +714,7 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     (void)hipFree(E);
     (void)hipFree(coeff);
     A->lanes = choose_lanes(n > 0 ? (double)nnz / n : 1.0);
+    mat_build_blocks(A);
     return A;
 }
 

@@ -171,6 +171,9 @@ struct jfem_mat {
     float* vals32 = nullptr; // optional FP32 copy of the values
     double* diag = nullptr;  // optional: diagonal (device) for Jacobi-type smoothers
     int lanes = 8;           // lanes per row for the SpMV, from the mean row length
+    int* rblk = nullptr;     // device: first row of each CSR-stream block (nrblk+1)
+    long nrblk = 0;
+    double* part = nullptr;  // device: one partial sum per stream block
 };
 
 namespace jfem {
@@ -203,10 +206,15 @@ void spmv(jfem_mat* A, const double* x, double* y);
 void spmv32(jfem_mat* A, const float* x, float* y);
 // SpMV y = A x and partial dot(x, y) into reduction slot 0 (not finalised)
 int spmv_dot(jfem_mat* A, const double* x, double* y);   // returns the number of partials
+// y = A x with dot(x, y) reduced into the device scalar dst
+void spmv_dot_dev(jfem_mat* A, const double* x, double* y, double* dst);
 int choose_lanes(double mean_row);
 // reduce partials into a device scalar (no host sync)
 void reduce_partials_dev(jfem_ctx* c, int slots, int blocks, double* dst_dev);
 void mat_make_f32(jfem_mat* A);
+void mat_build_blocks(jfem_mat* A);
+// sum of n partials (device) into dst (device), deterministic
+void reduce_array_dev(jfem_ctx* c, const double* partials, long n, double* dst);
 void mat_diag(jfem_mat* A, double* out);
 void mat_free(jfem_mat* A);
 void apply_dirichlet_mask(jfem_mat* A, double* rhs, const unsigned char* fixed_dev, const double* g_dev);

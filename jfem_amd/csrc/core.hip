@@ -101,6 +101,28 @@ void reduce_partials(jfem_ctx* c, int slots, int blocks, bool to_host, double* o
     }
 }
 
+// one block of 1024 threads sums an arbitrary-length partial array in a fixed order
+__global__ __launch_bounds__(1024) void k_reduce_array(const double* __restrict__ a, long n, double* __restrict__ out)
+{
+    __shared__ double sh[16];
+    double acc = 0;
+    for (long i = threadIdx.x; i < n; i += 1024) acc += a[i];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0;
+        for (int i = 0; i < 16; i++) r += sh[i];
+        *out = r;
+    }
+}
+
+void reduce_array_dev(jfem_ctx* c, const double* partials, long n, double* dst)
+{
+    k_reduce_array<<<1, 1024, 0, c->stream>>>(partials, n, dst);
+    JFEM_HIP(hipGetLastError());
+}
+
 void reduce_partials_dev(jfem_ctx* c, int slots, int blocks, double* dst_dev)
 {
     k_reduce_partials<<<1, BS, 0, c->stream>>>(slots, blocks, c->red_partials, dst_dev);

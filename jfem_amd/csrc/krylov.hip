@@ -152,8 +152,7 @@ static void solve_cg(jfem_mat* A, jfem_mg* M, const double* b, double* x, double
     int it = 0;
     const long cap = std::min(n, maxit);
     for (long iter = 0; iter < cap && std::sqrt(rr) > tol; iter++) {
-        const int blocks = spmv_dot(A, p.p, q.p);           // q = A p, partial p.q
-        reduce_partials_dev(c, 1, blocks, sc + 1);           // sc[1] = p.Ap
+        spmv_dot_dev(A, p.p, q.p, sc + 1);                   // q = A p, sc[1] = p.Ap
         k_cg_update<<<kgrid(n), KB, 0, c->stream>>>(n, sc, 0, 1, p.p, q.p, x, r.p, c->red_partials);
         reduce_partials_dev(c, 1, kgrid(n), sc + 2);         // sc[2] = r.r (new)
         if (M) {
@@ -208,8 +207,7 @@ static void solve_fcg(jfem_mat* A, jfem_mg* M, const double* b, double* x, doubl
             const double* ys[1] = {r.p};
             dots_dev(c, n, 1, xs, ys, sc + 0);
         }
-        const int blocks = spmv_dot(A, p.p, q.p);
-        reduce_partials_dev(c, 1, blocks, sc + 1);
+        spmv_dot_dev(A, p.p, q.p, sc + 1);
         k_cg_update<<<kgrid(n), KB, 0, c->stream>>>(n, sc, 0, 1, p.p, q.p, x, r.p, c->red_partials);
         reduce_partials_dev(c, 1, kgrid(n), sc + 2);
         mg_apply(M, r.p, z.p);

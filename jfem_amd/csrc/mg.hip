@@ -14,6 +14,7 @@
 // matrix.  The coarsest level is solved with its dense inverse (a GEMV).
 // Every level operation is templated on the working precision (FP64 or FP32).
 #include "common.hpp"
+#include "stream.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -324,10 +325,7 @@ template <> const float* dinv_of<float>(const MgLevel& L) { return L.dinv32; }
 template <typename T>
 static void level_residual(jfem_mg* G, MgLevel& L, const T* b, const T* x, T* r)
 {
-    const int g = mg_grid(L, L.A->lanes);
-    hipStream_t st = G->ctx->stream;
-    DISPATCH_L(L.A->lanes, k_residual<LL, T><<<g, MB, 0, st>>>(L.A->rows, L.A->rowptr, L.A->colidx, vals_of<T>(L), b, x, r));
-    JFEM_HIP(hipGetLastError());
+    launch_stream<T, T>(L.A, vals_of<T>(L), x, EpiResidual<T>{b, r}, nullptr);
 }
 
 template <typename T>
@@ -353,10 +351,13 @@ static void smooth(jfem_mg* G, MgLevel& L, const T* b, T*& x, T*& xalt, bool x_z
     const int g = mg_grid(L, L.A->lanes);
     if (o.smoother == JFEM_SM_JACOBI) {
         for (int it = 0; it < o.runs; it++) {
-            DISPATCH_L(L.A->lanes, k_jacobi<LL, T><<<g, MB, 0, st>>>(n, L.A->rowptr, L.A->colidx, vals_of<T>(L),
-                                                                   dinv_of<T>(L), b, x, xalt, (T)o.omega,
-                                                                   (x_zero && it == 0) ? 1 : 0));
-            JFEM_HIP(hipGetLastError());
+            if (x_zero && it == 0) {
+                DISPATCH_L(L.A->lanes, k_jacobi<LL, T><<<g, MB, 0, st>>>(n, L.A->rowptr, L.A->colidx, vals_of<T>(L),
+                                                                       dinv_of<T>(L), b, x, xalt, (T)o.omega, 1));
+                JFEM_HIP(hipGetLastError());
+            } else {
+                launch_stream<T, T>(L.A, vals_of<T>(L), x, EpiJacobi<T>{b, x, dinv_of<T>(L), xalt, (T)o.omega}, nullptr);
+            }
             std::swap(x, xalt);
         }
     } else if (o.smoother == JFEM_SM_RICHARDSON) {
@@ -382,10 +383,15 @@ static void smooth(jfem_mg* G, MgLevel& L, const T* b, T*& x, T*& xalt, bool x_z
                     c2 = 2.0 * rho / delta;
                     rho_old = rho;
                 }
-                DISPATCH_L(L.A->lanes, k_cheby<LL, T><<<g, MB, 0, st>>>(n, L.A->rowptr, L.A->colidx, vals_of<T>(L),
-                                                                      dinv_of<T>(L), b, x, xalt, dv, kd == 0 ? 1 : 0,
-                                                                      (T)theta, (T)c1, (T)c2, zero ? 1 : 0));
-                JFEM_HIP(hipGetLastError());
+                if (zero) {
+                    DISPATCH_L(L.A->lanes, k_cheby<LL, T><<<g, MB, 0, st>>>(n, L.A->rowptr, L.A->colidx, vals_of<T>(L),
+                                                                          dinv_of<T>(L), b, x, xalt, dv, kd == 0 ? 1 : 0,
+                                                                          (T)theta, (T)c1, (T)c2, 1));
+                    JFEM_HIP(hipGetLastError());
+                } else {
+                    EpiCheby<T> e{b, x, dinv_of<T>(L), xalt, dv, (T)theta, (T)c1, (T)c2, kd == 0 ? 1 : 0};
+                    launch_stream<T, T>(L.A, vals_of<T>(L), x, e, nullptr);
+                }
                 std::swap(x, xalt);
                 zero = false;
             }

@@ -9,6 +9,7 @@
 // shuffles.  The kernel is HBM-bound: 12 bytes per entry (8-byte value,
 // 4-byte column) plus the row pointer, x (once, through L2/MALL) and y.
 #include "common.hpp"
+#include "stream.hpp"
 
 #include <cstring>
 
@@ -84,12 +85,25 @@ static void launch_spmv(jfem_mat* A, const VT* vals, const XT* x, XT* y)
     JFEM_HIP(hipGetLastError());
 }
 
-void spmv(jfem_mat* A, const double* x, double* y) { launch_spmv<double, double>(A, A->vals, x, y); }
+void spmv(jfem_mat* A, const double* x, double* y)
+{
+    if (A->rblk) launch_stream<double, double>(A, A->vals, x, EpiStore<double>{y}, nullptr);
+    else launch_spmv<double, double>(A, A->vals, x, y);
+}
 
 void spmv32(jfem_mat* A, const float* x, float* y)
 {
     JFEM_CHECK(A->vals32, JFEM_ERR_STATE, "matrix has no FP32 values");
-    launch_spmv<float, float>(A, A->vals32, x, y);
+    if (A->rblk) launch_stream<float, float>(A, A->vals32, x, EpiStore<float>{y}, nullptr);
+    else launch_spmv<float, float>(A, A->vals32, x, y);
+}
+
+// y = A x; dot(x, y) reduced into dst (device scalar)
+void spmv_dot_dev(jfem_mat* A, const double* x, double* y, double* dst)
+{
+    JFEM_CHECK(A->rblk, JFEM_ERR_STATE, "matrix has no stream blocks");
+    launch_stream<double, double>(A, A->vals, x, EpiStoreDot{y, x}, A->part);
+    reduce_array_dev(A->ctx, A->part, A->nrblk, dst);
 }
 
 int spmv_dot(jfem_mat* A, const double* x, double* y)
@@ -108,6 +122,33 @@ int spmv_dot(jfem_mat* A, const double* x, double* y)
     }
     JFEM_HIP(hipGetLastError());
     return grid;
+}
+
+// greedy split of the rows into CSR-stream blocks of at most ST_CAP entries
+// and at most ST_BS rows (a longer row gets a block of its own)
+void mat_build_blocks(jfem_mat* A)
+{
+    (void)hipFree(A->rblk);
+    (void)hipFree(A->part);
+    A->rblk = nullptr;
+    A->part = nullptr;
+    const long n = A->rows;
+    std::vector<int> rp(n + 1);
+    JFEM_HIP(hipMemcpy(rp.data(), A->rowptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+    std::vector<int> blk;
+    blk.reserve(n / 16 + 2);
+    long r = 0;
+    while (r < n) {
+        blk.push_back((int)r);
+        long e = r + 1;
+        while (e < n && e - r < ST_BS && (long)rp[e + 1] - rp[r] <= ST_CAP) e++;
+        r = e;
+    }
+    blk.push_back((int)n);
+    A->nrblk = (long)blk.size() - 1;
+    JFEM_HIP(hipMalloc(&A->rblk, sizeof(int) * blk.size()));
+    JFEM_HIP(hipMemcpy(A->rblk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice));
+    JFEM_HIP(hipMalloc(&A->part, sizeof(double) * (A->nrblk > 0 ? A->nrblk : 1)));
 }
 
 // ---- Dirichlet elimination, one thread group per row ----
@@ -203,6 +244,8 @@ void mat_free(jfem_mat* A)
     (void)hipFree(A->vals);
     (void)hipFree(A->vals32);
     (void)hipFree(A->diag);
+    (void)hipFree(A->rblk);
+    (void)hipFree(A->part);
     delete A;
 }
 
@@ -239,6 +282,7 @@ int jfem_mat_from_csr(jfem_ctx* ctx, long rows, long cols, const long* rowptr, c
         JFEM_HIP(hipMemcpy(A->vals, vals, sizeof(double) * nnz, hipMemcpyHostToDevice));
     }
     A->lanes = choose_lanes(rows > 0 ? (double)nnz / rows : 1.0);
+    mat_build_blocks(A);
     *out = A;
     API_END
 }
