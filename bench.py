@@ -87,6 +87,28 @@ def cpu_baseline(seconds_budget=30.0):
     return dict(solve_s=solve, iterations=it, n=A.n, setup_s=setup, spmv_gbs=bytes_spmv / t_spmv / 1e9)
 
 
+def copy_calibration(jf, n=1 << 27, reps=20):
+    import ctypes as C
+    from jfem_amd import _native as N
+    ms = C.c_double()
+    N.check(N.lib().jfem_copy_bench(jf.context().h, n, reps, C.byref(ms)))
+    return 16 * n / (ms.value * 1e-3) / 1e9, ms.value
+
+
+def spmv_only(jf, cells):
+    s = jf.ContinuousTPFESpace([0.0] * 3, [1.0] * 3, [cells] * 3)
+    s.assembleCells()
+    s.assembleFunctions(2)
+    A = s.assemble_matrix([jf.TPCellIntegral("GradGrad")])
+    n, _, nnz = A.info()
+    x = jf.DenseVector(np.random.default_rng(0).standard_normal(n))
+    y = jf.DenseVector(n)
+    ms = A.spmv_time_ms(x, y, reps=20)
+    gbs, cms = copy_calibration(jf)
+    print(json.dumps({"spmv_fp64_ms": ms, "spmv_fp64_gbs": (12 * nnz + 20 * n + 4) / (ms * 1e-3) / 1e9,
+                      "copy_gbs": gbs, "copy_ms": cms, "n": n, "nnz": nnz}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +117,8 @@ def main():
     ap.add_argument("--cells", type=int, default=96, help="cells per axis of the finest grid (3 * 2^m)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="assemble the finest matrix and time only the SpMV and the copy calibration (for PMC runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the solve path is single-GPU in this build "
@@ -102,6 +126,10 @@ def main():
         sys.exit(2)
 
     import jfem_amd as jf
+
+    if args.spmv_only:
+        spmv_only(jf, args.cells)
+        return
 
     nlev = 1
     c = args.cells
@@ -165,6 +193,8 @@ def main():
     spmv32_ms = A.spmv_time_ms(xs, ys, reps=50, fp32=True)
     spmv32_gbs = (8 * nnz + 4 * (n + 1) + 8 * n) / (spmv32_ms * 1e-3) / 1e9
     log(f"SpMV fp64 {spmv_ms:.3f} ms {spmv_gbs:.0f} GB/s; fp32 {spmv32_ms:.3f} ms {spmv32_gbs:.0f} GB/s")
+    copy_gbs, _ = copy_calibration(jf)
+    log(f"copy calibration {copy_gbs:.0f} GB/s")
 
     extra = {}
     if not args.no_fp32:
@@ -204,8 +234,8 @@ def main():
         "spmv_fp64_ms": spmv_ms, "spmv_fp64_gbs": spmv_gbs, "spmv_fp32_ms": spmv32_ms, "spmv_fp32_gbs": spmv32_gbs,
         "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_spmv<64,double,double> (finest-level CSR SpMV)",
-                     "bytes_per_launch": spmv_bytes},
+                     "kernel": "k_stream<double,double,EpiStore<double>> (finest-level CSR SpMV)",
+                     "bytes_per_launch": spmv_bytes, "copy_kernel_gbs": copy_gbs},
         "cpu_baseline": cpu,
     }
     line.update(extra)

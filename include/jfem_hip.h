@@ -62,6 +62,9 @@ int jfem_ctx_sync(jfem_ctx *ctx);
 /* event timer on the context's stream: start, then stop returns milliseconds */
 int jfem_timer_start(jfem_ctx *ctx);
 int jfem_timer_stop(jfem_ctx *ctx, double *ms);
+/* HBM calibration (no reference counterpart): mean time of a plain copy of n
+ * doubles (8n bytes read + 8n written) over `reps` launches */
+int jfem_copy_bench(jfem_ctx *ctx, long n, int reps, double *ms);
 
 /* ---- space: new ContinuousTPFESpace(start, end, cells) + assembleFunctions(k)
  *      (M/tensorproduct/ContinuousTPFESpace.java:17-50) -------------------- */

@@ -18,6 +18,22 @@ __global__ void k_fill(double* __restrict__ x, double v, long n)
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] = v;
 }
 
+// plain copy b = a with 16-byte accesses, four in flight per thread (HBM calibration kernel)
+__global__ __launch_bounds__(256) void k_copy(long n, const double* __restrict__ a, double* __restrict__ b)
+{
+    const long n2 = n / 2;
+    const double2* __restrict__ a2 = reinterpret_cast<const double2*>(a);
+    double2* __restrict__ b2 = reinterpret_cast<double2*>(b);
+    const long stride = (long)gridDim.x * blockDim.x;
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n2; i += 4 * stride) {
+        const double2 v0 = a2[i], v1 = a2[i + stride], v2 = a2[i + 2 * stride], v3 = a2[i + 3 * stride];
+        b2[i] = v0; b2[i + stride] = v1; b2[i + 2 * stride] = v2; b2[i + 3 * stride] = v3;
+    }
+    for (; i < n2; i += stride) b2[i] = a2[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) b[n - 1] = a[n - 1];
+}
+
 __global__ void k_axpby(long n, double a, const double* __restrict__ x, double b, double* __restrict__ y)
 {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -101,25 +117,23 @@ void reduce_partials(jfem_ctx* c, int slots, int blocks, bool to_host, double* o
     }
 }
 
-// one block of 1024 threads sums an arbitrary-length partial array in a fixed order
-__global__ __launch_bounds__(1024) void k_reduce_array(const double* __restrict__ a, long n, double* __restrict__ out)
+// arbitrary-length partial array -> one value: RED_BLOCKS blocks write fixed-order
+// partials, then a single block sums them (deterministic)
+__global__ __launch_bounds__(BS) void k_reduce_array(const double* __restrict__ a, long n, double* __restrict__ partials)
 {
-    __shared__ double sh[16];
+    __shared__ double sh[BS / 64];
     double acc = 0;
-    for (long i = threadIdx.x; i < n; i += 1024) acc += a[i];
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double r = 0;
-        for (int i = 0; i < 16; i++) r += sh[i];
-        *out = r;
-    }
+    for (long i = blockIdx.x * (long)BS + threadIdx.x; i < n; i += (long)gridDim.x * BS) acc += a[i];
+    const double r = block_sum<BS>(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
 }
 
 void reduce_array_dev(jfem_ctx* c, const double* partials, long n, double* dst)
 {
-    k_reduce_array<<<1, 1024, 0, c->stream>>>(partials, n, dst);
+    const int blocks = grid_for(n, BS * 4, 256);
+    k_reduce_array<<<blocks, BS, 0, c->stream>>>(partials, n, c->red_partials);
+    JFEM_HIP(hipGetLastError());
+    k_reduce_partials<<<1, BS, 0, c->stream>>>(1, blocks, c->red_partials, dst);
     JFEM_HIP(hipGetLastError());
 }
 
@@ -281,6 +295,34 @@ int jfem_vec_dot(const jfem_vec* x, const jfem_vec* y, double* out)
     JFEM_CHECK(x && y && out, JFEM_ERR_ARG, "null argument");
     JFEM_CHECK(x->n == y->n, JFEM_ERR_ARG, "Vectors have different size");
     *out = vec_dot(x->ctx, x->n, x->d, y->d);
+    API_END
+}
+
+int jfem_copy_bench(jfem_ctx* c, long n, int reps, double* ms)
+{
+    API_BEGIN
+    JFEM_CHECK(c && ms && n > 0 && reps > 0, JFEM_ERR_ARG, "bad argument");
+    double *a = nullptr, *b = nullptr;
+    JFEM_HIP(hipMalloc(&a, sizeof(double) * n));
+    JFEM_HIP(hipMalloc(&b, sizeof(double) * n));
+    vec_fill(c, a, 1.0, n);
+    vec_fill(c, b, 0.0, n);
+    hipEvent_t e0, e1;
+    JFEM_HIP(hipEventCreate(&e0));
+    JFEM_HIP(hipEventCreate(&e1));
+    const int g = grid_for(n / 2, BS, c->cu_count * 8);
+    k_copy<<<g, BS, 0, c->stream>>>(n, a, b);
+    JFEM_HIP(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; r++) k_copy<<<g, BS, 0, c->stream>>>(n, a, b);
+    JFEM_HIP(hipEventRecord(e1, c->stream));
+    JFEM_HIP(hipEventSynchronize(e1));
+    float f = 0;
+    JFEM_HIP(hipEventElapsedTime(&f, e0, e1));
+    *ms = f / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
     API_END
 }
 

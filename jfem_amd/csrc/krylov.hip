@@ -483,7 +483,7 @@ static void solve_gmres(jfem_mat* A, jfem_mg* M, const double* b, double* x, dou
         (void)hipFree(V);
         throw;
     }
-    hipFree(V);
+    (void)hipFree(V);
     info->iterations = iterations;
     info->resnorm = residual >= 0 ? residual : true_resnorm(A, b, x, t.p);
 }

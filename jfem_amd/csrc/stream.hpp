@@ -15,8 +15,19 @@
 
 namespace jfem {
 
-constexpr int ST_BS = 256;
-constexpr int ST_CAP = 2048;
+#ifndef JFEM_ST_BS
+#define JFEM_ST_BS 256
+#endif
+#ifndef JFEM_ST_CAP
+#define JFEM_ST_CAP 2048
+#endif
+#ifndef JFEM_ST_NT
+#define JFEM_ST_NT 0
+#endif
+constexpr int ST_BS = JFEM_ST_BS;     // threads per workgroup
+constexpr int ST_CAP = JFEM_ST_CAP;   // entries per row block
+constexpr int ST_PER = ST_CAP / ST_BS;
+static_assert(ST_CAP % ST_BS == 0, "block capacity must be a multiple of the workgroup size");
 
 template <int NT>
 __device__ inline double st_block_sum(double v, double* sh)
@@ -52,7 +63,29 @@ __global__ __launch_bounds__(ST_BS) void k_stream(const int* __restrict__ rblk, 
         const double s = st_block_sum<ST_BS>(acc, sh);
         if (tid == 0) dsum = epi(r0, (T)s);
     } else {
-        for (int i = tid; i < cnt; i += ST_BS) prod[i] = (T)vals[p0 + i] * x[cols[p0 + i]];
+        // all loads of a thread are issued before any product is formed
+        VT v[ST_PER];
+        int c[ST_PER];
+#pragma unroll
+        for (int k = 0; k < ST_PER; k++) {
+            const int i = tid + k * ST_BS;
+#if JFEM_ST_NT
+            if (i < cnt) { v[k] = __builtin_nontemporal_load(&vals[p0 + i]); c[k] = __builtin_nontemporal_load(&cols[p0 + i]); }
+#else
+            if (i < cnt) { v[k] = vals[p0 + i]; c[k] = cols[p0 + i]; }
+#endif
+        }
+        T xv[ST_PER];
+#pragma unroll
+        for (int k = 0; k < ST_PER; k++) {
+            const int i = tid + k * ST_BS;
+            if (i < cnt) xv[k] = x[c[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < ST_PER; k++) {
+            const int i = tid + k * ST_BS;
+            if (i < cnt) prod[i] = (T)v[k] * xv[k];
+        }
         __syncthreads();
         int G = 64;
         while (G > 1 && G * nr > ST_BS) G >>= 1;
