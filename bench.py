@@ -87,6 +87,17 @@ def cpu_baseline(seconds_budget=30.0):
     return dict(solve_s=solve, iterations=it, n=A.n, setup_s=setup, spmv_gbs=bytes_spmv / t_spmv / 1e9)
 
 
+def measured_traffic(cells):
+    """HBM bytes per SpMV launch from the committed rocprofv3 PMC summary (96^3 only)."""
+    path = os.path.join(ROOT, "profiles", "pmc_spmv_fp64_q2_96.md")
+    if cells != 96 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def copy_calibration(jf, n=1 << 27, reps=20):
     import ctypes as C
     from jfem_amd import _native as N
@@ -233,7 +244,7 @@ def main():
         "assembly_s": t_asm, "hierarchy_setup_s": t_mg,
         "spmv_fp64_ms": spmv_ms, "spmv_fp64_gbs": spmv_gbs, "spmv_fp32_ms": spmv32_ms, "spmv_fp32_gbs": spmv32_gbs,
         "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(args.cells),
                      "kernel": "k_stream<double,double,EpiStore<double>> (finest-level CSR SpMV)",
                      "bytes_per_launch": spmv_bytes, "copy_kernel_gbs": copy_gbs},
         "cpu_baseline": cpu,
