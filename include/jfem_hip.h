@@ -51,6 +51,7 @@ typedef struct jfem_space jfem_space;
 typedef struct jfem_vec jfem_vec;
 typedef struct jfem_mat jfem_mat;
 typedef struct jfem_mg jfem_mg;
+typedef struct jfem_nn jfem_nn;
 
 const char *jfem_last_error(void);
 
@@ -180,6 +181,26 @@ int jfem_mg_apply(jfem_mg *mg, const jfem_vec *r, jfem_vec *z);
 int jfem_mg_vcycle(jfem_mg *mg, jfem_vec *x, const jfem_vec *b);
 /* Chebyshev interval upper bound used on each level (after safety factor) */
 int jfem_mg_lmax(jfem_mg *mg, double *out);
+
+/* ---- learned smoother network (not in the reference) --------------------
+ * four 3^dim-tap convolutions on the node lattice: cin -> 32 -> 32 -> 32 -> 1
+ * with ReLU; BF16 MFMA for the 32->32 layers, FP32 accumulation, FP32 master
+ * weights and Adam.  Lattice arrays are in lattice order, axis 0 fastest;
+ * inputs are pixel-major (np x cin). */
+int jfem_nn_create(jfem_ctx *ctx, int dim, int cin, unsigned long long seed, jfem_nn **out);
+int jfem_nn_destroy(jfem_nn *nn);
+long jfem_nn_num_params(const jfem_nn *nn);
+/* parameters in the order W0[t][32][cin], b0[32], W1[t][32][32], b1, W2, b2, W3[t][32], b3 */
+int jfem_nn_get_params(jfem_nn *nn, float *out);
+int jfem_nn_set_params(jfem_nn *nn, const float *in);
+int jfem_nn_forward_host(jfem_nn *nn, const int *shape, const float *in, float *out);
+/* gradient of sum(gout * forward(in)) with respect to the parameters */
+int jfem_nn_gradient_host(jfem_nn *nn, const int *shape, const float *in, const float *gout, float *grad);
+/* one Adam step (grad NULL: the gradient left by the last backward pass) */
+int jfem_nn_adam_step(jfem_nn *nn, const float *grad, float lr);
+/* versioned checkpoint of weights and optimiser state */
+int jfem_nn_save(jfem_nn *nn, const char *path);
+int jfem_nn_load(jfem_ctx *ctx, const char *path, jfem_nn **out);
 
 #ifdef __cplusplus
 }

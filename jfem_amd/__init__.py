@@ -5,12 +5,12 @@ libjfem_hip.so behind the C ABI of include/jfem_hip.h."""
 from ._native import (IllegalArgumentException, IllegalStateException, JfemError, NativeLibraryError,
                       UnsupportedOperationException, lib)
 from .api import (FP32, FP64, ChebyshevSmoother, ContinuousTPFESpace, Context, DenseVector, IterativeSolver,
-                  JacobiSmoother, MGPreconditioner, QuadratureRule1D, RichardsonSmoother, SinusoidSource,
+                  JacobiSmoother, MGPreconditioner, QuadratureRule1D, RichardsonSmoother, SinusoidSource, SmootherNet,
                   SparseMatrix, TPCellIntegral, TPRightHandSideIntegral, context)
 
 __all__ = [
     "IllegalArgumentException", "IllegalStateException", "JfemError", "NativeLibraryError",
     "UnsupportedOperationException", "lib", "FP32", "FP64", "ChebyshevSmoother", "ContinuousTPFESpace", "Context",
     "DenseVector", "IterativeSolver", "JacobiSmoother", "MGPreconditioner", "QuadratureRule1D",
-    "RichardsonSmoother", "SinusoidSource", "SparseMatrix", "TPCellIntegral", "TPRightHandSideIntegral", "context",
+    "RichardsonSmoother", "SinusoidSource", "SmootherNet", "SparseMatrix", "TPCellIntegral", "TPRightHandSideIntegral", "context",
 ]

@@ -97,6 +97,16 @@ _SIGS = {
     "jfem_mg_apply": (C.c_int, [_vp, _vp, _vp]),
     "jfem_mg_vcycle": (C.c_int, [_vp, _vp, _vp]),
     "jfem_mg_lmax": (C.c_int, [_vp, _dp]),
+    "jfem_nn_create": (C.c_int, [_vp, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(_vp)]),
+    "jfem_nn_destroy": (C.c_int, [_vp]),
+    "jfem_nn_num_params": (C.c_long, [_vp]),
+    "jfem_nn_get_params": (C.c_int, [_vp, _fp]),
+    "jfem_nn_set_params": (C.c_int, [_vp, _fp]),
+    "jfem_nn_forward_host": (C.c_int, [_vp, _ip, _fp, _fp]),
+    "jfem_nn_gradient_host": (C.c_int, [_vp, _ip, _fp, _fp, _fp]),
+    "jfem_nn_adam_step": (C.c_int, [_vp, _fp, C.c_float]),
+    "jfem_nn_save": (C.c_int, [_vp, C.c_char_p]),
+    "jfem_nn_load": (C.c_int, [_vp, C.c_char_p, C.POINTER(_vp)]),
 }
 
 _lib = None
@@ -145,6 +155,14 @@ def iptr(a):
 
 def lptr(a):
     return a.ctypes.data_as(_lp)
+
+
+def fptr(a):
+    return a.ctypes.data_as(_fp)
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
 
 
 def f64(a):

@@ -14,7 +14,7 @@ from typing import Callable, Optional, Sequence
 import numpy as np
 
 from . import _native as N
-from ._native import check, dptr, f64, i32, iptr, lptr
+from ._native import check, dptr, f32, f64, fptr, i32, iptr, lptr
 
 # --------------------------------------------------------------------------
 # context
@@ -553,3 +553,74 @@ class IterativeSolver:
         check(N.lib().jfem_solve_gmres(operator.h, preconditioner.h, rhs.h, x.h, tol, restart, max_cycles,
                                        C.byref(self.info)))
         return self._finish(x)
+
+
+# --------------------------------------------------------------------------
+# learned smoother network
+# --------------------------------------------------------------------------
+class SmootherNet:
+    """Convolutional network cin -> 32 -> 32 -> 32 -> 1 with 3^dim taps on the node
+    lattice (jfem_nn): BF16 MFMA layers, FP32 master weights and Adam."""
+
+    def __init__(self, dim: int = 2, cin: int = 2, seed: int = 0, ctx: Optional[Context] = None, handle=None):
+        self.ctx = ctx or context()
+        if handle is None:
+            h = C.c_void_p()
+            check(N.lib().jfem_nn_create(self.ctx.h, dim, cin, seed, C.byref(h)))
+            handle = h
+        self.h = handle
+        self.dim, self.cin = dim, cin
+
+    @classmethod
+    def load(cls, path: str, ctx: Optional[Context] = None):
+        ctx = ctx or context()
+        h = C.c_void_p()
+        check(N.lib().jfem_nn_load(ctx.h, path.encode(), C.byref(h)))
+        net = cls.__new__(cls)
+        net.ctx, net.h = ctx, h
+        n = N.lib().jfem_nn_num_params(h)
+        # recover dim/cin from the parameter count
+        for dim in (1, 2, 3):
+            for cin in range(1, 9):
+                T = 3 ** dim
+                if T * 32 * cin + 32 + 2 * (T * 1024 + 32) + T * 32 + 1 == n:
+                    net.dim, net.cin = dim, cin
+        return net
+
+    def __del__(self):
+        if getattr(self, "h", None) and N._lib is not None:
+            N.lib().jfem_nn_destroy(self.h)
+            self.h = None
+
+    def num_params(self) -> int:
+        return N.lib().jfem_nn_num_params(self.h)
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.num_params(), dtype=np.float32)
+        check(N.lib().jfem_nn_get_params(self.h, fptr(out)))
+        return out
+
+    def set_params(self, p):
+        p = f32(p)
+        if p.shape[0] != self.num_params():
+            raise N.IllegalArgumentException(1, "wrong parameter count")
+        check(N.lib().jfem_nn_set_params(self.h, fptr(p)))
+
+    def forward(self, shape, inp) -> np.ndarray:
+        shp = i32(shape)
+        x = f32(inp)
+        out = np.zeros(int(np.prod(shape)), dtype=np.float32)
+        check(N.lib().jfem_nn_forward_host(self.h, iptr(shp), fptr(x), fptr(out)))
+        return out
+
+    def gradient(self, shape, inp, gout) -> np.ndarray:
+        shp = i32(shape)
+        g = np.zeros(self.num_params(), dtype=np.float32)
+        check(N.lib().jfem_nn_gradient_host(self.h, iptr(shp), fptr(f32(inp)), fptr(f32(gout)), fptr(g)))
+        return g
+
+    def adam_step(self, grad=None, lr: float = 1e-3):
+        check(N.lib().jfem_nn_adam_step(self.h, fptr(f32(grad)) if grad is not None else None, lr))
+
+    def save(self, path: str):
+        check(N.lib().jfem_nn_save(self.h, path.encode()))

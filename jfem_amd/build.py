@@ -20,7 +20,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libjfem_hip.so")
 ARCH = os.environ.get("JFEM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["core.hip", "assemble.hip", "spmv.hip", "krylov.hip", "mg.hip"]
+SOURCES = ["core.hip", "assemble.hip", "spmv.hip", "krylov.hip", "mg.hip", "nn.hip"]
 
 CFLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -1,0 +1,646 @@
+// Learned smoother network on the structured node lattice: four 3^d-tap
+// convolutions (cin -> 32 -> 32 -> 32 -> 1, ReLU between), trained on the GPU.
+//
+// Layout: pixels are lattice nodes in lattice order (axis 0 fastest); hidden
+// activations are NHWC bf16 (32 channels, 64 bytes per pixel).  The two dense
+// 32 -> 32 layers (forward, input gradient, weight gradient) run on MFMA
+// (v_mfma_f32_32x32x16_bf16: BF16 operands, FP32 accumulation); the thin first
+// and last layers (cin and 1 channels) run as FP32 FMA kernels.  Master
+// weights and Adam state are FP32; the MFMA layers read BF16 copies refreshed
+// after every optimiser step.
+#include "common.hpp"
+#include "nn.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <fstream>
+
+namespace jfem {
+
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// round-to-nearest-even float -> bf16 bits
+__host__ __device__ inline unsigned short f2bf(float f)
+{
+    unsigned int u;
+    __builtin_memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)(u >> 16);   // inf/nan
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__host__ __device__ inline float bf2f(unsigned short h)
+{
+    unsigned int u = (unsigned int)h << 16;
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+}
+
+// neighbour of pixel p at tap t (offsets -1/0/+1 per axis, axis 0 fastest in t);
+// sign = +1 for the forward conv, -1 for its transpose; -1 if outside
+__device__ inline long lat_nb(const Lat& L, long p, int t, int sign)
+{
+    long q = p;
+    long stride = 1;
+    long r = p;
+    for (int a = 0; a < L.d; a++) {
+        const int ia = (int)(r % L.n[a]);
+        r /= L.n[a];
+        const int o = sign * ((t % 3) - 1);
+        t /= 3;
+        const int j = ia + o;
+        if (j < 0 || j >= L.n[a]) return -1;
+        q += (long)o * stride;
+        stride *= L.n[a];
+    }
+    return q;
+}
+
+// ------------------------------------------------------------- thin layers
+// first layer: out[p][co] = relu(b0[co] + sum_t sum_ci W0[t][co][ci] in[nb][ci])
+__global__ __launch_bounds__(256) void k_first(Lat L, int cin, int taps, const float* __restrict__ in,
+                                               const float* __restrict__ W0, const float* __restrict__ b0,
+                                               unsigned short* __restrict__ out)
+{
+    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < L.np; p += (long)gridDim.x * blockDim.x) {
+        float acc[NN_H];
+#pragma unroll
+        for (int c = 0; c < NN_H; c++) acc[c] = b0[c];
+        for (int t = 0; t < taps; t++) {
+            const long q = lat_nb(L, p, t, 1);
+            if (q < 0) continue;
+            for (int ci = 0; ci < cin; ci++) {
+                const float v = in[q * cin + ci];
+#pragma unroll
+                for (int c = 0; c < NN_H; c++) acc[c] += W0[(t * NN_H + c) * cin + ci] * v;
+            }
+        }
+        unsigned short* o = out + p * NN_H;
+#pragma unroll
+        for (int c = 0; c < NN_H; c++) o[c] = f2bf(acc[c] > 0.f ? acc[c] : 0.f);
+    }
+}
+
+// last layer: out[p] = b3 + sum_t sum_ci W3[t][ci] act[nb][ci]
+__global__ __launch_bounds__(256) void k_last(Lat L, int taps, const unsigned short* __restrict__ act,
+                                              const float* __restrict__ W3, const float* __restrict__ b3,
+                                              float* __restrict__ out)
+{
+    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < L.np; p += (long)gridDim.x * blockDim.x) {
+        float acc = b3[0];
+        for (int t = 0; t < taps; t++) {
+            const long q = lat_nb(L, p, t, 1);
+            if (q < 0) continue;
+            const unsigned short* a = act + q * NN_H;
+#pragma unroll
+            for (int c = 0; c < NN_H; c++) acc += W3[t * NN_H + c] * bf2f(a[c]);
+        }
+        out[p] = acc;
+    }
+}
+
+// ------------------------------------------------------------- MFMA layer
+// D[co][pixel] = sum_k W[co][k] * act[k][pixel], k = (tap, ci), 32x32 tile per
+// wavefront (32 consecutive pixels), K = taps * 32 in steps of 16.
+// mode 0: forward, out = relu(D + bias)            (sign +1, weights [t][co][ci])
+// mode 1: input gradient, out = D * [mask > 0]     (sign -1, weights [t][ci][co])
+template <int MODE>
+__global__ __launch_bounds__(256) void k_mid(Lat L, int taps, const unsigned short* __restrict__ in,
+                                             const unsigned short* __restrict__ Wh, const float* __restrict__ bias,
+                                             const unsigned short* __restrict__ mask, unsigned short* __restrict__ out)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const long ntiles = (L.np + 31) / 32;
+    const int j = lane & 31;       // pixel column of the tile / output row of W
+    const int kh = lane >> 5;      // which 8-wide half of the 16-wide K step
+    for (long tile = blockIdx.x * 4L + wave; tile < ntiles; tile += (long)gridDim.x * 4) {
+        const long p = tile * 32 + j;
+        const bool valid = p < L.np;
+        f32x16 acc;
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[r] = 0.f;
+        for (int t = 0; t < taps; t++) {
+            const long q = valid ? lat_nb(L, p, t, MODE == 0 ? 1 : -1) : -1;
+#pragma unroll
+            for (int kb = 0; kb < 2; kb++) {
+                const int c0 = kb * 16 + kh * 8;
+                bf16x8 a = *reinterpret_cast<const bf16x8*>(Wh + ((long)t * NN_H + j) * NN_H + c0);
+                bf16x8 b;
+                if (q >= 0) b = *reinterpret_cast<const bf16x8*>(in + q * NN_H + c0);
+                else {
+#pragma unroll
+                    for (int e = 0; e < 8; e++) b[e] = 0;
+                }
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            }
+        }
+        if (!valid) continue;
+        // lane holds D[c][p] for c = 8*(r/4) + 4*kh + (r%4)
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+            unsigned short v4[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int c = 8 * g + 4 * kh + e;
+                float v = acc[4 * g + e];
+                if (MODE == 0) { v += bias[c]; v = v > 0.f ? v : 0.f; }
+                else v = bf2f(mask[p * NN_H + c]) > 0.f ? v : 0.f;
+                v4[e] = f2bf(v);
+            }
+            unsigned long long pk = (unsigned long long)v4[0] | ((unsigned long long)v4[1] << 16) |
+                                    ((unsigned long long)v4[2] << 32) | ((unsigned long long)v4[3] << 48);
+            *reinterpret_cast<unsigned long long*>(out + p * NN_H + 8 * g + 4 * kh) = pk;
+        }
+    }
+}
+
+// weight gradient of a 32->32 layer on MFMA:
+// dW[t][co][ci] = sum_p gout[p][co] * in[nb(p,t)][ci]; each block covers a
+// chunk of pixels and writes its partial (taps*32*32 floats); waves split taps.
+__global__ __launch_bounds__(256) void k_mid_dw(Lat L, int taps, long chunk, const unsigned short* __restrict__ gout,
+                                                const unsigned short* __restrict__ in, float* __restrict__ partials)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int i = lane & 31;   // A row (co) / B column (ci)
+    const int kh = lane >> 5;
+    const long p_begin = blockIdx.x * chunk;
+    const long p_end = min(p_begin + chunk, L.np);
+    for (int t = wave; t < taps; t += 4) {
+        f32x16 acc;
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[r] = 0.f;
+        for (long pb = p_begin; pb < p_end; pb += 16) {
+            bf16x8 a, b;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const long p = pb + kh * 8 + e;
+                unsigned short av = 0, bv = 0;
+                if (p < p_end) {
+                    av = gout[p * NN_H + i];
+                    const long q = lat_nb(L, p, t, 1);
+                    if (q >= 0) bv = in[q * NN_H + i];
+                }
+                a[e] = (short)av;
+                b[e] = (short)bv;
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        float* out = partials + (long)blockIdx.x * taps * NN_H * NN_H + (long)t * NN_H * NN_H;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int co = 8 * (r / 4) + 4 * kh + (r % 4);
+            out[co * NN_H + i] = acc[r];
+        }
+    }
+}
+
+// bias gradient partials: db[c] = sum_p g[p][c]
+__global__ __launch_bounds__(256) void k_bias_grad(long np, long chunk, const unsigned short* __restrict__ g,
+                                                   float* __restrict__ partials)
+{
+    __shared__ float sh[256];
+    const int c = threadIdx.x & 31, sub = threadIdx.x >> 5;   // 8 pixel lanes per channel
+    const long p0 = blockIdx.x * chunk, p1 = min(p0 + chunk, np);
+    float acc = 0;
+    for (long p = p0 + sub; p < p1; p += 8) acc += bf2f(g[p * NN_H + c]);
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        float s = 0;
+        for (int k = 0; k < 8; k++) s += sh[k * 32 + threadIdx.x];
+        partials[blockIdx.x * NN_H + threadIdx.x] = s;
+    }
+}
+
+// last layer backward: g2[p][ci] = [act2[p][ci] > 0] * sum_t W3[t][ci] gout[nb(p,t,-1)]
+__global__ __launch_bounds__(256) void k_last_dx(Lat L, int taps, const float* __restrict__ gout,
+                                                 const float* __restrict__ W3, const unsigned short* __restrict__ act2,
+                                                 unsigned short* __restrict__ g2)
+{
+    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < L.np; p += (long)gridDim.x * blockDim.x) {
+        float acc[NN_H];
+#pragma unroll
+        for (int c = 0; c < NN_H; c++) acc[c] = 0;
+        for (int t = 0; t < taps; t++) {
+            const long q = lat_nb(L, p, t, -1);
+            if (q < 0) continue;
+            const float g = gout[q];
+#pragma unroll
+            for (int c = 0; c < NN_H; c++) acc[c] += W3[t * NN_H + c] * g;
+        }
+        const unsigned short* a = act2 + p * NN_H;
+#pragma unroll
+        for (int c = 0; c < NN_H; c++) g2[p * NN_H + c] = f2bf(bf2f(a[c]) > 0.f ? acc[c] : 0.f);
+    }
+}
+
+// weight gradients of the thin layers, per-block partials:
+// last:  dW3[t][ci] = sum_p gout[p] act2[nb(p,t)][ci],  db3 = sum_p gout[p]
+// first: dW0[t][co][ci] = sum_p g0[p][co] in[nb(p,t)][ci]
+// one thread per output weight, pixels of the block's chunk staged in LDS
+__global__ __launch_bounds__(256) void k_last_dw(Lat L, int taps, long chunk, const float* __restrict__ gout,
+                                                 const unsigned short* __restrict__ act2, float* __restrict__ partials)
+{
+    const int nout = taps * NN_H + 1;
+    const long p0 = blockIdx.x * chunk, p1 = min(p0 + chunk, L.np);
+    for (int o = threadIdx.x; o < nout; o += blockDim.x) {
+        float acc = 0;
+        if (o == nout - 1) {
+            for (long p = p0; p < p1; p++) acc += gout[p];
+        } else {
+            const int t = o / NN_H, ci = o % NN_H;
+            for (long p = p0; p < p1; p++) {
+                const long q = lat_nb(L, p, t, 1);
+                if (q >= 0) acc += gout[p] * bf2f(act2[q * NN_H + ci]);
+            }
+        }
+        partials[blockIdx.x * (long)nout + o] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_first_dw(Lat L, int taps, int cin, long chunk,
+                                                  const unsigned short* __restrict__ g0, const float* __restrict__ in,
+                                                  float* __restrict__ partials)
+{
+    const int nout = taps * NN_H * cin;
+    const long p0 = blockIdx.x * chunk, p1 = min(p0 + chunk, L.np);
+    for (int o = threadIdx.x; o < nout; o += blockDim.x) {
+        const int t = o / (NN_H * cin), co = (o / cin) % NN_H, ci = o % cin;
+        float acc = 0;
+        for (long p = p0; p < p1; p++) {
+            const long q = lat_nb(L, p, t, 1);
+            if (q >= 0) acc += bf2f(g0[p * NN_H + co]) * in[q * cin + ci];
+        }
+        partials[blockIdx.x * (long)nout + o] = acc;
+    }
+}
+
+// out[o] (+)= sum_b partials[b][o]   (fixed order over blocks)
+__global__ void k_sum_blocks(int nblocks, int nout, const float* __restrict__ partials, float* __restrict__ out, int add)
+{
+    for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < nout; o += gridDim.x * blockDim.x) {
+        float s = 0;
+        for (int b = 0; b < nblocks; b++) s += partials[(long)b * nout + o];
+        out[o] = add ? out[o] + s : s;
+    }
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master weights
+__global__ void k_adam(long n, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2, float clip)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        float gi = g[i] * clip;
+        const float mi = b1 * m[i] + (1 - b1) * gi;
+        const float vi = b2 * v[i] + (1 - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+// bf16 copies of a 32->32 layer: Wh[t][co][ci] and the transposed, flipped
+// WT[t][ci][co] used by the input gradient (tap t of the transpose reads nb(p,t,-1))
+__global__ void k_refresh_bf16(int taps, const float* __restrict__ W, unsigned short* __restrict__ Wh,
+                               unsigned short* __restrict__ WT)
+{
+    const int n = taps * NN_H * NN_H;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int t = i / (NN_H * NN_H), co = (i / NN_H) % NN_H, ci = i % NN_H;
+        const unsigned short h = f2bf(W[i]);
+        Wh[i] = h;
+        WT[((long)t * NN_H + ci) * NN_H + co] = h;
+    }
+}
+
+// ------------------------------------------------------------- host side
+static long param_count(int taps, int cin)
+{
+    return (long)taps * NN_H * cin + NN_H + 2L * ((long)taps * NN_H * NN_H + NN_H) + taps * NN_H + 1;
+}
+
+void nn_offsets(const jfem_nn* nn, long* o)
+{
+    // W0, b0, W1, b1, W2, b2, W3, b3
+    const long t = nn->taps, c = nn->cin;
+    o[0] = 0;
+    o[1] = o[0] + t * NN_H * c;
+    o[2] = o[1] + NN_H;
+    o[3] = o[2] + t * NN_H * NN_H;
+    o[4] = o[3] + NN_H;
+    o[5] = o[4] + t * NN_H * NN_H;
+    o[6] = o[5] + NN_H;
+    o[7] = o[6] + t * NN_H;
+    o[8] = o[7] + 1;
+}
+
+void nn_refresh(jfem_nn* nn)
+{
+    long o[9];
+    nn_offsets(nn, o);
+    hipStream_t st = nn->ctx->stream;
+    k_refresh_bf16<<<64, 256, 0, st>>>(nn->taps, nn->w + o[2], nn->wh1, nn->wt1);
+    k_refresh_bf16<<<64, 256, 0, st>>>(nn->taps, nn->w + o[4], nn->wh2, nn->wt2);
+    JFEM_HIP(hipGetLastError());
+}
+
+void nn_reserve(jfem_nn* nn, long np)
+{
+    if (np <= nn->cap) return;
+    for (void** p : {(void**)&nn->act0, (void**)&nn->act1, (void**)&nn->act2, (void**)&nn->g0, (void**)&nn->g1,
+                     (void**)&nn->g2, (void**)&nn->in, (void**)&nn->out, (void**)&nn->gout})
+        (void)hipFree(*p);
+    const size_t hb = sizeof(unsigned short) * NN_H * (size_t)np;
+    JFEM_HIP(hipMalloc(&nn->act0, hb));
+    JFEM_HIP(hipMalloc(&nn->act1, hb));
+    JFEM_HIP(hipMalloc(&nn->act2, hb));
+    JFEM_HIP(hipMalloc(&nn->g0, hb));
+    JFEM_HIP(hipMalloc(&nn->g1, hb));
+    JFEM_HIP(hipMalloc(&nn->g2, hb));
+    JFEM_HIP(hipMalloc(&nn->in, sizeof(float) * nn->cin * (size_t)np));
+    JFEM_HIP(hipMalloc(&nn->out, sizeof(float) * (size_t)np));
+    JFEM_HIP(hipMalloc(&nn->gout, sizeof(float) * (size_t)np));
+    nn->cap = np;
+}
+
+static int nn_grid(long np) { return grid_for(np, 256, 8192); }
+
+// forward on nn->in (np x cin FP32, lattice order) into nn->out (np FP32)
+void nn_forward(jfem_nn* nn, const Lat& L)
+{
+    long o[9];
+    nn_offsets(nn, o);
+    hipStream_t st = nn->ctx->stream;
+    const long ntiles = (L.np + 31) / 32;
+    const int gm = (int)std::min<long>((ntiles + 3) / 4, 16384);
+    k_first<<<nn_grid(L.np), 256, 0, st>>>(L, nn->cin, nn->taps, nn->in, nn->w + o[0], nn->w + o[1], nn->act0);
+    k_mid<0><<<gm, 256, 0, st>>>(L, nn->taps, nn->act0, nn->wh1, nn->w + o[3], nullptr, nn->act1);
+    k_mid<0><<<gm, 256, 0, st>>>(L, nn->taps, nn->act1, nn->wh2, nn->w + o[5], nullptr, nn->act2);
+    k_last<<<nn_grid(L.np), 256, 0, st>>>(L, nn->taps, nn->act2, nn->w + o[6], nn->w + o[7], nn->out);
+    JFEM_HIP(hipGetLastError());
+}
+
+// backward from nn->gout (dLoss/dout) after nn_forward on the same input;
+// writes (add = 0) or accumulates (add = 1) the gradient into nn->grad
+void nn_backward(jfem_nn* nn, const Lat& L, int add)
+{
+    long o[9];
+    nn_offsets(nn, o);
+    hipStream_t st = nn->ctx->stream;
+    const int taps = nn->taps;
+    const long chunk = 4096;
+    const int nb = (int)((L.np + chunk - 1) / chunk);
+    const long need = (long)nb * std::max<long>(taps * NN_H * NN_H, taps * NN_H * std::max(nn->cin, 1) + NN_H + 1);
+    if (need > nn->part_cap) {
+        (void)hipFree(nn->part);
+        JFEM_HIP(hipMalloc(&nn->part, sizeof(float) * need));
+        nn->part_cap = need;
+    }
+    const long ntiles = (L.np + 31) / 32;
+    const int gm = (int)std::min<long>((ntiles + 3) / 4, 16384);
+    // layer 3 (last)
+    k_last_dw<<<nb, 256, 0, st>>>(L, taps, chunk, nn->gout, nn->act2, nn->part);
+    k_sum_blocks<<<4, 256, 0, st>>>(nb, taps * NN_H + 1, nn->part, nn->grad + o[6], add);
+    k_last_dx<<<nn_grid(L.np), 256, 0, st>>>(L, taps, nn->gout, nn->w + o[6], nn->act2, nn->g2);
+    // layer 2: dW2 from g2 and act1, bias, input gradient masked by act1
+    k_mid_dw<<<nb, 256, 0, st>>>(L, taps, chunk, nn->g2, nn->act1, nn->part);
+    k_sum_blocks<<<64, 256, 0, st>>>(nb, taps * NN_H * NN_H, nn->part, nn->grad + o[4], add);
+    k_bias_grad<<<nb, 256, 0, st>>>(L.np, chunk, nn->g2, nn->part);
+    k_sum_blocks<<<1, 32, 0, st>>>(nb, NN_H, nn->part, nn->grad + o[5], add);
+    k_mid<1><<<gm, 256, 0, st>>>(L, taps, nn->g2, nn->wt2, nullptr, nn->act1, nn->g1);
+    // layer 1
+    k_mid_dw<<<nb, 256, 0, st>>>(L, taps, chunk, nn->g1, nn->act0, nn->part);
+    k_sum_blocks<<<64, 256, 0, st>>>(nb, taps * NN_H * NN_H, nn->part, nn->grad + o[2], add);
+    k_bias_grad<<<nb, 256, 0, st>>>(L.np, chunk, nn->g1, nn->part);
+    k_sum_blocks<<<1, 32, 0, st>>>(nb, NN_H, nn->part, nn->grad + o[3], add);
+    k_mid<1><<<gm, 256, 0, st>>>(L, taps, nn->g1, nn->wt1, nullptr, nn->act0, nn->g0);
+    // layer 0 (first): weights and bias
+    k_first_dw<<<nb, 256, 0, st>>>(L, taps, nn->cin, chunk, nn->g0, nn->in, nn->part);
+    k_sum_blocks<<<16, 256, 0, st>>>(nb, taps * NN_H * nn->cin, nn->part, nn->grad + o[0], add);
+    k_bias_grad<<<nb, 256, 0, st>>>(L.np, chunk, nn->g0, nn->part);
+    k_sum_blocks<<<1, 32, 0, st>>>(nb, NN_H, nn->part, nn->grad + o[1], add);
+    JFEM_HIP(hipGetLastError());
+}
+
+void nn_adam_step(jfem_nn* nn, float lr, float grad_scale)
+{
+    nn->step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - std::pow(b1, (float)nn->step), c2 = 1.f - std::pow(b2, (float)nn->step);
+    k_adam<<<grid_for(nn->nparams, 256, 1024), 256, 0, nn->ctx->stream>>>(nn->nparams, nn->w, nn->grad, nn->m, nn->v,
+                                                                         lr, b1, b2, 1e-8f, c1, c2, grad_scale);
+    JFEM_HIP(hipGetLastError());
+    nn_refresh(nn);
+}
+
+jfem_nn* nn_create(jfem_ctx* c, int dim, int cin, unsigned long long seed)
+{
+    JFEM_CHECK(dim >= 1 && dim <= 3, JFEM_ERR_ARG, "network dimension must be 1..3");
+    JFEM_CHECK(cin >= 1 && cin <= 8, JFEM_ERR_ARG, "1..8 input channels");
+    auto* nn = new jfem_nn();
+    nn->ctx = c;
+    nn->dim = dim;
+    nn->cin = cin;
+    nn->taps = dim == 1 ? 3 : (dim == 2 ? 9 : 27);
+    nn->nparams = param_count(nn->taps, cin);
+    const size_t bytes = sizeof(float) * nn->nparams;
+    JFEM_HIP(hipMalloc(&nn->w, bytes));
+    JFEM_HIP(hipMalloc(&nn->grad, bytes));
+    JFEM_HIP(hipMalloc(&nn->m, bytes));
+    JFEM_HIP(hipMalloc(&nn->v, bytes));
+    const size_t hb = sizeof(unsigned short) * nn->taps * NN_H * NN_H;
+    JFEM_HIP(hipMalloc(&nn->wh1, hb));
+    JFEM_HIP(hipMalloc(&nn->wt1, hb));
+    JFEM_HIP(hipMalloc(&nn->wh2, hb));
+    JFEM_HIP(hipMalloc(&nn->wt2, hb));
+    // He initialisation from SplitMix64 + Box-Muller; biases 0, last layer scaled down
+    std::vector<float> w(nn->nparams, 0.f);
+    long o[9];
+    nn_offsets(nn, o);
+    unsigned long long s = seed;
+    auto next = [&]() {
+        unsigned long long z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    auto normal = [&]() {
+        double u1 = ((next() >> 11) + 1.0) * (1.0 / 9007199254740993.0);
+        double u2 = (next() >> 11) * (1.0 / 9007199254740992.0);
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+    };
+    auto fill = [&](long a, long b, double fan_in, double gain) {
+        const double sd = gain * std::sqrt(2.0 / fan_in);
+        for (long i = a; i < b; i++) w[i] = (float)(sd * normal());
+    };
+    fill(o[0], o[1], nn->taps * cin, 1.0);
+    fill(o[2], o[3], nn->taps * NN_H, 1.0);
+    fill(o[4], o[5], nn->taps * NN_H, 1.0);
+    fill(o[6], o[7], nn->taps * NN_H, 0.1);
+    JFEM_HIP(hipMemcpy(nn->w, w.data(), bytes, hipMemcpyHostToDevice));
+    JFEM_HIP(hipMemset(nn->grad, 0, bytes));
+    JFEM_HIP(hipMemset(nn->m, 0, bytes));
+    JFEM_HIP(hipMemset(nn->v, 0, bytes));
+    nn_refresh(nn);
+    JFEM_HIP(hipStreamSynchronize(c->stream));
+    return nn;
+}
+
+void nn_free(jfem_nn* nn)
+{
+    if (!nn) return;
+    (void)hipStreamSynchronize(nn->ctx->stream);
+    for (void* p : {(void*)nn->w, (void*)nn->grad, (void*)nn->m, (void*)nn->v, (void*)nn->wh1, (void*)nn->wt1,
+                    (void*)nn->wh2, (void*)nn->wt2, (void*)nn->act0, (void*)nn->act1, (void*)nn->act2, (void*)nn->g0,
+                    (void*)nn->g1, (void*)nn->g2, (void*)nn->in, (void*)nn->out, (void*)nn->gout, (void*)nn->part})
+        (void)hipFree(p);
+    delete nn;
+}
+
+Lat make_lat(int d, const int* n)
+{
+    Lat L;
+    L.d = d;
+    L.np = 1;
+    for (int a = 0; a < 3; a++) {
+        L.n[a] = a < d ? n[a] : 1;
+        L.np *= L.n[a];
+    }
+    return L;
+}
+
+}  // namespace jfem
+
+using namespace jfem;
+
+static const char NN_MAGIC[8] = {'J', 'F', 'E', 'M', 'N', 'N', '0', '1'};
+
+extern "C" {
+
+int jfem_nn_create(jfem_ctx* ctx, int dim, int cin, unsigned long long seed, jfem_nn** out)
+{
+    API_BEGIN
+    JFEM_CHECK(ctx && out, JFEM_ERR_ARG, "null argument");
+    *out = nn_create(ctx, dim, cin, seed);
+    API_END
+}
+
+int jfem_nn_destroy(jfem_nn* nn)
+{
+    API_BEGIN
+    nn_free(nn);
+    API_END
+}
+
+long jfem_nn_num_params(const jfem_nn* nn) { return nn ? nn->nparams : -1; }
+
+int jfem_nn_get_params(jfem_nn* nn, float* out)
+{
+    API_BEGIN
+    JFEM_CHECK(nn && out, JFEM_ERR_ARG, "null argument");
+    JFEM_HIP(hipMemcpy(out, nn->w, sizeof(float) * nn->nparams, hipMemcpyDeviceToHost));
+    API_END
+}
+
+int jfem_nn_set_params(jfem_nn* nn, const float* in)
+{
+    API_BEGIN
+    JFEM_CHECK(nn && in, JFEM_ERR_ARG, "null argument");
+    JFEM_HIP(hipMemcpy(nn->w, in, sizeof(float) * nn->nparams, hipMemcpyHostToDevice));
+    nn_refresh(nn);
+    JFEM_HIP(hipStreamSynchronize(nn->ctx->stream));
+    API_END
+}
+
+int jfem_nn_forward_host(jfem_nn* nn, const int* shape, const float* in, float* out)
+{
+    API_BEGIN
+    JFEM_CHECK(nn && shape && in && out, JFEM_ERR_ARG, "null argument");
+    for (int a = 0; a < nn->dim; a++) JFEM_CHECK(shape[a] >= 1, JFEM_ERR_ARG, "empty lattice");
+    Lat L = make_lat(nn->dim, shape);
+    nn_reserve(nn, L.np);
+    JFEM_HIP(hipMemcpy(nn->in, in, sizeof(float) * nn->cin * L.np, hipMemcpyHostToDevice));
+    nn_forward(nn, L);
+    JFEM_HIP(hipMemcpyAsync(out, nn->out, sizeof(float) * L.np, hipMemcpyDeviceToHost, nn->ctx->stream));
+    JFEM_HIP(hipStreamSynchronize(nn->ctx->stream));
+    API_END
+}
+
+int jfem_nn_gradient_host(jfem_nn* nn, const int* shape, const float* in, const float* gout, float* grad)
+{
+    API_BEGIN
+    JFEM_CHECK(nn && shape && in && gout && grad, JFEM_ERR_ARG, "null argument");
+    Lat L = make_lat(nn->dim, shape);
+    nn_reserve(nn, L.np);
+    JFEM_HIP(hipMemcpy(nn->in, in, sizeof(float) * nn->cin * L.np, hipMemcpyHostToDevice));
+    nn_forward(nn, L);
+    JFEM_HIP(hipMemcpyAsync(nn->gout, gout, sizeof(float) * L.np, hipMemcpyHostToDevice, nn->ctx->stream));
+    nn_backward(nn, L, 0);
+    JFEM_HIP(hipMemcpyAsync(grad, nn->grad, sizeof(float) * nn->nparams, hipMemcpyDeviceToHost, nn->ctx->stream));
+    JFEM_HIP(hipStreamSynchronize(nn->ctx->stream));
+    API_END
+}
+
+int jfem_nn_adam_step(jfem_nn* nn, const float* grad, float lr)
+{
+    API_BEGIN
+    JFEM_CHECK(nn && lr > 0, JFEM_ERR_ARG, "bad argument");
+    if (grad) JFEM_HIP(hipMemcpy(nn->grad, grad, sizeof(float) * nn->nparams, hipMemcpyHostToDevice));
+    nn_adam_step(nn, lr, 1.0f);
+    JFEM_HIP(hipStreamSynchronize(nn->ctx->stream));
+    API_END
+}
+
+// checkpoint: magic, dim, cin, step, nparams, then weights, Adam m, Adam v (FP32)
+int jfem_nn_save(jfem_nn* nn, const char* path)
+{
+    API_BEGIN
+    JFEM_CHECK(nn && path, JFEM_ERR_ARG, "null argument");
+    std::vector<float> w(nn->nparams), m(nn->nparams), v(nn->nparams);
+    JFEM_HIP(hipMemcpy(w.data(), nn->w, sizeof(float) * nn->nparams, hipMemcpyDeviceToHost));
+    JFEM_HIP(hipMemcpy(m.data(), nn->m, sizeof(float) * nn->nparams, hipMemcpyDeviceToHost));
+    JFEM_HIP(hipMemcpy(v.data(), nn->v, sizeof(float) * nn->nparams, hipMemcpyDeviceToHost));
+    std::ofstream f(path, std::ios::binary);
+    JFEM_CHECK(f.good(), JFEM_ERR_ARG, "cannot open checkpoint for writing");
+    const long hdr[4] = {nn->dim, nn->cin, nn->step, nn->nparams};
+    f.write(NN_MAGIC, 8);
+    f.write(reinterpret_cast<const char*>(hdr), sizeof(hdr));
+    f.write(reinterpret_cast<const char*>(w.data()), sizeof(float) * nn->nparams);
+    f.write(reinterpret_cast<const char*>(m.data()), sizeof(float) * nn->nparams);
+    f.write(reinterpret_cast<const char*>(v.data()), sizeof(float) * nn->nparams);
+    JFEM_CHECK(f.good(), JFEM_ERR_STATE, "checkpoint write failed");
+    API_END
+}
+
+int jfem_nn_load(jfem_ctx* ctx, const char* path, jfem_nn** out)
+{
+    API_BEGIN
+    JFEM_CHECK(ctx && path && out, JFEM_ERR_ARG, "null argument");
+    std::ifstream f(path, std::ios::binary);
+    JFEM_CHECK(f.good(), JFEM_ERR_ARG, "cannot open checkpoint");
+    char magic[8];
+    long hdr[4];
+    f.read(magic, 8);
+    f.read(reinterpret_cast<char*>(hdr), sizeof(hdr));
+    JFEM_CHECK(f.good() && std::memcmp(magic, NN_MAGIC, 8) == 0, JFEM_ERR_ARG, "not a network checkpoint");
+    jfem_nn* nn = nn_create(ctx, (int)hdr[0], (int)hdr[1], 0);
+    if (nn->nparams != hdr[3]) { nn_free(nn); throw Error{JFEM_ERR_ARG, "checkpoint parameter count mismatch"}; }
+    std::vector<float> w(nn->nparams), m(nn->nparams), v(nn->nparams);
+    f.read(reinterpret_cast<char*>(w.data()), sizeof(float) * nn->nparams);
+    f.read(reinterpret_cast<char*>(m.data()), sizeof(float) * nn->nparams);
+    f.read(reinterpret_cast<char*>(v.data()), sizeof(float) * nn->nparams);
+    if (!f.good()) { nn_free(nn); throw Error{JFEM_ERR_ARG, "truncated checkpoint"}; }
+    nn->step = (int)hdr[2];
+    JFEM_HIP(hipMemcpy(nn->w, w.data(), sizeof(float) * nn->nparams, hipMemcpyHostToDevice));
+    JFEM_HIP(hipMemcpy(nn->m, m.data(), sizeof(float) * nn->nparams, hipMemcpyHostToDevice));
+    JFEM_HIP(hipMemcpy(nn->v, v.data(), sizeof(float) * nn->nparams, hipMemcpyHostToDevice));
+    nn_refresh(nn);
+    JFEM_HIP(hipStreamSynchronize(ctx->stream));
+    *out = nn;
+    API_END
+}
+
+}  // extern "C"

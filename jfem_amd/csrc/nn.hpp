@@ -1,0 +1,45 @@
+// Learned-smoother network: handle layout and the device entry points shared
+// by the network kernels (nn.hip) and the multigrid smoother (mg.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace jfem {
+
+constexpr int NN_H = 32;   // hidden channels
+
+// structured lattice of pixels (nodes), axis 0 fastest in memory
+struct Lat {
+    int d;
+    int n[3];
+    long np;
+};
+
+Lat make_lat(int d, const int* n);
+
+}  // namespace jfem
+
+struct jfem_nn {
+    jfem_ctx* ctx = nullptr;
+    int dim = 2, cin = 2, taps = 9;
+    long nparams = 0;
+    int step = 0;                       // Adam step counter
+    float *w = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;   // FP32 master weights, gradient, Adam moments
+    unsigned short *wh1 = nullptr, *wt1 = nullptr, *wh2 = nullptr, *wt2 = nullptr;   // bf16 copies of the 32->32 layers
+    // work buffers for np <= cap pixels
+    long cap = 0;
+    unsigned short *act0 = nullptr, *act1 = nullptr, *act2 = nullptr;   // hidden activations (post-ReLU), NHWC bf16
+    unsigned short *g0 = nullptr, *g1 = nullptr, *g2 = nullptr;         // gradients w.r.t. hidden pre-activations
+    float* in = nullptr;     // np x cin input (lattice order)
+    float* out = nullptr;    // np output
+    float* gout = nullptr;   // dLoss/dout
+    float* part = nullptr;   // per-block weight-gradient partials
+    long part_cap = 0;
+};
+
+namespace jfem {
+void nn_reserve(jfem_nn* nn, long np);
+void nn_forward(jfem_nn* nn, const Lat& L);
+void nn_backward(jfem_nn* nn, const Lat& L, int add);
+void nn_adam_step(jfem_nn* nn, float lr, float grad_scale);
+}  // namespace jfem
