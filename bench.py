@@ -120,6 +120,99 @@ def spmv_only(jf, cells):
                       "copy_gbs": gbs, "copy_ms": cms, "n": n, "nnz": nnz}), flush=True)
 
 
+def c4_hierarchy(jf, field, precision, cells=1024, base=16):
+    nlev = 1
+    c = cells
+    while c > base:
+        c //= 2
+        nlev += 1
+    spaces = []
+    for l in range(nlev):
+        sp = jf.ContinuousTPFESpace([0.0, 0.0], [1.0, 1.0], [base << l, base << l])
+        sp.assembleCells()
+        sp.assembleFunctions(1)
+        spaces.append(sp)
+    G = jf.MGPreconditioner(spaces, [jf.TPCellIntegral("GradGrad")], jf.ChebyshevSmoother(4, 1), cycles=1, fmg=False,
+                            precision=precision, cell_coeff=np.exp(field))
+    return G, spaces
+
+
+def bench_c4(jf, args):
+    """2D -div(a grad u) = f, Q1 on 1024^2 cells (1,050,625 DOFs), a = exp(g) with g a
+    squared-exponential Gaussian random field (length 0.1, contrast 1e3).  The
+    learned smoother (4 conv layers, 32 channels, BF16 MFMA) is trained on
+    residual/error pairs of plain V-cycles on training fields (seeds 4000+), then
+    evaluated on held-out fields (seeds 5000+): time to ||r|| <= 1e-8 ||b|| of
+    (a) FP64 MG-PCG with the Chebyshev-Jacobi V-cycle and (b) FP64 flexible CG
+    around the FP32 V-cycle with the learned smoother (Chebyshev fallback)."""
+    from jfem_amd.workloads import gaussian_random_field, sinusoid_terms
+    cells = 1024
+    net = jf.SmootherNet(2, 2, seed=7)
+    t0 = time.perf_counter()
+    losses = []
+    for f in range(args.train_fields):
+        field = gaussian_random_field(4000 + f, (cells, cells), 0.1, 1e3)
+        G, spaces = c4_hierarchy(jf, field, jf.FP64)
+        G.set_learned(net, field, runs=2, fallback_ratio=0.5)
+        h = G.train(rhs_per_field=max(1, args.pairs // 8), cycles_per_rhs=8, lr=3e-4, seed=f + 1, mode=3)
+        losses.append(float(np.mean(h[-8:])))
+        log(f"train field {f}: loss {np.mean(h[:8]):.3f} -> {np.mean(h[-8:]):.3f}")
+        del G
+    t_train = time.perf_counter() - t0
+    res = {"pcg64": [], "fcg32_learned": [], "vcyc_cheb": [], "vcyc_learned": [], "err": [], "fallbacks": 0}
+    kw, amp = sinusoid_terms(4, 8, 2)
+    for f in range(args.eval_fields):
+        field = gaussian_random_field(5000 + f, (cells, cells), 0.1, 1e3)
+        G64, spaces = c4_hierarchy(jf, field, jf.FP64)
+        fine = spaces[-1]
+        fine.initializeRhs()
+        fine.writeCellIntegralsToRhs([jf.TPRightHandSideIntegral(jf.SinusoidSource(kw, amp))], fine.getRhs())
+        r = fine.getRhs().numpy()
+        r[fine.boundary_nodes()] = 0
+        b = jf.DenseVector(r)
+        tol = 1e-8 * np.linalg.norm(r)
+        A = G64.getFinestSystem()
+        it = jf.IterativeSolver()
+        x64 = it.solvePCG(A, G64, b, tol, x0=jf.DenseVector(len(r)))
+        res["pcg64"].append((it.info.seconds, it.iterations))
+        info, _ = G64.solve(b, jf.DenseVector(len(r)), tol, 200)
+        res["vcyc_cheb"].append((info.seconds, info.iterations))
+        G64.set_learned(net, field, runs=2, fallback_ratio=0.5)
+        info, _ = G64.solve(b, jf.DenseVector(len(r)), tol, 200)
+        res["vcyc_learned"].append((info.seconds, info.iterations))
+        res["fallbacks"] += int(G64.learned_status()[1])
+        del G64
+        G32, _ = c4_hierarchy(jf, field, jf.FP32)
+        G32.set_learned(net, field, runs=2, fallback_ratio=0.5)
+        it2 = jf.IterativeSolver()
+        x32 = it2.solveFCG(G32.getFinestSystem(), G32, b, tol)
+        res["fcg32_learned"].append((it2.info.seconds, it2.iterations))
+        res["fallbacks"] += int(G32.learned_status()[1])
+        res["err"].append(float(np.linalg.norm(x32.numpy() - x64.numpy()) / np.linalg.norm(x64.numpy())))
+        log(f"eval field {f}: pcg64 {res['pcg64'][-1]}, fcg32+learned {res['fcg32_learned'][-1]}, "
+            f"V cheb {res['vcyc_cheb'][-1]}, V learned {res['vcyc_learned'][-1]}")
+        del G32
+    med = lambda k, i: float(np.median([v[i] for v in res[k]]))
+    value = med("fcg32_learned", 0)
+    line = {
+        "metric": "time-to-1e-8 residual (s): learned+FP32 MG-FCG vs FP64 MG-PCG, 2D Q1 1024^2 log-normal coefficient",
+        "value": value, "unit": "s", "n_gpus": 1, "steps": args.eval_fields, "warmup": 0,
+        "ms_per_step": value * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f64 outer / f32 multigrid / bf16 network", "data": "synthetic",
+        "config": {"workload": "2D -div(a grad u)=f, Q1 1024^2 cells, a=exp(GRF) contrast 1e3",
+                   "train_fields": args.train_fields, "pairs_per_field": max(1, args.pairs // 8) * 8,
+                   "eval_fields": args.eval_fields},
+        "fp64_mgpcg_s": med("pcg64", 0), "fp64_mgpcg_iterations": med("pcg64", 1),
+        "learned_fcg32_iterations": med("fcg32_learned", 1),
+        "speedup_vs_fp64_mgpcg": med("pcg64", 0) / value,
+        "vcycles_chebyshev": med("vcyc_cheb", 1), "vcycles_learned": med("vcyc_learned", 1),
+        "vcycle_solve_s_chebyshev": med("vcyc_cheb", 0), "vcycle_solve_s_learned": med("vcyc_learned", 0),
+        "fallbacks": res["fallbacks"], "max_rel_l2_vs_fp64": float(np.max(res["err"])),
+        "train_s": t_train, "final_train_loss": losses[-1] if losses else None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +221,12 @@ def main():
     ap.add_argument("--cells", type=int, default=96, help="cells per axis of the finest grid (3 * 2^m)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c4"],
+                    help="c2: 3D Q2 96^3 FP64 MG-PCG (default); c4: 2D Q1 1024^2 log-normal coefficient, "
+                         "learned smoother vs Chebyshev")
+    ap.add_argument("--train-fields", type=int, default=16)
+    ap.add_argument("--pairs", type=int, default=64, help="(residual, error) pairs per training field")
+    ap.add_argument("--eval-fields", type=int, default=8)
     ap.add_argument("--spmv-only", action="store_true",
                     help="assemble the finest matrix and time only the SpMV and the copy calibration (for PMC runs)")
     args = ap.parse_args()
@@ -140,6 +239,9 @@ def main():
 
     if args.spmv_only:
         spmv_only(jf, args.cells)
+        return
+    if args.workload == "c4":
+        bench_c4(jf, args)
         return
 
     nlev = 1

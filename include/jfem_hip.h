@@ -198,6 +198,37 @@ int jfem_nn_forward_host(jfem_nn *nn, const int *shape, const float *in, float *
 int jfem_nn_gradient_host(jfem_nn *nn, const int *shape, const float *in, const float *gout, float *grad);
 /* one Adam step (grad NULL: the gradient left by the last backward pass) */
 int jfem_nn_adam_step(jfem_nn *nn, const float *grad, float lr);
+/* learned smoother on the finest multigrid level: with r = b - A x and
+ * u = N(r / rms(r)), the step x += rms(r) u, or with min_residual_step the
+ * minimal-residual step x += alpha u, alpha = (r.Au)/(Au.Au); `runs` steps per
+ * smoothing call.  Input channels: the
+ * normalised residual r^, then r^ times the mean log-coefficient around each
+ * node (cell_log_coeff of the finest grid, nullable = 0) and, for cin = 2 + dim,
+ * r^ times the lattice parity per axis.  Attaching a network sets its biases
+ * to zero and keeps them there, so N(0) = 0 and N(c r) = c N(r) for c > 0.  The hierarchy's Chebyshev smoother is the
+ * fallback, taken once a cycle (or a flexible-CG iteration) reduces the
+ * residual by less than fallback_ratio.  nn NULL detaches. */
+int jfem_mg_set_learned(jfem_mg *mg, jfem_nn *nn, const double *cell_log_coeff, int runs, double fallback_ratio,
+                        int min_residual_step);
+int jfem_mg_set_learned_active(jfem_mg *mg, int active);
+int jfem_mg_learned_status(jfem_mg *mg, int *active, int *fell_back);
+typedef struct {
+    int rhs_per_field;        /* seeded random right-hand sides */
+    int cycles_per_rhs;       /* (residual, error) samples per right-hand side: one per plain V-cycle */
+    double lr;                /* Adam learning rate */
+    unsigned long long seed;
+    int mode;                 /* samples: 0 = residuals of plain V-cycles, 1 = A e for white-noise
+                                 errors e, 2 = alternate 0 and 1, 3 = alternate the residual at cycle
+                                 entry and the residual before finest post-smoothing */
+} jfem_train_opts;
+/* trains the network attached to mg on (residual, error) pairs of its plain
+ * V-cycles, one Adam step per sample; the loss is the energy of the error left
+ * by the learned step, |e - N(r)|_A^2 - |e|_A^2 per node (input normalised);
+ * loss_hist (nullable) gets it per step */
+int jfem_nn_train_mg(jfem_mg *mg, const jfem_train_opts *opts, double *loss_hist, int *nsteps);
+/* stationary multigrid iteration to ||b - A x|| <= tol (x: start iterate in, solution out) */
+int jfem_mg_solve(jfem_mg *mg, const jfem_vec *b, jfem_vec *x, double tol, int maxcycles, jfem_solve_info *info,
+                  double *res_hist);
 /* versioned checkpoint of weights and optimiser state */
 int jfem_nn_save(jfem_nn *nn, const char *path);
 int jfem_nn_load(jfem_ctx *ctx, const char *path, jfem_nn **out);

@@ -43,6 +43,11 @@ class SolveInfo(C.Structure):
     _fields_ = [("iterations", C.c_int), ("fell_back", C.c_int), ("resnorm", C.c_double), ("seconds", C.c_double)]
 
 
+class TrainOpts(C.Structure):
+    _fields_ = [("rhs_per_field", C.c_int), ("cycles_per_rhs", C.c_int), ("lr", C.c_double), ("seed", C.c_ulonglong),
+                ("mode", C.c_int)]
+
+
 class MgOpts(C.Structure):
     _fields_ = [("smoother", C.c_int), ("runs", C.c_int), ("omega", C.c_double), ("degree", C.c_int),
                 ("cheb_ratio", C.c_double), ("cheb_safety", C.c_double), ("power_its", C.c_int),
@@ -107,6 +112,11 @@ _SIGS = {
     "jfem_nn_adam_step": (C.c_int, [_vp, _fp, C.c_float]),
     "jfem_nn_save": (C.c_int, [_vp, C.c_char_p]),
     "jfem_nn_load": (C.c_int, [_vp, C.c_char_p, C.POINTER(_vp)]),
+    "jfem_mg_set_learned": (C.c_int, [_vp, _vp, _dp, C.c_int, C.c_double, C.c_int]),
+    "jfem_mg_set_learned_active": (C.c_int, [_vp, C.c_int]),
+    "jfem_mg_learned_status": (C.c_int, [_vp, _ip, _ip]),
+    "jfem_nn_train_mg": (C.c_int, [_vp, C.POINTER(TrainOpts), _dp, _ip]),
+    "jfem_mg_solve": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int, C.POINTER(SolveInfo), _dp]),
 }
 
 _lib = None

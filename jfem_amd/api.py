@@ -470,6 +470,39 @@ class MGPreconditioner:
         check(N.lib().jfem_mg_vcycle(self.h, x.h, b.h))
         return x
 
+    def set_learned(self, net, cell_log_coeff=None, runs: int = 1, fallback_ratio: float = 0.5,
+                    min_residual_step: bool = False):
+        """attach a SmootherNet as the finest-level smoother (None detaches)"""
+        cc = None if cell_log_coeff is None else f64(cell_log_coeff)
+        self._net = net
+        check(N.lib().jfem_mg_set_learned(self.h, net.h if net is not None else None,
+                                          dptr(cc) if cc is not None else None, runs, fallback_ratio,
+                                          1 if min_residual_step else 0))
+
+    def set_learned_active(self, active: bool):
+        check(N.lib().jfem_mg_set_learned_active(self.h, 1 if active else 0))
+
+    def learned_status(self):
+        a, f = C.c_int(), C.c_int()
+        check(N.lib().jfem_mg_learned_status(self.h, C.byref(a), C.byref(f)))
+        return bool(a.value), bool(f.value)
+
+    def train(self, rhs_per_field: int = 10, cycles_per_rhs: int = 10, lr: float = 1e-3, seed: int = 0, mode: int = 0):
+        """train the attached network on residuals of this hierarchy's plain V-cycles
+        (mode 0), on A e for white-noise errors e (mode 1), or alternating (mode 2)"""
+        o = N.TrainOpts(rhs_per_field, cycles_per_rhs, lr, seed, mode)
+        hist = np.zeros(rhs_per_field * cycles_per_rhs)
+        ns = C.c_int()
+        check(N.lib().jfem_nn_train_mg(self.h, C.byref(o), dptr(hist), C.byref(ns)))
+        return hist[:ns.value]
+
+    def solve(self, b: DenseVector, x: DenseVector, tol: float, maxcycles: int = 200):
+        """stationary V-cycle iteration to ||b - A x|| <= tol; returns (info, residual history)"""
+        info = N.SolveInfo()
+        hist = np.zeros(maxcycles)
+        check(N.lib().jfem_mg_solve(self.h, b.h, x.h, tol, maxcycles, C.byref(info), dptr(hist)))
+        return info, hist[:info.iterations].copy()
+
     def lmax(self) -> np.ndarray:
         out = np.zeros(self.nlev)
         check(N.lib().jfem_mg_lmax(self.h, dptr(out)))

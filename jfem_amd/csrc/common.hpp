@@ -224,5 +224,7 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
 // multigrid preconditioner application z = M r (FP64 vectors on the context stream)
 void mg_apply(jfem_mg* mg, const double* r, double* z);
 long mg_size(const jfem_mg* mg);
+// learned-smoother fallback rule: called with the residual reduction of a cycle/iteration
+void mg_observe_reduction(jfem_mg* mg, double ratio);
 
 }  // namespace jfem

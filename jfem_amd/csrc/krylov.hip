@@ -217,7 +217,9 @@ static void solve_fcg(jfem_mat* A, jfem_mg* M, const double* b, double* x, doubl
             dots_dev(c, n, 1, xs, ys, sc + 3);   // sc[3] = z.q
         }
         k_xpby_neg<<<kgrid(n), KB, 0, c->stream>>>(n, sc, 3, 1, z.p, p.p);
+        const double rr_old = rr;
         rr = read_scalar(c, sc + 2);
+        mg_observe_reduction(M, rr_old > 0 ? std::sqrt(rr / rr_old) : 0.0);
         if (hist) hist[it] = std::sqrt(rr);
         it++;
         JFEM_CHECK(std::isfinite(rr), JFEM_ERR_STATE, "FCG diverged (non-finite residual)");

@@ -14,8 +14,10 @@
 // matrix.  The coarsest level is solved with its dense inverse (a GEMV).
 // Every level operation is templated on the working precision (FP64 or FP32).
 #include "common.hpp"
+#include "nn.hpp"
 #include "stream.hpp"
 
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -30,6 +32,25 @@ struct MgLevel {
     double lmax = 0;                    // Chebyshev upper bound on D^-1 A
 };
 
+// learned smoother on the finest level (a network applied to the normalised
+// residual on the node lattice, x += N(r / rms(r)) * rms(r))
+struct LearnedState {
+    jfem_nn* nn = nullptr;       // borrowed
+    jfem::Lat lat;               // finest node lattice
+    float* stat = nullptr;       // np x (cin - 1) static input channels
+    int runs = 1;                // learned steps per smoothing call
+    int active = 0;
+    int fell_back = 0;
+    double fallback = 0.5;       // switch to the plain smoother when a cycle reduces less than this
+    double* rr = nullptr;        // device scalar: squared residual norm
+    double *u = nullptr, *z = nullptr, *gd = nullptr, *rh = nullptr, *xs = nullptr, *bs = nullptr;   // training work
+    double* loss = nullptr;      // device loss history
+    long loss_cap = 0;
+    int mr = 0;                  // 1: minimal-residual step along N(r); 0: plain step x += N(r)
+    int capture = 0;             // training: record the residual before finest post-smoothing
+    double* capbuf = nullptr;
+};
+
 struct jfem_mg {
     jfem_ctx* ctx = nullptr;
     std::vector<MgLevel> lev;   // 0 = coarsest
@@ -38,6 +59,7 @@ struct jfem_mg {
     double* init = nullptr;             // FP64 scratch for the reference mvMul
     double* defect = nullptr;
     double* tmp = nullptr;
+    LearnedState ls;
 };
 
 namespace jfem {
@@ -341,6 +363,9 @@ double level_norm<float>(jfem_mg* G, long n, const float* r)
     return std::sqrt(vec_dot(G->ctx, n, tmp, tmp));
 }
 
+template <typename T>
+static void learned_smooth(jfem_mg* G, MgLevel& L, const T* b, T* x, T* u, bool x_zero);
+
 // Smoother.smooth(op, rhs, iterate): x (in/out, in L.x) for rhs b
 template <typename T>
 static void smooth(jfem_mg* G, MgLevel& L, const T* b, T*& x, T*& xalt, bool x_zero)
@@ -349,6 +374,10 @@ static void smooth(jfem_mg* G, MgLevel& L, const T* b, T*& x, T*& xalt, bool x_z
     hipStream_t st = G->ctx->stream;
     const long n = L.A->rows;
     const int g = mg_grid(L, L.A->lanes);
+    if (G->ls.active && &L == &G->lev.back()) {
+        learned_smooth<T>(G, L, b, x, xalt, x_zero);
+        return;
+    }
     if (o.smoother == JFEM_SM_JACOBI) {
         for (int it = 0; it < o.runs; it++) {
             if (x_zero && it == 0) {
@@ -445,6 +474,11 @@ static void step_v(jfem_mg* G, int l, bool x_zero)
     restrict_to<T>(G, l, r, (T*)C.b);
     step_v<T>(G, l - 1, true);
     prolong_to<T>(G, l, (const T*)C.x, x, true, true);   // guess.add(P e) with zero BC
+    if (G->ls.capture && l == (int)G->lev.size() - 1) {
+        // training: keep the residual that post-smoothing will see
+        level_residual<T>(G, L, b, x, r);
+        k_convert<T, double><<<grid_for(L.A->rows, 256, 16384), 256, 0, G->ctx->stream>>>(L.A->rows, r, G->ls.capbuf);
+    }
     smooth<T>(G, L, b, x, xalt, false);
     // keep the result in L.x / the spare in L.t
     if (x != (T*)L.x) { L.t = L.x; L.x = x; }
@@ -573,6 +607,10 @@ static void free_mg(jfem_mg* G)
     }
     (void)hipFree(G->coarse_inv);
     (void)hipFree(G->init); (void)hipFree(G->defect); (void)hipFree(G->tmp);
+    LearnedState& S = G->ls;
+    for (void* p : {(void*)S.stat, (void*)S.rr, (void*)S.u, (void*)S.z, (void*)S.gd, (void*)S.rh, (void*)S.xs,
+                    (void*)S.bs, (void*)S.loss, (void*)S.capbuf})
+        (void)hipFree(p);
     delete G;
 }
 
@@ -641,7 +679,7 @@ static jfem_mg* create_mg(jfem_space* const* spaces, int nlev, int nint, const i
             mat_diag(L.A, (double*)L.t);
             k_dinv_from_diag<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(n, (double*)L.t, L.dinv, L.dinv32);
             JFEM_HIP(hipGetLastError());
-            if (opts->smoother == JFEM_SM_CHEBYSHEV && l > 0)
+            if ((opts->smoother == JFEM_SM_CHEBYSHEV || opts->degree > 0) && l > 0)
                 L.lmax = (opts->cheb_safety > 0 ? opts->cheb_safety : 1.1) *
                          power_lmax(G, L, opts->power_its > 0 ? opts->power_its : 30);
         }
@@ -668,6 +706,410 @@ static jfem_mg* create_mg(jfem_space* const* spaces, int nlev, int nint, const i
         throw;
     }
     return G;
+}
+
+
+// ------------------------------------------------ learned smoother
+template <typename T>
+__global__ __launch_bounds__(256) void k_sumsq(long n, const T* __restrict__ x, double* __restrict__ partials)
+{
+    __shared__ double sh[4];
+    double acc = 0;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) acc += (double)x[i] * (double)x[i];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+template <typename T>
+static void sumsq_dev(jfem_ctx* c, long n, const T* x, double* dst)
+{
+    const int blocks = grid_for(n, 256, RED_BLOCKS);
+    k_sumsq<T><<<blocks, 256, 0, c->stream>>>(n, x, c->red_partials);
+    JFEM_HIP(hipGetLastError());
+    reduce_partials_dev(c, 1, blocks, dst);
+}
+
+__device__ inline long pix_of(const GridDev& g, const Lat& L, long i)
+{
+    int p[MAXD] = {0, 0, 0};
+    lattice_of_dof(g, i, p);
+    return p[0] + (long)L.n[0] * (p[1] + (long)L.n[1] * p[2]);
+}
+
+// network input: channel 0 = r * sqrt(n / rr), channel c >= 1 = channel 0 times
+// static channel c (so that the bias-free network is positively homogeneous in r)
+template <typename T>
+__global__ void k_gather_in(GridDev g, Lat L, long n, const T* __restrict__ r, const double* __restrict__ rr,
+                            const float* __restrict__ stat, int cin, float* __restrict__ in)
+{
+    const double scale = rr[0] > 0 ? sqrt((double)n / rr[0]) : 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const long pix = pix_of(g, L, i);
+        const float v = (float)((double)r[i] * scale);
+        in[pix * cin] = v;
+        for (int c = 1; c < cin; c++) in[pix * cin + c] = v * stat[pix * (cin - 1) + c - 1];
+    }
+}
+
+// x[i] += out[pix] * rms(r) on free nodes
+template <typename T>
+__global__ void k_scatter_add(GridDev g, Lat L, long n, const float* __restrict__ out, const double* __restrict__ rr,
+                              const unsigned char* __restrict__ bnd, T* __restrict__ x)
+{
+    const double inv = rr[0] > 0 ? sqrt(rr[0] / (double)n) : 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        if (bnd[i]) continue;
+        x[i] = (T)((double)x[i] + (double)out[pix_of(g, L, i)] * inv);
+    }
+}
+
+// training helpers: u = masked network output (DOF order), rh = r * sqrt(n / rr)
+__global__ void k_train_unpack(GridDev g, Lat L, long n, const float* __restrict__ out, const double* __restrict__ r,
+                               const double* __restrict__ rr, const unsigned char* __restrict__ bnd,
+                               double* __restrict__ u, double* __restrict__ rh)
+{
+    const double scale = rr[0] > 0 ? sqrt((double)n / rr[0]) : 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        u[i] = bnd[i] ? 0.0 : (double)out[pix_of(g, L, i)];
+        rh[i] = r[i] * scale;
+    }
+}
+
+// dLoss/dout for the energy loss (u.A u - 2 u.rh) / n: gout = -2 z / n on free nodes, z = rh - A u
+__global__ void k_train_gout(GridDev g, Lat L, long n, const double* __restrict__ z,
+                             const unsigned char* __restrict__ bnd, float* __restrict__ gout)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        gout[pix_of(g, L, i)] = bnd[i] ? 0.f : (float)(-2.0 * z[i] / (double)n);
+}
+
+// partials of -u.(z + rh)
+__global__ __launch_bounds__(256) void k_energy(long n, const double* __restrict__ u, const double* __restrict__ z,
+                                                const double* __restrict__ rh, double* __restrict__ partials)
+{
+    __shared__ double sh[4];
+    double acc = 0;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) acc -= u[i] * (z[i] + rh[i]);
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// seeded standard-normal right-hand side (SplitMix64 + Box-Muller), zero on the boundary
+__global__ void k_rand_rhs(long n, unsigned long long seed, const unsigned char* __restrict__ bnd, double* __restrict__ b)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        auto mix = [](unsigned long long z) {
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            return z ^ (z >> 31);
+        };
+        const unsigned long long z1 = mix(seed + (2 * (unsigned long long)i + 1) * 0x9E3779B97F4A7C15ull);
+        const unsigned long long z2 = mix(seed + (2 * (unsigned long long)i + 2) * 0x9E3779B97F4A7C15ull);
+        const double u1 = ((z1 >> 11) + 1.0) * (1.0 / 9007199254740993.0);
+        const double u2 = (z2 >> 11) * (1.0 / 9007199254740992.0);
+        b[i] = bnd[i] ? 0.0 : sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+    }
+}
+
+// u = network output on free nodes (DOF order)
+template <typename T>
+__global__ void k_unpack_out(GridDev g, Lat L, long n, const float* __restrict__ out, const unsigned char* __restrict__ bnd,
+                             T* __restrict__ u)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        u[i] = bnd[i] ? (T)0 : (T)out[pix_of(g, L, i)];
+}
+
+// partials of (r.w) and (w.w) into slots 0 and 1
+template <typename T>
+__global__ __launch_bounds__(256) void k_dot2(long n, const T* __restrict__ r, const T* __restrict__ w,
+                                              double* __restrict__ partials)
+{
+    __shared__ double sh[2][4];
+    double a = 0, c = 0;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        a += (double)r[i] * (double)w[i];
+        c += (double)w[i] * (double)w[i];
+    }
+    for (int o = 32; o > 0; o >>= 1) { a += __shfl_xor(a, o, 64); c += __shfl_xor(c, o, 64); }
+    if ((threadIdx.x & 63) == 0) { sh[0][threadIdx.x >> 6] = a; sh[1][threadIdx.x >> 6] = c; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
+        partials[RED_BLOCKS + blockIdx.x] = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+    }
+}
+
+// x += alpha u with alpha = (r.Au)/(Au.Au): the step never increases ||r||_2
+template <typename T>
+__global__ void k_mr_update(long n, const double* __restrict__ d, const T* __restrict__ u, T* __restrict__ x)
+{
+    const double alpha = d[1] > 0 ? d[0] / d[1] : 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        x[i] = (T)((double)x[i] + alpha * (double)u[i]);
+}
+
+// learned smoothing step: r = b - A x, u = N(r / rms r) (direction), then a
+// minimal-residual step along u
+template <typename T>
+static void learned_smooth(jfem_mg* G, MgLevel& L, const T* b, T* x, T* u, bool x_zero)
+{
+    LearnedState& S = G->ls;
+    jfem_ctx* c = G->ctx;
+    hipStream_t st = c->stream;
+    const long n = L.A->rows;
+    const int gr = grid_for(n, 256, 16384);
+    T* r = (T*)L.r;
+    T* w = (T*)L.dv;
+    for (int it = 0; it < S.runs; it++) {
+        if (x_zero && it == 0) {
+            k_fill_t<T><<<gr, 256, 0, st>>>(n, (T)0, x);
+            JFEM_HIP(hipMemcpyAsync(r, b, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
+        } else {
+            level_residual<T>(G, L, b, x, r);
+        }
+        sumsq_dev<T>(c, n, r, S.rr);
+        k_gather_in<T><<<gr, 256, 0, st>>>(to_dev(L.grid), S.lat, n, r, S.rr, S.stat, S.nn->cin, S.nn->in);
+        nn_forward(S.nn, S.lat);
+        if (!S.mr) {
+            k_scatter_add<T><<<gr, 256, 0, st>>>(to_dev(L.grid), S.lat, n, S.nn->out, S.rr, L.bnd, x);
+            JFEM_HIP(hipGetLastError());
+            continue;
+        }
+        k_unpack_out<T><<<gr, 256, 0, st>>>(to_dev(L.grid), S.lat, n, S.nn->out, L.bnd, u);
+        launch_stream<T, T>(L.A, vals_of<T>(L), u, EpiStore<T>{w}, nullptr);
+        const int blocks = grid_for(n, 256, RED_BLOCKS);
+        k_dot2<T><<<blocks, 256, 0, st>>>(n, r, w, c->red_partials);
+        reduce_partials_dev(c, 2, blocks, S.rr + 2);
+        k_mr_update<T><<<gr, 256, 0, st>>>(n, S.rr + 2, u, x);
+        JFEM_HIP(hipGetLastError());
+    }
+}
+
+// one V-cycle x <- mgStepV(finest, x, b) on FP64 device vectors, any precision inside
+static void vcycle_dev(jfem_mg* G, double* x, const double* b)
+{
+    const int Lf = (int)G->lev.size() - 1;
+    MgLevel& F = G->lev[Lf];
+    const long n = F.A->rows;
+    hipStream_t st = G->ctx->stream;
+    const int g = grid_for(n, 256, 16384);
+    if (G->opt.precision == JFEM_FP32) {
+        k_convert<double, float><<<g, 256, 0, st>>>(n, b, (float*)F.b);
+        k_convert<double, float><<<g, 256, 0, st>>>(n, x, (float*)F.x);
+        step_v<float>(G, Lf, false);
+        k_convert<float, double><<<g, 256, 0, st>>>(n, (const float*)F.x, x);
+    } else {
+        vec_copy(G->ctx, (double*)F.b, b, n);
+        vec_copy(G->ctx, (double*)F.x, x, n);
+        step_v<double>(G, Lf, false);
+        vec_copy(G->ctx, x, (const double*)F.x, n);
+    }
+    JFEM_HIP(hipGetLastError());
+}
+
+void mg_observe_reduction(jfem_mg* G, double ratio)
+{
+    if (G->ls.active && ratio > G->ls.fallback) {
+        G->ls.active = 0;
+        G->ls.fell_back = 1;
+    }
+}
+
+static void set_learned(jfem_mg* G, jfem_nn* nn, const double* cell_logc, int runs, double fallback, int mr)
+{
+    LearnedState& S = G->ls;
+    if (!nn) {
+        S.nn = nullptr;
+        S.active = 0;
+        return;
+    }
+    MgLevel& F = G->lev.back();
+    const Grid& g = F.grid;
+    JFEM_CHECK(nn->dim == g.d, JFEM_ERR_ARG, "network dimension differs from the grid");
+    JFEM_CHECK(nn->cin == 2 || nn->cin == 2 + g.d, JFEM_ERR_ARG, "network needs 2 or 2+dim input channels");
+    JFEM_CHECK(F.lmax > 0, JFEM_ERR_STATE, "hierarchy has no Chebyshev fallback (create it with a Chebyshev smoother)");
+    int dims[3] = {1, 1, 1};
+    for (int a = 0; a < g.d; a++) dims[a] = (int)g.nodes_axis(a);
+    S.lat = make_lat(g.d, dims);
+    const long np = S.lat.np;
+    JFEM_CHECK(np == F.A->rows, JFEM_ERR_STATE, "lattice and DOF counts differ");
+    const int ns = nn->cin - 1;
+    std::vector<float> stat((size_t)np * ns, 0.f);
+    // channel 1: mean log-coefficient of the cells around each node
+    if (cell_logc) {
+        for (long pix = 0; pix < np; pix++) {
+            int p[3] = {(int)(pix % dims[0]), (int)((pix / dims[0]) % dims[1]), (int)(pix / ((long)dims[0] * dims[1]))};
+            int lo[3], hi[3];
+            for (int a = 0; a < 3; a++) {
+                if (a >= g.d) { lo[a] = hi[a] = 0; continue; }
+                if (p[a] % g.k == 0) { lo[a] = p[a] / g.k - 1; hi[a] = p[a] / g.k; } else { lo[a] = hi[a] = p[a] / g.k; }
+                lo[a] = std::max(lo[a], 0);
+                hi[a] = std::min(hi[a], g.n[a] - 1);
+            }
+            double s = 0;
+            int cnt = 0;
+            for (int c2 = lo[2]; c2 <= hi[2]; c2++)
+                for (int c1 = lo[1]; c1 <= hi[1]; c1++)
+                    for (int c0 = lo[0]; c0 <= hi[0]; c0++) {
+                        const int cc[3] = {c0, c1, c2};
+                        long cell = 0;
+                        for (int a = 0; a < g.d; a++) cell = cell * g.n[a] + cc[a];
+                        s += cell_logc[cell];
+                        cnt++;
+                    }
+            stat[(size_t)pix * ns] = (float)(s / cnt);
+        }
+    }
+    // channels 2..: lattice parity per axis (node position inside its cell for Q2)
+    if (nn->cin == 2 + g.d)
+        for (long pix = 0; pix < np; pix++) {
+            long r = pix;
+            for (int a = 0; a < g.d; a++) {
+                stat[(size_t)pix * ns + 1 + a] = (float)((r % dims[a]) % 2);
+                r /= dims[a];
+            }
+        }
+    (void)hipFree(S.stat);
+    S.stat = nullptr;
+    JFEM_HIP(hipMalloc(&S.stat, sizeof(float) * stat.size()));
+    JFEM_HIP(hipMemcpy(S.stat, stat.data(), sizeof(float) * stat.size(), hipMemcpyHostToDevice));
+    if (!S.rr) JFEM_HIP(hipMalloc(&S.rr, sizeof(double) * 8));
+    nn_reserve(nn, np);
+    nn_set_bias_free(nn, 1);   // N(0) = 0 and N(c r) = c N(r): a consistent smoother
+    S.nn = nn;
+    S.runs = runs > 0 ? runs : 1;
+    S.mr = mr ? 1 : 0;
+    S.fallback = fallback > 0 ? fallback : 0.5;
+    S.active = 1;
+    S.fell_back = 0;
+}
+
+// Train the attached network on residuals of plain V-cycles of this hierarchy:
+// for each seeded random right-hand side b (white noise, or for odd b smooth:
+// noise three levels down, prolongated), x = 0, and for each cycle k the
+// sample is r_k = b - A x_k with error e_k = A^-1 r_k; with the input
+// normalised to rms 1 the loss is the energy of the error left by the step,
+// |e_k - N(r_k)|_A^2 - |e_k|_A^2 = u.A u - 2 u.r (per node), which needs only
+// r_k; then x_{k+1} = V(x_k) with the plain smoother.
+static int train_mg(jfem_mg* G, const jfem_train_opts* o, double* loss_host)
+{
+    LearnedState& S = G->ls;
+    JFEM_CHECK(S.nn, JFEM_ERR_STATE, "no network attached (jfem_mg_set_learned)");
+    jfem_ctx* c = G->ctx;
+    hipStream_t st = c->stream;
+    MgLevel& F = G->lev.back();
+    const long n = F.A->rows;
+    const int gr = grid_for(n, 256, 16384);
+    const int nsteps = o->rhs_per_field * o->cycles_per_rhs;
+    for (double** p : {&S.u, &S.z, &S.gd, &S.rh, &S.xs, &S.bs, &S.capbuf}) {
+        (void)hipFree(*p);
+        *p = nullptr;
+        JFEM_HIP(hipMalloc(p, sizeof(double) * n));
+    }
+    if (nsteps > S.loss_cap) {
+        (void)hipFree(S.loss);
+        JFEM_HIP(hipMalloc(&S.loss, sizeof(double) * nsteps));
+        S.loss_cap = nsteps;
+    }
+    const int was_active = S.active;
+    S.active = 0;   // data come from plain V-cycles
+    double* r = G->tmp;
+    int step = 0;
+    const int Lf = (int)G->lev.size() - 1;
+    for (int k = 0; k < o->rhs_per_field; k++) {
+        const unsigned long long sk = o->seed * 0x100000001B3ull + (unsigned long long)k * 7919ull;
+        if ((k & 1) && Lf >= 1) {
+            // smooth right-hand side: white noise three levels down, prolongated to the finest grid
+            const int ls = Lf >= 3 ? Lf - 3 : 0;
+            MgLevel& C = G->lev[ls];
+            k_rand_rhs<<<grid_for(C.A->rows, 256, 16384), 256, 0, st>>>(C.A->rows, sk, C.bnd, (double*)C.t);
+            for (int l = ls + 1; l <= Lf; l++)
+                prolong_to<double>(G, l, (const double*)G->lev[l - 1].t, (double*)G->lev[l].t, false, true);
+            vec_copy(c, S.bs, (const double*)F.t, n);
+        } else {
+            k_rand_rhs<<<gr, 256, 0, st>>>(n, sk, F.bnd, S.bs);
+        }
+        vec_fill(c, S.xs, 0.0, n);
+        for (int cyc = 0; cyc < o->cycles_per_rhs; cyc++) {
+            const bool noise = o->mode == 1 || (o->mode == 2 && (step & 1));
+            const bool captured = o->mode == 3 && (step & 1);
+            if (captured) {
+                vec_copy(c, r, S.capbuf, n);
+            } else if (noise) {
+                // r = A e for a white-noise error e (zero on the boundary)
+                k_rand_rhs<<<gr, 256, 0, st>>>(n, o->seed * 0x9E3779B97F4A7C15ull + 0x51ull * (step + 1), F.bnd, S.gd);
+                spmv(F.A, S.gd, r);
+            } else {
+                launch_stream<double, double>(F.A, F.A->vals, S.xs, EpiResidual<double>{S.bs, r}, nullptr);
+            }
+            sumsq_dev<double>(c, n, r, S.rr);
+            k_gather_in<double><<<gr, 256, 0, st>>>(to_dev(F.grid), S.lat, n, r, S.rr, S.stat, S.nn->cin, S.nn->in);
+            nn_forward(S.nn, S.lat);
+            k_train_unpack<<<gr, 256, 0, st>>>(to_dev(F.grid), S.lat, n, S.nn->out, r, S.rr, F.bnd, S.u, S.rh);
+            launch_stream<double, double>(F.A, F.A->vals, S.u, EpiResidual<double>{S.rh, S.z}, nullptr);   // z = rh - A u
+            {
+                // energy loss |e - u|_A^2 - |e|_A^2 = u.A u - 2 u.rh = -u.(z + rh), gradient 2 (A u - rh) = -2 z
+                const int blocks = grid_for(n, 256, RED_BLOCKS);
+                k_energy<<<blocks, 256, 0, st>>>(n, S.u, S.z, S.rh, c->red_partials);
+                reduce_partials_dev(c, 1, blocks, S.loss + step);
+            }
+            k_train_gout<<<gr, 256, 0, st>>>(to_dev(F.grid), S.lat, n, S.z, F.bnd, S.nn->gout);
+            JFEM_HIP(hipGetLastError());
+            nn_backward(S.nn, S.lat, 0);
+            nn_adam_step(S.nn, (float)o->lr, 1.0f);
+            step++;
+            if (o->mode == 3) {
+                // even steps train on the cycle-entry residual; the V-cycle that
+                // follows records the residual before post-smoothing for the odd step
+                if (!captured) {
+                    S.capture = 1;
+                    vcycle_dev(G, S.xs, S.bs);
+                    S.capture = 0;
+                }
+            } else if (!noise && cyc + 1 < o->cycles_per_rhs) {
+                vcycle_dev(G, S.xs, S.bs);
+            }
+        }
+    }
+    std::vector<double> h(step);
+    JFEM_HIP(hipMemcpyAsync(h.data(), S.loss, sizeof(double) * step, hipMemcpyDeviceToHost, st));
+    JFEM_HIP(hipStreamSynchronize(st));
+    if (loss_host)
+        for (int i = 0; i < step; i++) loss_host[i] = h[i] / (double)n;
+    S.active = was_active;
+    return step;
+}
+
+// stationary multigrid iteration x <- V(x) until ||b - A x|| <= tol (FP64
+// residual on the finest matrix), with the learned smoother's fallback rule
+static void mg_solve(jfem_mg* G, const double* b, double* x, double tol, int maxcycles, jfem_solve_info* info,
+                     double* hist)
+{
+    MgLevel& F = G->lev.back();
+    jfem_ctx* c = G->ctx;
+    const long n = F.A->rows;
+    double* r = G->tmp;
+    launch_stream<double, double>(F.A, F.A->vals, x, EpiResidual<double>{b, r}, nullptr);
+    double rn = std::sqrt(vec_dot(c, n, r, r));
+    int it = 0;
+    while (rn > tol && it < maxcycles) {
+        vcycle_dev(G, x, b);
+        launch_stream<double, double>(F.A, F.A->vals, x, EpiResidual<double>{b, r}, nullptr);
+        const double rnew = std::sqrt(vec_dot(c, n, r, r));
+        JFEM_CHECK(std::isfinite(rnew), JFEM_ERR_STATE, "multigrid iteration diverged");
+        mg_observe_reduction(G, rn > 0 ? rnew / rn : 0.0);
+        rn = rnew;
+        if (hist) hist[it] = rn;
+        it++;
+    }
+    info->iterations = it;
+    info->resnorm = rn;
+    info->fell_back = G->ls.fell_back;
 }
 
 }  // namespace jfem
@@ -734,6 +1176,61 @@ int jfem_mg_vcycle(jfem_mg* mg, jfem_vec* x, const jfem_vec* b)
         vec_copy(mg->ctx, x->d, (const double*)F.x, n);
     }
     JFEM_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
+int jfem_mg_set_learned(jfem_mg* mg, jfem_nn* nn, const double* cell_log_coeff, int runs, double fallback_ratio,
+                        int min_residual_step)
+{
+    API_BEGIN
+    JFEM_CHECK(mg, JFEM_ERR_ARG, "null argument");
+    set_learned(mg, nn, cell_log_coeff, runs, fallback_ratio, min_residual_step);
+    JFEM_HIP(hipStreamSynchronize(mg->ctx->stream));
+    API_END
+}
+
+int jfem_mg_learned_status(jfem_mg* mg, int* active, int* fell_back)
+{
+    API_BEGIN
+    JFEM_CHECK(mg, JFEM_ERR_ARG, "null argument");
+    if (active) *active = mg->ls.active;
+    if (fell_back) *fell_back = mg->ls.fell_back;
+    API_END
+}
+
+int jfem_mg_set_learned_active(jfem_mg* mg, int active)
+{
+    API_BEGIN
+    JFEM_CHECK(mg, JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(!active || mg->ls.nn, JFEM_ERR_STATE, "no network attached");
+    mg->ls.active = active ? 1 : 0;
+    mg->ls.fell_back = 0;
+    API_END
+}
+
+int jfem_nn_train_mg(jfem_mg* mg, const jfem_train_opts* opts, double* loss_hist, int* nsteps)
+{
+    API_BEGIN
+    JFEM_CHECK(mg && opts, JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(opts->rhs_per_field >= 1 && opts->cycles_per_rhs >= 1 && opts->lr > 0, JFEM_ERR_ARG,
+               "bad training options");
+    const int s = train_mg(mg, opts, loss_hist);
+    if (nsteps) *nsteps = s;
+    API_END
+}
+
+int jfem_mg_solve(jfem_mg* mg, const jfem_vec* b, jfem_vec* x, double tol, int maxcycles, jfem_solve_info* info,
+                  double* res_hist)
+{
+    API_BEGIN
+    JFEM_CHECK(mg && b && x && info, JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(b->n == mg_size(mg) && x->n == mg_size(mg), JFEM_ERR_ARG, "wrong size");
+    std::memset(info, 0, sizeof(*info));
+    JFEM_HIP(hipStreamSynchronize(mg->ctx->stream));
+    auto t0 = std::chrono::steady_clock::now();
+    mg_solve(mg, b->d, x->d, tol, maxcycles, info, res_hist);
+    JFEM_HIP(hipStreamSynchronize(mg->ctx->stream));
+    info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     API_END
 }
 

@@ -425,6 +425,29 @@ void nn_backward(jfem_nn* nn, const Lat& L, int add)
     JFEM_HIP(hipGetLastError());
 }
 
+__global__ void k_zero_range(float* __restrict__ w, long a, long b)
+{
+    for (long i = a + blockIdx.x * (long)blockDim.x + threadIdx.x; i < b; i += (long)gridDim.x * blockDim.x) w[i] = 0.f;
+}
+
+static void zero_biases(jfem_nn* nn)
+{
+    long o[9];
+    nn_offsets(nn, o);
+    hipStream_t st = nn->ctx->stream;
+    k_zero_range<<<1, 64, 0, st>>>(nn->w, o[1], o[2]);
+    k_zero_range<<<1, 64, 0, st>>>(nn->w, o[3], o[4]);
+    k_zero_range<<<1, 64, 0, st>>>(nn->w, o[5], o[6]);
+    k_zero_range<<<1, 64, 0, st>>>(nn->w, o[7], o[8]);
+    JFEM_HIP(hipGetLastError());
+}
+
+void nn_set_bias_free(jfem_nn* nn, int on)
+{
+    nn->bias_free = on ? 1 : 0;
+    if (on) zero_biases(nn);
+}
+
 void nn_adam_step(jfem_nn* nn, float lr, float grad_scale)
 {
     nn->step++;
@@ -433,6 +456,7 @@ void nn_adam_step(jfem_nn* nn, float lr, float grad_scale)
     k_adam<<<grid_for(nn->nparams, 256, 1024), 256, 0, nn->ctx->stream>>>(nn->nparams, nn->w, nn->grad, nn->m, nn->v,
                                                                          lr, b1, b2, 1e-8f, c1, c2, grad_scale);
     JFEM_HIP(hipGetLastError());
+    if (nn->bias_free) zero_biases(nn);
     nn_refresh(nn);
 }
 

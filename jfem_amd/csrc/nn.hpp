@@ -24,6 +24,7 @@ struct jfem_nn {
     int dim = 2, cin = 2, taps = 9;
     long nparams = 0;
     int step = 0;                       // Adam step counter
+    int bias_free = 0;                  // biases held at zero: the network is positively homogeneous
     float *w = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;   // FP32 master weights, gradient, Adam moments
     unsigned short *wh1 = nullptr, *wt1 = nullptr, *wh2 = nullptr, *wt2 = nullptr;   // bf16 copies of the 32->32 layers
     // work buffers for np <= cap pixels
@@ -42,4 +43,5 @@ void nn_reserve(jfem_nn* nn, long np);
 void nn_forward(jfem_nn* nn, const Lat& L);
 void nn_backward(jfem_nn* nn, const Lat& L, int add);
 void nn_adam_step(jfem_nn* nn, float lr, float grad_scale);
+void nn_set_bias_free(jfem_nn* nn, int on);
 }  // namespace jfem

@@ -49,6 +49,47 @@ def sinusoid_terms(seed: int, nterms: int, dim: int, kmax: int = 8):
     return kw, amp
 
 
+def splitmix_normals(seed: int, count: int) -> np.ndarray:
+    """count standard normals from the SplitMix64 stream of `seed` (vectorised:
+    output i is the mix of seed + (i+1)*golden), one Box-Muller normal per
+    consecutive pair of uniforms."""
+    m = 2 * count
+    i = np.arange(1, m + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed & MASK) + i * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+    u1 = np.maximum(u[0::2], 1e-300)
+    u2 = u[1::2]
+    return (np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2))[:count]
+
+
+def gaussian_random_field(seed: int, cells, length: float = 0.1, contrast: float = 1e3):
+    """log-coefficient g per cell (reference cell order: last axis fastest) of a
+    zero-mean Gaussian random field with squared-exponential covariance
+    exp(-|x-y|^2 / (2 length^2)) on the cell centres of the unit box, sampled
+    exactly through the separable kernel: g = (S_0 (x) S_1 (x) ...) xi with S_a the
+    symmetric square root of the 1D covariance matrix (eigendecomposition), then
+    rescaled so that max g - min g = ln(contrast).  a = exp(g)."""
+    cells = tuple(int(c) for c in cells)
+    roots = []
+    for n in cells:
+        x = (np.arange(n) + 0.5) / n
+        K = np.exp(-((x[:, None] - x[None, :]) ** 2) / (2 * length ** 2))
+        lam, V = np.linalg.eigh(K)
+        roots.append((V * np.sqrt(np.maximum(lam, 0.0))) @ V.T)
+    g = splitmix_normals(seed, int(np.prod(cells))).reshape(cells)
+    for a, S in enumerate(roots):
+        g = np.moveaxis(np.tensordot(S, g, axes=([1], [a])), 0, a)
+    g = g - g.mean()
+    span = g.max() - g.min()
+    if span > 0:
+        g = g * (np.log(contrast) / span)
+    return g.reshape(-1)
+
+
 # the five workloads: name -> (dim, degree, cells, start, end, seed, nterms)
 CONFIGS = {
     "c1_2d_q1_64": dict(dim=2, degree=1, cells=(64, 64), seed=1, nterms=8),

@@ -1,0 +1,112 @@
+"""Learned smoother on the GPU: training on residuals of plain V-cycles lowers
+the loss, the trained network helps the V-cycle more than an untrained one,
+the fallback to Chebyshev triggers on a network that does not smooth, and the
+FP32 multigrid with the learned smoother inside FP64 flexible CG meets the FP64
+residual judge."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _hier(jf, base, nlev, field=None, precision=0):
+    spaces = []
+    for l in range(nlev):
+        s = jf.ContinuousTPFESpace([0.0, 0.0], [1.0, 1.0], [base << l, base << l])
+        s.assembleFunctions(1)
+        spaces.append(s)
+    coeff = None if field is None else np.exp(field)
+    G = jf.MGPreconditioner(spaces, [jf.TPCellIntegral("GradGrad")], jf.ChebyshevSmoother(4, 1), cycles=1,
+                            fmg=False, precision=precision, cell_coeff=coeff)
+    return G, spaces
+
+
+def _rhs(jf, space, seed=1):
+    from jfem_amd.workloads import sinusoid_terms
+    kw, amp = sinusoid_terms(seed, 8, 2)
+    space.initializeRhs()
+    space.writeCellIntegralsToRhs([jf.TPRightHandSideIntegral(jf.SinusoidSource(kw, amp))], space.getRhs())
+    r = space.getRhs().numpy()
+    r[space.boundary_nodes()] = 0
+    return r
+
+
+@pytest.fixture(scope="module")
+def trained(jf):
+    from jfem_amd.workloads import gaussian_random_field
+    field = gaussian_random_field(4000, (128, 128), 0.1, 1e3)
+    G, spaces = _hier(jf, 8, 5, field)
+    net = jf.SmootherNet(2, 2, seed=7)
+    G.set_learned(net, field, runs=2, fallback_ratio=0.5)
+    p0 = net.get_params()     # biases already zeroed by set_learned
+    hist = G.train(rhs_per_field=60, cycles_per_rhs=8, lr=3e-4, seed=1, mode=3)
+    return dict(G=G, spaces=spaces, net=net, field=field, hist=hist, p0=p0)
+
+
+def test_training_lowers_loss(trained):
+    """the energy loss |e - N(r)|_A^2 - |e|_A^2 (per node) falls well below zero"""
+    h = trained["hist"]
+    assert np.all(np.isfinite(h))
+    assert np.mean(h[-40:]) < -1.0 and np.mean(h[-40:]) < 5 * np.mean(h[:10]), (np.mean(h[:10]), np.mean(h[-40:]))
+
+
+def test_trained_network_beats_untrained(jf, trained):
+    G, spaces, net = trained["G"], trained["spaces"], trained["net"]
+    rhs = _rhs(jf, spaces[-1])
+    b = jf.DenseVector(rhs)
+    tol = 1e-8 * np.linalg.norm(rhs)
+    G.set_learned_active(True)
+    info_t, hist_t = G.solve(b, jf.DenseVector(len(rhs)), tol, maxcycles=100)
+    fb_t = G.learned_status()[1]
+    # same hierarchy with the untrained initial weights
+    p_trained = net.get_params()
+    net.set_params(trained["p0"])
+    G.set_learned_active(True)
+    info_u, hist_u = G.solve(b, jf.DenseVector(len(rhs)), tol, maxcycles=100)
+    net.set_params(p_trained)
+    # plain Chebyshev V-cycle
+    G.set_learned_active(False)
+    info_c, hist_c = G.solve(b, jf.DenseVector(len(rhs)), tol, maxcycles=100)
+    med = lambda h: float(np.median(h[1:] / h[:-1]))
+    print(f"V-cycles to 1e-8: trained {info_t.iterations} (fallback {fb_t}, median reduction {med(hist_t):.3f}), "
+          f"untrained {info_u.iterations} (fallback {G.learned_status()[1]}), chebyshev {info_c.iterations} "
+          f"(median reduction {med(hist_c):.3f})")
+    assert info_t.resnorm <= tol and info_c.resnorm <= tol
+    # the trained network smooths on its own (no fallback) and contracts each
+    # cycle more than the degree-4 Chebyshev-Jacobi smoother it replaces
+    assert not fb_t
+    assert med(hist_t) < med(hist_c)
+
+
+def test_fallback_on_a_network_that_does_not_smooth(jf, trained):
+    G, spaces, net = trained["G"], trained["spaces"], trained["net"]
+    zero = jf.SmootherNet(2, 2, seed=1)
+    zero.set_params(np.zeros(zero.num_params(), np.float32))
+    G.set_learned(zero, trained["field"], runs=2, fallback_ratio=0.5)
+    rhs = _rhs(jf, spaces[-1], seed=3)
+    b = jf.DenseVector(rhs)
+    tol = 1e-8 * np.linalg.norm(rhs)
+    info, hist = G.solve(b, jf.DenseVector(len(rhs)), tol, maxcycles=200)
+    active, fell_back = G.learned_status()
+    assert fell_back and not active
+    assert info.resnorm <= tol
+    G.set_learned(net, trained["field"], runs=2, fallback_ratio=0.5)
+
+
+def test_fp32_learned_mg_in_fp64_fcg(jf, trained):
+    import scipy.sparse.linalg as spla
+    field, net = trained["field"], trained["net"]
+    G32, spaces = _hier(jf, 8, 5, field, precision=1)
+    G32.set_learned(net, field, runs=2, fallback_ratio=0.5)
+    rhs = _rhs(jf, spaces[-1], seed=5)
+    b = jf.DenseVector(rhs)
+    tol = 1e-8 * np.linalg.norm(rhs)
+    it = jf.IterativeSolver()
+    A = G32.getFinestSystem()
+    x = it.solveFCG(A, G32, b, tol).numpy()
+    rp, cols, vals = A.to_csr()
+    import scipy.sparse as sp
+    S = sp.csr_matrix((vals, cols, rp), shape=(len(rhs), len(rhs)))
+    assert np.linalg.norm(rhs - S @ x) <= 1.01 * tol
+    exact = spla.spsolve(S.tocsc(), rhs)
+    assert np.linalg.norm(x - exact) <= 1e-6 * np.linalg.norm(exact)
