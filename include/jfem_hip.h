@@ -199,9 +199,10 @@ int jfem_nn_gradient_host(jfem_nn *nn, const int *shape, const float *in, const 
 /* one Adam step (grad NULL: the gradient left by the last backward pass) */
 int jfem_nn_adam_step(jfem_nn *nn, const float *grad, float lr);
 /* learned smoother on the finest multigrid level: with r = b - A x and
- * u = N(r / rms(r)), the step x += rms(r) u, or with min_residual_step the
- * minimal-residual step x += alpha u, alpha = (r.Au)/(Au.Au); `runs` steps per
- * smoothing call.  Input channels: the
+ * u = N(r / rms(r)), step_rule 0 takes x += rms(r) u, 1 the minimal-residual
+ * step x += alpha u with alpha = (r.Au)/(Au.Au), 2 the minimal-energy step
+ * alpha = (u.r)/(u.Au) (never increases the A-norm of the error); `runs`
+ * steps per smoothing call.  Input channels: the
  * normalised residual r^, then r^ times the mean log-coefficient around each
  * node (cell_log_coeff of the finest grid, nullable = 0) and, for cin = 2 + dim,
  * r^ times the lattice parity per axis.  Attaching a network sets its biases
@@ -209,7 +210,7 @@ int jfem_nn_adam_step(jfem_nn *nn, const float *grad, float lr);
  * fallback, taken once a cycle (or a flexible-CG iteration) reduces the
  * residual by less than fallback_ratio.  nn NULL detaches. */
 int jfem_mg_set_learned(jfem_mg *mg, jfem_nn *nn, const double *cell_log_coeff, int runs, double fallback_ratio,
-                        int min_residual_step);
+                        int step_rule);
 int jfem_mg_set_learned_active(jfem_mg *mg, int active);
 int jfem_mg_learned_status(jfem_mg *mg, int *active, int *fell_back);
 typedef struct {

@@ -470,14 +470,16 @@ class MGPreconditioner:
         check(N.lib().jfem_mg_vcycle(self.h, x.h, b.h))
         return x
 
-    def set_learned(self, net, cell_log_coeff=None, runs: int = 1, fallback_ratio: float = 0.5,
-                    min_residual_step: bool = False):
-        """attach a SmootherNet as the finest-level smoother (None detaches)"""
+    STEP_PLAIN, STEP_MIN_RESIDUAL, STEP_MIN_ENERGY = 0, 1, 2
+
+    def set_learned(self, net, cell_log_coeff=None, runs: int = 2, fallback_ratio: float = 0.5,
+                    step_rule: int = 0):
+        """attach a SmootherNet as the finest-level smoother (None detaches); step_rule 0 plain
+        (default), 1 minimal residual, 2 minimal energy along the network's direction"""
         cc = None if cell_log_coeff is None else f64(cell_log_coeff)
         self._net = net
         check(N.lib().jfem_mg_set_learned(self.h, net.h if net is not None else None,
-                                          dptr(cc) if cc is not None else None, runs, fallback_ratio,
-                                          1 if min_residual_step else 0))
+                                          dptr(cc) if cc is not None else None, runs, fallback_ratio, step_rule))
 
     def set_learned_active(self, active: bool):
         check(N.lib().jfem_mg_set_learned_active(self.h, 1 if active else 0))

@@ -46,7 +46,7 @@ struct LearnedState {
     double *u = nullptr, *z = nullptr, *gd = nullptr, *rh = nullptr, *xs = nullptr, *bs = nullptr;   // training work
     double* loss = nullptr;      // device loss history
     long loss_cap = 0;
-    int mr = 0;                  // 1: minimal-residual step along N(r); 0: plain step x += N(r)
+    int mr = 0;                  // step rule: 0 plain x += N(r); 1 minimal residual; 2 minimal energy along N(r)
     int capture = 0;             // training: record the residual before finest post-smoothing
     double* capbuf = nullptr;
 };
@@ -824,16 +824,24 @@ __global__ void k_unpack_out(GridDev g, Lat L, long n, const float* __restrict__
         u[i] = bnd[i] ? (T)0 : (T)out[pix_of(g, L, i)];
 }
 
-// partials of (r.w) and (w.w) into slots 0 and 1
+// partials of (r.w), (w.w) (minimal residual, energy = 0) or (u.r), (u.w)
+// (minimal energy, energy = 1) into slots 0 and 1
 template <typename T>
 __global__ __launch_bounds__(256) void k_dot2(long n, const T* __restrict__ r, const T* __restrict__ w,
-                                              double* __restrict__ partials)
+                                              const T* __restrict__ u, int energy, double* __restrict__ partials)
 {
     __shared__ double sh[2][4];
     double a = 0, c = 0;
     for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
-        a += (double)r[i] * (double)w[i];
-        c += (double)w[i] * (double)w[i];
+        const double wi = (double)w[i];
+        if (energy) {
+            const double ui = (double)u[i];
+            a += ui * (double)r[i];
+            c += ui * wi;
+        } else {
+            a += (double)r[i] * wi;
+            c += wi * wi;
+        }
     }
     for (int o = 32; o > 0; o >>= 1) { a += __shfl_xor(a, o, 64); c += __shfl_xor(c, o, 64); }
     if ((threadIdx.x & 63) == 0) { sh[0][threadIdx.x >> 6] = a; sh[1][threadIdx.x >> 6] = c; }
@@ -844,7 +852,8 @@ __global__ __launch_bounds__(256) void k_dot2(long n, const T* __restrict__ r, c
     }
 }
 
-// x += alpha u with alpha = (r.Au)/(Au.Au): the step never increases ||r||_2
+// x += alpha u with alpha = (r.Au)/(Au.Au) (the step never increases ||r||_2) or
+// alpha = (u.r)/(u.Au) (it never increases the energy norm of the error)
 template <typename T>
 __global__ void k_mr_update(long n, const double* __restrict__ d, const T* __restrict__ u, T* __restrict__ x)
 {
@@ -883,7 +892,7 @@ static void learned_smooth(jfem_mg* G, MgLevel& L, const T* b, T* x, T* u, bool 
         k_unpack_out<T><<<gr, 256, 0, st>>>(to_dev(L.grid), S.lat, n, S.nn->out, L.bnd, u);
         launch_stream<T, T>(L.A, vals_of<T>(L), u, EpiStore<T>{w}, nullptr);
         const int blocks = grid_for(n, 256, RED_BLOCKS);
-        k_dot2<T><<<blocks, 256, 0, st>>>(n, r, w, c->red_partials);
+        k_dot2<T><<<blocks, 256, 0, st>>>(n, r, w, u, S.mr == 2 ? 1 : 0, c->red_partials);
         reduce_partials_dev(c, 2, blocks, S.rr + 2);
         k_mr_update<T><<<gr, 256, 0, st>>>(n, S.rr + 2, u, x);
         JFEM_HIP(hipGetLastError());
@@ -983,7 +992,7 @@ static void set_learned(jfem_mg* G, jfem_nn* nn, const double* cell_logc, int ru
     nn_set_bias_free(nn, 1);   // N(0) = 0 and N(c r) = c N(r): a consistent smoother
     S.nn = nn;
     S.runs = runs > 0 ? runs : 1;
-    S.mr = mr ? 1 : 0;
+    S.mr = (mr >= 0 && mr <= 2) ? mr : 0;
     S.fallback = fallback > 0 ? fallback : 0.5;
     S.active = 1;
     S.fell_back = 0;
@@ -1180,11 +1189,12 @@ int jfem_mg_vcycle(jfem_mg* mg, jfem_vec* x, const jfem_vec* b)
 }
 
 int jfem_mg_set_learned(jfem_mg* mg, jfem_nn* nn, const double* cell_log_coeff, int runs, double fallback_ratio,
-                        int min_residual_step)
+                        int step_rule)
 {
     API_BEGIN
     JFEM_CHECK(mg, JFEM_ERR_ARG, "null argument");
-    set_learned(mg, nn, cell_log_coeff, runs, fallback_ratio, min_residual_step);
+    JFEM_CHECK(step_rule >= 0 && step_rule <= 2, JFEM_ERR_ARG, "unknown step rule");
+    set_learned(mg, nn, cell_log_coeff, runs, fallback_ratio, step_rule);
     JFEM_HIP(hipStreamSynchronize(mg->ctx->stream));
     API_END
 }
