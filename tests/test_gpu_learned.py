@@ -63,20 +63,21 @@ def test_trained_network_beats_untrained(jf, trained):
     net.set_params(trained["p0"])
     G.set_learned_active(True)
     info_u, hist_u = G.solve(b, jf.DenseVector(len(rhs)), tol, maxcycles=100)
+    fb_u = G.learned_status()[1]
     net.set_params(p_trained)
     # plain Chebyshev V-cycle
     G.set_learned_active(False)
     info_c, hist_c = G.solve(b, jf.DenseVector(len(rhs)), tol, maxcycles=100)
     med = lambda h: float(np.median(h[1:] / h[:-1]))
     print(f"V-cycles to 1e-8: trained {info_t.iterations} (fallback {fb_t}, median reduction {med(hist_t):.3f}), "
-          f"untrained {info_u.iterations} (fallback {G.learned_status()[1]}), chebyshev {info_c.iterations} "
+          f"untrained {info_u.iterations} (fallback {fb_u}), chebyshev {info_c.iterations} "
           f"(median reduction {med(hist_c):.3f})")
     assert info_t.resnorm <= tol and info_c.resnorm <= tol
     # training is what makes the learned step a smoother: the trained network
     # contracts the typical cycle more than the degree-4 Chebyshev-Jacobi
     # smoother, the untrained one falls back or converges no faster
     assert med(hist_t) < med(hist_c)
-    assert info_t.iterations <= info_u.iterations or G.learned_status()[1]
+    assert info_t.iterations <= info_u.iterations or fb_u
 
 
 def test_fallback_on_a_network_that_does_not_smooth(jf, trained):
