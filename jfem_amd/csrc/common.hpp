@@ -171,8 +171,10 @@ struct jfem_mat {
     float* vals32 = nullptr; // optional FP32 copy of the values
     double* diag = nullptr;  // optional: diagonal (device) for Jacobi-type smoothers
     int lanes = 8;           // lanes per row for the SpMV, from the mean row length
-    int* rblk = nullptr;     // device: first row of each CSR-stream block (nrblk+1)
+    int* rblk = nullptr;     // device: first row of each CSR-stream block (nrblk+1), FP64 capacity
     long nrblk = 0;
+    int* rblk32 = nullptr;   // the same for FP32 values (twice the entries per block)
+    long nrblk32 = 0;
     double* part = nullptr;  // device: one partial sum per stream block
 };
 

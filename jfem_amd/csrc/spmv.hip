@@ -130,30 +130,41 @@ int spmv_dot(jfem_mat* A, const double* x, double* y)
     return grid;
 }
 
-// greedy split of the rows into CSR-stream blocks of at most ST_CAP entries
+// greedy split of the rows into CSR-stream blocks of at most `cap` entries
 // and at most ST_BS rows (a longer row gets a block of its own)
-void mat_build_blocks(jfem_mat* A)
+static std::vector<int> split_blocks(const std::vector<int>& rp, long n, long cap)
 {
-    (void)hipFree(A->rblk);
-    (void)hipFree(A->part);
-    A->rblk = nullptr;
-    A->part = nullptr;
-    const long n = A->rows;
-    std::vector<int> rp(n + 1);
-    JFEM_HIP(hipMemcpy(rp.data(), A->rowptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
     std::vector<int> blk;
     blk.reserve(n / 16 + 2);
     long r = 0;
     while (r < n) {
         blk.push_back((int)r);
         long e = r + 1;
-        while (e < n && e - r < ST_BS && (long)rp[e + 1] - rp[r] <= ST_CAP) e++;
+        while (e < n && e - r < ST_BS && (long)rp[e + 1] - rp[r] <= cap) e++;
         r = e;
     }
     blk.push_back((int)n);
+    return blk;
+}
+
+void mat_build_blocks(jfem_mat* A)
+{
+    (void)hipFree(A->rblk);
+    (void)hipFree(A->rblk32);
+    (void)hipFree(A->part);
+    A->rblk = A->rblk32 = nullptr;
+    A->part = nullptr;
+    const long n = A->rows;
+    std::vector<int> rp(n + 1);
+    JFEM_HIP(hipMemcpy(rp.data(), A->rowptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+    std::vector<int> blk = split_blocks(rp, n, ST_CAP);
     A->nrblk = (long)blk.size() - 1;
     JFEM_HIP(hipMalloc(&A->rblk, sizeof(int) * blk.size()));
     JFEM_HIP(hipMemcpy(A->rblk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice));
+    std::vector<int> blk32 = split_blocks(rp, n, ST_CAP32);
+    A->nrblk32 = (long)blk32.size() - 1;
+    JFEM_HIP(hipMalloc(&A->rblk32, sizeof(int) * blk32.size()));
+    JFEM_HIP(hipMemcpy(A->rblk32, blk32.data(), sizeof(int) * blk32.size(), hipMemcpyHostToDevice));
     JFEM_HIP(hipMalloc(&A->part, sizeof(double) * (A->nrblk > 0 ? A->nrblk : 1)));
 }
 
@@ -251,6 +262,7 @@ void mat_free(jfem_mat* A)
     (void)hipFree(A->vals32);
     (void)hipFree(A->diag);
     (void)hipFree(A->rblk);
+    (void)hipFree(A->rblk32);
     (void)hipFree(A->part);
     delete A;
 }

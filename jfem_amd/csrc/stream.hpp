@@ -25,8 +25,8 @@ namespace jfem {
 #define JFEM_ST_NT 0
 #endif
 constexpr int ST_BS = JFEM_ST_BS;     // threads per workgroup
-constexpr int ST_CAP = JFEM_ST_CAP;   // entries per row block
-constexpr int ST_PER = ST_CAP / ST_BS;
+constexpr int ST_CAP = JFEM_ST_CAP;   // entries per row block (FP64 values)
+constexpr int ST_CAP32 = 2 * ST_CAP;  // entries per row block (FP32 values): same bytes per block
 static_assert(ST_CAP % ST_BS == 0, "block capacity must be a multiple of the workgroup size");
 
 template <int NT>
@@ -43,12 +43,13 @@ __device__ inline double st_block_sum(double v, double* sh)
 
 // Epi must provide:  static constexpr bool reduce;  __device__ double operator()(long row, T ax)
 // (the returned value is summed over the block when reduce is true)
-template <typename VT, typename T, typename Epi>
+template <int CAP, typename VT, typename T, typename Epi>
 __global__ __launch_bounds__(ST_BS) void k_stream(const int* __restrict__ rblk, const int* __restrict__ rowptr,
                                                   const int* __restrict__ cols, const VT* __restrict__ vals,
                                                   const T* __restrict__ x, Epi epi, double* __restrict__ partials)
 {
-    __shared__ T prod[ST_CAP];
+    constexpr int ST_PER = CAP / ST_BS;
+    __shared__ T prod[CAP];
     __shared__ double sh[ST_BS / 64];
     const int tid = threadIdx.x;
     const int b = blockIdx.x;
@@ -56,7 +57,7 @@ __global__ __launch_bounds__(ST_BS) void k_stream(const int* __restrict__ rblk, 
     const int p0 = rowptr[r0], p1 = rowptr[r1];
     const int nr = r1 - r0, cnt = p1 - p0;
     double dsum = 0;
-    if (cnt > ST_CAP) {
+    if (cnt > CAP) {
         // one row longer than the block capacity: strided sum over the block
         double acc = 0;
         for (int p = p0 + tid; p < p1; p += ST_BS) acc += (double)((T)vals[p] * x[cols[p]]);
@@ -162,13 +163,20 @@ struct EpiCheby {
     }
 };
 
-// launch over the matrix's row blocks
+// launch over the matrix's row blocks (FP32 values use blocks of twice the entries)
 template <typename VT, typename T, typename Epi>
 inline void launch_stream(const jfem_mat* A, const VT* vals, const T* x, const Epi& epi, double* partials)
 {
-    if (A->nrblk == 0) return;
-    k_stream<VT, T, Epi><<<(unsigned)A->nrblk, ST_BS, 0, A->ctx->stream>>>(A->rblk, A->rowptr, A->colidx, vals, x, epi,
-                                                                           partials);
+    if (sizeof(VT) == 4) {
+        JFEM_CHECK(A->rblk32, JFEM_ERR_STATE, "matrix has no FP32 stream blocks");
+        if (A->nrblk32 == 0) return;
+        k_stream<ST_CAP32, VT, T, Epi><<<(unsigned)A->nrblk32, ST_BS, 0, A->ctx->stream>>>(A->rblk32, A->rowptr, A->colidx,
+                                                                                         vals, x, epi, partials);
+    } else {
+        if (A->nrblk == 0) return;
+        k_stream<ST_CAP, VT, T, Epi><<<(unsigned)A->nrblk, ST_BS, 0, A->ctx->stream>>>(A->rblk, A->rowptr, A->colidx,
+                                                                                     vals, x, epi, partials);
+    }
     JFEM_HIP(hipGetLastError());
 }
 
