@@ -181,6 +181,11 @@ int jfem_mg_apply(jfem_mg *mg, const jfem_vec *r, jfem_vec *z);
 int jfem_mg_vcycle(jfem_mg *mg, jfem_vec *x, const jfem_vec *b);
 /* Chebyshev interval upper bound used on each level (after safety factor) */
 int jfem_mg_lmax(jfem_mg *mg, double *out);
+/* prolongationOperators.get(level).mvMul / .tvMul
+ * (MGPreconditionerSpace.java:20,61-70): fine = P coarse and coarse = P^T fine
+ * between level `level` and level + 1, FP64, no boundary treatment */
+int jfem_mg_prolongate(jfem_mg *mg, int level, const jfem_vec *coarse, jfem_vec *fine);
+int jfem_mg_restrict(jfem_mg *mg, int level, const jfem_vec *fine, jfem_vec *coarse);
 
 /* ---- learned smoother network (not in the reference) --------------------
  * four 3^dim-tap convolutions on the node lattice: cin -> 32 -> 32 -> 32 -> 1

@@ -102,6 +102,8 @@ _SIGS = {
     "jfem_mg_apply": (C.c_int, [_vp, _vp, _vp]),
     "jfem_mg_vcycle": (C.c_int, [_vp, _vp, _vp]),
     "jfem_mg_lmax": (C.c_int, [_vp, _dp]),
+    "jfem_mg_prolongate": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+    "jfem_mg_restrict": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "jfem_nn_create": (C.c_int, [_vp, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(_vp)]),
     "jfem_nn_destroy": (C.c_int, [_vp]),
     "jfem_nn_num_params": (C.c_long, [_vp]),

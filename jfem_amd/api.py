@@ -513,6 +513,18 @@ class MGPreconditioner:
     def tvMul(self, v):
         raise N.UnsupportedOperationException(3, "not implemented yet")
 
+    def prolongate(self, level: int, coarse: DenseVector) -> DenseVector:
+        """prolongationOperators.get(level).mvMul(coarse): level -> level + 1"""
+        out = DenseVector(self.getSystem(level + 1).info()[0], self.ctx)
+        check(N.lib().jfem_mg_prolongate(self.h, level, coarse.h, out.h))
+        return out
+
+    def restrict(self, level: int, fine: DenseVector) -> DenseVector:
+        """prolongationOperators.get(level).tvMul(fine): level + 1 -> level"""
+        out = DenseVector(self.getSystem(level).info()[0], self.ctx)
+        check(N.lib().jfem_mg_restrict(self.h, level, fine.h, out.h))
+        return out
+
 
 # --------------------------------------------------------------------------
 # iterative solvers

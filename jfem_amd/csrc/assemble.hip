@@ -706,8 +706,8 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     launch(false);
     const long nnz = scan_counts(c, counts, n, A->rowptr);
     A->nnz = nnz;
-    JFEM_HIP(hipMalloc(&A->colidx, sizeof(int) * (nnz > 0 ? nnz : 1)));
-    JFEM_HIP(hipMalloc(&A->vals, sizeof(double) * (nnz > 0 ? nnz : 1)));
+    JFEM_HIP(hipMalloc(&A->colidx, sizeof(int) * (nnz + NNZ_PAD)));
+    JFEM_HIP(hipMalloc(&A->vals, sizeof(double) * (nnz + NNZ_PAD)));
     launch(true);
     JFEM_HIP(hipStreamSynchronize(c->stream));
     (void)hipFree(counts);

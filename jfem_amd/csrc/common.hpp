@@ -162,6 +162,15 @@ struct jfem_vec {
     double* d = nullptr;     // device storage (float64)
 };
 
+// CSR-stream row block: rows [r0, r1), entries [p0, p1)
+struct __attribute__((aligned(16))) StBlk {
+    int r0, r1, p0, p1;
+};
+
+// entries allocated past nnz in colidx / vals / vals32, so that 16-byte loads
+// that start at an entry below nnz stay inside the allocation
+constexpr long NNZ_PAD = 4;
+
 struct jfem_mat {
     jfem_ctx* ctx = nullptr;
     long rows = 0, cols = 0, nnz = 0;
@@ -171,9 +180,9 @@ struct jfem_mat {
     float* vals32 = nullptr; // optional FP32 copy of the values
     double* diag = nullptr;  // optional: diagonal (device) for Jacobi-type smoothers
     int lanes = 8;           // lanes per row for the SpMV, from the mean row length
-    int* rblk = nullptr;     // device: first row of each CSR-stream block (nrblk+1), FP64 capacity
+    StBlk* blk = nullptr;    // device: CSR-stream block descriptors for FP64 values
     long nrblk = 0;
-    int* rblk32 = nullptr;   // the same for FP32 values (twice the entries per block)
+    StBlk* blk32 = nullptr;  // the same for FP32 values (twice the entries per block)
     long nrblk32 = 0;
     double* part = nullptr;  // device: one partial sum per stream block
 };

@@ -30,6 +30,14 @@ struct MgLevel {
     // work vectors, sized for doubles (reused as floats in FP32 mode)
     void *x = nullptr, *b = nullptr, *r = nullptr, *t = nullptr, *dv = nullptr;
     double lmax = 0;                    // Chebyshev upper bound on D^-1 A
+    // transfer from the next coarser level (levels > 0): per axis the 1D
+    // prolongation P_a (fine lattice x coarse lattice) and R_a = P_a^T as CSR
+    // in one device buffer, and two lattice-ordered work arrays
+    int* tp_rp[2][jfem::MAXD] = {};           // [0] = P, [1] = R: row pointers
+    int* tp_ix[2][jfem::MAXD] = {};           // column (lattice) indices
+    double* tp_w[2][jfem::MAXD] = {};         // weights
+    void* tp_buf = nullptr;             // owns the tables above
+    void *tw0 = nullptr, *tw1 = nullptr;
 };
 
 // learned smoother on the finest level (a network applied to the normalised
@@ -66,81 +74,41 @@ namespace jfem {
 
 constexpr int MB = 256;
 
+// ------------------------------------------------ smoother kernels
+// first Jacobi step from x = 0: xn = x + (dinv * (b - A x)) * omega with x = 0
+// (JacobiSmoother.java:45-48); no matrix is read
 template <typename T>
-__device__ inline T ld(const T* p, long i) { return p[i]; }
-
-// ------------------------------------------------ fused smoother kernels
-// residual of one row with L lanes: returns (A x)_row in lane 0 of the group
-template <int L, typename VT, typename T>
-__device__ inline T row_dot(const int* __restrict__ rowptr, const int* __restrict__ cols, const VT* __restrict__ vals,
-                            const T* __restrict__ x, long row, int lane)
-{
-    const int s = rowptr[row], e = rowptr[row + 1];
-    T acc = 0;
-    for (int p = s + lane; p < e; p += L) acc += (T)vals[p] * x[cols[p]];
-    for (int o = L / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, L);
-    return acc;
-}
-
-// Jacobi: xn = x + (dinv * (b - A x)) * omega
-template <int L, typename T>
-__global__ __launch_bounds__(MB) void k_jacobi(long n, const int* __restrict__ rowptr, const int* __restrict__ cols,
-                                               const T* __restrict__ vals, const T* __restrict__ dinv,
-                                               const T* __restrict__ b, const T* __restrict__ x, T* __restrict__ xn,
-                                               T omega, int x_zero)
+__global__ __launch_bounds__(MB) void k_jacobi0(long n, const T* __restrict__ dinv, const T* __restrict__ b,
+                                                T* __restrict__ xn, T omega)
 {
 #pragma clang fp contract(off)
-    const int lane = threadIdx.x % L;
-    const long groups = (long)gridDim.x * (MB / L);
-    for (long row = (blockIdx.x * (long)MB + threadIdx.x) / L; row < n; row += groups) {
-        const T ax = x_zero ? (T)0 : row_dot<L>(rowptr, cols, vals, x, row, lane);
-        if (lane == 0) {
-            const T xo = x_zero ? (T)0 : x[row];
-            xn[row] = xo + (dinv[row] * (b[row] - ax)) * omega;
-        }
-    }
+    for (long i = blockIdx.x * (long)MB + threadIdx.x; i < n; i += (long)gridDim.x * MB)
+        xn[i] = (T)0 + (dinv[i] * (b[i] - (T)0)) * omega;
 }
 
-// Chebyshev step: r = b - A x; d = first ? (dinv*r)/theta : c1*d + c2*(dinv*r); xn = x + d  (c0 = theta)
-template <int L, typename T>
-__global__ __launch_bounds__(MB) void k_cheby(long n, const int* __restrict__ rowptr, const int* __restrict__ cols,
-                                              const T* __restrict__ vals, const T* __restrict__ dinv,
-                                              const T* __restrict__ b, const T* __restrict__ x, T* __restrict__ xn,
-                                              T* __restrict__ d, int first, T c0, T c1, T c2, int x_zero)
+// first Chebyshev step from x = 0: d = dinv * (b - A x) / theta, xn = x + d with x = 0
+template <typename T>
+__global__ __launch_bounds__(MB) void k_cheby0(long n, const T* __restrict__ dinv, const T* __restrict__ b,
+                                               T* __restrict__ xn, T* __restrict__ d, T theta)
 {
-    const int lane = threadIdx.x % L;
-    const long groups = (long)gridDim.x * (MB / L);
-    for (long row = (blockIdx.x * (long)MB + threadIdx.x) / L; row < n; row += groups) {
-        const T ax = x_zero ? (T)0 : row_dot<L>(rowptr, cols, vals, x, row, lane);
-        if (lane == 0) {
-            const T z = dinv[row] * (b[row] - ax);
-            const T dn = first ? z / c0 : c1 * d[row] + c2 * z;
-            d[row] = dn;
-            const T xo = x_zero ? (T)0 : x[row];
-            xn[row] = xo + dn;
-        }
-    }
-}
-
-// r = b - A x
-template <int L, typename T>
-__global__ __launch_bounds__(MB) void k_residual(long n, const int* __restrict__ rowptr, const int* __restrict__ cols,
-                                                 const T* __restrict__ vals, const T* __restrict__ b,
-                                                 const T* __restrict__ x, T* __restrict__ r)
-{
-    const int lane = threadIdx.x % L;
-    const long groups = (long)gridDim.x * (MB / L);
-    for (long row = (blockIdx.x * (long)MB + threadIdx.x) / L; row < n; row += groups) {
-        const T ax = row_dot<L>(rowptr, cols, vals, x, row, lane);
-        if (lane == 0) r[row] = b[row] - ax;
+#pragma clang fp contract(off)
+    for (long i = blockIdx.x * (long)MB + threadIdx.x; i < n; i += (long)gridDim.x * MB) {
+        const T dn = (dinv[i] * (b[i] - (T)0)) / theta;
+        d[i] = dn;
+        xn[i] = (T)0 + dn;
     }
 }
 
 // ------------------------------------------------ transfers
-// 1D weights of the nested Qk prolongation: fine lattice p on the fine axis,
-// coarse cell C = min(p / 2k, nc - 1), position t = (p - 2kC) / 2k in C,
-// weight of coarse local node l = phi_l(t) (LagrangeBasisFunction1D values).
-__device__ inline double lag1(double t, int l, int k)
+// The prolongation of nested tensor-product Qk spaces is a Kronecker product
+// P = P_{d-1} x ... x P_0 of 1D matrices: fine lattice node p on an axis lies in
+// coarse cell C = min(p / 2k, nc - 1) at t = (p - 2kC) / 2k, and its weight for
+// coarse local node l is the Lagrange value phi_l(t) (LagrangeBasisFunction1D
+// values).  P and P^T are applied one axis at a time (sum factorisation): each
+// pass is a 1D sparse matrix on one lattice axis, the first pass reads the
+// source in DOF order and the last writes the target in DOF order, the passes
+// between work on lattice-ordered arrays (axis 0 fastest).
+__host__ __device__ inline double lag1(double t, int l, int k)
 {
     if (k == 1) return l == 0 ? 1.0 - t : t;
     if (k == 2) {
@@ -154,94 +122,217 @@ __device__ inline double lag1(double t, int l, int k)
     return r;
 }
 
-// xf = (add ? xf : 0) + P xc, with fine boundary entries zeroed if zero_bc
-template <typename T>
-__global__ __launch_bounds__(MB) void k_prolong(GridDev gf, GridDev gc, long nf, const T* __restrict__ xc,
-                                                T* __restrict__ xf, const unsigned char* __restrict__ bndf, int add,
-                                                int zero_bc)
+template <int D>
+__device__ inline long dof_of_lat(const GridDev& g, const int* p)
 {
-    const int k = gf.k, d = gf.d;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nf; i += (long)gridDim.x * blockDim.x) {
-        if (zero_bc && bndf[i]) {
-            if (!add) xf[i] = 0;
-            continue;
-        }
-        int p[MAXD];
-        lattice_of_dof(gf, i, p);
-        int C[MAXD];
-        double w[MAXD][5];
-        for (int a = 0; a < d; a++) {
-            int c = p[a] / (2 * k);
-            if (c > gc.n[a] - 1) c = gc.n[a] - 1;
-            C[a] = c;
-            const double t = (double)(p[a] - 2 * k * c) / (2 * k);
-            for (int l = 0; l <= k; l++) w[a][l] = lag1(t, l, k);
-        }
-        int nl = 1;
-        for (int a = 0; a < d; a++) nl *= k + 1;
-        double acc = 0;
-        for (int li = 0; li < nl; li++) {
-            int r = li, q[MAXD];
-            double wt = 1;
-            for (int a = 0; a < d; a++) {
-                const int l = r % (k + 1);
-                r /= k + 1;
-                wt *= w[a][l];
-                q[a] = C[a] * k + l;
-            }
-            if (wt != 0) acc += wt * (double)xc[dof_of_lattice(gc, q)];
-        }
-        xf[i] = add ? (T)((double)xf[i] + acc) : (T)acc;
+    long tail[D];
+    long t = 1;
+#pragma unroll
+    for (int a = D - 1; a >= 0; a--) { tail[a] = t; t *= (long)g.k * g.n[a] + 1; }
+    long pre = 0, mprod = 1, rank = 0;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        const int c = p[a] == 0 ? 0 : (p[a] - 1) / g.k;
+        const int l = p[a] - c * g.k;
+        const long m = c > 0 ? g.k : g.k + 1;
+        const long S = c > 0 ? (long)c * g.k + 1 : 0;
+        pre += mprod * S * tail[a];
+        rank += (long)(l - (c > 0 ? 1 : 0)) * mprod;
+        mprod *= m;
+    }
+    return pre + rank;
+}
+
+template <int D>
+__device__ inline void lat_of_dof(const GridDev& g, long dof, int* p)
+{
+    long tail[D];
+    long t = 1;
+#pragma unroll
+    for (int a = D - 1; a >= 0; a--) { tail[a] = t; t *= (long)g.k * g.n[a] + 1; }
+    long rem = dof, mprod = 1;
+    int c[D];
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        const long unit = mprod * tail[a];
+        const long q = rem / unit;
+        c[a] = q < g.k + 1 ? 0 : (int)((q - 1) / g.k);
+        rem -= (c[a] > 0 ? (long)c[a] * g.k + 1 : 0) * unit;
+        mprod *= c[a] > 0 ? g.k : g.k + 1;
+    }
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        const long m = c[a] > 0 ? g.k : g.k + 1;
+        p[a] = c[a] * g.k + (int)(rem % m) + (c[a] > 0 ? 1 : 0);
+        rem /= m;
     }
 }
 
-// xc = P^T xf with coarse boundary entries zeroed
-template <typename T>
-__global__ __launch_bounds__(MB) void k_restrict(GridDev gf, GridDev gc, long nc, const T* __restrict__ xf,
-                                                 T* __restrict__ xc, const unsigned char* __restrict__ bndc)
+struct Shape3 {
+    int n[3];
+};
+
+// One pass: out = M in along lattice axis AX (M = 1D CSR rp/ix/w).  IN_DOF: the
+// source is a DOF-ordered vector of grid gin; otherwise a lattice array of
+// shape sin.  OUT_DOF: the target is a DOF-ordered vector of grid gout with
+// one thread per DOF (coalesced stores), entries with bnd set are left alone
+// (add) or zeroed; otherwise a lattice array of shape sout.  Sums in FP64.
+template <typename T, int D, int AX, bool IN_DOF, bool OUT_DOF>
+__global__ __launch_bounds__(MB) void k_tp_pass(Shape3 sin, Shape3 sout, GridDev gin, GridDev gout,
+                                                const int* __restrict__ rp, const int* __restrict__ ix,
+                                                const double* __restrict__ w, const T* __restrict__ in,
+                                                T* __restrict__ out, const unsigned char* __restrict__ bnd, int add)
 {
-    const int k = gf.k, d = gf.d;
-    for (long j = blockIdx.x * (long)blockDim.x + threadIdx.x; j < nc; j += (long)gridDim.x * blockDim.x) {
-        if (bndc[j]) { xc[j] = 0; continue; }
-        int q[MAXD];
-        lattice_of_dof(gc, j, q);
-        int cnt[MAXD], pf[MAXD][20];
-        double w[MAXD][20];
-        for (int a = 0; a < d; a++) {
-            int clo, chi;
-            if (q[a] % k == 0) { clo = q[a] / k - 1; chi = q[a] / k; } else { clo = chi = q[a] / k; }
-            if (clo < 0) clo = 0;
-            if (chi > gc.n[a] - 1) chi = gc.n[a] - 1;
-            int m = 0;
-            for (int p = 2 * k * clo; p <= 2 * k * (chi + 1); p++) {
-                int c = p / (2 * k);
-                if (c > gc.n[a] - 1) c = gc.n[a] - 1;
-                const int l = q[a] - k * c;
-                if (l < 0 || l > k) continue;
-                const double wt = lag1((double)(p - 2 * k * c) / (2 * k), l, k);
-                if (wt == 0) continue;
-                pf[a][m] = p;
-                w[a][m] = wt;
-                m++;
+    const long total = (long)sout.n[0] * sout.n[1] * sout.n[2];
+    for (long t = blockIdx.x * (long)MB + threadIdx.x; t < total; t += (long)gridDim.x * MB) {
+        int q[3] = {0, 0, 0};
+        if (OUT_DOF) {
+            if (bnd && bnd[t]) {
+                if (!add) out[t] = (T)0;
+                continue;
             }
-            cnt[a] = m;
+            lat_of_dof<D>(gout, t, q);
+        } else {
+            q[0] = (int)(t % sout.n[0]);
+            const long r = t / sout.n[0];
+            q[1] = (int)(r % sout.n[1]);
+            q[2] = (int)(r / sout.n[1]);
         }
-        int tot = 1;
-        for (int a = 0; a < d; a++) tot *= cnt[a];
+        int p[3] = {q[0], q[1], q[2]};
+        const int row = q[AX];
         double acc = 0;
-        for (int t = 0; t < tot; t++) {
-            int r = t, p[MAXD];
-            double wt = 1;
-            for (int a = d - 1; a >= 0; a--) {
-                const int m = r % cnt[a];
-                r /= cnt[a];
-                p[a] = pf[a][m];
-                wt *= w[a][m];
-            }
-            acc += wt * (double)xf[dof_of_lattice(gf, p)];
+        for (int e = rp[row]; e < rp[row + 1]; e++) {
+            p[AX] = ix[e];
+            const long src = IN_DOF ? dof_of_lat<D>(gin, p) : p[0] + (long)sin.n[0] * (p[1] + (long)sin.n[1] * p[2]);
+            acc += w[e] * (double)in[src];
         }
-        xc[j] = (T)acc;
+        if (OUT_DOF && add) acc += (double)out[t];
+        out[t] = (T)acc;
     }
+}
+
+template <typename T, int D, int AX>
+static void tp_pass_ax(hipStream_t st, bool in_dof, bool out_dof, Shape3 sin, Shape3 sout, GridDev gin, GridDev gout,
+                       const int* rp, const int* ix, const double* w, const T* in, T* out, const unsigned char* bnd,
+                       int add)
+{
+    const long total = (long)sout.n[0] * sout.n[1] * sout.n[2];
+    const int g = grid_for(total, MB, 65536);
+    if (in_dof && out_dof)
+        k_tp_pass<T, D, AX, true, true><<<g, MB, 0, st>>>(sin, sout, gin, gout, rp, ix, w, in, out, bnd, add);
+    else if (in_dof)
+        k_tp_pass<T, D, AX, true, false><<<g, MB, 0, st>>>(sin, sout, gin, gout, rp, ix, w, in, out, bnd, add);
+    else if (out_dof)
+        k_tp_pass<T, D, AX, false, true><<<g, MB, 0, st>>>(sin, sout, gin, gout, rp, ix, w, in, out, bnd, add);
+    else
+        k_tp_pass<T, D, AX, false, false><<<g, MB, 0, st>>>(sin, sout, gin, gout, rp, ix, w, in, out, bnd, add);
+    JFEM_HIP(hipGetLastError());
+}
+
+template <typename T, int D>
+static void tp_pass_d(hipStream_t st, int ax, bool in_dof, bool out_dof, Shape3 sin, Shape3 sout, GridDev gin,
+                      GridDev gout, const int* rp, const int* ix, const double* w, const T* in, T* out,
+                      const unsigned char* bnd, int add)
+{
+    if (ax == 0) tp_pass_ax<T, D, 0>(st, in_dof, out_dof, sin, sout, gin, gout, rp, ix, w, in, out, bnd, add);
+    else if (ax == 1) tp_pass_ax<T, D, (D > 1 ? 1 : 0)>(st, in_dof, out_dof, sin, sout, gin, gout, rp, ix, w, in, out, bnd, add);
+    else tp_pass_ax<T, D, (D > 2 ? 2 : 0)>(st, in_dof, out_dof, sin, sout, gin, gout, rp, ix, w, in, out, bnd, add);
+}
+
+// Apply P (restrict = false: xf = (add ? xf : 0) + P xc, fine entries with bnd
+// set left alone / zeroed) or P^T (restrict = true: xc = P^T xf, coarse entries
+// with bnd set zeroed) of level F (fine) over level C (coarse).
+template <typename T>
+static void tp_transfer(hipStream_t st, const MgLevel& F, const MgLevel& C, bool restrict_, const T* src, T* dst,
+                        const unsigned char* bnd, int add)
+{
+    const int d = F.grid.d;
+    const GridDev gf = to_dev(F.grid), gc = to_dev(C.grid);
+    Shape3 cur;    // lattice shape of the current source
+    for (int a = 0; a < 3; a++) cur.n[a] = a < d ? (int)(restrict_ ? F.grid.nodes_axis(a) : C.grid.nodes_axis(a)) : 1;
+    const T* in = src;
+    for (int s = 0; s < d; s++) {
+        const int ax = restrict_ ? s : d - 1 - s;
+        const bool first = s == 0, last = s == d - 1;
+        Shape3 nxt = cur;
+        nxt.n[ax] = (int)(restrict_ ? C.grid.nodes_axis(ax) : F.grid.nodes_axis(ax));
+        // intermediate shapes: restriction (Nc0, Nf1, Nf2) -> tw0, (Nc0, Nc1, Nf2) -> tw1;
+        // prolongation (Nc0, Nc1, Nf2) -> tw1, (Nc0, Nf1, Nf2) -> tw0
+        const bool big = restrict_ ? s == 0 : s == d - 2;
+        T* out = last ? dst : (T*)(big ? F.tw0 : F.tw1);
+        const int which = restrict_ ? 1 : 0;
+        const GridDev gin = restrict_ ? gf : gc, gout = restrict_ ? gc : gf;
+        const unsigned char* b = last ? bnd : nullptr;
+        const int ad = last ? add : 0;
+        switch (d) {
+        case 1: tp_pass_d<T, 1>(st, ax, first, last, cur, nxt, gin, gout, F.tp_rp[which][ax], F.tp_ix[which][ax], F.tp_w[which][ax], in, out, b, ad); break;
+        case 2: tp_pass_d<T, 2>(st, ax, first, last, cur, nxt, gin, gout, F.tp_rp[which][ax], F.tp_ix[which][ax], F.tp_w[which][ax], in, out, b, ad); break;
+        default: tp_pass_d<T, 3>(st, ax, first, last, cur, nxt, gin, gout, F.tp_rp[which][ax], F.tp_ix[which][ax], F.tp_w[which][ax], in, out, b, ad); break;
+        }
+        in = out;
+        cur = nxt;
+    }
+}
+
+// 1D tables of the transfer onto fine level F from coarse level C, and the work arrays
+static void build_transfer(MgLevel& F, const MgLevel& C)
+{
+    const int d = F.grid.d, k = F.grid.k;
+    std::vector<int> ints;
+    std::vector<double> dbl;
+    struct Ofs { size_t rp, ix, w; };
+    Ofs ofs[2][MAXD];
+    for (int a = 0; a < d; a++) {
+        const int nc = C.grid.n[a];
+        const int Nf = (int)F.grid.nodes_axis(a), Nc = (int)C.grid.nodes_axis(a);
+        // P: row p (fine), entries (coarse node, weight) with nonzero weight
+        std::vector<std::vector<std::pair<int, double>>> P(Nf), R(Nc);
+        for (int p = 0; p < Nf; p++) {
+            int c = p / (2 * k);
+            if (c > nc - 1) c = nc - 1;
+            const double t = (double)(p - 2 * k * c) / (2 * k);
+            for (int l = 0; l <= k; l++) {
+                const double wt = lag1(t, l, k);
+                if (wt != 0) P[p].push_back({c * k + l, wt});
+            }
+        }
+        for (int p = 0; p < Nf; p++)
+            for (auto& e : P[p]) R[e.first].push_back({p, e.second});
+        for (int m = 0; m < 2; m++) {
+            const auto& M = m == 0 ? P : R;
+            ofs[m][a].rp = ints.size();
+            int cnt = 0;
+            ints.push_back(0);
+            for (auto& row : M) { cnt += (int)row.size(); ints.push_back(cnt); }
+            ofs[m][a].ix = ints.size();
+            ofs[m][a].w = dbl.size();
+            for (auto& row : M)
+                for (auto& e : row) { ints.push_back(e.first); dbl.push_back(e.second); }
+        }
+    }
+    const size_t ib = sizeof(int) * ints.size(), db = sizeof(double) * dbl.size();
+    const size_t ioff = (db + 255) / 256 * 256;
+    char* buf = nullptr;
+    JFEM_HIP(hipMalloc(&buf, ioff + ib));
+    F.tp_buf = buf;
+    JFEM_HIP(hipMemcpy(buf, dbl.data(), db, hipMemcpyHostToDevice));
+    JFEM_HIP(hipMemcpy(buf + ioff, ints.data(), ib, hipMemcpyHostToDevice));
+    for (int m = 0; m < 2; m++)
+        for (int a = 0; a < d; a++) {
+            F.tp_rp[m][a] = (int*)(buf + ioff) + ofs[m][a].rp;
+            F.tp_ix[m][a] = (int*)(buf + ioff) + ofs[m][a].ix;
+            F.tp_w[m][a] = (double*)buf + ofs[m][a].w;
+        }
+    // work arrays: restriction writes (Nc0, Nf1, Nf2) then (Nc0, Nc1, Nf2);
+    // prolongation writes (Nc0, Nc1, Nf2) then (Nc0, Nf1, Nf2)
+    long n0 = 1, n1 = 1;
+    for (int a = 0; a < d; a++) {
+        const long Nf = F.grid.nodes_axis(a), Nc = C.grid.nodes_axis(a);
+        n0 *= a == 0 ? Nc : Nf;
+        n1 *= a <= 1 ? Nc : Nf;
+    }
+    JFEM_HIP(hipMalloc(&F.tw0, sizeof(double) * n0));
+    JFEM_HIP(hipMalloc(&F.tw1, sizeof(double) * n1));
 }
 
 // x = Ainv b (dense, one wavefront per row, FP64 accumulation)
@@ -283,19 +374,15 @@ __global__ void k_copy_bnd(long n, const unsigned char* __restrict__ bnd, const 
         x[i] = bnd[i] ? v[i] : 0.0;
 }
 
-// w = dinv * (A v)   (power iteration on D^-1 A)
-template <int L>
-__global__ __launch_bounds__(MB) void k_dinv_av(long n, const int* __restrict__ rowptr, const int* __restrict__ cols,
-                                                const double* __restrict__ vals, const double* __restrict__ dinv,
-                                                const double* __restrict__ v, double* __restrict__ w)
-{
-    const int lane = threadIdx.x % L;
-    const long groups = (long)gridDim.x * (MB / L);
-    for (long row = (blockIdx.x * (long)MB + threadIdx.x) / L; row < n; row += groups) {
-        const double ax = row_dot<L>(rowptr, cols, vals, v, row, lane);
-        if (lane == 0) w[row] = dinv[row] * ax;
-    }
-}
+// w = dinv * (A v): stream epilogue of the power iteration on D^-1 A
+struct EpiDinv {
+    static constexpr bool reduce = false;
+    const double* dinv;
+    double* w;
+    struct Pre { double dinv; };
+    __device__ Pre load(long row) const { return {dinv[row]}; }
+    __device__ double operator()(long row, double ax, const Pre& p) const { w[row] = p.dinv * ax; return 0.0; }
+};
 
 __global__ void k_dinv_from_diag(long n, const double* __restrict__ diag, double* __restrict__ dinv,
                                  float* __restrict__ dinv32)
@@ -321,28 +408,12 @@ __global__ void k_hash_fill(long n, double* __restrict__ v)
 }
 
 // ------------------------------------------------ level operations
-static int mg_grid(const MgLevel& L, int lanes)
-{
-    const long need = (L.A->rows * lanes + MB - 1) / MB;
-    const long cap = 8192;
-    return (int)(need < 1 ? 1 : (need > cap ? cap : need));
-}
-
 template <typename T> static const T* vals_of(const MgLevel& L);
 template <> const double* vals_of<double>(const MgLevel& L) { return L.A->vals; }
 template <> const float* vals_of<float>(const MgLevel& L) { return L.A->vals32; }
 template <typename T> static const T* dinv_of(const MgLevel& L);
 template <> const double* dinv_of<double>(const MgLevel& L) { return L.dinv; }
 template <> const float* dinv_of<float>(const MgLevel& L) { return L.dinv32; }
-
-#define DISPATCH_L(L, ...)                              \
-    switch (L) {                                        \
-    case 4: { constexpr int LL = 4; __VA_ARGS__; break; }   \
-    case 8: { constexpr int LL = 8; __VA_ARGS__; break; }   \
-    case 16: { constexpr int LL = 16; __VA_ARGS__; break; } \
-    case 32: { constexpr int LL = 32; __VA_ARGS__; break; } \
-    default: { constexpr int LL = 64; __VA_ARGS__; break; } \
-    }
 
 template <typename T>
 static void level_residual(jfem_mg* G, MgLevel& L, const T* b, const T* x, T* r)
@@ -373,7 +444,6 @@ static void smooth(jfem_mg* G, MgLevel& L, const T* b, T*& x, T*& xalt, bool x_z
     const jfem_mg_opts& o = G->opt;
     hipStream_t st = G->ctx->stream;
     const long n = L.A->rows;
-    const int g = mg_grid(L, L.A->lanes);
     if (G->ls.active && &L == &G->lev.back()) {
         learned_smooth<T>(G, L, b, x, xalt, x_zero);
         return;
@@ -381,8 +451,7 @@ static void smooth(jfem_mg* G, MgLevel& L, const T* b, T*& x, T*& xalt, bool x_z
     if (o.smoother == JFEM_SM_JACOBI) {
         for (int it = 0; it < o.runs; it++) {
             if (x_zero && it == 0) {
-                DISPATCH_L(L.A->lanes, k_jacobi<LL, T><<<g, MB, 0, st>>>(n, L.A->rowptr, L.A->colidx, vals_of<T>(L),
-                                                                       dinv_of<T>(L), b, x, xalt, (T)o.omega, 1));
+                k_jacobi0<T><<<grid_for(n, MB, 16384), MB, 0, st>>>(n, dinv_of<T>(L), b, xalt, (T)o.omega);
                 JFEM_HIP(hipGetLastError());
             } else {
                 launch_stream<T, T>(L.A, vals_of<T>(L), x, EpiJacobi<T>{b, x, dinv_of<T>(L), xalt, (T)o.omega}, nullptr);
@@ -413,9 +482,8 @@ static void smooth(jfem_mg* G, MgLevel& L, const T* b, T*& x, T*& xalt, bool x_z
                     rho_old = rho;
                 }
                 if (zero) {
-                    DISPATCH_L(L.A->lanes, k_cheby<LL, T><<<g, MB, 0, st>>>(n, L.A->rowptr, L.A->colidx, vals_of<T>(L),
-                                                                          dinv_of<T>(L), b, x, xalt, dv, kd == 0 ? 1 : 0,
-                                                                          (T)theta, (T)c1, (T)c2, 1));
+                    // only the very first step of a smoothing call starts from x = 0 (kd == 0)
+                    k_cheby0<T><<<grid_for(n, MB, 16384), MB, 0, st>>>(n, dinv_of<T>(L), b, xalt, dv, (T)theta);
                     JFEM_HIP(hipGetLastError());
                 } else {
                     EpiCheby<T> e{b, x, dinv_of<T>(L), xalt, dv, (T)theta, (T)c1, (T)c2, kd == 0 ? 1 : 0};
@@ -431,22 +499,17 @@ static void smooth(jfem_mg* G, MgLevel& L, const T* b, T*& x, T*& xalt, bool x_z
 template <typename T>
 static void restrict_to(jfem_mg* G, int lf, const T* xf, T* xc)
 {
-    MgLevel& F = G->lev[lf];
-    MgLevel& C = G->lev[lf - 1];
-    const long nc = C.A->rows;
-    k_restrict<T><<<grid_for(nc, MB, 16384), MB, 0, G->ctx->stream>>>(to_dev(F.grid), to_dev(C.grid), nc, xf, xc, C.bnd);
-    JFEM_HIP(hipGetLastError());
+    const MgLevel& F = G->lev[lf];
+    const MgLevel& C = G->lev[lf - 1];
+    tp_transfer<T>(G->ctx->stream, F, C, true, xf, xc, C.bnd, 0);
 }
 
 template <typename T>
 static void prolong_to(jfem_mg* G, int lf, const T* xc, T* xf, bool add, bool zero_bc)
 {
-    MgLevel& F = G->lev[lf];
-    MgLevel& C = G->lev[lf - 1];
-    const long nf = F.A->rows;
-    k_prolong<T><<<grid_for(nf, MB, 16384), MB, 0, G->ctx->stream>>>(to_dev(F.grid), to_dev(C.grid), nf, xc, xf, F.bnd,
-                                                                      add ? 1 : 0, zero_bc ? 1 : 0);
-    JFEM_HIP(hipGetLastError());
+    const MgLevel& F = G->lev[lf];
+    const MgLevel& C = G->lev[lf - 1];
+    tp_transfer<T>(G->ctx->stream, F, C, false, xc, xf, zero_bc ? F.bnd : nullptr, add ? 1 : 0);
 }
 
 template <typename T>
@@ -584,10 +647,8 @@ static double power_lmax(jfem_mg* G, MgLevel& L, int its)
     double nv = std::sqrt(vec_dot(c, n, v, v));
     vec_axpby(c, n, 0.0, v, 1.0 / nv, v);
     double lam = 0;
-    const int g = mg_grid(L, L.A->lanes);
     for (int i = 0; i < its; i++) {
-        DISPATCH_L(L.A->lanes, k_dinv_av<LL><<<g, MB, 0, st>>>(n, L.A->rowptr, L.A->colidx, L.A->vals, L.dinv, v, w));
-        JFEM_HIP(hipGetLastError());
+        launch_stream<double, double>(L.A, L.A->vals, v, EpiDinv{L.dinv, w}, nullptr);
         const double nw = std::sqrt(vec_dot(c, n, w, w));
         lam = nw;
         if (nw == 0) break;
@@ -604,6 +665,7 @@ static void free_mg(jfem_mg* G)
         mat_free(L.A);
         (void)hipFree(L.bnd); (void)hipFree(L.dinv); (void)hipFree(L.dinv32);
         (void)hipFree(L.x); (void)hipFree(L.b); (void)hipFree(L.r); (void)hipFree(L.t); (void)hipFree(L.dv);
+        (void)hipFree(L.tp_buf); (void)hipFree(L.tw0); (void)hipFree(L.tw1);
     }
     (void)hipFree(G->coarse_inv);
     (void)hipFree(G->init); (void)hipFree(G->defect); (void)hipFree(G->tmp);
@@ -676,6 +738,7 @@ static jfem_mg* create_mg(jfem_space* const* spaces, int nlev, int nint, const i
                 JFEM_HIP(hipMalloc(p, sizeof(double) * n));
                 JFEM_HIP(hipMemsetAsync(*p, 0, sizeof(double) * n, c->stream));
             }
+            if (l > 0) build_transfer(L, G->lev[l - 1]);
             mat_diag(L.A, (double*)L.t);
             k_dinv_from_diag<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(n, (double*)L.t, L.dinv, L.dinv32);
             JFEM_HIP(hipGetLastError());
@@ -710,12 +773,17 @@ static jfem_mg* create_mg(jfem_space* const* spaces, int nlev, int nint, const i
 
 
 // ------------------------------------------------ learned smoother
+// partials of sum (w_i x_i)^2 (w = nullptr: weights 1)
 template <typename T>
-__global__ __launch_bounds__(256) void k_sumsq(long n, const T* __restrict__ x, double* __restrict__ partials)
+__global__ __launch_bounds__(256) void k_sumsq(long n, const T* __restrict__ x, const T* __restrict__ w,
+                                               double* __restrict__ partials)
 {
     __shared__ double sh[4];
     double acc = 0;
-    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) acc += (double)x[i] * (double)x[i];
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const double v = w ? (double)w[i] * (double)x[i] : (double)x[i];
+        acc += v * v;
+    }
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
     __syncthreads();
@@ -723,10 +791,10 @@ __global__ __launch_bounds__(256) void k_sumsq(long n, const T* __restrict__ x, 
 }
 
 template <typename T>
-static void sumsq_dev(jfem_ctx* c, long n, const T* x, double* dst)
+static void sumsq_dev(jfem_ctx* c, long n, const T* x, const T* w, double* dst)
 {
     const int blocks = grid_for(n, 256, RED_BLOCKS);
-    k_sumsq<T><<<blocks, 256, 0, c->stream>>>(n, x, c->red_partials);
+    k_sumsq<T><<<blocks, 256, 0, c->stream>>>(n, x, w, c->red_partials);
     JFEM_HIP(hipGetLastError());
     reduce_partials_dev(c, 1, blocks, dst);
 }
@@ -738,22 +806,25 @@ __device__ inline long pix_of(const GridDev& g, const Lat& L, long i)
     return p[0] + (long)L.n[0] * (p[1] + (long)L.n[1] * p[2]);
 }
 
-// network input: channel 0 = r * sqrt(n / rr), channel c >= 1 = channel 0 times
+// network input: channel 0 = s * sqrt(n / ss) with s = D^-1 r the Jacobi-scaled
+// residual and ss = |s|^2 (so the input has rms 1 and does not depend on the
+// mesh size or the local coefficient scale), channel c >= 1 = channel 0 times
 // static channel c (so that the bias-free network is positively homogeneous in r)
 template <typename T>
-__global__ void k_gather_in(GridDev g, Lat L, long n, const T* __restrict__ r, const double* __restrict__ rr,
-                            const float* __restrict__ stat, int cin, float* __restrict__ in)
+__global__ void k_gather_in(GridDev g, Lat L, long n, const T* __restrict__ r, const T* __restrict__ dinv,
+                            const double* __restrict__ rr, const float* __restrict__ stat, int cin,
+                            float* __restrict__ in)
 {
     const double scale = rr[0] > 0 ? sqrt((double)n / rr[0]) : 0.0;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         const long pix = pix_of(g, L, i);
-        const float v = (float)((double)r[i] * scale);
+        const float v = (float)((double)dinv[i] * (double)r[i] * scale);
         in[pix * cin] = v;
         for (int c = 1; c < cin; c++) in[pix * cin + c] = v * stat[pix * (cin - 1) + c - 1];
     }
 }
 
-// x[i] += out[pix] * rms(r) on free nodes
+// x[i] += out[pix] * rms(D^-1 r) on free nodes
 template <typename T>
 __global__ void k_scatter_add(GridDev g, Lat L, long n, const float* __restrict__ out, const double* __restrict__ rr,
                               const unsigned char* __restrict__ bnd, T* __restrict__ x)
@@ -765,7 +836,7 @@ __global__ void k_scatter_add(GridDev g, Lat L, long n, const float* __restrict_
     }
 }
 
-// training helpers: u = masked network output (DOF order), rh = r * sqrt(n / rr)
+// training helpers: u = masked network output (DOF order), rh = r * sqrt(n / ss)
 __global__ void k_train_unpack(GridDev g, Lat L, long n, const float* __restrict__ out, const double* __restrict__ r,
                                const double* __restrict__ rr, const unsigned char* __restrict__ bnd,
                                double* __restrict__ u, double* __restrict__ rh)
@@ -881,8 +952,9 @@ static void learned_smooth(jfem_mg* G, MgLevel& L, const T* b, T* x, T* u, bool 
         } else {
             level_residual<T>(G, L, b, x, r);
         }
-        sumsq_dev<T>(c, n, r, S.rr);
-        k_gather_in<T><<<gr, 256, 0, st>>>(to_dev(L.grid), S.lat, n, r, S.rr, S.stat, S.nn->cin, S.nn->in);
+        sumsq_dev<T>(c, n, r, dinv_of<T>(L), S.rr);
+        k_gather_in<T><<<gr, 256, 0, st>>>(to_dev(L.grid), S.lat, n, r, dinv_of<T>(L), S.rr, S.stat, S.nn->cin,
+                                           S.nn->in);
         nn_forward(S.nn, S.lat);
         if (!S.mr) {
             k_scatter_add<T><<<gr, 256, 0, st>>>(to_dev(L.grid), S.lat, n, S.nn->out, S.rr, L.bnd, x);
@@ -1056,8 +1128,9 @@ static int train_mg(jfem_mg* G, const jfem_train_opts* o, double* loss_host)
             } else {
                 launch_stream<double, double>(F.A, F.A->vals, S.xs, EpiResidual<double>{S.bs, r}, nullptr);
             }
-            sumsq_dev<double>(c, n, r, S.rr);
-            k_gather_in<double><<<gr, 256, 0, st>>>(to_dev(F.grid), S.lat, n, r, S.rr, S.stat, S.nn->cin, S.nn->in);
+            sumsq_dev<double>(c, n, r, F.dinv, S.rr);
+            k_gather_in<double><<<gr, 256, 0, st>>>(to_dev(F.grid), S.lat, n, r, F.dinv, S.rr, S.stat, S.nn->cin,
+                                                    S.nn->in);
             nn_forward(S.nn, S.lat);
             k_train_unpack<<<gr, 256, 0, st>>>(to_dev(F.grid), S.lat, n, S.nn->out, r, S.rr, F.bnd, S.u, S.rh);
             launch_stream<double, double>(F.A, F.A->vals, S.u, EpiResidual<double>{S.rh, S.z}, nullptr);   // z = rh - A u
@@ -1160,6 +1233,30 @@ int jfem_mg_apply(jfem_mg* mg, const jfem_vec* r, jfem_vec* z)
     JFEM_CHECK(r->d != z->d, JFEM_ERR_ARG, "output aliases input");
     mg_apply(mg, r->d, z->d);
     JFEM_HIP(hipStreamSynchronize(mg->ctx->stream));
+    API_END
+}
+
+int jfem_mg_prolongate(jfem_mg* mg, int level, const jfem_vec* coarse, jfem_vec* fine)
+{
+    API_BEGIN
+    JFEM_CHECK(mg && coarse && fine, JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(level >= 0 && level + 1 < (int)mg->lev.size(), JFEM_ERR_ARG, "level out of range");
+    const MgLevel& F = mg->lev[level + 1];
+    const MgLevel& C = mg->lev[level];
+    JFEM_CHECK(coarse->n == C.A->rows && fine->n == F.A->rows, JFEM_ERR_ARG, "wrong size");
+    tp_transfer<double>(mg->ctx->stream, F, C, false, coarse->d, fine->d, nullptr, 0);
+    API_END
+}
+
+int jfem_mg_restrict(jfem_mg* mg, int level, const jfem_vec* fine, jfem_vec* coarse)
+{
+    API_BEGIN
+    JFEM_CHECK(mg && coarse && fine, JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(level >= 0 && level + 1 < (int)mg->lev.size(), JFEM_ERR_ARG, "level out of range");
+    const MgLevel& F = mg->lev[level + 1];
+    const MgLevel& C = mg->lev[level];
+    JFEM_CHECK(coarse->n == C.A->rows && fine->n == F.A->rows, JFEM_ERR_ARG, "wrong size");
+    tp_transfer<double>(mg->ctx->stream, F, C, true, fine->d, coarse->d, nullptr, 0);
     API_END
 }
 

@@ -93,21 +93,21 @@ constexpr bool kUseStream = true;
 
 void spmv(jfem_mat* A, const double* x, double* y)
 {
-    if (kUseStream && A->rblk) launch_stream<double, double>(A, A->vals, x, EpiStore<double>{y}, nullptr);
+    if (kUseStream && A->blk) launch_stream<double, double>(A, A->vals, x, EpiStore<double>{y}, nullptr);
     else launch_spmv<double, double>(A, A->vals, x, y);
 }
 
 void spmv32(jfem_mat* A, const float* x, float* y)
 {
     JFEM_CHECK(A->vals32, JFEM_ERR_STATE, "matrix has no FP32 values");
-    if (kUseStream && A->rblk) launch_stream<float, float>(A, A->vals32, x, EpiStore<float>{y}, nullptr);
+    if (kUseStream && A->blk) launch_stream<float, float>(A, A->vals32, x, EpiStore<float>{y}, nullptr);
     else launch_spmv<float, float>(A, A->vals32, x, y);
 }
 
 // y = A x; dot(x, y) reduced into dst (device scalar)
 void spmv_dot_dev(jfem_mat* A, const double* x, double* y, double* dst)
 {
-    JFEM_CHECK(A->rblk, JFEM_ERR_STATE, "matrix has no stream blocks");
+    JFEM_CHECK(A->blk, JFEM_ERR_STATE, "matrix has no stream blocks");
     launch_stream<double, double>(A, A->vals, x, EpiStoreDot{y, x}, A->part);
     reduce_array_dev(A->ctx, A->part, A->nrblk, dst);
 }
@@ -130,41 +130,48 @@ int spmv_dot(jfem_mat* A, const double* x, double* y)
     return grid;
 }
 
-// greedy split of the rows into CSR-stream blocks of at most `cap` entries
-// and at most ST_BS rows (a longer row gets a block of its own)
-static std::vector<int> split_blocks(const std::vector<int>& rp, long n, long cap)
+// greedy split of the rows into CSR-stream blocks: at most ST_BS rows and at
+// most `cap` entries counted from the entry index rounded down to a multiple of
+// `vec` (a row that alone exceeds that gets a block of its own)
+static std::vector<StBlk> split_blocks(const std::vector<int>& rp, long n, long cap, int vec)
 {
-    std::vector<int> blk;
+    std::vector<StBlk> blk;
     blk.reserve(n / 16 + 2);
     long r = 0;
     while (r < n) {
-        blk.push_back((int)r);
+        const long a0 = rp[r] & ~(long)(vec - 1);
         long e = r + 1;
-        while (e < n && e - r < ST_BS && (long)rp[e + 1] - rp[r] <= cap) e++;
+        while (e < n && e - r < ST_BS && (long)rp[e + 1] - a0 <= cap) e++;
+        blk.push_back(StBlk{(int)r, (int)e, rp[r], rp[e]});
         r = e;
     }
-    blk.push_back((int)n);
     return blk;
+}
+
+static StBlk* upload_blocks(const std::vector<StBlk>& blk)
+{
+    StBlk* d = nullptr;
+    JFEM_HIP(hipMalloc(&d, sizeof(StBlk) * (blk.empty() ? 1 : blk.size())));
+    if (!blk.empty()) JFEM_HIP(hipMemcpy(d, blk.data(), sizeof(StBlk) * blk.size(), hipMemcpyHostToDevice));
+    return d;
 }
 
 void mat_build_blocks(jfem_mat* A)
 {
-    (void)hipFree(A->rblk);
-    (void)hipFree(A->rblk32);
+    (void)hipFree(A->blk);
+    (void)hipFree(A->blk32);
     (void)hipFree(A->part);
-    A->rblk = A->rblk32 = nullptr;
+    A->blk = A->blk32 = nullptr;
     A->part = nullptr;
     const long n = A->rows;
     std::vector<int> rp(n + 1);
     JFEM_HIP(hipMemcpy(rp.data(), A->rowptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
-    std::vector<int> blk = split_blocks(rp, n, ST_CAP);
-    A->nrblk = (long)blk.size() - 1;
-    JFEM_HIP(hipMalloc(&A->rblk, sizeof(int) * blk.size()));
-    JFEM_HIP(hipMemcpy(A->rblk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice));
-    std::vector<int> blk32 = split_blocks(rp, n, ST_CAP32);
-    A->nrblk32 = (long)blk32.size() - 1;
-    JFEM_HIP(hipMalloc(&A->rblk32, sizeof(int) * blk32.size()));
-    JFEM_HIP(hipMemcpy(A->rblk32, blk32.data(), sizeof(int) * blk32.size(), hipMemcpyHostToDevice));
+    const std::vector<StBlk> blk = split_blocks(rp, n, ST_CAP, st_vec<double>());
+    A->nrblk = (long)blk.size();
+    A->blk = upload_blocks(blk);
+    const std::vector<StBlk> blk32 = split_blocks(rp, n, ST_CAP32, st_vec<float>());
+    A->nrblk32 = (long)blk32.size();
+    A->blk32 = upload_blocks(blk32);
     JFEM_HIP(hipMalloc(&A->part, sizeof(double) * (A->nrblk > 0 ? A->nrblk : 1)));
 }
 
@@ -219,7 +226,7 @@ __global__ void k_diag(long n, const int* __restrict__ rowptr, const int* __rest
 void mat_make_f32(jfem_mat* A)
 {
     if (A->vals32) return;
-    JFEM_HIP(hipMalloc(&A->vals32, sizeof(float) * (A->nnz > 0 ? A->nnz : 1)));
+    JFEM_HIP(hipMalloc(&A->vals32, sizeof(float) * (A->nnz + NNZ_PAD)));
     if (A->nnz) k_to_f32<<<grid_for(A->nnz, 256, 16384), 256, 0, A->ctx->stream>>>(A->nnz, A->vals, A->vals32);
     JFEM_HIP(hipGetLastError());
 }
@@ -261,8 +268,8 @@ void mat_free(jfem_mat* A)
     (void)hipFree(A->vals);
     (void)hipFree(A->vals32);
     (void)hipFree(A->diag);
-    (void)hipFree(A->rblk);
-    (void)hipFree(A->rblk32);
+    (void)hipFree(A->blk);
+    (void)hipFree(A->blk32);
     (void)hipFree(A->part);
     delete A;
 }
@@ -292,8 +299,8 @@ int jfem_mat_from_csr(jfem_ctx* ctx, long rows, long cols, const long* rowptr, c
     A->cols = cols;
     A->nnz = nnz;
     JFEM_HIP(hipMalloc(&A->rowptr, sizeof(int) * (rows + 1)));
-    JFEM_HIP(hipMalloc(&A->colidx, sizeof(int) * (nnz > 0 ? nnz : 1)));
-    JFEM_HIP(hipMalloc(&A->vals, sizeof(double) * (nnz > 0 ? nnz : 1)));
+    JFEM_HIP(hipMalloc(&A->colidx, sizeof(int) * (nnz + NNZ_PAD)));
+    JFEM_HIP(hipMalloc(&A->vals, sizeof(double) * (nnz + NNZ_PAD)));
     JFEM_HIP(hipMemcpy(A->rowptr, rp.data(), sizeof(int) * (rows + 1), hipMemcpyHostToDevice));
     if (nnz) {
         JFEM_HIP(hipMemcpy(A->colidx, colidx, sizeof(int) * nnz, hipMemcpyHostToDevice));

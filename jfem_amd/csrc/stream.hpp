@@ -1,14 +1,18 @@
 // CSR-stream sparse row kernel with fused per-row epilogues.
 //
-// The rows are cut into blocks of consecutive rows holding at most ST_CAP
-// entries (a greedy split of the row pointer, built once per matrix).  A
-// 256-thread workgroup streams its block's values and column indices with
-// fully coalesced loads (eight independent loads per thread in flight), keeps
-// the products val*x[col] in LDS, then reduces each row with a group of G lanes
-// (G = largest power of two with G * rows <= 256) and hands the row sum to an
-// epilogue: plain store (SpMV), residual, Jacobi or Chebyshev update, with an
-// optional per-block partial of a dot product.  Summation order depends only
-// on the matrix, so results are deterministic run to run.
+// The rows are cut into blocks of consecutive rows whose entries, counted from
+// the 16-byte-aligned entry at or below the block's first one, number at most
+// ST_CAP (a greedy split of the row pointer, built once per matrix; FP32 values
+// use blocks of twice the entries).  A 512-thread workgroup reads its block
+// descriptor {first row, end row, first entry, end entry} with one load, then
+// issues the row bounds and the epilogue operands of its row and the block's
+// values and column indices as 16-byte loads (two rounds of four entries per
+// thread in flight), keeps the products val*x[col] in LDS, reduces each row
+// with a group of G lanes (G = largest power of two with G * rows <= 512) and
+// hands the row sum to an epilogue: plain store (SpMV), residual, Jacobi or
+// Chebyshev update, with an optional per-block partial of a dot product.
+// Summation order depends only on the matrix, so results are deterministic run
+// to run.
 #pragma once
 
 #include "common.hpp"
@@ -16,18 +20,22 @@
 namespace jfem {
 
 #ifndef JFEM_ST_BS
-#define JFEM_ST_BS 256
+#define JFEM_ST_BS 512
 #endif
 #ifndef JFEM_ST_CAP
-#define JFEM_ST_CAP 2048
+#define JFEM_ST_CAP 4096
 #endif
-#ifndef JFEM_ST_NT
-#define JFEM_ST_NT 0
+#ifndef JFEM_ST_VEC64
+#define JFEM_ST_VEC64 4
 #endif
 constexpr int ST_BS = JFEM_ST_BS;     // threads per workgroup
 constexpr int ST_CAP = JFEM_ST_CAP;   // entries per row block (FP64 values)
 constexpr int ST_CAP32 = 2 * ST_CAP;  // entries per row block (FP32 values): same bytes per block
 static_assert(ST_CAP % ST_BS == 0, "block capacity must be a multiple of the workgroup size");
+
+// entries per staging load: four (one 16-byte column load; values in one or two 16-byte loads)
+template <typename VT>
+constexpr int st_vec() { return sizeof(VT) == 8 ? JFEM_ST_VEC64 : 4; }
 
 template <int NT>
 __device__ inline double st_block_sum(double v, double* sh)
@@ -41,68 +49,90 @@ __device__ inline double st_block_sum(double v, double* sh)
     return r;   // thread 0
 }
 
-// Epi must provide:  static constexpr bool reduce;  __device__ double operator()(long row, T ax)
+// vector types of the staging loads
+template <typename S, int N>
+struct StVec {
+    typedef S type __attribute__((ext_vector_type(N)));
+};
+
+// Epi must provide:
+//   static constexpr bool reduce;
+//   struct Pre;  __device__ Pre load(long row) const;   (operands of the row, loaded early)
+//   __device__ double operator()(long row, T ax, const Pre&) const
 // (the returned value is summed over the block when reduce is true)
 template <int CAP, typename VT, typename T, typename Epi>
-__global__ __launch_bounds__(ST_BS) void k_stream(const int* __restrict__ rblk, const int* __restrict__ rowptr,
+__global__ __launch_bounds__(ST_BS) void k_stream(const StBlk* __restrict__ blk, const int* __restrict__ rowptr,
                                                   const int* __restrict__ cols, const VT* __restrict__ vals,
                                                   const T* __restrict__ x, Epi epi, double* __restrict__ partials)
 {
-    constexpr int ST_PER = CAP / ST_BS;
+    constexpr int VEC = st_vec<VT>();              // entries per staging load
+    constexpr int R = CAP / (ST_BS * VEC);         // staging loads per thread
+    static_assert(R * ST_BS * VEC == CAP, "block capacity must be a multiple of the loads per round");
+    typedef typename StVec<VT, VEC>::type vload_t;
+    typedef typename StVec<int, VEC>::type cload_t;
     __shared__ T prod[CAP];
     __shared__ double sh[ST_BS / 64];
     const int tid = threadIdx.x;
-    const int b = blockIdx.x;
-    const int r0 = rblk[b], r1 = rblk[b + 1];
-    const int p0 = rowptr[r0], p1 = rowptr[r1];
-    const int nr = r1 - r0, cnt = p1 - p0;
+    const StBlk B = blk[blockIdx.x];
+    const int nr = B.r1 - B.r0;
+    const int a0 = B.p0 & ~(VEC - 1);              // aligned start of the staging window
     double dsum = 0;
-    if (cnt > CAP) {
+    if (B.p1 - a0 > CAP) {
         // one row longer than the block capacity: strided sum over the block
         double acc = 0;
-        for (int p = p0 + tid; p < p1; p += ST_BS) acc += (double)((T)vals[p] * x[cols[p]]);
+        for (int p = B.p0 + tid; p < B.p1; p += ST_BS) acc += (double)((T)vals[p] * x[cols[p]]);
         const double s = st_block_sum<ST_BS>(acc, sh);
-        if (tid == 0) dsum = epi(r0, (T)s);
+        if (tid == 0) dsum = epi(B.r0, (T)s, epi.load(B.r0));
     } else {
-        // all loads of a thread are issued before any product is formed
-        VT v[ST_PER];
-        int c[ST_PER];
-#pragma unroll
-        for (int k = 0; k < ST_PER; k++) {
-            const int i = tid + k * ST_BS;
-#if JFEM_ST_NT
-            if (i < cnt) { v[k] = __builtin_nontemporal_load(&vals[p0 + i]); c[k] = __builtin_nontemporal_load(&cols[p0 + i]); }
-#else
-            if (i < cnt) { v[k] = vals[p0 + i]; c[k] = cols[p0 + i]; }
-#endif
-        }
-        T xv[ST_PER];
-#pragma unroll
-        for (int k = 0; k < ST_PER; k++) {
-            const int i = tid + k * ST_BS;
-            if (i < cnt) xv[k] = x[c[k]];
-        }
-#pragma unroll
-        for (int k = 0; k < ST_PER; k++) {
-            const int i = tid + k * ST_BS;
-            if (i < cnt) prod[i] = (T)v[k] * xv[k];
-        }
-        __syncthreads();
+        // the row of this thread's lane group: G lanes per row, G * rows <= ST_BS
         int G = 64;
         while (G > 1 && G * nr > ST_BS) G >>= 1;
-        const int lane = tid & (G - 1);
-        for (int j = tid / G; j < nr; j += ST_BS / G) {
-            const int s = rowptr[r0 + j] - p0, e = rowptr[r0 + j + 1] - p0;
+        const int j = tid / G, lane = tid & (G - 1);
+        int rs = 0, re = 0;
+        typename Epi::Pre pre{};
+        if (j < nr) {
+            rs = rowptr[B.r0 + j] - B.p0;
+            re = rowptr[B.r0 + j + 1] - B.p0;
+            if (lane == 0) pre = epi.load(B.r0 + j);
+        }
+        // all staging loads of a thread are issued before any product is formed
+        vload_t v[R];
+        cload_t c[R];
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            const int q = a0 + (tid + k * ST_BS) * VEC;
+            if (q < B.p1) {
+                v[k] = *reinterpret_cast<const vload_t*>(vals + q);
+                c[k] = *reinterpret_cast<const cload_t*>(cols + q);
+            }
+        }
+        T xv[R][VEC];
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            const int q = a0 + (tid + k * ST_BS) * VEC;
+#pragma unroll
+            for (int u = 0; u < VEC; u++)
+                if (q + u >= B.p0 && q + u < B.p1) xv[k][u] = x[c[k][u]];
+        }
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            const int q = a0 + (tid + k * ST_BS) * VEC;
+#pragma unroll
+            for (int u = 0; u < VEC; u++)
+                if (q + u >= B.p0 && q + u < B.p1) prod[q + u - B.p0] = (T)v[k][u] * xv[k][u];
+        }
+        __syncthreads();
+        if (j < nr) {
             T acc = 0;
-            for (int i = s + lane; i < e; i += G) acc += prod[i];
+            for (int i = rs + lane; i < re; i += G) acc += prod[i];
             for (int o = G >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
-            if (lane == 0) dsum += epi(r0 + j, acc);
+            if (lane == 0) dsum = epi(B.r0 + j, acc, pre);
         }
     }
     if (Epi::reduce) {
         __syncthreads();
         const double s = st_block_sum<ST_BS>(dsum, sh);
-        if (tid == 0) partials[b] = s;
+        if (tid == 0) partials[blockIdx.x] = s;
     }
 }
 
@@ -110,7 +140,9 @@ template <typename T>
 struct EpiStore {
     static constexpr bool reduce = false;
     T* y;
-    __device__ double operator()(long row, T ax) const { y[row] = ax; return 0.0; }
+    struct Pre {};
+    __device__ Pre load(long) const { return {}; }
+    __device__ double operator()(long row, T ax, const Pre&) const { y[row] = ax; return 0.0; }
 };
 
 // y = A x and partial of dot(x, y)
@@ -118,7 +150,9 @@ struct EpiStoreDot {
     static constexpr bool reduce = true;
     double* y;
     const double* x;
-    __device__ double operator()(long row, double ax) const { y[row] = ax; return x[row] * ax; }
+    struct Pre { double x; };
+    __device__ Pre load(long row) const { return {x[row]}; }
+    __device__ double operator()(long row, double ax, const Pre& p) const { y[row] = ax; return p.x * ax; }
 };
 
 template <typename T>
@@ -126,7 +160,9 @@ struct EpiResidual {
     static constexpr bool reduce = false;
     const T* b;
     T* r;
-    __device__ double operator()(long row, T ax) const { r[row] = b[row] - ax; return 0.0; }
+    struct Pre { T b; };
+    __device__ Pre load(long row) const { return {b[row]}; }
+    __device__ double operator()(long row, T ax, const Pre& p) const { r[row] = p.b - ax; return 0.0; }
 };
 
 // Jacobi: xn = x + (dinv * (b - A x)) * omega   (JacobiSmoother.java:45-48)
@@ -136,10 +172,12 @@ struct EpiJacobi {
     const T *b, *x, *dinv;
     T* xn;
     T omega;
-    __device__ double operator()(long row, T ax) const
+    struct Pre { T b, x, dinv; };
+    __device__ Pre load(long row) const { return {b[row], x[row], dinv[row]}; }
+    __device__ double operator()(long row, T ax, const Pre& p) const
     {
 #pragma clang fp contract(off)
-        xn[row] = x[row] + (dinv[row] * (b[row] - ax)) * omega;
+        xn[row] = p.x + (p.dinv * (p.b - ax)) * omega;
         return 0.0;
     }
 };
@@ -152,13 +190,15 @@ struct EpiCheby {
     T *xn, *d;
     T theta, c1, c2;
     int first;
-    __device__ double operator()(long row, T ax) const
+    struct Pre { T b, x, dinv, d; };
+    __device__ Pre load(long row) const { return {b[row], x[row], dinv[row], first ? (T)0 : d[row]}; }
+    __device__ double operator()(long row, T ax, const Pre& p) const
     {
 #pragma clang fp contract(off)
-        const T z = dinv[row] * (b[row] - ax);
-        const T dn = first ? z / theta : c1 * d[row] + c2 * z;
+        const T z = p.dinv * (p.b - ax);
+        const T dn = first ? z / theta : c1 * p.d + c2 * z;
         d[row] = dn;
-        xn[row] = x[row] + dn;
+        xn[row] = p.x + dn;
         return 0.0;
     }
 };
@@ -167,14 +207,14 @@ struct EpiCheby {
 template <typename VT, typename T, typename Epi>
 inline void launch_stream(const jfem_mat* A, const VT* vals, const T* x, const Epi& epi, double* partials)
 {
-    if (sizeof(VT) == 4) {
-        JFEM_CHECK(A->rblk32, JFEM_ERR_STATE, "matrix has no FP32 stream blocks");
+    if constexpr (sizeof(VT) == 4) {
+        JFEM_CHECK(A->blk32, JFEM_ERR_STATE, "matrix has no FP32 stream blocks");
         if (A->nrblk32 == 0) return;
-        k_stream<ST_CAP32, VT, T, Epi><<<(unsigned)A->nrblk32, ST_BS, 0, A->ctx->stream>>>(A->rblk32, A->rowptr, A->colidx,
+        k_stream<ST_CAP32, VT, T, Epi><<<(unsigned)A->nrblk32, ST_BS, 0, A->ctx->stream>>>(A->blk32, A->rowptr, A->colidx,
                                                                                          vals, x, epi, partials);
     } else {
         if (A->nrblk == 0) return;
-        k_stream<ST_CAP, VT, T, Epi><<<(unsigned)A->nrblk, ST_BS, 0, A->ctx->stream>>>(A->rblk, A->rowptr, A->colidx,
+        k_stream<ST_CAP, VT, T, Epi><<<(unsigned)A->nrblk, ST_BS, 0, A->ctx->stream>>>(A->blk, A->rowptr, A->colidx,
                                                                                      vals, x, epi, partials);
     }
     JFEM_HIP(hipGetLastError());

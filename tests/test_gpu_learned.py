@@ -44,10 +44,11 @@ def trained(jf):
 
 
 def test_training_lowers_loss(trained):
-    """the energy loss |e - N(r)|_A^2 - |e|_A^2 (per node) falls well below zero"""
+    """the energy loss |e - N(r)|_A^2 - |e|_A^2 (per node, in units of the
+    normalised input) starts near zero and ends at least five times lower"""
     h = trained["hist"]
     assert np.all(np.isfinite(h))
-    assert np.mean(h[-40:]) < -1.0 and np.mean(h[-40:]) < 5 * np.mean(h[:10]), (np.mean(h[:10]), np.mean(h[-40:]))
+    assert np.mean(h[-40:]) < 5 * np.mean(h[:10]) < 0, (np.mean(h[:10]), np.mean(h[-40:]))
 
 
 def test_trained_network_beats_untrained(jf, trained):

@@ -232,6 +232,25 @@ def test_mg_level_matrices_and_vcycle(jf, oracle, which):
     assert np.linalg.norm(got - exp) <= 1e-11 * np.linalg.norm(exp)
 
 
+@pytest.mark.parametrize("base,k", [([5], 2), ([2, 3], 1), ([2, 3], 2), ([2, 2], 3), ([2, 3, 2], 1), ([2, 2, 3], 2)])
+def test_mg_transfers_match_prolongation_matrix(jf, oracle, base, k):
+    """prolongationOperators.get(l).mvMul / tvMul against the oracle's sparse P
+    (coarse basis functions evaluated at the fine nodes); the device applies P
+    one axis at a time, so sums differ in order only"""
+    d = len(base)
+    G, O, spaces = _mg_pair(jf, oracle, base, k, 3, jf.RichardsonSmoother(0.1, 1), cycles=1, fmg=False)
+    rng = np.random.default_rng(11)
+    for l in range(2):
+        P = oracle.prolongation([c << l for c in base], k, [0.0] * d, [1.0] * d).to_scipy()
+        xc = rng.standard_normal(P.shape[1])
+        xf = rng.standard_normal(P.shape[0])
+        got_p = G.prolongate(l, jf.DenseVector(xc)).numpy()
+        got_r = G.restrict(l, jf.DenseVector(xf)).numpy()
+        exp_p, exp_r = P @ xc, P.T @ xf
+        assert np.linalg.norm(got_p - exp_p) <= 1e-14 * np.linalg.norm(exp_p) * (1 + k)
+        assert np.linalg.norm(got_r - exp_r) <= 1e-14 * np.linalg.norm(exp_r) * (1 + k)
+
+
 def test_mg_reference_apply_fmg(jf, oracle):
     """MGPreconditionerInterface.mvMul: copy BC, full-V correction, 3 V-cycles."""
     G, O, spaces = _mg_pair(jf, oracle, [2, 2], 1, 4, jf.RichardsonSmoother(0.1, 8), cycles=3, fmg=True)
