@@ -223,6 +223,69 @@ struct AxisSegs {
     int clo, chi;       // cells containing the row's node
 };
 
+// The row kernel is instantiated for fixed (dimension, degree) pairs, so that
+// index loops unroll and divisions by the degree become constant divisions;
+// DD = KK = 0 is the generic instantiation that reads them from the grid.
+template <int DD, int KK>
+struct DK {
+    __device__ static int d(const GridDev& g) { return DD ? DD : g.d; }
+    __device__ static int k(const GridDev& g) { return KK ? KK : g.k; }
+};
+
+template <int DD, int KK>
+__device__ inline void lat_of_dof_t(const GridDev& g, long dof, int* p)
+{
+    const int d = DK<DD, KK>::d(g), k = DK<DD, KK>::k(g);
+    long tail[MAXD];
+    long t = 1;
+#pragma unroll
+    for (int a = MAXD - 1; a >= 0; a--)
+        if (a < d) { tail[a] = t; t *= (long)k * g.n[a] + 1; }
+    long rem = dof, mprod = 1;
+    int c[MAXD];
+#pragma unroll
+    for (int a = 0; a < MAXD; a++)
+        if (a < d) {
+            const long unit = mprod * tail[a];
+            const long q = rem / unit;
+            c[a] = q < k + 1 ? 0 : (int)((q - 1) / k);
+            rem -= (c[a] > 0 ? (long)c[a] * k + 1 : 0) * unit;
+            mprod *= c[a] > 0 ? k : k + 1;
+        }
+#pragma unroll
+    for (int a = 0; a < MAXD; a++)
+        if (a < d) {
+            const int m = c[a] > 0 ? k : k + 1;
+            p[a] = c[a] * k + (int)(rem % m) + (c[a] > 0 ? 1 : 0);
+            rem /= m;
+        }
+}
+
+template <int DD, int KK>
+__device__ inline long dof_of_lat_t(const GridDev& g, const int* p)
+{
+    const int d = DK<DD, KK>::d(g), k = DK<DD, KK>::k(g);
+    long tail[MAXD];
+    long t = 1;
+#pragma unroll
+    for (int a = MAXD - 1; a >= 0; a--)
+        if (a < d) { tail[a] = t; t *= (long)k * g.n[a] + 1; }
+    long pre = 0, mprod = 1, rank = 0;
+#pragma unroll
+    for (int a = 0; a < MAXD; a++)
+        if (a < d) {
+            const int c = p[a] == 0 ? 0 : (p[a] - 1) / k;
+            const int l = p[a] - c * k;
+            pre += mprod * (c > 0 ? (long)c * k + 1 : 0) * tail[a];
+            rank += (long)(l - (c > 0 ? 1 : 0)) * mprod;
+            mprod *= c > 0 ? k : k + 1;
+        }
+    return pre + rank;
+}
+
+// Candidate column points of a row on one axis, in DOF order of their owner
+// cells: up to three segments (owner cell, first lattice point, count).  All
+// array indices are constants, so the segments stay in registers.
 __device__ inline void axis_segments(int p, int k, int n, AxisSegs& s)
 {
     int lo, hi;
@@ -230,73 +293,105 @@ __device__ inline void axis_segments(int p, int k, int n, AxisSegs& s)
     if (lo < 0) lo = 0;
     if (hi > n - 1) hi = n - 1;
     s.clo = lo; s.chi = hi;
-    s.nseg = 0;
-    if (lo > 0) { s.owner[s.nseg] = lo - 1; s.first[s.nseg] = k * lo; s.cnt[s.nseg] = 1; s.nseg++;
-                  s.owner[s.nseg] = lo; s.first[s.nseg] = k * lo + 1; s.cnt[s.nseg] = k; s.nseg++; }
-    else { s.owner[s.nseg] = 0; s.first[s.nseg] = 0; s.cnt[s.nseg] = k + 1; s.nseg++; }
-    if (hi > lo) { s.owner[s.nseg] = hi; s.first[s.nseg] = k * hi + 1; s.cnt[s.nseg] = k; s.nseg++; }
+    if (lo > 0) {
+        s.owner[0] = lo - 1; s.first[0] = k * lo;     s.cnt[0] = 1;
+        s.owner[1] = lo;     s.first[1] = k * lo + 1; s.cnt[1] = k;
+        s.owner[2] = hi;     s.first[2] = k * hi + 1; s.cnt[2] = k;
+        s.nseg = hi > lo ? 3 : 2;
+    } else {
+        s.owner[0] = 0;  s.first[0] = 0;          s.cnt[0] = k + 1;
+        s.owner[1] = hi; s.first[1] = k * hi + 1; s.cnt[1] = k;
+        s.owner[2] = 0;  s.first[2] = 0;          s.cnt[2] = 0;
+        s.nseg = hi > lo ? 2 : 1;
+    }
     s.W = k * (hi - lo + 1) + 1;
 }
 
 // decode candidate index t of a row into its column lattice point q
-__device__ inline void decode_candidate(int d, const AxisSegs* S, int t, int* q)
+template <int DD, int KK>
+__device__ inline void decode_candidate(const GridDev& g, const AxisSegs* S, int t, int* q)
 {
+    const int d = DK<DD, KK>::d(g);
     int tailW[MAXD];
     int tw = 1;
-    for (int a = d - 1; a >= 0; a--) { tailW[a] = tw; tw *= S[a].W; }
-    int rem = t, M = 1, seg[MAXD];
-    for (int a = 0; a < d; a++) {
-        int s = 0;
-        for (; s < S[a].nseg - 1; s++) {
-            const int block = M * S[a].cnt[s] * tailW[a];
-            if (rem < block) break;
-            rem -= block;
+#pragma unroll
+    for (int a = MAXD - 1; a >= 0; a--)
+        if (a < d) { tailW[a] = tw; tw *= S[a].W; }
+    int rem = t, M = 1, cs[MAXD], fs[MAXD];
+#pragma unroll
+    for (int a = 0; a < MAXD; a++)
+        if (a < d) {
+            // segment of axis a: the first whose block (M * cnt * tailW) still contains rem
+            int c = S[a].cnt[0], f = S[a].first[0];
+            const int b0 = M * S[a].cnt[0] * tailW[a];
+            if (S[a].nseg > 1 && rem >= b0) {
+                rem -= b0;
+                c = S[a].cnt[1];
+                f = S[a].first[1];
+                const int b1 = M * S[a].cnt[1] * tailW[a];
+                if (S[a].nseg > 2 && rem >= b1) {
+                    rem -= b1;
+                    c = S[a].cnt[2];
+                    f = S[a].first[2];
+                }
+            }
+            cs[a] = c;
+            fs[a] = f;
+            M *= c;
         }
-        seg[a] = s;
-        M *= S[a].cnt[s];
-    }
     // rem indexes inside the owner box, axis 0 fastest; strip the tail factor
     // (all later-axis segments were already resolved, so tailW contributions are 1)
     int stride = 1;
-    for (int a = 0; a < d; a++) {
-        const int c = S[a].cnt[seg[a]];
-        q[a] = S[a].first[seg[a]] + (rem / stride) % c;
-        stride *= c;
-    }
+#pragma unroll
+    for (int a = 0; a < MAXD; a++)
+        if (a < d) {
+            q[a] = fs[a] + (rem / stride) % cs[a];
+            stride *= cs[a];
+        }
 }
 
-// value of entry (row node p, column node q) summed over the cells containing both
+// value of entry (row node p, column node q) summed over the cells containing
+// both, in lexicographic cell order (last axis fastest): sum = integral + sum
+// for every cell whose integral is not exactly zero
+template <int DD, int KK>
 __device__ inline double entry_value(const RowParams& P, const int* p, const int* q, bool& present)
 {
-    const int d = P.g.d, k = P.g.k;
+#pragma clang fp contract(off)
+    const int d = DK<DD, KK>::d(P.g), k = DK<DD, KK>::k(P.g);
     int lo[MAXD], cnt[MAXD];
-    for (int a = 0; a < d; a++) {
-        // cells containing p_a and q_a
-        int plo, phi, qlo, qhi;
-        if (p[a] % k == 0) { plo = p[a] / k - 1; phi = p[a] / k; } else { plo = phi = p[a] / k; }
-        if (q[a] % k == 0) { qlo = q[a] / k - 1; qhi = q[a] / k; } else { qlo = qhi = q[a] / k; }
-        int l0 = max(max(plo, qlo), 0), h0 = min(min(phi, qhi), P.g.n[a] - 1);
-        lo[a] = l0;
-        cnt[a] = h0 - l0 + 1;
-    }
     int ncell = 1;
-    for (int a = 0; a < d; a++) ncell *= cnt[a] > 0 ? cnt[a] : 0;
+#pragma unroll
+    for (int a = 0; a < MAXD; a++)
+        if (a < d) {
+            // cells containing p_a and q_a
+            int plo, phi, qlo, qhi;
+            if (p[a] % k == 0) { plo = p[a] / k - 1; phi = p[a] / k; } else { plo = phi = p[a] / k; }
+            if (q[a] % k == 0) { qlo = q[a] / k - 1; qhi = q[a] / k; } else { qlo = qhi = q[a] / k; }
+            const int l0 = max(max(plo, qlo), 0), h0 = min(min(phi, qhi), P.g.n[a] - 1);
+            lo[a] = l0;
+            cnt[a] = h0 - l0 + 1;
+            ncell *= cnt[a] > 0 ? cnt[a] : 0;
+        }
     double sum = 0;
     present = false;
     const long nent = (long)P.nloc * P.nloc;
     for (int ci = 0; ci < ncell; ci++) {
         int c[MAXD];
         int r = ci;
-        for (int a = d - 1; a >= 0; a--) { c[a] = lo[a] + r % cnt[a]; r /= cnt[a]; }   // lexicographic
+#pragma unroll
+        for (int a = MAXD - 1; a >= 0; a--)
+            if (a < d) { c[a] = lo[a] + r % cnt[a]; r /= cnt[a]; }   // lexicographic
         long tup = 0, cell = 0;
         int iv = 0, iu = 0, mul = 1;
-        for (int a = 0; a < d; a++) {
-            tup = tup * P.ncls[a] + P.cls[a][c[a]];
-            cell = cell * P.g.n[a] + c[a];
-            iv += (p[a] - k * c[a]) * mul;
-            iu += (q[a] - k * c[a]) * mul;
-            mul *= k + 1;
-        }
+#pragma unroll
+        for (int a = 0; a < MAXD; a++)
+            if (a < d) {
+                tup = tup * P.ncls[a] + P.cls[a][c[a]];
+                cell = cell * P.g.n[a] + c[a];
+                iv += (p[a] - k * c[a]) * mul;
+                iu += (q[a] - k * c[a]) * mul;
+                mul *= k + 1;
+            }
         double integral = 0;
         for (int i = 0; i < P.nint; i++) {
             const double e = P.E[(i * P.ntuples + tup) * nent + (long)iv * P.nloc + iu];
@@ -311,12 +406,13 @@ __device__ inline double entry_value(const RowParams& P, const int* p, const int
 }
 
 // G lanes per row; FILL = false counts, FILL = true writes cols/vals
-template <int G, bool FILL>
+template <int DD, int KK, int G, bool FILL>
 __global__ __launch_bounds__(256) void k_assemble_rows(RowParams P, long nrows, int* __restrict__ counts,
                                                       const int* __restrict__ rowptr, int* __restrict__ cols,
                                                       double* __restrict__ vals)
 {
 #pragma clang fp contract(off)
+    const int d = DK<DD, KK>::d(P.g), k = DK<DD, KK>::k(P.g);
     const int lane = threadIdx.x % G;
     const int wlane = threadIdx.x & 63;
     const int gbase = wlane - lane;   // first lane of the group inside the wavefront
@@ -331,9 +427,11 @@ __global__ __launch_bounds__(256) void k_assemble_rows(RowParams P, long nrows, 
         int p[MAXD] = {0, 0, 0};
         int total = 0;
         if (active) {
-            lattice_of_dof(P.g, row, p);
+            lat_of_dof_t<DD, KK>(P.g, row, p);
             total = 1;
-            for (int a = 0; a < P.g.d; a++) { axis_segments(p[a], P.g.k, P.g.n[a], S[a]); total *= S[a].W; }
+#pragma unroll
+            for (int a = 0; a < MAXD; a++)
+                if (a < d) { axis_segments(p[a], k, P.g.n[a], S[a]); total *= S[a].W; }
         }
         // uniform trip count across the wavefront
         int my_chunks = (total + G - 1) / G;
@@ -348,9 +446,9 @@ __global__ __launch_bounds__(256) void k_assemble_rows(RowParams P, long nrows, 
             int colv = 0;
             if (active && t < total) {
                 int q[MAXD] = {0, 0, 0};
-                decode_candidate(P.g.d, S, t, q);
-                v = entry_value(P, p, q, present);
-                colv = (int)dof_of_lattice(P.g, q);
+                decode_candidate<DD, KK>(P.g, S, t, q);
+                v = entry_value<DD, KK>(P, p, q, present);
+                if (FILL || row == 0) colv = (int)dof_of_lat_t<DD, KK>(P.g, q);
                 if (row == 0 && colv == 0) present = true;   // dummy (0,0) entry
             }
             const unsigned long long bal = __ballot(present);
@@ -690,18 +788,28 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     const int G = maxW <= 16 ? 16 : (maxW <= 32 ? 32 : 64);
     const int groups_per_block = 256 / G;
     const int blocks = grid_for(n, groups_per_block, 65536);
-    auto launch = [&](bool fill) {
-        if (G == 16) {
-            if (fill) k_assemble_rows<16, true><<<blocks, 256, 0, c->stream>>>(RP, n, counts, A->rowptr, A->colidx, A->vals);
-            else k_assemble_rows<16, false><<<blocks, 256, 0, c->stream>>>(RP, n, counts, A->rowptr, A->colidx, A->vals);
-        } else if (G == 32) {
-            if (fill) k_assemble_rows<32, true><<<blocks, 256, 0, c->stream>>>(RP, n, counts, A->rowptr, A->colidx, A->vals);
-            else k_assemble_rows<32, false><<<blocks, 256, 0, c->stream>>>(RP, n, counts, A->rowptr, A->colidx, A->vals);
-        } else {
-            if (fill) k_assemble_rows<64, true><<<blocks, 256, 0, c->stream>>>(RP, n, counts, A->rowptr, A->colidx, A->vals);
-            else k_assemble_rows<64, false><<<blocks, 256, 0, c->stream>>>(RP, n, counts, A->rowptr, A->colidx, A->vals);
-        }
+    auto run = [&](auto kern) {
+        kern<<<blocks, 256, 0, c->stream>>>(RP, n, counts, A->rowptr, A->colidx, A->vals);
         JFEM_HIP(hipGetLastError());
+    };
+    // G follows from (d, k): (2k+1)^d candidates per row
+    auto launch = [&](bool fill) {
+        const int dk = g.d * 10 + g.k;
+        switch (dk) {
+        case 11: fill ? run(k_assemble_rows<1, 1, 16, true>) : run(k_assemble_rows<1, 1, 16, false>); break;
+        case 12: fill ? run(k_assemble_rows<1, 2, 16, true>) : run(k_assemble_rows<1, 2, 16, false>); break;
+        case 13: fill ? run(k_assemble_rows<1, 3, 16, true>) : run(k_assemble_rows<1, 3, 16, false>); break;
+        case 21: fill ? run(k_assemble_rows<2, 1, 16, true>) : run(k_assemble_rows<2, 1, 16, false>); break;
+        case 22: fill ? run(k_assemble_rows<2, 2, 32, true>) : run(k_assemble_rows<2, 2, 32, false>); break;
+        case 23: fill ? run(k_assemble_rows<2, 3, 64, true>) : run(k_assemble_rows<2, 3, 64, false>); break;
+        case 31: fill ? run(k_assemble_rows<3, 1, 32, true>) : run(k_assemble_rows<3, 1, 32, false>); break;
+        case 32: fill ? run(k_assemble_rows<3, 2, 64, true>) : run(k_assemble_rows<3, 2, 64, false>); break;
+        case 33: fill ? run(k_assemble_rows<3, 3, 64, true>) : run(k_assemble_rows<3, 3, 64, false>); break;
+        default:
+            if (G == 16) fill ? run(k_assemble_rows<0, 0, 16, true>) : run(k_assemble_rows<0, 0, 16, false>);
+            else if (G == 32) fill ? run(k_assemble_rows<0, 0, 32, true>) : run(k_assemble_rows<0, 0, 32, false>);
+            else fill ? run(k_assemble_rows<0, 0, 64, true>) : run(k_assemble_rows<0, 0, 64, false>);
+        }
     };
     launch(false);
     const long nnz = scan_counts(c, counts, n, A->rowptr);

@@ -130,29 +130,68 @@ int spmv_dot(jfem_mat* A, const double* x, double* y)
     return grid;
 }
 
-// greedy split of the rows into CSR-stream blocks: at most ST_BS rows and at
-// most `cap` entries counted from the entry index rounded down to a multiple of
-// `vec` (a row that alone exceeds that gets a block of its own)
-static std::vector<StBlk> split_blocks(const std::vector<int>& rp, long n, long cap, int vec)
+// Split of the rows into CSR-stream blocks: at most ST_BS rows and at most
+// `cap` entries counted from the entry index rounded down to a multiple of
+// `vec` (a row that alone exceeds that gets a block of its own).  The rows are
+// cut into segments of ST_SEG_ROWS rows and one thread splits each segment
+// greedily; pass 1 counts the blocks of each segment, pass 2 writes them at
+// the segment's offset.  Blocks never cross a segment boundary, so the split
+// depends only on the row pointer.
+constexpr long ST_SEG_ROWS = 4096;
+
+__global__ void k_split_blocks(const int* __restrict__ rowptr, long n, long cap, int vec, long nseg,
+                               int* __restrict__ seg_count, const long* __restrict__ seg_off, StBlk* __restrict__ out)
 {
-    std::vector<StBlk> blk;
-    blk.reserve(n / 16 + 2);
-    long r = 0;
-    while (r < n) {
-        const long a0 = rp[r] & ~(long)(vec - 1);
+    const long sg = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (sg >= nseg) return;
+    long r = sg * ST_SEG_ROWS;
+    const long end = r + ST_SEG_ROWS < n ? r + ST_SEG_ROWS : n;
+    long pos = out ? seg_off[sg] : 0;
+    int cnt = 0;
+    while (r < end) {
+        const long a0 = rowptr[r] & ~(long)(vec - 1);
         long e = r + 1;
-        while (e < n && e - r < ST_BS && (long)rp[e + 1] - a0 <= cap) e++;
-        blk.push_back(StBlk{(int)r, (int)e, rp[r], rp[e]});
+        while (e < end && e - r < ST_BS && (long)rowptr[e + 1] - a0 <= cap) e++;
+        if (out) out[pos++] = StBlk{(int)r, (int)e, rowptr[r], rowptr[e]};
+        cnt++;
         r = e;
     }
-    return blk;
+    if (!out) seg_count[sg] = cnt;
 }
 
-static StBlk* upload_blocks(const std::vector<StBlk>& blk)
+static StBlk* split_blocks(jfem_mat* A, long cap, int vec, long* nblk)
 {
+    jfem_ctx* c = A->ctx;
+    const long n = A->rows;
+    const long nseg = (n + ST_SEG_ROWS - 1) / ST_SEG_ROWS;
     StBlk* d = nullptr;
-    JFEM_HIP(hipMalloc(&d, sizeof(StBlk) * (blk.empty() ? 1 : blk.size())));
-    if (!blk.empty()) JFEM_HIP(hipMemcpy(d, blk.data(), sizeof(StBlk) * blk.size(), hipMemcpyHostToDevice));
+    if (nseg == 0) {
+        JFEM_HIP(hipMalloc(&d, sizeof(StBlk)));
+        *nblk = 0;
+        return d;
+    }
+    int* cnt_d = nullptr;
+    long* off_d = nullptr;
+    JFEM_HIP(hipMalloc(&cnt_d, sizeof(int) * nseg));
+    JFEM_HIP(hipMalloc(&off_d, sizeof(long) * nseg));
+    const int tb = 64;
+    const int nb = (int)((nseg + tb - 1) / tb);
+    k_split_blocks<<<nb, tb, 0, c->stream>>>(A->rowptr, n, cap, vec, nseg, cnt_d, nullptr, nullptr);
+    JFEM_HIP(hipGetLastError());
+    std::vector<int> cnt(nseg);
+    std::vector<long> off(nseg);
+    JFEM_HIP(hipMemcpyAsync(cnt.data(), cnt_d, sizeof(int) * nseg, hipMemcpyDeviceToHost, c->stream));
+    JFEM_HIP(hipStreamSynchronize(c->stream));
+    long tot = 0;
+    for (long i = 0; i < nseg; i++) { off[i] = tot; tot += cnt[i]; }
+    JFEM_HIP(hipMemcpyAsync(off_d, off.data(), sizeof(long) * nseg, hipMemcpyHostToDevice, c->stream));
+    JFEM_HIP(hipMalloc(&d, sizeof(StBlk) * tot));
+    k_split_blocks<<<nb, tb, 0, c->stream>>>(A->rowptr, n, cap, vec, nseg, nullptr, off_d, d);
+    JFEM_HIP(hipGetLastError());
+    JFEM_HIP(hipStreamSynchronize(c->stream));
+    (void)hipFree(cnt_d);
+    (void)hipFree(off_d);
+    *nblk = tot;
     return d;
 }
 
@@ -163,15 +202,8 @@ void mat_build_blocks(jfem_mat* A)
     (void)hipFree(A->part);
     A->blk = A->blk32 = nullptr;
     A->part = nullptr;
-    const long n = A->rows;
-    std::vector<int> rp(n + 1);
-    JFEM_HIP(hipMemcpy(rp.data(), A->rowptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
-    const std::vector<StBlk> blk = split_blocks(rp, n, ST_CAP, st_vec<double>());
-    A->nrblk = (long)blk.size();
-    A->blk = upload_blocks(blk);
-    const std::vector<StBlk> blk32 = split_blocks(rp, n, ST_CAP32, st_vec<float>());
-    A->nrblk32 = (long)blk32.size();
-    A->blk32 = upload_blocks(blk32);
+    A->blk = split_blocks(A, ST_CAP, st_vec<double>(), &A->nrblk);
+    A->blk32 = split_blocks(A, ST_CAP32, st_vec<float>(), &A->nrblk32);
     JFEM_HIP(hipMalloc(&A->part, sizeof(double) * (A->nrblk > 0 ? A->nrblk : 1)));
 }
 

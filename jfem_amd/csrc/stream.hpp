@@ -73,7 +73,8 @@ __global__ __launch_bounds__(ST_BS) void k_stream(const StBlk* __restrict__ blk,
     __shared__ T prod[CAP];
     __shared__ double sh[ST_BS / 64];
     const int tid = threadIdx.x;
-    const StBlk B = blk[blockIdx.x];
+    const int bid = blockIdx.x;
+    const StBlk B = blk[bid];
     const int nr = B.r1 - B.r0;
     const int a0 = B.p0 & ~(VEC - 1);              // aligned start of the staging window
     double dsum = 0;
@@ -132,7 +133,7 @@ __global__ __launch_bounds__(ST_BS) void k_stream(const StBlk* __restrict__ blk,
     if (Epi::reduce) {
         __syncthreads();
         const double s = st_block_sum<ST_BS>(dsum, sh);
-        if (tid == 0) partials[blockIdx.x] = s;
+        if (tid == 0) partials[bid] = s;
     }
 }
 
