@@ -406,10 +406,12 @@ __device__ inline double entry_value(const RowParams& P, const int* p, const int
 }
 
 // G lanes per row; FILL = false counts, FILL = true writes cols/vals
+// (FILL also checks that each row received exactly the entries its row pointer
+// reserved and raises *bad otherwise)
 template <int DD, int KK, int G, bool FILL>
 __global__ __launch_bounds__(256) void k_assemble_rows(RowParams P, long nrows, int* __restrict__ counts,
                                                       const int* __restrict__ rowptr, int* __restrict__ cols,
-                                                      double* __restrict__ vals)
+                                                      double* __restrict__ vals, int* __restrict__ bad)
 {
 #pragma clang fp contract(off)
     const int d = DK<DD, KK>::d(P.g), k = DK<DD, KK>::k(P.g);
@@ -463,6 +465,43 @@ __global__ __launch_bounds__(256) void k_assemble_rows(RowParams P, long nrows, 
             written += __popcll(mine);
         }
         if (!FILL && active && lane == 0) counts[row] = written;
+        if (FILL && active && lane == 0 && written != rowptr[row + 1] - base) atomicOr(bad, 1);
+    }
+}
+
+// smallest |E| over the element tables (bit patterns of non-negative doubles
+// order like the values, so an integer atomicMin is exact and deterministic)
+__global__ void k_absmin(long n, const double* __restrict__ e, unsigned long long* __restrict__ out)
+{
+    unsigned long long m = ~0ull;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(e[i]));
+        m = b < m ? b : m;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long t = __shfl_xor(m, o, 64);
+        m = t < m ? t : m;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMin(out, m);
+}
+
+// row lengths when every structural candidate is present: prod_a W_a
+template <int DD, int KK>
+__global__ __launch_bounds__(256) void k_count_structural(GridDev g, long nrows, int* __restrict__ counts)
+{
+    const int d = DK<DD, KK>::d(g), k = DK<DD, KK>::k(g);
+    for (long row = blockIdx.x * 256L + threadIdx.x; row < nrows; row += (long)gridDim.x * 256) {
+        int p[MAXD] = {0, 0, 0};
+        lat_of_dof_t<DD, KK>(g, row, p);
+        int tot = 1;
+#pragma unroll
+        for (int a = 0; a < MAXD; a++)
+            if (a < d) {
+                AxisSegs S;
+                axis_segments(p[a], k, g.n[a], S);
+                tot *= S.W;
+            }
+        counts[row] = tot;
     }
 }
 
@@ -788,8 +827,11 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     const int G = maxW <= 16 ? 16 : (maxW <= 32 ? 32 : 64);
     const int groups_per_block = 256 / G;
     const int blocks = grid_for(n, groups_per_block, 65536);
+    int* bad = nullptr;
+    JFEM_HIP(hipMalloc(&bad, sizeof(int)));
+    JFEM_HIP(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
     auto run = [&](auto kern) {
-        kern<<<blocks, 256, 0, c->stream>>>(RP, n, counts, A->rowptr, A->colidx, A->vals);
+        kern<<<blocks, 256, 0, c->stream>>>(RP, n, counts, A->rowptr, A->colidx, A->vals, bad);
         JFEM_HIP(hipGetLastError());
     };
     // G follows from (d, k): (2k+1)^d candidates per row
@@ -811,16 +853,59 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
             else fill ? run(k_assemble_rows<0, 0, 64, true>) : run(k_assemble_rows<0, 0, 64, false>);
         }
     };
-    launch(false);
+    // With one integral, no exactly-zero element entry, and coefficients whose
+    // products with the entries cannot round to zero, every cell contribution
+    // is nonzero, so every structural candidate is in the pattern and the row
+    // lengths have a closed form; otherwise the count pass evaluates them.
+    bool structural = nint == 1;
+    if (structural) {
+        unsigned long long* mn = nullptr;
+        JFEM_HIP(hipMalloc(&mn, sizeof(unsigned long long)));
+        JFEM_HIP(hipMemsetAsync(mn, 0xff, sizeof(unsigned long long), c->stream));
+        k_absmin<<<grid_for(Esize, 256, 2048), 256, 0, c->stream>>>(Esize, E, mn);
+        JFEM_HIP(hipGetLastError());
+        unsigned long long bits = 0;
+        JFEM_HIP(hipMemcpyAsync(&bits, mn, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+        JFEM_HIP(hipStreamSynchronize(c->stream));
+        (void)hipFree(mn);
+        double emin;
+        std::memcpy(&emin, &bits, sizeof(emin));
+        double cmin = 1.0;
+        if (coeff_host)
+            for (long i = 0; i < g.ncells(); i++) cmin = std::min(cmin, std::fabs(coeff_host[i]));
+        structural = emin > 0 && cmin > 0 && emin * cmin > 1e-300;
+    }
+    if (structural) {
+        const int cb = grid_for(n, 256, 65536);
+        switch (g.d * 10 + g.k) {
+        case 11: k_count_structural<1, 1><<<cb, 256, 0, c->stream>>>(RP.g, n, counts); break;
+        case 12: k_count_structural<1, 2><<<cb, 256, 0, c->stream>>>(RP.g, n, counts); break;
+        case 21: k_count_structural<2, 1><<<cb, 256, 0, c->stream>>>(RP.g, n, counts); break;
+        case 22: k_count_structural<2, 2><<<cb, 256, 0, c->stream>>>(RP.g, n, counts); break;
+        case 31: k_count_structural<3, 1><<<cb, 256, 0, c->stream>>>(RP.g, n, counts); break;
+        case 32: k_count_structural<3, 2><<<cb, 256, 0, c->stream>>>(RP.g, n, counts); break;
+        default: k_count_structural<0, 0><<<cb, 256, 0, c->stream>>>(RP.g, n, counts); break;
+        }
+        JFEM_HIP(hipGetLastError());
+    } else {
+        launch(false);
+    }
     const long nnz = scan_counts(c, counts, n, A->rowptr);
     A->nnz = nnz;
     JFEM_HIP(hipMalloc(&A->colidx, sizeof(int) * (nnz + NNZ_PAD)));
     JFEM_HIP(hipMalloc(&A->vals, sizeof(double) * (nnz + NNZ_PAD)));
     launch(true);
+    int bad_h = 0;
+    JFEM_HIP(hipMemcpyAsync(&bad_h, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     JFEM_HIP(hipStreamSynchronize(c->stream));
+    (void)hipFree(bad);
     (void)hipFree(counts);
     (void)hipFree(E);
     (void)hipFree(coeff);
+    if (bad_h) {
+        mat_free(A);
+        JFEM_CHECK(false, JFEM_ERR_STATE, "assembled rows differ from their reserved lengths");
+    }
     A->lanes = choose_lanes(n > 0 ? (double)nnz / n : 1.0);
     mat_build_blocks(A);
     return A;
