@@ -347,7 +347,7 @@ def main():
         "spmv_fp64_ms": spmv_ms, "spmv_fp64_gbs": spmv_gbs, "spmv_fp32_ms": spmv32_ms, "spmv_fp32_gbs": spmv32_gbs,
         "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(args.cells),
-                     "kernel": "k_stream<double,double,EpiStore<double>> (finest-level CSR SpMV)",
+                     "kernel": "k_stream<4096,double,double,EpiStore<double>> (finest-level CSR SpMV)",
                      "bytes_per_launch": spmv_bytes, "copy_kernel_gbs": copy_gbs},
         "cpu_baseline": cpu,
     }
