@@ -106,6 +106,15 @@ def copy_calibration(jf, n=1 << 27, reps=20):
     return 16 * n / (ms.value * 1e-3) / 1e9, ms.value
 
 
+def read_calibration(jf, n=700 << 20, reps=20):
+    """streaming read of n doubles (5.9 GB at the default, the size of the SpMV's streams)"""
+    import ctypes as C
+    from jfem_amd import _native as N
+    ms = C.c_double()
+    N.check(N.lib().jfem_read_bench(jf.context().h, n, reps, C.byref(ms)))
+    return 8 * n / (ms.value * 1e-3) / 1e9, ms.value
+
+
 def spmv_only(jf, cells):
     s = jf.ContinuousTPFESpace([0.0] * 3, [1.0] * 3, [cells] * 3)
     s.assembleCells()
@@ -307,7 +316,8 @@ def main():
     spmv32_gbs = (8 * nnz + 4 * (n + 1) + 8 * n) / (spmv32_ms * 1e-3) / 1e9
     log(f"SpMV fp64 {spmv_ms:.3f} ms {spmv_gbs:.0f} GB/s; fp32 {spmv32_ms:.3f} ms {spmv32_gbs:.0f} GB/s")
     copy_gbs, _ = copy_calibration(jf)
-    log(f"copy calibration {copy_gbs:.0f} GB/s")
+    read_gbs, _ = read_calibration(jf)
+    log(f"copy calibration {copy_gbs:.0f} GB/s, streaming read {read_gbs:.0f} GB/s")
 
     extra = {}
     if not args.no_fp32:
@@ -348,7 +358,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(args.cells),
                      "kernel": "k_stream<4096,double,double,EpiStore<double>> (finest-level CSR SpMV)",
-                     "bytes_per_launch": spmv_bytes, "copy_kernel_gbs": copy_gbs},
+                     "bytes_per_launch": spmv_bytes, "copy_kernel_gbs": copy_gbs,
+                     "read_kernel_gbs": read_gbs, "frac_of_read_kernel": spmv_gbs / read_gbs},
         "cpu_baseline": cpu,
     }
     line.update(extra)

@@ -66,6 +66,10 @@ int jfem_timer_stop(jfem_ctx *ctx, double *ms);
 /* HBM calibration (no reference counterpart): mean time of a plain copy of n
  * doubles (8n bytes read + 8n written) over `reps` launches */
 int jfem_copy_bench(jfem_ctx *ctx, long n, int reps, double *ms);
+/* calibration (not in the reference): average time of a streaming read of n
+ * doubles (16-byte loads, 16 KiB per workgroup), for the achievable read
+ * bandwidth that the SpMV roofline is compared with */
+int jfem_read_bench(jfem_ctx *ctx, long n, int reps, double *ms);
 
 /* ---- space: new ContinuousTPFESpace(start, end, cells) + assembleFunctions(k)
  *      (M/tensorproduct/ContinuousTPFESpace.java:17-50) -------------------- */

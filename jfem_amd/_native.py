@@ -68,6 +68,7 @@ _SIGS = {
     "jfem_timer_start": (C.c_int, [_vp]),
     "jfem_timer_stop": (C.c_int, [_vp, _dp]),
     "jfem_copy_bench": (C.c_int, [_vp, C.c_long, C.c_int, _dp]),
+    "jfem_read_bench": (C.c_int, [_vp, C.c_long, C.c_int, _dp]),
     "jfem_space_create": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp, C.c_int, C.POINTER(_vp)]),
     "jfem_space_destroy": (C.c_int, [_vp]),
     "jfem_space_ndofs": (C.c_long, [_vp]),

@@ -34,6 +34,24 @@ __global__ __launch_bounds__(256) void k_copy(long n, const double* __restrict__
     if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) b[n - 1] = a[n - 1];
 }
 
+// streaming read of n doubles (calibration of the achievable read bandwidth):
+// each workgroup reads one 16 KiB chunk with four 16-byte loads per thread; the
+// sum is stored only if it equals a value no input here produces, so the loads
+// cannot be removed
+__global__ __launch_bounds__(256) void k_read(long n, const double* __restrict__ a, double* __restrict__ sink)
+{
+    const long n2 = n / 2;
+    const double2* __restrict__ a2 = reinterpret_cast<const double2*>(a);
+    const long base = (long)blockIdx.x * 1024 + threadIdx.x;
+    double acc = 0;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const long i = base + u * 256;
+        if (i < n2) { const double2 v = a2[i]; acc += v.x + v.y; }
+    }
+    if (acc == -1.2345e300) sink[0] = acc;
+}
+
 __global__ void k_axpby(long n, double a, const double* __restrict__ x, double b, double* __restrict__ y)
 {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -323,6 +341,34 @@ int jfem_copy_bench(jfem_ctx* c, long n, int reps, double* ms)
     (void)hipEventDestroy(e1);
     (void)hipFree(a);
     (void)hipFree(b);
+    API_END
+}
+
+int jfem_read_bench(jfem_ctx* c, long n, int reps, double* ms)
+{
+    API_BEGIN
+    JFEM_CHECK(c && ms && n > 0 && reps > 0, JFEM_ERR_ARG, "bad argument");
+    double *a = nullptr, *sink = nullptr;
+    JFEM_HIP(hipMalloc(&a, sizeof(double) * n));
+    JFEM_HIP(hipMalloc(&sink, sizeof(double)));
+    vec_fill(c, a, 1.0, n);
+    hipEvent_t e0, e1;
+    JFEM_HIP(hipEventCreate(&e0));
+    JFEM_HIP(hipEventCreate(&e1));
+    const long g = (n / 2 + 1023) / 1024;
+    JFEM_CHECK(g < (1l << 31), JFEM_ERR_ARG, "n too large");
+    k_read<<<(unsigned)g, 256, 0, c->stream>>>(n, a, sink);
+    JFEM_HIP(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; r++) k_read<<<(unsigned)g, 256, 0, c->stream>>>(n, a, sink);
+    JFEM_HIP(hipEventRecord(e1, c->stream));
+    JFEM_HIP(hipEventSynchronize(e1));
+    float f = 0;
+    JFEM_HIP(hipEventElapsedTime(&f, e0, e1));
+    *ms = f / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(sink);
     API_END
 }
 

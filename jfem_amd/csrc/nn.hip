@@ -37,38 +37,70 @@ __host__ __device__ inline float bf2f(unsigned short h)
     return f;
 }
 
-// neighbour of pixel p at tap t (offsets -1/0/+1 per axis, axis 0 fastest in t);
-// sign = +1 for the forward conv, -1 for its transpose; -1 if outside
-__device__ inline long lat_nb(const Lat& L, long p, int t, int sign)
+// lattice coordinates of pixel p (32-bit arithmetic: np < 2^31 is checked in
+// make_lat); unused axes have extent 1 and coordinate 0
+__device__ inline void lat_xyz(const Lat& L, long p, int* i)
 {
-    long q = p;
-    long stride = 1;
-    long r = p;
-    for (int a = 0; a < L.d; a++) {
-        const int ia = (int)(r % L.n[a]);
-        r /= L.n[a];
-        const int o = sign * ((t % 3) - 1);
-        t /= 3;
-        const int j = ia + o;
-        if (j < 0 || j >= L.n[a]) return -1;
-        q += (long)o * stride;
-        stride *= L.n[a];
-    }
-    return q;
+    const unsigned u = (unsigned)p;
+    const unsigned r = u / (unsigned)L.n[0];
+    i[0] = (int)(u - r * (unsigned)L.n[0]);
+    i[1] = (int)(r % (unsigned)L.n[1]);
+    i[2] = (int)(r / (unsigned)L.n[1]);
+}
+
+// neighbour of pixel p (coordinates i) at tap t: offset ((t / 3^a) % 3) - 1 on
+// axis a (axis 0 fastest in t), times sign (+1 forward conv, -1 its transpose);
+// -1 if outside the lattice
+__device__ inline long lat_nb(const Lat& L, long p, const int* i, int t, int sign)
+{
+    const int o0 = sign * (t % 3 - 1);
+    const int o1 = L.d > 1 ? sign * ((t / 3) % 3 - 1) : 0;
+    const int o2 = L.d > 2 ? sign * ((t / 9) % 3 - 1) : 0;
+    if ((unsigned)(i[0] + o0) >= (unsigned)L.n[0] || (unsigned)(i[1] + o1) >= (unsigned)L.n[1] ||
+        (unsigned)(i[2] + o2) >= (unsigned)L.n[2])
+        return -1;
+    return p + o0 + (long)L.n[0] * (o1 + (long)L.n[1] * o2);
 }
 
 // ------------------------------------------------------------- thin layers
-// first layer: out[p][co] = relu(b0[co] + sum_t sum_ci W0[t][co][ci] in[nb][ci])
-__global__ __launch_bounds__(256) void k_first(Lat L, int cin, int taps, const float* __restrict__ in,
+// 32 bf16 values of one pixel as four 16-byte stores / loads
+__device__ inline void store_bf16x32(unsigned short* dst, const float* v)
+{
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        bf16x8 h;
+#pragma unroll
+        for (int e = 0; e < 8; e++) h[e] = (short)f2bf(v[8 * g + e]);
+        *reinterpret_cast<bf16x8*>(dst + 8 * g) = h;
+    }
+}
+__device__ inline void load_bf16x32(const unsigned short* src, float* v)
+{
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        const bf16x8 h = *reinterpret_cast<const bf16x8*>(src + 8 * g);
+#pragma unroll
+        for (int e = 0; e < 8; e++) v[8 * g + e] = bf2f((unsigned short)h[e]);
+    }
+}
+
+// first layer: out[p][co] = relu(b0[co] + sum_t sum_ci W0[t][co][ci] in[nb][ci]).
+// TAPS = CIN = 0: extents read at run time; otherwise compile-time.  The weight
+// indices are the same in every lane, so the weights come through scalar loads.
+template <int TAPS, int CIN>
+__global__ __launch_bounds__(256) void k_first(Lat L, int cin_rt, int taps_rt, const float* __restrict__ in,
                                                const float* __restrict__ W0, const float* __restrict__ b0,
                                                unsigned short* __restrict__ out)
 {
+    const int taps = TAPS ? TAPS : taps_rt, cin = CIN ? CIN : cin_rt;
     for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < L.np; p += (long)gridDim.x * blockDim.x) {
+        int ip[3];
+        lat_xyz(L, p, ip);
         float acc[NN_H];
 #pragma unroll
         for (int c = 0; c < NN_H; c++) acc[c] = b0[c];
         for (int t = 0; t < taps; t++) {
-            const long q = lat_nb(L, p, t, 1);
+            const long q = lat_nb(L, p, ip, t, 1);
             if (q < 0) continue;
             for (int ci = 0; ci < cin; ci++) {
                 const float v = in[q * cin + ci];
@@ -76,25 +108,30 @@ __global__ __launch_bounds__(256) void k_first(Lat L, int cin, int taps, const f
                 for (int c = 0; c < NN_H; c++) acc[c] += W0[(t * NN_H + c) * cin + ci] * v;
             }
         }
-        unsigned short* o = out + p * NN_H;
 #pragma unroll
-        for (int c = 0; c < NN_H; c++) o[c] = f2bf(acc[c] > 0.f ? acc[c] : 0.f);
+        for (int c = 0; c < NN_H; c++) acc[c] = acc[c] > 0.f ? acc[c] : 0.f;
+        store_bf16x32(out + p * NN_H, acc);
     }
 }
 
 // last layer: out[p] = b3 + sum_t sum_ci W3[t][ci] act[nb][ci]
-__global__ __launch_bounds__(256) void k_last(Lat L, int taps, const unsigned short* __restrict__ act,
+template <int TAPS>
+__global__ __launch_bounds__(256) void k_last(Lat L, int taps_rt, const unsigned short* __restrict__ act,
                                               const float* __restrict__ W3, const float* __restrict__ b3,
                                               float* __restrict__ out)
 {
+    const int taps = TAPS ? TAPS : taps_rt;
     for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < L.np; p += (long)gridDim.x * blockDim.x) {
+        int ip[3];
+        lat_xyz(L, p, ip);
         float acc = b3[0];
         for (int t = 0; t < taps; t++) {
-            const long q = lat_nb(L, p, t, 1);
+            const long q = lat_nb(L, p, ip, t, 1);
             if (q < 0) continue;
-            const unsigned short* a = act + q * NN_H;
+            float a[NN_H];
+            load_bf16x32(act + q * NN_H, a);
 #pragma unroll
-            for (int c = 0; c < NN_H; c++) acc += W3[t * NN_H + c] * bf2f(a[c]);
+            for (int c = 0; c < NN_H; c++) acc += W3[t * NN_H + c] * a[c];
         }
         out[p] = acc;
     }
@@ -118,11 +155,13 @@ __global__ __launch_bounds__(256) void k_mid(Lat L, int taps, const unsigned sho
     for (long tile = blockIdx.x * 4L + wave; tile < ntiles; tile += (long)gridDim.x * 4) {
         const long p = tile * 32 + j;
         const bool valid = p < L.np;
+        int ip[3] = {0, 0, 0};
+        if (valid) lat_xyz(L, p, ip);
         f32x16 acc;
 #pragma unroll
         for (int r = 0; r < 16; r++) acc[r] = 0.f;
         for (int t = 0; t < taps; t++) {
-            const long q = valid ? lat_nb(L, p, t, MODE == 0 ? 1 : -1) : -1;
+            const long q = valid ? lat_nb(L, p, ip, t, MODE == 0 ? 1 : -1) : -1;
 #pragma unroll
             for (int kb = 0; kb < 2; kb++) {
                 const int c0 = kb * 16 + kh * 8;
@@ -156,62 +195,144 @@ __global__ __launch_bounds__(256) void k_mid(Lat L, int taps, const unsigned sho
     }
 }
 
-// weight gradient of a 32->32 layer on MFMA:
-// dW[t][co][ci] = sum_p gout[p][co] * in[nb(p,t)][ci]; each block covers a
-// chunk of pixels and writes its partial (taps*32*32 floats); waves split taps.
-__global__ __launch_bounds__(256) void k_mid_dw(Lat L, int taps, long chunk, const unsigned short* __restrict__ gout,
-                                                const unsigned short* __restrict__ in, float* __restrict__ partials)
+// Weight gradient of a 32 -> 32 layer on MFMA:
+//   dW[t][co][ci] = sum_p G[p][co] * X[nb(p,t)][ci]
+// (G, X: NHWC bf16, 32 channels).  The pixels are the reduction dimension, so a
+// block stages tiles of 64 pixels transposed in LDS (channel-major): G^T once
+// for all its waves, and X_t^T per wave and tap.  Wave w accumulates the taps
+// t = w, w+4, ...; each block covers a contiguous pixel range and writes its
+// partial dW (taps x 32 x 32 floats).  The thin first and last layers use the
+// same kernel on channel-padded bf16 copies of their input / output gradient.
+constexpr int DW_TP = 64;   // pixels per LDS tile
+
+template <int TAPS>
+__global__ __launch_bounds__(256) void k_conv_dw(Lat L, long chunk, const unsigned short* __restrict__ G,
+                                                 const unsigned short* __restrict__ X, float* __restrict__ partials)
 {
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int i = lane & 31;   // A row (co) / B column (ci)
-    const int kh = lane >> 5;
+    constexpr int TW = (TAPS + 3) / 4;   // taps per wave
+    __shared__ __attribute__((aligned(16))) unsigned short gt[NN_H][DW_TP];
+    __shared__ __attribute__((aligned(16))) unsigned short xt[4][NN_H][DW_TP];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i = lane & 31, kh = lane >> 5;
     const long p_begin = blockIdx.x * chunk;
     const long p_end = min(p_begin + chunk, L.np);
-    for (int t = wave; t < taps; t += 4) {
-        f32x16 acc;
+    f32x16 acc[TW];
 #pragma unroll
-        for (int r = 0; r < 16; r++) acc[r] = 0.f;
-        for (long pb = p_begin; pb < p_end; pb += 16) {
-            bf16x8 a, b;
+    for (int tt = 0; tt < TW; tt++)
 #pragma unroll
-            for (int e = 0; e < 8; e++) {
-                const long p = pb + kh * 8 + e;
-                unsigned short av = 0, bv = 0;
-                if (p < p_end) {
-                    av = gout[p * NN_H + i];
-                    const long q = lat_nb(L, p, t, 1);
-                    if (q >= 0) bv = in[q * NN_H + i];
-                }
-                a[e] = (short)av;
-                b[e] = (short)bv;
+        for (int r = 0; r < 16; r++) acc[tt][r] = 0.f;
+    for (long pb = p_begin; pb < p_end; pb += DW_TP) {
+        // G tile: thread -> (pixel tid / 4, channels 8 * (tid % 4) .. +7), stored transposed
+        {
+            const int px = tid >> 2, c0 = (tid & 3) * 8;
+            const long p = pb + px;
+            bf16x8 v;
+            if (p < p_end) v = *reinterpret_cast<const bf16x8*>(G + p * NN_H + c0);
+            else {
+#pragma unroll
+                for (int e = 0; e < 8; e++) v[e] = 0;
             }
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+#pragma unroll
+            for (int e = 0; e < 8; e++) gt[c0 + e][px] = (unsigned short)v[e];
         }
-        float* out = partials + (long)blockIdx.x * taps * NN_H * NN_H + (long)t * NN_H * NN_H;
+        __syncthreads();
+#pragma unroll
+        for (int tt = 0; tt < TW; tt++) {
+            const int t = wave + 4 * tt;
+            if (t < TAPS) {
+                // X_t tile: lane -> pixel lane (64 pixels), all 32 channels, transposed
+                const long p = pb + lane;
+                long q = -1;
+                if (p < p_end) {
+                    int ip[3];
+                    lat_xyz(L, p, ip);
+                    q = lat_nb(L, p, ip, t, 1);
+                }
+#pragma unroll
+                for (int c0 = 0; c0 < NN_H; c0 += 8) {
+                    bf16x8 v;
+                    if (q >= 0) v = *reinterpret_cast<const bf16x8*>(X + q * NN_H + c0);
+                    else {
+#pragma unroll
+                        for (int e = 0; e < 8; e++) v[e] = 0;
+                    }
+#pragma unroll
+                    for (int e = 0; e < 8; e++) xt[wave][c0 + e][lane] = (unsigned short)v[e];
+                }
+                __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0): the wave's LDS writes are done
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int ks = 0; ks < DW_TP / 16; ks++) {
+                    const bf16x8 a = *reinterpret_cast<const bf16x8*>(&gt[i][ks * 16 + kh * 8]);
+                    const bf16x8 b = *reinterpret_cast<const bf16x8*>(&xt[wave][i][ks * 16 + kh * 8]);
+                    acc[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[tt], 0, 0, 0);
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int tt = 0; tt < TW; tt++) {
+        const int t = wave + 4 * tt;
+        if (t >= TAPS) continue;
+        float* out = partials + ((long)blockIdx.x * TAPS + t) * NN_H * NN_H;
 #pragma unroll
         for (int r = 0; r < 16; r++) {
             const int co = 8 * (r / 4) + 4 * kh + (r % 4);
-            out[co * NN_H + i] = acc[r];
+            out[co * NN_H + i] = acc[tt][r];
         }
     }
 }
 
-// bias gradient partials: db[c] = sum_p g[p][c]
+// sum of the per-block dW partials in a fixed block order, written (add = 0) or
+// added into grad[t * st + co * sco + ci] for co < nco, ci < nci
+__global__ void k_sum_dw(int nblocks, int taps, const float* __restrict__ partials, float* __restrict__ grad, int nco,
+                         int nci, long st, long sco, int add)
+{
+    const int n = taps * nco * nci;
+    for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
+        const int t = o / (nco * nci), co = (o / nci) % nco, ci = o % nci;
+        const long src = ((long)t * NN_H + co) * NN_H + ci;
+        float s = 0;
+        for (int b = 0; b < nblocks; b++) s += partials[(long)b * taps * NN_H * NN_H + src];
+        float* dst = grad + t * st + co * sco + ci;
+        *dst = add ? *dst + s : s;
+    }
+}
+
+// channel-padded bf16 copies for the thin layers' weight gradients:
+// dst[p][c] = c < n ? src[p * n + c] : 0
+__global__ void k_pad_bf16(long np, int n, const float* __restrict__ src, unsigned short* __restrict__ dst)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < np * NN_H; i += (long)gridDim.x * blockDim.x) {
+        const long p = i / NN_H;
+        const int c = (int)(i % NN_H);
+        dst[i] = c < n ? f2bf(src[p * n + c]) : (unsigned short)0;
+    }
+}
+
+// bias gradient partials: db[c] = sum_p g[p][c]; thread -> (pixel, 8 channels),
+// 16-byte loads, per-thread sums over the chunk, then a fixed-order LDS reduction
 __global__ __launch_bounds__(256) void k_bias_grad(long np, long chunk, const unsigned short* __restrict__ g,
                                                    float* __restrict__ partials)
 {
-    __shared__ float sh[256];
-    const int c = threadIdx.x & 31, sub = threadIdx.x >> 5;   // 8 pixel lanes per channel
+    __shared__ float sh[64][NN_H + 1];
+    const int c0 = (threadIdx.x & 3) * 8, px = threadIdx.x >> 2;
     const long p0 = blockIdx.x * chunk, p1 = min(p0 + chunk, np);
-    float acc = 0;
-    for (long p = p0 + sub; p < p1; p += 8) acc += bf2f(g[p * NN_H + c]);
-    sh[threadIdx.x] = acc;
+    float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (long p = p0 + px; p < p1; p += 64) {
+        const bf16x8 h = *reinterpret_cast<const bf16x8*>(g + p * NN_H + c0);
+#pragma unroll
+        for (int e = 0; e < 8; e++) acc[e] += bf2f((unsigned short)h[e]);
+    }
+#pragma unroll
+    for (int e = 0; e < 8; e++) sh[px][c0 + e] = acc[e];
     __syncthreads();
-    if (threadIdx.x < 32) {
-        float s = 0;
-        for (int k = 0; k < 8; k++) s += sh[k * 32 + threadIdx.x];
-        partials[blockIdx.x * NN_H + threadIdx.x] = s;
+    if (threadIdx.x < NN_H) {
+        float s2 = 0;
+        for (int k = 0; k < 64; k++) s2 += sh[k][threadIdx.x];
+        partials[blockIdx.x * NN_H + threadIdx.x] = s2;
     }
 }
 
@@ -221,70 +342,55 @@ __global__ __launch_bounds__(256) void k_last_dx(Lat L, int taps, const float* _
                                                  unsigned short* __restrict__ g2)
 {
     for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < L.np; p += (long)gridDim.x * blockDim.x) {
+        int ip[3];
+        lat_xyz(L, p, ip);
         float acc[NN_H];
 #pragma unroll
         for (int c = 0; c < NN_H; c++) acc[c] = 0;
         for (int t = 0; t < taps; t++) {
-            const long q = lat_nb(L, p, t, -1);
+            const long q = lat_nb(L, p, ip, t, -1);
             if (q < 0) continue;
             const float g = gout[q];
 #pragma unroll
             for (int c = 0; c < NN_H; c++) acc[c] += W3[t * NN_H + c] * g;
         }
-        const unsigned short* a = act2 + p * NN_H;
+        float a[NN_H];
+        load_bf16x32(act2 + p * NN_H, a);
 #pragma unroll
-        for (int c = 0; c < NN_H; c++) g2[p * NN_H + c] = f2bf(bf2f(a[c]) > 0.f ? acc[c] : 0.f);
+        for (int c = 0; c < NN_H; c++) acc[c] = a[c] > 0.f ? acc[c] : 0.f;
+        store_bf16x32(g2 + p * NN_H, acc);
     }
 }
 
-// weight gradients of the thin layers, per-block partials:
-// last:  dW3[t][ci] = sum_p gout[p] act2[nb(p,t)][ci],  db3 = sum_p gout[p]
-// first: dW0[t][co][ci] = sum_p g0[p][co] in[nb(p,t)][ci]
-// one thread per output weight, pixels of the block's chunk staged in LDS
-__global__ __launch_bounds__(256) void k_last_dw(Lat L, int taps, long chunk, const float* __restrict__ gout,
-                                                 const unsigned short* __restrict__ act2, float* __restrict__ partials)
+// partials of sum_p g[p] over 4096-pixel chunks (one per block)
+__global__ __launch_bounds__(256) void k_sum_chunks(long np, long chunk, const float* __restrict__ g,
+                                                    float* __restrict__ partials)
 {
-    const int nout = taps * NN_H + 1;
-    const long p0 = blockIdx.x * chunk, p1 = min(p0 + chunk, L.np);
-    for (int o = threadIdx.x; o < nout; o += blockDim.x) {
-        float acc = 0;
-        if (o == nout - 1) {
-            for (long p = p0; p < p1; p++) acc += gout[p];
-        } else {
-            const int t = o / NN_H, ci = o % NN_H;
-            for (long p = p0; p < p1; p++) {
-                const long q = lat_nb(L, p, t, 1);
-                if (q >= 0) acc += gout[p] * bf2f(act2[q * NN_H + ci]);
-            }
-        }
-        partials[blockIdx.x * (long)nout + o] = acc;
-    }
+    __shared__ float sh[4];
+    const long p0 = blockIdx.x * chunk, p1 = min(p0 + chunk, np);
+    float s = 0;
+    for (long p = p0 + threadIdx.x; p < p1; p += 256) s += g[p];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
-__global__ __launch_bounds__(256) void k_first_dw(Lat L, int taps, int cin, long chunk,
-                                                  const unsigned short* __restrict__ g0, const float* __restrict__ in,
-                                                  float* __restrict__ partials)
+// out[o] (+)= sum_b partials[b][o]: one 256-thread block per output, strided
+// per-thread sums then a fixed-order tree (deterministic for a given nblocks)
+__global__ __launch_bounds__(256) void k_sum_blocks(int nblocks, int nout, const float* __restrict__ partials,
+                                                    float* __restrict__ out, int add)
 {
-    const int nout = taps * NN_H * cin;
-    const long p0 = blockIdx.x * chunk, p1 = min(p0 + chunk, L.np);
-    for (int o = threadIdx.x; o < nout; o += blockDim.x) {
-        const int t = o / (NN_H * cin), co = (o / cin) % NN_H, ci = o % cin;
-        float acc = 0;
-        for (long p = p0; p < p1; p++) {
-            const long q = lat_nb(L, p, t, 1);
-            if (q >= 0) acc += bf2f(g0[p * NN_H + co]) * in[q * cin + ci];
-        }
-        partials[blockIdx.x * (long)nout + o] = acc;
-    }
-}
-
-// out[o] (+)= sum_b partials[b][o]   (fixed order over blocks)
-__global__ void k_sum_blocks(int nblocks, int nout, const float* __restrict__ partials, float* __restrict__ out, int add)
-{
-    for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < nout; o += gridDim.x * blockDim.x) {
-        float s = 0;
-        for (int b = 0; b < nblocks; b++) s += partials[(long)b * nout + o];
-        out[o] = add ? out[o] + s : s;
+    __shared__ float sh[4];
+    const int o = blockIdx.x;
+    float s = 0;
+    for (int b = threadIdx.x; b < nblocks; b += 256) s += partials[(long)b * nout + o];
+    for (int k = 32; k > 0; k >>= 1) s += __shfl_xor(s, k, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+        out[o] = add ? out[o] + r : r;
     }
 }
 
@@ -376,10 +482,25 @@ void nn_forward(jfem_nn* nn, const Lat& L)
     hipStream_t st = nn->ctx->stream;
     const long ntiles = (L.np + 31) / 32;
     const int gm = (int)std::min<long>((ntiles + 3) / 4, 16384);
-    k_first<<<nn_grid(L.np), 256, 0, st>>>(L, nn->cin, nn->taps, nn->in, nn->w + o[0], nn->w + o[1], nn->act0);
+    const int g = nn_grid(L.np);
+    const float *W0 = nn->w + o[0], *b0 = nn->w + o[1];
+    const int tc = nn->taps * 10 + nn->cin;
+    switch (tc) {
+    case 91: k_first<9, 1><<<g, 256, 0, st>>>(L, 1, 9, nn->in, W0, b0, nn->act0); break;
+    case 92: k_first<9, 2><<<g, 256, 0, st>>>(L, 2, 9, nn->in, W0, b0, nn->act0); break;
+    case 93: k_first<9, 3><<<g, 256, 0, st>>>(L, 3, 9, nn->in, W0, b0, nn->act0); break;
+    case 94: k_first<9, 4><<<g, 256, 0, st>>>(L, 4, 9, nn->in, W0, b0, nn->act0); break;
+    case 271: k_first<27, 1><<<g, 256, 0, st>>>(L, 1, 27, nn->in, W0, b0, nn->act0); break;
+    case 272: k_first<27, 2><<<g, 256, 0, st>>>(L, 2, 27, nn->in, W0, b0, nn->act0); break;
+    case 274: k_first<27, 4><<<g, 256, 0, st>>>(L, 4, 27, nn->in, W0, b0, nn->act0); break;
+    case 275: k_first<27, 5><<<g, 256, 0, st>>>(L, 5, 27, nn->in, W0, b0, nn->act0); break;
+    default: k_first<0, 0><<<g, 256, 0, st>>>(L, nn->cin, nn->taps, nn->in, W0, b0, nn->act0); break;
+    }
     k_mid<0><<<gm, 256, 0, st>>>(L, nn->taps, nn->act0, nn->wh1, nn->w + o[3], nullptr, nn->act1);
     k_mid<0><<<gm, 256, 0, st>>>(L, nn->taps, nn->act1, nn->wh2, nn->w + o[5], nullptr, nn->act2);
-    k_last<<<nn_grid(L.np), 256, 0, st>>>(L, nn->taps, nn->act2, nn->w + o[6], nn->w + o[7], nn->out);
+    if (nn->taps == 27) k_last<27><<<g, 256, 0, st>>>(L, 27, nn->act2, nn->w + o[6], nn->w + o[7], nn->out);
+    else if (nn->taps == 9) k_last<9><<<g, 256, 0, st>>>(L, 9, nn->act2, nn->w + o[6], nn->w + o[7], nn->out);
+    else k_last<0><<<g, 256, 0, st>>>(L, nn->taps, nn->act2, nn->w + o[6], nn->w + o[7], nn->out);
     JFEM_HIP(hipGetLastError());
 }
 
@@ -391,9 +512,14 @@ void nn_backward(jfem_nn* nn, const Lat& L, int add)
     nn_offsets(nn, o);
     hipStream_t st = nn->ctx->stream;
     const int taps = nn->taps;
-    const long chunk = 4096;
-    const int nb = (int)((L.np + chunk - 1) / chunk);
-    const long need = (long)nb * std::max<long>(taps * NN_H * NN_H, taps * NN_H * std::max(nn->cin, 1) + NN_H + 1);
+    // weight gradients: nbw blocks over contiguous pixel ranges (multiples of the LDS tile)
+    const long tiles = (L.np + DW_TP - 1) / DW_TP;
+    const int nbw = (int)std::min<long>(tiles, 512);
+    const long wchunk = (tiles + nbw - 1) / nbw * DW_TP;
+    // bias gradients: 4096-pixel chunks
+    const long bchunk = 4096;
+    const int nbb = (int)((L.np + bchunk - 1) / bchunk);
+    const long need = std::max<long>((long)nbw * taps * NN_H * NN_H, (long)nbb * NN_H);
     if (need > nn->part_cap) {
         (void)hipFree(nn->part);
         JFEM_HIP(hipMalloc(&nn->part, sizeof(float) * need));
@@ -401,27 +527,43 @@ void nn_backward(jfem_nn* nn, const Lat& L, int add)
     }
     const long ntiles = (L.np + 31) / 32;
     const int gm = (int)std::min<long>((ntiles + 3) / 4, 16384);
-    // layer 3 (last)
-    k_last_dw<<<nb, 256, 0, st>>>(L, taps, chunk, nn->gout, nn->act2, nn->part);
-    k_sum_blocks<<<4, 256, 0, st>>>(nb, taps * NN_H + 1, nn->part, nn->grad + o[6], add);
+    auto conv_dw = [&](const unsigned short* G, const unsigned short* X) {
+        if (taps == 27) k_conv_dw<27><<<nbw, 256, 0, st>>>(L, wchunk, G, X, nn->part);
+        else if (taps == 9) k_conv_dw<9><<<nbw, 256, 0, st>>>(L, wchunk, G, X, nn->part);
+        else k_conv_dw<3><<<nbw, 256, 0, st>>>(L, wchunk, G, X, nn->part);
+        JFEM_HIP(hipGetLastError());
+    };
+    const int padg = grid_for(L.np * NN_H, 256, 16384);
+    // layer 3 (last): dW3[t][ci] from the output gradient padded to 32 channels (in g0's buffer)
+    k_pad_bf16<<<padg, 256, 0, st>>>(L.np, 1, nn->gout, nn->g0);
+    conv_dw(nn->g0, nn->act2);
+    k_sum_dw<<<grid_for((long)taps * NN_H, 64, 4096), 64, 0, st>>>(nbw, taps, nn->part, nn->grad + o[6], 1, NN_H,
+                                                                  NN_H, 0, add);
+    k_sum_chunks<<<nbb, 256, 0, st>>>(L.np, bchunk, nn->gout, nn->part);
+    k_sum_blocks<<<1, 256, 0, st>>>(nbb, 1, nn->part, nn->grad + o[7], add);
     k_last_dx<<<nn_grid(L.np), 256, 0, st>>>(L, taps, nn->gout, nn->w + o[6], nn->act2, nn->g2);
     // layer 2: dW2 from g2 and act1, bias, input gradient masked by act1
-    k_mid_dw<<<nb, 256, 0, st>>>(L, taps, chunk, nn->g2, nn->act1, nn->part);
-    k_sum_blocks<<<64, 256, 0, st>>>(nb, taps * NN_H * NN_H, nn->part, nn->grad + o[4], add);
-    k_bias_grad<<<nb, 256, 0, st>>>(L.np, chunk, nn->g2, nn->part);
-    k_sum_blocks<<<1, 32, 0, st>>>(nb, NN_H, nn->part, nn->grad + o[5], add);
+    conv_dw(nn->g2, nn->act1);
+    k_sum_dw<<<grid_for((long)taps * NN_H * NN_H, 64, 4096), 64, 0, st>>>(nbw, taps, nn->part, nn->grad + o[4], NN_H,
+                                                                         NN_H, NN_H * NN_H, NN_H, add);
+    k_bias_grad<<<nbb, 256, 0, st>>>(L.np, bchunk, nn->g2, nn->part);
+    k_sum_blocks<<<NN_H, 256, 0, st>>>(nbb, NN_H, nn->part, nn->grad + o[5], add);
     k_mid<1><<<gm, 256, 0, st>>>(L, taps, nn->g2, nn->wt2, nullptr, nn->act1, nn->g1);
     // layer 1
-    k_mid_dw<<<nb, 256, 0, st>>>(L, taps, chunk, nn->g1, nn->act0, nn->part);
-    k_sum_blocks<<<64, 256, 0, st>>>(nb, taps * NN_H * NN_H, nn->part, nn->grad + o[2], add);
-    k_bias_grad<<<nb, 256, 0, st>>>(L.np, chunk, nn->g1, nn->part);
-    k_sum_blocks<<<1, 32, 0, st>>>(nb, NN_H, nn->part, nn->grad + o[3], add);
+    conv_dw(nn->g1, nn->act0);
+    k_sum_dw<<<grid_for((long)taps * NN_H * NN_H, 64, 4096), 64, 0, st>>>(nbw, taps, nn->part, nn->grad + o[2], NN_H,
+                                                                         NN_H, NN_H * NN_H, NN_H, add);
+    k_bias_grad<<<nbb, 256, 0, st>>>(L.np, bchunk, nn->g1, nn->part);
+    k_sum_blocks<<<NN_H, 256, 0, st>>>(nbb, NN_H, nn->part, nn->grad + o[3], add);
     k_mid<1><<<gm, 256, 0, st>>>(L, taps, nn->g1, nn->wt1, nullptr, nn->act0, nn->g0);
-    // layer 0 (first): weights and bias
-    k_first_dw<<<nb, 256, 0, st>>>(L, taps, nn->cin, chunk, nn->g0, nn->in, nn->part);
-    k_sum_blocks<<<16, 256, 0, st>>>(nb, taps * NN_H * nn->cin, nn->part, nn->grad + o[0], add);
-    k_bias_grad<<<nb, 256, 0, st>>>(L.np, chunk, nn->g0, nn->part);
-    k_sum_blocks<<<1, 32, 0, st>>>(nb, NN_H, nn->part, nn->grad + o[1], add);
+    // layer 0 (first): dW0[t][co][ci] from g0 and the input padded to 32 channels (in g2's buffer)
+    k_pad_bf16<<<padg, 256, 0, st>>>(L.np, nn->cin, nn->in, nn->g2);
+    conv_dw(nn->g0, nn->g2);
+    k_sum_dw<<<grid_for((long)taps * NN_H * nn->cin, 64, 4096), 64, 0, st>>>(nbw, taps, nn->part, nn->grad + o[0],
+                                                                            NN_H, nn->cin, (long)NN_H * nn->cin,
+                                                                            nn->cin, add);
+    k_bias_grad<<<nbb, 256, 0, st>>>(L.np, bchunk, nn->g0, nn->part);
+    k_sum_blocks<<<NN_H, 256, 0, st>>>(nbb, NN_H, nn->part, nn->grad + o[1], add);
     JFEM_HIP(hipGetLastError());
 }
 
@@ -531,8 +673,10 @@ Lat make_lat(int d, const int* n)
     L.np = 1;
     for (int a = 0; a < 3; a++) {
         L.n[a] = a < d ? n[a] : 1;
+        JFEM_CHECK(L.n[a] >= 1, JFEM_ERR_ARG, "lattice extents must be positive");
         L.np *= L.n[a];
     }
+    JFEM_CHECK(L.np < (1l << 31), JFEM_ERR_ARG, "lattice too large (2^31 nodes at most)");
     return L;
 }
 
