@@ -229,12 +229,17 @@ typedef struct {
     unsigned long long seed;
     int mode;                 /* samples: 0 = residuals of plain V-cycles, 1 = A e for white-noise
                                  errors e, 2 = alternate 0 and 1, 3 = alternate the residual at cycle
-                                 entry and the residual before finest post-smoothing */
+                                 entry and the residual before finest post-smoothing;
+                                 4 = as 3 on A x = A x* for random x* (the error is known) with the
+                                 two-grid loss below; 5 = as 4, but every second right-hand side
+                                 advances with the V-cycle that uses the learned smoother */
 } jfem_train_opts;
 /* trains the network attached to mg on (residual, error) pairs of its plain
- * V-cycles, one Adam step per sample; the loss is the energy of the error left
- * by the learned step, |e - N(r)|_A^2 - |e|_A^2 per node (input normalised);
- * loss_hist (nullable) gets it per step */
+ * V-cycles, one Adam step per sample.  Modes 0-3: the loss is the energy of the
+ * error left by the learned step, |e - N(r)|_A^2 - |e|_A^2 per node (input
+ * normalised); mode 4: the energy of the error left by the learned step followed
+ * by the coarse-grid correction, |T (e - N(r))|_A^2, T = I - P A_c^-1 P^T A, and
+ * loss_hist gets |T (e - N(r))|_A^2 / |e|_A^2 per step.  loss_hist is nullable. */
 int jfem_nn_train_mg(jfem_mg *mg, const jfem_train_opts *opts, double *loss_hist, int *nsteps);
 /* stationary multigrid iteration to ||b - A x|| <= tol (x: start iterate in, solution out) */
 int jfem_mg_solve(jfem_mg *mg, const jfem_vec *b, jfem_vec *x, double tol, int maxcycles, jfem_solve_info *info,

@@ -348,7 +348,7 @@ class ContinuousTPFESpace:
             raise N.UnsupportedOperationException(3, "all integrals must share one quadrature rule")
         kinds = i32([ci.kind for ci in cellIntegrals])
         w = f64([ci.weight for ci in cellIntegrals])
-        cc = None if cell_coeff is None else f64(cell_coeff)
+        cc = _cell_array(cell_coeff, self, "cell_coeff")
         h = C.c_void_p()
         check(N.lib().jfem_assemble(self.h, len(kinds), iptr(kinds), dptr(w), dptr(cc) if cc is not None else None,
                                     rules.pop(), C.byref(h)))
@@ -414,6 +414,18 @@ class ChebyshevSmoother:
 FP64, FP32 = 0, 1
 
 
+def _cell_array(values, space, what):
+    """float64 copy of a per-cell array, checked against the space's cell count
+    (the C ABI reads exactly that many entries)"""
+    if values is None:
+        return None
+    a = f64(values).reshape(-1)
+    nc = int(np.prod(space.cells.astype(np.int64)))
+    if a.shape[0] != nc:
+        raise N.IllegalArgumentException(1, f"{what} has {a.shape[0]} entries, the finest grid has {nc} cells")
+    return a
+
+
 class MGPreconditioner:
     """Geometric multigrid preconditioner over nested ContinuousTPFESpaces
     (MGPreconditionerSpace + MGPreconditionerInterface.mvMul,
@@ -437,7 +449,7 @@ class MGPreconditioner:
         kinds = i32([ci.kind for ci in cellIntegrals])
         w = f64([ci.weight for ci in cellIntegrals])
         arr = (C.c_void_p * len(spaces))(*[s.h for s in spaces])
-        cc = None if cell_coeff is None else f64(cell_coeff)
+        cc = _cell_array(cell_coeff, spaces[-1], "cell_coeff")
         h = C.c_void_p()
         check(N.lib().jfem_mg_create(arr, len(spaces), len(kinds), iptr(kinds), dptr(w),
                                      dptr(cc) if cc is not None else None, cellIntegrals[0].quadratureRule1D,
@@ -476,7 +488,7 @@ class MGPreconditioner:
                     step_rule: int = 0):
         """attach a SmootherNet as the finest-level smoother (None detaches); step_rule 0 plain
         (default), 1 minimal residual, 2 minimal energy along the network's direction"""
-        cc = None if cell_log_coeff is None else f64(cell_log_coeff)
+        cc = _cell_array(cell_log_coeff, self.spaces[-1], "cell_log_coeff")
         self._net = net
         check(N.lib().jfem_mg_set_learned(self.h, net.h if net is not None else None,
                                           dptr(cc) if cc is not None else None, runs, fallback_ratio, step_rule))
@@ -491,7 +503,11 @@ class MGPreconditioner:
 
     def train(self, rhs_per_field: int = 10, cycles_per_rhs: int = 10, lr: float = 1e-3, seed: int = 0, mode: int = 0):
         """train the attached network on residuals of this hierarchy's plain V-cycles
-        (mode 0), on A e for white-noise errors e (mode 1), or alternating (mode 2)"""
+        (mode 0), on A e for white-noise errors e (mode 1), alternating (mode 2),
+        on residuals at cycle entry and before post-smoothing (mode 3), or as mode 3
+        with known errors and the two-grid loss |T(e - N(r))|_A^2 (mode 4; the
+        history then holds the per-sample ratio |T(e - N(r))|_A^2 / |e|_A^2), or as
+        mode 4 with every second right-hand side advanced by the learned V-cycle (mode 5)"""
         o = N.TrainOpts(rhs_per_field, cycles_per_rhs, lr, seed, mode)
         hist = np.zeros(rhs_per_field * cycles_per_rhs)
         ns = C.c_int()

@@ -57,6 +57,10 @@ struct LearnedState {
     int mr = 0;                  // step rule: 0 plain x += N(r); 1 minimal residual; 2 minimal energy along N(r)
     int capture = 0;             // training: record the residual before finest post-smoothing
     double* capbuf = nullptr;
+    double* capx = nullptr;      // training: the iterate before finest post-smoothing
+    double *xt = nullptr, *eh = nullptr, *w = nullptr, *y = nullptr;   // two-grid training work
+    double* loss2 = nullptr;     // two-grid training: (numerator, denominator) per step
+    long loss2_cap = 0;
 };
 
 struct jfem_mg {
@@ -541,6 +545,8 @@ static void step_v(jfem_mg* G, int l, bool x_zero)
         // training: keep the residual that post-smoothing will see
         level_residual<T>(G, L, b, x, r);
         k_convert<T, double><<<grid_for(L.A->rows, 256, 16384), 256, 0, G->ctx->stream>>>(L.A->rows, r, G->ls.capbuf);
+        if (G->ls.capx)
+            k_convert<T, double><<<grid_for(L.A->rows, 256, 16384), 256, 0, G->ctx->stream>>>(L.A->rows, x, G->ls.capx);
     }
     smooth<T>(G, L, b, x, xalt, false);
     // keep the result in L.x / the spare in L.t
@@ -671,7 +677,8 @@ static void free_mg(jfem_mg* G)
     (void)hipFree(G->init); (void)hipFree(G->defect); (void)hipFree(G->tmp);
     LearnedState& S = G->ls;
     for (void* p : {(void*)S.stat, (void*)S.rr, (void*)S.u, (void*)S.z, (void*)S.gd, (void*)S.rh, (void*)S.xs,
-                    (void*)S.bs, (void*)S.loss, (void*)S.capbuf})
+                    (void*)S.bs, (void*)S.loss, (void*)S.capbuf, (void*)S.capx, (void*)S.xt, (void*)S.eh,
+                    (void*)S.w, (void*)S.y, (void*)S.loss2})
         (void)hipFree(p);
     delete G;
 }
@@ -869,6 +876,35 @@ __global__ __launch_bounds__(256) void k_energy(long n, const double* __restrict
     if (threadIdx.x == 0) partials[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
+// eh = (xt - x) * sqrt(n / ss): the error of the sample in the input's units
+__global__ void k_scaled_error(long n, const double* __restrict__ xt, const double* __restrict__ x,
+                               const double* __restrict__ rr, double* __restrict__ eh)
+{
+    const double scale = rr[0] > 0 ? sqrt((double)n / rr[0]) : 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        eh[i] = (xt[i] - x[i]) * scale;
+}
+
+// partials of (eh - u - w).y (energy of the two-grid error) and eh.rh (energy of the error)
+__global__ __launch_bounds__(256) void k_twogrid_dots(long n, const double* __restrict__ eh, const double* __restrict__ u,
+                                                      const double* __restrict__ w, const double* __restrict__ y,
+                                                      const double* __restrict__ rh, double* __restrict__ partials)
+{
+    __shared__ double sh[2][4];
+    double a = 0, b = 0;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        a += (eh[i] - u[i] - w[i]) * y[i];
+        b += eh[i] * rh[i];
+    }
+    for (int o = 32; o > 0; o >>= 1) { a += __shfl_xor(a, o, 64); b += __shfl_xor(b, o, 64); }
+    if ((threadIdx.x & 63) == 0) { sh[0][threadIdx.x >> 6] = a; sh[1][threadIdx.x >> 6] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
+        partials[RED_BLOCKS + blockIdx.x] = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+    }
+}
+
 // seeded standard-normal right-hand side (SplitMix64 + Box-Muller), zero on the boundary
 __global__ void k_rand_rhs(long n, unsigned long long seed, const unsigned char* __restrict__ bnd, double* __restrict__ b)
 {
@@ -969,6 +1005,34 @@ static void learned_smooth(jfem_mg* G, MgLevel& L, const T* b, T* x, T* u, bool 
         k_mr_update<T><<<gr, 256, 0, st>>>(n, S.rr + 2, u, x);
         JFEM_HIP(hipGetLastError());
     }
+}
+
+// w = P V_c(P^T z): the coarse-grid correction of the finest level for the
+// residual z (restriction and prolongation with zero boundary conditions, one
+// V-cycle from zero on the coarser levels), in the hierarchy's precision
+template <typename T>
+static void coarse_correction_t(jfem_mg* G, const double* z, double* w)
+{
+    const int Lf = (int)G->lev.size() - 1;
+    MgLevel& F = G->lev[Lf];
+    MgLevel& C = G->lev[Lf - 1];
+    const long n = F.A->rows;
+    hipStream_t st = G->ctx->stream;
+    const int g = grid_for(n, 256, 16384);
+    T* zt = (T*)F.r;
+    T* wt = (T*)F.dv;
+    k_convert<double, T><<<g, 256, 0, st>>>(n, z, zt);
+    restrict_to<T>(G, Lf, zt, (T*)C.b);
+    step_v<T>(G, Lf - 1, true);
+    prolong_to<T>(G, Lf, (const T*)C.x, wt, false, true);
+    k_convert<T, double><<<g, 256, 0, st>>>(n, wt, w);
+    JFEM_HIP(hipGetLastError());
+}
+
+static void coarse_correction(jfem_mg* G, const double* z, double* w)
+{
+    if (G->opt.precision == JFEM_FP32) coarse_correction_t<float>(G, z, w);
+    else coarse_correction_t<double>(G, z, w);
 }
 
 // one V-cycle x <- mgStepV(finest, x, b) on FP64 device vectors, any precision inside
@@ -1087,15 +1151,28 @@ static int train_mg(jfem_mg* G, const jfem_train_opts* o, double* loss_host)
     const long n = F.A->rows;
     const int gr = grid_for(n, 256, 16384);
     const int nsteps = o->rhs_per_field * o->cycles_per_rhs;
+    const bool twogrid = o->mode == 4 || o->mode == 5;
+    const bool onpolicy = o->mode == 5;   // odd right-hand sides advance with the learned V-cycle
+    JFEM_CHECK(!twogrid || G->lev.size() >= 2, JFEM_ERR_STATE, "two-grid training needs at least two levels");
     for (double** p : {&S.u, &S.z, &S.gd, &S.rh, &S.xs, &S.bs, &S.capbuf}) {
         (void)hipFree(*p);
         *p = nullptr;
         JFEM_HIP(hipMalloc(p, sizeof(double) * n));
     }
+    for (double** p : {&S.capx, &S.xt, &S.eh, &S.w, &S.y}) {
+        (void)hipFree(*p);
+        *p = nullptr;
+        if (twogrid) JFEM_HIP(hipMalloc(p, sizeof(double) * n));
+    }
     if (nsteps > S.loss_cap) {
         (void)hipFree(S.loss);
         JFEM_HIP(hipMalloc(&S.loss, sizeof(double) * nsteps));
         S.loss_cap = nsteps;
+    }
+    if (twogrid && 2L * nsteps > S.loss2_cap) {
+        (void)hipFree(S.loss2);
+        JFEM_HIP(hipMalloc(&S.loss2, sizeof(double) * 2 * nsteps));
+        S.loss2_cap = 2L * nsteps;
     }
     const int was_active = S.active;
     S.active = 0;   // data come from plain V-cycles
@@ -1111,14 +1188,16 @@ static int train_mg(jfem_mg* G, const jfem_train_opts* o, double* loss_host)
             k_rand_rhs<<<grid_for(C.A->rows, 256, 16384), 256, 0, st>>>(C.A->rows, sk, C.bnd, (double*)C.t);
             for (int l = ls + 1; l <= Lf; l++)
                 prolong_to<double>(G, l, (const double*)G->lev[l - 1].t, (double*)G->lev[l].t, false, true);
-            vec_copy(c, S.bs, (const double*)F.t, n);
+            vec_copy(c, twogrid ? S.xt : S.bs, (const double*)F.t, n);
         } else {
-            k_rand_rhs<<<gr, 256, 0, st>>>(n, sk, F.bnd, S.bs);
+            k_rand_rhs<<<gr, 256, 0, st>>>(n, sk, F.bnd, twogrid ? S.xt : S.bs);
         }
+        // two-grid training: the random vector is the exact solution x*, b = A x*
+        if (twogrid) spmv(F.A, S.xt, S.bs);
         vec_fill(c, S.xs, 0.0, n);
         for (int cyc = 0; cyc < o->cycles_per_rhs; cyc++) {
             const bool noise = o->mode == 1 || (o->mode == 2 && (step & 1));
-            const bool captured = o->mode == 3 && (step & 1);
+            const bool captured = (o->mode == 3 || twogrid) && (step & 1);
             if (captured) {
                 vec_copy(c, r, S.capbuf, n);
             } else if (noise) {
@@ -1134,23 +1213,35 @@ static int train_mg(jfem_mg* G, const jfem_train_opts* o, double* loss_host)
             nn_forward(S.nn, S.lat);
             k_train_unpack<<<gr, 256, 0, st>>>(to_dev(F.grid), S.lat, n, S.nn->out, r, S.rr, F.bnd, S.u, S.rh);
             launch_stream<double, double>(F.A, F.A->vals, S.u, EpiResidual<double>{S.rh, S.z}, nullptr);   // z = rh - A u
-            {
+            const int blocks = grid_for(n, 256, RED_BLOCKS);
+            if (twogrid) {
+                // two-grid loss: energy of e2 = T (e - u), T = I - P A_c^-1 P^T A the coarse-grid
+                // correction (A_c^-1 approximated by one V-cycle); with T an A-orthogonal
+                // projector the gradient is -2 A e2 = -2 y, y = z - A w, w = P A_c^-1 P^T z
+                k_scaled_error<<<gr, 256, 0, st>>>(n, S.xt, captured ? S.capx : S.xs, S.rr, S.eh);
+                coarse_correction(G, S.z, S.w);
+                launch_stream<double, double>(F.A, F.A->vals, S.w, EpiResidual<double>{S.z, S.y}, nullptr);
+                k_twogrid_dots<<<blocks, 256, 0, st>>>(n, S.eh, S.u, S.w, S.y, S.rh, c->red_partials);
+                reduce_partials_dev(c, 2, blocks, S.loss2 + 2 * step);
+                k_train_gout<<<gr, 256, 0, st>>>(to_dev(F.grid), S.lat, n, S.y, F.bnd, S.nn->gout);
+            } else {
                 // energy loss |e - u|_A^2 - |e|_A^2 = u.A u - 2 u.rh = -u.(z + rh), gradient 2 (A u - rh) = -2 z
-                const int blocks = grid_for(n, 256, RED_BLOCKS);
                 k_energy<<<blocks, 256, 0, st>>>(n, S.u, S.z, S.rh, c->red_partials);
                 reduce_partials_dev(c, 1, blocks, S.loss + step);
+                k_train_gout<<<gr, 256, 0, st>>>(to_dev(F.grid), S.lat, n, S.z, F.bnd, S.nn->gout);
             }
-            k_train_gout<<<gr, 256, 0, st>>>(to_dev(F.grid), S.lat, n, S.z, F.bnd, S.nn->gout);
             JFEM_HIP(hipGetLastError());
             nn_backward(S.nn, S.lat, 0);
             nn_adam_step(S.nn, (float)o->lr, 1.0f);
             step++;
-            if (o->mode == 3) {
+            if (o->mode == 3 || twogrid) {
                 // even steps train on the cycle-entry residual; the V-cycle that
                 // follows records the residual before post-smoothing for the odd step
                 if (!captured) {
                     S.capture = 1;
+                    S.active = onpolicy && (k & 1) ? 1 : 0;
                     vcycle_dev(G, S.xs, S.bs);
+                    S.active = 0;
                     S.capture = 0;
                 }
             } else if (!noise && cyc + 1 < o->cycles_per_rhs) {
@@ -1158,11 +1249,20 @@ static int train_mg(jfem_mg* G, const jfem_train_opts* o, double* loss_host)
             }
         }
     }
-    std::vector<double> h(step);
-    JFEM_HIP(hipMemcpyAsync(h.data(), S.loss, sizeof(double) * step, hipMemcpyDeviceToHost, st));
-    JFEM_HIP(hipStreamSynchronize(st));
-    if (loss_host)
-        for (int i = 0; i < step; i++) loss_host[i] = h[i] / (double)n;
+    if (twogrid) {
+        // per step: squared two-grid energy contraction of the sample, |e2|_A^2 / |e|_A^2
+        std::vector<double> h(2 * (size_t)step);
+        JFEM_HIP(hipMemcpyAsync(h.data(), S.loss2, sizeof(double) * 2 * step, hipMemcpyDeviceToHost, st));
+        JFEM_HIP(hipStreamSynchronize(st));
+        if (loss_host)
+            for (int i = 0; i < step; i++) loss_host[i] = h[2 * i + 1] > 0 ? h[2 * i] / h[2 * i + 1] : 0.0;
+    } else {
+        std::vector<double> h(step);
+        JFEM_HIP(hipMemcpyAsync(h.data(), S.loss, sizeof(double) * step, hipMemcpyDeviceToHost, st));
+        JFEM_HIP(hipStreamSynchronize(st));
+        if (loss_host)
+            for (int i = 0; i < step; i++) loss_host[i] = h[i] / (double)n;
+    }
     S.active = was_active;
     return step;
 }

@@ -142,57 +142,229 @@ __global__ __launch_bounds__(256) void k_last(Lat L, int taps_rt, const unsigned
 // wavefront (32 consecutive pixels), K = taps * 32 in steps of 16.
 // mode 0: forward, out = relu(D + bias)            (sign +1, weights [t][co][ci])
 // mode 1: input gradient, out = D * [mask > 0]     (sign -1, weights [t][ci][co])
-template <int MODE>
-__global__ __launch_bounds__(256) void k_mid(Lat L, int taps, const unsigned short* __restrict__ in,
+template <int MODE, int TAPS, int NT>
+__global__ __launch_bounds__(256) void k_mid(Lat L, int taps_rt, const unsigned short* __restrict__ in,
                                              const unsigned short* __restrict__ Wh, const float* __restrict__ bias,
                                              const unsigned short* __restrict__ mask, unsigned short* __restrict__ out)
 {
+    // The A fragments (weights) of every (tap, K half) live in LDS for the whole
+    // block: wl[(t * 2 + kb) * 64 + lane] is what lane `lane` feeds the MFMA.
+    // Each wavefront computes NT tiles of 32 consecutive pixels, so every A
+    // fragment read from LDS feeds NT MFMAs and NT independent activation loads
+    // are in flight per (tap, K half).
+    extern __shared__ bf16x8 wl[];
+    const int taps = TAPS ? TAPS : taps_rt;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const long ntiles = (L.np + 31) / 32;
     const int j = lane & 31;       // pixel column of the tile / output row of W
     const int kh = lane >> 5;      // which 8-wide half of the 16-wide K step
-    for (long tile = blockIdx.x * 4L + wave; tile < ntiles; tile += (long)gridDim.x * 4) {
-        const long p = tile * 32 + j;
-        const bool valid = p < L.np;
-        int ip[3] = {0, 0, 0};
-        if (valid) lat_xyz(L, p, ip);
-        f32x16 acc;
+    for (int f = threadIdx.x; f < taps * 2 * 64; f += blockDim.x) {
+        const int t = f / 128, kb = (f / 64) & 1, l = f & 63;
+        wl[f] = *reinterpret_cast<const bf16x8*>(Wh + ((long)t * NN_H + (l & 31)) * NN_H + kb * 16 + (l >> 5) * 8);
+    }
+    __syncthreads();
+    const long ngroups = (L.np + 32 * NT - 1) / (32 * NT);
+    for (long grp = blockIdx.x * 4L + wave; grp < ngroups; grp += (long)gridDim.x * 4) {
+        long p[NT];
+        int ip[NT][3];
+        f32x16 acc[NT];
 #pragma unroll
-        for (int r = 0; r < 16; r++) acc[r] = 0.f;
+        for (int u = 0; u < NT; u++) {
+            p[u] = (grp * NT + u) * 32 + j;
+            ip[u][0] = ip[u][1] = ip[u][2] = 0;
+            if (p[u] < L.np) lat_xyz(L, p[u], ip[u]);
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[u][r] = 0.f;
+        }
         for (int t = 0; t < taps; t++) {
-            const long q = valid ? lat_nb(L, p, ip, t, MODE == 0 ? 1 : -1) : -1;
+            long q[NT];
+#pragma unroll
+            for (int u = 0; u < NT; u++) q[u] = p[u] < L.np ? lat_nb(L, p[u], ip[u], t, MODE == 0 ? 1 : -1) : -1;
 #pragma unroll
             for (int kb = 0; kb < 2; kb++) {
                 const int c0 = kb * 16 + kh * 8;
-                bf16x8 a = *reinterpret_cast<const bf16x8*>(Wh + ((long)t * NN_H + j) * NN_H + c0);
-                bf16x8 b;
-                if (q >= 0) b = *reinterpret_cast<const bf16x8*>(in + q * NN_H + c0);
-                else {
+                const bf16x8 a = wl[(t * 2 + kb) * 64 + lane];
+                bf16x8 b[NT];
 #pragma unroll
-                    for (int e = 0; e < 8; e++) b[e] = 0;
+                for (int u = 0; u < NT; u++) {
+                    if (q[u] >= 0) b[u] = *reinterpret_cast<const bf16x8*>(in + q[u] * NN_H + c0);
+                    else {
+#pragma unroll
+                        for (int e = 0; e < 8; e++) b[u][e] = 0;
+                    }
                 }
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+#pragma unroll
+                for (int u = 0; u < NT; u++) acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[u], acc[u], 0, 0, 0);
             }
         }
-        if (!valid) continue;
-        // lane holds D[c][p] for c = 8*(r/4) + 4*kh + (r%4)
 #pragma unroll
-        for (int g = 0; g < 4; g++) {
-            unsigned short v4[4];
+        for (int u = 0; u < NT; u++) {
+            if (p[u] >= L.np) continue;
+            // lane holds D[c][p] for c = 8*(r/4) + 4*kh + (r%4)
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int c = 8 * g + 4 * kh + e;
-                float v = acc[4 * g + e];
-                if (MODE == 0) { v += bias[c]; v = v > 0.f ? v : 0.f; }
-                else v = bf2f(mask[p * NN_H + c]) > 0.f ? v : 0.f;
-                v4[e] = f2bf(v);
+            for (int g = 0; g < 4; g++) {
+                unsigned short v4[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int c = 8 * g + 4 * kh + e;
+                    float v = acc[u][4 * g + e];
+                    if (MODE == 0) { v += bias[c]; v = v > 0.f ? v : 0.f; }
+                    else v = bf2f(mask[p[u] * NN_H + c]) > 0.f ? v : 0.f;
+                    v4[e] = f2bf(v);
+                }
+                const unsigned long long pk = (unsigned long long)v4[0] | ((unsigned long long)v4[1] << 16) |
+                                              ((unsigned long long)v4[2] << 32) | ((unsigned long long)v4[3] << 48);
+                *reinterpret_cast<unsigned long long*>(out + p[u] * NN_H + 8 * g + 4 * kh) = pk;
             }
-            unsigned long long pk = (unsigned long long)v4[0] | ((unsigned long long)v4[1] << 16) |
-                                    ((unsigned long long)v4[2] << 32) | ((unsigned long long)v4[3] << 48);
-            *reinterpret_cast<unsigned long long*>(out + p * NN_H + 8 * g + 4 * kh) = pk;
         }
     }
+}
+
+// 32 -> 32 layer with the activations of a block's tile staged in LDS (2D and
+// 3D lattices).  A block owns a tile of 32 x TY x TZ pixels (x fastest) and
+// loads it with a one-pixel halo (zeros outside the lattice) into LDS once;
+// every tap then reads its shifted B fragments from LDS instead of L2.  Wave w
+// computes the 32-pixel rows w*NR .. w*NR+NR-1 of the tile, so one weight
+// fragment (from global memory, the same for all waves) feeds NR MFMAs.
+template <int D>
+struct MidTile {
+    static constexpr int NW = D == 3 ? 8 : 4;     // wavefronts per block
+    static constexpr int TY = D == 3 ? 8 : 16;    // rows along axis 1
+    static constexpr int TZ = D == 3 ? 4 : 1;     // rows along axis 2
+    static constexpr int NR = TY * TZ / NW;       // rows per wavefront
+    static constexpr int HX = 34, HY = TY + 2, HZ = D == 3 ? TZ + 2 : 1;
+    static constexpr int HP = HX * HY * HZ;       // halo pixels (3D: 2040 = 130 KB of LDS, 2D: 612 = 39 KB)
+};
+
+template <int MODE, int D>
+__global__ __launch_bounds__(MidTile<D>::NW * 64) void k_mid_lds(Lat L, const unsigned short* __restrict__ in,
+                                                 const unsigned short* __restrict__ Wh, const float* __restrict__ bias,
+                                                 const unsigned short* __restrict__ mask, unsigned short* __restrict__ out)
+{
+    using T = MidTile<D>;
+    constexpr int TAPS = D == 3 ? 27 : 9;
+    extern __shared__ bf16x8 tile[];   // [hz][hy][hx][4 x 8 channels]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j = lane & 31, kh = lane >> 5;
+    const int nx = (L.n[0] + 31) / 32, ny = (L.n[1] + T::TY - 1) / T::TY, nz = (L.n[2] + T::TZ - 1) / T::TZ;
+    const long ntile = (long)nx * ny * nz;
+    for (long tb = blockIdx.x; tb < ntile; tb += gridDim.x) {
+        const int bx = (int)(tb % nx), by = (int)((tb / nx) % ny), bz = (int)(tb / ((long)nx * ny));
+        const int x0 = bx * 32, y0 = by * T::TY, z0 = bz * T::TZ;
+        // stage the halo tile: 4 chunks of 16 bytes per pixel
+        for (int f = threadIdx.x; f < T::HP * 4; f += T::NW * 64) {
+            const int hp = f >> 2, ch = f & 3;
+            const int hx = hp % T::HX, hy = (hp / T::HX) % T::HY, hz = hp / (T::HX * T::HY);
+            const int gx = x0 - 1 + hx, gy = D >= 2 ? y0 - 1 + hy : 0, gz = D == 3 ? z0 - 1 + hz : 0;
+            bf16x8 v;
+            if ((unsigned)gx < (unsigned)L.n[0] && (unsigned)gy < (unsigned)L.n[1] && (unsigned)gz < (unsigned)L.n[2])
+                v = *reinterpret_cast<const bf16x8*>(in + ((long)gx + (long)L.n[0] * (gy + (long)L.n[1] * gz)) * NN_H +
+                                                     ch * 8);
+            else {
+#pragma unroll
+                for (int e = 0; e < 8; e++) v[e] = 0;
+            }
+            tile[f] = v;
+        }
+        __syncthreads();
+        f32x16 acc[T::NR];
+#pragma unroll
+        for (int u = 0; u < T::NR; u++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[u][r] = 0.f;
+        for (int t = 0; t < TAPS; t++) {
+            const int sg = MODE == 0 ? 1 : -1;
+            const int o0 = sg * (t % 3 - 1), o1 = sg * ((t / 3) % 3 - 1), o2 = D == 3 ? sg * (t / 9 - 1) : 0;
+#pragma unroll
+            for (int kb = 0; kb < 2; kb++) {
+                const bf16x8 a = *reinterpret_cast<const bf16x8*>(Wh + ((long)t * NN_H + j) * NN_H + kb * 16 + kh * 8);
+#pragma unroll
+                for (int u = 0; u < T::NR; u++) {
+                    const int row = wave * T::NR + u;
+                    const int yy = row % T::TY, zz = row / T::TY;
+                    const int hx = j + 1 + o0, hy = yy + 1 + o1, hz = D == 3 ? zz + 1 + o2 : 0;
+                    const bf16x8 b = tile[((hz * T::HY + hy) * T::HX + hx) * 4 + kb * 2 + kh];
+                    acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[u], 0, 0, 0);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < T::NR; u++) {
+            const int row = wave * T::NR + u;
+            const int gx = x0 + j, gy = y0 + row % T::TY, gz = z0 + row / T::TY;
+            if (gx >= L.n[0] || gy >= L.n[1] || gz >= L.n[2]) continue;
+            const long p = (long)gx + (long)L.n[0] * (gy + (long)L.n[1] * gz);
+            // lane holds D[c][p] for c = 8*(r/4) + 4*kh + (r%4)
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                unsigned short v4[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int c = 8 * g + 4 * kh + e;
+                    float v = acc[u][4 * g + e];
+                    if (MODE == 0) { v += bias[c]; v = v > 0.f ? v : 0.f; }
+                    else v = bf2f(mask[p * NN_H + c]) > 0.f ? v : 0.f;
+                    v4[e] = f2bf(v);
+                }
+                const unsigned long long pk = (unsigned long long)v4[0] | ((unsigned long long)v4[1] << 16) |
+                                              ((unsigned long long)v4[2] << 32) | ((unsigned long long)v4[3] << 48);
+                *reinterpret_cast<unsigned long long*>(out + p * NN_H + 8 * g + 4 * kh) = pk;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+#ifndef JFEM_NN_NT
+#define JFEM_NN_NT 4
+#endif
+#ifndef JFEM_NN_NOLDS
+#define JFEM_NN_NOLDS 0
+#endif
+constexpr int MID_NT = JFEM_NN_NT;   // pixel tiles per wavefront in the 32 -> 32 layers
+
+// LDS bytes one workgroup may allocate on the current device (160 KiB on gfx950)
+static long lds_per_block()
+{
+    static long cached = -1;
+    if (cached < 0) {
+        int dev = 0, v = 0;
+        JFEM_HIP(hipGetDevice(&dev));
+        JFEM_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        cached = v;
+    }
+    return cached;
+}
+
+// launch of one 32 -> 32 layer (MODE 0 forward, 1 input gradient): a persistent
+// grid of at most 1024 blocks, each loading the weights into LDS once
+template <int MODE>
+static void launch_mid(hipStream_t st, const Lat& L, int taps, const unsigned short* in, const unsigned short* W,
+                       const float* bias, const unsigned short* mask, unsigned short* out)
+{
+#if !JFEM_NN_NOLDS
+    if ((L.d == 3 || L.d == 2) && lds_per_block() >= (long)sizeof(bf16x8) * 4 * MidTile<3>::HP) {
+        const int d = L.d;
+        const int ty = d == 3 ? MidTile<3>::TY : MidTile<2>::TY, tz = d == 3 ? MidTile<3>::TZ : MidTile<2>::TZ;
+        const long ntile = (long)((L.n[0] + 31) / 32) * ((L.n[1] + ty - 1) / ty) * ((L.n[2] + tz - 1) / tz);
+        const int grid = (int)std::min<long>(ntile, 4096);
+        if (d == 3)
+            k_mid_lds<MODE, 3><<<grid, MidTile<3>::NW * 64, sizeof(bf16x8) * 4 * MidTile<3>::HP, st>>>(L, in, W, bias,
+                                                                                                      mask, out);
+        else
+            k_mid_lds<MODE, 2><<<grid, MidTile<2>::NW * 64, sizeof(bf16x8) * 4 * MidTile<2>::HP, st>>>(L, in, W, bias,
+                                                                                                      mask, out);
+        JFEM_HIP(hipGetLastError());
+        return;
+    }
+#endif
+    const long ngroups = (L.np + 32 * MID_NT - 1) / (32 * MID_NT);
+    const int grid = (int)std::min<long>((ngroups + 3) / 4, 1024);
+    const size_t lds = sizeof(bf16x8) * taps * 2 * 64;
+    if (taps == 27) k_mid<MODE, 27, MID_NT><<<grid, 256, lds, st>>>(L, taps, in, W, bias, mask, out);
+    else if (taps == 9) k_mid<MODE, 9, MID_NT><<<grid, 256, lds, st>>>(L, taps, in, W, bias, mask, out);
+    else k_mid<MODE, 0, MID_NT><<<grid, 256, lds, st>>>(L, taps, in, W, bias, mask, out);
+    JFEM_HIP(hipGetLastError());
 }
 
 // Weight gradient of a 32 -> 32 layer on MFMA:
@@ -480,8 +652,6 @@ void nn_forward(jfem_nn* nn, const Lat& L)
     long o[9];
     nn_offsets(nn, o);
     hipStream_t st = nn->ctx->stream;
-    const long ntiles = (L.np + 31) / 32;
-    const int gm = (int)std::min<long>((ntiles + 3) / 4, 16384);
     const int g = nn_grid(L.np);
     const float *W0 = nn->w + o[0], *b0 = nn->w + o[1];
     const int tc = nn->taps * 10 + nn->cin;
@@ -496,8 +666,8 @@ void nn_forward(jfem_nn* nn, const Lat& L)
     case 275: k_first<27, 5><<<g, 256, 0, st>>>(L, 5, 27, nn->in, W0, b0, nn->act0); break;
     default: k_first<0, 0><<<g, 256, 0, st>>>(L, nn->cin, nn->taps, nn->in, W0, b0, nn->act0); break;
     }
-    k_mid<0><<<gm, 256, 0, st>>>(L, nn->taps, nn->act0, nn->wh1, nn->w + o[3], nullptr, nn->act1);
-    k_mid<0><<<gm, 256, 0, st>>>(L, nn->taps, nn->act1, nn->wh2, nn->w + o[5], nullptr, nn->act2);
+    launch_mid<0>(st, L, nn->taps, nn->act0, nn->wh1, nn->w + o[3], nullptr, nn->act1);
+    launch_mid<0>(st, L, nn->taps, nn->act1, nn->wh2, nn->w + o[5], nullptr, nn->act2);
     if (nn->taps == 27) k_last<27><<<g, 256, 0, st>>>(L, 27, nn->act2, nn->w + o[6], nn->w + o[7], nn->out);
     else if (nn->taps == 9) k_last<9><<<g, 256, 0, st>>>(L, 9, nn->act2, nn->w + o[6], nn->w + o[7], nn->out);
     else k_last<0><<<g, 256, 0, st>>>(L, nn->taps, nn->act2, nn->w + o[6], nn->w + o[7], nn->out);
@@ -525,8 +695,6 @@ void nn_backward(jfem_nn* nn, const Lat& L, int add)
         JFEM_HIP(hipMalloc(&nn->part, sizeof(float) * need));
         nn->part_cap = need;
     }
-    const long ntiles = (L.np + 31) / 32;
-    const int gm = (int)std::min<long>((ntiles + 3) / 4, 16384);
     auto conv_dw = [&](const unsigned short* G, const unsigned short* X) {
         if (taps == 27) k_conv_dw<27><<<nbw, 256, 0, st>>>(L, wchunk, G, X, nn->part);
         else if (taps == 9) k_conv_dw<9><<<nbw, 256, 0, st>>>(L, wchunk, G, X, nn->part);
@@ -548,14 +716,14 @@ void nn_backward(jfem_nn* nn, const Lat& L, int add)
                                                                          NN_H, NN_H * NN_H, NN_H, add);
     k_bias_grad<<<nbb, 256, 0, st>>>(L.np, bchunk, nn->g2, nn->part);
     k_sum_blocks<<<NN_H, 256, 0, st>>>(nbb, NN_H, nn->part, nn->grad + o[5], add);
-    k_mid<1><<<gm, 256, 0, st>>>(L, taps, nn->g2, nn->wt2, nullptr, nn->act1, nn->g1);
+    launch_mid<1>(st, L, taps, nn->g2, nn->wt2, nullptr, nn->act1, nn->g1);
     // layer 1
     conv_dw(nn->g1, nn->act0);
     k_sum_dw<<<grid_for((long)taps * NN_H * NN_H, 64, 4096), 64, 0, st>>>(nbw, taps, nn->part, nn->grad + o[2], NN_H,
                                                                          NN_H, NN_H * NN_H, NN_H, add);
     k_bias_grad<<<nbb, 256, 0, st>>>(L.np, bchunk, nn->g1, nn->part);
     k_sum_blocks<<<NN_H, 256, 0, st>>>(nbb, NN_H, nn->part, nn->grad + o[3], add);
-    k_mid<1><<<gm, 256, 0, st>>>(L, taps, nn->g1, nn->wt1, nullptr, nn->act0, nn->g0);
+    launch_mid<1>(st, L, taps, nn->g1, nn->wt1, nullptr, nn->act0, nn->g0);
     // layer 0 (first): dW0[t][co][ci] from g0 and the input padded to 32 channels (in g2's buffer)
     k_pad_bf16<<<padg, 256, 0, st>>>(L.np, nn->cin, nn->in, nn->g2);
     conv_dw(nn->g0, nn->g2);

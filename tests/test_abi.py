@@ -54,3 +54,18 @@ def test_oracle_splitmix_matches_python():
     s = SplitMix64(12345)
     for _ in range(5):
         assert O.lib().orc_splitmix64(ctypes.byref(st)) == s.next_u64()
+
+
+def test_cell_arrays_are_checked_against_the_grid():
+    """per-cell coefficient arrays must have one entry per finest cell: the C ABI
+    reads exactly that many, so the mirror rejects other lengths first"""
+    import numpy as np
+    from jfem_amd import api, _native as N
+
+    class _Space:
+        cells = np.array([4, 3], dtype=np.int32)
+
+    assert api._cell_array(None, _Space(), "c") is None
+    assert api._cell_array(np.ones((3, 4)), _Space(), "c").shape == (12,)
+    with pytest.raises(N.IllegalArgumentException):
+        api._cell_array(np.ones(16), _Space(), "c")

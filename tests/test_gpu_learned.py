@@ -1,8 +1,8 @@
-"""Learned smoother on the GPU: training on residuals of plain V-cycles lowers
-the loss, the trained network helps the V-cycle more than an untrained one,
-the fallback to Chebyshev triggers on a network that does not smooth, and the
-FP32 multigrid with the learned smoother inside FP64 flexible CG meets the FP64
-residual judge."""
+"""Learned smoother on the GPU: two-grid training on (residual, error) pairs of
+the V-cycle lowers the loss, the trained network smooths (the V-cycle converges
+without falling back) where the untrained one does not, the fallback to
+Chebyshev triggers on a network that does not smooth, and the FP32 multigrid
+with the learned smoother inside FP64 flexible CG meets the FP64 residual judge."""
 import numpy as np
 import pytest
 
@@ -37,18 +37,18 @@ def trained(jf):
     field = gaussian_random_field(4000, (128, 128), 0.1, 1e3)
     G, spaces = _hier(jf, 8, 5, field)
     net = jf.SmootherNet(2, 2, seed=7)
-    G.set_learned(net, field, runs=2, fallback_ratio=0.5)
+    G.set_learned(net, field, runs=1, fallback_ratio=0.5)
     p0 = net.get_params()     # biases already zeroed by set_learned
-    hist = G.train(rhs_per_field=60, cycles_per_rhs=8, lr=3e-4, seed=1, mode=3)
+    hist = G.train(rhs_per_field=60, cycles_per_rhs=8, lr=3e-4, seed=1, mode=5)
     return dict(G=G, spaces=spaces, net=net, field=field, hist=hist, p0=p0)
 
 
 def test_training_lowers_loss(trained):
-    """the energy loss |e - N(r)|_A^2 - |e|_A^2 (per node, in units of the
-    normalised input) starts near zero and ends at least five times lower"""
+    """the two-grid loss |T(e - N(r))|_A^2 / |e|_A^2 of the samples falls at
+    least fivefold from the untrained network's"""
     h = trained["hist"]
-    assert np.all(np.isfinite(h))
-    assert np.mean(h[-40:]) < 5 * np.mean(h[:10]) < 0, (np.mean(h[:10]), np.mean(h[-40:]))
+    assert np.all(np.isfinite(h)) and np.all(h >= 0)
+    assert np.mean(h[-40:]) < 0.2 * np.mean(h[:10]), (np.mean(h[:10]), np.mean(h[-40:]))
 
 
 def test_trained_network_beats_untrained(jf, trained):
@@ -74,11 +74,15 @@ def test_trained_network_beats_untrained(jf, trained):
           f"untrained {info_u.iterations} (fallback {fb_u}), chebyshev {info_c.iterations} "
           f"(median reduction {med(hist_c):.3f})")
     assert info_t.resnorm <= tol and info_c.resnorm <= tol
-    # training is what makes the learned step a smoother: the trained network
-    # contracts the typical cycle more than the degree-4 Chebyshev-Jacobi
-    # smoother, the untrained one falls back or converges no faster
-    assert med(hist_t) < med(hist_c)
-    assert info_t.iterations <= info_u.iterations or fb_u
+    # training is what makes the learned step a smoother: with one network
+    # application per smoothing call (one residual SpMV) the trained V-cycle
+    # converges without falling back, in at most 1.5x the cycles of the
+    # Chebyshev-Jacobi V-cycle (four SpMVs per smoothing call); the untrained
+    # network falls back or needs more cycles
+    assert not fb_t
+    assert med(hist_t) < 0.3
+    assert info_t.iterations <= 1.5 * info_c.iterations
+    assert fb_u or info_u.iterations > info_t.iterations
 
 
 def test_fallback_on_a_network_that_does_not_smooth(jf, trained):
@@ -93,14 +97,14 @@ def test_fallback_on_a_network_that_does_not_smooth(jf, trained):
     active, fell_back = G.learned_status()
     assert fell_back and not active
     assert info.resnorm <= tol
-    G.set_learned(net, trained["field"], runs=2, fallback_ratio=0.5)
+    G.set_learned(net, trained["field"], runs=1, fallback_ratio=0.5)
 
 
 def test_fp32_learned_mg_in_fp64_fcg(jf, trained):
     import scipy.sparse.linalg as spla
     field, net = trained["field"], trained["net"]
     G32, spaces = _hier(jf, 8, 5, field, precision=1)
-    G32.set_learned(net, field, runs=2, fallback_ratio=0.5)
+    G32.set_learned(net, field, runs=1, fallback_ratio=0.5)
     rhs = _rhs(jf, spaces[-1], seed=5)
     b = jf.DenseVector(rhs)
     tol = 1e-8 * np.linalg.norm(rhs)
