@@ -228,13 +228,45 @@ __global__ __launch_bounds__(256) void k_mid(Lat L, int taps_rt, const unsigned 
 // fragment (from global memory, the same for all waves) feeds NR MFMAs.
 template <int D>
 struct MidTile {
-    static constexpr int NW = D == 3 ? 8 : 4;     // wavefronts per block
-    static constexpr int TY = D == 3 ? 8 : 16;    // rows along axis 1
+    static constexpr int NW = 4;                  // wavefronts per block
+    static constexpr int TY = D == 3 ? 4 : 16;    // rows along axis 1
     static constexpr int TZ = D == 3 ? 4 : 1;     // rows along axis 2
     static constexpr int NR = TY * TZ / NW;       // rows per wavefront
     static constexpr int HX = 34, HY = TY + 2, HZ = D == 3 ? TZ + 2 : 1;
-    static constexpr int HP = HX * HY * HZ;       // halo pixels (3D: 2040 = 130 KB of LDS, 2D: 612 = 39 KB)
+    static constexpr int HP = HX * HY * HZ;       // halo pixels (3D: 1224 = 78 KB of LDS at 32 channels, 2D: 612)
 };
+
+// LDS position of 16-byte chunk c of halo pixel hp: the chunk index is XORed
+// with bits of the pixel index so that 16 lanes reading one chunk of 16
+// consecutive pixels touch 16 different 16-byte bank groups
+template <int CH>
+__device__ inline int halo_slot(int hp, int c)
+{
+    if (CH == 4) return hp * 4 + (c ^ ((hp >> 2) & 3));
+    if (CH == 2) return hp * 2 + (c ^ ((hp >> 3) & 1));
+    return hp * CH + c;
+}
+
+template <int D, int CH>
+__device__ inline void stage_halo(const Lat& L, const unsigned short* __restrict__ src, int x0, int y0, int z0,
+                                  bf16x8* tile)
+{
+    using T = MidTile<D>;
+    for (int f = threadIdx.x; f < T::HP * CH; f += T::NW * 64) {
+        const int hp = f / CH, ch = f % CH;
+        const int hx = hp % T::HX, hy = (hp / T::HX) % T::HY, hz = hp / (T::HX * T::HY);
+        const int gx = x0 - 1 + hx, gy = D >= 2 ? y0 - 1 + hy : 0, gz = D == 3 ? z0 - 1 + hz : 0;
+        bf16x8 v;
+        if ((unsigned)gx < (unsigned)L.n[0] && (unsigned)gy < (unsigned)L.n[1] && (unsigned)gz < (unsigned)L.n[2])
+            v = *reinterpret_cast<const bf16x8*>(src + ((long)gx + (long)L.n[0] * (gy + (long)L.n[1] * gz)) * (CH * 8) +
+                                                 ch * 8);
+        else {
+#pragma unroll
+            for (int e = 0; e < 8; e++) v[e] = 0;
+        }
+        tile[halo_slot<CH>(hp, ch)] = v;
+    }
+}
 
 template <int MODE, int D>
 __global__ __launch_bounds__(MidTile<D>::NW * 64) void k_mid_lds(Lat L, const unsigned short* __restrict__ in,
@@ -251,21 +283,7 @@ __global__ __launch_bounds__(MidTile<D>::NW * 64) void k_mid_lds(Lat L, const un
     for (long tb = blockIdx.x; tb < ntile; tb += gridDim.x) {
         const int bx = (int)(tb % nx), by = (int)((tb / nx) % ny), bz = (int)(tb / ((long)nx * ny));
         const int x0 = bx * 32, y0 = by * T::TY, z0 = bz * T::TZ;
-        // stage the halo tile: 4 chunks of 16 bytes per pixel
-        for (int f = threadIdx.x; f < T::HP * 4; f += T::NW * 64) {
-            const int hp = f >> 2, ch = f & 3;
-            const int hx = hp % T::HX, hy = (hp / T::HX) % T::HY, hz = hp / (T::HX * T::HY);
-            const int gx = x0 - 1 + hx, gy = D >= 2 ? y0 - 1 + hy : 0, gz = D == 3 ? z0 - 1 + hz : 0;
-            bf16x8 v;
-            if ((unsigned)gx < (unsigned)L.n[0] && (unsigned)gy < (unsigned)L.n[1] && (unsigned)gz < (unsigned)L.n[2])
-                v = *reinterpret_cast<const bf16x8*>(in + ((long)gx + (long)L.n[0] * (gy + (long)L.n[1] * gz)) * NN_H +
-                                                     ch * 8);
-            else {
-#pragma unroll
-                for (int e = 0; e < 8; e++) v[e] = 0;
-            }
-            tile[f] = v;
-        }
+        stage_halo<D, 4>(L, in, x0, y0, z0, tile);
         __syncthreads();
         f32x16 acc[T::NR];
 #pragma unroll
@@ -283,7 +301,7 @@ __global__ __launch_bounds__(MidTile<D>::NW * 64) void k_mid_lds(Lat L, const un
                     const int row = wave * T::NR + u;
                     const int yy = row % T::TY, zz = row / T::TY;
                     const int hx = j + 1 + o0, hy = yy + 1 + o1, hz = D == 3 ? zz + 1 + o2 : 0;
-                    const bf16x8 b = tile[((hz * T::HY + hy) * T::HX + hx) * 4 + kb * 2 + kh];
+                    const bf16x8 b = tile[halo_slot<4>((hz * T::HY + hy) * T::HX + hx, kb * 2 + kh)];
                     acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[u], 0, 0, 0);
                 }
             }
@@ -310,6 +328,147 @@ __global__ __launch_bounds__(MidTile<D>::NW * 64) void k_mid_lds(Lat L, const un
                                               ((unsigned long long)v4[2] << 32) | ((unsigned long long)v4[3] << 48);
                 *reinterpret_cast<unsigned long long*>(out + p * NN_H + 8 * g + 4 * kh) = pk;
             }
+        }
+        __syncthreads();
+    }
+}
+
+// Stage the halo tile of block (x0, y0, z0) into LDS: CH 16-byte chunks per
+// pixel of a (32 + 2) x (TY + 2) x (TZ + 2) box, zeros outside the lattice.
+// split-BF16 copy of the input: inb[p][0..7] = hi(in[p][c]), inb[p][8..15] =
+// lo(in[p][c]) = bf16(in - hi) for c < cin, zeros past cin; hi + lo carries
+// about 16 mantissa bits
+__global__ void k_in_split(long np, int cin, const float* __restrict__ in, unsigned short* __restrict__ inb)
+{
+    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
+        bf16x8 hi, lo;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const float v = c < cin ? in[p * cin + c] : 0.f;
+            const unsigned short h = f2bf(v);
+            hi[c] = (short)h;
+            lo[c] = (short)f2bf(v - bf2f(h));
+        }
+        *reinterpret_cast<bf16x8*>(inb + p * 16) = hi;
+        *reinterpret_cast<bf16x8*>(inb + p * 16 + 8) = lo;
+    }
+}
+
+// first layer on MFMA (2D/3D, cin <= 8) in split-BF16: one 16-wide K step per
+// tap, lanes 0-31 feed the input's high parts and lanes 32-63 its low parts,
+// so sum_c (hi + lo)[c] w_hi[c] + hi[c] w_lo[c] (two MFMAs per tap) keeps the
+// layer at close to FP32 accuracy; the input tile is staged in LDS
+template <int D>
+__global__ __launch_bounds__(MidTile<D>::NW * 64) void k_first_lds(Lat L, int cin, const unsigned short* __restrict__ inb,
+                                                                  const float* __restrict__ W0,
+                                                                  const float* __restrict__ b0,
+                                                                  unsigned short* __restrict__ out)
+{
+    using T = MidTile<D>;
+    constexpr int TAPS = D == 3 ? 27 : 9;
+    extern __shared__ bf16x8 sm[];
+    bf16x8* wf = sm;                     // TAPS x (32 w_hi, 32 w_lo) weight fragments
+    bf16x8* tile = sm + TAPS * 64;       // HP pixels x (hi, lo)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j = lane & 31, kh = lane >> 5;
+    for (int f = threadIdx.x; f < TAPS * 64; f += T::NW * 64) {
+        const int t = f / 64, part = (f / 32) & 1, co = f & 31;
+        bf16x8 v;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const float w = e < cin ? W0[(t * NN_H + co) * cin + e] : 0.f;
+            const unsigned short h = f2bf(w);
+            v[e] = (short)(part == 0 ? h : f2bf(w - bf2f(h)));
+        }
+        wf[f] = v;
+    }
+    bf16x8 zero8;
+#pragma unroll
+    for (int e = 0; e < 8; e++) zero8[e] = 0;
+    const int nx = (L.n[0] + 31) / 32, ny = (L.n[1] + T::TY - 1) / T::TY, nz = (L.n[2] + T::TZ - 1) / T::TZ;
+    const long ntile = (long)nx * ny * nz;
+    for (long tb = blockIdx.x; tb < ntile; tb += gridDim.x) {
+        const int bx = (int)(tb % nx), by = (int)((tb / nx) % ny), bz = (int)(tb / ((long)nx * ny));
+        const int x0 = bx * 32, y0 = by * T::TY, z0 = bz * T::TZ;
+        stage_halo<D, 2>(L, inb, x0, y0, z0, tile);
+        __syncthreads();
+        f32x16 acc[T::NR];
+#pragma unroll
+        for (int u = 0; u < T::NR; u++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[u][r] = 0.f;
+        for (int t = 0; t < TAPS; t++) {
+            const int o0 = t % 3 - 1, o1 = (t / 3) % 3 - 1, o2 = D == 3 ? t / 9 - 1 : 0;
+            // lanes 0-31 multiply the input's high parts by w_hi + w_lo, lanes 32-63 its low parts by w_hi
+            const bf16x8 ahi = wf[t * 64 + j], alo = kh == 0 ? wf[t * 64 + 32 + j] : zero8;
+#pragma unroll
+            for (int u = 0; u < T::NR; u++) {
+                const int row = wave * T::NR + u;
+                const int yy = row % T::TY, zz = row / T::TY;
+                const int hx = j + 1 + o0, hy = yy + 1 + o1, hz = D == 3 ? zz + 1 + o2 : 0;
+                const bf16x8 b = tile[halo_slot<2>((hz * T::HY + hy) * T::HX + hx, kh)];
+                acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, b, acc[u], 0, 0, 0);
+                acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, b, acc[u], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < T::NR; u++) {
+            const int row = wave * T::NR + u;
+            const int gx = x0 + j, gy = y0 + row % T::TY, gz = z0 + row / T::TY;
+            if (gx >= L.n[0] || gy >= L.n[1] || gz >= L.n[2]) continue;
+            const long p = (long)gx + (long)L.n[0] * (gy + (long)L.n[1] * gz);
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                unsigned short v4[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int c = 8 * g + 4 * kh + e;
+                    float v = acc[u][4 * g + e] + b0[c];
+                    v4[e] = f2bf(v > 0.f ? v : 0.f);
+                }
+                const unsigned long long pk = (unsigned long long)v4[0] | ((unsigned long long)v4[1] << 16) |
+                                              ((unsigned long long)v4[2] << 32) | ((unsigned long long)v4[3] << 48);
+                *reinterpret_cast<unsigned long long*>(out + p * NN_H + 8 * g + 4 * kh) = pk;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// last layer with the activation tile staged in LDS (2D/3D): each thread owns
+// pixels tid, tid + NW*64, ... of the tile
+template <int D>
+__global__ __launch_bounds__(MidTile<D>::NW * 64) void k_last_lds(Lat L, const unsigned short* __restrict__ act,
+                                                                 const float* __restrict__ W3,
+                                                                 const float* __restrict__ b3, float* __restrict__ out)
+{
+    using T = MidTile<D>;
+    constexpr int TAPS = D == 3 ? 27 : 9;
+    extern __shared__ bf16x8 tile[];     // HP pixels x 32 channels
+    const int nx = (L.n[0] + 31) / 32, ny = (L.n[1] + T::TY - 1) / T::TY, nz = (L.n[2] + T::TZ - 1) / T::TZ;
+    const long ntile = (long)nx * ny * nz;
+    for (long tb = blockIdx.x; tb < ntile; tb += gridDim.x) {
+        const int bx = (int)(tb % nx), by = (int)((tb / nx) % ny), bz = (int)(tb / ((long)nx * ny));
+        const int x0 = bx * 32, y0 = by * T::TY, z0 = bz * T::TZ;
+        stage_halo<D, 4>(L, act, x0, y0, z0, tile);
+        __syncthreads();
+        for (int q = threadIdx.x; q < 32 * T::TY * T::TZ; q += T::NW * 64) {
+            const int xx = q % 32, yy = (q / 32) % T::TY, zz = q / (32 * T::TY);
+            const int gx = x0 + xx, gy = y0 + yy, gz = z0 + zz;
+            if (gx >= L.n[0] || gy >= L.n[1] || gz >= L.n[2]) continue;
+            float acc = b3[0];
+            for (int t = 0; t < TAPS; t++) {
+                const int o0 = t % 3 - 1, o1 = (t / 3) % 3 - 1, o2 = D == 3 ? t / 9 - 1 : 0;
+                const int hx = xx + 1 + o0, hy = yy + 1 + o1, hz = D == 3 ? zz + 1 + o2 : 0;
+                const int hp = (hz * T::HY + hy) * T::HX + hx;
+#pragma unroll
+                for (int g = 0; g < 4; g++) {
+                    const bf16x8 h = tile[halo_slot<4>(hp, g)];
+#pragma unroll
+                    for (int e = 0; e < 8; e++) acc += W3[t * NN_H + 8 * g + e] * bf2f((unsigned short)h[e]);
+                }
+            }
+            out[(long)gx + (long)L.n[0] * (gy + (long)L.n[1] * gz)] = acc;
         }
         __syncthreads();
     }
@@ -629,7 +788,7 @@ void nn_reserve(jfem_nn* nn, long np)
 {
     if (np <= nn->cap) return;
     for (void** p : {(void**)&nn->act0, (void**)&nn->act1, (void**)&nn->act2, (void**)&nn->g0, (void**)&nn->g1,
-                     (void**)&nn->g2, (void**)&nn->in, (void**)&nn->out, (void**)&nn->gout})
+                     (void**)&nn->g2, (void**)&nn->in, (void**)&nn->inb, (void**)&nn->out, (void**)&nn->gout})
         (void)hipFree(*p);
     const size_t hb = sizeof(unsigned short) * NN_H * (size_t)np;
     JFEM_HIP(hipMalloc(&nn->act0, hb));
@@ -639,6 +798,7 @@ void nn_reserve(jfem_nn* nn, long np)
     JFEM_HIP(hipMalloc(&nn->g1, hb));
     JFEM_HIP(hipMalloc(&nn->g2, hb));
     JFEM_HIP(hipMalloc(&nn->in, sizeof(float) * nn->cin * (size_t)np));
+    JFEM_HIP(hipMalloc(&nn->inb, sizeof(unsigned short) * 16 * (size_t)np));
     JFEM_HIP(hipMalloc(&nn->out, sizeof(float) * (size_t)np));
     JFEM_HIP(hipMalloc(&nn->gout, sizeof(float) * (size_t)np));
     nn->cap = np;
@@ -654,6 +814,32 @@ void nn_forward(jfem_nn* nn, const Lat& L)
     hipStream_t st = nn->ctx->stream;
     const int g = nn_grid(L.np);
     const float *W0 = nn->w + o[0], *b0 = nn->w + o[1];
+    if ((L.d == 2 || L.d == 3) && nn->cin <= 8 && lds_per_block() >= (long)sizeof(bf16x8) * 4 * MidTile<3>::HP) {
+        // 2D / 3D lattices: first and last layers with LDS tiles, first layer on MFMA
+        k_in_split<<<g, 256, 0, st>>>(L.np, nn->cin, nn->in, nn->inb);
+        if (L.d == 3) {
+            using T = MidTile<3>;
+            const long nt = (long)((L.n[0] + 31) / 32) * ((L.n[1] + T::TY - 1) / T::TY) * ((L.n[2] + T::TZ - 1) / T::TZ);
+            const int grid = (int)std::min<long>(nt, 4096);
+            k_first_lds<3><<<grid, T::NW * 64, sizeof(bf16x8) * (27 * 64 + 2 * T::HP), st>>>(L, nn->cin, nn->inb, W0,
+                                                                                                 b0, nn->act0);
+            launch_mid<0>(st, L, nn->taps, nn->act0, nn->wh1, nn->w + o[3], nullptr, nn->act1);
+            launch_mid<0>(st, L, nn->taps, nn->act1, nn->wh2, nn->w + o[5], nullptr, nn->act2);
+            k_last<27><<<g, 256, 0, st>>>(L, 27, nn->act2, nn->w + o[6], nn->w + o[7], nn->out);
+        } else {
+            using T = MidTile<2>;
+            const long nt = (long)((L.n[0] + 31) / 32) * ((L.n[1] + T::TY - 1) / T::TY);
+            const int grid = (int)std::min<long>(nt, 4096);
+            k_first_lds<2><<<grid, T::NW * 64, sizeof(bf16x8) * (9 * 64 + 2 * T::HP), st>>>(L, nn->cin, nn->inb, W0,
+                                                                                                b0, nn->act0);
+            launch_mid<0>(st, L, nn->taps, nn->act0, nn->wh1, nn->w + o[3], nullptr, nn->act1);
+            launch_mid<0>(st, L, nn->taps, nn->act1, nn->wh2, nn->w + o[5], nullptr, nn->act2);
+            k_last_lds<2><<<grid, T::NW * 64, sizeof(bf16x8) * 4 * T::HP, st>>>(L, nn->act2, nn->w + o[6], nn->w + o[7],
+                                                                               nn->out);
+        }
+        JFEM_HIP(hipGetLastError());
+        return;
+    }
     const int tc = nn->taps * 10 + nn->cin;
     switch (tc) {
     case 91: k_first<9, 1><<<g, 256, 0, st>>>(L, 1, 9, nn->in, W0, b0, nn->act0); break;

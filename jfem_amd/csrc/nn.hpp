@@ -32,6 +32,7 @@ struct jfem_nn {
     unsigned short *act0 = nullptr, *act1 = nullptr, *act2 = nullptr;   // hidden activations (post-ReLU), NHWC bf16
     unsigned short *g0 = nullptr, *g1 = nullptr, *g2 = nullptr;         // gradients w.r.t. hidden pre-activations
     float* in = nullptr;     // np x cin input (lattice order)
+    unsigned short* inb = nullptr;   // np x 16 split-bf16 copy of the input (8 high parts, 8 low parts)
     float* out = nullptr;    // np output
     float* gout = nullptr;   // dLoss/dout
     float* part = nullptr;   // per-block weight-gradient partials

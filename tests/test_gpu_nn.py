@@ -31,14 +31,17 @@ def test_forward_and_gradient_vs_oracle(jf, d, shape, cin):
     g = net.gradient(shape, inp, gout)
     gr = NO.gradient(p, shape, inp, gout, cin)
     assert rel(g, gr) <= 4e-2, rel(g, gr)
-    # per-layer check so that a wrong layer cannot hide behind the others; bias
-    # gradients are sums over all pixels of BF16 gradients with heavy
-    # cancellation, hence the looser 1.2e-1 bound for them
+    # per-layer check so that a wrong layer cannot hide behind the others (a
+    # transposed tap, a wrong channel or a dropped term gives errors of order
+    # one).  The device rounds activations and back-propagated gradients to
+    # BF16 (8 significant bits) at every layer, so a weight gradient, a sum of
+    # such products over all pixels, agrees to a few percent (8e-2 bound);
+    # bias gradients cancel more heavily (1.2e-1)
     sizes = [3 ** d * 32 * cin, 32, 3 ** d * 1024, 32, 3 ** d * 1024, 32, 3 ** d * 32, 1]
     o = 0
     for i, s in enumerate(sizes):
         if np.linalg.norm(gr[o:o + s]) > 1e-6:
-            bound = 1.2e-1 if i % 2 else 6e-2
+            bound = 1.2e-1 if i % 2 else 8e-2
             assert rel(g[o:o + s], gr[o:o + s]) <= bound, (o, s, rel(g[o:o + s], gr[o:o + s]))
         o += s
 
