@@ -2,13 +2,21 @@
 
 Workload (N=1): 3D Poisson -div grad u = f on the unit cube, Q2 ContinuousTPFESpace
 on 96^3 cells (7,189,057 DOFs), f = seeded sum of 16 3D sinusoids, zero Dirichlet,
-FP64 geometric multigrid (levels 3..96 cells, V(2,2) with Chebyshev-Jacobi degree 4,
-coarsest 3^3 solved directly) preconditioning CG to ||r|| <= 1e-8 ||b||.
+geometric multigrid on levels 3..96 cells (coarsest 3^3 solved directly), to
+||r|| <= 1e-8 ||b||:
+  - FP64 MG-PCG: V(2,2) with Chebyshev-Jacobi degree 4 preconditioning CG (the
+    plain drop-in path);
+  - FP32 MG inside FP64 flexible CG, same smoother;
+  - learned + FP32 MG inside FP64 flexible CG: the finest-level smoother is the
+    BF16-MFMA network, trained here on the same operator on 24^3 cells (one
+    application per smoothing call, Chebyshev fallback).
+`value` is the learned + FP32 time (the metric compares it with the FP64 MG-PCG
+time, also in the line); with --no-fp32 / --no-learned it is the FP64 time.
 
 A step is one solve from x0 = 0 with the matrices, hierarchy and right-hand side
 already resident in HBM.  The line also reports assembly time, the FP64 SpMV's
-roofline, a flexible-CG solve around an FP32 V-cycle, and the CPU oracle timed
-on a bounded sample.
+roofline (with the measured HBM traffic and a streaming-read calibration), and
+the CPU oracle timed on a bounded sample.
 """
 from __future__ import annotations
 
@@ -232,6 +240,7 @@ def main():
     ap.add_argument("--cells", type=int, default=96, help="cells per axis of the finest grid (3 * 2^m)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-smoother solve")
     ap.add_argument("--workload", default="c2", choices=["c2", "c4"],
                     help="c2: 3D Q2 96^3 FP64 MG-PCG (default); c4: 2D Q1 1024^2 log-normal coefficient, "
                          "learned smoother vs Chebyshev")
@@ -335,6 +344,50 @@ def main():
         err = float(np.linalg.norm(x32.numpy() - x64) / np.linalg.norm(x64))
         extra = {"fp32mg_fcg_s": float(np.median(t32)), "fp32mg_fcg_iterations": it.iterations,
                  "fp32mg_vs_fp64_rel_l2": err, "speedup_fp32mg_vs_fp64": float(np.median(times) / np.median(t32))}
+        if not args.no_learned:
+            # learned smoother: trained on the same Q2 Poisson operator on 24^3 cells (the
+            # network input is the Jacobi-scaled residual, independent of the mesh size),
+            # then attached to the finest level of the FP32 hierarchy (one application per
+            # smoothing call, Chebyshev fallback when a cycle reduces less than 0.5x)
+            t0 = time.perf_counter()
+            _, Gt, _ = build_problem(jf, base, 4, 2, smoother, jf.FP64, 2, 16)
+            net = jf.SmootherNet(3, 5, seed=7)
+            Gt.set_learned(net, np.zeros((base[0] << 3) ** 3), runs=1, fallback_ratio=0.5)
+            h = Gt.train(rhs_per_field=40, cycles_per_rhs=8, lr=3e-4, seed=1, mode=5)
+            ctx.sync()
+            t_train = time.perf_counter() - t0
+            del Gt
+            log(f"learned smoother trained on 24^3 in {t_train:.1f}s, two-grid energy ratio "
+                f"{np.mean(h[:8]):.3f} -> {np.mean(h[-8:]):.4f}")
+            ncell = int(np.prod(fine.cells.astype(np.int64)))
+            tl, fb = [], False
+
+            G32.set_learned(net, np.zeros(ncell), runs=1, fallback_ratio=0.5)
+
+            def solve_learned():
+                G32.set_learned_active(True)   # re-arms the smoother and clears the fallback flag
+                itl = jf.IterativeSolver()
+                xs_ = itl.solveFCG(A32, G32, b, tol)
+                return itl, xs_, G32.learned_status()[1]
+
+            for w in range(args.warmup):
+                solve_learned()
+            ctx.sync()
+            t_start = time.perf_counter()
+            for s in range(args.steps):
+                it, xl, f_ = solve_learned()
+                tl.append(it.info.seconds)
+                fb = fb or f_
+                log(f"learned + fp32-MG FCG: {it.iterations} its, {it.info.seconds:.3f}s, fell back {f_}")
+            ctx.sync()
+            extra["learned_ms_per_step"] = (time.perf_counter() - t_start) / args.steps * 1e3
+            rl = A.mvMul(xl)
+            extra.update({
+                "learned_fp32mg_fcg_s": float(np.median(tl)), "learned_fp32mg_fcg_iterations": it.iterations,
+                "learned_fell_back": bool(fb), "learned_train_s": t_train,
+                "learned_true_rel_residual": float(np.linalg.norm(rhs - rl.numpy()) / bnorm),
+                "learned_vs_fp64_rel_l2": float(np.linalg.norm(xl.numpy() - x64) / np.linalg.norm(x64)),
+                "speedup_learned_fp32mg_vs_fp64": float(np.median(times) / np.median(tl))})
 
     cpu = None
     if not args.no_cpu:
@@ -346,14 +399,22 @@ def main():
                          f"{cb['iterations']} its in {cb['solve_s']:.3f}s, scaled by DOF ratio {scale:.1f}; "
                          f"oracle SpMV {cb['spmv_gbs']:.2f} GB/s"}
 
-    value = float(np.median(times))
+    fp64_s = float(np.median(times))
+    learned = "learned_fp32mg_fcg_s" in extra
+    value = extra["learned_fp32mg_fcg_s"] if learned else fp64_s
+    workload = (f"3D Poisson Q2 {args.cells}^3 cells; learned smoother + FP32 GMG V-cycle inside FP64 flexible CG "
+                f"vs FP64 GMG V(2,2) Chebyshev-Jacobi deg 4 PCG, both to 1e-8" if learned else
+                f"3D Poisson Q2 {args.cells}^3 cells, FP64 GMG V(2,2) Chebyshev-Jacobi deg 4, CG to 1e-8")
     line = {
-        "metric": "time-to-1e-8 residual (s): FP64 MG-PCG on 3D Q2 96^3 (7.19M DOFs)",
+        "metric": "time-to-1e-8 residual (s): learned+FP32 MG-PCG vs FP64 MG-PCG; SpMV GB/s" if learned else
+                  "time-to-1e-8 residual (s): FP64 MG-PCG",
         "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_per_step, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"3D Poisson Q2 {args.cells}^3 cells, FP64 GMG V(2,2) Chebyshev-Jacobi deg 4, CG to 1e-8",
-                   "ndofs": n, "nnz": nnz, "levels": nlev, "iterations": int(np.median(iters))},
+        "ms_per_step": extra.pop("learned_ms_per_step") if learned else ms_per_step, "higher_is_better": False,
+        "scaling": "strong", "vs_baseline": None,
+        "dtype": "f64 outer / f32 multigrid / bf16 network" if learned else "f64", "data": "synthetic",
+        "config": {"workload": workload, "ndofs": n, "nnz": nnz, "levels": nlev,
+                   "fp64_iterations": int(np.median(iters))},
+        "fp64_mgpcg_s": fp64_s, "fp64_ms_per_step": ms_per_step,
         "true_rel_residual": true_rel,
         "assembly_s": t_asm, "hierarchy_setup_s": t_mg,
         "spmv_fp64_ms": spmv_ms, "spmv_fp64_gbs": spmv_gbs, "spmv_fp32_ms": spmv32_ms, "spmv_fp32_gbs": spmv32_gbs,
