@@ -38,7 +38,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def build_problem(jf, base, nlev, degree, smoother, precision, seed, nterms):
+def build_problem(jf, base, nlev, degree, smoother, precision, seed, nterms, cell_coeff=None):
     d = len(base)
     spaces = []
     for l in range(nlev):
@@ -48,7 +48,7 @@ def build_problem(jf, base, nlev, degree, smoother, precision, seed, nterms):
         spaces.append(s)
     ints = [jf.TPCellIntegral("GradGrad")]
     t0 = time.perf_counter()
-    G = jf.MGPreconditioner(spaces, ints, smoother, cycles=1, fmg=False, precision=precision)
+    G = jf.MGPreconditioner(spaces, ints, smoother, cycles=1, fmg=False, precision=precision, cell_coeff=cell_coeff)
     t_mg = time.perf_counter() - t0
     return spaces, G, t_mg
 
@@ -232,6 +232,79 @@ def bench_c4(jf, args):
     print(json.dumps(line), flush=True)
 
 
+def bench_c3(jf, args):
+    """3D -div(a grad u) = f, Q2 on 96^3 cells, a = exp(g) per cell with g a seeded
+    squared-exponential Gaussian random field (length 0.1, contrast 1e4); the
+    coarser levels average the coefficient of their 2^3 children.  Time to
+    ||r|| <= 1e-8 ||b|| of FP64 MG-PCG and of the FP32 multigrid inside FP64
+    flexible CG (the FP64 residual decides convergence), and the relative L2
+    difference of the two solutions."""
+    from jfem_amd.workloads import gaussian_random_field
+    nlev, c = 1, args.cells
+    while c % 2 == 0 and c > 3:
+        c //= 2
+        nlev += 1
+    base = [c, c, c]
+    g = gaussian_random_field(3, (args.cells,) * 3, 0.1, 1e4)
+    coeff = np.exp(g)
+    smoother = jf.ChebyshevSmoother(degree=4, runs=2, ratio=20.0)
+    spaces, G, t_mg = build_problem(jf, base, nlev, 2, smoother, jf.FP64, 3, 16, cell_coeff=coeff)
+    fine = spaces[-1]
+    A = G.getFinestSystem()
+    n, _, nnz = A.info()
+    b, rhs = rhs_for(jf, fine, 3, 16)
+    bnorm = float(np.linalg.norm(rhs))
+    tol = 1e-8 * bnorm
+    ctx = jf.context()
+
+    def timed(solve):
+        for _ in range(args.warmup):
+            solve()
+        ctx.sync()
+        t0 = time.perf_counter()
+        ts = []
+        for _ in range(args.steps):
+            it, x = solve()
+            ts.append(it.info.seconds)
+        ctx.sync()
+        return it, x, float(np.median(ts)), (time.perf_counter() - t0) / args.steps * 1e3
+
+    def pcg64():
+        it = jf.IterativeSolver()
+        x = it.solvePCG(A, G, b, tol, x0=jf.DenseVector(n))
+        return it, x
+
+    it64, x64, t64, ms64 = timed(pcg64)
+    log(f"FP64 MG-PCG: {it64.iterations} its, {t64:.3f}s")
+    x64 = x64.numpy()
+    del G
+    _, G32, _ = build_problem(jf, base, nlev, 2, smoother, jf.FP32, 3, 16, cell_coeff=coeff)
+    A32 = G32.getFinestSystem()
+
+    def fcg32():
+        it = jf.IterativeSolver()
+        x = it.solveFCG(A32, G32, b, tol)
+        return it, x
+
+    it32, x32, t32, ms32 = timed(fcg32)
+    log(f"FP32 MG in FP64 FCG: {it32.iterations} its, {t32:.3f}s")
+    r = A.mvMul(x32)
+    line = {
+        "metric": "time-to-1e-8 residual (s): FP32 MG-FCG vs FP64 MG-PCG, 3D Q2 heterogeneous coefficient",
+        "value": t32, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms32,
+        "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f64 outer / f32 multigrid",
+        "data": "synthetic",
+        "config": {"workload": f"3D -div(a grad u)=f, Q2 {args.cells}^3 cells, a=exp(GRF) length 0.1 contrast 1e4",
+                   "ndofs": n, "nnz": nnz, "levels": nlev},
+        "fp64_mgpcg_s": t64, "fp64_mgpcg_iterations": it64.iterations, "fp64_ms_per_step": ms64,
+        "fp32mg_fcg_iterations": it32.iterations, "speedup_fp32mg_vs_fp64": t64 / t32,
+        "fp32mg_true_rel_residual": float(np.linalg.norm(rhs - r.numpy()) / bnorm),
+        "fp32mg_vs_fp64_rel_l2": float(np.linalg.norm(x32.numpy() - x64) / np.linalg.norm(x64)),
+        "hierarchy_setup_s": t_mg,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,9 +314,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-smoother solve")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c4"],
-                    help="c2: 3D Q2 96^3 FP64 MG-PCG (default); c4: 2D Q1 1024^2 log-normal coefficient, "
-                         "learned smoother vs Chebyshev")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"],
+                    help="c2: 3D Q2 96^3 learned + FP32 MG vs FP64 MG-PCG (default); c3: the same grid with a "
+                         "log-normal coefficient (contrast 1e4), FP32 MG in FP64 FCG vs FP64 MG-PCG; c4: 2D Q1 "
+                         "1024^2 log-normal coefficient, learned smoother vs Chebyshev")
     ap.add_argument("--train-fields", type=int, default=16)
     ap.add_argument("--pairs", type=int, default=64, help="(residual, error) pairs per training field")
     ap.add_argument("--eval-fields", type=int, default=8)
@@ -262,6 +336,9 @@ def main():
         return
     if args.workload == "c4":
         bench_c4(jf, args)
+        return
+    if args.workload == "c3":
+        bench_c3(jf, args)
         return
 
     nlev = 1
