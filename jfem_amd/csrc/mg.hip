@@ -46,6 +46,7 @@ struct LearnedState {
     jfem_nn* nn = nullptr;       // borrowed
     jfem::Lat lat;               // finest node lattice
     float* stat = nullptr;       // np x (cin - 1) static input channels
+    int* perm = nullptr;         // lattice index of every finest DOF
     int runs = 1;                // learned steps per smoothing call
     int active = 0;
     int fell_back = 0;
@@ -676,7 +677,7 @@ static void free_mg(jfem_mg* G)
     (void)hipFree(G->coarse_inv);
     (void)hipFree(G->init); (void)hipFree(G->defect); (void)hipFree(G->tmp);
     LearnedState& S = G->ls;
-    for (void* p : {(void*)S.stat, (void*)S.rr, (void*)S.u, (void*)S.z, (void*)S.gd, (void*)S.rh, (void*)S.xs,
+    for (void* p : {(void*)S.stat, (void*)S.perm, (void*)S.rr, (void*)S.u, (void*)S.z, (void*)S.gd, (void*)S.rh, (void*)S.xs,
                     (void*)S.bs, (void*)S.loss, (void*)S.capbuf, (void*)S.capx, (void*)S.xt, (void*)S.eh,
                     (void*)S.w, (void*)S.y, (void*)S.loss2})
         (void)hipFree(p);
@@ -813,18 +814,25 @@ __device__ inline long pix_of(const GridDev& g, const Lat& L, long i)
     return p[0] + (long)L.n[0] * (p[1] + (long)L.n[1] * p[2]);
 }
 
+// lattice index of every DOF (computed once per attached network)
+__global__ void k_lattice_perm(GridDev g, Lat L, long n, int* __restrict__ perm)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        perm[i] = (int)pix_of(g, L, i);
+}
+
 // network input: channel 0 = s * sqrt(n / ss) with s = D^-1 r the Jacobi-scaled
 // residual and ss = |s|^2 (so the input has rms 1 and does not depend on the
 // mesh size or the local coefficient scale), channel c >= 1 = channel 0 times
 // static channel c (so that the bias-free network is positively homogeneous in r)
 template <typename T>
-__global__ void k_gather_in(GridDev g, Lat L, long n, const T* __restrict__ r, const T* __restrict__ dinv,
+__global__ void k_gather_in(const int* __restrict__ perm, long n, const T* __restrict__ r, const T* __restrict__ dinv,
                             const double* __restrict__ rr, const float* __restrict__ stat, int cin,
                             float* __restrict__ in)
 {
     const double scale = rr[0] > 0 ? sqrt((double)n / rr[0]) : 0.0;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const long pix = pix_of(g, L, i);
+        const long pix = perm[i];
         const float v = (float)((double)dinv[i] * (double)r[i] * scale);
         in[pix * cin] = v;
         for (int c = 1; c < cin; c++) in[pix * cin + c] = v * stat[pix * (cin - 1) + c - 1];
@@ -833,34 +841,34 @@ __global__ void k_gather_in(GridDev g, Lat L, long n, const T* __restrict__ r, c
 
 // x[i] += out[pix] * rms(D^-1 r) on free nodes
 template <typename T>
-__global__ void k_scatter_add(GridDev g, Lat L, long n, const float* __restrict__ out, const double* __restrict__ rr,
+__global__ void k_scatter_add(const int* __restrict__ perm, long n, const float* __restrict__ out, const double* __restrict__ rr,
                               const unsigned char* __restrict__ bnd, T* __restrict__ x)
 {
     const double inv = rr[0] > 0 ? sqrt(rr[0] / (double)n) : 0.0;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         if (bnd[i]) continue;
-        x[i] = (T)((double)x[i] + (double)out[pix_of(g, L, i)] * inv);
+        x[i] = (T)((double)x[i] + (double)out[perm[i]] * inv);
     }
 }
 
 // training helpers: u = masked network output (DOF order), rh = r * sqrt(n / ss)
-__global__ void k_train_unpack(GridDev g, Lat L, long n, const float* __restrict__ out, const double* __restrict__ r,
+__global__ void k_train_unpack(const int* __restrict__ perm, long n, const float* __restrict__ out, const double* __restrict__ r,
                                const double* __restrict__ rr, const unsigned char* __restrict__ bnd,
                                double* __restrict__ u, double* __restrict__ rh)
 {
     const double scale = rr[0] > 0 ? sqrt((double)n / rr[0]) : 0.0;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        u[i] = bnd[i] ? 0.0 : (double)out[pix_of(g, L, i)];
+        u[i] = bnd[i] ? 0.0 : (double)out[perm[i]];
         rh[i] = r[i] * scale;
     }
 }
 
 // dLoss/dout for the energy loss (u.A u - 2 u.rh) / n: gout = -2 z / n on free nodes, z = rh - A u
-__global__ void k_train_gout(GridDev g, Lat L, long n, const double* __restrict__ z,
+__global__ void k_train_gout(const int* __restrict__ perm, long n, const double* __restrict__ z,
                              const unsigned char* __restrict__ bnd, float* __restrict__ gout)
 {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        gout[pix_of(g, L, i)] = bnd[i] ? 0.f : (float)(-2.0 * z[i] / (double)n);
+        gout[perm[i]] = bnd[i] ? 0.f : (float)(-2.0 * z[i] / (double)n);
 }
 
 // partials of -u.(z + rh)
@@ -924,11 +932,11 @@ __global__ void k_rand_rhs(long n, unsigned long long seed, const unsigned char*
 
 // u = network output on free nodes (DOF order)
 template <typename T>
-__global__ void k_unpack_out(GridDev g, Lat L, long n, const float* __restrict__ out, const unsigned char* __restrict__ bnd,
+__global__ void k_unpack_out(const int* __restrict__ perm, long n, const float* __restrict__ out, const unsigned char* __restrict__ bnd,
                              T* __restrict__ u)
 {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        u[i] = bnd[i] ? (T)0 : (T)out[pix_of(g, L, i)];
+        u[i] = bnd[i] ? (T)0 : (T)out[perm[i]];
 }
 
 // partials of (r.w), (w.w) (minimal residual, energy = 0) or (u.r), (u.w)
@@ -989,15 +997,15 @@ static void learned_smooth(jfem_mg* G, MgLevel& L, const T* b, T* x, T* u, bool 
             level_residual<T>(G, L, b, x, r);
         }
         sumsq_dev<T>(c, n, r, dinv_of<T>(L), S.rr);
-        k_gather_in<T><<<gr, 256, 0, st>>>(to_dev(L.grid), S.lat, n, r, dinv_of<T>(L), S.rr, S.stat, S.nn->cin,
+        k_gather_in<T><<<gr, 256, 0, st>>>(S.perm, n, r, dinv_of<T>(L), S.rr, S.stat, S.nn->cin,
                                            S.nn->in);
         nn_forward(S.nn, S.lat);
         if (!S.mr) {
-            k_scatter_add<T><<<gr, 256, 0, st>>>(to_dev(L.grid), S.lat, n, S.nn->out, S.rr, L.bnd, x);
+            k_scatter_add<T><<<gr, 256, 0, st>>>(S.perm, n, S.nn->out, S.rr, L.bnd, x);
             JFEM_HIP(hipGetLastError());
             continue;
         }
-        k_unpack_out<T><<<gr, 256, 0, st>>>(to_dev(L.grid), S.lat, n, S.nn->out, L.bnd, u);
+        k_unpack_out<T><<<gr, 256, 0, st>>>(S.perm, n, S.nn->out, L.bnd, u);
         launch_stream<T, T>(L.A, vals_of<T>(L), u, EpiStore<T>{w}, nullptr);
         const int blocks = grid_for(n, 256, RED_BLOCKS);
         k_dot2<T><<<blocks, 256, 0, st>>>(n, r, w, u, S.mr == 2 ? 1 : 0, c->red_partials);
@@ -1123,6 +1131,11 @@ static void set_learned(jfem_mg* G, jfem_nn* nn, const double* cell_logc, int ru
     S.stat = nullptr;
     JFEM_HIP(hipMalloc(&S.stat, sizeof(float) * stat.size()));
     JFEM_HIP(hipMemcpy(S.stat, stat.data(), sizeof(float) * stat.size(), hipMemcpyHostToDevice));
+    (void)hipFree(S.perm);
+    S.perm = nullptr;
+    JFEM_HIP(hipMalloc(&S.perm, sizeof(int) * np));
+    k_lattice_perm<<<grid_for(np, 256, 16384), 256, 0, G->ctx->stream>>>(to_dev(g), S.lat, np, S.perm);
+    JFEM_HIP(hipGetLastError());
     if (!S.rr) JFEM_HIP(hipMalloc(&S.rr, sizeof(double) * 8));
     nn_reserve(nn, np);
     nn_set_bias_free(nn, 1);   // N(0) = 0 and N(c r) = c N(r): a consistent smoother
@@ -1208,10 +1221,10 @@ static int train_mg(jfem_mg* G, const jfem_train_opts* o, double* loss_host)
                 launch_stream<double, double>(F.A, F.A->vals, S.xs, EpiResidual<double>{S.bs, r}, nullptr);
             }
             sumsq_dev<double>(c, n, r, F.dinv, S.rr);
-            k_gather_in<double><<<gr, 256, 0, st>>>(to_dev(F.grid), S.lat, n, r, F.dinv, S.rr, S.stat, S.nn->cin,
+            k_gather_in<double><<<gr, 256, 0, st>>>(S.perm, n, r, F.dinv, S.rr, S.stat, S.nn->cin,
                                                     S.nn->in);
             nn_forward(S.nn, S.lat);
-            k_train_unpack<<<gr, 256, 0, st>>>(to_dev(F.grid), S.lat, n, S.nn->out, r, S.rr, F.bnd, S.u, S.rh);
+            k_train_unpack<<<gr, 256, 0, st>>>(S.perm, n, S.nn->out, r, S.rr, F.bnd, S.u, S.rh);
             launch_stream<double, double>(F.A, F.A->vals, S.u, EpiResidual<double>{S.rh, S.z}, nullptr);   // z = rh - A u
             const int blocks = grid_for(n, 256, RED_BLOCKS);
             if (twogrid) {
@@ -1223,12 +1236,12 @@ static int train_mg(jfem_mg* G, const jfem_train_opts* o, double* loss_host)
                 launch_stream<double, double>(F.A, F.A->vals, S.w, EpiResidual<double>{S.z, S.y}, nullptr);
                 k_twogrid_dots<<<blocks, 256, 0, st>>>(n, S.eh, S.u, S.w, S.y, S.rh, c->red_partials);
                 reduce_partials_dev(c, 2, blocks, S.loss2 + 2 * step);
-                k_train_gout<<<gr, 256, 0, st>>>(to_dev(F.grid), S.lat, n, S.y, F.bnd, S.nn->gout);
+                k_train_gout<<<gr, 256, 0, st>>>(S.perm, n, S.y, F.bnd, S.nn->gout);
             } else {
                 // energy loss |e - u|_A^2 - |e|_A^2 = u.A u - 2 u.rh = -u.(z + rh), gradient 2 (A u - rh) = -2 z
                 k_energy<<<blocks, 256, 0, st>>>(n, S.u, S.z, S.rh, c->red_partials);
                 reduce_partials_dev(c, 1, blocks, S.loss + step);
-                k_train_gout<<<gr, 256, 0, st>>>(to_dev(F.grid), S.lat, n, S.z, F.bnd, S.nn->gout);
+                k_train_gout<<<gr, 256, 0, st>>>(S.perm, n, S.z, F.bnd, S.nn->gout);
             }
             JFEM_HIP(hipGetLastError());
             nn_backward(S.nn, S.lat, 0);

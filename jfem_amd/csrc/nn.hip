@@ -870,7 +870,7 @@ void nn_backward(jfem_nn* nn, const Lat& L, int add)
     const int taps = nn->taps;
     // weight gradients: nbw blocks over contiguous pixel ranges (multiples of the LDS tile)
     const long tiles = (L.np + DW_TP - 1) / DW_TP;
-    const int nbw = (int)std::min<long>(tiles, 512);
+    const int nbw = (int)std::max<long>(1, std::min<long>(tiles / 8, 512));   // >= 8 tiles per block
     const long wchunk = (tiles + nbw - 1) / nbw * DW_TP;
     // bias gradients: 4096-pixel chunks
     const long bchunk = 4096;
