@@ -459,6 +459,17 @@ def main():
             ctx.sync()
             extra["learned_ms_per_step"] = (time.perf_counter() - t_start) / args.steps * 1e3
             rl = A.mvMul(xl)
+            # stationary V-cycle iteration with each smoother (FP64 hierarchy, same network)
+            G64 = build_problem(jf, base, nlev, 2, smoother, jf.FP64, 2, 16)[1]
+            G64.set_learned(net, np.zeros(ncell), runs=1, fallback_ratio=0.5)
+            for name, active in (("chebyshev", False), ("learned", True)):
+                G64.set_learned_active(active)
+                G64.solve(b, jf.DenseVector(n), tol, 100)
+                info, _ = G64.solve(b, jf.DenseVector(n), tol, 100)
+                extra[f"vcycle_{name}_cycles"] = info.iterations
+                extra[f"vcycle_{name}_s"] = info.seconds
+                log(f"V-cycle iteration, {name}: {info.iterations} cycles, {info.seconds:.3f}s")
+            del G64
             extra.update({
                 "learned_fp32mg_fcg_s": float(np.median(tl)), "learned_fp32mg_fcg_iterations": it.iterations,
                 "learned_fell_back": bool(fb), "learned_train_s": t_train,

@@ -18,6 +18,7 @@
 namespace jfem {
 
 typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bfv8 __attribute__((ext_vector_type(8)));   // the same bits as bf16 for the dot-product builtins
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // round-to-nearest-even float -> bf16 bits
@@ -251,20 +252,29 @@ template <int D, int CH>
 __device__ inline void stage_halo(const Lat& L, const unsigned short* __restrict__ src, int x0, int y0, int z0,
                                   bf16x8* tile)
 {
+    // all loads of a thread are issued before any LDS store, so that they are in flight together
     using T = MidTile<D>;
-    for (int f = threadIdx.x; f < T::HP * CH; f += T::NW * 64) {
+    constexpr int NT = T::NW * 64, NF = T::HP * CH, PER = (NF + NT - 1) / NT;
+    bf16x8 v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int f = threadIdx.x + k * NT;
         const int hp = f / CH, ch = f % CH;
         const int hx = hp % T::HX, hy = (hp / T::HX) % T::HY, hz = hp / (T::HX * T::HY);
         const int gx = x0 - 1 + hx, gy = D >= 2 ? y0 - 1 + hy : 0, gz = D == 3 ? z0 - 1 + hz : 0;
-        bf16x8 v;
-        if ((unsigned)gx < (unsigned)L.n[0] && (unsigned)gy < (unsigned)L.n[1] && (unsigned)gz < (unsigned)L.n[2])
-            v = *reinterpret_cast<const bf16x8*>(src + ((long)gx + (long)L.n[0] * (gy + (long)L.n[1] * gz)) * (CH * 8) +
-                                                 ch * 8);
+        if (f < NF && (unsigned)gx < (unsigned)L.n[0] && (unsigned)gy < (unsigned)L.n[1] &&
+            (unsigned)gz < (unsigned)L.n[2])
+            v[k] = *reinterpret_cast<const bf16x8*>(src + ((long)gx + (long)L.n[0] * (gy + (long)L.n[1] * gz)) *
+                                                              (CH * 8) + ch * 8);
         else {
 #pragma unroll
-            for (int e = 0; e < 8; e++) v[e] = 0;
+            for (int e = 0; e < 8; e++) v[k][e] = 0;
         }
-        tile[halo_slot<CH>(hp, ch)] = v;
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int f = threadIdx.x + k * NT;
+        if (f < NF) tile[halo_slot<CH>(f / CH, f % CH)] = v[k];
     }
 }
 
@@ -439,7 +449,7 @@ __global__ __launch_bounds__(MidTile<D>::NW * 64) void k_first_lds(Lat L, int ci
 // pixels tid, tid + NW*64, ... of the tile
 template <int D>
 __global__ __launch_bounds__(MidTile<D>::NW * 64) void k_last_lds(Lat L, const unsigned short* __restrict__ act,
-                                                                 const float* __restrict__ W3,
+                                                                 const unsigned short* __restrict__ W3h,
                                                                  const float* __restrict__ b3, float* __restrict__ out)
 {
     using T = MidTile<D>;
@@ -463,9 +473,17 @@ __global__ __launch_bounds__(MidTile<D>::NW * 64) void k_last_lds(Lat L, const u
                 const int hp = (hz * T::HY + hy) * T::HX + hx;
 #pragma unroll
                 for (int g = 0; g < 4; g++) {
-                    const bf16x8 h = tile[halo_slot<4>(hp, g)];
-#pragma unroll
-                    for (int e = 0; e < 8; e++) acc += W3[t * NN_H + 8 * g + e] * bf2f((unsigned short)h[e]);
+                    const bf16x8 hs = tile[halo_slot<4>(hp, g)];
+                    const bfv8 a = __builtin_bit_cast(bfv8, hs);
+                    const bfv8 w = *reinterpret_cast<const bfv8*>(W3h + t * NN_H + 8 * g);
+                    acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_shufflevector(a, a, 0, 1),
+                                                          __builtin_shufflevector(w, w, 0, 1), acc, false);
+                    acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_shufflevector(a, a, 2, 3),
+                                                          __builtin_shufflevector(w, w, 2, 3), acc, false);
+                    acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_shufflevector(a, a, 4, 5),
+                                                          __builtin_shufflevector(w, w, 4, 5), acc, false);
+                    acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_shufflevector(a, a, 6, 7),
+                                                          __builtin_shufflevector(w, w, 6, 7), acc, false);
                 }
             }
             out[(long)gx + (long)L.n[0] * (gy + (long)L.n[1] * gz)] = acc;
@@ -753,6 +771,43 @@ __global__ void k_refresh_bf16(int taps, const float* __restrict__ W, unsigned s
     }
 }
 
+__global__ void k_to_bf16(int n, const float* __restrict__ src, unsigned short* __restrict__ dst)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = f2bf(src[i]);
+}
+
+// last layer with packed BF16 dot products (v_dot2_f32_bf16, FP32 accumulation):
+// out[p] = b3 + sum_t sum_c W3[t][c] act[nb(p, t)][c], W3 in BF16
+template <int TAPS>
+__global__ __launch_bounds__(256) void k_last_dot(Lat L, const unsigned short* __restrict__ act,
+                                                  const unsigned short* __restrict__ W3h,
+                                                  const float* __restrict__ b3, float* __restrict__ out)
+{
+    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < L.np; p += (long)gridDim.x * blockDim.x) {
+        int ip[3];
+        lat_xyz(L, p, ip);
+        float acc = b3[0];
+        for (int t = 0; t < TAPS; t++) {
+            const long q = lat_nb(L, p, ip, t, 1);
+            if (q < 0) continue;
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const bfv8 a = *reinterpret_cast<const bfv8*>(act + q * NN_H + 8 * g);
+                const bfv8 w = *reinterpret_cast<const bfv8*>(W3h + t * NN_H + 8 * g);
+                acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_shufflevector(a, a, 0, 1),
+                                                      __builtin_shufflevector(w, w, 0, 1), acc, false);
+                acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_shufflevector(a, a, 2, 3),
+                                                      __builtin_shufflevector(w, w, 2, 3), acc, false);
+                acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_shufflevector(a, a, 4, 5),
+                                                      __builtin_shufflevector(w, w, 4, 5), acc, false);
+                acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_shufflevector(a, a, 6, 7),
+                                                      __builtin_shufflevector(w, w, 6, 7), acc, false);
+            }
+        }
+        out[p] = acc;
+    }
+}
+
 // ------------------------------------------------------------- host side
 static long param_count(int taps, int cin)
 {
@@ -781,6 +836,7 @@ void nn_refresh(jfem_nn* nn)
     hipStream_t st = nn->ctx->stream;
     k_refresh_bf16<<<64, 256, 0, st>>>(nn->taps, nn->w + o[2], nn->wh1, nn->wt1);
     k_refresh_bf16<<<64, 256, 0, st>>>(nn->taps, nn->w + o[4], nn->wh2, nn->wt2);
+    k_to_bf16<<<4, 256, 0, st>>>(nn->taps * NN_H, nn->w + o[6], nn->wh3);
     JFEM_HIP(hipGetLastError());
 }
 
@@ -825,7 +881,8 @@ void nn_forward(jfem_nn* nn, const Lat& L)
                                                                                                  b0, nn->act0);
             launch_mid<0>(st, L, nn->taps, nn->act0, nn->wh1, nn->w + o[3], nullptr, nn->act1);
             launch_mid<0>(st, L, nn->taps, nn->act1, nn->wh2, nn->w + o[5], nullptr, nn->act2);
-            k_last<27><<<g, 256, 0, st>>>(L, 27, nn->act2, nn->w + o[6], nn->w + o[7], nn->out);
+            k_last_lds<3><<<grid, T::NW * 64, sizeof(bf16x8) * 4 * T::HP, st>>>(L, nn->act2, nn->wh3, nn->w + o[7],
+                                                                               nn->out);
         } else {
             using T = MidTile<2>;
             const long nt = (long)((L.n[0] + 31) / 32) * ((L.n[1] + T::TY - 1) / T::TY);
@@ -834,7 +891,7 @@ void nn_forward(jfem_nn* nn, const Lat& L)
                                                                                                 b0, nn->act0);
             launch_mid<0>(st, L, nn->taps, nn->act0, nn->wh1, nn->w + o[3], nullptr, nn->act1);
             launch_mid<0>(st, L, nn->taps, nn->act1, nn->wh2, nn->w + o[5], nullptr, nn->act2);
-            k_last_lds<2><<<grid, T::NW * 64, sizeof(bf16x8) * 4 * T::HP, st>>>(L, nn->act2, nn->w + o[6], nn->w + o[7],
+            k_last_lds<2><<<grid, T::NW * 64, sizeof(bf16x8) * 4 * T::HP, st>>>(L, nn->act2, nn->wh3, nn->w + o[7],
                                                                                nn->out);
         }
         JFEM_HIP(hipGetLastError());
@@ -976,6 +1033,7 @@ jfem_nn* nn_create(jfem_ctx* c, int dim, int cin, unsigned long long seed)
     JFEM_HIP(hipMalloc(&nn->wt1, hb));
     JFEM_HIP(hipMalloc(&nn->wh2, hb));
     JFEM_HIP(hipMalloc(&nn->wt2, hb));
+    JFEM_HIP(hipMalloc(&nn->wh3, sizeof(unsigned short) * nn->taps * NN_H));
     // He initialisation from SplitMix64 + Box-Muller; biases 0, last layer scaled down
     std::vector<float> w(nn->nparams, 0.f);
     long o[9];
@@ -1014,8 +1072,9 @@ void nn_free(jfem_nn* nn)
     if (!nn) return;
     (void)hipStreamSynchronize(nn->ctx->stream);
     for (void* p : {(void*)nn->w, (void*)nn->grad, (void*)nn->m, (void*)nn->v, (void*)nn->wh1, (void*)nn->wt1,
-                    (void*)nn->wh2, (void*)nn->wt2, (void*)nn->act0, (void*)nn->act1, (void*)nn->act2, (void*)nn->g0,
-                    (void*)nn->g1, (void*)nn->g2, (void*)nn->in, (void*)nn->out, (void*)nn->gout, (void*)nn->part})
+                    (void*)nn->wh2, (void*)nn->wt2, (void*)nn->wh3, (void*)nn->act0, (void*)nn->act1, (void*)nn->act2,
+                    (void*)nn->g0, (void*)nn->g1, (void*)nn->g2, (void*)nn->in, (void*)nn->inb, (void*)nn->out,
+                    (void*)nn->gout, (void*)nn->part})
         (void)hipFree(p);
     delete nn;
 }

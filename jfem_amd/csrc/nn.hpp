@@ -27,6 +27,7 @@ struct jfem_nn {
     int bias_free = 0;                  // biases held at zero: the network is positively homogeneous
     float *w = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;   // FP32 master weights, gradient, Adam moments
     unsigned short *wh1 = nullptr, *wt1 = nullptr, *wh2 = nullptr, *wt2 = nullptr;   // bf16 copies of the 32->32 layers
+    unsigned short* wh3 = nullptr;      // bf16 copy of the last layer's weights
     // work buffers for np <= cap pixels
     long cap = 0;
     unsigned short *act0 = nullptr, *act1 = nullptr, *act2 = nullptr;   // hidden activations (post-ReLU), NHWC bf16
