@@ -17,26 +17,8 @@
 
 namespace jfem {
 
-typedef short bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bfv8 __attribute__((ext_vector_type(8)));   // the same bits as bf16 for the dot-product builtins
 typedef float f32x16 __attribute__((ext_vector_type(16)));
-
-// round-to-nearest-even float -> bf16 bits
-__host__ __device__ inline unsigned short f2bf(float f)
-{
-    unsigned int u;
-    __builtin_memcpy(&u, &f, 4);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)(u >> 16);   // inf/nan
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
-__host__ __device__ inline float bf2f(unsigned short h)
-{
-    unsigned int u = (unsigned int)h << 16;
-    float f;
-    __builtin_memcpy(&f, &u, 4);
-    return f;
-}
 
 // lattice coordinates of pixel p (32-bit arithmetic: np < 2^31 is checked in
 // make_lat); unused axes have extent 1 and coordinate 0
@@ -354,10 +336,7 @@ __global__ void k_in_split(long np, int cin, const float* __restrict__ in, unsig
         bf16x8 hi, lo;
 #pragma unroll
         for (int c = 0; c < 8; c++) {
-            const float v = c < cin ? in[p * cin + c] : 0.f;
-            const unsigned short h = f2bf(v);
-            hi[c] = (short)h;
-            lo[c] = (short)f2bf(v - bf2f(h));
+            split_bf16(c < cin ? in[p * cin + c] : 0.f, hi, lo, c);
         }
         *reinterpret_cast<bf16x8*>(inb + p * 16) = hi;
         *reinterpret_cast<bf16x8*>(inb + p * 16 + 8) = lo;
@@ -863,16 +842,23 @@ void nn_reserve(jfem_nn* nn, long np)
 static int nn_grid(long np) { return grid_for(np, 256, 8192); }
 
 // forward on nn->in (np x cin FP32, lattice order) into nn->out (np FP32)
-void nn_forward(jfem_nn* nn, const Lat& L)
+bool nn_split_input(const jfem_nn* nn, const Lat& L)
+{
+    return (L.d == 2 || L.d == 3) && nn->cin <= 8 && lds_per_block() >= (long)sizeof(bf16x8) * 4 * MidTile<3>::HP;
+}
+
+void nn_forward(jfem_nn* nn, const Lat& L, bool inb_ready)
 {
     long o[9];
     nn_offsets(nn, o);
     hipStream_t st = nn->ctx->stream;
     const int g = nn_grid(L.np);
     const float *W0 = nn->w + o[0], *b0 = nn->w + o[1];
-    if ((L.d == 2 || L.d == 3) && nn->cin <= 8 && lds_per_block() >= (long)sizeof(bf16x8) * 4 * MidTile<3>::HP) {
+    if (inb_ready && !nn_split_input(nn, L))
+        throw Error{JFEM_ERR_STATE, "nn_forward: split-BF16 input on a lattice without the LDS-tiled layers"};
+    if (nn_split_input(nn, L)) {
         // 2D / 3D lattices: first and last layers with LDS tiles, first layer on MFMA
-        k_in_split<<<g, 256, 0, st>>>(L.np, nn->cin, nn->in, nn->inb);
+        if (!inb_ready) k_in_split<<<g, 256, 0, st>>>(L.np, nn->cin, nn->in, nn->inb);
         if (L.d == 3) {
             using T = MidTile<3>;
             const long nt = (long)((L.n[0] + 31) / 32) * ((L.n[1] + T::TY - 1) / T::TY) * ((L.n[2] + T::TZ - 1) / T::TZ);

@@ -17,6 +17,34 @@ struct Lat {
 
 Lat make_lat(int d, const int* n);
 
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+
+// round-to-nearest-even float -> bf16 bits
+__host__ __device__ inline unsigned short f2bf(float f)
+{
+    unsigned int u;
+    __builtin_memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)(u >> 16);   // inf/nan
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__host__ __device__ inline float bf2f(unsigned short h)
+{
+    unsigned int u = (unsigned int)h << 16;
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+}
+
+// split-BF16 pair of an FP32 value: hi = bf16(v), lo = bf16(v - hi); hi + lo
+// carries about 16 mantissa bits
+__device__ inline void split_bf16(float v, bf16x8& hi, bf16x8& lo, int c)
+{
+    const unsigned short h = f2bf(v);
+    hi[c] = (short)h;
+    lo[c] = (short)f2bf(v - bf2f(h));
+}
+
 }  // namespace jfem
 
 struct jfem_nn {
@@ -42,7 +70,11 @@ struct jfem_nn {
 
 namespace jfem {
 void nn_reserve(jfem_nn* nn, long np);
-void nn_forward(jfem_nn* nn, const Lat& L);
+// forward on nn->in, or, when inb_ready, on nn->inb already written by the
+// caller (only valid where nn_split_input(nn, L) holds)
+void nn_forward(jfem_nn* nn, const Lat& L, bool inb_ready = false);
+// true where the forward reads its input as the split-BF16 copy nn->inb
+bool nn_split_input(const jfem_nn* nn, const Lat& L);
 void nn_backward(jfem_nn* nn, const Lat& L, int add);
 void nn_adam_step(jfem_nn* nn, float lr, float grad_scale);
 void nn_set_bias_free(jfem_nn* nn, int on);

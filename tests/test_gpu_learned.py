@@ -117,3 +117,90 @@ def test_fp32_learned_mg_in_fp64_fcg(jf, trained):
     assert np.linalg.norm(rhs - S @ x) <= 1.01 * tol
     exact = spla.spsolve(S.tocsc(), rhs)
     assert np.linalg.norm(x - exact) <= 1e-6 * np.linalg.norm(exact)
+
+
+def _learned_step_oracle(NO, params, cin, A, dinv, bnd, perm, shape, stat, x, b):
+    """One plain learned smoothing step in FP64: s = D^-1 (b - A x), the network
+    input in lattice order is s * sqrt(n / |s|^2) times (1, static channels),
+    and x += N(input) * sqrt(|s|^2 / n) on the free nodes"""
+    n = len(b)
+    s = dinv * (b - A @ x)
+    ss = float(s @ s)
+    v = np.zeros(n)
+    v[perm] = s * np.sqrt(n / ss)
+    inp = np.concatenate([v[:, None], v[:, None] * stat], axis=1)
+    out, _ = NO.forward(params, shape, inp, cin)
+    x = x.copy()
+    free = np.ones(n, bool)
+    free[bnd] = False
+    x[free] += out[perm[free]] * np.sqrt(ss / n)
+    return x
+
+
+@pytest.mark.parametrize("base,k,precision", [([4, 5], 1, 0), ([2, 2, 3], 2, 0), ([2, 2, 3], 2, 1)])
+def test_learned_vcycle_matches_oracle(jf, oracle, base, k, precision):
+    """A two-level V-cycle with the learned smoother on the finest level against
+    its FP64 restatement: learned step, coarse-grid correction (the oracle's
+    V-cycle with no smoothing steps), learned step.  The DOF -> lattice map comes
+    from the node coordinates and the static channels from the oracle's cell ->
+    DOF table, so a wrong permutation, scaling or channel fails by O(1); the
+    device network runs in BF16 (2e-2 relative, as in the network tests), in
+    an FP64 or an FP32 hierarchy."""
+    from oracle import nn_oracle as NO
+    d = len(base)
+    start, end = [0.0] * d, [1.0] * d
+    cells = [c << 1 for c in base]
+    spaces = []
+    for l in range(2):
+        s = jf.ContinuousTPFESpace(start, end, [c << l for c in base])
+        s.assembleCells()
+        s.assembleFunctions(k)
+        spaces.append(s)
+    G = jf.MGPreconditioner(spaces, [jf.TPCellIntegral("GradGrad")], jf.ChebyshevSmoother(4, 2), cycles=1,
+                            fmg=False, precision=precision)
+    cin = 2 + d
+    net = jf.SmootherNet(d, cin, seed=3)
+    ncell = int(np.prod(cells))
+    rng = np.random.default_rng(17)
+    logc = rng.standard_normal(ncell)
+    G.set_learned(net, logc, runs=1, fallback_ratio=10.0)
+    params = net.get_params().astype(np.float64)   # biases zeroed by set_learned
+    O = oracle.MG(base, k, 2, start, end, [0], [1.0], smoother=oracle.SM_RICHARDSON, runs=0, omega=0.1)
+    A = O.level_matrix(1).to_scipy()
+    dinv = O.level_dinv(1)
+    bnd = oracle.boundary_nodes(cells, k)
+    shape = [c * k + 1 for c in cells]
+    q = np.rint(oracle.node_coords(cells, k, start, end) * np.array(cells) * k).astype(np.int64)
+    stride = np.cumprod([1] + shape[:-1])
+    perm = q @ stride
+    n = A.shape[0]
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    # static channels: mean log-coefficient over the cells that hold the node, lattice parity per axis
+    _, table = oracle.dof_table(cells, k)
+    acc, cnt = np.zeros(n), np.zeros(n)
+    for c in range(ncell):
+        acc[table[c]] += logc[c]
+        cnt[table[c]] += 1
+    stat = np.zeros((n, 1 + d))
+    stat[perm, 0] = acc / cnt
+    stat[perm, 1:] = q % 2
+    stat = stat.astype(np.float32).astype(np.float64)
+
+    b = rng.standard_normal(n)
+    x0 = rng.standard_normal(n)
+    got = G.vCycle(jf.DenseVector(x0), jf.DenseVector(b)).numpy()
+
+    def cycle(perm_, shape_, stat_):
+        x1 = _learned_step_oracle(NO, params, cin, A, dinv, bnd, perm_, shape_, stat_, x0, b)
+        x2 = O.vcycle(x1, b)
+        return _learned_step_oracle(NO, params, cin, A, dinv, bnd, perm_, shape_, stat_, x2, b)
+
+    exp = cycle(perm, shape, stat)
+    err = np.linalg.norm(got - exp) / np.linalg.norm(exp - x0)
+    assert err <= 2e-2, err
+    # the same cycle with the DOF index read as the lattice index (no permutation) is far away
+    ident = np.arange(n)
+    istat = np.zeros_like(stat)
+    istat[ident] = stat[perm]
+    planted = cycle(ident, shape, istat)
+    assert np.linalg.norm(planted - exp) / np.linalg.norm(exp - x0) > 5 * 2e-2
