@@ -18,6 +18,7 @@
 namespace jfem {
 
 typedef __bf16 bfv8 __attribute__((ext_vector_type(8)));   // the same bits as bf16 for the dot-product builtins
+typedef __bf16 bfv2 __attribute__((ext_vector_type(2)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // lattice coordinates of pixel p (32-bit arithmetic: np < 2^31 is checked in
@@ -217,18 +218,30 @@ struct MidTile {
     static constexpr int NR = TY * TZ / NW;       // rows per wavefront
     static constexpr int HX = 34, HY = TY + 2, HZ = D == 3 ? TZ + 2 : 1;
     static constexpr int HP = HX * HY * HZ;       // halo pixels (3D: 1224 = 78 KB of LDS at 32 channels, 2D: 612)
+    // The halo tile is chunk-major: 16-byte chunk c (8 channels) of halo pixel
+    // hp sits at slot c * CS + hp.  The chunk stride CS is HP rounded up to
+    // 16 / CH modulo 16, so that 16 lanes storing the CH chunks of 16 / CH
+    // consecutive pixels, and 16 lanes loading one chunk of 16 consecutive
+    // pixels, each cover the 64 LDS banks once.  A tap is then a constant
+    // slot offset from the lane's base slot.
+    template <int CH>
+    static constexpr int cs()
+    {
+        int c = HP;
+        while (c % 16 != (16 / CH) % 16) c++;
+        return c;
+    }
+    // slot offset of tap t (x fastest, offsets -1..1) from the halo pixel of
+    // offset (-1, -1, -1); the input gradient (mirror) uses the negated offsets
+    template <bool MIRROR>
+    static constexpr int tap(int t)
+    {
+        const int a0 = t % 3, a1 = (t / 3) % 3, a2 = D == 3 ? t / 9 : 0;
+        return MIRROR ? ((D == 3 ? 2 - a2 : 0) * HY + 2 - a1) * HX + 2 - a0 : (a2 * HY + a1) * HX + a0;
+    }
+    // halo pixel of row `row`'s first pixel at offset (-1, -1, -1)
+    static __device__ int row_base(int row) { return ((row / TY) * HY + row % TY) * HX; }
 };
-
-// LDS position of 16-byte chunk c of halo pixel hp: the chunk index is XORed
-// with bits of the pixel index so that 16 lanes reading one chunk of 16
-// consecutive pixels touch 16 different 16-byte bank groups
-template <int CH>
-__device__ inline int halo_slot(int hp, int c)
-{
-    if (CH == 4) return hp * 4 + (c ^ ((hp >> 2) & 3));
-    if (CH == 2) return hp * 2 + (c ^ ((hp >> 3) & 1));
-    return hp * CH + c;
-}
 
 template <int D, int CH>
 __device__ inline void stage_halo(const Lat& L, const unsigned short* __restrict__ src, int x0, int y0, int z0,
@@ -236,7 +249,7 @@ __device__ inline void stage_halo(const Lat& L, const unsigned short* __restrict
 {
     // all loads of a thread are issued before any LDS store, so that they are in flight together
     using T = MidTile<D>;
-    constexpr int NT = T::NW * 64, NF = T::HP * CH, PER = (NF + NT - 1) / NT;
+    constexpr int NT = T::NW * 64, NF = T::HP * CH, PER = (NF + NT - 1) / NT, CS = T::template cs<CH>();
     bf16x8 v[PER];
 #pragma unroll
     for (int k = 0; k < PER; k++) {
@@ -256,88 +269,137 @@ __device__ inline void stage_halo(const Lat& L, const unsigned short* __restrict
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         const int f = threadIdx.x + k * NT;
-        if (f < NF) tile[halo_slot<CH>(f / CH, f % CH)] = v[k];
+        if (f < NF) tile[(f % CH) * CS + f / CH] = v[k];
+    }
+}
+
+// two floats -> packed bf16 pair (round to nearest even, v_cvt_pk_bf16_f32)
+__device__ inline unsigned int pack_bf16x2(float a, float b)
+{
+    bfv2 v;
+    v[0] = (__bf16)a;
+    v[1] = (__bf16)b;
+    return __builtin_bit_cast(unsigned int, v);
+}
+
+// D[c][p] fragment of row u -> NHWC bf16: lane (j, kh) holds channels
+// c = 8 g + 4 kh + e (g, e = 0..3) of pixel p; MODE 0 adds the bias and applies
+// ReLU, MODE 1 keeps the values where the forward activation is positive
+template <int MODE>
+__device__ inline void store_fragment(const f32x16& acc, long p, int kh, const float* __restrict__ bias,
+                                      const unsigned short* __restrict__ mask, unsigned short* __restrict__ out)
+{
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        float v[4];
+        unsigned long long mk = 0;
+        if (MODE == 1) mk = *reinterpret_cast<const unsigned long long*>(mask + p * NN_H + 8 * g + 4 * kh);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            float x = acc[4 * g + e];
+            if (MODE == 0) {
+                x += bias[8 * g + 4 * kh + e];
+                x = x > 0.f ? x : 0.f;
+            } else {
+                // bf16 bits m encode a value > 0 iff 0 < m <= 0x7f80 (sign clear, not zero, not NaN)
+                const unsigned int m = (unsigned int)(mk >> (16 * e)) & 0xffffu;
+                x = m - 1u < 0x7f80u ? x : 0.f;
+            }
+            v[e] = x;
+        }
+        uint2 pk;
+        pk.x = pack_bf16x2(v[0], v[1]);
+        pk.y = pack_bf16x2(v[2], v[3]);
+        *reinterpret_cast<uint2*>(out + p * NN_H + 8 * g + 4 * kh) = pk;
     }
 }
 
 template <int MODE, int D>
-__global__ __launch_bounds__(MidTile<D>::NW * 64) void k_mid_lds(Lat L, const unsigned short* __restrict__ in,
+__global__ __launch_bounds__(MidTile<D>::NW * 64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mid_lds(Lat L, const unsigned short* __restrict__ in,
                                                  const unsigned short* __restrict__ Wh, const float* __restrict__ bias,
                                                  const unsigned short* __restrict__ mask, unsigned short* __restrict__ out)
 {
     using T = MidTile<D>;
-    constexpr int TAPS = D == 3 ? 27 : 9;
-    extern __shared__ bf16x8 tile[];   // [hz][hy][hx][4 x 8 channels]
+    constexpr int TAPS = D == 3 ? 27 : 9, CS = T::template cs<4>(), NR = T::NR;
+    constexpr int WD = 3;                // taps of weight fragments in flight ahead of their use
+    extern __shared__ bf16x8 tile[];     // chunk-major halo tile, 4 chunks of 8 channels
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int j = lane & 31, kh = lane >> 5;
+    // A fragment of tap t and K step kb: W[t][j][16 kb + 8 kh .. + 7] = wl[128 t + 2 kb]
+    const bf16x8* wl = reinterpret_cast<const bf16x8*>(Wh) + j * 4 + kh;
     const int nx = (L.n[0] + 31) / 32, ny = (L.n[1] + T::TY - 1) / T::TY, nz = (L.n[2] + T::TZ - 1) / T::TZ;
     const long ntile = (long)nx * ny * nz;
     for (long tb = blockIdx.x; tb < ntile; tb += gridDim.x) {
         const int bx = (int)(tb % nx), by = (int)((tb / nx) % ny), bz = (int)(tb / ((long)nx * ny));
         const int x0 = bx * 32, y0 = by * T::TY, z0 = bz * T::TZ;
+        // the weight loads are the same for every tile; an opaque zero offset keeps
+        // the compiler from hoisting all of them out of the tile loop (which spills)
+        int wz = 0;
+        asm volatile("" : "+s"(wz));
+        const bf16x8* wt = wl + wz;
+        bf16x8 aw[WD + 1][2];
+#pragma unroll
+        for (int t = 0; t < WD; t++)
+#pragma unroll
+            for (int kb = 0; kb < 2; kb++) aw[t][kb] = wt[128 * t + 2 * kb];
         stage_halo<D, 4>(L, in, x0, y0, z0, tile);
         __syncthreads();
-        f32x16 acc[T::NR];
+        const bf16x8* bl[NR];
 #pragma unroll
-        for (int u = 0; u < T::NR; u++)
+        for (int u = 0; u < NR; u++) bl[u] = tile + kh * CS + T::row_base(wave * NR + u) + j;
+        f32x16 acc[NR];
+#pragma unroll
+        for (int u = 0; u < NR; u++)
 #pragma unroll
             for (int r = 0; r < 16; r++) acc[u][r] = 0.f;
+        bf16x8 bc[2][NR];
+#pragma unroll
+        for (int kb = 0; kb < 2; kb++)
+#pragma unroll
+            for (int u = 0; u < NR; u++) bc[kb][u] = bl[u][T::template tap<MODE == 1>(0) + 2 * kb * CS];
+#pragma unroll
         for (int t = 0; t < TAPS; t++) {
-            const int sg = MODE == 0 ? 1 : -1;
-            const int o0 = sg * (t % 3 - 1), o1 = sg * ((t / 3) % 3 - 1), o2 = D == 3 ? sg * (t / 9 - 1) : 0;
+            // keep each tap's loads in its own step (unbounded hoisting would spill)
+            __builtin_amdgcn_sched_barrier(0);
+            if (t + WD < TAPS)
 #pragma unroll
-            for (int kb = 0; kb < 2; kb++) {
-                const bf16x8 a = *reinterpret_cast<const bf16x8*>(Wh + ((long)t * NN_H + j) * NN_H + kb * 16 + kh * 8);
+                for (int kb = 0; kb < 2; kb++) aw[(t + WD) % (WD + 1)][kb] = wt[128 * (t + WD) + 2 * kb];
+            bf16x8 bn[2][NR];
+            if (t + 1 < TAPS)
 #pragma unroll
-                for (int u = 0; u < T::NR; u++) {
-                    const int row = wave * T::NR + u;
-                    const int yy = row % T::TY, zz = row / T::TY;
-                    const int hx = j + 1 + o0, hy = yy + 1 + o1, hz = D == 3 ? zz + 1 + o2 : 0;
-                    const bf16x8 b = tile[halo_slot<4>((hz * T::HY + hy) * T::HX + hx, kb * 2 + kh)];
-                    acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[u], 0, 0, 0);
-                }
-            }
+                for (int kb = 0; kb < 2; kb++)
+#pragma unroll
+                    for (int u = 0; u < NR; u++) bn[kb][u] = bl[u][T::template tap<MODE == 1>(t + 1) + 2 * kb * CS];
+#pragma unroll
+            for (int kb = 0; kb < 2; kb++)
+#pragma unroll
+                for (int u = 0; u < NR; u++)
+                    acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aw[t % (WD + 1)][kb], bc[kb][u], acc[u], 0, 0, 0);
+            if (t + 1 < TAPS)
+#pragma unroll
+                for (int kb = 0; kb < 2; kb++)
+#pragma unroll
+                    for (int u = 0; u < NR; u++) bc[kb][u] = bn[kb][u];
         }
 #pragma unroll
-        for (int u = 0; u < T::NR; u++) {
-            const int row = wave * T::NR + u;
+        for (int u = 0; u < NR; u++) {
+            const int row = wave * NR + u;
             const int gx = x0 + j, gy = y0 + row % T::TY, gz = z0 + row / T::TY;
             if (gx >= L.n[0] || gy >= L.n[1] || gz >= L.n[2]) continue;
-            const long p = (long)gx + (long)L.n[0] * (gy + (long)L.n[1] * gz);
-            // lane holds D[c][p] for c = 8*(r/4) + 4*kh + (r%4)
-#pragma unroll
-            for (int g = 0; g < 4; g++) {
-                unsigned short v4[4];
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    const int c = 8 * g + 4 * kh + e;
-                    float v = acc[u][4 * g + e];
-                    if (MODE == 0) { v += bias[c]; v = v > 0.f ? v : 0.f; }
-                    else v = bf2f(mask[p * NN_H + c]) > 0.f ? v : 0.f;
-                    v4[e] = f2bf(v);
-                }
-                const unsigned long long pk = (unsigned long long)v4[0] | ((unsigned long long)v4[1] << 16) |
-                                              ((unsigned long long)v4[2] << 32) | ((unsigned long long)v4[3] << 48);
-                *reinterpret_cast<unsigned long long*>(out + p * NN_H + 8 * g + 4 * kh) = pk;
-            }
+            store_fragment<MODE>(acc[u], (long)gx + (long)L.n[0] * (gy + (long)L.n[1] * gz), kh, bias, mask, out);
         }
         __syncthreads();
     }
 }
 
-// Stage the halo tile of block (x0, y0, z0) into LDS: CH 16-byte chunks per
-// pixel of a (32 + 2) x (TY + 2) x (TZ + 2) box, zeros outside the lattice.
 // split-BF16 copy of the input: inb[p][0..7] = hi(in[p][c]), inb[p][8..15] =
-// lo(in[p][c]) = bf16(in - hi) for c < cin, zeros past cin; hi + lo carries
-// about 16 mantissa bits
+// lo(in[p][c]) = bf16(in - hi) for c < cin, zeros past cin
 __global__ void k_in_split(long np, int cin, const float* __restrict__ in, unsigned short* __restrict__ inb)
 {
     for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
         bf16x8 hi, lo;
 #pragma unroll
-        for (int c = 0; c < 8; c++) {
-            split_bf16(c < cin ? in[p * cin + c] : 0.f, hi, lo, c);
-        }
+        for (int c = 0; c < 8; c++) split_bf16(c < cin ? in[p * cin + c] : 0.f, hi, lo, c);
         *reinterpret_cast<bf16x8*>(inb + p * 16) = hi;
         *reinterpret_cast<bf16x8*>(inb + p * 16 + 8) = lo;
     }
@@ -345,8 +407,9 @@ __global__ void k_in_split(long np, int cin, const float* __restrict__ in, unsig
 
 // first layer on MFMA (2D/3D, cin <= 8) in split-BF16: one 16-wide K step per
 // tap, lanes 0-31 feed the input's high parts and lanes 32-63 its low parts,
-// so sum_c (hi + lo)[c] w_hi[c] + hi[c] w_lo[c] (two MFMAs per tap) keeps the
-// layer at close to FP32 accuracy; the input tile is staged in LDS
+// so sum_c (hi + lo)[c] (w_hi + w_lo)[c] (two MFMAs per tap, A = w_hi then
+// w_lo in both lane halves) keeps the layer at close to FP32 accuracy; the
+// input tile is staged in LDS
 template <int D>
 __global__ __launch_bounds__(MidTile<D>::NW * 64) void k_first_lds(Lat L, int cin, const unsigned short* __restrict__ inb,
                                                                   const float* __restrict__ W0,
@@ -354,10 +417,10 @@ __global__ __launch_bounds__(MidTile<D>::NW * 64) void k_first_lds(Lat L, int ci
                                                                   unsigned short* __restrict__ out)
 {
     using T = MidTile<D>;
-    constexpr int TAPS = D == 3 ? 27 : 9;
+    constexpr int TAPS = D == 3 ? 27 : 9, CS = T::template cs<2>(), NR = T::NR;
     extern __shared__ bf16x8 sm[];
     bf16x8* wf = sm;                     // TAPS x (32 w_hi, 32 w_lo) weight fragments
-    bf16x8* tile = sm + TAPS * 64;       // HP pixels x (hi, lo)
+    bf16x8* tile = sm + TAPS * 64;       // chunk-major halo tile: chunk 0 high parts, chunk 1 low parts
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int j = lane & 31, kh = lane >> 5;
     for (int f = threadIdx.x; f < TAPS * 64; f += T::NW * 64) {
@@ -371,9 +434,6 @@ __global__ __launch_bounds__(MidTile<D>::NW * 64) void k_first_lds(Lat L, int ci
         }
         wf[f] = v;
     }
-    bf16x8 zero8;
-#pragma unroll
-    for (int e = 0; e < 8; e++) zero8[e] = 0;
     const int nx = (L.n[0] + 31) / 32, ny = (L.n[1] + T::TY - 1) / T::TY, nz = (L.n[2] + T::TZ - 1) / T::TZ;
     const long ntile = (long)nx * ny * nz;
     for (long tb = blockIdx.x; tb < ntile; tb += gridDim.x) {
@@ -381,91 +441,97 @@ __global__ __launch_bounds__(MidTile<D>::NW * 64) void k_first_lds(Lat L, int ci
         const int x0 = bx * 32, y0 = by * T::TY, z0 = bz * T::TZ;
         stage_halo<D, 2>(L, inb, x0, y0, z0, tile);
         __syncthreads();
-        f32x16 acc[T::NR];
+        const bf16x8* bl[NR];
 #pragma unroll
-        for (int u = 0; u < T::NR; u++)
+        for (int u = 0; u < NR; u++) bl[u] = tile + kh * CS + T::row_base(wave * NR + u) + j;
+        f32x16 acc[NR];
+#pragma unroll
+        for (int u = 0; u < NR; u++)
 #pragma unroll
             for (int r = 0; r < 16; r++) acc[u][r] = 0.f;
-        for (int t = 0; t < TAPS; t++) {
-            const int o0 = t % 3 - 1, o1 = (t / 3) % 3 - 1, o2 = D == 3 ? t / 9 - 1 : 0;
-            // lanes 0-31 multiply the input's high parts by w_hi + w_lo, lanes 32-63 its low parts by w_hi
-            const bf16x8 ahi = wf[t * 64 + j], alo = kh == 0 ? wf[t * 64 + 32 + j] : zero8;
+        bf16x8 ahi = wf[j], alo = wf[32 + j], bc[NR];
 #pragma unroll
-            for (int u = 0; u < T::NR; u++) {
-                const int row = wave * T::NR + u;
-                const int yy = row % T::TY, zz = row / T::TY;
-                const int hx = j + 1 + o0, hy = yy + 1 + o1, hz = D == 3 ? zz + 1 + o2 : 0;
-                const bf16x8 b = tile[halo_slot<2>((hz * T::HY + hy) * T::HX + hx, kh)];
-                acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, b, acc[u], 0, 0, 0);
-                acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, b, acc[u], 0, 0, 0);
+        for (int u = 0; u < NR; u++) bc[u] = bl[u][T::template tap<false>(0)];
+#pragma unroll
+        for (int t = 0; t < TAPS; t++) {
+            bf16x8 ahn, aln, bn[NR];
+            if (t + 1 < TAPS) {
+                ahn = wf[(t + 1) * 64 + j];
+                aln = wf[(t + 1) * 64 + 32 + j];
+#pragma unroll
+                for (int u = 0; u < NR; u++) bn[u] = bl[u][T::template tap<false>(t + 1)];
+            }
+#pragma unroll
+            for (int u = 0; u < NR; u++) {
+                acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bc[u], acc[u], 0, 0, 0);
+                acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bc[u], acc[u], 0, 0, 0);
+            }
+            if (t + 1 < TAPS) {
+                ahi = ahn;
+                alo = aln;
+#pragma unroll
+                for (int u = 0; u < NR; u++) bc[u] = bn[u];
             }
         }
 #pragma unroll
-        for (int u = 0; u < T::NR; u++) {
-            const int row = wave * T::NR + u;
+        for (int u = 0; u < NR; u++) {
+            const int row = wave * NR + u;
             const int gx = x0 + j, gy = y0 + row % T::TY, gz = z0 + row / T::TY;
             if (gx >= L.n[0] || gy >= L.n[1] || gz >= L.n[2]) continue;
-            const long p = (long)gx + (long)L.n[0] * (gy + (long)L.n[1] * gz);
-#pragma unroll
-            for (int g = 0; g < 4; g++) {
-                unsigned short v4[4];
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    const int c = 8 * g + 4 * kh + e;
-                    float v = acc[u][4 * g + e] + b0[c];
-                    v4[e] = f2bf(v > 0.f ? v : 0.f);
-                }
-                const unsigned long long pk = (unsigned long long)v4[0] | ((unsigned long long)v4[1] << 16) |
-                                              ((unsigned long long)v4[2] << 32) | ((unsigned long long)v4[3] << 48);
-                *reinterpret_cast<unsigned long long*>(out + p * NN_H + 8 * g + 4 * kh) = pk;
-            }
+            store_fragment<0>(acc[u], (long)gx + (long)L.n[0] * (gy + (long)L.n[1] * gz), kh, b0, nullptr, out);
         }
         __syncthreads();
     }
 }
 
-// last layer with the activation tile staged in LDS (2D/3D): each thread owns
-// pixels tid, tid + NW*64, ... of the tile
+// last layer with the activation tile staged in LDS (2D/3D): thread q owns
+// the tile pixels q and q + NW*64 (two rows apart by NW*64 / 32), so every
+// weight pair fetched feeds two dot products; BF16 pairs, FP32 accumulation
 template <int D>
 __global__ __launch_bounds__(MidTile<D>::NW * 64) void k_last_lds(Lat L, const unsigned short* __restrict__ act,
                                                                  const unsigned short* __restrict__ W3h,
                                                                  const float* __restrict__ b3, float* __restrict__ out)
 {
     using T = MidTile<D>;
-    constexpr int TAPS = D == 3 ? 27 : 9;
-    extern __shared__ bf16x8 tile[];     // HP pixels x 32 channels
+    constexpr int TAPS = D == 3 ? 27 : 9, CS = T::template cs<4>(), NT = T::NW * 64;
+    static_assert(32 * T::TY * T::TZ == 2 * NT, "two pixels per thread");
+    extern __shared__ bf16x8 tile[];     // chunk-major halo tile, 4 chunks of 8 channels
+    bf16x8* wl = tile + 4 * CS;          // the weights, TAPS x 4 chunks
+    for (int f = threadIdx.x; f < TAPS * 4; f += NT) wl[f] = reinterpret_cast<const bf16x8*>(W3h)[f];
     const int nx = (L.n[0] + 31) / 32, ny = (L.n[1] + T::TY - 1) / T::TY, nz = (L.n[2] + T::TZ - 1) / T::TZ;
     const long ntile = (long)nx * ny * nz;
+    const int xx = threadIdx.x % 32, r0 = threadIdx.x / 32, r1 = r0 + NT / 32;
     for (long tb = blockIdx.x; tb < ntile; tb += gridDim.x) {
         const int bx = (int)(tb % nx), by = (int)((tb / nx) % ny), bz = (int)(tb / ((long)nx * ny));
         const int x0 = bx * 32, y0 = by * T::TY, z0 = bz * T::TZ;
         stage_halo<D, 4>(L, act, x0, y0, z0, tile);
         __syncthreads();
-        for (int q = threadIdx.x; q < 32 * T::TY * T::TZ; q += T::NW * 64) {
-            const int xx = q % 32, yy = (q / 32) % T::TY, zz = q / (32 * T::TY);
-            const int gx = x0 + xx, gy = y0 + yy, gz = z0 + zz;
-            if (gx >= L.n[0] || gy >= L.n[1] || gz >= L.n[2]) continue;
-            float acc = b3[0];
-            for (int t = 0; t < TAPS; t++) {
-                const int o0 = t % 3 - 1, o1 = (t / 3) % 3 - 1, o2 = D == 3 ? t / 9 - 1 : 0;
-                const int hx = xx + 1 + o0, hy = yy + 1 + o1, hz = D == 3 ? zz + 1 + o2 : 0;
-                const int hp = (hz * T::HY + hy) * T::HX + hx;
+        const bf16x8* p0 = tile + T::row_base(r0) + xx;
+        const bf16x8* p1 = tile + T::row_base(r1) + xx;
+        float acc0 = b3[0], acc1 = acc0;
+        // the tap loop stays rolled (fully unrolled, the compiler issues all tap loads at once and spills);
+        // a tap is a uniform slot offset
+#pragma unroll 1
+        for (int t = 0; t < TAPS; t++) {
+            const int to = T::template tap<false>(t);
 #pragma unroll
-                for (int g = 0; g < 4; g++) {
-                    const bf16x8 hs = tile[halo_slot<4>(hp, g)];
-                    const bfv8 a = __builtin_bit_cast(bfv8, hs);
-                    const bfv8 w = *reinterpret_cast<const bfv8*>(W3h + t * NN_H + 8 * g);
-                    acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_shufflevector(a, a, 0, 1),
-                                                          __builtin_shufflevector(w, w, 0, 1), acc, false);
-                    acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_shufflevector(a, a, 2, 3),
-                                                          __builtin_shufflevector(w, w, 2, 3), acc, false);
-                    acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_shufflevector(a, a, 4, 5),
-                                                          __builtin_shufflevector(w, w, 4, 5), acc, false);
-                    acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_shufflevector(a, a, 6, 7),
-                                                          __builtin_shufflevector(w, w, 6, 7), acc, false);
+            for (int g = 0; g < 4; g++) {
+                const bfv8 a0 = __builtin_bit_cast(bfv8, p0[to + g * CS]);
+                const bfv8 a1 = __builtin_bit_cast(bfv8, p1[to + g * CS]);
+                const bfv8 w = __builtin_bit_cast(bfv8, wl[4 * t + g]);
+#pragma unroll
+                for (int e = 0; e < 8; e += 2) {
+                    const bfv2 we = {w[e], w[e + 1]};
+                    acc0 = __builtin_amdgcn_fdot2_f32_bf16(bfv2{a0[e], a0[e + 1]}, we, acc0, false);
+                    acc1 = __builtin_amdgcn_fdot2_f32_bf16(bfv2{a1[e], a1[e + 1]}, we, acc1, false);
                 }
             }
-            out[(long)gx + (long)L.n[0] * (gy + (long)L.n[1] * gz)] = acc;
+        }
+        const int gx = x0 + xx;
+        if (gx < L.n[0]) {
+            const int gy0 = y0 + r0 % T::TY, gz0 = z0 + r0 / T::TY, gy1 = y0 + r1 % T::TY, gz1 = z0 + r1 / T::TY;
+            if (gy0 < L.n[1] && gz0 < L.n[2]) out[(long)gx + (long)L.n[0] * (gy0 + (long)L.n[1] * gz0)] = acc0;
+            if (gy1 < L.n[1] && gz1 < L.n[2]) out[(long)gx + (long)L.n[0] * (gy1 + (long)L.n[1] * gz1)] = acc1;
         }
         __syncthreads();
     }
@@ -499,16 +565,16 @@ static void launch_mid(hipStream_t st, const Lat& L, int taps, const unsigned sh
                        const float* bias, const unsigned short* mask, unsigned short* out)
 {
 #if !JFEM_NN_NOLDS
-    if ((L.d == 3 || L.d == 2) && lds_per_block() >= (long)sizeof(bf16x8) * 4 * MidTile<3>::HP) {
+    if ((L.d == 3 || L.d == 2) && lds_per_block() >= (long)sizeof(bf16x8) * 4 * MidTile<3>::cs<4>()) {
         const int d = L.d;
         const int ty = d == 3 ? MidTile<3>::TY : MidTile<2>::TY, tz = d == 3 ? MidTile<3>::TZ : MidTile<2>::TZ;
         const long ntile = (long)((L.n[0] + 31) / 32) * ((L.n[1] + ty - 1) / ty) * ((L.n[2] + tz - 1) / tz);
         const int grid = (int)std::min<long>(ntile, 4096);
         if (d == 3)
-            k_mid_lds<MODE, 3><<<grid, MidTile<3>::NW * 64, sizeof(bf16x8) * 4 * MidTile<3>::HP, st>>>(L, in, W, bias,
+            k_mid_lds<MODE, 3><<<grid, MidTile<3>::NW * 64, sizeof(bf16x8) * 4 * MidTile<3>::cs<4>(), st>>>(L, in, W, bias,
                                                                                                       mask, out);
         else
-            k_mid_lds<MODE, 2><<<grid, MidTile<2>::NW * 64, sizeof(bf16x8) * 4 * MidTile<2>::HP, st>>>(L, in, W, bias,
+            k_mid_lds<MODE, 2><<<grid, MidTile<2>::NW * 64, sizeof(bf16x8) * 4 * MidTile<2>::cs<4>(), st>>>(L, in, W, bias,
                                                                                                       mask, out);
         JFEM_HIP(hipGetLastError());
         return;
@@ -844,7 +910,7 @@ static int nn_grid(long np) { return grid_for(np, 256, 8192); }
 // forward on nn->in (np x cin FP32, lattice order) into nn->out (np FP32)
 bool nn_split_input(const jfem_nn* nn, const Lat& L)
 {
-    return (L.d == 2 || L.d == 3) && nn->cin <= 8 && lds_per_block() >= (long)sizeof(bf16x8) * 4 * MidTile<3>::HP;
+    return (L.d == 2 || L.d == 3) && nn->cin <= 8 && lds_per_block() >= (long)sizeof(bf16x8) * 4 * MidTile<3>::cs<4>();
 }
 
 void nn_forward(jfem_nn* nn, const Lat& L, bool inb_ready)
@@ -863,21 +929,21 @@ void nn_forward(jfem_nn* nn, const Lat& L, bool inb_ready)
             using T = MidTile<3>;
             const long nt = (long)((L.n[0] + 31) / 32) * ((L.n[1] + T::TY - 1) / T::TY) * ((L.n[2] + T::TZ - 1) / T::TZ);
             const int grid = (int)std::min<long>(nt, 4096);
-            k_first_lds<3><<<grid, T::NW * 64, sizeof(bf16x8) * (27 * 64 + 2 * T::HP), st>>>(L, nn->cin, nn->inb, W0,
+            k_first_lds<3><<<grid, T::NW * 64, sizeof(bf16x8) * (27 * 64 + 2 * T::cs<2>()), st>>>(L, nn->cin, nn->inb, W0,
                                                                                                  b0, nn->act0);
             launch_mid<0>(st, L, nn->taps, nn->act0, nn->wh1, nn->w + o[3], nullptr, nn->act1);
             launch_mid<0>(st, L, nn->taps, nn->act1, nn->wh2, nn->w + o[5], nullptr, nn->act2);
-            k_last_lds<3><<<grid, T::NW * 64, sizeof(bf16x8) * 4 * T::HP, st>>>(L, nn->act2, nn->wh3, nn->w + o[7],
+            k_last_lds<3><<<grid, T::NW * 64, sizeof(bf16x8) * (4 * T::cs<4>() + 4 * 27), st>>>(L, nn->act2, nn->wh3, nn->w + o[7],
                                                                                nn->out);
         } else {
             using T = MidTile<2>;
             const long nt = (long)((L.n[0] + 31) / 32) * ((L.n[1] + T::TY - 1) / T::TY);
             const int grid = (int)std::min<long>(nt, 4096);
-            k_first_lds<2><<<grid, T::NW * 64, sizeof(bf16x8) * (9 * 64 + 2 * T::HP), st>>>(L, nn->cin, nn->inb, W0,
+            k_first_lds<2><<<grid, T::NW * 64, sizeof(bf16x8) * (9 * 64 + 2 * T::cs<2>()), st>>>(L, nn->cin, nn->inb, W0,
                                                                                                 b0, nn->act0);
             launch_mid<0>(st, L, nn->taps, nn->act0, nn->wh1, nn->w + o[3], nullptr, nn->act1);
             launch_mid<0>(st, L, nn->taps, nn->act1, nn->wh2, nn->w + o[5], nullptr, nn->act2);
-            k_last_lds<2><<<grid, T::NW * 64, sizeof(bf16x8) * 4 * T::HP, st>>>(L, nn->act2, nn->wh3, nn->w + o[7],
+            k_last_lds<2><<<grid, T::NW * 64, sizeof(bf16x8) * (4 * T::cs<4>() + 4 * 9), st>>>(L, nn->act2, nn->wh3, nn->w + o[7],
                                                                                nn->out);
         }
         JFEM_HIP(hipGetLastError());
