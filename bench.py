@@ -63,9 +63,11 @@ def rhs_for(jf, space, seed, nterms):
     return jf.DenseVector(r), r
 
 
-def cpu_baseline(seconds_budget=30.0):
+def cpu_baseline(seconds_budget=10.0):
     """Oracle (single-threaded C restatement) MG-PCG on a bounded sample: 3D Q2
-    16^3 cells (35,937 DOFs), same smoother and cycle; time scaled by the DOF ratio."""
+    16^3 cells (35,937 DOFs), same smoother and cycle; the solve is repeated for
+    about seconds_budget of CPU time and the median is scaled by the DOF ratio.
+    The sample's matrices are far smaller than C2's, so caches flatter the CPU."""
     from oracle import oracle as O
     from jfem_amd.workloads import sinusoid_terms
     base, nlev, k = [2, 2, 2], 4, 2
@@ -81,9 +83,12 @@ def cpu_baseline(seconds_budget=30.0):
     rhs = O.rhs_sines(cells, k, [0, 0, 0], [1, 1, 1], kw, amp)
     rhs[O.boundary_nodes(cells, k)] = 0
     setup = time.perf_counter() - t0
-    t1 = time.perf_counter()
-    x, it, res, _ = O.solve("cg", A, rhs, np.zeros(A.n), 1e-8 * np.linalg.norm(rhs), precond=G)
-    solve = time.perf_counter() - t1
+    solves = []
+    while not solves or (sum(solves) < seconds_budget and len(solves) < 1000):
+        t1 = time.perf_counter()
+        x, it, res, _ = O.solve("cg", A, rhs, np.zeros(A.n), 1e-8 * np.linalg.norm(rhs), precond=G)
+        solves.append(time.perf_counter() - t1)
+    solve = float(np.median(solves))
     # SpMV throughput of the oracle on the sample matrix
     xx = np.random.default_rng(0).standard_normal(A.n)
     reps = 20
@@ -92,7 +97,8 @@ def cpu_baseline(seconds_budget=30.0):
         A.matvec(xx)
     t_spmv = (time.perf_counter() - t2) / reps
     bytes_spmv = 12 * A.nnz + 4 * (A.n + 1) + 16 * A.n
-    return dict(solve_s=solve, iterations=it, n=A.n, setup_s=setup, spmv_gbs=bytes_spmv / t_spmv / 1e9)
+    return dict(solve_s=solve, repeats=len(solves), iterations=it, n=A.n, setup_s=setup,
+                spmv_gbs=bytes_spmv / t_spmv / 1e9)
 
 
 def measured_traffic(cells):
@@ -484,7 +490,8 @@ def main():
         scale = n / cb["n"]
         cpu = {"value": cb["solve_s"] * scale, "unit": "s", "cores": 1, "kind": "port",
                "sample": f"oracle MG-PCG (Chebyshev V(2,2) deg 4) on 3D Q2 16^3 cells ({cb['n']} DOFs), "
-                         f"{cb['iterations']} its in {cb['solve_s']:.3f}s, scaled by DOF ratio {scale:.1f}; "
+                         f"{cb['iterations']} its in {cb['solve_s']:.3f}s (median of {cb['repeats']} solves), "
+                         f"scaled by DOF ratio {scale:.1f}; "
                          f"oracle SpMV {cb['spmv_gbs']:.2f} GB/s"}
 
     fp64_s = float(np.median(times))

@@ -219,17 +219,13 @@ struct MidTile {
     static constexpr int HX = 34, HY = TY + 2, HZ = D == 3 ? TZ + 2 : 1;
     static constexpr int HP = HX * HY * HZ;       // halo pixels (3D: 1224 = 78 KB of LDS at 32 channels, 2D: 612)
     // The halo tile is chunk-major: 16-byte chunk c (8 channels) of halo pixel
-    // hp sits at slot c * CS + hp.  The chunk stride CS is HP rounded up to
-    // 16 / CH modulo 16, so that 16 lanes storing the CH chunks of 16 / CH
-    // consecutive pixels, and 16 lanes loading one chunk of 16 consecutive
-    // pixels, each cover the 64 LDS banks once.  A tap is then a constant
+    // hp sits at slot c * CS + hp, CS = HP.  16 lanes reading one chunk of 16
+    // consecutive pixels touch the 64 LDS banks once, and a tap is a constant
     // slot offset from the lane's base slot.
     template <int CH>
     static constexpr int cs()
     {
-        int c = HP;
-        while (c % 16 != (16 / CH) % 16) c++;
-        return c;
+        return HP;
     }
     // slot offset of tap t (x fastest, offsets -1..1) from the halo pixel of
     // offset (-1, -1, -1); the input gradient (mirror) uses the negated offsets
@@ -247,17 +243,21 @@ template <int D, int CH>
 __device__ inline void stage_halo(const Lat& L, const unsigned short* __restrict__ src, int x0, int y0, int z0,
                                   bf16x8* tile)
 {
-    // all loads of a thread are issued before any LDS store, so that they are in flight together
+    // all loads of a thread are issued before any LDS store, so that they are in flight together.
+    // One wave-wide step covers PPW = 64 / CH consecutive halo pixels (contiguous in global
+    // memory); lane i takes chunk i / PPW of pixel i % PPW, so each 16-lane group stores
+    // 16 consecutive slots of one chunk (no LDS bank conflicts)
     using T = MidTile<D>;
     constexpr int NT = T::NW * 64, NF = T::HP * CH, PER = (NF + NT - 1) / NT, CS = T::template cs<CH>();
+    constexpr int PPW = 64 / CH;
     bf16x8 v[PER];
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         const int f = threadIdx.x + k * NT;
-        const int hp = f / CH, ch = f % CH;
+        const int hp = (f / 64) * PPW + f % PPW, ch = (f % 64) / PPW;
         const int hx = hp % T::HX, hy = (hp / T::HX) % T::HY, hz = hp / (T::HX * T::HY);
         const int gx = x0 - 1 + hx, gy = D >= 2 ? y0 - 1 + hy : 0, gz = D == 3 ? z0 - 1 + hz : 0;
-        if (f < NF && (unsigned)gx < (unsigned)L.n[0] && (unsigned)gy < (unsigned)L.n[1] &&
+        if (hp < T::HP && (unsigned)gx < (unsigned)L.n[0] && (unsigned)gy < (unsigned)L.n[1] &&
             (unsigned)gz < (unsigned)L.n[2])
             v[k] = *reinterpret_cast<const bf16x8*>(src + ((long)gx + (long)L.n[0] * (gy + (long)L.n[1] * gz)) *
                                                               (CH * 8) + ch * 8);
@@ -269,7 +269,8 @@ __device__ inline void stage_halo(const Lat& L, const unsigned short* __restrict
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         const int f = threadIdx.x + k * NT;
-        if (f < NF) tile[(f % CH) * CS + f / CH] = v[k];
+        const int hp = (f / 64) * PPW + f % PPW, ch = (f % 64) / PPW;
+        if (hp < T::HP) tile[ch * CS + hp] = v[k];
     }
 }
 
