@@ -152,56 +152,83 @@ struct ElemParams {
 // E[(i*ntuples + t)*nloc*nloc + iv*nloc + iu] = integral i on class tuple t
 // between u = shapeFunction1 (local iu) and v = shapeFunction2 (local iv).
 // Operation order follows integrateNonTensorProduct exactly (no contraction).
-__global__ void k_element_tables(ElemParams P, const Tab1D* __restrict__ tabs0, const Tab1D* __restrict__ tabs1,
-                                 const Tab1D* __restrict__ tabs2, double* __restrict__ E)
+// One thread per (integral, class tuple, node pair iv <= iu): the value is
+// written to E[iv][iu] and E[iu][iv].  Swapping the two functions leaves every
+// operation of the sum unchanged (products commute exactly in IEEE arithmetic
+// and the quadrature order is the same), so the table is symmetric bit for
+// bit.  DD, KK, QQ > 0 fix dimension, degree and points per axis, so the
+// quadrature loop unrolls and the 1D tables stay in registers.
+template <int DD, int KK, int QQ>
+__global__ __launch_bounds__(256) void k_element_tables(ElemParams P, const Tab1D* __restrict__ tabs0,
+                                                        const Tab1D* __restrict__ tabs1,
+                                                        const Tab1D* __restrict__ tabs2,
+                                                        const int* __restrict__ pairs, double* __restrict__ E)
 {
 #pragma clang fp contract(off)
+    const int d = DD ? DD : P.d, k = KK ? KK : P.k, Q = QQ ? QQ : P.Q;
     const long nent = (long)P.nloc * P.nloc;
-    const long total = nent * P.ntuples * P.nint;
+    const long npair = (long)P.nloc * (P.nloc + 1) / 2;
+    const long total = npair * P.ntuples * P.nint;
     for (long id = blockIdx.x * (long)blockDim.x + threadIdx.x; id < total; id += (long)gridDim.x * blockDim.x) {
-        const int iu = (int)(id % P.nloc);
-        const int iv = (int)((id / P.nloc) % P.nloc);
-        const long t = (id / nent) % P.ntuples;
-        const int ii = (int)(id / (nent * P.ntuples));
+        const int pr = pairs[id % npair];
+        const int iv = pr >> 16, iu = pr & 0xffff;
+        const long t = (id / npair) % P.ntuples;
+        const int ii = (int)(id / (npair * P.ntuples));
         int cls[MAXD] = {0, 0, 0};
         long r = t;
-        for (int a = P.d - 1; a >= 0; a--) { cls[a] = (int)(r % P.ncls[a]); r /= P.ncls[a]; }
-        const Tab1D* tb[MAXD] = {tabs0 + cls[0], P.d > 1 ? tabs1 + cls[1] : nullptr, P.d > 2 ? tabs2 + cls[2] : nullptr};
-        int lu[MAXD], lv[MAXD];
+        for (int a = d - 1; a >= 0; a--) { cls[a] = (int)(r % P.ncls[a]); r /= P.ncls[a]; }
+        const Tab1D* tb[MAXD] = {tabs0 + cls[0], d > 1 ? tabs1 + cls[1] : tabs0, d > 2 ? tabs2 + cls[2] : tabs0};
+        int lu[MAXD] = {0, 0, 0}, lv[MAXD] = {0, 0, 0};
         int ru = iu, rv = iv;
-        for (int a = 0; a < P.d; a++) { lu[a] = ru % (P.k + 1); ru /= P.k + 1; lv[a] = rv % (P.k + 1); rv /= P.k + 1; }
+#pragma unroll
+        for (int a = 0; a < MAXD; a++)
+            if (a < d) { lu[a] = ru % (k + 1); ru /= k + 1; lv[a] = rv % (k + 1); rv /= k + 1; }
         const int kind = P.kinds[ii];
         const double wc = P.wconst[ii];
         int nq = 1;
-        for (int a = 0; a < P.d; a++) nq *= P.Q;
+        for (int a = 0; a < d; a++) nq *= Q;
         double ret = 0;
-        for (int q = 0; q < nq; q++) {
-            int c[MAXD];
+#pragma unroll
+        for (int q = 0; q < (QQ && DD ? (DD == 3 ? QQ * QQ * QQ : DD == 2 ? QQ * QQ : QQ) : 1 << 30); q++) {
+            if (!(QQ && DD) && q >= nq) break;
+            int c[MAXD] = {0, 0, 0};
             int qq = q;
-            for (int a = P.d - 1; a >= 0; a--) { c[a] = qq % P.Q; qq /= P.Q; }   // last axis fastest
+#pragma unroll
+            for (int a = MAXD - 1; a >= 0; a--)
+                if (a < d) { c[a] = qq % Q; qq /= Q; }   // last axis fastest
             double val;
             double vu = 1, vv = 1;
             if (kind != 0) {
-                for (int a = 0; a < P.d; a++) { vu *= tb[a]->val[lu[a]][c[a]]; vv *= tb[a]->val[lv[a]][c[a]]; }
+#pragma unroll
+                for (int a = 0; a < MAXD; a++)
+                    if (a < d) { vu *= tb[a]->val[lu[a]][c[a]]; vv *= tb[a]->val[lv[a]][c[a]]; }
             }
             if (kind == 1) {
                 val = vu * vv * wc;
             } else {
                 double inner = 0;
-                for (int i = 0; i < P.d; i++) {
+#pragma unroll
+                for (int i = 0; i < MAXD; i++) {
+                    if (i >= d) break;
                     double g1 = 1, g2 = 1;
-                    for (int j = 0; j < P.d; j++) {
-                        if (i == j) { g1 *= tb[j]->der[lu[j]][c[j]]; g2 *= tb[j]->der[lv[j]][c[j]]; }
-                        else { g1 *= tb[j]->val[lu[j]][c[j]]; g2 *= tb[j]->val[lv[j]][c[j]]; }
-                    }
+#pragma unroll
+                    for (int j = 0; j < MAXD; j++)
+                        if (j < d) {
+                            if (i == j) { g1 *= tb[j]->der[lu[j]][c[j]]; g2 *= tb[j]->der[lv[j]][c[j]]; }
+                            else { g1 *= tb[j]->val[lu[j]][c[j]]; g2 *= tb[j]->val[lv[j]][c[j]]; }
+                        }
                     inner += g1 * g2;
                 }
                 val = kind == 0 ? inner * wc : (vu * vv + inner) * wc;
             }
-            for (int a = 0; a < P.d; a++) val *= tb[a]->w[c[a]];
+#pragma unroll
+            for (int a = 0; a < MAXD; a++)
+                if (a < d) val *= tb[a]->w[c[a]];
             ret += val;
         }
-        E[id] = ret;
+        const long base = ((long)ii * P.ntuples + t) * nent;
+        E[base + (long)iv * P.nloc + iu] = ret;
+        E[base + (long)iu * P.nloc + iv] = ret;
     }
 }
 
@@ -232,33 +259,64 @@ struct DK {
     __device__ static int k(const GridDev& g) { return KK ? KK : g.k; }
 };
 
+// DOF index -> lattice point (the inverse of dof_of_lat_t).  All quantities
+// are below the DOF count (< 2^31), so 32-bit unsigned arithmetic is exact.
 template <int DD, int KK>
 __device__ inline void lat_of_dof_t(const GridDev& g, long dof, int* p)
 {
     const int d = DK<DD, KK>::d(g), k = DK<DD, KK>::k(g);
-    long tail[MAXD];
-    long t = 1;
+    unsigned tail[MAXD] = {1, 1, 1};
+    unsigned t = 1;
 #pragma unroll
     for (int a = MAXD - 1; a >= 0; a--)
-        if (a < d) { tail[a] = t; t *= (long)k * g.n[a] + 1; }
-    long rem = dof, mprod = 1;
-    int c[MAXD];
+        if (a < d) { tail[a] = t; t *= (unsigned)(k * g.n[a] + 1); }
+    unsigned rem = (unsigned)dof, mprod = 1;
+    int c[MAXD] = {0, 0, 0};
 #pragma unroll
     for (int a = 0; a < MAXD; a++)
         if (a < d) {
-            const long unit = mprod * tail[a];
-            const long q = rem / unit;
-            c[a] = q < k + 1 ? 0 : (int)((q - 1) / k);
-            rem -= (c[a] > 0 ? (long)c[a] * k + 1 : 0) * unit;
+            const unsigned unit = mprod * tail[a];
+            const unsigned q = rem / unit;
+            c[a] = q < (unsigned)(k + 1) ? 0 : (int)((q - 1) / (unsigned)k);
+            rem -= (c[a] > 0 ? (unsigned)(c[a] * k + 1) : 0u) * unit;
             mprod *= c[a] > 0 ? k : k + 1;
         }
 #pragma unroll
     for (int a = 0; a < MAXD; a++)
         if (a < d) {
-            const int m = c[a] > 0 ? k : k + 1;
+            const unsigned m = c[a] > 0 ? k : k + 1;
             p[a] = c[a] * k + (int)(rem % m) + (c[a] > 0 ? 1 : 0);
             rem /= m;
         }
+}
+
+// lattice points per axis product of the later axes: tail[a] = prod_{b > a} (k n_b + 1)
+template <int DD, int KK>
+__device__ inline void lattice_tails(const GridDev& g, long* tail)
+{
+    const int d = DK<DD, KK>::d(g), k = DK<DD, KK>::k(g);
+    long t = 1;
+#pragma unroll
+    for (int a = MAXD - 1; a >= 0; a--)
+        if (a < d) { tail[a] = t; t *= (long)k * g.n[a] + 1; }
+}
+
+template <int DD, int KK>
+__device__ inline long dof_of_lat_tails(const GridDev& g, const long* tail, const int* p)
+{
+    // every partial sum is at most the DOF index (< 2^31): 32-bit arithmetic
+    const int d = DK<DD, KK>::d(g), k = DK<DD, KK>::k(g);
+    unsigned pre = 0, mprod = 1, rank = 0;
+#pragma unroll
+    for (int a = 0; a < MAXD; a++)
+        if (a < d) {
+            const int c = p[a] == 0 ? 0 : (p[a] - 1) / k;
+            const int l = p[a] - c * k;
+            pre += mprod * (c > 0 ? (unsigned)(c * k + 1) : 0u) * (unsigned)tail[a];
+            rank += (unsigned)(l - (c > 0 ? 1 : 0)) * mprod;
+            mprod *= c > 0 ? k : k + 1;
+        }
+    return (long)(pre + rank);
 }
 
 template <int DD, int KK>
@@ -307,6 +365,15 @@ __device__ inline void axis_segments(int p, int k, int n, AxisSegs& s)
     s.W = k * (hi - lo + 1) + 1;
 }
 
+// x / c for a segment length c in {1, k, k + 1} (k <= 4) and 0 <= x < 8192:
+// a multiply by ceil(2^16 / c) and a shift is exact there (checked exhaustively
+// for c <= 8), and with a constant k the three multipliers are constants
+__device__ inline int div_seg(int x, int c, int k)
+{
+    const unsigned m = c == 1 ? 0x10001u : (c == k ? 0x10000u / (unsigned)k + 1u : 0x10000u / (unsigned)(k + 1) + 1u);
+    return (int)(((unsigned)x * m) >> 16);
+}
+
 // decode candidate index t of a row into its column lattice point q
 template <int DD, int KK>
 __device__ inline void decode_candidate(const GridDev& g, const AxisSegs* S, int t, int* q)
@@ -339,90 +406,128 @@ __device__ inline void decode_candidate(const GridDev& g, const AxisSegs* S, int
             fs[a] = f;
             M *= c;
         }
-    // rem indexes inside the owner box, axis 0 fastest; strip the tail factor
-    // (all later-axis segments were already resolved, so tailW contributions are 1)
-    int stride = 1;
+    // rem indexes inside the owner box, axis 0 fastest (mixed radix cs[0], cs[1], ...)
+    const int k = DK<DD, KK>::k(g);
 #pragma unroll
     for (int a = 0; a < MAXD; a++)
         if (a < d) {
-            q[a] = fs[a] + (rem / stride) % cs[a];
-            stride *= cs[a];
+            const int qd = div_seg(rem, cs[a], k);
+            q[a] = fs[a] + (rem - qd * cs[a]);
+            rem = qd;
         }
 }
 
 // value of entry (row node p, column node q) summed over the cells containing
 // both, in lexicographic cell order (last axis fastest): sum = integral + sum
 // for every cell whose integral is not exactly zero
+// value of entry (row node p, column node q) summed over the cells containing
+// both, in lexicographic cell order (last axis fastest): sum = integral + sum
+// for every cell whose integral is not exactly zero
 template <int DD, int KK>
-__device__ inline double entry_value(const RowParams& P, const int* p, const int* q, bool& present)
+__device__ inline double entry_value(const RowParams& P, const double* __restrict__ E, const int* const* cls,
+                                    const int* p, const int* q, bool& present)
 {
 #pragma clang fp contract(off)
     const int d = DK<DD, KK>::d(P.g), k = DK<DD, KK>::k(P.g);
-    int lo[MAXD], cnt[MAXD];
-    int ncell = 1;
+    // cells containing both p_a and q_a: lo[a] and, if cnt[a] == 2, lo[a] + 1
+    int lo[MAXD] = {0, 0, 0}, cnt[MAXD] = {1, 1, 1};
 #pragma unroll
     for (int a = 0; a < MAXD; a++)
         if (a < d) {
-            // cells containing p_a and q_a
             int plo, phi, qlo, qhi;
             if (p[a] % k == 0) { plo = p[a] / k - 1; phi = p[a] / k; } else { plo = phi = p[a] / k; }
             if (q[a] % k == 0) { qlo = q[a] / k - 1; qhi = q[a] / k; } else { qlo = qhi = q[a] / k; }
             const int l0 = max(max(plo, qlo), 0), h0 = min(min(phi, qhi), P.g.n[a] - 1);
             lo[a] = l0;
             cnt[a] = h0 - l0 + 1;
-            ncell *= cnt[a] > 0 ? cnt[a] : 0;
         }
     double sum = 0;
     present = false;
     const long nent = (long)P.nloc * P.nloc;
-    for (int ci = 0; ci < ncell; ci++) {
-        int c[MAXD];
-        int r = ci;
+    // lexicographic cell order (last axis fastest); every axis has at most 2 cells
 #pragma unroll
-        for (int a = MAXD - 1; a >= 0; a--)
-            if (a < d) { c[a] = lo[a] + r % cnt[a]; r /= cnt[a]; }   // lexicographic
-        long tup = 0, cell = 0;
-        int iv = 0, iu = 0, mul = 1;
+    for (int i0 = 0; i0 < 2; i0++)
 #pragma unroll
-        for (int a = 0; a < MAXD; a++)
-            if (a < d) {
-                tup = tup * P.ncls[a] + P.cls[a][c[a]];
-                cell = cell * P.g.n[a] + c[a];
-                iv += (p[a] - k * c[a]) * mul;
-                iu += (q[a] - k * c[a]) * mul;
-                mul *= k + 1;
+        for (int i1 = 0; i1 < 2; i1++)
+#pragma unroll
+            for (int i2 = 0; i2 < 2; i2++) {
+                const int ii[MAXD] = {i0, i1, i2};
+                bool in = true;
+#pragma unroll
+                for (int a = 0; a < MAXD; a++) in = in && ii[a] < cnt[a];
+                if (!in) continue;
+                // element-table index < 2^30 and cell index < 2^31 (checked on the host): 32-bit
+                unsigned tup = 0, cell = 0;
+                int iv = 0, iu = 0, mul = 1;
+#pragma unroll
+                for (int a = 0; a < MAXD; a++)
+                    if (a < d) {
+                        const int c = lo[a] + ii[a];
+                        tup = tup * (unsigned)P.ncls[a] + (P.ncls[a] == 1 ? 0u : (unsigned)cls[a][c]);
+                        cell = cell * (unsigned)P.g.n[a] + (unsigned)c;
+                        iv += (p[a] - k * c) * mul;
+                        iu += (q[a] - k * c) * mul;
+                        mul *= k + 1;
+                    }
+                double integral = 0;
+                for (int i = 0; i < P.nint; i++) {
+                    const double e = E[((unsigned)i * (unsigned)P.ntuples + tup) * (unsigned)nent +
+                                       (unsigned)(iv * P.nloc + iu)];
+                    integral += P.coeff ? P.coeff[cell] * e : e;
+                }
+                if (integral != 0) {
+                    sum = integral + sum;
+                    present = true;
+                }
             }
-        double integral = 0;
-        for (int i = 0; i < P.nint; i++) {
-            const double e = P.E[(i * P.ntuples + tup) * nent + (long)iv * P.nloc + iu];
-            integral += P.coeff ? P.coeff[cell] * e : e;
-        }
-        if (integral != 0) {
-            sum = integral + sum;
-            present = true;
-        }
-    }
     return sum;
 }
 
 // G lanes per row; FILL = false counts, FILL = true writes cols/vals
 // (FILL also checks that each row received exactly the entries its row pointer
 // reserved and raises *bad otherwise)
-template <int DD, int KK, int G, bool FILL>
+template <int DD, int KK, int G, bool FILL, bool LDS>
 __global__ __launch_bounds__(256) void k_assemble_rows(RowParams P, long nrows, int* __restrict__ counts,
                                                       const int* __restrict__ rowptr, int* __restrict__ cols,
                                                       double* __restrict__ vals, int* __restrict__ bad)
 {
 #pragma clang fp contract(off)
     const int d = DK<DD, KK>::d(P.g), k = DK<DD, KK>::k(P.g);
+    // element tables and cell classes: staged in LDS when they fit (uniform and
+    // few-class grids), read from global memory otherwise
+    extern __shared__ double lds_tab[];
+    const double* E = P.E;
+    const int* cls[MAXD] = {P.cls[0], P.cls[1], P.cls[2]};
+    if constexpr (LDS) {
+        const long esize = (long)P.nint * P.ntuples * P.nloc * P.nloc;
+        for (long i = threadIdx.x; i < esize; i += blockDim.x) lds_tab[i] = P.E[i];
+        int* lc = reinterpret_cast<int*>(lds_tab + esize);
+        int off = 0;
+#pragma unroll
+        for (int a = 0; a < MAXD; a++)
+            if (a < d) {
+                for (int i = threadIdx.x; i < P.g.n[a]; i += blockDim.x) lc[off + i] = P.cls[a][i];
+                off += P.g.n[a];
+            }
+        __syncthreads();
+        E = lds_tab;
+        off = 0;
+#pragma unroll
+        for (int a = 0; a < MAXD; a++)
+            if (a < d) { cls[a] = lc + off; off += P.g.n[a]; }
+    }
     const int lane = threadIdx.x % G;
     const int wlane = threadIdx.x & 63;
     const int gbase = wlane - lane;   // first lane of the group inside the wavefront
     const long groups_total = (long)gridDim.x * (blockDim.x / G);
     const long gi = (blockIdx.x * (long)blockDim.x + threadIdx.x) / G;
     const long wave_first = gi - wlane / G;   // first group of this wavefront
+    long tails[MAXD] = {1, 1, 1};
+    lattice_tails<DD, KK>(P.g, tails);
     for (long it = 0; wave_first + it < nrows; it += groups_total) {
-        const long row = gi + it;
+        // with one row per wavefront the row is uniform: its setup (lattice point,
+        // segments, cells, element-table offsets) then lives in scalar registers
+        const long row = G == 64 ? (long)__builtin_amdgcn_readfirstlane((int)(gi + it)) : gi + it;
         // all lanes of a wavefront iterate together so that ballots stay uniform
         const bool active = row < nrows;
         AxisSegs S[MAXD];
@@ -449,8 +554,8 @@ __global__ __launch_bounds__(256) void k_assemble_rows(RowParams P, long nrows, 
             if (active && t < total) {
                 int q[MAXD] = {0, 0, 0};
                 decode_candidate<DD, KK>(P.g, S, t, q);
-                v = entry_value<DD, KK>(P, p, q, present);
-                if (FILL || row == 0) colv = (int)dof_of_lat_t<DD, KK>(P.g, q);
+                v = entry_value<DD, KK>(P, E, cls, p, q, present);
+                if (FILL || row == 0) colv = (int)dof_of_lat_tails<DD, KK>(P.g, tails, q);
                 if (row == 0 && colv == 0) present = true;   // dummy (0,0) entry
             }
             const unsigned long long bal = __ballot(present);
@@ -801,8 +906,28 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     JFEM_CHECK(Esize < (1l << 30), JFEM_ERR_UNSUPPORTED, "too many distinct cell geometries");
     double* E = nullptr;
     JFEM_HIP(hipMalloc(&E, sizeof(double) * Esize));
-    k_element_tables<<<grid_for(Esize, 256, 16384), 256, 0, c->stream>>>(EP, D.tabs[0], D.tabs[1], D.tabs[2], E);
-    JFEM_HIP(hipGetLastError());
+    {
+        // node pairs iv <= iu, packed iv << 16 | iu
+        std::vector<int> pairs;
+        for (int v = 0; v < EP.nloc; v++)
+            for (int u = v; u < EP.nloc; u++) pairs.push_back(v << 16 | u);
+        int* pairs_d = nullptr;
+        JFEM_HIP(hipMalloc(&pairs_d, sizeof(int) * pairs.size()));
+        JFEM_HIP(hipMemcpyAsync(pairs_d, pairs.data(), sizeof(int) * pairs.size(), hipMemcpyHostToDevice, c->stream));
+        const long work = (long)pairs.size() * EP.ntuples * nint;
+        const int eg = grid_for(work, 256, 16384);
+        const int key = (g.d * 10 + g.k) * 10 + C.Q;
+        switch (key) {
+        case 325: k_element_tables<3, 2, 5><<<eg, 256, 0, c->stream>>>(EP, D.tabs[0], D.tabs[1], D.tabs[2], pairs_d, E); break;
+        case 315: k_element_tables<3, 1, 5><<<eg, 256, 0, c->stream>>>(EP, D.tabs[0], D.tabs[1], D.tabs[2], pairs_d, E); break;
+        case 215: k_element_tables<2, 1, 5><<<eg, 256, 0, c->stream>>>(EP, D.tabs[0], D.tabs[1], D.tabs[2], pairs_d, E); break;
+        case 225: k_element_tables<2, 2, 5><<<eg, 256, 0, c->stream>>>(EP, D.tabs[0], D.tabs[1], D.tabs[2], pairs_d, E); break;
+        default: k_element_tables<0, 0, 0><<<eg, 256, 0, c->stream>>>(EP, D.tabs[0], D.tabs[1], D.tabs[2], pairs_d, E); break;
+        }
+        JFEM_HIP(hipGetLastError());
+        JFEM_HIP(hipStreamSynchronize(c->stream));
+        (void)hipFree(pairs_d);
+    }
 
     double* coeff = nullptr;
     if (coeff_host) {
@@ -830,29 +955,39 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     int* bad = nullptr;
     JFEM_HIP(hipMalloc(&bad, sizeof(int)));
     JFEM_HIP(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
+    // tables in LDS when they fit in 48 KiB (any uniform or few-class grid)
+    long cls_ints = 0;
+    for (int a = 0; a < g.d; a++) cls_ints += g.n[a];
+    const size_t lds_bytes = sizeof(double) * Esize + sizeof(int) * cls_ints;
+    const bool use_lds = lds_bytes <= 48 * 1024;
     auto run = [&](auto kern) {
-        kern<<<blocks, 256, 0, c->stream>>>(RP, n, counts, A->rowptr, A->colidx, A->vals, bad);
+        kern<<<blocks, 256, use_lds ? lds_bytes : 0, c->stream>>>(RP, n, counts, A->rowptr, A->colidx, A->vals, bad);
         JFEM_HIP(hipGetLastError());
     };
+    auto run2 = [&](auto kern_lds, auto kern_glob) { use_lds ? run(kern_lds) : run(kern_glob); };
+#define JFEM_ASM_CASE(DD, KK, GG)                                                                   \
+    fill ? run2(k_assemble_rows<DD, KK, GG, true, true>, k_assemble_rows<DD, KK, GG, true, false>)  \
+         : run2(k_assemble_rows<DD, KK, GG, false, true>, k_assemble_rows<DD, KK, GG, false, false>)
     // G follows from (d, k): (2k+1)^d candidates per row
     auto launch = [&](bool fill) {
         const int dk = g.d * 10 + g.k;
         switch (dk) {
-        case 11: fill ? run(k_assemble_rows<1, 1, 16, true>) : run(k_assemble_rows<1, 1, 16, false>); break;
-        case 12: fill ? run(k_assemble_rows<1, 2, 16, true>) : run(k_assemble_rows<1, 2, 16, false>); break;
-        case 13: fill ? run(k_assemble_rows<1, 3, 16, true>) : run(k_assemble_rows<1, 3, 16, false>); break;
-        case 21: fill ? run(k_assemble_rows<2, 1, 16, true>) : run(k_assemble_rows<2, 1, 16, false>); break;
-        case 22: fill ? run(k_assemble_rows<2, 2, 32, true>) : run(k_assemble_rows<2, 2, 32, false>); break;
-        case 23: fill ? run(k_assemble_rows<2, 3, 64, true>) : run(k_assemble_rows<2, 3, 64, false>); break;
-        case 31: fill ? run(k_assemble_rows<3, 1, 32, true>) : run(k_assemble_rows<3, 1, 32, false>); break;
-        case 32: fill ? run(k_assemble_rows<3, 2, 64, true>) : run(k_assemble_rows<3, 2, 64, false>); break;
-        case 33: fill ? run(k_assemble_rows<3, 3, 64, true>) : run(k_assemble_rows<3, 3, 64, false>); break;
+        case 11: JFEM_ASM_CASE(1, 1, 16); break;
+        case 12: JFEM_ASM_CASE(1, 2, 16); break;
+        case 13: JFEM_ASM_CASE(1, 3, 16); break;
+        case 21: JFEM_ASM_CASE(2, 1, 16); break;
+        case 22: JFEM_ASM_CASE(2, 2, 32); break;
+        case 23: JFEM_ASM_CASE(2, 3, 64); break;
+        case 31: JFEM_ASM_CASE(3, 1, 32); break;
+        case 32: JFEM_ASM_CASE(3, 2, 64); break;
+        case 33: JFEM_ASM_CASE(3, 3, 64); break;
         default:
-            if (G == 16) fill ? run(k_assemble_rows<0, 0, 16, true>) : run(k_assemble_rows<0, 0, 16, false>);
-            else if (G == 32) fill ? run(k_assemble_rows<0, 0, 32, true>) : run(k_assemble_rows<0, 0, 32, false>);
-            else fill ? run(k_assemble_rows<0, 0, 64, true>) : run(k_assemble_rows<0, 0, 64, false>);
+            if (G == 16) JFEM_ASM_CASE(0, 0, 16);
+            else if (G == 32) JFEM_ASM_CASE(0, 0, 32);
+            else JFEM_ASM_CASE(0, 0, 64);
         }
     };
+#undef JFEM_ASM_CASE
     // With one integral, no exactly-zero element entry, and coefficients whose
     // products with the entries cannot round to zero, every cell contribution
     // is nonzero, so every structural candidate is in the pattern and the row

@@ -139,20 +139,38 @@ int spmv_dot(jfem_mat* A, const double* x, double* y)
 // depends only on the row pointer.
 constexpr long ST_SEG_ROWS = 4096;
 
-__global__ void k_split_blocks(const int* __restrict__ rowptr, long n, long cap, int vec, long nseg,
-                               int* __restrict__ seg_count, const long* __restrict__ seg_off, StBlk* __restrict__ out)
+__global__ __launch_bounds__(64) void k_split_blocks(const int* __restrict__ rowptr, long n, long cap, int vec, long nseg,
+                                                    int* __restrict__ seg_count, const long* __restrict__ seg_off,
+                                                    StBlk* __restrict__ out)
 {
-    const long sg = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    // one wavefront per segment: its row pointers are staged in LDS, then lane 0
+    // walks the segment; the end of each block is found by binary search (the
+    // row pointer is nondecreasing), which gives the same split as extending the
+    // block one row at a time while the next row still fits
+    __shared__ int rp[ST_SEG_ROWS + 1];
+    const long sg = blockIdx.x;
     if (sg >= nseg) return;
-    long r = sg * ST_SEG_ROWS;
-    const long end = r + ST_SEG_ROWS < n ? r + ST_SEG_ROWS : n;
+    const long r0 = sg * ST_SEG_ROWS;
+    const long end = r0 + ST_SEG_ROWS < n ? r0 + ST_SEG_ROWS : n;
+    const int len = (int)(end - r0);
+    for (int i = threadIdx.x; i <= len; i += blockDim.x) rp[i] = rowptr[r0 + i];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     long pos = out ? seg_off[sg] : 0;
     int cnt = 0;
-    while (r < end) {
-        const long a0 = rowptr[r] & ~(long)(vec - 1);
-        long e = r + 1;
-        while (e < end && e - r < ST_BS && (long)rowptr[e + 1] - a0 <= cap) e++;
-        if (out) out[pos++] = StBlk{(int)r, (int)e, rowptr[r], rowptr[e]};
+    int r = 0;
+    while (r < len) {
+        const long a0 = (long)rp[r] & ~(long)(vec - 1);
+        // smallest e in [r + 1, hi) whose next row does not fit, else hi
+        const int hi = r + ST_BS < len ? r + ST_BS : len;
+        int lo_e = r + 1, hi_e = hi;
+        while (lo_e < hi_e) {
+            const int mid = (lo_e + hi_e) >> 1;
+            if ((long)rp[mid + 1] - a0 <= cap) lo_e = mid + 1;
+            else hi_e = mid;
+        }
+        const int e = lo_e;
+        if (out) out[pos++] = StBlk{(int)(r0 + r), (int)(r0 + e), rp[r], rp[e]};
         cnt++;
         r = e;
     }
@@ -174,9 +192,8 @@ static StBlk* split_blocks(jfem_mat* A, long cap, int vec, long* nblk)
     long* off_d = nullptr;
     JFEM_HIP(hipMalloc(&cnt_d, sizeof(int) * nseg));
     JFEM_HIP(hipMalloc(&off_d, sizeof(long) * nseg));
-    const int tb = 64;
-    const int nb = (int)((nseg + tb - 1) / tb);
-    k_split_blocks<<<nb, tb, 0, c->stream>>>(A->rowptr, n, cap, vec, nseg, cnt_d, nullptr, nullptr);
+    const int nb = (int)nseg;
+    k_split_blocks<<<nb, 64, 0, c->stream>>>(A->rowptr, n, cap, vec, nseg, cnt_d, nullptr, nullptr);
     JFEM_HIP(hipGetLastError());
     std::vector<int> cnt(nseg);
     std::vector<long> off(nseg);
@@ -186,7 +203,7 @@ static StBlk* split_blocks(jfem_mat* A, long cap, int vec, long* nblk)
     for (long i = 0; i < nseg; i++) { off[i] = tot; tot += cnt[i]; }
     JFEM_HIP(hipMemcpyAsync(off_d, off.data(), sizeof(long) * nseg, hipMemcpyHostToDevice, c->stream));
     JFEM_HIP(hipMalloc(&d, sizeof(StBlk) * tot));
-    k_split_blocks<<<nb, tb, 0, c->stream>>>(A->rowptr, n, cap, vec, nseg, nullptr, off_d, d);
+    k_split_blocks<<<nb, 64, 0, c->stream>>>(A->rowptr, n, cap, vec, nseg, nullptr, off_d, d);
     JFEM_HIP(hipGetLastError());
     JFEM_HIP(hipStreamSynchronize(c->stream));
     (void)hipFree(cnt_d);
