@@ -47,7 +47,7 @@ struct LearnedState {
     jfem::Lat lat;               // finest node lattice
     float* stat = nullptr;       // np x (cin - 1) static input channels
     int* perm = nullptr;         // lattice index of every finest DOF
-    int* iperm = nullptr;        // finest DOF of every lattice pixel (-1: none)
+    float* stat_dof = nullptr;   // the static channels in DOF order
     int runs = 1;                // learned steps per smoothing call
     int active = 0;
     int fell_back = 0;
@@ -678,7 +678,7 @@ static void free_mg(jfem_mg* G)
     (void)hipFree(G->coarse_inv);
     (void)hipFree(G->init); (void)hipFree(G->defect); (void)hipFree(G->tmp);
     LearnedState& S = G->ls;
-    for (void* p : {(void*)S.stat, (void*)S.perm, (void*)S.iperm, (void*)S.rr, (void*)S.u, (void*)S.z, (void*)S.gd, (void*)S.rh, (void*)S.xs,
+    for (void* p : {(void*)S.stat, (void*)S.stat_dof, (void*)S.perm, (void*)S.rr, (void*)S.u, (void*)S.z, (void*)S.gd, (void*)S.rh, (void*)S.xs,
                     (void*)S.bs, (void*)S.loss, (void*)S.capbuf, (void*)S.capx, (void*)S.xt, (void*)S.eh,
                     (void*)S.w, (void*)S.y, (void*)S.loss2})
         (void)hipFree(p);
@@ -841,32 +841,34 @@ __global__ void k_gather_in(const int* __restrict__ perm, long n, const T* __res
 }
 
 // the network input of k_gather_in already split for the LDS-tiled first
-// layer: one thread per pixel in lattice order, so the 32-byte stores are
-// contiguous and only the DOF-order reads are gathered; the values are those
-// of k_gather_in followed by the split (the same float products, rounded once)
+// layer: one thread per DOF, reading r, D^-1 and the static channels (kept in
+// DOF order for this) contiguously and storing the pixel's 32 bytes (one
+// sector) at its lattice position; the values are those of k_gather_in
+// followed by the split (the same float products, rounded once)
 template <typename T>
-__global__ void k_gather_split(const int* __restrict__ iperm, long np, const T* __restrict__ r,
+__global__ void k_gather_split(const int* __restrict__ perm, long n, const T* __restrict__ r,
                                const T* __restrict__ dinv, const double* __restrict__ rr,
-                               const float* __restrict__ stat, int cin, unsigned short* __restrict__ inb)
+                               const float* __restrict__ stat_dof, int cin, unsigned short* __restrict__ inb)
 {
-    const double scale = rr[0] > 0 ? sqrt((double)np / rr[0]) : 0.0;   // np equals the DOF count
-    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
-        const int i = iperm[p];
-        const float v = i >= 0 ? (float)((double)dinv[i] * (double)r[i] * scale) : 0.f;
+    const double scale = rr[0] > 0 ? sqrt((double)n / rr[0]) : 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const float v = (float)((double)dinv[i] * (double)r[i] * scale);
         bf16x8 hi, lo;
         split_bf16(v, hi, lo, 0);
 #pragma unroll
-        for (int c = 1; c < 8; c++) split_bf16(c < cin ? v * stat[p * (cin - 1) + c - 1] : 0.f, hi, lo, c);
+        for (int c = 1; c < 8; c++) split_bf16(c < cin ? v * stat_dof[i * (cin - 1) + c - 1] : 0.f, hi, lo, c);
+        const long p = perm[i];
         *reinterpret_cast<bf16x8*>(inb + p * 16) = hi;
         *reinterpret_cast<bf16x8*>(inb + p * 16 + 8) = lo;
     }
 }
 
-// inverse of the DOF -> lattice permutation
-__global__ void k_invert_perm(long n, const int* __restrict__ perm, int* __restrict__ iperm)
+// static channels in DOF order: out[i][c] = stat[perm[i]][c]
+__global__ void k_stat_to_dof(long n, int ns, const int* __restrict__ perm, const float* __restrict__ stat,
+                              float* __restrict__ out)
 {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        iperm[perm[i]] = (int)i;
+        for (int c = 0; c < ns; c++) out[i * ns + c] = stat[(long)perm[i] * ns + c];
 }
 
 // x[i] += out[pix] * rms(D^-1 r) on free nodes
@@ -1028,7 +1030,7 @@ static void learned_smooth(jfem_mg* G, MgLevel& L, const T* b, T* x, T* u, bool 
         }
         sumsq_dev<T>(c, n, r, dinv_of<T>(L), S.rr);
         if (nn_split_input(S.nn, S.lat)) {
-            k_gather_split<T><<<gr, 256, 0, st>>>(S.iperm, n, r, dinv_of<T>(L), S.rr, S.stat, S.nn->cin,
+            k_gather_split<T><<<gr, 256, 0, st>>>(S.perm, n, r, dinv_of<T>(L), S.rr, S.stat_dof, S.nn->cin,
                                                   S.nn->inb);
             nn_forward(S.nn, S.lat, true);
         } else {
@@ -1172,11 +1174,10 @@ static void set_learned(jfem_mg* G, jfem_nn* nn, const double* cell_logc, int ru
     JFEM_HIP(hipMalloc(&S.perm, sizeof(int) * np));
     k_lattice_perm<<<grid_for(np, 256, 16384), 256, 0, G->ctx->stream>>>(to_dev(g), S.lat, np, S.perm);
     JFEM_HIP(hipGetLastError());
-    (void)hipFree(S.iperm);
-    S.iperm = nullptr;
-    JFEM_HIP(hipMalloc(&S.iperm, sizeof(int) * np));
-    k_fill_t<int><<<grid_for(np, 256, 16384), 256, 0, G->ctx->stream>>>(np, -1, S.iperm);
-    k_invert_perm<<<grid_for(np, 256, 16384), 256, 0, G->ctx->stream>>>(np, S.perm, S.iperm);
+    (void)hipFree(S.stat_dof);
+    S.stat_dof = nullptr;
+    JFEM_HIP(hipMalloc(&S.stat_dof, sizeof(float) * stat.size()));
+    k_stat_to_dof<<<grid_for(np, 256, 16384), 256, 0, G->ctx->stream>>>(np, ns, S.perm, S.stat, S.stat_dof);
     JFEM_HIP(hipGetLastError());
     if (!S.rr) JFEM_HIP(hipMalloc(&S.rr, sizeof(double) * 8));
     nn_reserve(nn, np);
