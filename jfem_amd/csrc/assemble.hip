@@ -757,54 +757,82 @@ struct RhsParams {
     const double* sinq[MAXD];   // sin(kw * pi * x_q) per axis: [term][cell][q]
 };
 
-__global__ __launch_bounds__(128) void k_rhs_color(RhsParams P, const Tab1D* __restrict__ t0, const Tab1D* __restrict__ t1,
-                                                   const Tab1D* __restrict__ t2, int color, long ncolor_cells,
-                                                   double* __restrict__ rhs)
+// One workgroup of 256 threads takes CPB = 256 / nloc cells of one colour:
+// first the source at the cells' quadrature points goes to LDS, then thread
+// (cell, iv) sums its basis function's quadrature (the arithmetic of each value
+// in the same order as before).  DD, KK, QQ > 0 fix dimension, degree and
+// points per axis, so the point loops unroll.
+constexpr int RHS_BS = 256;
+
+template <int DD, int KK, int QQ>
+__global__ __launch_bounds__(RHS_BS) void k_rhs_color(RhsParams P, const Tab1D* __restrict__ t0,
+                                                      const Tab1D* __restrict__ t1, const Tab1D* __restrict__ t2,
+                                                      int color, long ncolor_cells, double* __restrict__ rhs)
 {
 #pragma clang fp contract(off)
-    __shared__ double fq[MAXQ * MAXQ * MAXQ];
-    const int d = P.g.d;
-    int half[MAXD];
+    extern __shared__ double fq[];   // [CPB][nq]
+    const int d = DD ? DD : P.g.d, k = KK ? KK : P.g.k, Q = QQ ? QQ : P.Q;
+    int nq = 1;
+    for (int a = 0; a < d; a++) nq *= Q;
+    const int nloc = P.nloc, cpb = RHS_BS / nloc;
+    int half[MAXD] = {1, 1, 1};
     for (int a = 0; a < d; a++) half[a] = (P.g.n[a] - ((color >> a) & 1) + 1) / 2;
-    for (long cid = blockIdx.x; cid < ncolor_cells; cid += gridDim.x) {
-        int c[MAXD];
-        long r = cid;
-        for (int a = d - 1; a >= 0; a--) { c[a] = 2 * (int)(r % half[a]) + ((color >> a) & 1); r /= half[a]; }
-        int nq = 1;
-        for (int a = 0; a < d; a++) nq *= P.Q;
+    for (long cb = (long)blockIdx.x * cpb; cb < ncolor_cells; cb += (long)gridDim.x * cpb) {
         __syncthreads();
-        for (int q = threadIdx.x; q < nq; q += blockDim.x) {
-            int qa[MAXD];
-            int qq = q;
-            for (int a = d - 1; a >= 0; a--) { qa[a] = qq % P.Q; qq /= P.Q; }
-            double s = 0;
+        // the source at the quadrature points of the block's cells
+        for (int f = threadIdx.x; f < cpb * nq; f += RHS_BS) {
+            const long cid = cb + f / nq;
+            if (cid >= ncolor_cells) continue;
+            int c[MAXD] = {0, 0, 0};
+            long r = cid;
+            for (int a = d - 1; a >= 0; a--) { c[a] = 2 * (int)(r % half[a]) + ((color >> a) & 1); r /= half[a]; }
+            int qa[MAXD] = {0, 0, 0};
+            int qq = f % nq;
+            for (int a = d - 1; a >= 0; a--) { qa[a] = qq % Q; qq /= Q; }
+            double sv = 0;
             for (int m = 0; m < P.nterms; m++) {
                 double t = 1;   // amplitude folded into the axis-0 table
-                for (int a = 0; a < d; a++) t *= P.sinq[a][((long)m * P.g.n[a] + c[a]) * P.Q + qa[a]];
-                s += t;
+                for (int a = 0; a < d; a++) t *= P.sinq[a][((long)m * P.g.n[a] + c[a]) * Q + qa[a]];
+                sv += t;
             }
-            fq[q] = s;
+            fq[f] = sv;
         }
         __syncthreads();
-        const Tab1D* tb[MAXD] = {t0 + P.cls[0][c[0]], d > 1 ? t1 + P.cls[1][c[1]] : nullptr,
-                                 d > 2 ? t2 + P.cls[2][c[2]] : nullptr};
-        for (int iv = threadIdx.x; iv < P.nloc; iv += blockDim.x) {
-            int lv[MAXD], rr = iv, p[MAXD];
-            for (int a = 0; a < d; a++) { lv[a] = rr % (P.g.k + 1); rr /= P.g.k + 1; p[a] = c[a] * P.g.k + lv[a]; }
-            double ret = 0;
-            for (int q = 0; q < nq; q++) {
-                int qa[MAXD];
-                int qq = q;
-                for (int a = d - 1; a >= 0; a--) { qa[a] = qq % P.Q; qq /= P.Q; }
-                double v = 1;
-                for (int a = 0; a < d; a++) v *= tb[a]->val[lv[a]][qa[a]];
-                double val = v * fq[q];
-                for (int a = 0; a < d; a++) val *= tb[a]->w[qa[a]];
-                ret += val;
-            }
-            const long dof = dof_of_lattice(P.g, p);
-            if (ret != 0) rhs[dof] += ret;
+        const int ci = threadIdx.x / nloc, iv = threadIdx.x % nloc;
+        const long cid = cb + ci;
+        if (ci >= cpb || cid >= ncolor_cells) continue;
+        int c[MAXD] = {0, 0, 0};
+        long r = cid;
+        for (int a = d - 1; a >= 0; a--) { c[a] = 2 * (int)(r % half[a]) + ((color >> a) & 1); r /= half[a]; }
+        const Tab1D* tb[MAXD] = {t0 + P.cls[0][c[0]], d > 1 ? t1 + P.cls[1][c[1]] : t0,
+                                 d > 2 ? t2 + P.cls[2][c[2]] : t0};
+        int lv[MAXD] = {0, 0, 0}, p[MAXD] = {0, 0, 0};
+        int rr = iv;
+#pragma unroll
+        for (int a = 0; a < MAXD; a++)
+            if (a < d) { lv[a] = rr % (k + 1); rr /= k + 1; p[a] = c[a] * k + lv[a]; }
+        const double* fc = fq + ci * nq;
+        double ret = 0;
+#pragma unroll
+        for (int q = 0; q < (QQ && DD ? (DD == 3 ? QQ * QQ * QQ : DD == 2 ? QQ * QQ : QQ) : 1 << 30); q++) {
+            if (!(QQ && DD) && q >= nq) break;
+            int qa[MAXD] = {0, 0, 0};
+            int qq = q;
+#pragma unroll
+            for (int a = MAXD - 1; a >= 0; a--)
+                if (a < d) { qa[a] = qq % Q; qq /= Q; }
+            double v = 1;
+#pragma unroll
+            for (int a = 0; a < MAXD; a++)
+                if (a < d) v *= tb[a]->val[lv[a]][qa[a]];
+            double val = v * fc[q];
+#pragma unroll
+            for (int a = 0; a < MAXD; a++)
+                if (a < d) val *= tb[a]->w[qa[a]];
+            ret += val;
         }
+        const long dof = dof_of_lattice(P.g, p);
+        if (ret != 0) rhs[dof] += ret;
     }
 }
 
@@ -1090,7 +1118,19 @@ void assemble_rhs_sines(jfem_space* s, int nterms, const int* kw, const double* 
             ncc *= h;
         }
         if (empty) continue;
-        k_rhs_color<<<grid_for(ncc, 1, 65536), 128, 0, c->stream>>>(P, D.tabs[0], D.tabs[1], D.tabs[2], color, ncc, rhs);
+        const int cpb = RHS_BS / P.nloc;
+        int nq = 1;
+        for (int a = 0; a < g.d; a++) nq *= C.Q;
+        const size_t lds = sizeof(double) * cpb * nq;
+        const int blocks = grid_for((ncc + cpb - 1) / cpb, 1, 65536);
+        const int key = (g.d * 10 + g.k) * 10 + C.Q;
+        switch (key) {
+        case 325: k_rhs_color<3, 2, 5><<<blocks, RHS_BS, lds, c->stream>>>(P, D.tabs[0], D.tabs[1], D.tabs[2], color, ncc, rhs); break;
+        case 315: k_rhs_color<3, 1, 5><<<blocks, RHS_BS, lds, c->stream>>>(P, D.tabs[0], D.tabs[1], D.tabs[2], color, ncc, rhs); break;
+        case 215: k_rhs_color<2, 1, 5><<<blocks, RHS_BS, lds, c->stream>>>(P, D.tabs[0], D.tabs[1], D.tabs[2], color, ncc, rhs); break;
+        case 225: k_rhs_color<2, 2, 5><<<blocks, RHS_BS, lds, c->stream>>>(P, D.tabs[0], D.tabs[1], D.tabs[2], color, ncc, rhs); break;
+        default: k_rhs_color<0, 0, 0><<<blocks, RHS_BS, lds, c->stream>>>(P, D.tabs[0], D.tabs[1], D.tabs[2], color, ncc, rhs); break;
+        }
         JFEM_HIP(hipGetLastError());
     }
     JFEM_HIP(hipStreamSynchronize(c->stream));
