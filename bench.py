@@ -431,12 +431,16 @@ def main():
             # learned smoother: trained on the same Q2 Poisson operator on 24^3 cells (the
             # network input is the Jacobi-scaled residual, independent of the mesh size),
             # then attached to the finest level of the FP32 hierarchy (one application per
-            # smoothing call, Chebyshev fallback when a cycle reduces less than 0.5x)
+            # smoothing call, Chebyshev fallback when a cycle reduces less than 0.5x).  The
+            # coarser levels smooth with Chebyshev-Jacobi deg 3 V(1,1), in training too.
+            # 400 right-hand sides x 8 cycles: shorter training leaves a weaker smoother
+            # (40 x 8: 7 FCG iterations instead of 5), longer gains nothing
+            coarse_sm = jf.ChebyshevSmoother(degree=3, runs=1, ratio=20.0)
             t0 = time.perf_counter()
-            _, Gt, _ = build_problem(jf, base, 4, 2, smoother, jf.FP64, 2, 16)
+            _, Gt, _ = build_problem(jf, base, 4, 2, coarse_sm, jf.FP64, 2, 16)
             net = jf.SmootherNet(3, 5, seed=7)
             Gt.set_learned(net, np.zeros((base[0] << 3) ** 3), runs=1, fallback_ratio=0.5)
-            h = Gt.train(rhs_per_field=40, cycles_per_rhs=8, lr=3e-4, seed=1, mode=5)
+            h = Gt.train(rhs_per_field=400, cycles_per_rhs=8, lr=3e-4, seed=1, mode=5)
             ctx.sync()
             t_train = time.perf_counter() - t0
             del Gt
@@ -449,8 +453,7 @@ def main():
             # Chebyshev-Jacobi, degree 3, one application per side (V(1,1)), which measured fastest
             # for this pairing; the same hierarchy without the network is timed below
             del G32
-            _, G32, _ = build_problem(jf, base, nlev, 2, jf.ChebyshevSmoother(degree=3, runs=1, ratio=20.0),
-                                      jf.FP32, 2, 16)
+            _, G32, _ = build_problem(jf, base, nlev, 2, coarse_sm, jf.FP32, 2, 16)
             A32 = G32.getFinestSystem()
             G32.set_learned(net, np.zeros(ncell), runs=1, fallback_ratio=0.5)
 
@@ -484,6 +487,19 @@ def main():
             log(f"same FP32 hierarchy, Chebyshev deg 3 V(1,1) on every level: {itc.iterations} its, "
                 f"{np.median(tc):.3f}s")
             del G32
+            # the fastest plain FP32 cycle found: degree-1 Chebyshev (damped Jacobi) V(1,1)
+            _, G1, _ = build_problem(jf, base, nlev, 2, jf.ChebyshevSmoother(degree=1, runs=1, ratio=20.0),
+                                     jf.FP32, 2, 16)
+            A1 = G1.getFinestSystem()
+            tc = []
+            for s in range(max(1, args.steps) + 1):
+                itc = jf.IterativeSolver()
+                itc.solveFCG(A1, G1, b, tol)
+                tc.append(itc.info.seconds)
+            extra["fp32mg_cheb_v11_deg1_fcg_s"] = float(np.median(tc[1:]))
+            extra["fp32mg_cheb_v11_deg1_fcg_iterations"] = itc.iterations
+            log(f"FP32 hierarchy, Chebyshev deg 1 V(1,1): {itc.iterations} its, {np.median(tc[1:]):.3f}s")
+            del G1
             # stationary V-cycle iteration with each smoother (FP64 hierarchy, same network)
             G64 = build_problem(jf, base, nlev, 2, smoother, jf.FP64, 2, 16)[1]
             G64.set_learned(net, np.zeros(ncell), runs=1, fallback_ratio=0.5)
