@@ -221,6 +221,12 @@ int jfem_nn_adam_step(jfem_nn *nn, const float *grad, float lr);
 int jfem_mg_set_learned(jfem_mg *mg, jfem_nn *nn, const double *cell_log_coeff, int runs, double fallback_ratio,
                         int step_rule);
 int jfem_mg_set_learned_active(jfem_mg *mg, int active);
+/* which smoothing calls of a V-cycle on the finest level use the attached
+ * network: 1 pre-smoothing, 2 post-smoothing, 3 both (default); the other
+ * side keeps the hierarchy's smoother.  New; the reference's
+ * MGPreconditionerSpace takes one smoother per level for both sides
+ * (M/multigrid/MGPreconditionerInterface.java:125-198). */
+int jfem_mg_set_learned_sides(jfem_mg *mg, int sides);
 int jfem_mg_learned_status(jfem_mg *mg, int *active, int *fell_back);
 typedef struct {
     int rhs_per_field;        /* seeded random right-hand sides */

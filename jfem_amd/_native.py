@@ -117,6 +117,7 @@ _SIGS = {
     "jfem_nn_load": (C.c_int, [_vp, C.c_char_p, C.POINTER(_vp)]),
     "jfem_mg_set_learned": (C.c_int, [_vp, _vp, _dp, C.c_int, C.c_double, C.c_int]),
     "jfem_mg_set_learned_active": (C.c_int, [_vp, C.c_int]),
+    "jfem_mg_set_learned_sides": (C.c_int, [_vp, C.c_int]),
     "jfem_mg_learned_status": (C.c_int, [_vp, _ip, _ip]),
     "jfem_nn_train_mg": (C.c_int, [_vp, C.POINTER(TrainOpts), _dp, _ip]),
     "jfem_mg_solve": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int, C.POINTER(SolveInfo), _dp]),

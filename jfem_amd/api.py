@@ -496,6 +496,13 @@ class MGPreconditioner:
     def set_learned_active(self, active: bool):
         check(N.lib().jfem_mg_set_learned_active(self.h, 1 if active else 0))
 
+    LEARNED_PRE, LEARNED_POST, LEARNED_BOTH = 1, 2, 3
+
+    def set_learned_sides(self, sides: int):
+        """smoothing calls on the finest level that use the network (1 pre, 2 post, 3 both); the
+        other side keeps the hierarchy's smoother"""
+        check(N.lib().jfem_mg_set_learned_sides(self.h, sides))
+
     def learned_status(self):
         a, f = C.c_int(), C.c_int()
         check(N.lib().jfem_mg_learned_status(self.h, C.byref(a), C.byref(f)))

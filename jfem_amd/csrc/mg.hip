@@ -49,6 +49,7 @@ struct LearnedState {
     int* perm = nullptr;         // lattice index of every finest DOF
     float* stat_dof = nullptr;   // the static channels in DOF order
     int runs = 1;                // learned steps per smoothing call
+    int sides = 3;               // smoothing calls that use the network: 1 pre-smoothing, 2 post-smoothing, 3 both
     int active = 0;
     int fell_back = 0;
     double fallback = 0.5;       // switch to the plain smoother when a cycle reduces less than this
@@ -443,14 +444,15 @@ double level_norm<float>(jfem_mg* G, long n, const float* r)
 template <typename T>
 static void learned_smooth(jfem_mg* G, MgLevel& L, const T* b, T* x, T* u, bool x_zero);
 
-// Smoother.smooth(op, rhs, iterate): x (in/out, in L.x) for rhs b
+// Smoother.smooth(op, rhs, iterate): x (in/out, in L.x) for rhs b; post marks
+// the post-smoothing call of a V-cycle
 template <typename T>
-static void smooth(jfem_mg* G, MgLevel& L, const T* b, T*& x, T*& xalt, bool x_zero)
+static void smooth(jfem_mg* G, MgLevel& L, const T* b, T*& x, T*& xalt, bool x_zero, bool post)
 {
     const jfem_mg_opts& o = G->opt;
     hipStream_t st = G->ctx->stream;
     const long n = L.A->rows;
-    if (G->ls.active && &L == &G->lev.back()) {
+    if (G->ls.active && &L == &G->lev.back() && (G->ls.sides & (post ? 2 : 1))) {
         learned_smooth<T>(G, L, b, x, xalt, x_zero);
         return;
     }
@@ -536,7 +538,7 @@ static void step_v(jfem_mg* G, int l, bool x_zero)
     if (l == 0) { coarse_solve<T>(G, b, (T*)L.x); return; }
     T* x = (T*)L.x;
     T* xalt = (T*)L.t;
-    smooth<T>(G, L, b, x, xalt, x_zero);
+    smooth<T>(G, L, b, x, xalt, x_zero, false);
     MgLevel& C = G->lev[l - 1];
     T* r = (T*)L.r;
     level_residual<T>(G, L, b, x, r);
@@ -550,7 +552,7 @@ static void step_v(jfem_mg* G, int l, bool x_zero)
         if (G->ls.capx)
             k_convert<T, double><<<grid_for(L.A->rows, 256, 16384), 256, 0, G->ctx->stream>>>(L.A->rows, x, G->ls.capx);
     }
-    smooth<T>(G, L, b, x, xalt, false);
+    smooth<T>(G, L, b, x, xalt, false, true);
     // keep the result in L.x / the spare in L.t
     if (x != (T*)L.x) { L.t = L.x; L.x = x; }
 }
@@ -1458,6 +1460,15 @@ int jfem_mg_learned_status(jfem_mg* mg, int* active, int* fell_back)
     JFEM_CHECK(mg, JFEM_ERR_ARG, "null argument");
     if (active) *active = mg->ls.active;
     if (fell_back) *fell_back = mg->ls.fell_back;
+    API_END
+}
+
+int jfem_mg_set_learned_sides(jfem_mg* mg, int sides)
+{
+    API_BEGIN
+    JFEM_CHECK(mg, JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(sides >= 1 && sides <= 3, JFEM_ERR_ARG, "sides must be 1 (pre), 2 (post) or 3 (both)");
+    mg->ls.sides = sides;
     API_END
 }
 
