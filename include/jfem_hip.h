@@ -85,6 +85,18 @@ int jfem_space_cell_dofs(jfem_space *s, int *out);
 int jfem_space_boundary_nodes(jfem_space *s, int *out, long *count);
 /* node point of every DOF (LagrangeNodeFunctional point), out[dof*dim + a] */
 int jfem_space_node_coords(jfem_space *s, double *out);
+/* ScalarFESpaceFunction(functions, coeffs).value(p) (M/basic/ScalarFESpaceFunction.java:40-45)
+ * at npts points (points[i*dim + a], host): sum of coefficient * shape-function value,
+ * each shape function evaluated in the first of its cells containing the point
+ * (ScalarShapeFunction.value, M/basic/ScalarShapeFunction.java:26-31); 0 outside
+ * the domain.  values: npts doubles (host). */
+int jfem_space_evaluate(jfem_space *s, const jfem_vec *coeffs, long npts, const double *points, double *values);
+/* ConvergenceOrderEstimator.normL2Difference (Core/src/test/java/examples/
+ * ConvergenceOrderEstimator.java:49-57) of two FE functions (spaces of the same
+ * dimension, e.g. two refinement levels) at npts host points:
+ * sqrt(sum_i (f1(p_i) - f2(p_i))^2). */
+int jfem_fe_l2_difference(jfem_space *s1, const jfem_vec *c1, jfem_space *s2, const jfem_vec *c2, long npts,
+                          const double *points, double *result);
 
 /* ---- vectors (DenseVector) ---------------------------------------------- */
 int jfem_vec_create(jfem_ctx *ctx, long n, jfem_vec **out);

@@ -5,12 +5,14 @@ libjfem_hip.so behind the C ABI of include/jfem_hip.h."""
 from ._native import (IllegalArgumentException, IllegalStateException, JfemError, NativeLibraryError,
                       UnsupportedOperationException, lib)
 from .api import (FP32, FP64, ChebyshevSmoother, ContinuousTPFESpace, Context, DenseVector, IterativeSolver,
-                  JacobiSmoother, MGPreconditioner, QuadratureRule1D, RichardsonSmoother, SinusoidSource, SmootherNet,
-                  SparseMatrix, TPCellIntegral, TPRightHandSideIntegral, context)
+                  JacobiSmoother, MGPreconditioner, QuadratureRule1D, RichardsonSmoother, ScalarFESpaceFunction,
+                  SinusoidSource, SmootherNet, SparseMatrix, TPCellIntegral, TPRightHandSideIntegral, context,
+                  normL2Difference)
 
 __all__ = [
     "IllegalArgumentException", "IllegalStateException", "JfemError", "NativeLibraryError",
     "UnsupportedOperationException", "lib", "FP32", "FP64", "ChebyshevSmoother", "ContinuousTPFESpace", "Context",
     "DenseVector", "IterativeSolver", "JacobiSmoother", "MGPreconditioner", "QuadratureRule1D",
-    "RichardsonSmoother", "SinusoidSource", "SmootherNet", "SparseMatrix", "TPCellIntegral", "TPRightHandSideIntegral", "context",
+    "RichardsonSmoother", "ScalarFESpaceFunction", "SinusoidSource", "SmootherNet", "SparseMatrix", "TPCellIntegral",
+    "TPRightHandSideIntegral", "context", "normL2Difference",
 ]

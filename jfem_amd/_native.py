@@ -75,6 +75,8 @@ _SIGS = {
     "jfem_space_cell_dofs": (C.c_int, [_vp, _ip]),
     "jfem_space_boundary_nodes": (C.c_int, [_vp, _ip, _lp]),
     "jfem_space_node_coords": (C.c_int, [_vp, _dp]),
+    "jfem_space_evaluate": (C.c_int, [_vp, _vp, C.c_long, _dp, _dp]),
+    "jfem_fe_l2_difference": (C.c_int, [_vp, _vp, _vp, _vp, C.c_long, _dp, _dp]),
     "jfem_vec_create": (C.c_int, [_vp, C.c_long, C.POINTER(_vp)]),
     "jfem_vec_destroy": (C.c_int, [_vp]),
     "jfem_vec_upload": (C.c_int, [_vp, _dp]),

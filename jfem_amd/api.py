@@ -383,6 +383,39 @@ class ContinuousTPFESpace:
         self.fixedNodes.update(int(i) for i in nodes)
 
 
+class ScalarFESpaceFunction:
+    """ScalarFESpaceFunction(functions, coefficients) (M/basic/ScalarFESpaceFunction.java:8-52):
+    the FE function sum_i c_i phi_i of a ContinuousTPFESpace, evaluated on the GPU."""
+
+    def __init__(self, space: ContinuousTPFESpace, coefficients: DenseVector):
+        space._need()
+        if coefficients.n != space.ndofs():
+            raise N.IllegalArgumentException(1, "coefficient vector size differs from the space")
+        self.space, self.coefficients = space, coefficients
+
+    def getDomainDimension(self) -> int:
+        return self.space.dim
+
+    def value(self, points) -> np.ndarray:
+        """values at an (npts, dim) array of points (or one point)"""
+        pts = f64(points).reshape(-1, self.space.dim)
+        out = np.zeros(pts.shape[0])
+        check(N.lib().jfem_space_evaluate(self.space.h, self.coefficients.h, pts.shape[0], dptr(pts), dptr(out)))
+        return out
+
+
+def normL2Difference(f1: ScalarFESpaceFunction, f2: ScalarFESpaceFunction, points) -> float:
+    """ConvergenceOrderEstimator.normL2Difference (Core/src/test/java/examples/ConvergenceOrderEstimator.java:49-57):
+    sqrt(sum over the points of (f1(p) - f2(p))^2)"""
+    if f1.space.dim != f2.space.dim:
+        raise N.IllegalArgumentException(1, "the two functions have different dimensions")
+    pts = f64(points).reshape(-1, f1.space.dim)
+    out = C.c_double()
+    check(N.lib().jfem_fe_l2_difference(f1.space.h, f1.coefficients.h, f2.space.h, f2.coefficients.h, pts.shape[0],
+                                        dptr(pts), C.byref(out)))
+    return out.value
+
+
 # --------------------------------------------------------------------------
 # multigrid
 # --------------------------------------------------------------------------

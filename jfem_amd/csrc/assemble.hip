@@ -75,7 +75,7 @@ static int quad_rule(int rule, double* pts, double* wts)
 
 // LagrangeBasisFunction1D.valueOnReferenceCell / derivativeOnReferenceCell
 // (M/tensorproduct/LagrangeBasisFunction1D.java:55-154)
-static double lag_value(double pos, int l, int k)
+__host__ __device__ static double lag_value(double pos, int l, int k)
 {
     if (k == 1) return l == 0 ? 1.0 - pos : pos;
     if (k == 2) {
@@ -89,7 +89,7 @@ static double lag_value(double pos, int l, int k)
     return ret;
 }
 
-static double lag_deriv(double pos, int l, int k)
+__host__ __device__ static double lag_deriv(double pos, int l, int k)
 {
     if (k == 1) return l == 0 ? -1.0 : 1.0;
     if (k == 2) {
@@ -109,7 +109,7 @@ static double lag_deriv(double pos, int l, int k)
 }
 
 // Cell1D.isInCell with PerformanceArguments.doubleTolerance = 1e-10 (Cell1D.java:55-59)
-static bool in_cell(double s, double e, double pos) { return pos >= s - 100 * 1e-10 && pos <= e + 100 * 1e-10; }
+__host__ __device__ static bool in_cell(double s, double e, double pos) { return pos >= s - 100 * 1e-10 && pos <= e + 100 * 1e-10; }
 #pragma clang fp contract(on)
 
 // 1D table of one cell: weights w[q] = refw*len, value[l][q], deriv[l][q]
@@ -745,6 +745,91 @@ __global__ void k_node_coords(GridDev g, long n, const double* __restrict__ s0, 
     }
 }
 
+// ScalarFESpaceFunction.value (M/basic/ScalarFESpaceFunction.java:40-45) at
+// a point: sum over the shape functions of coefficient * value.  A shape
+// function's value is its product of 1D Lagrange values in the first of its
+// cells that contains the point (ScalarShapeFunction.value, Cell1D.isInCell
+// with 1e-8 slack), 0 if none does; only the (k+1)^d functions of the point's
+// cell are nonzero.  Per axis the cell is the one the point's coordinate falls
+// in (the lower one on a shared face, where continuity makes both agree up to
+// rounding); a point outside the domain evaluates to 0.
+__global__ void k_eval_fe(GridDev g, const double* __restrict__ s0, const double* __restrict__ s1,
+                          const double* __restrict__ s2, const double* __restrict__ e0, const double* __restrict__ e1,
+                          const double* __restrict__ e2, const double* __restrict__ coeff, long npts,
+                          const double* __restrict__ pts, double* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    const double* S[MAXD] = {s0, s1, s2};
+    const double* Ee[MAXD] = {e0, e1, e2};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < npts; i += (long)gridDim.x * blockDim.x) {
+        int c[MAXD] = {0, 0, 0};
+        double t[MAXD] = {0, 0, 0};
+        bool inside = true;
+        for (int a = 0; a < g.d; a++) {
+            const double x = pts[i * g.d + a];
+            const double w = (Ee[a][g.n[a] - 1] - S[a][0]) / g.n[a];
+            int ca = (int)floor((x - S[a][0]) / w);
+            ca = ca < 0 ? 0 : (ca > g.n[a] - 1 ? g.n[a] - 1 : ca);
+            // the rounded division can land one cell off near a face
+            if (!in_cell(S[a][ca], Ee[a][ca], x) && ca > 0 && in_cell(S[a][ca - 1], Ee[a][ca - 1], x)) ca--;
+            if (!in_cell(S[a][ca], Ee[a][ca], x) && ca < g.n[a] - 1 && in_cell(S[a][ca + 1], Ee[a][ca + 1], x)) ca++;
+            if (!in_cell(S[a][ca], Ee[a][ca], x)) inside = false;
+            c[a] = ca;
+            t[a] = (x - S[a][ca]) / (Ee[a][ca] - S[a][ca]);
+        }
+        double v = 0;
+        if (inside) {
+            int nloc = 1;
+            for (int a = 0; a < g.d; a++) nloc *= g.k + 1;
+            for (int li = 0; li < nloc; li++) {
+                int p[MAXD] = {0, 0, 0};
+                double phi = 1;
+                int r = li;
+                for (int a = 0; a < g.d; a++) {
+                    const int l = r % (g.k + 1);
+                    r /= g.k + 1;
+                    p[a] = c[a] * g.k + l;
+                    phi *= lag_value(t[a], l, g.k);
+                }
+                v += phi * coeff[dof_of_lattice(g, p)];
+            }
+        }
+        out[i] = v;
+    }
+}
+
+// v1 - v2 per point
+__global__ void k_diff(long n, const double* a, const double* __restrict__ b, double* out)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = a[i] - b[i];
+}
+
+static void cells_1d(double start, double end, int n, std::vector<double>& s, std::vector<double>& e);
+
+// device values of an FE function at npts device points
+void space_evaluate_dev(jfem_space* s, const double* coeff, long npts, const double* pts_dev, double* out_dev)
+{
+    const Grid& g = s->grid;
+    double* se[2 * MAXD] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    for (int a = 0; a < g.d; a++) {
+        std::vector<double> s1, e1;
+        cells_1d(g.start[a], g.end[a], g.n[a], s1, e1);
+        JFEM_HIP(hipMalloc(&se[a], sizeof(double) * g.n[a]));
+        JFEM_HIP(hipMalloc(&se[MAXD + a], sizeof(double) * g.n[a]));
+        JFEM_HIP(hipMemcpy(se[a], s1.data(), sizeof(double) * g.n[a], hipMemcpyHostToDevice));
+        JFEM_HIP(hipMemcpy(se[MAXD + a], e1.data(), sizeof(double) * g.n[a], hipMemcpyHostToDevice));
+    }
+    if (npts > 0)
+        k_eval_fe<<<grid_for(npts, 256, 16384), 256, 0, s->ctx->stream>>>(to_dev(g), se[0], se[1], se[2], se[3], se[4],
+                                                                         se[5], coeff, npts, pts_dev, out_dev);
+    const hipError_t e = hipGetLastError();
+    const hipError_t e2 = hipStreamSynchronize(s->ctx->stream);
+    for (auto p : se) (void)hipFree(p);
+    JFEM_HIP(e);
+    JFEM_HIP(e2);
+}
+
 // ------------------------------ RHS (sines) --------------------------------
 // TPRightHandSideIntegral VALUE: per cell, ret_v = sum_q (phi_v(x_q) f(x_q)) w_0 .. w_{d-1};
 // rhs[dof] += ret_v.  Cells of one parity colour share no node, so 2^d passes
@@ -1246,6 +1331,58 @@ int jfem_space_node_coords(jfem_space* s, double* out)
     for (auto p : se) (void)hipFree(p);
     JFEM_HIP(e);
     JFEM_HIP(e2);
+    API_END
+}
+
+int jfem_space_evaluate(jfem_space* s, const jfem_vec* coeffs, long npts, const double* points, double* values)
+{
+    API_BEGIN
+    JFEM_CHECK(s && coeffs && (npts == 0 || (points && values)), JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(npts >= 0, JFEM_ERR_ARG, "negative point count");
+    JFEM_CHECK(coeffs->n == s->grid.ndofs(), JFEM_ERR_ARG, "coefficient vector size differs from the space");
+    if (npts == 0) return JFEM_OK;
+    const int d = s->grid.d;
+    double *p = nullptr, *v = nullptr;
+    JFEM_HIP(hipMalloc(&p, sizeof(double) * npts * d));
+    JFEM_HIP(hipMalloc(&v, sizeof(double) * npts));
+    JFEM_HIP(hipMemcpy(p, points, sizeof(double) * npts * d, hipMemcpyHostToDevice));
+    space_evaluate_dev(s, coeffs->d, npts, p, v);
+    const hipError_t e = hipMemcpy(values, v, sizeof(double) * npts, hipMemcpyDeviceToHost);
+    (void)hipFree(p);
+    (void)hipFree(v);
+    JFEM_HIP(e);
+    API_END
+}
+
+int jfem_fe_l2_difference(jfem_space* s1, const jfem_vec* c1, jfem_space* s2, const jfem_vec* c2, long npts,
+                          const double* points, double* result)
+{
+    API_BEGIN
+    JFEM_CHECK(s1 && c1 && s2 && c2 && result && (npts == 0 || points), JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(npts >= 0, JFEM_ERR_ARG, "negative point count");
+    JFEM_CHECK(s1->grid.d == s2->grid.d, JFEM_ERR_ARG, "the two spaces have different dimensions");
+    JFEM_CHECK(c1->n == s1->grid.ndofs() && c2->n == s2->grid.ndofs(), JFEM_ERR_ARG,
+               "coefficient vector size differs from its space");
+    *result = 0;
+    if (npts == 0) return JFEM_OK;
+    const int d = s1->grid.d;
+    double *p = nullptr, *v1 = nullptr, *v2 = nullptr;
+    JFEM_HIP(hipMalloc(&p, sizeof(double) * npts * d));
+    JFEM_HIP(hipMalloc(&v1, sizeof(double) * npts));
+    JFEM_HIP(hipMalloc(&v2, sizeof(double) * npts));
+    JFEM_HIP(hipMemcpy(p, points, sizeof(double) * npts * d, hipMemcpyHostToDevice));
+    space_evaluate_dev(s1, c1->d, npts, p, v1);
+    space_evaluate_dev(s2, c2->d, npts, p, v2);
+    // squaredDifferences = sum_i (f1(p_i) - f2(p_i))^2 (ConvergenceOrderEstimator.java:49-57), as the
+    // dot product of the differences with themselves (the reference sums sequentially; the values
+    // themselves are sums in nondeterministic order there)
+    k_diff<<<grid_for(npts, 256, 16384), 256, 0, s1->ctx->stream>>>(npts, v1, v2, v1);
+    JFEM_HIP(hipGetLastError());
+    const double acc = vec_dot(s1->ctx, npts, v1, v1);
+    (void)hipFree(p);
+    (void)hipFree(v1);
+    (void)hipFree(v2);
+    *result = std::sqrt(acc);
     API_END
 }
 

@@ -237,6 +237,49 @@ def rhs_sines(cells, k, start, end, kw, amp, rule=5):
     return rhs
 
 
+def fe_value(cells, k, start, end, coeffs, points):
+    """ScalarFESpaceFunction.value (M/basic/ScalarFESpaceFunction.java:40-45) by
+    brute force over all shape functions, for small grids: each function is
+    evaluated in the first of its cells (cell order) that contains the point
+    (ScalarShapeFunction.value, M/basic/ScalarShapeFunction.java:26-31;
+    Cell1D.isInCell with 1e-8 slack), as the product of its 1D Lagrange values
+    there; sum over the functions in DOF order."""
+    d = len(cells)
+    n, table = dof_table(cells, k)
+    s1 = [cells1d(start[a], end[a], cells[a])[0] for a in range(d)]
+    e1 = [cells1d(start[a], end[a], cells[a])[1] for a in range(d)]
+    ncell = table.shape[0]
+    # cell multi-index in IntCoordinates order (last axis fastest)
+    cidx = np.zeros((ncell, d), dtype=np.int64)
+    r = np.arange(ncell)
+    for a in range(d - 1, -1, -1):
+        cidx[:, a] = r % cells[a]
+        r //= cells[a]
+    fcells = [[] for _ in range(n)]   # (cell, local index) per function, in cell order
+    for c in range(ncell):
+        for li, f in enumerate(table[c]):
+            fcells[f].append((c, li))
+    pts = np.asarray(points, float).reshape(-1, d)
+    out = np.zeros(len(pts))
+    for i, x in enumerate(pts):
+        v = 0.0
+        for f in range(n):
+            for c, li in fcells[f]:
+                ca = cidx[c]
+                if all(s1[a][ca[a]] - 1e-8 <= x[a] <= e1[a][ca[a]] + 1e-8 for a in range(d)):
+                    phi = 1.0
+                    rr = li
+                    for a in range(d):
+                        la = rr % (k + 1)
+                        rr //= k + 1
+                        s, e = s1[a][ca[a]], e1[a][ca[a]]
+                        phi *= lagrange_value((x[a] - s) / (e - s), la, k)
+                    v += coeffs[f] * phi
+                    break
+        out[i] = v
+    return out
+
+
 def node_coords(cells, k, start, end):
     cells = np.ascontiguousarray(cells, dtype=np.int32)
     d = len(cells)

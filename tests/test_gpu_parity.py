@@ -300,3 +300,45 @@ def test_mg_pcg_fcg_converge(jf, oracle, precision):
     assert np.linalg.norm(rhs - ref @ x) <= 1.01 * tol
     assert np.linalg.norm(x - exact) <= 1e-6 * np.linalg.norm(exact)
     assert it.iterations < 30
+
+
+@pytest.mark.parametrize("cells,k", [([5, 3], 1), ([4, 3], 2), ([3, 2, 2], 2)])
+def test_fe_evaluate_and_l2_difference(jf, oracle, cells, k):
+    """ScalarFESpaceFunction.value and ConvergenceOrderEstimator.normL2Difference
+    on the GPU: against the oracle's brute-force sum over all shape functions
+    for random coefficients (the reference sums in parallel-stream order, so
+    1e-13 relative), exact on Qk polynomials (interpolation reproduces them),
+    0 outside the domain; the L2 difference of a coarse and a fine function
+    against the same sum of squares of oracle values."""
+    d = len(cells)
+    s, start, end = _space(jf, cells, k)
+    rng = np.random.default_rng(21)
+    n = s.ndofs()
+    coeffs = rng.standard_normal(n)
+    pts = np.column_stack([rng.uniform(start[a], end[a], 200) for a in range(d)])
+    xyz = s.node_coords()
+    pts = np.vstack([pts, xyz[::7], [[end[a] + 1.0 for a in range(d)]]])   # nodes, one outside point
+    f = jf.ScalarFESpaceFunction(s, jf.DenseVector(coeffs))
+    got = f.value(pts)
+    exp = oracle.fe_value(cells, k, start, end, coeffs, pts)
+    assert np.linalg.norm(got - exp) <= 1e-13 * np.linalg.norm(exp)
+    assert got[-1] == 0.0
+
+    def u(p):
+        v = 1.0 - 0.25 * p[:, 0] + 3.0 * p[:, 1]
+        mono = np.ones(len(p))
+        for a in range(d):
+            mono = mono * p[:, a] ** k
+        return v + mono
+
+    fu = jf.ScalarFESpaceFunction(s, jf.DenseVector(u(xyz)))
+    inside = pts[:-1]
+    assert np.abs(fu.value(inside) - u(inside)).max() <= 1e-12 * np.abs(u(inside)).max()
+
+    fine_cells = [2 * c for c in cells]
+    sf, _, _ = _space(jf, fine_cells, k, start, end)
+    cf = rng.standard_normal(sf.ndofs())
+    ff = jf.ScalarFESpaceFunction(sf, jf.DenseVector(cf))
+    got_l2 = jf.normL2Difference(f, ff, pts)
+    exp_l2 = np.sqrt(np.sum((exp - oracle.fe_value(fine_cells, k, start, end, cf, pts)) ** 2))
+    assert abs(got_l2 - exp_l2) <= 1e-13 * exp_l2

@@ -394,3 +394,30 @@ def test_mg_vcycle_contracts(oracle, sm):
     for _ in range(5):
         x = G.vcycle(x, b)
     assert np.linalg.norm(b - A @ x) < 1e-3 * r0
+
+
+@pytest.mark.parametrize("cells,k", [([3, 2], 1), ([3, 2], 2), ([2, 2, 2], 1), ([2, 1, 2], 2)])
+def test_fe_value_reproduces_qk_polynomials(oracle, cells, k):
+    """Qk interpolation is exact on Qk: with the nodal values of a polynomial of
+    degree <= k per axis as coefficients, the FE function equals the polynomial
+    everywhere in the domain (on faces and nodes too), and is 0 outside it."""
+    O = oracle
+    d = len(cells)
+    start, end = [0.0, -1.0, 0.25][:d], [1.0, 1.0, 2.0][:d]
+    xyz = O.node_coords(cells, k, start, end)
+
+    def u(p):
+        v = 1.0 + 0.5 * p[:, 0] - 2.0 * p[:, 1]
+        mono = np.ones(len(p))
+        for a in range(d):
+            mono = mono * p[:, a] ** k
+        return v + 0.75 * mono
+
+    rng = np.random.default_rng(3)
+    pts = np.column_stack([rng.uniform(start[a], end[a], 25) for a in range(d)])
+    # a node and a face point
+    pts = np.vstack([pts, xyz[len(xyz) // 2], 0.5 * (xyz[1] + xyz[2])])
+    got = O.fe_value(cells, k, start, end, u(xyz), pts)
+    assert np.abs(got - u(pts)).max() <= 1e-12 * np.abs(u(pts)).max()
+    outside = np.array([[end[a] + 0.5 for a in range(d)]])
+    assert O.fe_value(cells, k, start, end, u(xyz), outside)[0] == 0.0
