@@ -187,10 +187,11 @@ __global__ __launch_bounds__(256) void k_element_tables(ElemParams P, const Tab1
         const double wc = P.wconst[ii];
         int nq = 1;
         for (int a = 0; a < d; a++) nq *= Q;
+        // quadrature points of the cell, a constant with fixed d and Q (1: the loop is not unrolled)
+        constexpr int QN = QQ && DD ? (DD == 3 ? QQ * QQ * QQ : DD == 2 ? QQ * QQ : QQ) : 1;
         double ret = 0;
-#pragma unroll
-        for (int q = 0; q < (QQ && DD ? (DD == 3 ? QQ * QQ * QQ : DD == 2 ? QQ * QQ : QQ) : 1 << 30); q++) {
-            if (!(QQ && DD) && q >= nq) break;
+#pragma unroll QN
+        for (int q = 0; q < (QN > 1 ? QN : nq); q++) {
             int c[MAXD] = {0, 0, 0};
             int qq = q;
 #pragma unroll
@@ -897,10 +898,10 @@ __global__ __launch_bounds__(RHS_BS) void k_rhs_color(RhsParams P, const Tab1D* 
         for (int a = 0; a < MAXD; a++)
             if (a < d) { lv[a] = rr % (k + 1); rr /= k + 1; p[a] = c[a] * k + lv[a]; }
         const double* fc = fq + ci * nq;
+        constexpr int QN = QQ && DD ? (DD == 3 ? QQ * QQ * QQ : DD == 2 ? QQ * QQ : QQ) : 1;
         double ret = 0;
-#pragma unroll
-        for (int q = 0; q < (QQ && DD ? (DD == 3 ? QQ * QQ * QQ : DD == 2 ? QQ * QQ : QQ) : 1 << 30); q++) {
-            if (!(QQ && DD) && q >= nq) break;
+#pragma unroll QN
+        for (int q = 0; q < (QN > 1 ? QN : nq); q++) {
             int qa[MAXD] = {0, 0, 0};
             int qq = q;
 #pragma unroll

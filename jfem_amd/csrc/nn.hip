@@ -406,11 +406,13 @@ __global__ void k_in_split(long np, int cin, const float* __restrict__ in, unsig
     }
 }
 
-// first layer on MFMA (2D/3D, cin <= 8) in split-BF16: one 16-wide K step per
-// tap, lanes 0-31 feed the input's high parts and lanes 32-63 its low parts,
-// so sum_c (hi + lo)[c] (w_hi + w_lo)[c] (two MFMAs per tap, A = w_hi then
-// w_lo in both lane halves) keeps the layer at close to FP32 accuracy; the
-// input tile is staged in LDS
+// first layer on MFMA (2D/3D, cin <= 8) in split-BF16, two taps per 16-wide K
+// step: lanes 0-31 feed tap t0 = 2p and lanes 32-63 tap t1 = 2p + 1 of pair p
+// (zero weights past the last tap).  Input x = hi + lo and weights
+// w = w_hi + w_lo in BF16 parts; three MFMAs per pair accumulate
+// w_hi*hi + w_hi*lo + w_lo*hi, i.e. the product up to the w_lo*lo term
+// (below 2^-16 of it), close to FP32 accuracy.  The input tile is staged in
+// LDS, chunk 0 the high parts, chunk 1 the low parts.
 template <int D>
 __global__ __launch_bounds__(MidTile<D>::NW * 64) void k_first_lds(Lat L, int cin, const unsigned short* __restrict__ inb,
                                                                   const float* __restrict__ W0,
@@ -418,18 +420,19 @@ __global__ __launch_bounds__(MidTile<D>::NW * 64) void k_first_lds(Lat L, int ci
                                                                   unsigned short* __restrict__ out)
 {
     using T = MidTile<D>;
-    constexpr int TAPS = D == 3 ? 27 : 9, CS = T::template cs<2>(), NR = T::NR;
+    constexpr int TAPS = D == 3 ? 27 : 9, NP = (TAPS + 1) / 2, CS = T::template cs<2>(), NR = T::NR;
     extern __shared__ bf16x8 sm[];
-    bf16x8* wf = sm;                     // TAPS x (32 w_hi, 32 w_lo) weight fragments
-    bf16x8* tile = sm + TAPS * 64;       // chunk-major halo tile: chunk 0 high parts, chunk 1 low parts
+    bf16x8* wf = sm;                     // NP x {w_hi, w_lo} x 64 lanes: A fragments of the tap pairs
+    bf16x8* tile = sm + NP * 2 * 64;     // chunk-major halo tile
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int j = lane & 31, kh = lane >> 5;
-    for (int f = threadIdx.x; f < TAPS * 64; f += T::NW * 64) {
-        const int t = f / 64, part = (f / 32) & 1, co = f & 31;
+    for (int f = threadIdx.x; f < NP * 2 * 64; f += T::NW * 64) {
+        const int p = f / 128, part = (f / 64) & 1, l = f & 63;
+        const int t = 2 * p + (l >> 5), co = l & 31;
         bf16x8 v;
 #pragma unroll
         for (int e = 0; e < 8; e++) {
-            const float w = e < cin ? W0[(t * NN_H + co) * cin + e] : 0.f;
+            const float w = e < cin && t < TAPS ? W0[(t * NN_H + co) * cin + e] : 0.f;
             const unsigned short h = f2bf(w);
             v[e] = (short)(part == 0 ? h : f2bf(w - bf2f(h)));
         }
@@ -444,34 +447,44 @@ __global__ __launch_bounds__(MidTile<D>::NW * 64) void k_first_lds(Lat L, int ci
         __syncthreads();
         const bf16x8* bl[NR];
 #pragma unroll
-        for (int u = 0; u < NR; u++) bl[u] = tile + kh * CS + T::row_base(wave * NR + u) + j;
+        for (int u = 0; u < NR; u++) bl[u] = tile + T::row_base(wave * NR + u) + j;
         f32x16 acc[NR];
 #pragma unroll
         for (int u = 0; u < NR; u++)
 #pragma unroll
             for (int r = 0; r < 16; r++) acc[u][r] = 0.f;
-        bf16x8 ahi = wf[j], alo = wf[32 + j], bc[NR];
+        // halo slot offset of this lane's tap in pair p (the last pair's second tap: any slot, zero weights)
+        auto toff = [&](int p) {
+            const int t0 = 2 * p, t1 = 2 * p + 1 < TAPS ? 2 * p + 1 : 2 * p;
+            return kh ? T::template tap<false>(t1) : T::template tap<false>(t0);
+        };
+        bf16x8 ahi = wf[lane], alo = wf[64 + lane], bh[NR], bo[NR];
+        {
+            const int o = toff(0);
 #pragma unroll
-        for (int u = 0; u < NR; u++) bc[u] = bl[u][T::template tap<false>(0)];
+            for (int u = 0; u < NR; u++) { bh[u] = bl[u][o]; bo[u] = bl[u][o + CS]; }
+        }
 #pragma unroll
-        for (int t = 0; t < TAPS; t++) {
-            bf16x8 ahn, aln, bn[NR];
-            if (t + 1 < TAPS) {
-                ahn = wf[(t + 1) * 64 + j];
-                aln = wf[(t + 1) * 64 + 32 + j];
+        for (int p = 0; p < NP; p++) {
+            bf16x8 ahn, aln, bhn[NR], bon[NR];
+            if (p + 1 < NP) {
+                ahn = wf[(p + 1) * 128 + lane];
+                aln = wf[(p + 1) * 128 + 64 + lane];
+                const int o = toff(p + 1);
 #pragma unroll
-                for (int u = 0; u < NR; u++) bn[u] = bl[u][T::template tap<false>(t + 1)];
+                for (int u = 0; u < NR; u++) { bhn[u] = bl[u][o]; bon[u] = bl[u][o + CS]; }
             }
 #pragma unroll
             for (int u = 0; u < NR; u++) {
-                acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bc[u], acc[u], 0, 0, 0);
-                acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bc[u], acc[u], 0, 0, 0);
+                acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bh[u], acc[u], 0, 0, 0);
+                acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bo[u], acc[u], 0, 0, 0);
+                acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bh[u], acc[u], 0, 0, 0);
             }
-            if (t + 1 < TAPS) {
+            if (p + 1 < NP) {
                 ahi = ahn;
                 alo = aln;
 #pragma unroll
-                for (int u = 0; u < NR; u++) bc[u] = bn[u];
+                for (int u = 0; u < NR; u++) { bh[u] = bhn[u]; bo[u] = bon[u]; }
             }
         }
 #pragma unroll
@@ -930,7 +943,7 @@ void nn_forward(jfem_nn* nn, const Lat& L, bool inb_ready)
             using T = MidTile<3>;
             const long nt = (long)((L.n[0] + 31) / 32) * ((L.n[1] + T::TY - 1) / T::TY) * ((L.n[2] + T::TZ - 1) / T::TZ);
             const int grid = (int)std::min<long>(nt, 4096);
-            k_first_lds<3><<<grid, T::NW * 64, sizeof(bf16x8) * (27 * 64 + 2 * T::cs<2>()), st>>>(L, nn->cin, nn->inb, W0,
+            k_first_lds<3><<<grid, T::NW * 64, sizeof(bf16x8) * (14 * 128 + 2 * T::cs<2>()), st>>>(L, nn->cin, nn->inb, W0,
                                                                                                  b0, nn->act0);
             launch_mid<0>(st, L, nn->taps, nn->act0, nn->wh1, nn->w + o[3], nullptr, nn->act1);
             launch_mid<0>(st, L, nn->taps, nn->act1, nn->wh2, nn->w + o[5], nullptr, nn->act2);
@@ -940,7 +953,7 @@ void nn_forward(jfem_nn* nn, const Lat& L, bool inb_ready)
             using T = MidTile<2>;
             const long nt = (long)((L.n[0] + 31) / 32) * ((L.n[1] + T::TY - 1) / T::TY);
             const int grid = (int)std::min<long>(nt, 4096);
-            k_first_lds<2><<<grid, T::NW * 64, sizeof(bf16x8) * (9 * 64 + 2 * T::cs<2>()), st>>>(L, nn->cin, nn->inb, W0,
+            k_first_lds<2><<<grid, T::NW * 64, sizeof(bf16x8) * (5 * 128 + 2 * T::cs<2>()), st>>>(L, nn->cin, nn->inb, W0,
                                                                                                 b0, nn->act0);
             launch_mid<0>(st, L, nn->taps, nn->act0, nn->wh1, nn->w + o[3], nullptr, nn->act1);
             launch_mid<0>(st, L, nn->taps, nn->act1, nn->wh2, nn->w + o[5], nullptr, nn->act2);

@@ -30,7 +30,10 @@ namespace jfem {
 #endif
 constexpr int ST_BS = JFEM_ST_BS;     // threads per workgroup
 constexpr int ST_CAP = JFEM_ST_CAP;   // entries per row block (FP64 values)
-constexpr int ST_CAP32 = 2 * ST_CAP;  // entries per row block (FP32 values): same bytes per block
+#ifndef JFEM_ST_CAP32
+#define JFEM_ST_CAP32 (2 * JFEM_ST_CAP)
+#endif
+constexpr int ST_CAP32 = JFEM_ST_CAP32;   // entries per row block (FP32 values): same bytes per block
 static_assert(ST_CAP % ST_BS == 0, "block capacity must be a multiple of the workgroup size");
 
 // entries per staging load: four (one 16-byte column load; values in one or two 16-byte loads)
