@@ -331,8 +331,8 @@ def main():
                     help="assemble the finest matrix and time only the SpMV and the copy calibration (for PMC runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the solve path is single-GPU in this build "
-              "(multi-GPU slab decomposition is a design note only); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the solve path runs on a single GPU "
+              "(a multi-GPU slab decomposition is designed but not implemented); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import jfem_amd as jf

@@ -2,7 +2,7 @@
 the V-cycle lowers the loss, the trained network smooths (the V-cycle converges
 without falling back) where the untrained one does not, the fallback to
 Chebyshev triggers on a network that does not smooth, and the FP32 multigrid
-with the learned smoother inside FP64 flexible CG meets the FP64 residual judge."""
+with the learned smoother inside FP64 flexible CG meets the FP64 residual criterion."""
 import numpy as np
 import pytest
 
