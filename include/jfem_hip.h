@@ -179,6 +179,10 @@ typedef struct {
     int cycles;        /* V-cycles per application (reference: 3) */
     int fmg;           /* 1: reference mvMul (copy BC, full-V correction, cycles V) */
     int precision;     /* JFEM_FP64 or JFEM_FP32 */
+    int galerkin;      /* 1: coarse systems A_{l-1} = P^T A_l P from the finest assembled one
+                        * (AMGFromMatrix, M/multigrid/MGPreconditionerInterface.java:387-410)
+                        * instead of re-assembly per level; the transfers keep their zero
+                        * boundary conditions */
 } jfem_mg_opts;
 
 /* levels: nlev spaces, coarsest first, each with twice the cells of the

@@ -465,7 +465,7 @@ class MGPreconditioner:
     M/multigrid/MGPreconditionerInterface.java:352-379).  Levels: coarsest first."""
 
     def __init__(self, spaces: Sequence[ContinuousTPFESpace], cellIntegrals: Sequence[TPCellIntegral], smoother,
-                 cycles: int = 3, fmg: bool = True, precision: int = FP64, cell_coeff=None):
+                 cycles: int = 3, fmg: bool = True, precision: int = FP64, cell_coeff=None, galerkin: bool = False):
         self.ctx = spaces[0].ctx
         self.spaces = list(spaces)
         opts = N.MgOpts()
@@ -479,6 +479,7 @@ class MGPreconditioner:
         opts.cycles = cycles
         opts.fmg = 1 if fmg else 0
         opts.precision = precision
+        opts.galerkin = 1 if galerkin else 0   # coarse systems P^T A P (AMGFromMatrix) instead of re-assembly
         kinds = i32([ci.kind for ci in cellIntegrals])
         w = f64([ci.weight for ci in cellIntegrals])
         arr = (C.c_void_p * len(spaces))(*[s.h for s in spaces])

@@ -507,9 +507,11 @@ static void smooth(jfem_mg* G, MgLevel& L, const T* b, T*& x, T*& xalt, bool x_z
 template <typename T>
 static void restrict_to(jfem_mg* G, int lf, const T* xf, T* xc)
 {
+    // with Galerkin coarse systems the boundary conditions are built into them and the
+    // transfers apply none (AMGFromMatrix's applyZeroBoundaryConditions is empty)
     const MgLevel& F = G->lev[lf];
     const MgLevel& C = G->lev[lf - 1];
-    tp_transfer<T>(G->ctx->stream, F, C, true, xf, xc, C.bnd, 0);
+    tp_transfer<T>(G->ctx->stream, F, C, true, xf, xc, G->opt.galerkin ? nullptr : C.bnd, 0);
 }
 
 template <typename T>
@@ -517,7 +519,7 @@ static void prolong_to(jfem_mg* G, int lf, const T* xc, T* xf, bool add, bool ze
 {
     const MgLevel& F = G->lev[lf];
     const MgLevel& C = G->lev[lf - 1];
-    tp_transfer<T>(G->ctx->stream, F, C, false, xc, xf, zero_bc ? F.bnd : nullptr, add ? 1 : 0);
+    tp_transfer<T>(G->ctx->stream, F, C, false, xc, xf, zero_bc && !G->opt.galerkin ? F.bnd : nullptr, add ? 1 : 0);
 }
 
 template <typename T>
@@ -687,6 +689,66 @@ static void free_mg(jfem_mg* G)
     delete G;
 }
 
+// ---- Galerkin coarse systems by structured probing ----
+// A_c = P^T A P couples coarse nodes I, J only if they share a coarse cell, so
+// |I_a - J_a| <= k on every lattice axis.  Colouring the coarse lattice by
+// (I_a mod (2k+1)) per axis gives (2k+1)^d colours such that a row's stencil
+// holds one node of each: with v_c the indicator of colour c,
+// (P^T A P v_c)[I] = A_c[I][J] for the node J of colour c next to I.
+__device__ inline int probe_color(const GridDev& g, long dof)
+{
+    int p[MAXD] = {0, 0, 0};
+    lattice_of_dof(g, dof, p);
+    const int m = 2 * g.k + 1;
+    int c = 0;
+    for (int a = g.d - 1; a >= 0; a--) c = c * m + p[a] % m;
+    return c;
+}
+
+__global__ void k_color_indicator(GridDev g, long n, int color, double* __restrict__ v)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        v[i] = probe_color(g, i) == color ? 1.0 : 0.0;
+}
+
+// vals[p] of coarse entry (I, J) = Y[colour(J)][I]; entries outside every
+// colour's reach stay as they are (none with a structural pattern)
+__global__ void k_galerkin_fill(GridDev g, long n, const int* __restrict__ rowptr, const int* __restrict__ cols,
+                                const double* __restrict__ Y, double* __restrict__ vals)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        for (int p = rowptr[i]; p < rowptr[i + 1]; p++) vals[p] = Y[(long)probe_color(g, cols[p]) * n + i];
+}
+
+// A_{lf-1} = P^T A_lf P (FP64, plain P and P^T: no boundary conditions), into
+// the coarse level's existing pattern
+static void galerkin_level(jfem_mg* G, int lf)
+{
+    MgLevel& F = G->lev[lf];
+    MgLevel& C = G->lev[lf - 1];
+    jfem_ctx* c = G->ctx;
+    hipStream_t st = c->stream;
+    const long nf = F.A->rows, nc = C.A->rows;
+    const GridDev gc = to_dev(C.grid);
+    int ncol = 1;
+    for (int a = 0; a < C.grid.d; a++) ncol *= 2 * C.grid.k + 1;
+    double *Y = nullptr, *v = nullptr, *w = nullptr, *z = nullptr;
+    JFEM_HIP(hipMalloc(&Y, sizeof(double) * nc * ncol));
+    JFEM_HIP(hipMalloc(&v, sizeof(double) * nc));
+    JFEM_HIP(hipMalloc(&w, sizeof(double) * nf));
+    JFEM_HIP(hipMalloc(&z, sizeof(double) * nf));
+    for (int col = 0; col < ncol; col++) {
+        k_color_indicator<<<grid_for(nc, 256, 16384), 256, 0, st>>>(gc, nc, col, v);
+        tp_transfer<double>(st, F, C, false, v, w, nullptr, 0);   // w = P v
+        spmv(F.A, w, z);                                          // z = A w
+        tp_transfer<double>(st, F, C, true, z, Y + (long)col * nc, nullptr, 0);   // P^T z
+    }
+    k_galerkin_fill<<<grid_for(nc, 256, 16384), 256, 0, st>>>(gc, nc, C.A->rowptr, C.A->colidx, Y, C.A->vals);
+    JFEM_HIP(hipGetLastError());
+    JFEM_HIP(hipStreamSynchronize(st));
+    for (double* p : {Y, v, w, z}) (void)hipFree(p);
+}
+
 static jfem_mg* create_mg(jfem_space* const* spaces, int nlev, int nint, const int* kinds, const double* weights,
                           const double* coeff, int rule, const jfem_mg_opts* opts)
 {
@@ -742,7 +804,6 @@ static jfem_mg* create_mg(jfem_space* const* spaces, int nlev, int nint, const i
             JFEM_HIP(hipMalloc(&L.bnd, n));
             space_boundary_mask(spaces[l], L.bnd);
             apply_dirichlet_mask(L.A, nullptr, L.bnd, nullptr);
-            if (opts->precision == JFEM_FP32) mat_make_f32(L.A);
             JFEM_HIP(hipMalloc(&L.dinv, sizeof(double) * n));
             JFEM_HIP(hipMalloc(&L.dinv32, sizeof(float) * n));
             for (void** p : {&L.x, &L.b, &L.r, &L.t, &L.dv}) {
@@ -750,6 +811,14 @@ static jfem_mg* create_mg(jfem_space* const* spaces, int nlev, int nint, const i
                 JFEM_HIP(hipMemsetAsync(*p, 0, sizeof(double) * n, c->stream));
             }
             if (l > 0) build_transfer(L, G->lev[l - 1]);
+        }
+        // AMGFromMatrix: coarse systems from the finest one, top down
+        if (opts->galerkin)
+            for (int l = nlev - 1; l >= 1; l--) galerkin_level(G, l);
+        for (int l = 0; l < nlev; l++) {
+            MgLevel& L = G->lev[l];
+            const long n = L.A->rows;
+            if (opts->precision == JFEM_FP32) mat_make_f32(L.A);
             mat_diag(L.A, (double*)L.t);
             k_dinv_from_diag<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(n, (double*)L.t, L.dinv, L.dinv32);
             JFEM_HIP(hipGetLastError());

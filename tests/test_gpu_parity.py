@@ -342,3 +342,48 @@ def test_fe_evaluate_and_l2_difference(jf, oracle, cells, k):
     got_l2 = jf.normL2Difference(f, ff, pts)
     exp_l2 = np.sqrt(np.sum((exp - oracle.fe_value(fine_cells, k, start, end, cf, pts)) ** 2))
     assert abs(got_l2 - exp_l2) <= 1e-13 * exp_l2
+
+
+@pytest.mark.parametrize("base,k", [([2, 3], 1), ([2, 2], 2), ([2, 2, 2], 2)])
+def test_mg_galerkin_coarse_systems(jf, oracle, base, k):
+    """AMGFromMatrix (MGPreconditionerInterface.java:387-410): every coarser
+    system is P^T A P of the one above (the finest assembled with its Dirichlet
+    rows), with P the plain prolongation matrix; checked against scipy's
+    product with the oracle's P (sums in another order: 1e-13 relative), and
+    with a per-cell coefficient.  The Galerkin hierarchy then preconditions
+    flexible CG to 1e-8."""
+    import scipy.sparse as sp
+    d = len(base)
+    start, end = [0.0] * d, [1.0] * d
+    nlev = 3
+    spaces = []
+    for l in range(nlev):
+        s, _, _ = _space(jf, [c << l for c in base], k, start, end)
+        spaces.append(s)
+    rng = np.random.default_rng(8)
+    ncell = int(np.prod([c << (nlev - 1) for c in base]))
+    coeff = np.exp(rng.standard_normal(ncell))
+    G = jf.MGPreconditioner(spaces, [jf.TPCellIntegral("GradGrad")], jf.ChebyshevSmoother(4, 2), cycles=1,
+                            fmg=False, cell_coeff=coeff, galerkin=True)
+
+    def csr(l):
+        rp, cols, vals = G.getSystem(l).to_csr()
+        n = len(rp) - 1
+        return sp.csr_matrix((vals, cols, rp), shape=(n, n))
+
+    for l in range(nlev - 1):
+        P = oracle.prolongation([c << l for c in base], k, start, end).to_scipy()
+        exp = (P.T @ csr(l + 1) @ P).tocsr()
+        got = csr(l)
+        assert sp.linalg.norm(got - exp) <= 1e-13 * sp.linalg.norm(exp)
+    # solve with the Galerkin hierarchy
+    Af = G.getSystem(nlev - 1)
+    n = Af.to_csr()[0].shape[0] - 1
+    b = rng.standard_normal(n)
+    b[spaces[-1].boundary_nodes()] = 0
+    it = jf.IterativeSolver()
+    x = it.solveFCG(Af, G, jf.DenseVector(b), 1e-8 * np.linalg.norm(b)).numpy()
+    rp, cols, vals = Af.to_csr()
+    A = sp.csr_matrix((vals, cols, rp), shape=(n, n))
+    assert np.linalg.norm(b - A @ x) <= 1.01e-8 * np.linalg.norm(b)
+    assert it.iterations < 40
