@@ -144,7 +144,7 @@ def spmv_only(jf, cells):
                       "copy_gbs": gbs, "copy_ms": cms, "n": n, "nnz": nnz}), flush=True)
 
 
-def c4_hierarchy(jf, field, precision, cells=1024, base=16):
+def c4_hierarchy(jf, field, precision, cells=1024, base=16, galerkin=False):
     nlev = 1
     c = cells
     while c > base:
@@ -157,7 +157,7 @@ def c4_hierarchy(jf, field, precision, cells=1024, base=16):
         sp.assembleFunctions(1)
         spaces.append(sp)
     G = jf.MGPreconditioner(spaces, [jf.TPCellIntegral("GradGrad")], jf.ChebyshevSmoother(4, 1), cycles=1, fmg=False,
-                            precision=precision, cell_coeff=np.exp(field))
+                            precision=precision, cell_coeff=np.exp(field), galerkin=galerkin)
     return G, spaces
 
 
@@ -178,7 +178,7 @@ def bench_c4(jf, args):
     losses = []
     for f in range(args.train_fields):
         field = gaussian_random_field(4000 + f, (cells, cells), 0.1, 1e3)
-        G, spaces = c4_hierarchy(jf, field, jf.FP64)
+        G, spaces = c4_hierarchy(jf, field, jf.FP64, galerkin=args.galerkin)
         G.set_learned(net, field, runs=1, fallback_ratio=0.5)
         h = G.train(rhs_per_field=max(1, args.pairs // 8), cycles_per_rhs=8, lr=3e-4, seed=f + 1, mode=5)
         losses.append(float(np.mean(h[-8:])))
@@ -189,7 +189,7 @@ def bench_c4(jf, args):
     kw, amp = sinusoid_terms(4, 8, 2)
     for f in range(args.eval_fields):
         field = gaussian_random_field(5000 + f, (cells, cells), 0.1, 1e3)
-        G64, spaces = c4_hierarchy(jf, field, jf.FP64)
+        G64, spaces = c4_hierarchy(jf, field, jf.FP64, galerkin=args.galerkin)
         fine = spaces[-1]
         fine.initializeRhs()
         fine.writeCellIntegralsToRhs([jf.TPRightHandSideIntegral(jf.SinusoidSource(kw, amp))], fine.getRhs())
@@ -208,7 +208,7 @@ def bench_c4(jf, args):
         res["vcyc_learned"].append((info.seconds, info.iterations))
         res["fallbacks"] += int(G64.learned_status()[1])
         del G64
-        G32, _ = c4_hierarchy(jf, field, jf.FP32)
+        G32, _ = c4_hierarchy(jf, field, jf.FP32, galerkin=args.galerkin)
         G32.set_learned(net, field, runs=1, fallback_ratio=0.5)
         it2 = jf.IterativeSolver()
         x32 = it2.solveFCG(G32.getFinestSystem(), G32, b, tol)
@@ -225,7 +225,8 @@ def bench_c4(jf, args):
         "value": value, "unit": "s", "n_gpus": 1, "steps": args.eval_fields, "warmup": 0,
         "ms_per_step": value * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": "f64 outer / f32 multigrid / bf16 network", "data": "synthetic",
-        "config": {"workload": "2D -div(a grad u)=f, Q1 1024^2 cells, a=exp(GRF) contrast 1e3",
+        "config": {"workload": "2D -div(a grad u)=f, Q1 1024^2 cells, a=exp(GRF) contrast 1e3"
+                               + ("; Galerkin coarse systems" if args.galerkin else ""),
                    "train_fields": args.train_fields, "pairs_per_field": max(1, args.pairs // 8) * 8,
                    "eval_fields": args.eval_fields},
         "fp64_mgpcg_s": med("pcg64", 0), "fp64_mgpcg_iterations": med("pcg64", 1),
@@ -360,6 +361,7 @@ def main():
     ap.add_argument("--train-fields", type=int, default=16)
     ap.add_argument("--pairs", type=int, default=64, help="(residual, error) pairs per training field")
     ap.add_argument("--eval-fields", type=int, default=8)
+    ap.add_argument("--galerkin", action="store_true", help="C4: Galerkin coarse systems on every hierarchy")
     ap.add_argument("--spmv-only", action="store_true",
                     help="assemble the finest matrix and time only the SpMV and the copy calibration (for PMC runs)")
     args = ap.parse_args()
