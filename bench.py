@@ -240,6 +240,132 @@ def bench_c4(jf, args):
     print(json.dumps(line), flush=True)
 
 
+def c5_velocity_mg(jf, cells, nu, precision, smoother, base=4):
+    """scalar Q2 hierarchy base^2 ... cells^2 on the unit square with GRAD_GRAD weight nu"""
+    spaces = []
+    c = base
+    while c <= cells:
+        sp = jf.ContinuousTPFESpace([0.0, 0.0], [1.0, 1.0], [c, c])
+        sp.assembleCells()
+        sp.assembleFunctions(2)
+        spaces.append(sp)
+        c *= 2
+    G = jf.MGPreconditioner(spaces, [jf.TPCellIntegral(nu, "GradGrad")], smoother, cycles=1, fmg=False,
+                            precision=precision)
+    return G, spaces
+
+
+def c5_system(jf, cells, nu):
+    """Taylor-Hood Q2-Q1 on cells^2: lid-driven cavity (u = (1, 0) on y = 1 between the
+    corners, 0 on the other walls) plus a seeded sinusoidal body force, first pressure pinned"""
+    from jfem_amd.workloads import sinusoid_terms
+    s = jf.TaylorHoodSpace([0.0, 0.0], [1.0, 1.0], [cells, cells])
+    s.assembleCells()
+    s.assembleFunctions(1)
+    ints = [jf.MixedCellIntegral.fromVelocityIntegral(jf.TPVectorCellIntegral(nu, "GradGrad")),
+            jf.MixedTPCellIntegral(-1.0, "DivValue")]
+    jf.context().sync()
+    t0 = time.perf_counter()
+    A = s.writeCellIntegralsToMatrix(ints)
+    jf.context().sync()
+    t_asm = time.perf_counter() - t0
+    comps = []
+    for c in range(2):
+        kw, amp = sinusoid_terms(60 + c, 8, 2)
+        comps.append(jf.TrigSource(kw, amp))
+    rhs = jf.DenseVector(s.ndofs())
+    s.writeCellIntegralsToRhs([jf.MixedRightHandSideIntegral.fromVelocityIntegral(
+        jf.TPVectorRightHandSideIntegral(jf.VectorTrigSource(comps), "Value"))], rhs)
+
+    def lid(x):
+        return (1.0 if abs(x[1] - 1.0) < 1e-12 and 0.0 < x[0] < 1.0 else 0.0, 0.0)
+
+    s.writeVelocityBoundaryValuesTo(lid, A, rhs)
+    s.overWriteValue(s.getVelocitySize(), 0.0, A, rhs)
+    return s, A, rhs, t_asm
+
+
+def bench_c5(jf, args):
+    """2D Stokes, Taylor-Hood Q2-Q1 on 256^2 cells (526,338 velocity + 66,049 pressure
+    unknowns), viscosity 1, lid-driven cavity plus a seeded sinusoidal body force, the
+    first pressure pinned to 0.  FP64 flexible GMRES(50) to ||r|| <= 1e-8 ||b|| with the
+    block lower-triangular preconditioner: velocity block = one FP32 V(1,1) cycle per
+    component (scalar Q2 hierarchy 4^2 ... 256^2, Chebyshev-Jacobi degree 3), with the
+    learned smoother on the finest level (network trained on the 128^2 hierarchy) or with
+    Chebyshev there too; Schur complement = lumped pressure mass / viscosity.  The velocity
+    error is measured against an FP64 reference solve (FP64 multigrid, FGMRES to 1e-12)."""
+    cells, nu, tol = 256, 1.0, 1e-8
+    s, A, rhs, t_asm = c5_system(jf, cells, nu)
+    n, _, nnz = A.info()
+    log(f"C5 Taylor-Hood {cells}^2: n={n} nnz={nnz}, assembly {t_asm * 1e3:.1f} ms")
+    cheb = jf.ChebyshevSmoother(degree=3, runs=1, ratio=20.0)
+    nv2 = s.getVelocitySize()
+
+    def solve(G, rtol, reps):
+        prec = jf.StokesBlockPreconditioner(s, G, nu)
+        out = []
+        for _ in range(reps):
+            x = jf.DenseVector(n)
+            info, hist = jf.solveStokesFGMRES(A, prec, rhs, x, rtol, restart=50, maxit=2000)
+            out.append((info.seconds, info.iterations, info.resnorm))
+        return x, out
+
+    G64, _ = c5_velocity_mg(jf, cells, nu, jf.FP64, cheb)
+    xref, r = solve(G64, 1e-12, 1)
+    log(f"FP64 reference solve: {r[0][1]} its, {r[0][0]:.3f}s")
+    xref_u = xref.numpy()[:nv2]
+    fp64 = solve(G64, tol, args.warmup + args.steps)[1][args.warmup:]
+    log(f"FP64 multigrid FGMRES to 1e-8: {fp64[-1][1]} its, {np.median([v[0] for v in fp64]):.3f}s")
+    del G64
+    G32, _ = c5_velocity_mg(jf, cells, nu, jf.FP32, cheb)
+    x32, fp32 = solve(G32, tol, args.warmup + args.steps)
+    fp32 = fp32[args.warmup:]
+    err32 = float(np.linalg.norm(x32.numpy()[:nv2] - xref_u) / np.linalg.norm(xref_u))
+    log(f"FP32 Chebyshev multigrid FGMRES: {fp32[-1][1]} its, {np.median([v[0] for v in fp32]):.3f}s")
+    line_extra = {}
+    value = float(np.median([v[0] for v in fp32]))
+    if not args.no_learned:
+        t0 = time.perf_counter()
+        Gt, _ = c5_velocity_mg(jf, 128, nu, jf.FP64, cheb)
+        net = jf.SmootherNet(2, 4, seed=7)
+        Gt.set_learned(net, np.zeros(128 * 128), runs=1, fallback_ratio=0.5)
+        h = Gt.train(rhs_per_field=400, cycles_per_rhs=8, lr=3e-4, seed=1, mode=5)
+        jf.context().sync()
+        t_train = time.perf_counter() - t0
+        del Gt
+        log(f"learned smoother trained on 128^2 in {t_train:.1f}s, two-grid energy ratio "
+            f"{np.mean(h[:8]):.3f} -> {np.mean(h[-8:]):.4f}")
+        G32.set_learned(net, np.zeros(cells * cells), runs=1, fallback_ratio=0.5)
+        res = []
+        for _ in range(args.warmup + args.steps):
+            G32.set_learned_active(True)
+            xl, r = solve(G32, tol, 1)
+            res.append(r[0])
+        res = res[args.warmup:]
+        errl = float(np.linalg.norm(xl.numpy()[:nv2] - xref_u) / np.linalg.norm(xref_u))
+        value = float(np.median([v[0] for v in res]))
+        log(f"learned + FP32 multigrid FGMRES: {res[-1][1]} its, {value:.3f}s, fell back {G32.learned_status()[1]}")
+        line_extra = {"learned_fgmres_iterations": res[-1][1], "learned_velocity_rel_l2_vs_fp64": errl,
+                      "learned_fell_back": bool(G32.learned_status()[1]), "train_s": t_train,
+                      "final_train_loss": float(np.mean(h[-8:]))}
+    med = lambda r, i: float(np.median([v[i] for v in r]))
+    line = {
+        "metric": "time-to-1e-8 residual (s): Taylor-Hood Q2-Q1 Stokes FGMRES(50), block preconditioner with "
+                  "learned + FP32 velocity multigrid",
+        "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": value * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f64 outer / f32 multigrid / bf16 network", "data": "synthetic",
+        "config": {"workload": "2D Stokes, Taylor-Hood Q2-Q1 256^2 cells, lid-driven + sinusoidal body force, nu=1",
+                   "unknowns": n, "nnz": nnz},
+        "th_assembly_s": t_asm,
+        "fp64_mg_fgmres_s": med(fp64, 0), "fp64_mg_fgmres_iterations": fp64[-1][1],
+        "fp32_cheb_mg_fgmres_s": med(fp32, 0), "fp32_cheb_mg_fgmres_iterations": fp32[-1][1],
+        "fp32_cheb_velocity_rel_l2_vs_fp64": err32,
+    }
+    line.update(line_extra)
+    print(json.dumps(line), flush=True)
+
+
 def bench_c3(jf, args):
     """3D -div(a grad u) = f, Q2 on 96^3 cells, a = exp(g) per cell with g a seeded
     squared-exponential Gaussian random field (length 0.1, contrast 1e4); the
@@ -354,10 +480,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-smoother solve")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
                     help="c2: 3D Q2 96^3 learned + FP32 MG vs FP64 MG-PCG (default); c3: the same grid with a "
                          "log-normal coefficient (contrast 1e4), FP32 MG in FP64 FCG vs FP64 MG-PCG; c4: 2D Q1 "
-                         "1024^2 log-normal coefficient, learned smoother vs Chebyshev")
+                         "1024^2 log-normal coefficient, learned smoother vs Chebyshev; c5: 2D Taylor-Hood Stokes "
+                         "256^2, block-preconditioned FGMRES")
     ap.add_argument("--train-fields", type=int, default=16)
     ap.add_argument("--pairs", type=int, default=64, help="(residual, error) pairs per training field")
     ap.add_argument("--eval-fields", type=int, default=8)
@@ -380,6 +507,9 @@ def main():
         return
     if args.workload == "c3":
         bench_c3(jf, args)
+        return
+    if args.workload == "c5":
+        bench_c5(jf, args)
         return
 
     nlev = 1

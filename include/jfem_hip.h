@@ -122,6 +122,34 @@ int jfem_assemble(jfem_space *s, int nint, const int *kinds, const double *weigh
  * of f(x) = sum_j amp[j] prod_a sin(kw[j*dim+a] pi x_a), into rhs (overwritten) */
 int jfem_assemble_rhs_sines(jfem_space *s, int nterms, const int *kw, const double *amp, int quad_rule,
                             jfem_vec *rhs);
+/* the same with a cosine factor on axis a of term j where cosine[j*dim+a] != 0
+ * (cosine nullable = all sines): f(x) = sum_j amp[j] prod_a trig(kw[j*dim+a] pi x_a) */
+int jfem_assemble_rhs_trig(jfem_space *s, int nterms, const int *kw, const int *cosine, const double *amp,
+                           int quad_rule, jfem_vec *rhs);
+/* rank of every DOF's node in CoordinateComparator order (M/linalg/CoordinateComparator.java:15-27;
+ * first coordinate most significant), the order the mixed spaces number nodes in */
+int jfem_space_lex_rank(jfem_space *s, int *out);
+
+/* ---- Taylor-Hood Q(k+1)-Qk mixed system (M/mixed/TaylorHoodSpace.java:17-92) ----
+ * Unknowns in TreeSet order of QkQkFunction.compareTo (M/mixed/QkQkFunction.java:82-93):
+ * velocity component 0, ..., component dim-1, then pressure, each by node point
+ * in CoordinateComparator order; size dim*ndofs(velocity) + ndofs(pressure).
+ * `velocity` is the scalar Q(k+1) space and `pressure` the Qk space on the same grid. */
+#define JFEM_MIXED_VELOCITY_GRAD_GRAD 0 /* MixedCellIntegral.fromVelocityIntegral(TPVectorCellIntegral GRAD_GRAD)
+                                           (M/mixed/MixedCellIntegral.java:58-68, M/tensorproduct/TPVectorCellIntegral.java:80-88) */
+#define JFEM_MIXED_DIV_VALUE 1          /* MixedTPCellIntegral DIV_VALUE (M/mixed/MixedTPCellIntegral.java:37-59),
+                                           placed at (velocity, pressure) and (pressure, velocity) */
+/* TaylorHoodSpace.writeCellIntegralsToMatrix (M/basic/MatrixFESpace.java:22-42 with
+ * MixedCellIntegral.evaluateCellIntegral, M/mixed/MixedCellIntegral.java:93-124): at most one
+ * integral of each kind, constant weights; pattern = COO key set as for jfem_assemble */
+int jfem_th_assemble(jfem_space *velocity, jfem_space *pressure, int nint, const int *kinds, const double *weights,
+                     int quad_rule, jfem_mat **out);
+/* MixedRightHandSideIntegral.fromVelocityIntegral(TPVectorRightHandSideIntegral VALUE)
+ * (M/mixed/MixedRightHandSideIntegral.java:35-74, M/tensorproduct/TPVectorRightHandSideIntegral.java:25-33)
+ * for one component of the body force, f_c as in jfem_assemble_rhs_trig: ADDED into the
+ * component's slice of the mixed vector rhs (writeCellIntegralsToRhs adds) */
+int jfem_th_assemble_rhs_trig(jfem_space *velocity, jfem_space *pressure, int component, int nterms, const int *kw,
+                              const int *cosine, const double *amp, int quad_rule, jfem_vec *rhs);
 
 /* ---- matrices (SparseMatrix / VectorMultiplyable) ----------------------- */
 int jfem_mat_from_csr(jfem_ctx *ctx, long rows, long cols, const long *rowptr, const int *colidx,
@@ -165,6 +193,20 @@ int jfem_solve_gmres(jfem_mat *A, jfem_mg *M, const jfem_vec *b, jfem_vec *x, do
  * preconditioner of any precision; not in the reference. */
 int jfem_solve_fcg(jfem_mat *A, jfem_mg *M, const jfem_vec *b, jfem_vec *x, double tol, long maxit,
                    jfem_solve_info *info, double *res_hist);
+/* Stokes saddle-point system of jfem_th_assemble (after Dirichlet data and the pressure
+ * pin): right-preconditioned flexible GMRES(restart) with the block lower-triangular
+ * preconditioner  z_u = M^-1 r_u (velocity_mg once per component, in the scalar space's
+ * DOF order),  z_p = pressure_scale * (B z_u - r_p)  with B the pressure rows of A.
+ * pressure_scale: ndofs(pressure) entries, nu / lumped mass for the Schur complement
+ * S ~ M_p / nu, and -1 on a pinned (identity) pressure row.  Stops when
+ * ||b - A x||_2 <= rtol ||b||_2 (the Arnoldi estimate, confirmed on the true residual at
+ * each restart) or after maxit iterations.  The reference solves these systems with
+ * GMRES2 around a monolithic multigrid (Examples/src/StokesMG2.java:26-129); this
+ * preconditioner is not in the reference.  res_hist (nullable, maxit entries): relative
+ * residual estimate per iteration. */
+int jfem_solve_stokes_fgmres(jfem_mat *A, jfem_mg *velocity_mg, const jfem_vec *pressure_scale, const jfem_vec *b,
+                             jfem_vec *x, double rtol, int restart, long maxit, jfem_solve_info *info,
+                             double *res_hist);
 
 /* ---- geometric multigrid (MGPreconditionerInterface / MGPreconditionerSpace,
  *      M/multigrid/MGPreconditionerInterface.java:125-379) -------------------- */

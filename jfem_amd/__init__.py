@@ -9,10 +9,16 @@ from .api import (FP32, FP64, ChebyshevSmoother, ContinuousTPFESpace, Context, D
                   SinusoidSource, SmootherNet, SparseMatrix, TPCellIntegral, TPRightHandSideIntegral, context,
                   normL2Difference)
 
+from .mixed import (MixedCellIntegral, MixedRightHandSideIntegral, MixedTPCellIntegral, StokesBlockPreconditioner,
+                    TaylorHoodSpace, TPVectorCellIntegral, TPVectorRightHandSideIntegral, TrigSource, VectorTrigSource,
+                    solveStokesFGMRES)
+
 __all__ = [
     "IllegalArgumentException", "IllegalStateException", "JfemError", "NativeLibraryError",
     "UnsupportedOperationException", "lib", "FP32", "FP64", "ChebyshevSmoother", "ContinuousTPFESpace", "Context",
     "DenseVector", "IterativeSolver", "JacobiSmoother", "MGPreconditioner", "QuadratureRule1D",
     "RichardsonSmoother", "ScalarFESpaceFunction", "SinusoidSource", "SmootherNet", "SparseMatrix", "TPCellIntegral",
-    "TPRightHandSideIntegral", "context", "normL2Difference",
+    "TPRightHandSideIntegral", "context", "normL2Difference", "MixedCellIntegral", "MixedRightHandSideIntegral",
+    "MixedTPCellIntegral", "StokesBlockPreconditioner", "TaylorHoodSpace", "TPVectorCellIntegral",
+    "TPVectorRightHandSideIntegral", "TrigSource", "VectorTrigSource", "solveStokesFGMRES",
 ]

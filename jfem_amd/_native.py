@@ -86,6 +86,10 @@ _SIGS = {
     "jfem_vec_axpby": (C.c_int, [C.c_double, _vp, C.c_double, _vp]),
     "jfem_assemble": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp, C.c_int, C.POINTER(_vp)]),
     "jfem_assemble_rhs_sines": (C.c_int, [_vp, C.c_int, _ip, _dp, C.c_int, _vp]),
+    "jfem_assemble_rhs_trig": (C.c_int, [_vp, C.c_int, _ip, _ip, _dp, C.c_int, _vp]),
+    "jfem_space_lex_rank": (C.c_int, [_vp, _ip]),
+    "jfem_th_assemble": (C.c_int, [_vp, _vp, C.c_int, _ip, _dp, C.c_int, C.POINTER(_vp)]),
+    "jfem_th_assemble_rhs_trig": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _ip, _ip, _dp, C.c_int, _vp]),
     "jfem_mat_from_csr": (C.c_int, [_vp, C.c_long, C.c_long, _lp, _ip, _dp, C.POINTER(_vp)]),
     "jfem_mat_destroy": (C.c_int, [_vp]),
     "jfem_mat_info": (C.c_int, [_vp, _lp, _lp, _lp]),
@@ -98,6 +102,8 @@ _SIGS = {
     "jfem_solve_bicgstab": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_long, C.POINTER(SolveInfo)]),
     "jfem_solve_gmres": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, C.POINTER(SolveInfo)]),
     "jfem_solve_fcg": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_long, C.POINTER(SolveInfo), _dp]),
+    "jfem_solve_stokes_fgmres": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_long,
+                                            C.POINTER(SolveInfo), _dp]),
     "jfem_mg_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, _ip, _dp, _dp, C.c_int, C.POINTER(MgOpts),
                                  C.POINTER(_vp)]),
     "jfem_mg_destroy": (C.c_int, [_vp]),

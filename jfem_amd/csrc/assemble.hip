@@ -675,7 +675,7 @@ __global__ void k_scan_finish(const long* __restrict__ tmp, const long* __restri
 }
 
 // rowptr[0..n] from counts[0..n-1]; returns total (host)
-static long scan_counts(jfem_ctx* c, const int* counts, long n, int* rowptr)
+long scan_counts(jfem_ctx* c, const int* counts, long n, int* rowptr)
 {
     const long tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
     long *tmp = nullptr, *sums = nullptr;
@@ -812,23 +812,20 @@ static void cells_1d(double start, double end, int n, std::vector<double>& s, st
 void space_evaluate_dev(jfem_space* s, const double* coeff, long npts, const double* pts_dev, double* out_dev)
 {
     const Grid& g = s->grid;
-    double* se[2 * MAXD] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    DevBuf<double> se[2 * MAXD];
     for (int a = 0; a < g.d; a++) {
         std::vector<double> s1, e1;
         cells_1d(g.start[a], g.end[a], g.n[a], s1, e1);
-        JFEM_HIP(hipMalloc(&se[a], sizeof(double) * g.n[a]));
-        JFEM_HIP(hipMalloc(&se[MAXD + a], sizeof(double) * g.n[a]));
+        se[a].alloc(g.n[a]);
+        se[MAXD + a].alloc(g.n[a]);
         JFEM_HIP(hipMemcpy(se[a], s1.data(), sizeof(double) * g.n[a], hipMemcpyHostToDevice));
         JFEM_HIP(hipMemcpy(se[MAXD + a], e1.data(), sizeof(double) * g.n[a], hipMemcpyHostToDevice));
     }
     if (npts > 0)
         k_eval_fe<<<grid_for(npts, 256, 16384), 256, 0, s->ctx->stream>>>(to_dev(g), se[0], se[1], se[2], se[3], se[4],
                                                                          se[5], coeff, npts, pts_dev, out_dev);
-    const hipError_t e = hipGetLastError();
-    const hipError_t e2 = hipStreamSynchronize(s->ctx->stream);
-    for (auto p : se) (void)hipFree(p);
-    JFEM_HIP(e);
-    JFEM_HIP(e2);
+    JFEM_HIP(hipGetLastError());
+    JFEM_HIP(hipStreamSynchronize(s->ctx->stream));
 }
 
 // ------------------------------ RHS (sines) --------------------------------
@@ -1160,7 +1157,8 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     return A;
 }
 
-void assemble_rhs_sines(jfem_space* s, int nterms, const int* kw, const double* amp, int rule, double* rhs)
+void assemble_rhs_sines(jfem_space* s, int nterms, const int* kw, const double* amp, int rule, double* rhs,
+                        const int* cosine)
 {
     jfem_ctx* c = s->ctx;
     const Grid& g = s->grid;
@@ -1170,7 +1168,7 @@ void assemble_rhs_sines(jfem_space* s, int nterms, const int* kw, const double* 
     upload_classes(c, g, C, D);
     double qp[8], qw[8];
     quad_rule(rule, qp, qw);
-    // per-axis tables sin(kw * pi * x_q); the amplitude is folded into axis 0
+    // per-axis tables sin(kw * pi * x_q) (cos where the cosine flag is set); the amplitude is folded into axis 0
     std::vector<double*> dsin(g.d, nullptr);
     RhsParams P;
     std::memset(&P, 0, sizeof(P));
@@ -1184,7 +1182,8 @@ void assemble_rhs_sines(jfem_space* s, int nterms, const int* kw, const double* 
 #pragma clang fp contract(off)
                     const double len = C.e1d[a][j] - C.s1d[a][j];
                     const double x = qp[q] * len + C.s1d[a][j];
-                    double v = std::sin(kw[m * g.d + a] * M_PI * x);
+                    double v = cosine && cosine[m * g.d + a] ? std::cos(kw[m * g.d + a] * M_PI * x)
+                                                             : std::sin(kw[m * g.d + a] * M_PI * x);
                     if (a == 0) v = amp[m] * v;
                     t[((size_t)m * g.n[a] + j) * C.Q + q] = v;
                 }
@@ -1230,6 +1229,394 @@ void space_boundary_mask(jfem_space* s, unsigned char* mask_dev)
     JFEM_HIP(hipGetLastError());
 }
 
+
+// ------------------------- Taylor-Hood mixed system -------------------------
+// TaylorHoodSpace (M/mixed/TaylorHoodSpace.java:17-92): pressure Qk and
+// velocity Q(k+1)^d on one Cartesian grid.  The shape functions live in a
+// TreeSet ordered by QkQkFunction.compareTo (M/mixed/QkQkFunction.java:82-93):
+// velocity before pressure; velocity component-major
+// (SingleComponentVectorShapeFunction.compareTo, M/basic/SingleComponentVectorShapeFunction.java:168-173);
+// inside a component, and for the pressure, by node point under
+// CoordinateComparator (first coordinate most significant, tolerance 1e-10,
+// M/linalg/CoordinateComparator.java:15-27; ContinuousTPShapeFunction.compareTo
+// :232-242).  Node spacings far above 1e-10 make that the lexicographic
+// lattice order with the last axis fastest: lex(p) = sum_a p_a prod_{b>a} N_b.
+
+__host__ __device__ inline long lex_of_lattice(int d, const int* N, const int* p)
+{
+    long r = 0;
+    for (int a = 0; a < d; a++) r = r * N[a] + p[a];
+    return r;
+}
+
+__host__ __device__ inline void lattice_of_lex(int d, const int* N, long r, int* p)
+{
+    for (int a = d - 1; a >= 0; a--) { p[a] = (int)(r % N[a]); r /= N[a]; }
+}
+
+// lexicographic rank of every DOF of a scalar space (DOF order -> node order)
+__global__ void k_lex_rank(GridDev g, long n, int* __restrict__ out)
+{
+    int N[MAXD];
+    for (int a = 0; a < MAXD; a++) N[a] = a < g.d ? g.k * g.n[a] + 1 : 1;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        int p[MAXD] = {0, 0, 0};
+        lattice_of_dof(g, i, p);
+        out[i] = (int)lex_of_lattice(g.d, N, p);
+    }
+}
+
+void space_lex_rank(jfem_space* s, int* out_dev)
+{
+    const long n = s->grid.ndofs();
+    JFEM_CHECK(n < (1l << 31), JFEM_ERR_UNSUPPORTED, "2^31 or more nodes");
+    if (n) k_lex_rank<<<grid_for(n, 256, 16384), 256, 0, s->ctx->stream>>>(to_dev(s->grid), n, out_dev);
+    JFEM_HIP(hipGetLastError());
+}
+
+struct ThParams {
+    int kv, kp, nlocv, nlocp;
+    int n[MAXD];
+    int NV[MAXD], NP[MAXD];   // velocity and pressure nodes per axis
+    long nv, np;              // nodes of one velocity component, pressure nodes
+    int ncls[MAXD];
+    const int* cls[MAXD];     // class of every 1D cell per axis
+    const double* Evv;        // [tuple][iv][iu], nlocv^2 per tuple (null: no velocity integral)
+    const double* EB;         // [tuple][c][iv][ip], d*nlocv*nlocp per tuple (null: no DIV_VALUE)
+};
+
+// cells of one axis that contain lattice coordinate x of a degree-k lattice
+__device__ inline void cells_of(int x, int k, int n, int& lo, int& hi)
+{
+    if (x % k == 0) { lo = x / k - 1; hi = x / k; } else { lo = hi = x / k; }
+    if (lo < 0) lo = 0;
+    if (hi > n - 1) hi = n - 1;
+}
+
+// entry between velocity node p (component c) and a node q of the lattice of
+// degree kq: sum over the cells containing both, lexicographic cell order,
+// of the element values that are not exactly zero (MatrixFESpace.addToMatrix
+// skips those).  vv: q is a velocity node of the same component; otherwise q
+// is a pressure node.
+template <int DD>
+__device__ inline double th_entry(const ThParams& P, int c, const int* p, const int* q, bool vv, bool& present)
+{
+#pragma clang fp contract(off)
+    int lo[MAXD] = {0, 0, 0}, cnt[MAXD] = {1, 1, 1};
+    const int kq = vv ? P.kv : P.kp;
+#pragma unroll
+    for (int a = 0; a < DD; a++) {
+        int plo, phi, qlo, qhi;
+        cells_of(p[a], P.kv, P.n[a], plo, phi);
+        cells_of(q[a], kq, P.n[a], qlo, qhi);
+        lo[a] = max(plo, qlo);
+        cnt[a] = min(phi, qhi) - lo[a] + 1;
+    }
+    double sum = 0;
+    present = false;
+#pragma unroll
+    for (int i0 = 0; i0 < 2; i0++)
+#pragma unroll
+        for (int i1 = 0; i1 < 2; i1++)
+#pragma unroll
+            for (int i2 = 0; i2 < (DD > 2 ? 2 : 1); i2++) {
+                const int ii[MAXD] = {i0, i1, i2};
+                bool in = true;
+#pragma unroll
+                for (int a = 0; a < DD; a++) in = in && ii[a] < cnt[a];
+                if (!in) continue;
+                long tup = 0;
+                int iv = 0, iq = 0, mv = 1, mq = 1;
+#pragma unroll
+                for (int a = 0; a < DD; a++) {
+                    const int C = lo[a] + ii[a];
+                    tup = tup * P.ncls[a] + (P.ncls[a] == 1 ? 0 : P.cls[a][C]);
+                    iv += (p[a] - P.kv * C) * mv;
+                    iq += (q[a] - kq * C) * mq;
+                    mv *= P.kv + 1;
+                    mq *= kq + 1;
+                }
+                const double e = vv ? P.Evv[(tup * P.nlocv + iv) * P.nlocv + iq]
+                                    : P.EB[((tup * DD + c) * P.nlocv + iv) * P.nlocp + iq];
+                if (e != 0) {
+                    sum = e + sum;
+                    present = true;
+                }
+            }
+    return sum;
+}
+
+// one thread per row of the mixed matrix; FILL = false counts the entries,
+// FILL = true writes them (and flags rows whose count differs)
+template <int DD, bool FILL>
+__global__ __launch_bounds__(256) void k_th_rows(ThParams P, long nrows, int* __restrict__ counts,
+                                                 const int* __restrict__ rowptr, int* __restrict__ cols,
+                                                 double* __restrict__ vals, int* __restrict__ bad)
+{
+#pragma clang fp contract(off)
+    const long nvel = DD * P.nv;
+    for (long row = blockIdx.x * (long)blockDim.x + threadIdx.x; row < nrows; row += (long)gridDim.x * blockDim.x) {
+        int written = 0;
+        const int base = FILL ? rowptr[row] : 0;
+        auto emit = [&](long col, double v) {
+            if (FILL) { cols[base + written] = (int)col; vals[base + written] = v; }
+            written++;
+        };
+        int lo[MAXD] = {0, 0, 0}, hi[MAXD] = {0, 0, 0};
+        if (row < nvel) {
+            const int c = (int)(row / P.nv);
+            int p[MAXD] = {0, 0, 0};
+            lattice_of_lex(DD, P.NV, row - c * P.nv, p);
+#pragma unroll
+            for (int a = 0; a < DD; a++) cells_of(p[a], P.kv, P.n[a], lo[a], hi[a]);
+            if (P.Evv) {
+                int q[MAXD] = {0, 0, 0};
+                int qlo[MAXD], qhi[MAXD];
+#pragma unroll
+                for (int a = 0; a < DD; a++) { qlo[a] = lo[a] * P.kv; qhi[a] = (hi[a] + 1) * P.kv; q[a] = qlo[a]; }
+                for (;;) {
+                    bool present;
+                    const double v = th_entry<DD>(P, c, p, q, true, present);
+                    const long col = c * P.nv + lex_of_lattice(DD, P.NV, q);
+                    if (row == 0 && col == 0) present = true;   // dummy (0,0) entry of a new SparseMatrix
+                    if (present) emit(col, v);
+                    int a = DD - 1;
+                    while (a >= 0 && q[a] == qhi[a]) { q[a] = qlo[a]; a--; }
+                    if (a < 0) break;
+                    q[a]++;
+                }
+            } else if (row == 0) {
+                emit(0, 0.0);
+            }
+            if (P.EB) {
+                int q[MAXD] = {0, 0, 0};
+                int qlo[MAXD], qhi[MAXD];
+#pragma unroll
+                for (int a = 0; a < DD; a++) { qlo[a] = lo[a] * P.kp; qhi[a] = (hi[a] + 1) * P.kp; q[a] = qlo[a]; }
+                for (;;) {
+                    bool present;
+                    const double v = th_entry<DD>(P, c, p, q, false, present);
+                    if (present) emit(nvel + lex_of_lattice(DD, P.NP, q), v);
+                    int a = DD - 1;
+                    while (a >= 0 && q[a] == qhi[a]) { q[a] = qlo[a]; a--; }
+                    if (a < 0) break;
+                    q[a]++;
+                }
+            }
+        } else if (P.EB) {
+            int q[MAXD] = {0, 0, 0};
+            lattice_of_lex(DD, P.NP, row - nvel, q);
+#pragma unroll
+            for (int a = 0; a < DD; a++) cells_of(q[a], P.kp, P.n[a], lo[a], hi[a]);
+            for (int c = 0; c < DD; c++) {
+                int p[MAXD] = {0, 0, 0};
+                int plo[MAXD], phi[MAXD];
+#pragma unroll
+                for (int a = 0; a < DD; a++) { plo[a] = lo[a] * P.kv; phi[a] = (hi[a] + 1) * P.kv; p[a] = plo[a]; }
+                for (;;) {
+                    bool present;
+                    const double v = th_entry<DD>(P, c, p, q, false, present);
+                    if (present) emit(c * P.nv + lex_of_lattice(DD, P.NV, p), v);
+                    int a = DD - 1;
+                    while (a >= 0 && p[a] == phi[a]) { p[a] = plo[a]; a--; }
+                    if (a < 0) break;
+                    p[a]++;
+                }
+            }
+        }
+        if (!FILL) counts[row] = written;
+        else if (written != rowptr[row + 1] - base) atomicOr(bad, 1);
+    }
+}
+
+// DIV_VALUE element values (MixedTPCellIntegral.evaluatePressureVelocityIntegral,
+// M/mixed/MixedTPCellIntegral.java:37-59): for velocity function (component c,
+// local node iv) and pressure function ip, integrateNonTensorProduct
+// (M/tensorproduct/TPCellIntegral.java:78-102) of
+//   divergence * value * weight,  divergence = gradient component c of the
+// scalar component function (SingleComponentVectorShapeFunction.divergence
+// :116-120, fastGradientInCell product order, ContinuousTPShapeFunction.java:173-198),
+// value = product of the 1D values (fastValueInCell :154-171), then times each
+// axis' quadrature weight, summed over the points with the last axis fastest.
+#pragma clang fp contract(off)
+static void div_value_table(int d, int kv, int kp, int Q, const Tab1D* const* tv, const Tab1D* const* tp, double w,
+                            double* out)
+{
+    const int nlv = (int)std::lround(std::pow(kv + 1, d)), nlp = (int)std::lround(std::pow(kp + 1, d));
+    int nq = 1;
+    for (int a = 0; a < d; a++) nq *= Q;
+    for (int c = 0; c < d; c++)
+        for (int iv = 0; iv < nlv; iv++)
+            for (int ip = 0; ip < nlp; ip++) {
+                int lv[MAXD] = {0, 0, 0}, lp[MAXD] = {0, 0, 0};
+                int rv = iv, rp = ip;
+                for (int a = 0; a < d; a++) { lv[a] = rv % (kv + 1); rv /= kv + 1; lp[a] = rp % (kp + 1); rp /= kp + 1; }
+                double ret = 0;
+                for (int q = 0; q < nq; q++) {
+                    int qa[MAXD] = {0, 0, 0};
+                    int qq = q;
+                    for (int a = d - 1; a >= 0; a--) { qa[a] = qq % Q; qq /= Q; }
+                    double div = 1;
+                    for (int j = 0; j < d; j++)
+                        div *= j == c ? tv[j]->der[lv[j]][qa[j]] : tv[j]->val[lv[j]][qa[j]];
+                    double pv = 1;
+                    for (int a = 0; a < d; a++) pv *= tp[a]->val[lp[a]][qa[a]];
+                    double val = div * pv * w;
+                    for (int a = 0; a < d; a++) val *= tv[a]->w[qa[a]];
+                    ret += val;
+                }
+                out[((long)c * nlv + iv) * nlp + ip] = ret;
+            }
+}
+#pragma clang fp contract(on)
+
+jfem_mat* th_assemble(jfem_space* V, jfem_space* Pp, bool has_vv, double vv_w, bool has_div, double div_w, int rule)
+{
+    jfem_ctx* c = V->ctx;
+    const Grid& gv = V->grid;
+    const Grid& gp = Pp->grid;
+    const int d = gv.d;
+    JFEM_CHECK(d == 2 || d == 3, JFEM_ERR_UNSUPPORTED, "Taylor-Hood spaces in 2 or 3 dimensions");
+    JFEM_CHECK(gp.d == d && gv.k == gp.k + 1, JFEM_ERR_ARG,
+               "velocity space must have degree pressure degree + 1 on the same grid");
+    for (int a = 0; a < d; a++)
+        JFEM_CHECK(gv.n[a] == gp.n[a] && gv.start[a] == gp.start[a] && gv.end[a] == gp.end[a], JFEM_ERR_ARG,
+                   "velocity and pressure spaces on different grids");
+    JFEM_CHECK(has_vv || has_div, JFEM_ERR_ARG, "no cell integrals");
+    double qp[8], qw[8];
+    const int Q = quad_rule(rule, qp, qw);
+    JFEM_CHECK(Q > 0, JFEM_ERR_ARG, "unknown quadrature rule");
+    JFEM_CHECK(gv.k < MAXK1, JFEM_ERR_UNSUPPORTED, "velocity degree must be 2..4");
+    // cell classes: 1D cells with bitwise equal velocity and pressure tables
+    ClassTables CV;
+    CV.Q = Q;
+    std::vector<Tab1D> tabp[MAXD];
+    for (int a = 0; a < d; a++) {
+        cells_1d(gv.start[a], gv.end[a], gv.n[a], CV.s1d[a], CV.e1d[a]);
+        std::map<std::string, int> seen;
+        CV.cls[a].resize(gv.n[a]);
+        for (int j = 0; j < gv.n[a]; j++) {
+            const Tab1D tv = make_tab(CV.s1d[a][j], CV.e1d[a][j], gv.k, qp, qw, Q);
+            const Tab1D tq = make_tab(CV.s1d[a][j], CV.e1d[a][j], gp.k, qp, qw, Q);
+            std::string key(reinterpret_cast<const char*>(&tv), sizeof(tv));
+            key.append(reinterpret_cast<const char*>(&tq), sizeof(tq));
+            auto it = seen.find(key);
+            if (it == seen.end()) {
+                const int id = (int)CV.tabs[a].size();
+                seen.emplace(key, id);
+                CV.tabs[a].push_back(tv);
+                tabp[a].push_back(tq);
+                CV.cls[a][j] = id;
+            } else {
+                CV.cls[a][j] = it->second;
+            }
+        }
+        CV.ncls[a] = (int)CV.tabs[a].size();
+    }
+    long ntuples = 1;
+    for (int a = 0; a < d; a++) ntuples *= CV.ncls[a];
+    const int nlv = gv.nloc(), nlp = gp.nloc();
+    JFEM_CHECK(ntuples * (long)nlv * nlv < (1l << 30), JFEM_ERR_UNSUPPORTED, "too many distinct cell geometries");
+    DevTables D;
+    upload_classes(c, gv, CV, D);
+    DevBuf<double> Evv, EB;
+    if (has_vv) {
+        ElemParams EP;
+        std::memset(&EP, 0, sizeof(EP));
+        EP.d = d; EP.k = gv.k; EP.Q = Q; EP.nloc = nlv; EP.nint = 1; EP.ntuples = ntuples;
+        EP.kinds[0] = 0;   // GRAD_GRAD: the Frobenius product of single-component gradients is the scalar one
+        EP.wconst[0] = vv_w;
+        for (int a = 0; a < d; a++) EP.ncls[a] = CV.ncls[a];
+        Evv.alloc((size_t)ntuples * nlv * nlv);
+        std::vector<int> pairs;
+        for (int v = 0; v < nlv; v++)
+            for (int u = v; u < nlv; u++) pairs.push_back(v << 16 | u);
+        DevBuf<int> pairs_d(pairs.size());
+        JFEM_HIP(hipMemcpyAsync(pairs_d, pairs.data(), sizeof(int) * pairs.size(), hipMemcpyHostToDevice, c->stream));
+        const int eg = grid_for((long)pairs.size() * ntuples, 256, 16384);
+        switch ((d * 10 + gv.k) * 10 + Q) {
+        case 325: k_element_tables<3, 2, 5><<<eg, 256, 0, c->stream>>>(EP, D.tabs[0], D.tabs[1], D.tabs[2], pairs_d, Evv); break;
+        case 225: k_element_tables<2, 2, 5><<<eg, 256, 0, c->stream>>>(EP, D.tabs[0], D.tabs[1], D.tabs[2], pairs_d, Evv); break;
+        default: k_element_tables<0, 0, 0><<<eg, 256, 0, c->stream>>>(EP, D.tabs[0], D.tabs[1], D.tabs[2], pairs_d, Evv); break;
+        }
+        JFEM_HIP(hipGetLastError());
+        JFEM_HIP(hipStreamSynchronize(c->stream));
+    }
+    if (has_div) {
+        const long per = (long)d * nlv * nlp;
+        std::vector<double> eb((size_t)ntuples * per);
+        for (long t = 0; t < ntuples; t++) {
+            int cl[MAXD] = {0, 0, 0};
+            long r = t;
+            for (int a = d - 1; a >= 0; a--) { cl[a] = (int)(r % CV.ncls[a]); r /= CV.ncls[a]; }
+            const Tab1D* tv[MAXD] = {&CV.tabs[0][cl[0]], &CV.tabs[1][cl[1]], d > 2 ? &CV.tabs[2][cl[2]] : nullptr};
+            const Tab1D* tq[MAXD] = {&tabp[0][cl[0]], &tabp[1][cl[1]], d > 2 ? &tabp[2][cl[2]] : nullptr};
+            div_value_table(d, gv.k, gp.k, Q, tv, tq, div_w, eb.data() + t * per);
+        }
+        EB.alloc(eb.size());
+        JFEM_HIP(hipMemcpy(EB, eb.data(), sizeof(double) * eb.size(), hipMemcpyHostToDevice));
+    }
+    ThParams P;
+    std::memset(&P, 0, sizeof(P));
+    P.kv = gv.k; P.kp = gp.k; P.nlocv = nlv; P.nlocp = nlp;
+    P.nv = gv.ndofs(); P.np = gp.ndofs();
+    for (int a = 0; a < MAXD; a++) {
+        P.n[a] = a < d ? gv.n[a] : 1;
+        P.NV[a] = a < d ? (int)gv.nodes_axis(a) : 1;
+        P.NP[a] = a < d ? (int)gp.nodes_axis(a) : 1;
+        P.ncls[a] = a < d ? CV.ncls[a] : 1;
+        P.cls[a] = D.cls[a];
+    }
+    P.Evv = has_vv ? Evv.p : nullptr;
+    P.EB = has_div ? EB.p : nullptr;
+    const long n = d * P.nv + P.np;
+    JFEM_CHECK(n < (1l << 31), JFEM_ERR_UNSUPPORTED, "2^31 or more unknowns");
+    auto* A = new jfem_mat();
+    A->ctx = c;
+    A->rows = A->cols = n;
+    try {
+        DevBuf<int> counts(n), bad(1);
+        JFEM_HIP(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
+        JFEM_HIP(hipMalloc(&A->rowptr, sizeof(int) * (n + 1)));
+        const int blocks = grid_for(n, 256, 65536);
+        auto launch = [&](bool fill) {
+            if (d == 2) {
+                if (fill) k_th_rows<2, true><<<blocks, 256, 0, c->stream>>>(P, n, counts, A->rowptr, A->colidx, A->vals, bad);
+                else k_th_rows<2, false><<<blocks, 256, 0, c->stream>>>(P, n, counts, A->rowptr, A->colidx, A->vals, bad);
+            } else {
+                if (fill) k_th_rows<3, true><<<blocks, 256, 0, c->stream>>>(P, n, counts, A->rowptr, A->colidx, A->vals, bad);
+                else k_th_rows<3, false><<<blocks, 256, 0, c->stream>>>(P, n, counts, A->rowptr, A->colidx, A->vals, bad);
+            }
+            JFEM_HIP(hipGetLastError());
+        };
+        launch(false);
+        A->nnz = scan_counts(c, counts, n, A->rowptr);
+        JFEM_HIP(hipMalloc(&A->colidx, sizeof(int) * (A->nnz + NNZ_PAD)));
+        JFEM_HIP(hipMalloc(&A->vals, sizeof(double) * (A->nnz + NNZ_PAD)));
+        launch(true);
+        int bad_h = 0;
+        JFEM_HIP(hipMemcpyAsync(&bad_h, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        JFEM_HIP(hipStreamSynchronize(c->stream));
+        JFEM_CHECK(!bad_h, JFEM_ERR_STATE, "assembled rows differ from their reserved lengths");
+        A->lanes = choose_lanes(n > 0 ? (double)A->nnz / n : 1.0);
+        mat_build_blocks(A);
+    } catch (...) {
+        mat_free(A);
+        throw;
+    }
+    return A;
+}
+
+// scatter a scalar vector (DOF order) into slice [off, off + n) of dst in node order, adding
+__global__ void k_scatter_add_lex(long n, const int* __restrict__ rank, const double* __restrict__ src, long off,
+                                  double* __restrict__ dst)
+{
+#pragma clang fp contract(off)
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const long j = off + rank[i];
+        dst[j] = dst[j] + src[i];
+    }
+}
 }  // namespace jfem
 
 using namespace jfem;
@@ -1343,15 +1730,10 @@ int jfem_space_evaluate(jfem_space* s, const jfem_vec* coeffs, long npts, const 
     JFEM_CHECK(coeffs->n == s->grid.ndofs(), JFEM_ERR_ARG, "coefficient vector size differs from the space");
     if (npts == 0) return JFEM_OK;
     const int d = s->grid.d;
-    double *p = nullptr, *v = nullptr;
-    JFEM_HIP(hipMalloc(&p, sizeof(double) * npts * d));
-    JFEM_HIP(hipMalloc(&v, sizeof(double) * npts));
+    DevBuf<double> p(npts * d), v(npts);
     JFEM_HIP(hipMemcpy(p, points, sizeof(double) * npts * d, hipMemcpyHostToDevice));
     space_evaluate_dev(s, coeffs->d, npts, p, v);
-    const hipError_t e = hipMemcpy(values, v, sizeof(double) * npts, hipMemcpyDeviceToHost);
-    (void)hipFree(p);
-    (void)hipFree(v);
-    JFEM_HIP(e);
+    JFEM_HIP(hipMemcpy(values, v, sizeof(double) * npts, hipMemcpyDeviceToHost));
     API_END
 }
 
@@ -1367,10 +1749,7 @@ int jfem_fe_l2_difference(jfem_space* s1, const jfem_vec* c1, jfem_space* s2, co
     *result = 0;
     if (npts == 0) return JFEM_OK;
     const int d = s1->grid.d;
-    double *p = nullptr, *v1 = nullptr, *v2 = nullptr;
-    JFEM_HIP(hipMalloc(&p, sizeof(double) * npts * d));
-    JFEM_HIP(hipMalloc(&v1, sizeof(double) * npts));
-    JFEM_HIP(hipMalloc(&v2, sizeof(double) * npts));
+    DevBuf<double> p(npts * d), v1(npts), v2(npts);
     JFEM_HIP(hipMemcpy(p, points, sizeof(double) * npts * d, hipMemcpyHostToDevice));
     space_evaluate_dev(s1, c1->d, npts, p, v1);
     space_evaluate_dev(s2, c2->d, npts, p, v2);
@@ -1380,9 +1759,6 @@ int jfem_fe_l2_difference(jfem_space* s1, const jfem_vec* c1, jfem_space* s2, co
     k_diff<<<grid_for(npts, 256, 16384), 256, 0, s1->ctx->stream>>>(npts, v1, v2, v1);
     JFEM_HIP(hipGetLastError());
     const double acc = vec_dot(s1->ctx, npts, v1, v1);
-    (void)hipFree(p);
-    (void)hipFree(v1);
-    (void)hipFree(v2);
     *result = std::sqrt(acc);
     API_END
 }
@@ -1401,7 +1777,75 @@ int jfem_assemble_rhs_sines(jfem_space* s, int nterms, const int* kw, const doub
     API_BEGIN
     JFEM_CHECK(s && kw && amp && rhs, JFEM_ERR_ARG, "null argument");
     JFEM_CHECK(rhs->n == s->grid.ndofs(), JFEM_ERR_ARG, "Incompatible sizes");
-    assemble_rhs_sines(s, nterms, kw, amp, quad_rule, rhs->d);
+    assemble_rhs_sines(s, nterms, kw, amp, quad_rule, rhs->d, nullptr);
+    API_END
+}
+
+int jfem_assemble_rhs_trig(jfem_space* s, int nterms, const int* kw, const int* cosine, const double* amp,
+                           int quad_rule, jfem_vec* rhs)
+{
+    API_BEGIN
+    JFEM_CHECK(s && kw && amp && rhs, JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(rhs->n == s->grid.ndofs(), JFEM_ERR_ARG, "Incompatible sizes");
+    assemble_rhs_sines(s, nterms, kw, amp, quad_rule, rhs->d, cosine);
+    API_END
+}
+
+int jfem_space_lex_rank(jfem_space* s, int* out)
+{
+    API_BEGIN
+    JFEM_CHECK(s && out, JFEM_ERR_ARG, "null argument");
+    const long n = s->grid.ndofs();
+    DevBuf<int> r(n);
+    space_lex_rank(s, r);
+    // the context stream is non-blocking: copy on it, then wait
+    JFEM_HIP(hipMemcpyAsync(out, r, sizeof(int) * n, hipMemcpyDeviceToHost, s->ctx->stream));
+    JFEM_HIP(hipStreamSynchronize(s->ctx->stream));
+    API_END
+}
+
+int jfem_th_assemble(jfem_space* velocity, jfem_space* pressure, int nint, const int* kinds, const double* weights,
+                     int quad_rule, jfem_mat** out)
+{
+    API_BEGIN
+    JFEM_CHECK(velocity && pressure && out && (nint == 0 || (kinds && weights)), JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(velocity->ctx == pressure->ctx, JFEM_ERR_ARG, "spaces on different contexts");
+    bool has_vv = false, has_div = false;
+    double wv = 0, wd = 0;
+    for (int i = 0; i < nint; i++) {
+        if (kinds[i] == JFEM_MIXED_VELOCITY_GRAD_GRAD) {
+            JFEM_CHECK(!has_vv, JFEM_ERR_UNSUPPORTED, "one velocity GRAD_GRAD integral at most");
+            has_vv = true;
+            wv = weights[i];
+        } else if (kinds[i] == JFEM_MIXED_DIV_VALUE) {
+            JFEM_CHECK(!has_div, JFEM_ERR_UNSUPPORTED, "one DIV_VALUE integral at most");
+            has_div = true;
+            wd = weights[i];
+        } else {
+            JFEM_CHECK(false, JFEM_ERR_UNSUPPORTED, "Integral type not supported");
+        }
+    }
+    *out = th_assemble(velocity, pressure, has_vv, wv, has_div, wd, quad_rule);
+    API_END
+}
+
+int jfem_th_assemble_rhs_trig(jfem_space* velocity, jfem_space* pressure, int component, int nterms, const int* kw,
+                              const int* cosine, const double* amp, int quad_rule, jfem_vec* rhs)
+{
+    API_BEGIN
+    JFEM_CHECK(velocity && pressure && kw && amp && rhs, JFEM_ERR_ARG, "null argument");
+    const int d = velocity->grid.d;
+    JFEM_CHECK(component >= 0 && component < d, JFEM_ERR_ARG, "velocity component out of range");
+    const long nv = velocity->grid.ndofs();
+    JFEM_CHECK(rhs->n == d * nv + pressure->grid.ndofs(), JFEM_ERR_ARG, "Incompatible sizes");
+    DevBuf<double> tmp(nv);
+    DevBuf<int> rank(nv);
+    assemble_rhs_sines(velocity, nterms, kw, amp, quad_rule, tmp, cosine);
+    space_lex_rank(velocity, rank);
+    k_scatter_add_lex<<<grid_for(nv, 256, 16384), 256, 0, velocity->ctx->stream>>>(nv, rank, tmp, component * nv,
+                                                                                     rhs->d);
+    JFEM_HIP(hipGetLastError());
+    JFEM_HIP(hipStreamSynchronize(velocity->ctx->stream));
     API_END
 }
 

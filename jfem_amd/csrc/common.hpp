@@ -207,6 +207,26 @@ void vec_copy(jfem_ctx* c, double* dst, const double* src, long n);
 void vec_axpby(jfem_ctx* c, long n, double a, const double* x, double b, double* y);
 // dots: computes up to RED_SLOTS dot products (xs[i], ys[i]) in one pass; result on host
 void vec_dots(jfem_ctx* c, long n, int m, const double* const* xs, const double* const* ys, double* out_host);
+// device allocation freed when the owner leaves scope (also on a thrown Error)
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    void alloc(size_t n)
+    {
+        if (p) { (void)hipFree(p); p = nullptr; }
+        if (hipMalloc(&p, sizeof(T) * (n > 0 ? n : 1)) != hipSuccess) {
+            p = nullptr;
+            throw Error{JFEM_ERR_NOMEM, "device allocation failed"};
+        }
+    }
+    ~DevBuf() { (void)hipFree(p); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    operator T*() const { return p; }
+};
+
 double vec_dot(jfem_ctx* c, long n, const double* x, const double* y);
 // deterministic two-stage reduction of partials already written into ctx->red_partials
 void reduce_partials(jfem_ctx* c, int slots, int blocks, bool to_host, double* out_host);
@@ -220,6 +240,14 @@ int spmv_dot(jfem_mat* A, const double* x, double* y);   // returns the number o
 // y = A x with dot(x, y) reduced into the device scalar dst
 void spmv_dot_dev(jfem_mat* A, const double* x, double* y, double* dst);
 int choose_lanes(double mean_row);
+// rowptr[0..n] (device) from per-row counts (device); returns the total
+long scan_counts(jfem_ctx* c, const int* counts, long n, int* rowptr);
+// lexicographic node rank (last axis fastest) of every DOF of a scalar space, into a device array
+void space_lex_rank(jfem_space* s, int* out_dev);
+// finest grid of a multigrid hierarchy
+const Grid& mg_finest_grid(const jfem_mg* mg);
+// boundary mask of the finest level (device, DOF order)
+const unsigned char* mg_finest_bnd(const jfem_mg* mg);
 // reduce partials into a device scalar (no host sync)
 void reduce_partials_dev(jfem_ctx* c, int slots, int blocks, double* dst_dev);
 void mat_make_f32(jfem_mat* A);

@@ -110,12 +110,6 @@ static void read_scalars(jfem_ctx* c, const double* dev, int m, double* out)
     for (int i = 0; i < m; i++) out[i] = c->red_host[i];
 }
 
-struct DevBuf {
-    double* p = nullptr;
-    DevBuf(long n) { JFEM_HIP(hipMalloc(&p, sizeof(double) * (n > 0 ? n : 1))); }
-    ~DevBuf() { (void)hipFree(p); }
-};
-
 static int kgrid(long n) { return grid_for(n, KB, RED_BLOCKS); }
 
 static double true_resnorm(jfem_mat* A, const double* b, const double* x, double* tmp)
@@ -134,7 +128,7 @@ static void solve_cg(jfem_mat* A, jfem_mg* M, const double* b, double* x, double
     jfem_ctx* c = A->ctx;
     const long n = A->rows;
     double* sc = c->scal;
-    DevBuf r(n), z(n), p(n), q(n);
+    DevBuf<double> r(n), z(n), p(n), q(n);
     // r = b - A x
     spmv(A, x, q.p);
     k_sub<<<kgrid(n), KB, 0, c->stream>>>(n, b, q.p, r.p);
@@ -185,7 +179,7 @@ static void solve_fcg(jfem_mat* A, jfem_mg* M, const double* b, double* x, doubl
     jfem_ctx* c = A->ctx;
     const long n = A->rows;
     double* sc = c->scal;
-    DevBuf r(n), z(n), p(n), q(n);
+    DevBuf<double> r(n), z(n), p(n), q(n);
     spmv(A, x, q.p);
     k_sub<<<kgrid(n), KB, 0, c->stream>>>(n, b, q.p, r.p);
     mg_apply(M, r.p, z.p);
@@ -263,7 +257,7 @@ static void solve_bicgstab(jfem_mat* A, jfem_mg* M, const double* b, double* x, 
 {
     jfem_ctx* c = A->ctx;
     const long n = A->rows;
-    DevBuf v(n), s(n), t(n), r0(n), r(n), p(n), y(n), z(n), pt(n), tmp(n);
+    DevBuf<double> v(n), s(n), t(n), r0(n), r(n), p(n), y(n), z(n), pt(n), tmp(n);
     vec_fill(c, v.p, 0, n);
     vec_fill(c, p.p, 0, n);
     spmv(A, x, tmp.p);
@@ -418,7 +412,7 @@ static void solve_gmres(jfem_mat* A, jfem_mg* M, const double* b, double* x, dou
     double* V = nullptr;
     JFEM_CHECK(maxv * (double)n * 8 < 200e9, JFEM_ERR_NOMEM, "Krylov basis exceeds the device memory budget");
     JFEM_HIP(hipMalloc(&V, sizeof(double) * (size_t)maxv * n));
-    DevBuf t(n), r0(n), q(n), tq(n), w(n), fdev(maxv), part((long)maxv * RED_BLOCKS);
+    DevBuf<double> t(n), r0(n), q(n), tq(n), w(n), fdev(maxv), part((long)maxv * RED_BLOCKS);
     std::vector<double> f(maxv), H((size_t)ld * ld), gamma(ld), cs(ld), sn(ld), alpha(ld);
     int restarts = 0, iterations = 0, cycles = 0;
     const int nb = kgrid(n);
@@ -490,6 +484,198 @@ static void solve_gmres(jfem_mat* A, jfem_mg* M, const double* b, double* x, dou
     info->resnorm = residual >= 0 ? residual : true_resnorm(A, b, x, t.p);
 }
 
+
+// ------------------------------------------------- Stokes: block-preconditioned FGMRES
+// velocity component c of a mixed vector (node order) <-> scalar space vector (DOF order)
+__global__ void k_gather_rank(long n, const int* __restrict__ rank, const double* __restrict__ src,
+                              double* __restrict__ dst)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        dst[i] = src[rank[i]];
+}
+
+// dst[rank[i]] = z[i], or r[i] on a Dirichlet row (the identity block of the eliminated system)
+__global__ void k_scatter_rank(long n, const int* __restrict__ rank, const double* __restrict__ z,
+                               const double* __restrict__ r, const unsigned char* __restrict__ bnd,
+                               double* __restrict__ dst)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        dst[rank[i]] = bnd && bnd[i] ? r[i] : z[i];
+}
+
+// z_p = s * (B z_u - r_p)
+__global__ void k_schur_diag(long n, const double* __restrict__ s, const double* __restrict__ bz,
+                             const double* __restrict__ r, double* __restrict__ z)
+{
+#pragma clang fp contract(off)
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        z[i] = s[i] * (bz[i] - r[i]);
+}
+
+__global__ void k_row_slice(long n, const int* __restrict__ rowptr, long first, int* __restrict__ out)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i <= n; i += (long)gridDim.x * blockDim.x)
+        out[i] = rowptr[first + i] - rowptr[first];
+}
+
+struct StokesPrec {
+    jfem_mat* A = nullptr;
+    jfem_mg* M = nullptr;
+    int d = 0;
+    long nv = 0, np = 0;
+    const double* ps = nullptr;
+    jfem_mat* B = nullptr;       // copy of the pressure rows of A (np rows over all n columns)
+    DevBuf<int> rank;            // lexicographic node rank of every velocity DOF
+    DevBuf<double> rs, zs, bz;
+    StokesPrec(jfem_mat* A_, jfem_mg* M_, const double* ps_, long np_) : A(A_), M(M_), np(np_), ps(ps_)
+    {
+        nv = mg_size(M);
+        const long n = A->rows;
+        JFEM_CHECK(nv > 0 && np > 0 && (n - np) % nv == 0, JFEM_ERR_ARG, "system size is not dim * velocity + pressure");
+        d = (int)((n - np) / nv);
+        const Grid& g = mg_finest_grid(M);
+        JFEM_CHECK(d == g.d, JFEM_ERR_ARG, "system size is not dim * velocity + pressure");
+        jfem_ctx* c = A->ctx;
+        rank.alloc(nv);
+        rs.alloc(nv);
+        zs.alloc(nv);
+        bz.alloc(np);
+        jfem_space vs;
+        vs.ctx = c;
+        vs.grid = g;
+        space_lex_rank(&vs, rank);
+        const long first = d * nv;
+        int base = 0;
+        JFEM_HIP(hipMemcpyAsync(&base, A->rowptr + first, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        JFEM_HIP(hipStreamSynchronize(c->stream));
+        B = new jfem_mat();
+        try {
+            B->ctx = c;
+            B->rows = np;
+            B->cols = n;
+            B->nnz = A->nnz - base;
+            JFEM_HIP(hipMalloc(&B->rowptr, sizeof(int) * (np + 1)));
+            JFEM_HIP(hipMalloc(&B->colidx, sizeof(int) * (B->nnz + NNZ_PAD)));
+            JFEM_HIP(hipMalloc(&B->vals, sizeof(double) * (B->nnz + NNZ_PAD)));
+            k_row_slice<<<grid_for(np + 1, 256, 4096), 256, 0, c->stream>>>(np, A->rowptr, first, B->rowptr);
+            JFEM_HIP(hipGetLastError());
+            JFEM_HIP(hipMemcpyAsync(B->colidx, A->colidx + base, sizeof(int) * B->nnz, hipMemcpyDeviceToDevice, c->stream));
+            JFEM_HIP(hipMemcpyAsync(B->vals, A->vals + base, sizeof(double) * B->nnz, hipMemcpyDeviceToDevice, c->stream));
+            B->lanes = choose_lanes((double)B->nnz / np);
+            mat_build_blocks(B);
+        } catch (...) {
+            mat_free(B);
+            throw;
+        }
+    }
+    ~StokesPrec() { mat_free(B); }
+    // z = P^-1 r
+    void apply(const double* r, double* z)
+    {
+        jfem_ctx* c = A->ctx;
+        const int gv = kgrid(nv);
+        const unsigned char* bnd = mg_finest_bnd(M);
+        for (int k = 0; k < d; k++) {
+            k_gather_rank<<<gv, KB, 0, c->stream>>>(nv, rank, r + k * nv, rs);
+            mg_apply(M, rs, zs);
+            k_scatter_rank<<<gv, KB, 0, c->stream>>>(nv, rank, zs, rs, bnd, z + k * nv);
+        }
+        vec_fill(c, z + d * nv, 0.0, np);
+        spmv(B, z, bz);
+        k_schur_diag<<<kgrid(np), KB, 0, c->stream>>>(np, ps, bz, r + d * nv, z + d * nv);
+        JFEM_HIP(hipGetLastError());
+    }
+};
+
+// right-preconditioned flexible GMRES (Saad 1993) with classical Gram-Schmidt
+// applied twice and Givens rotations
+static void solve_stokes_fgmres(jfem_mat* A, StokesPrec& P, const double* b, double* x, double rtol, int restart,
+                                long maxit, jfem_solve_info* info, double* hist)
+{
+#pragma clang fp contract(off)
+    jfem_ctx* c = A->ctx;
+    const long n = A->rows;
+    JFEM_CHECK(restart >= 1 && restart <= 1000, JFEM_ERR_ARG, "bad restart length");
+    const int m = restart;
+    JFEM_CHECK((2.0 * m + 1) * n * 8 < 200e9, JFEM_ERR_NOMEM, "Krylov basis exceeds the device memory budget");
+    DevBuf<double> V((size_t)(m + 1) * n), Z((size_t)m * n), w(n), q(n), t(n), fdev(m + 1),
+        part((size_t)(m + 1) * RED_BLOCKS);
+    const int nb = kgrid(n);
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), f(m + 1), f2(m + 1);
+    auto Hij = [&](int i, int j) -> double& { return H[(size_t)i * m + j]; };
+    const double bn = std::sqrt(vec_dot(c, n, b, b));
+    long it = 0;
+    double res = 0;
+    if (bn == 0) {
+        vec_fill(c, x, 0.0, n);
+        info->iterations = 0;
+        info->resnorm = 0;
+        return;
+    }
+    const double target = rtol * bn;
+    for (;;) {
+        spmv(A, x, t);
+        k_sub<<<nb, KB, 0, c->stream>>>(n, b, t, w);
+        JFEM_HIP(hipGetLastError());
+        res = std::sqrt(vec_dot(c, n, w, w));
+        if (res <= target || it >= maxit) break;
+        k_scale<<<nb, KB, 0, c->stream>>>(n, 1.0 / res, w, V.p);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = res;
+        int jn = 0;   // columns of this cycle
+        bool stop = false;
+        for (int j = 0; j < m && !stop; j++) {
+            double* vj = V.p + (size_t)j * n;
+            double* zj = Z.p + (size_t)j * n;
+            P.apply(vj, zj);
+            spmv(A, zj, q);
+            // h = V^T q, w = q - V h; h2 = V^T w, q = w - V h2
+            k_multidot<<<nb, KB, 0, c->stream>>>(n, j + 1, V.p, n, q, part, nb);
+            k_sum_partials<<<1, KB, 0, c->stream>>>(j + 1, nb, part, fdev);
+            k_gs_sub<<<nb, KB, 0, c->stream>>>(n, j + 1, V.p, n, fdev, q, w);
+            JFEM_HIP(hipMemcpyAsync(f.data(), fdev, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, c->stream));
+            k_multidot<<<nb, KB, 0, c->stream>>>(n, j + 1, V.p, n, w, part, nb);
+            k_sum_partials<<<1, KB, 0, c->stream>>>(j + 1, nb, part, fdev);
+            k_gs_sub<<<nb, KB, 0, c->stream>>>(n, j + 1, V.p, n, fdev, w, q);
+            JFEM_HIP(hipMemcpyAsync(f2.data(), fdev, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, c->stream));
+            JFEM_HIP(hipGetLastError());
+            const double hn = std::sqrt(vec_dot(c, n, q, q));   // synchronises
+            for (int i = 0; i <= j; i++) Hij(i, j) = f[i] + f2[i];
+            Hij(j + 1, j) = hn;
+            for (int i = 0; i < j; i++) {
+                const double a1 = cs[i] * Hij(i, j) + sn[i] * Hij(i + 1, j);
+                const double a2 = -sn[i] * Hij(i, j) + cs[i] * Hij(i + 1, j);
+                Hij(i, j) = a1;
+                Hij(i + 1, j) = a2;
+            }
+            const double hjj = Hij(j, j), hj1 = Hij(j + 1, j);
+            const double beta = std::hypot(hjj, hj1);
+            JFEM_CHECK(beta > 0 && std::isfinite(beta), JFEM_ERR_STATE, "FGMRES breakdown (zero Hessenberg column)");
+            cs[j] = hjj / beta;
+            sn[j] = hj1 / beta;
+            Hij(j, j) = beta;
+            Hij(j + 1, j) = 0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            it++;
+            jn = j + 1;
+            if (hist) hist[it - 1] = std::fabs(g[j + 1]) / bn;
+            if (std::fabs(g[j + 1]) <= target || it >= maxit || hn == 0) stop = true;
+            else if (j + 1 < m) k_scale<<<nb, KB, 0, c->stream>>>(n, 1.0 / hn, q, V.p + (size_t)(j + 1) * n);
+        }
+        // y = H^-1 g (upper triangular), x += Z y
+        for (int i = jn - 1; i >= 0; i--) {
+            double s = g[i];
+            for (int k = i + 1; k < jn; k++) s -= Hij(i, k) * y[k];
+            y[i] = s / Hij(i, i);
+        }
+        JFEM_HIP(hipMemcpyAsync(fdev, y.data(), sizeof(double) * jn, hipMemcpyHostToDevice, c->stream));
+        k_lincomb<<<nb, KB, 0, c->stream>>>(n, jn, Z.p, n, fdev, x);
+        JFEM_HIP(hipGetLastError());
+    }
+    info->iterations = (int)it;
+    info->resnorm = res;
+}
 }  // namespace jfem
 
 using namespace jfem;
@@ -557,6 +743,22 @@ int jfem_solve_gmres(jfem_mat* A, jfem_mg* M, const jfem_vec* b, jfem_vec* x, do
     jfem_solve_info tmp;
     jfem_solve_info* inf = info ? info : &tmp;
     timed(A->ctx, inf, [&] { solve_gmres(A, M, b->d, x->d, tol, restart, max_cycles, inf); });
+    API_END
+}
+
+int jfem_solve_stokes_fgmres(jfem_mat* A, jfem_mg* velocity_mg, const jfem_vec* pressure_scale, const jfem_vec* b,
+                             jfem_vec* x, double rtol, int restart, long maxit, jfem_solve_info* info,
+                             double* res_hist)
+{
+    API_BEGIN
+    JFEM_CHECK(A && velocity_mg && pressure_scale && b && x, JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(A->rows == A->cols, JFEM_ERR_ARG, "Object is not square");
+    JFEM_CHECK(b->n == A->rows && x->n == A->rows, JFEM_ERR_ARG, "Incompatible sizes");
+    JFEM_CHECK(rtol > 0 && maxit >= 0, JFEM_ERR_ARG, "bad tolerance or iteration limit");
+    jfem_solve_info tmp;
+    jfem_solve_info* inf = info ? info : &tmp;
+    StokesPrec P(A, velocity_mg, pressure_scale->d, pressure_scale->n);
+    timed(A->ctx, inf, [&] { solve_stokes_fgmres(A, P, b->d, x->d, rtol, restart, maxit, inf, res_hist); });
     API_END
 }
 

@@ -608,6 +608,10 @@ void mg_apply(jfem_mg* G, const double* r, double* z)
 
 long mg_size(const jfem_mg* G) { return G->lev.back().A->rows; }
 
+const Grid& mg_finest_grid(const jfem_mg* G) { return G->lev.back().grid; }
+
+const unsigned char* mg_finest_bnd(const jfem_mg* G) { return G->lev.back().bnd; }
+
 // ------------------------------------------------ setup
 // textbook LU with partial pivoting (Golub & Van Loan 3.4.1) and the explicit
 // inverse of the coarsest matrix
@@ -732,21 +736,16 @@ static void galerkin_level(jfem_mg* G, int lf)
     const GridDev gc = to_dev(C.grid);
     int ncol = 1;
     for (int a = 0; a < C.grid.d; a++) ncol *= 2 * C.grid.k + 1;
-    double *Y = nullptr, *v = nullptr, *w = nullptr, *z = nullptr;
-    JFEM_HIP(hipMalloc(&Y, sizeof(double) * nc * ncol));
-    JFEM_HIP(hipMalloc(&v, sizeof(double) * nc));
-    JFEM_HIP(hipMalloc(&w, sizeof(double) * nf));
-    JFEM_HIP(hipMalloc(&z, sizeof(double) * nf));
+    DevBuf<double> Y((size_t)nc * ncol), v(nc), w(nf), z(nf);
     for (int col = 0; col < ncol; col++) {
         k_color_indicator<<<grid_for(nc, 256, 16384), 256, 0, st>>>(gc, nc, col, v);
         tp_transfer<double>(st, F, C, false, v, w, nullptr, 0);   // w = P v
         spmv(F.A, w, z);                                          // z = A w
-        tp_transfer<double>(st, F, C, true, z, Y + (long)col * nc, nullptr, 0);   // P^T z
+        tp_transfer<double>(st, F, C, true, z, Y.p + (long)col * nc, nullptr, 0);   // P^T z
     }
     k_galerkin_fill<<<grid_for(nc, 256, 16384), 256, 0, st>>>(gc, nc, C.A->rowptr, C.A->colidx, Y, C.A->vals);
     JFEM_HIP(hipGetLastError());
     JFEM_HIP(hipStreamSynchronize(st));
-    for (double* p : {Y, v, w, z}) (void)hipFree(p);
 }
 
 static jfem_mg* create_mg(jfem_space* const* spaces, int nlev, int nint, const int* kinds, const double* weights,
@@ -831,6 +830,7 @@ static jfem_mg* create_mg(jfem_space* const* spaces, int nlev, int nint, const i
         const int n0 = (int)C.A->rows;
         std::vector<int> rp(n0 + 1), ci(C.A->nnz);
         std::vector<double> cv(C.A->nnz), dense((size_t)n0 * n0, 0.0);
+        JFEM_HIP(hipStreamSynchronize(G->ctx->stream));   // blocking copies below do not wait for that stream
         JFEM_HIP(hipMemcpy(rp.data(), C.A->rowptr, sizeof(int) * (n0 + 1), hipMemcpyDeviceToHost));
         JFEM_HIP(hipMemcpy(ci.data(), C.A->colidx, sizeof(int) * C.A->nnz, hipMemcpyDeviceToHost));
         JFEM_HIP(hipMemcpy(cv.data(), C.A->vals, sizeof(double) * C.A->nnz, hipMemcpyDeviceToHost));

@@ -1188,6 +1188,7 @@ int jfem_nn_get_params(jfem_nn* nn, float* out)
 {
     API_BEGIN
     JFEM_CHECK(nn && out, JFEM_ERR_ARG, "null argument");
+    JFEM_HIP(hipStreamSynchronize(nn->ctx->stream));   // work on the non-blocking context stream
     JFEM_HIP(hipMemcpy(out, nn->w, sizeof(float) * nn->nparams, hipMemcpyDeviceToHost));
     API_END
 }
@@ -1196,6 +1197,7 @@ int jfem_nn_set_params(jfem_nn* nn, const float* in)
 {
     API_BEGIN
     JFEM_CHECK(nn && in, JFEM_ERR_ARG, "null argument");
+    JFEM_HIP(hipStreamSynchronize(nn->ctx->stream));   // work on the non-blocking context stream
     JFEM_HIP(hipMemcpy(nn->w, in, sizeof(float) * nn->nparams, hipMemcpyHostToDevice));
     nn_refresh(nn);
     JFEM_HIP(hipStreamSynchronize(nn->ctx->stream));
@@ -1246,6 +1248,7 @@ int jfem_nn_save(jfem_nn* nn, const char* path)
 {
     API_BEGIN
     JFEM_CHECK(nn && path, JFEM_ERR_ARG, "null argument");
+    JFEM_HIP(hipStreamSynchronize(nn->ctx->stream));   // work on the non-blocking context stream
     std::vector<float> w(nn->nparams), m(nn->nparams), v(nn->nparams);
     JFEM_HIP(hipMemcpy(w.data(), nn->w, sizeof(float) * nn->nparams, hipMemcpyDeviceToHost));
     JFEM_HIP(hipMemcpy(m.data(), nn->m, sizeof(float) * nn->nparams, hipMemcpyDeviceToHost));

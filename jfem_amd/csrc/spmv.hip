@@ -286,9 +286,105 @@ void mat_diag(jfem_mat* A, double* out)
     JFEM_HIP(hipGetLastError());
 }
 
+// row length plus one for a fixed row without a stored diagonal
+__global__ void k_count_with_diag(long n, const int* __restrict__ rowptr, const int* __restrict__ cols,
+                                  const unsigned char* __restrict__ fixed, int* __restrict__ counts,
+                                  int* __restrict__ nmissing)
+{
+    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) {
+        const int s = rowptr[r], e = rowptr[r + 1];
+        int add = 0;
+        if (fixed[r]) {
+            add = 1;
+            for (int p = s; p < e; p++) if (cols[p] == r) { add = 0; break; }
+        }
+        counts[r] = e - s + add;
+        if (add) atomicAdd(nmissing, 1);
+    }
+}
+
+// copy of the rows with an explicit 0.0 diagonal entry inserted where the
+// new row pointer reserves one (columns stay ascending)
+__global__ void k_insert_diag(long n, const int* __restrict__ rowptr, const int* __restrict__ cols,
+                              const double* __restrict__ vals, const int* __restrict__ nrowptr, int* __restrict__ ncols,
+                              double* __restrict__ nvals)
+{
+    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) {
+        const int s = rowptr[r], e = rowptr[r + 1];
+        int o = nrowptr[r];
+        const bool ins = nrowptr[r + 1] - o != e - s;
+        bool done = !ins;
+        for (int p = s; p < e; p++) {
+            if (!done && cols[p] > r) { ncols[o] = (int)r; nvals[o] = 0.0; o++; done = true; }
+            ncols[o] = cols[p];
+            nvals[o] = vals[p];
+            o++;
+        }
+        if (!done) { ncols[o] = (int)r; nvals[o] = 0.0; }
+    }
+}
+
+// SparseMatrix.set(1, i, i) of overWriteValue (M/basic/AcceptsMatrixBoundaryValues.java:25-43)
+// creates the diagonal entry of a fixed row that has none: the rows are
+// rebuilt with those entries inserted before the elimination
+static void insert_fixed_diagonals(jfem_mat* A, const unsigned char* fixed_dev)
+{
+    jfem_ctx* c = A->ctx;
+    const long n = A->rows;
+    DevBuf<int> counts(n), nmiss(1);
+    JFEM_HIP(hipMemsetAsync(nmiss, 0, sizeof(int), c->stream));
+    k_count_with_diag<<<grid_for(n, 256, 16384), 256, 0, c->stream>>>(n, A->rowptr, A->colidx, fixed_dev, counts,
+                                                                       nmiss);
+    JFEM_HIP(hipGetLastError());
+    int h = 0;
+    JFEM_HIP(hipMemcpyAsync(&h, nmiss, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    JFEM_HIP(hipStreamSynchronize(c->stream));
+    if (h == 0) return;
+    int* nrowptr = nullptr;
+    int* ncols = nullptr;
+    double* nvals = nullptr;
+    try {
+        JFEM_HIP(hipMalloc(&nrowptr, sizeof(int) * (n + 1)));
+        const long nnz = scan_counts(c, counts, n, nrowptr);
+        JFEM_HIP(hipMalloc(&ncols, sizeof(int) * (nnz + NNZ_PAD)));
+        JFEM_HIP(hipMalloc(&nvals, sizeof(double) * (nnz + NNZ_PAD)));
+        JFEM_HIP(hipMemsetAsync(ncols, 0, sizeof(int) * (nnz + NNZ_PAD), c->stream));
+        JFEM_HIP(hipMemsetAsync(nvals, 0, sizeof(double) * (nnz + NNZ_PAD), c->stream));
+        k_insert_diag<<<grid_for(n, 256, 16384), 256, 0, c->stream>>>(n, A->rowptr, A->colidx, A->vals, nrowptr, ncols,
+                                                                       nvals);
+        JFEM_HIP(hipGetLastError());
+        JFEM_HIP(hipStreamSynchronize(c->stream));
+        (void)hipFree(A->rowptr);
+        (void)hipFree(A->colidx);
+        (void)hipFree(A->vals);
+        A->rowptr = nrowptr;
+        A->colidx = ncols;
+        A->vals = nvals;
+        nrowptr = nullptr;
+        ncols = nullptr;
+        nvals = nullptr;
+        A->nnz = nnz;
+    } catch (...) {
+        (void)hipFree(nrowptr);
+        (void)hipFree(ncols);
+        (void)hipFree(nvals);
+        throw;
+    }
+    (void)hipFree(A->diag);
+    A->diag = nullptr;
+    A->lanes = choose_lanes(n > 0 ? (double)A->nnz / n : 1.0);
+    mat_build_blocks(A);
+    if (A->vals32) {
+        (void)hipFree(A->vals32);
+        A->vals32 = nullptr;
+        mat_make_f32(A);
+    }
+}
+
 void apply_dirichlet_mask(jfem_mat* A, double* rhs, const unsigned char* fixed_dev, const double* g_dev)
 {
     jfem_ctx* c = A->ctx;
+    insert_fixed_diagonals(A, fixed_dev);
     int* flag = nullptr;
     JFEM_HIP(hipMalloc(&flag, sizeof(int)));
     JFEM_HIP(hipMemsetAsync(flag, 0, sizeof(int), c->stream));
