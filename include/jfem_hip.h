@@ -196,17 +196,26 @@ int jfem_solve_fcg(jfem_mat *A, jfem_mg *M, const jfem_vec *b, jfem_vec *x, doub
 /* Stokes saddle-point system of jfem_th_assemble (after Dirichlet data and the pressure
  * pin): right-preconditioned flexible GMRES(restart) with the block lower-triangular
  * preconditioner  z_u = M^-1 r_u (velocity_mg once per component, in the scalar space's
- * DOF order),  z_p = pressure_scale * (B z_u - r_p)  with B the pressure rows of A.
- * pressure_scale: ndofs(pressure) entries, nu / lumped mass for the Schur complement
- * S ~ M_p / nu, and -1 on a pinned (identity) pressure row.  Stops when
- * ||b - A x||_2 <= rtol ||b||_2 (the Arnoldi estimate, confirmed on the true residual at
- * each restart) or after maxit iterations.  The reference solves these systems with
- * GMRES2 around a monolithic multigrid (Examples/src/StokesMG2.java:26-129); this
- * preconditioner is not in the reference.  res_hist (nullable, maxit entries): relative
- * residual estimate per iteration. */
-int jfem_solve_stokes_fgmres(jfem_mat *A, jfem_mg *velocity_mg, const jfem_vec *pressure_scale, const jfem_vec *b,
-                             jfem_vec *x, double rtol, int restart, long maxit, jfem_solve_info *info,
-                             double *res_hist);
+ * DOF order),  z_p = S^-1 (B z_u - r_p)  with B the pressure rows of A and the Schur
+ * complement S ~ M_p / nu.  pressure_scale: ndofs(pressure) entries in the mixed order,
+ * 1 / (row sum of M_p / nu) (the lumped mass), and -1 on a pinned (identity) pressure row.
+ * S^-1 is the lumped inverse, or with opts->pressure_mass (M_p / nu on the scalar
+ * pressure space, DOF order) mass_its Chebyshev steps preconditioned by the lumped mass
+ * on [mass_lmin, mass_lmax] (Q1: 3^-dim, 1).  Stops when ||b - A x||_2 <= rtol ||b||_2 (the
+ * Arnoldi estimate, confirmed on the true residual at each restart) or after opts->maxit
+ * iterations.  The reference solves these systems with GMRES2 around a monolithic
+ * multigrid (Examples/src/StokesMG2.java:26-129); this preconditioner is not in the
+ * reference.  res_hist (nullable, maxit entries): relative residual estimate per iteration. */
+typedef struct {
+    int restart;                   /* FGMRES restart length */
+    long maxit;                    /* iteration limit */
+    jfem_mat *pressure_mass;       /* nullable: M_p / nu (scalar pressure space, DOF order) */
+    int mass_its;                  /* Chebyshev steps on it (0 = lumped inverse only) */
+    double mass_lmin, mass_lmax;   /* spectrum of (lumped M_p)^-1 M_p */
+} jfem_stokes_opts;
+int jfem_solve_stokes_fgmres(jfem_mat *A, jfem_mg *velocity_mg, const jfem_vec *pressure_scale,
+                             const jfem_stokes_opts *opts, const jfem_vec *b, jfem_vec *x, double rtol,
+                             jfem_solve_info *info, double *res_hist);
 
 /* ---- geometric multigrid (MGPreconditionerInterface / MGPreconditionerSpace,
  *      M/multigrid/MGPreconditionerInterface.java:125-379) -------------------- */

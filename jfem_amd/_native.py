@@ -48,6 +48,11 @@ class TrainOpts(C.Structure):
                 ("mode", C.c_int)]
 
 
+class StokesOpts(C.Structure):
+    _fields_ = [("restart", C.c_int), ("maxit", C.c_long), ("pressure_mass", C.c_void_p), ("mass_its", C.c_int),
+                ("mass_lmin", C.c_double), ("mass_lmax", C.c_double)]
+
+
 class MgOpts(C.Structure):
     _fields_ = [("smoother", C.c_int), ("runs", C.c_int), ("omega", C.c_double), ("degree", C.c_int),
                 ("cheb_ratio", C.c_double), ("cheb_safety", C.c_double), ("power_its", C.c_int),
@@ -102,7 +107,7 @@ _SIGS = {
     "jfem_solve_bicgstab": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_long, C.POINTER(SolveInfo)]),
     "jfem_solve_gmres": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, C.POINTER(SolveInfo)]),
     "jfem_solve_fcg": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_long, C.POINTER(SolveInfo), _dp]),
-    "jfem_solve_stokes_fgmres": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_long,
+    "jfem_solve_stokes_fgmres": (C.c_int, [_vp, _vp, _vp, C.POINTER(StokesOpts), _vp, _vp, C.c_double,
                                             C.POINTER(SolveInfo), _dp]),
     "jfem_mg_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, _ip, _dp, _dp, C.c_int, C.POINTER(MgOpts),
                                  C.POINTER(_vp)]),

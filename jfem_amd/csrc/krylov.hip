@@ -343,30 +343,41 @@ struct JSum {
     }
 };
 
-// multi-dot f[j] = V_j . q for j < nv (partials [j][block])
+// multi-dot f[j] = V_j . q for j < nv (partials [j][block]); eight vectors per
+// sweep so that q is read nv / 8 times (each dot keeps its per-thread order)
 __global__ __launch_bounds__(KB) void k_multidot(long n, int nv, const double* __restrict__ V, long ldv,
                                                  const double* __restrict__ q, double* __restrict__ partials, int nb)
 {
+    constexpr int CH = 8;
     __shared__ double sh[KB / 64];
-    for (int j = 0; j < nv; j++) {
-        const double* __restrict__ vj = V + (long)j * ldv;
-        double acc = 0;
-        for (long i = blockIdx.x * (long)KB + threadIdx.x; i < n; i += (long)gridDim.x * KB) acc += vj[i] * q[i];
-        block_partial<KB>(acc, sh, &partials[(long)j * nb + blockIdx.x]);
-        __syncthreads();
+    for (int j0 = 0; j0 < nv; j0 += CH) {
+        const int m = nv - j0 < CH ? nv - j0 : CH;
+        double acc[CH];
+#pragma unroll
+        for (int t = 0; t < CH; t++) acc[t] = 0;
+        for (long i = blockIdx.x * (long)KB + threadIdx.x; i < n; i += (long)gridDim.x * KB) {
+            const double qi = q[i];
+#pragma unroll
+            for (int t = 0; t < CH; t++)
+                if (t < m) acc[t] += V[(long)(j0 + t) * ldv + i] * qi;
+        }
+        for (int t = 0; t < m; t++) {
+            block_partial<KB>(acc[t], sh, &partials[(long)(j0 + t) * nb + blockIdx.x]);
+            __syncthreads();
+        }
     }
 }
 
+// out[j] = sum of the nb partials of vector j; one workgroup per vector
 __global__ __launch_bounds__(KB) void k_sum_partials(int nv, int nb, const double* __restrict__ partials,
                                                      double* __restrict__ out)
 {
     __shared__ double sh[KB / 64];
-    for (int j = 0; j < nv; j++) {
-        double acc = 0;
-        for (int i = threadIdx.x; i < nb; i += KB) acc += partials[(long)j * nb + i];
-        block_partial<KB>(acc, sh, &out[j]);
-        __syncthreads();
-    }
+    const int j = blockIdx.x;
+    if (j >= nv) return;
+    double acc = 0;
+    for (int i = threadIdx.x; i < nb; i += KB) acc += partials[(long)j * nb + i];
+    block_partial<KB>(acc, sh, &out[j]);
 }
 
 // w = q - sum_j V_j f_j   (orthogonalPartInSubspace summed left to right from 0)
@@ -435,7 +446,7 @@ static void solve_gmres(jfem_mat* A, jfem_mg* M, const double* b, double* x, dou
                 const double* vj = V + (long)j * ldv;
                 if (M) { spmv(A, vj, tq.p); mg_apply(M, tq.p, q.p); } else spmv(A, vj, q.p);
                 k_multidot<<<nb, KB, 0, c->stream>>>(n, nv, V, ldv, q.p, part.p, nb);
-                k_sum_partials<<<1, KB, 0, c->stream>>>(nv, nb, part.p, fdev.p);
+                k_sum_partials<<<nv, KB, 0, c->stream>>>(nv, nb, part.p, fdev.p);
                 k_gs_sub<<<nb, KB, 0, c->stream>>>(n, nv, V, ldv, fdev.p, q.p, w.p);
                 JFEM_HIP(hipMemcpyAsync(f.data(), fdev.p, sizeof(double) * nv, hipMemcpyDeviceToHost, c->stream));
                 const double wn = std::sqrt(vec_dot(c, n, w.p, w.p));   // synchronises
@@ -512,6 +523,56 @@ __global__ void k_schur_diag(long n, const double* __restrict__ s, const double*
         z[i] = s[i] * (bz[i] - r[i]);
 }
 
+// t = B z_u - r_p (in place in bz)
+__global__ void k_sub_inplace(long n, const double* __restrict__ r, double* __restrict__ bz)
+{
+#pragma clang fp contract(off)
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        bz[i] = bz[i] - r[i];
+}
+
+// lumped-mass inverse in DOF order: pinv[i] = s[rank[i]] for free rows, 0 for pinned ones (s < 0)
+__global__ void k_gather_pos(long n, const int* __restrict__ rank, const double* __restrict__ s,
+                             double* __restrict__ pinv)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double v = s[rank[i]];
+        pinv[i] = v > 0 ? v : 0.0;
+    }
+}
+
+// first Chebyshev step from y = 0: d = pinv * t / theta, y = d
+__global__ void k_mass_cheb0(long n, double inv_theta, const double* __restrict__ pinv, const double* __restrict__ t,
+                             double* __restrict__ d, double* __restrict__ y)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double v = pinv[i] * t[i] * inv_theta;
+        d[i] = v;
+        y[i] = v;
+    }
+}
+
+// d = a d + b pinv (t - q), y += d   (q = M y)
+__global__ void k_mass_cheb(long n, double a, double b, const double* __restrict__ pinv, const double* __restrict__ t,
+                            const double* __restrict__ q, double* __restrict__ d, double* __restrict__ y)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double v = a * d[i] + b * pinv[i] * (t[i] - q[i]);
+        d[i] = v;
+        y[i] += v;
+    }
+}
+
+// node order <- DOF order, pinned rows (s < 0) keep the identity block: z = -t
+__global__ void k_scatter_mass(long n, const int* __restrict__ rank, const double* __restrict__ y,
+                               const double* __restrict__ s, const double* __restrict__ t, double* __restrict__ z)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const long j = rank[i];
+        z[j] = s[j] < 0 ? -t[j] : y[i];
+    }
+}
+
 __global__ void k_row_slice(long n, const int* __restrict__ rowptr, long first, int* __restrict__ out)
 {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i <= n; i += (long)gridDim.x * blockDim.x)
@@ -527,7 +588,14 @@ struct StokesPrec {
     jfem_mat* B = nullptr;       // copy of the pressure rows of A (np rows over all n columns)
     DevBuf<int> rank;            // lexicographic node rank of every velocity DOF
     DevBuf<double> rs, zs, bz;
-    StokesPrec(jfem_mat* A_, jfem_mg* M_, const double* ps_, long np_) : A(A_), M(M_), np(np_), ps(ps_)
+    // optional Schur complement solve: Chebyshev steps on the pressure mass matrix (DOF order)
+    jfem_mat* Mp = nullptr;
+    int mits = 0;
+    double mlo = 0, mhi = 1;
+    DevBuf<int> prank;           // lexicographic node rank of every pressure DOF
+    DevBuf<double> pinv, tp, yp, dp, qp;
+    StokesPrec(jfem_mat* A_, jfem_mg* M_, const double* ps_, long np_, const jfem_stokes_opts* o)
+        : A(A_), M(M_), np(np_), ps(ps_)
     {
         nv = mg_size(M);
         const long n = A->rows;
@@ -567,6 +635,29 @@ struct StokesPrec {
             mat_free(B);
             throw;
         }
+        if (o && o->pressure_mass && o->mass_its > 0) {
+            Mp = o->pressure_mass;
+            mits = o->mass_its;
+            mlo = o->mass_lmin;
+            mhi = o->mass_lmax;
+            JFEM_CHECK(Mp->rows == np && Mp->cols == np, JFEM_ERR_ARG, "pressure mass matrix has the wrong size");
+            JFEM_CHECK(mlo > 0 && mhi > mlo, JFEM_ERR_ARG, "bad mass spectrum bounds");
+            prank.alloc(np);
+            pinv.alloc(np);
+            tp.alloc(np);
+            yp.alloc(np);
+            dp.alloc(np);
+            qp.alloc(np);
+            jfem_space ps_space;
+            ps_space.ctx = c;
+            ps_space.grid = g;
+            ps_space.grid.k = g.k - 1;
+            JFEM_CHECK(ps_space.grid.k >= 1 && ps_space.grid.ndofs() == np, JFEM_ERR_ARG,
+                       "pressure space is not the Q(k-1) space of the velocity grid");
+            space_lex_rank(&ps_space, prank);
+            k_gather_pos<<<kgrid(np), KB, 0, c->stream>>>(np, prank, ps, pinv);
+            JFEM_HIP(hipGetLastError());
+        }
     }
     ~StokesPrec() { mat_free(B); }
     // z = P^-1 r
@@ -582,7 +673,26 @@ struct StokesPrec {
         }
         vec_fill(c, z + d * nv, 0.0, np);
         spmv(B, z, bz);
-        k_schur_diag<<<kgrid(np), KB, 0, c->stream>>>(np, ps, bz, r + d * nv, z + d * nv);
+        if (!Mp) {
+            k_schur_diag<<<kgrid(np), KB, 0, c->stream>>>(np, ps, bz, r + d * nv, z + d * nv);
+            JFEM_HIP(hipGetLastError());
+            return;
+        }
+        // z_p = M^-1 (B z_u - r_p) by Chebyshev iteration (Saad, Iterative Methods, Alg. 12.1)
+        // preconditioned by the lumped mass, on [mlo, mhi]; pinned rows keep z = -t
+        const int gp = kgrid(np);
+        k_sub_inplace<<<gp, KB, 0, c->stream>>>(np, r + d * nv, bz);
+        k_gather_rank<<<gp, KB, 0, c->stream>>>(np, prank, bz, tp);
+        const double theta = 0.5 * (mhi + mlo), delta = 0.5 * (mhi - mlo), sigma = theta / delta;
+        k_mass_cheb0<<<gp, KB, 0, c->stream>>>(np, 1.0 / theta, pinv, tp, dp, yp);
+        double rho = 1.0 / sigma;
+        for (int k = 1; k < mits; k++) {
+            spmv(Mp, yp, qp);
+            const double rn = 1.0 / (2.0 * sigma - rho);
+            k_mass_cheb<<<gp, KB, 0, c->stream>>>(np, rn * rho, 2.0 * rn / delta, pinv, tp, qp, dp, yp);
+            rho = rn;
+        }
+        k_scatter_mass<<<gp, KB, 0, c->stream>>>(np, prank, yp, ps, bz, z + d * nv);
         JFEM_HIP(hipGetLastError());
     }
 };
@@ -631,11 +741,11 @@ static void solve_stokes_fgmres(jfem_mat* A, StokesPrec& P, const double* b, dou
             spmv(A, zj, q);
             // h = V^T q, w = q - V h; h2 = V^T w, q = w - V h2
             k_multidot<<<nb, KB, 0, c->stream>>>(n, j + 1, V.p, n, q, part, nb);
-            k_sum_partials<<<1, KB, 0, c->stream>>>(j + 1, nb, part, fdev);
+            k_sum_partials<<<j + 1, KB, 0, c->stream>>>(j + 1, nb, part, fdev);
             k_gs_sub<<<nb, KB, 0, c->stream>>>(n, j + 1, V.p, n, fdev, q, w);
             JFEM_HIP(hipMemcpyAsync(f.data(), fdev, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, c->stream));
             k_multidot<<<nb, KB, 0, c->stream>>>(n, j + 1, V.p, n, w, part, nb);
-            k_sum_partials<<<1, KB, 0, c->stream>>>(j + 1, nb, part, fdev);
+            k_sum_partials<<<j + 1, KB, 0, c->stream>>>(j + 1, nb, part, fdev);
             k_gs_sub<<<nb, KB, 0, c->stream>>>(n, j + 1, V.p, n, fdev, w, q);
             JFEM_HIP(hipMemcpyAsync(f2.data(), fdev, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, c->stream));
             JFEM_HIP(hipGetLastError());
@@ -746,19 +856,19 @@ int jfem_solve_gmres(jfem_mat* A, jfem_mg* M, const jfem_vec* b, jfem_vec* x, do
     API_END
 }
 
-int jfem_solve_stokes_fgmres(jfem_mat* A, jfem_mg* velocity_mg, const jfem_vec* pressure_scale, const jfem_vec* b,
-                             jfem_vec* x, double rtol, int restart, long maxit, jfem_solve_info* info,
-                             double* res_hist)
+int jfem_solve_stokes_fgmres(jfem_mat* A, jfem_mg* velocity_mg, const jfem_vec* pressure_scale,
+                             const jfem_stokes_opts* opts, const jfem_vec* b, jfem_vec* x, double rtol,
+                             jfem_solve_info* info, double* res_hist)
 {
     API_BEGIN
-    JFEM_CHECK(A && velocity_mg && pressure_scale && b && x, JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(A && velocity_mg && pressure_scale && opts && b && x, JFEM_ERR_ARG, "null argument");
     JFEM_CHECK(A->rows == A->cols, JFEM_ERR_ARG, "Object is not square");
     JFEM_CHECK(b->n == A->rows && x->n == A->rows, JFEM_ERR_ARG, "Incompatible sizes");
-    JFEM_CHECK(rtol > 0 && maxit >= 0, JFEM_ERR_ARG, "bad tolerance or iteration limit");
+    JFEM_CHECK(rtol > 0 && opts->maxit >= 0, JFEM_ERR_ARG, "bad tolerance or iteration limit");
     jfem_solve_info tmp;
     jfem_solve_info* inf = info ? info : &tmp;
-    StokesPrec P(A, velocity_mg, pressure_scale->d, pressure_scale->n);
-    timed(A->ctx, inf, [&] { solve_stokes_fgmres(A, P, b->d, x->d, rtol, restart, maxit, inf, res_hist); });
+    StokesPrec P(A, velocity_mg, pressure_scale->d, pressure_scale->n, opts);
+    timed(A->ctx, inf, [&] { solve_stokes_fgmres(A, P, b->d, x->d, rtol, opts->restart, opts->maxit, inf, res_hist); });
     API_END
 }
 

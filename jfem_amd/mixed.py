@@ -247,23 +247,38 @@ class StokesBlockPreconditioner:
     """Block lower-triangular preconditioner for the Taylor-Hood Stokes system:
     velocity block = `velocity_mg` (a multigrid hierarchy of the scalar velocity space with
     GRAD_GRAD weight `viscosity` and the same Dirichlet boundary) once per component;
-    Schur complement S ~ M_p / viscosity with the lumped pressure mass matrix; the pinned
+    Schur complement S ~ M_p / viscosity, inverted by the lumped mass (mass_its = 0) or by
+    mass_its Chebyshev steps on M_p / viscosity preconditioned by the lumped mass; the pinned
     pressure row (default: the first pressure unknown) is kept as an identity row."""
 
     def __init__(self, space: TaylorHoodSpace, velocity_mg: MGPreconditioner, viscosity: float,
-                 pinned: Sequence[int] = (0,)):
+                 pinned: Sequence[int] = (0,), mass_its: int = 0):
         space._need()
         if velocity_mg.getVectorSize() != space.nv:
             raise N.IllegalArgumentException(1, "velocity multigrid does not match the velocity space")
         self.space, self.mg = space, velocity_mg
-        mass = space.pressure.assemble_matrix([TPCellIntegral(TPCellIntegral.VALUE_VALUE)])
+        self.mass = space.pressure.assemble_matrix([TPCellIntegral(1.0 / viscosity, TPCellIntegral.VALUE_VALUE)])
         ones = DenseVector(np.ones(space.np), space.ctx)
-        lumped_dof = mass.mvMul(ones).numpy()
+        lumped_dof = self.mass.mvMul(ones).numpy()
         lumped = np.empty(space.np)
         lumped[space._prank] = lumped_dof
-        scale = viscosity / lumped
+        scale = 1.0 / lumped
         scale[list(pinned)] = -1.0
         self.scale = DenseVector(scale, space.ctx)
+        self.mass_its = mass_its
+        if mass_its > 0 and space.pressure.degree != 1:
+            raise N.UnsupportedOperationException(3, "mass Chebyshev bounds are known for Q1 pressure only")
+
+    def opts(self, restart: int, maxit: int) -> N.StokesOpts:
+        o = N.StokesOpts()
+        o.restart, o.maxit = restart, maxit
+        if self.mass_its > 0:
+            # (lumped Q1 mass)^-1 Q1 mass has its spectrum in [3^-dim, 1] (1D: (2 + cos t) / 3)
+            o.pressure_mass, o.mass_its = self.mass.h, self.mass_its
+            o.mass_lmin, o.mass_lmax = 3.0 ** -self.space.dim * 0.95, 1.0
+        else:
+            o.pressure_mass, o.mass_its, o.mass_lmin, o.mass_lmax = None, 0, 0.0, 1.0
+        return o
 
 
 def solveStokesFGMRES(A: SparseMatrix, prec: StokesBlockPreconditioner, b: DenseVector, x: DenseVector,
@@ -272,6 +287,7 @@ def solveStokesFGMRES(A: SparseMatrix, prec: StokesBlockPreconditioner, b: Dense
     solution out); returns (info, relative residual estimate per iteration)"""
     info = N.SolveInfo()
     hist = np.zeros(max(maxit, 1))
-    check(N.lib().jfem_solve_stokes_fgmres(A.h, prec.mg.h, prec.scale.h, b.h, x.h, rtol, restart, maxit,
+    o = prec.opts(restart, maxit)
+    check(N.lib().jfem_solve_stokes_fgmres(A.h, prec.mg.h, prec.scale.h, C.byref(o), b.h, x.h, rtol,
                                            C.byref(info), dptr(hist)))
     return info, hist[:info.iterations]

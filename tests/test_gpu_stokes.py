@@ -131,14 +131,14 @@ def _velocity_mg(jf, cells, nu, precision, smoother=None):
                                fmg=False, precision=precision), spaces
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_stokes_fgmres_matches_direct_solve(jf, precision):
+@pytest.mark.parametrize("precision,mass_its", [(0, 0), (1, 0), (0, 4), (1, 4)])
+def test_stokes_fgmres_matches_direct_solve(jf, precision, mass_its):
     cells, nu = (16, 16), 0.5
-    print(f"stokes fgmres vs spsolve precision={precision}", flush=True)
+    print(f"stokes fgmres vs spsolve precision={precision} mass_its={mass_its}", flush=True)
     src = jf.VectorTrigSource([jf.TrigSource([[1, 2]], [3.0]), jf.TrigSource([[2, 1]], [-2.0], [[1, 0]])])
     s, A, rhs = _stokes_system(jf, cells, nu, src, _lid)
     mg, _ = _velocity_mg(jf, cells, nu, precision)
-    prec = jf.StokesBlockPreconditioner(s, mg, nu)
+    prec = jf.StokesBlockPreconditioner(s, mg, nu, mass_its=mass_its)
     x = jf.DenseVector(s.ndofs())
     info, hist = jf.solveStokesFGMRES(A, prec, rhs, x, 1e-12, restart=50, maxit=400)
     rp, cols, vals = A.to_csr()
