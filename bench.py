@@ -240,8 +240,10 @@ def bench_c4(jf, args):
     print(json.dumps(line), flush=True)
 
 
-def c5_velocity_mg(jf, cells, nu, precision, smoother, base=4):
-    """scalar Q2 hierarchy base^2 ... cells^2 on the unit square with GRAD_GRAD weight nu"""
+def c5_velocity_mg(jf, cells, nu, precision, smoother, base=16):
+    """scalar Q2 hierarchy base^2 ... cells^2 on the unit square with GRAD_GRAD weight nu (a
+    coarsest level of 16^2 cells measured fastest: 4^2 and 8^2 add launch-bound levels, 32^2
+    makes the dense coarse inverse 4225^2)"""
     spaces = []
     c = base
     while c <= cells:
@@ -290,7 +292,7 @@ def bench_c5(jf, args):
     unknowns), viscosity 1, lid-driven cavity plus a seeded sinusoidal body force, the
     first pressure pinned to 0.  FP64 flexible GMRES(50) to ||r|| <= 1e-8 ||b|| with the
     block lower-triangular preconditioner: velocity block = one FP32 V(1,1) cycle per
-    component (scalar Q2 hierarchy 4^2 ... 256^2, Chebyshev-Jacobi degree 3), with the
+    component (scalar Q2 hierarchy 16^2 ... 256^2, Chebyshev-Jacobi degree 3), with the
     learned smoother on the finest level (network trained on the 128^2 hierarchy) or with
     Chebyshev there too; Schur complement = pressure mass / viscosity, inverted by
     --mass-its Chebyshev steps preconditioned by the lumped mass (0: lumped inverse).  The velocity
@@ -494,7 +496,7 @@ def main():
     ap.add_argument("--pairs", type=int, default=64, help="(residual, error) pairs per training field")
     ap.add_argument("--eval-fields", type=int, default=8)
     ap.add_argument("--galerkin", action="store_true", help="C4: Galerkin coarse systems on every hierarchy")
-    ap.add_argument("--mass-its", type=int, default=4,
+    ap.add_argument("--mass-its", type=int, default=3,
                     help="C5: Chebyshev steps on the pressure mass matrix in the Schur complement (0 = lumped)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="assemble the finest matrix and time only the SpMV and the copy calibration (for PMC runs)")
