@@ -55,7 +55,10 @@ def test_th_assembly_matches_oracle(jf, oracle, cells, kp, nu, wd):
 
 
 def _lid(x):
-    return (1.0 if abs(x[1] - 1.0) < 1e-12 and 0 < x[0] < 1 else 0.0, 0.0)
+    """u = (1, 0[, 0]) on the face x_{d-1} = 1 away from its edges, 0 elsewhere on the boundary"""
+    d = len(x)
+    top = abs(x[d - 1] - 1.0) < 1e-12 and all(0 < x[a] < 1 for a in range(d - 1))
+    return (1.0 if top else 0.0,) + (0.0,) * (d - 1)
 
 
 def test_th_rhs_boundary_and_pin_match_oracle(jf, oracle):
@@ -100,7 +103,7 @@ def test_th_rhs_boundary_and_pin_match_oracle(jf, oracle):
 
 
 def _stokes_system(jf, cells, nu, source=None, boundary=None):
-    start, end = [0.0, 0.0], [1.0, 1.0]
+    start, end = [0.0] * len(cells), [1.0] * len(cells)
     s = _th(jf, cells, 1, start, end)
     A = s.writeCellIntegralsToMatrix(_integrals(jf, nu, -1.0))
     rhs = jf.DenseVector(s.ndofs())
@@ -122,7 +125,7 @@ def _velocity_mg(jf, cells, nu, precision, smoother=None):
             break
         c = [x // 2 for x in c]
     for lc in reversed(levels):
-        sp_ = jf.ContinuousTPFESpace([0.0, 0.0], [1.0, 1.0], lc)
+        sp_ = jf.ContinuousTPFESpace([0.0] * len(lc), [1.0] * len(lc), lc)
         sp_.assembleCells()
         sp_.assembleFunctions(2)
         spaces.append(sp_)
@@ -158,6 +161,29 @@ def test_stokes_fgmres_matches_direct_solve(jf, precision, mass_its):
     assert xs[nv2] == 0.0 or abs(xs[nv2]) <= 1e-12 * np.linalg.norm(b)
     assert info.iterations < 120, info.iterations
     assert np.all(hist[1:] <= hist[:-1] * (1 + 1e-6) + 1e-14)   # GMRES residuals do not increase
+
+
+@pytest.mark.parametrize("mass_its", [0, 4])
+def test_stokes_fgmres_3d_matches_direct_solve(jf, mass_its):
+    cells, nu = (4, 4, 4), 1.0
+    print(f"3d stokes fgmres vs spsolve mass_its={mass_its}", flush=True)
+    src = jf.VectorTrigSource([jf.TrigSource([[1, 2, 1]], [3.0]), jf.TrigSource([[2, 1, 1]], [-2.0], [[1, 0, 0]]),
+                               jf.TrigSource([[1, 1, 2]], [1.0], [[0, 0, 1]])])
+    s, A, rhs = _stokes_system(jf, cells, nu, src, _lid)
+    mg, _ = _velocity_mg(jf, cells, nu, 0)
+    prec = jf.StokesBlockPreconditioner(s, mg, nu, mass_its=mass_its)
+    x = jf.DenseVector(s.ndofs())
+    info, _ = jf.solveStokesFGMRES(A, prec, rhs, x, 1e-12, restart=50, maxit=400)
+    rp, cols, vals = A.to_csr()
+    M = sp.csr_matrix((vals, cols, rp), shape=(s.ndofs(), s.ndofs()))
+    b = rhs.numpy()
+    xd = spla.spsolve(M.tocsc(), b)
+    xs = x.numpy()
+    assert np.linalg.norm(M @ xs - b) <= 1.01e-12 * np.linalg.norm(b)
+    nv3 = s.getVelocitySize()
+    assert np.linalg.norm(xs[:nv3] - xd[:nv3]) <= 1e-8 * np.linalg.norm(xd[:nv3])
+    dp = xs[nv3 + 1:] - xd[nv3 + 1:]
+    assert np.linalg.norm(dp - dp.mean()) <= 1e-8 * np.linalg.norm(xd[nv3:])
 
 
 def _reference_source(jf):
