@@ -818,8 +818,10 @@ void space_evaluate_dev(jfem_space* s, const double* coeff, long npts, const dou
         cells_1d(g.start[a], g.end[a], g.n[a], s1, e1);
         se[a].alloc(g.n[a]);
         se[MAXD + a].alloc(g.n[a]);
-        JFEM_HIP(hipMemcpy(se[a], s1.data(), sizeof(double) * g.n[a], hipMemcpyHostToDevice));
-        JFEM_HIP(hipMemcpy(se[MAXD + a], e1.data(), sizeof(double) * g.n[a], hipMemcpyHostToDevice));
+        JFEM_HIP(hipMemcpyAsync(se[a], s1.data(), sizeof(double) * g.n[a], hipMemcpyHostToDevice, s->ctx->stream));
+        JFEM_HIP(hipMemcpyAsync(se[MAXD + a], e1.data(), sizeof(double) * g.n[a], hipMemcpyHostToDevice,
+                                s->ctx->stream));
+        JFEM_HIP(hipStreamSynchronize(s->ctx->stream));   // s1, e1 leave scope
     }
     if (npts > 0)
         k_eval_fe<<<grid_for(npts, 256, 16384), 256, 0, s->ctx->stream>>>(to_dev(g), se[0], se[1], se[2], se[3], se[4],
@@ -1554,7 +1556,9 @@ jfem_mat* th_assemble(jfem_space* V, jfem_space* Pp, bool has_vv, double vv_w, b
             div_value_table(d, gv.k, gp.k, Q, tv, tq, div_w, eb.data() + t * per);
         }
         EB.alloc(eb.size());
-        JFEM_HIP(hipMemcpy(EB, eb.data(), sizeof(double) * eb.size(), hipMemcpyHostToDevice));
+        // on the context stream: a blocking pageable copy pays a one-time runtime set-up (12 ms)
+        JFEM_HIP(hipMemcpyAsync(EB, eb.data(), sizeof(double) * eb.size(), hipMemcpyHostToDevice, c->stream));
+        JFEM_HIP(hipStreamSynchronize(c->stream));
     }
     ThParams P;
     std::memset(&P, 0, sizeof(P));
@@ -1708,8 +1712,10 @@ int jfem_space_node_coords(jfem_space* s, double* out)
         cells_1d(g.start[a], g.end[a], g.n[a], s1, e1);
         JFEM_HIP(hipMalloc(&se[a], sizeof(double) * g.n[a]));
         JFEM_HIP(hipMalloc(&se[MAXD + a], sizeof(double) * g.n[a]));
-        JFEM_HIP(hipMemcpy(se[a], s1.data(), sizeof(double) * g.n[a], hipMemcpyHostToDevice));
-        JFEM_HIP(hipMemcpy(se[MAXD + a], e1.data(), sizeof(double) * g.n[a], hipMemcpyHostToDevice));
+        JFEM_HIP(hipMemcpyAsync(se[a], s1.data(), sizeof(double) * g.n[a], hipMemcpyHostToDevice, s->ctx->stream));
+        JFEM_HIP(hipMemcpyAsync(se[MAXD + a], e1.data(), sizeof(double) * g.n[a], hipMemcpyHostToDevice,
+                                s->ctx->stream));
+        JFEM_HIP(hipStreamSynchronize(s->ctx->stream));
     }
     k_node_coords<<<grid_for(n, 256, 16384), 256, 0, s->ctx->stream>>>(to_dev(g), n, se[0], se[1], se[2], se[3], se[4],
                                                                       se[5], d);
@@ -1731,9 +1737,10 @@ int jfem_space_evaluate(jfem_space* s, const jfem_vec* coeffs, long npts, const 
     if (npts == 0) return JFEM_OK;
     const int d = s->grid.d;
     DevBuf<double> p(npts * d), v(npts);
-    JFEM_HIP(hipMemcpy(p, points, sizeof(double) * npts * d, hipMemcpyHostToDevice));
+    JFEM_HIP(hipMemcpyAsync(p, points, sizeof(double) * npts * d, hipMemcpyHostToDevice, s->ctx->stream));
     space_evaluate_dev(s, coeffs->d, npts, p, v);
-    JFEM_HIP(hipMemcpy(values, v, sizeof(double) * npts, hipMemcpyDeviceToHost));
+    JFEM_HIP(hipMemcpyAsync(values, v, sizeof(double) * npts, hipMemcpyDeviceToHost, s->ctx->stream));
+    JFEM_HIP(hipStreamSynchronize(s->ctx->stream));
     API_END
 }
 
@@ -1750,7 +1757,7 @@ int jfem_fe_l2_difference(jfem_space* s1, const jfem_vec* c1, jfem_space* s2, co
     if (npts == 0) return JFEM_OK;
     const int d = s1->grid.d;
     DevBuf<double> p(npts * d), v1(npts), v2(npts);
-    JFEM_HIP(hipMemcpy(p, points, sizeof(double) * npts * d, hipMemcpyHostToDevice));
+    JFEM_HIP(hipMemcpyAsync(p, points, sizeof(double) * npts * d, hipMemcpyHostToDevice, s1->ctx->stream));
     space_evaluate_dev(s1, c1->d, npts, p, v1);
     space_evaluate_dev(s2, c2->d, npts, p, v2);
     // squaredDifferences = sum_i (f1(p_i) - f2(p_i))^2 (ConvergenceOrderEstimator.java:49-57), as the

@@ -505,23 +505,14 @@ int jfem_apply_dirichlet(jfem_mat* A, jfem_vec* rhs, long nb, const int* nodes, 
         mask[nodes[b]] = 1;
         if (values) g[nodes[b]] = values[b];
     }
-    unsigned char* dm = nullptr;
-    double* dg = nullptr;
-    JFEM_HIP(hipMalloc(&dm, n > 0 ? n : 1));
-    if (n) JFEM_HIP(hipMemcpy(dm, mask.data(), n, hipMemcpyHostToDevice));
-    if (values) {
-        JFEM_HIP(hipMalloc(&dg, sizeof(double) * n));
-        JFEM_HIP(hipMemcpy(dg, g.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    }
-    try {
-        apply_dirichlet_mask(A, rhs ? rhs->d : nullptr, dm, dg);
-    } catch (...) {
-        (void)hipFree(dm);
-        (void)hipFree(dg);
-        throw;
-    }
-    (void)hipFree(dm);
-    (void)hipFree(dg);
+    DevBuf<unsigned char> dm(n);
+    DevBuf<double> dg;
+    if (values) dg.alloc(n);
+    if (n) JFEM_HIP(hipMemcpyAsync(dm, mask.data(), n, hipMemcpyHostToDevice, A->ctx->stream));
+    if (values) JFEM_HIP(hipMemcpyAsync(dg, g.data(), sizeof(double) * n, hipMemcpyHostToDevice, A->ctx->stream));
+    // the host arrays must outlive the copies
+    JFEM_HIP(hipStreamSynchronize(A->ctx->stream));
+    apply_dirichlet_mask(A, rhs ? rhs->d : nullptr, dm, values ? dg.p : nullptr);
     API_END
 }
 
