@@ -240,14 +240,14 @@ def bench_c4(jf, args):
     print(json.dumps(line), flush=True)
 
 
-def c5_velocity_mg(jf, cells, nu, precision, smoother, base=16):
+def c5_velocity_mg(jf, cells, nu, precision, smoother, base=16, ctx=None):
     """scalar Q2 hierarchy base^2 ... cells^2 on the unit square with GRAD_GRAD weight nu (a
     coarsest level of 16^2 cells measured fastest: 4^2 and 8^2 add launch-bound levels, 32^2
     makes the dense coarse inverse 4225^2)"""
     spaces = []
     c = base
     while c <= cells:
-        sp = jf.ContinuousTPFESpace([0.0, 0.0], [1.0, 1.0], [c, c])
+        sp = jf.ContinuousTPFESpace([0.0, 0.0], [1.0, 1.0], [c, c], ctx)
         sp.assembleCells()
         sp.assembleFunctions(2)
         spaces.append(sp)
@@ -292,7 +292,8 @@ def bench_c5(jf, args):
     unknowns), viscosity 1, lid-driven cavity plus a seeded sinusoidal body force, the
     first pressure pinned to 0.  FP64 flexible GMRES(50) to ||r|| <= 1e-8 ||b|| with the
     block lower-triangular preconditioner: velocity block = one FP32 V(1,1) cycle per
-    component (scalar Q2 hierarchy 16^2 ... 256^2, Chebyshev-Jacobi degree 3), with the
+    component (scalar Q2 hierarchy 16^2 ... 256^2, Chebyshev-Jacobi degree 3; the second
+    component's cycle on its own context and stream, concurrent with the first), with the
     learned smoother on the finest level (network trained on the 128^2 hierarchy) or with
     Chebyshev there too; Schur complement = pressure mass / viscosity, inverted by
     --mass-its Chebyshev steps preconditioned by the lumped mass (0: lumped inverse).  The velocity
@@ -304,8 +305,8 @@ def bench_c5(jf, args):
     cheb = jf.ChebyshevSmoother(degree=3, runs=1, ratio=20.0)
     nv2 = s.getVelocitySize()
 
-    def solve(G, rtol, reps, mass_its=args.mass_its):
-        prec = jf.StokesBlockPreconditioner(s, G, nu, mass_its=mass_its)
+    def solve(G, rtol, reps, mass_its=args.mass_its, comps=()):
+        prec = jf.StokesBlockPreconditioner(s, G, nu, mass_its=mass_its, component_mgs=comps)
         out = []
         for _ in range(reps):
             x = jf.DenseVector(n)
@@ -321,11 +322,17 @@ def bench_c5(jf, args):
     log(f"FP64 multigrid FGMRES to 1e-8: {fp64[-1][1]} its, {np.median([v[0] for v in fp64]):.3f}s")
     del G64
     G32, _ = c5_velocity_mg(jf, cells, nu, jf.FP32, cheb)
-    x32, fp32 = solve(G32, tol, args.warmup + args.steps)
+    # the second velocity component's V-cycle runs on its own context (stream), concurrently
+    ctx2 = jf.Context(0)
+    G32b, _ = c5_velocity_mg(jf, cells, nu, jf.FP32, cheb, ctx=ctx2)
+    seq = solve(G32, tol, args.warmup + args.steps)[1][args.warmup:]
+    log(f"FP32 Chebyshev multigrid FGMRES, components in sequence: {seq[-1][1]} its, "
+        f"{np.median([v[0] for v in seq]):.3f}s")
+    x32, fp32 = solve(G32, tol, args.warmup + args.steps, comps=(G32b,))
     fp32 = fp32[args.warmup:]
     err32 = float(np.linalg.norm(x32.numpy()[:nv2] - xref_u) / np.linalg.norm(xref_u))
     log(f"FP32 Chebyshev multigrid FGMRES: {fp32[-1][1]} its, {np.median([v[0] for v in fp32]):.3f}s")
-    lumped = solve(G32, tol, args.warmup + args.steps, 0)[1][args.warmup:]
+    lumped = solve(G32, tol, args.warmup + args.steps, 0, comps=(G32b,))[1][args.warmup:]
     log(f"... with the lumped-mass Schur complement: {lumped[-1][1]} its, {np.median([v[0] for v in lumped]):.3f}s")
     line_extra = {}
     value = float(np.median([v[0] for v in fp32]))
@@ -341,17 +348,22 @@ def bench_c5(jf, args):
         log(f"learned smoother trained on 128^2 in {t_train:.1f}s, two-grid energy ratio "
             f"{np.mean(h[:8]):.3f} -> {np.mean(h[-8:]):.4f}")
         G32.set_learned(net, np.zeros(cells * cells), runs=1, fallback_ratio=0.5)
+        net2 = jf.SmootherNet(2, 4, ctx=ctx2)   # the same weights for the second component's hierarchy
+        net2.set_params(net.get_params())
+        G32b.set_learned(net2, np.zeros(cells * cells), runs=1, fallback_ratio=0.5)
         res = []
         for _ in range(args.warmup + args.steps):
             G32.set_learned_active(True)
-            xl, r = solve(G32, tol, 1)
+            G32b.set_learned_active(True)
+            xl, r = solve(G32, tol, 1, comps=(G32b,))
             res.append(r[0])
         res = res[args.warmup:]
         errl = float(np.linalg.norm(xl.numpy()[:nv2] - xref_u) / np.linalg.norm(xref_u))
         value = float(np.median([v[0] for v in res]))
-        log(f"learned + FP32 multigrid FGMRES: {res[-1][1]} its, {value:.3f}s, fell back {G32.learned_status()[1]}")
+        fell = bool(G32.learned_status()[1] or G32b.learned_status()[1])
+        log(f"learned + FP32 multigrid FGMRES: {res[-1][1]} its, {value:.3f}s, fell back {fell}")
         line_extra = {"learned_fgmres_iterations": res[-1][1], "learned_velocity_rel_l2_vs_fp64": errl,
-                      "learned_fell_back": bool(G32.learned_status()[1]), "train_s": t_train,
+                      "learned_fell_back": fell, "train_s": t_train,
                       "final_train_loss": float(np.mean(h[-8:]))}
     med = lambda r, i: float(np.median([v[i] for v in r]))
     line = {
@@ -366,6 +378,7 @@ def bench_c5(jf, args):
         "fp64_mg_fgmres_s": med(fp64, 0), "fp64_mg_fgmres_iterations": fp64[-1][1],
         "fp32_cheb_mg_fgmres_s": med(fp32, 0), "fp32_cheb_mg_fgmres_iterations": fp32[-1][1],
         "fp32_cheb_velocity_rel_l2_vs_fp64": err32,
+        "fp32_cheb_components_in_sequence_s": med(seq, 0),
         "schur_mass_chebyshev_steps": args.mass_its,
         "fp32_cheb_lumped_schur_s": med(lumped, 0), "fp32_cheb_lumped_schur_iterations": lumped[-1][1],
     }

@@ -212,6 +212,10 @@ typedef struct {
     jfem_mat *pressure_mass;       /* nullable: M_p / nu (scalar pressure space, DOF order) */
     int mass_its;                  /* Chebyshev steps on it (0 = lumped inverse only) */
     double mass_lmin, mass_lmax;   /* spectrum of (lumped M_p)^-1 M_p */
+    jfem_mg *component_mg[2];      /* nullable: hierarchies equal to velocity_mg for components 1 and 2,
+                                      built on other contexts; their V-cycles then run on those contexts'
+                                      streams, concurrently with component 0's (Richardson smoothers
+                                      synchronise with the host and gain nothing) */
 } jfem_stokes_opts;
 int jfem_solve_stokes_fgmres(jfem_mat *A, jfem_mg *velocity_mg, const jfem_vec *pressure_scale,
                              const jfem_stokes_opts *opts, const jfem_vec *b, jfem_vec *x, double rtol,

@@ -50,7 +50,7 @@ class TrainOpts(C.Structure):
 
 class StokesOpts(C.Structure):
     _fields_ = [("restart", C.c_int), ("maxit", C.c_long), ("pressure_mass", C.c_void_p), ("mass_its", C.c_int),
-                ("mass_lmin", C.c_double), ("mass_lmax", C.c_double)]
+                ("mass_lmin", C.c_double), ("mass_lmax", C.c_double), ("component_mg", C.c_void_p * 2)]
 
 
 class MgOpts(C.Structure):

@@ -248,6 +248,8 @@ void space_lex_rank(jfem_space* s, int* out_dev);
 const Grid& mg_finest_grid(const jfem_mg* mg);
 // boundary mask of the finest level (device, DOF order)
 const unsigned char* mg_finest_bnd(const jfem_mg* mg);
+// the stream a hierarchy's cycles run on (its context's)
+hipStream_t mg_stream(const jfem_mg* mg);
 // reduce partials into a device scalar (no host sync)
 void reduce_partials_dev(jfem_ctx* c, int slots, int blocks, double* dst_dev);
 void mat_make_f32(jfem_mat* A);

@@ -587,7 +587,9 @@ struct StokesPrec {
     const double* ps = nullptr;
     jfem_mat* B = nullptr;       // copy of the pressure rows of A (np rows over all n columns)
     DevBuf<int> rank;            // lexicographic node rank of every velocity DOF
-    DevBuf<double> rs, zs, bz;
+    DevBuf<double> rs[MAXD], zs[MAXD], bz;
+    jfem_mg* Mc[MAXD] = {nullptr, nullptr, nullptr};   // hierarchy of each velocity component
+    hipEvent_t ev_in = nullptr, ev_done[MAXD] = {nullptr, nullptr, nullptr};
     // optional Schur complement solve: Chebyshev steps on the pressure mass matrix (DOF order)
     jfem_mat* Mp = nullptr;
     int mits = 0;
@@ -605,9 +607,29 @@ struct StokesPrec {
         JFEM_CHECK(d == g.d, JFEM_ERR_ARG, "system size is not dim * velocity + pressure");
         jfem_ctx* c = A->ctx;
         rank.alloc(nv);
-        rs.alloc(nv);
-        zs.alloc(nv);
         bz.alloc(np);
+        for (int k = 0; k < d; k++) {
+            Mc[k] = M;
+            if (k > 0 && o && o->component_mg[k - 1]) {
+                jfem_mg* Mk = o->component_mg[k - 1];
+                const Grid& gk = mg_finest_grid(Mk);
+                bool same = mg_size(Mk) == nv && gk.d == g.d && gk.k == g.k;
+                for (int a = 0; a < g.d; a++) same = same && gk.n[a] == g.n[a];
+                JFEM_CHECK(same, JFEM_ERR_ARG, "component hierarchy differs from the velocity hierarchy");
+                Mc[k] = Mk;
+            }
+            // a component on another stream gets its own work vectors
+            if (k == 0 || mg_stream(Mc[k]) != c->stream) {
+                rs[k].alloc(nv);
+                zs[k].alloc(nv);
+            }
+        }
+        // components on other streams than the solver's wait for the residual and are waited for
+        for (int k = 0; k < d; k++)
+            if (mg_stream(Mc[k]) != c->stream) {
+                if (!ev_in) JFEM_HIP(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
+                JFEM_HIP(hipEventCreateWithFlags(&ev_done[k], hipEventDisableTiming));
+            }
         jfem_space vs;
         vs.ctx = c;
         vs.grid = g;
@@ -659,18 +681,30 @@ struct StokesPrec {
             JFEM_HIP(hipGetLastError());
         }
     }
-    ~StokesPrec() { mat_free(B); }
+    ~StokesPrec()
+    {
+        mat_free(B);
+        if (ev_in) (void)hipEventDestroy(ev_in);
+        for (auto e : ev_done) if (e) (void)hipEventDestroy(e);
+    }
     // z = P^-1 r
     void apply(const double* r, double* z)
     {
         jfem_ctx* c = A->ctx;
         const int gv = kgrid(nv);
-        const unsigned char* bnd = mg_finest_bnd(M);
+        if (ev_in) JFEM_HIP(hipEventRecord(ev_in, c->stream));
         for (int k = 0; k < d; k++) {
-            k_gather_rank<<<gv, KB, 0, c->stream>>>(nv, rank, r + k * nv, rs);
-            mg_apply(M, rs, zs);
-            k_scatter_rank<<<gv, KB, 0, c->stream>>>(nv, rank, zs, rs, bnd, z + k * nv);
+            const hipStream_t sk = mg_stream(Mc[k]);
+            double* rk = rs[k].p ? rs[k].p : rs[0].p;
+            double* zk = zs[k].p ? zs[k].p : zs[0].p;
+            if (ev_done[k]) JFEM_HIP(hipStreamWaitEvent(sk, ev_in, 0));
+            k_gather_rank<<<gv, KB, 0, sk>>>(nv, rank, r + k * nv, rk);
+            mg_apply(Mc[k], rk, zk);
+            k_scatter_rank<<<gv, KB, 0, sk>>>(nv, rank, zk, rk, mg_finest_bnd(Mc[k]), z + k * nv);
+            if (ev_done[k]) JFEM_HIP(hipEventRecord(ev_done[k], sk));
         }
+        for (int k = 0; k < d; k++)
+            if (ev_done[k]) JFEM_HIP(hipStreamWaitEvent(c->stream, ev_done[k], 0));
         vec_fill(c, z + d * nv, 0.0, np);
         spmv(B, z, bz);
         if (!Mp) {

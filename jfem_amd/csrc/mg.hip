@@ -612,6 +612,8 @@ const Grid& mg_finest_grid(const jfem_mg* G) { return G->lev.back().grid; }
 
 const unsigned char* mg_finest_bnd(const jfem_mg* G) { return G->lev.back().bnd; }
 
+hipStream_t mg_stream(const jfem_mg* G) { return G->ctx->stream; }
+
 // ------------------------------------------------ setup
 // textbook LU with partial pivoting (Golub & Van Loan 3.4.1) and the explicit
 // inverse of the coarsest matrix

@@ -252,11 +252,17 @@ class StokesBlockPreconditioner:
     pressure row (default: the first pressure unknown) is kept as an identity row."""
 
     def __init__(self, space: TaylorHoodSpace, velocity_mg: MGPreconditioner, viscosity: float,
-                 pinned: Sequence[int] = (0,), mass_its: int = 0):
+                 pinned: Sequence[int] = (0,), mass_its: int = 0,
+                 component_mgs: Sequence[MGPreconditioner] = ()):
+        """component_mgs: optional copies of velocity_mg for components 1 (and 2), built on other
+        Contexts; their V-cycles then run on those contexts' streams, concurrently"""
         space._need()
         if velocity_mg.getVectorSize() != space.nv:
             raise N.IllegalArgumentException(1, "velocity multigrid does not match the velocity space")
+        if len(component_mgs) > space.dim - 1:
+            raise N.IllegalArgumentException(1, "more component hierarchies than velocity components")
         self.space, self.mg = space, velocity_mg
+        self.component_mgs = list(component_mgs)
         self.mass = space.pressure.assemble_matrix([TPCellIntegral(1.0 / viscosity, TPCellIntegral.VALUE_VALUE)])
         ones = DenseVector(np.ones(space.np), space.ctx)
         lumped_dof = self.mass.mvMul(ones).numpy()
@@ -278,6 +284,8 @@ class StokesBlockPreconditioner:
             o.mass_lmin, o.mass_lmax = 3.0 ** -self.space.dim * 0.95, 1.0
         else:
             o.pressure_mass, o.mass_its, o.mass_lmin, o.mass_lmax = None, 0, 0.0, 1.0
+        for i, m in enumerate(self.component_mgs):
+            o.component_mg[i] = m.h.value
         return o
 
 

@@ -115,7 +115,7 @@ def _stokes_system(jf, cells, nu, source=None, boundary=None):
     return s, A, rhs
 
 
-def _velocity_mg(jf, cells, nu, precision, smoother=None):
+def _velocity_mg(jf, cells, nu, precision, smoother=None, ctx=None):
     spaces = []
     c = list(cells)
     levels = []
@@ -125,7 +125,7 @@ def _velocity_mg(jf, cells, nu, precision, smoother=None):
             break
         c = [x // 2 for x in c]
     for lc in reversed(levels):
-        sp_ = jf.ContinuousTPFESpace([0.0] * len(lc), [1.0] * len(lc), lc)
+        sp_ = jf.ContinuousTPFESpace([0.0] * len(lc), [1.0] * len(lc), lc, ctx)
         sp_.assembleCells()
         sp_.assembleFunctions(2)
         spaces.append(sp_)
@@ -184,6 +184,27 @@ def test_stokes_fgmres_3d_matches_direct_solve(jf, mass_its):
     assert np.linalg.norm(xs[:nv3] - xd[:nv3]) <= 1e-8 * np.linalg.norm(xd[:nv3])
     dp = xs[nv3 + 1:] - xd[nv3 + 1:]
     assert np.linalg.norm(dp - dp.mean()) <= 1e-8 * np.linalg.norm(xd[nv3:])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_stokes_concurrent_component_cycles(jf, precision):
+    """component 1's V-cycle on a second context (stream), joined by events: the same
+    solution bit for bit as the sequential preconditioner"""
+    cells, nu = (32, 32), 1.0
+    print(f"stokes concurrent components precision={precision}", flush=True)
+    src = jf.VectorTrigSource([jf.TrigSource([[1, 2]], [3.0]), jf.TrigSource([[2, 1]], [-2.0], [[1, 0]])])
+    s, A, rhs = _stokes_system(jf, cells, nu, src, _lid)
+    mg, _ = _velocity_mg(jf, cells, nu, precision)
+    ctx2 = jf.Context(0)
+    mg2, _ = _velocity_mg(jf, cells, nu, precision, ctx=ctx2)
+    xs = []
+    for comps in ((), (mg2,)):
+        prec = jf.StokesBlockPreconditioner(s, mg, nu, mass_its=3, component_mgs=comps)
+        x = jf.DenseVector(s.ndofs())
+        info, _ = jf.solveStokesFGMRES(A, prec, rhs, x, 1e-10, restart=50, maxit=300)
+        assert info.resnorm <= 1.01e-10 * rhs.euclidianNorm()
+        xs.append(x.numpy())
+    assert np.array_equal(xs[0], xs[1])
 
 
 def _reference_source(jf):
