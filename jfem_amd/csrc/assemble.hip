@@ -1488,6 +1488,11 @@ jfem_mat* th_assemble(jfem_space* V, jfem_space* Pp, bool has_vv, double vv_w, b
     double qp[8], qw[8];
     const int Q = quad_rule(rule, qp, qw);
     JFEM_CHECK(Q > 0, JFEM_ERR_ARG, "unknown quadrature rule");
+    // the mixed integrals evaluate value(x) / gradient(x) of the whole shape function, i.e. in the
+    // first support cell containing x (ScalarShapeFunction.java:26-39); at Gauss-Lobatto points on
+    // a cell face that can be the neighbouring cell, which the per-cell tables do not model
+    JFEM_CHECK(rule >= 1 && rule <= 5, JFEM_ERR_UNSUPPORTED,
+               "mixed integrals need a Gauss rule (Gauss-Lobatto points lie on cell faces)");
     JFEM_CHECK(gv.k < MAXK1, JFEM_ERR_UNSUPPORTED, "velocity degree must be 2..4");
     // cell classes: 1D cells with bitwise equal velocity and pressure tables
     ClassTables CV;
@@ -1845,6 +1850,9 @@ int jfem_th_assemble_rhs_trig(jfem_space* velocity, jfem_space* pressure, int co
     JFEM_CHECK(component >= 0 && component < d, JFEM_ERR_ARG, "velocity component out of range");
     const long nv = velocity->grid.ndofs();
     JFEM_CHECK(rhs->n == d * nv + pressure->grid.ndofs(), JFEM_ERR_ARG, "Incompatible sizes");
+    // TPVectorRightHandSideIntegral evaluates value(x) of the whole shape function (see th_assemble)
+    JFEM_CHECK(quad_rule >= 1 && quad_rule <= 5, JFEM_ERR_UNSUPPORTED,
+               "mixed integrals need a Gauss rule (Gauss-Lobatto points lie on cell faces)");
     DevBuf<double> tmp(nv);
     DevBuf<int> rank(nv);
     assemble_rhs_sines(velocity, nterms, kw, amp, quad_rule, tmp, cosine);

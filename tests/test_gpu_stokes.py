@@ -21,13 +21,13 @@ def _th(jf, cells, kp, start, end):
     return s
 
 
-def _integrals(jf, nu, wd):
+def _integrals(jf, nu, wd, rule=5):
     out = []
     if nu is not None:
         out.append(jf.MixedCellIntegral.fromVelocityIntegral(
-            jf.TPVectorCellIntegral(nu, jf.TPVectorCellIntegral.GRAD_GRAD)))
+            jf.TPVectorCellIntegral(nu, jf.TPVectorCellIntegral.GRAD_GRAD, quadratureRule1D=rule)))
     if wd is not None:
-        out.append(jf.MixedTPCellIntegral(wd, jf.MixedTPCellIntegral.DIV_VALUE))
+        out.append(jf.MixedTPCellIntegral(wd, jf.MixedTPCellIntegral.DIV_VALUE, quadratureRule1D=rule))
     return out
 
 
@@ -52,6 +52,61 @@ def test_th_assembly_matches_oracle(jf, oracle, cells, kp, nu, wd):
     E = MO.th_assemble(cells, kp, start, end, nu, wd)
     assert A.getRows() == s.ndofs()
     _compare(A, E, s.ndofs())
+
+
+@pytest.mark.parametrize("rule", [3, 4])
+def test_th_assembly_other_quadrature_rules(jf, oracle, rule):
+    """Gauss3 and Gauss4 instead of the default Gauss5"""
+    cells, kp = (3, 2), 1
+    start, end = [0.0, 0.5], [1.0, 1.25]
+    print(f"th assembly rule={rule}", flush=True)
+    s = _th(jf, cells, kp, start, end)
+    A = s.writeCellIntegralsToMatrix(_integrals(jf, 1.3, -1.0, rule))
+    E = MO.th_assemble(cells, kp, start, end, 1.3, -1.0, rule)
+    _compare(A, E, s.ndofs())
+
+
+def test_th_assembly_rejects_face_quadrature(jf):
+    """Gauss-Lobatto points lie on cell faces, where the reference evaluates the mixed
+    integrands in the first support cell containing the point: not modelled, refused"""
+    s = _th(jf, (2, 2), 1, [0.0, 0.0], [1.0, 1.0])
+    with pytest.raises(jf.UnsupportedOperationException):
+        s.writeCellIntegralsToMatrix(_integrals(jf, 1.0, -1.0, 15))
+
+
+def test_th_argument_errors(jf):
+    s = _th(jf, (2, 2), 1, [0.0, 0.0], [1.0, 1.0])
+    other = _th(jf, (3, 2), 1, [0.0, 0.0], [1.0, 1.0])
+    ints = _integrals(jf, 1.0, -1.0)
+    # velocity and pressure spaces on different grids
+    h = jf.api.C.c_void_p()
+    kinds = np.array([0, 1], dtype=np.int32)
+    w = np.array([1.0, -1.0])
+    rc = jf.lib().jfem_th_assemble(s.velocity.h, other.pressure.h, 2, jf._native.iptr(kinds),
+                                   jf._native.dptr(w), 5, jf.api.C.byref(h))
+    assert rc == 1
+    # degrees that are not k+1 / k
+    rc = jf.lib().jfem_th_assemble(s.pressure.h, s.pressure.h, 2, jf._native.iptr(kinds),
+                                   jf._native.dptr(w), 5, jf.api.C.byref(h))
+    assert rc == 1
+    # unknown integral kind, two integrals of one kind
+    with pytest.raises(jf.UnsupportedOperationException):
+        jf.MixedTPCellIntegral(1.0, "ValueGrad")
+    bad = np.array([0, 0], dtype=np.int32)
+    rc = jf.lib().jfem_th_assemble(s.velocity.h, s.pressure.h, 2, jf._native.iptr(bad), jf._native.dptr(w), 5,
+                                   jf.api.C.byref(h))
+    assert rc == 3
+    A = s.writeCellIntegralsToMatrix(ints)
+    rhs = jf.DenseVector(s.ndofs())
+    # a body force with the wrong number of components
+    f = jf.VectorTrigSource([jf.TrigSource([[1, 1]], [1.0])])
+    with pytest.raises(jf.IllegalArgumentException):
+        s.writeCellIntegralsToRhs([jf.MixedRightHandSideIntegral.fromVelocityIntegral(
+            jf.TPVectorRightHandSideIntegral(f, "Value"))], rhs)
+    # a velocity hierarchy of another size
+    mg, _ = _velocity_mg(jf, (4, 4), 1.0, 0)
+    with pytest.raises(jf.IllegalArgumentException):
+        jf.StokesBlockPreconditioner(s, mg, 1.0)
 
 
 def _lid(x):
