@@ -189,6 +189,13 @@ int jfem_solve_bicgstab(jfem_mat *A, jfem_mg *M, const jfem_vec *b, jfem_vec *x,
  * classical Gram-Schmidt, Givens; restart = ITERATIONS_BEFORE_RESTART (150) */
 int jfem_solve_gmres(jfem_mat *A, jfem_mg *M, const jfem_vec *b, jfem_vec *x, double tol, int restart,
                      int max_cycles, jfem_solve_info *info);
+/* GMRES2.solve (M/linalg/GMRES2.java:34-104, Arnoldi M/linalg/Arnoldi.java:13-82): minimal
+ * residual steps along M A-orthonormalised directions of the power sequence of M A, stop when
+ * the recursively updated ||r||_2 <= tol (absolute), restart on a vanishing direction; M
+ * nullable (identity).  x in: start iterate (reference default e_0).  maxit bounds the
+ * iterations (unbounded in the reference); at most 512 directions are held between restarts. */
+int jfem_solve_gmres2(jfem_mat *A, jfem_mg *M, const jfem_vec *b, jfem_vec *x, double tol, long maxit,
+                      jfem_solve_info *info);
 /* flexible CG (Notay 2000, Polak-Ribiere beta) in FP64 around a multigrid
  * preconditioner of any precision; not in the reference. */
 int jfem_solve_fcg(jfem_mat *A, jfem_mg *M, const jfem_vec *b, jfem_vec *x, double tol, long maxit,
@@ -254,6 +261,9 @@ int jfem_mg_level_matrix(jfem_mg *mg, int level, jfem_mat **out);
 int jfem_mg_apply(jfem_mg *mg, const jfem_vec *r, jfem_vec *z);
 /* one V-cycle mgStepV(finest, x, b) updating x in place */
 int jfem_mg_vcycle(jfem_mg *mg, jfem_vec *x, const jfem_vec *b);
+/* wCycle -> mgStepW (MGPreconditionerInterface.java:200-305): pre-smoothing, two coarse
+ * corrections each by a V-cycle on the next coarser level, post-smoothing; x in/out */
+int jfem_mg_wcycle(jfem_mg *mg, jfem_vec *x, const jfem_vec *b);
 /* Chebyshev interval upper bound used on each level (after safety factor) */
 int jfem_mg_lmax(jfem_mg *mg, double *out);
 /* prolongationOperators.get(level).mvMul / .tvMul

@@ -106,6 +106,7 @@ _SIGS = {
     "jfem_solve_cg": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_long, C.POINTER(SolveInfo), _dp]),
     "jfem_solve_bicgstab": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_long, C.POINTER(SolveInfo)]),
     "jfem_solve_gmres": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, C.POINTER(SolveInfo)]),
+    "jfem_solve_gmres2": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_long, C.POINTER(SolveInfo)]),
     "jfem_solve_fcg": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_long, C.POINTER(SolveInfo), _dp]),
     "jfem_solve_stokes_fgmres": (C.c_int, [_vp, _vp, _vp, C.POINTER(StokesOpts), _vp, _vp, C.c_double,
                                             C.POINTER(SolveInfo), _dp]),
@@ -115,6 +116,7 @@ _SIGS = {
     "jfem_mg_level_matrix": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
     "jfem_mg_apply": (C.c_int, [_vp, _vp, _vp]),
     "jfem_mg_vcycle": (C.c_int, [_vp, _vp, _vp]),
+    "jfem_mg_wcycle": (C.c_int, [_vp, _vp, _vp]),
     "jfem_mg_lmax": (C.c_int, [_vp, _dp]),
     "jfem_mg_prolongate": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "jfem_mg_restrict": (C.c_int, [_vp, C.c_int, _vp, _vp]),

@@ -516,6 +516,11 @@ class MGPreconditioner:
         check(N.lib().jfem_mg_vcycle(self.h, x.h, b.h))
         return x
 
+    def wCycle(self, x: DenseVector, b: DenseVector) -> DenseVector:
+        """MGPreconditionerInterface.wCycle (:300-305): mgStepW on the finest level, x in place"""
+        check(N.lib().jfem_mg_wcycle(self.h, x.h, b.h))
+        return x
+
     STEP_PLAIN, STEP_MIN_RESIDUAL, STEP_MIN_ENERGY = 0, 1, 2
 
     def set_learned(self, net, cell_log_coeff=None, runs: int = 2, fallback_ratio: float = 0.5,
@@ -662,6 +667,35 @@ class IterativeSolver:
 # --------------------------------------------------------------------------
 # learned smoother network
 # --------------------------------------------------------------------------
+class GMRES2:
+    """GMRES2 (M/linalg/GMRES2.java:7-104): solve(A, b) or solve(A, M, b) with a multigrid
+    preconditioner M; the iterate persists between calls as in the reference (first start e_0);
+    stops when the recursively updated ||r|| <= tol (absolute).  maxit bounds the iterations
+    (the reference loops until convergence)."""
+
+    def __init__(self, tol: float, show: bool = False, maxit: int = 100000):
+        self.tol, self.maxit = tol, maxit
+        self.iterate: Optional[DenseVector] = None
+        self.info = None
+
+    def solve(self, A: SparseMatrix, *args) -> DenseVector:
+        if len(args) == 1:
+            M, b = None, args[0]
+        elif len(args) == 2:
+            M, b = args
+        else:
+            raise TypeError("solve(A, b) or solve(A, M, b)")
+        if self.iterate is None:
+            e0 = np.zeros(b.n)
+            e0[0] = 1.0
+            self.iterate = DenseVector(e0, A.ctx)
+        info = N.SolveInfo()
+        check(N.lib().jfem_solve_gmres2(A.h, M.h if M is not None else None, b.h, self.iterate.h, self.tol,
+                                        self.maxit, C.byref(info)))
+        self.info = info
+        return self.iterate
+
+
 class SmootherNet:
     """Convolutional network cin -> 32 -> 32 -> 32 -> 1 with 3^dim taps on the node
     lattice (jfem_nn): BF16 MFMA layers, FP32 master weights and Adam."""

@@ -496,6 +496,83 @@ static void solve_gmres(jfem_mat* A, jfem_mg* M, const double* b, double* x, dou
 }
 
 
+// ------------------------------------------------------------------- GMRES2
+// GMRES2.solveInternal (M/linalg/GMRES2.java:59-104) with Arnoldi (M/linalg/Arnoldi.java:13-82):
+// minimal-residual steps x += t v along MA-orthonormalised directions (MA = M(A .)) of the
+// power sequence q_{k+1} = MA (q_k / |q_k|); the residual and the preconditioned residual
+// are updated recursively, the stop test is |r| <= tol (absolute), and a direction whose
+// |MA w| falls below 1e-13 after orthogonalisation restarts from the current iterate.
+// The same operation sequence as the reference (y + x a forms as axpby with b = 1); the
+// dot products are the library's deterministic two-stage sums.  The reference keeps every
+// direction; here at most `cap` are held (device memory), reaching it restarts, and maxit
+// bounds the iterations.
+static void ma_apply(jfem_mat* A, jfem_mg* M, const double* x, double* tmp, double* y)
+{
+    spmv(A, x, M ? tmp : y);
+    if (M) mg_apply(M, tmp, y);
+}
+
+static void solve_gmres2(jfem_mat* A, jfem_mg* M, const double* b, double* x, double tol, long maxit,
+                         jfem_solve_info* info)
+{
+#pragma clang fp contract(off)
+    jfem_ctx* c = A->ctx;
+    const long n = A->rows;
+    long cap = maxit + 2 < 512 ? maxit + 2 : 512;
+    while (cap > 8 && 2.0 * cap * n * 8 > 100e9) cap /= 2;
+    DevBuf<double> r(n), Mr(n), t(n), q(n), qn(n), w(n), aw(n), V((size_t)cap * n), AV((size_t)cap * n);
+    const int nb = kgrid(n);
+    long it = 0;
+    int restarts = 0;
+    for (;;) {
+        spmv(A, x, t);
+        k_sub<<<nb, KB, 0, c->stream>>>(n, b, t, r);
+        if (M) mg_apply(M, r, Mr); else vec_copy(c, Mr, r, n);
+        // Arnoldi constructor
+        k_scale<<<nb, KB, 0, c->stream>>>(n, 1. / std::sqrt(vec_dot(c, n, Mr, Mr)), Mr, w);
+        ma_apply(A, M, w, t, q);
+        const double n1 = std::sqrt(vec_dot(c, n, q, q));
+        k_scale<<<nb, KB, 0, c->stream>>>(n, 1. / n1, w, V.p);
+        k_scale<<<nb, KB, 0, c->stream>>>(n, 1. / n1, q, AV.p);
+        long nv = 1;
+        k_scale<<<nb, KB, 0, c->stream>>>(n, 1. / n1, q, w);
+        ma_apply(A, M, w, t, qn);
+        vec_copy(c, q, qn, n);
+        bool restart = false;
+        while (std::sqrt(vec_dot(c, n, r, r)) > tol) {
+            if (it >= maxit) break;
+            const double* v = V.p + (size_t)(nv - 1) * n;
+            const double* av = AV.p + (size_t)(nv - 1) * n;
+            const double tt = vec_dot(c, n, Mr, av) / vec_dot(c, n, av, av);
+            vec_axpby(c, n, tt, v, 1.0, x);
+            spmv(A, v, t);
+            vec_axpby(c, n, -tt, t, 1.0, r);
+            vec_axpby(c, n, -tt, av, 1.0, Mr);
+            it++;
+            // arnoldiStep
+            k_scale<<<nb, KB, 0, c->stream>>>(n, 1. / std::sqrt(vec_dot(c, n, q, q)), q, w);
+            ma_apply(A, M, w, t, qn);
+            vec_copy(c, aw, qn, n);
+            for (long i = 0; i < nv; i++) {
+                const double p = vec_dot(c, n, aw, AV.p + (size_t)i * n);
+                vec_axpby(c, n, -p, V.p + (size_t)i * n, 1.0, w);
+                vec_axpby(c, n, -p, AV.p + (size_t)i * n, 1.0, aw);
+            }
+            const double na = std::sqrt(vec_dot(c, n, aw, aw));
+            vec_copy(c, q, qn, n);
+            if (na < 1e-13 || nv == cap) { restart = true; break; }
+            k_scale<<<nb, KB, 0, c->stream>>>(n, 1. / na, w, V.p + (size_t)nv * n);
+            k_scale<<<nb, KB, 0, c->stream>>>(n, 1. / na, aw, AV.p + (size_t)nv * n);
+            nv++;
+        }
+        JFEM_HIP(hipGetLastError());
+        if (!restart || it >= maxit) break;
+        restarts++;
+    }
+    info->iterations = (int)it;
+    info->resnorm = true_resnorm(A, b, x, t);
+}
+
 // ------------------------------------------------- Stokes: block-preconditioned FGMRES
 // velocity component c of a mixed vector (node order) <-> scalar space vector (DOF order)
 __global__ void k_gather_rank(long n, const int* __restrict__ rank, const double* __restrict__ src,
@@ -887,6 +964,18 @@ int jfem_solve_gmres(jfem_mat* A, jfem_mg* M, const jfem_vec* b, jfem_vec* x, do
     jfem_solve_info tmp;
     jfem_solve_info* inf = info ? info : &tmp;
     timed(A->ctx, inf, [&] { solve_gmres(A, M, b->d, x->d, tol, restart, max_cycles, inf); });
+    API_END
+}
+
+int jfem_solve_gmres2(jfem_mat* A, jfem_mg* M, const jfem_vec* b, jfem_vec* x, double tol, long maxit,
+                      jfem_solve_info* info)
+{
+    API_BEGIN
+    check_solve_args(A, M, b, x);
+    JFEM_CHECK(maxit >= 0, JFEM_ERR_ARG, "negative iteration limit");
+    jfem_solve_info tmp;
+    jfem_solve_info* inf = info ? info : &tmp;
+    timed(A->ctx, inf, [&] { solve_gmres2(A, M, b->d, x->d, tol, maxit, inf); });
     API_END
 }
 

@@ -559,6 +559,30 @@ static void step_v(jfem_mg* G, int l, bool x_zero)
     if (x != (T*)L.x) { L.t = L.x; L.x = x; }
 }
 
+// mgStepW (MGPreconditionerInterface.java:200-291): pre-smoothing, then twice the
+// coarse correction {defect, restriction, a V-cycle (mgStepV) on level l - 1 from
+// zero, prolongation with zero boundary values, addition}, then post-smoothing
+template <typename T>
+static void step_w(jfem_mg* G, int l, bool x_zero)
+{
+    MgLevel& L = G->lev[l];
+    T* b = (T*)L.b;
+    if (l == 0) { coarse_solve<T>(G, b, (T*)L.x); return; }
+    T* x = (T*)L.x;
+    T* xalt = (T*)L.t;
+    smooth<T>(G, L, b, x, xalt, x_zero, false);
+    MgLevel& C = G->lev[l - 1];
+    T* r = (T*)L.r;
+    for (int k = 0; k < 2; k++) {
+        level_residual<T>(G, L, b, x, r);
+        restrict_to<T>(G, l, r, (T*)C.b);
+        step_v<T>(G, l - 1, true);
+        prolong_to<T>(G, l, (const T*)C.x, x, true, true);
+    }
+    smooth<T>(G, L, b, x, xalt, false, true);
+    if (x != (T*)L.x) { L.t = L.x; L.x = x; }
+}
+
 template <typename T>
 static void apply_t(jfem_mg* G, const double* rin, double* zout)
 {
@@ -1510,6 +1534,32 @@ int jfem_mg_vcycle(jfem_mg* mg, jfem_vec* x, const jfem_vec* b)
         step_v<double>(mg, Lf, false);
         vec_copy(mg->ctx, x->d, (const double*)F.x, n);
     }
+    JFEM_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
+int jfem_mg_wcycle(jfem_mg* mg, jfem_vec* x, const jfem_vec* b)
+{
+    API_BEGIN
+    JFEM_CHECK(mg && x && b, JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(x->n == mg_size(mg) && b->n == mg_size(mg), JFEM_ERR_ARG, "wrong size");
+    const int Lf = (int)mg->lev.size() - 1;
+    MgLevel& F = mg->lev[Lf];
+    const long n = F.A->rows;
+    hipStream_t st = mg->ctx->stream;
+    const int g = grid_for(n, 256, 16384);
+    if (mg->opt.precision == JFEM_FP32) {
+        k_convert<double, float><<<g, 256, 0, st>>>(n, b->d, (float*)F.b);
+        k_convert<double, float><<<g, 256, 0, st>>>(n, x->d, (float*)F.x);
+        step_w<float>(mg, Lf, false);
+        k_convert<float, double><<<g, 256, 0, st>>>(n, (const float*)F.x, x->d);
+    } else {
+        vec_copy(mg->ctx, (double*)F.b, b->d, n);
+        vec_copy(mg->ctx, (double*)F.x, x->d, n);
+        step_w<double>(mg, Lf, false);
+        vec_copy(mg->ctx, x->d, (const double*)F.x, n);
+    }
+    JFEM_HIP(hipGetLastError());
     JFEM_HIP(hipStreamSynchronize(st));
     API_END
 }

@@ -1132,6 +1132,31 @@ static void mg_step_v(const orc_mg *G, int l, double *x, const double *b)
 
 void orc_mg_vcycle(const orc_mg *G, double *x, const double *b) { mg_step_v(G, G->nlev - 1, x, b); }
 
+/* mgStepW (MGPreconditionerInterface.java:200-291): pre-smooth, then twice
+ * { restrict the defect, V-cycle (mgStepV) on the next coarser level from zero,
+ *   prolong and add }, post-smooth; wCycle (:300-305) starts it on the finest level */
+static void mg_step_w(const orc_mg *G, int l, double *x, const double *b)
+{
+    const orc_level *L = &G->lev[l];
+    const long n = L->n;
+    if (l == 0) { coarse_solve(G, b, x); return; }
+    smooth(L, b, x);
+    double *t = vnew(n), *rc = vnew(G->lev[l - 1].n), *ec = vnew(G->lev[l - 1].n), *pe = vnew(n);
+    for (int k = 0; k < 2; k++) {
+        spmv_l(L, x, t);
+        for (long i = 0; i < n; i++) t[i] = b[i] - t[i];
+        restrict_(G, l - 1, t, rc);
+        for (long i = 0; i < G->lev[l - 1].n; i++) ec[i] = 0;
+        mg_step_v(G, l - 1, ec, rc);
+        prolong(G, l - 1, ec, pe, 1);
+        for (long i = 0; i < n; i++) x[i] = x[i] + pe[i];
+    }
+    smooth(L, b, x);
+    free(t); free(rc); free(ec); free(pe);
+}
+
+void orc_mg_wcycle(const orc_mg *G, double *x, const double *b) { mg_step_w(G, G->nlev - 1, x, b); }
+
 /* MGPreconditionerInterface.mvMul (:352-379), or plain cycles of V from zero when fmg == 0 */
 void orc_mg_apply(const orc_mg *G, const double *v, double *out)
 {
@@ -1176,7 +1201,7 @@ static void mg_apply_op(orc_op *op, const double *x, double *y) { orc_mg_apply((
  * order) of its support that contains the node.  Returns a CSR (fine x coarse). */
 orc_csr *orc_prolongation(int d, int k, const int *cells_c, const double *start, const double *end)
 {
-    int cells_f[ORC_MAXD];
+    int cells_f[ORC_MAXD] = {0, 0, 0};
     long ncc = 1;
     int nloc = 1;
     for (int a = 0; a < d; a++) { cells_f[a] = 2 * cells_c[a]; ncc *= cells_c[a]; nloc *= k + 1; }
@@ -1357,6 +1382,93 @@ typedef struct {
     double resnorm;
 } orc_solve_out;
 
+/* GMRES2.solveInternal (M/linalg/GMRES2.java:59-104) with Arnoldi (M/linalg/Arnoldi.java:13-82),
+ * M = identity when NULL (GMRES2.solve(A, b) uses SparseMatrix.identity).  MA = M(A .):
+ *   Arnoldi(Mr): q0 = Mr, w0 = q0/|q0|, q1 = MA w0, v0 = w0 * (1/|q1|), Av0 = q1 * (1/|q1|),
+ *                q2 = MA (q1/|q1|)
+ *   while |r| > tol:  t = (Mr.Av)/(Av.Av) for the latest (v, Av); x += v t; r -= (A v) t;
+ *                     Mr -= Av t; arnoldiStep: w = q/|q| (latest q), q' = MA w, Aw = q';
+ *                     for every (v_i, Av_i): p = Aw.Av_i, w -= v_i p, Aw -= Av_i p;
+ *                     append w * (1/|Aw|), Aw * (1/|Aw|); restart (from the new iterate,
+ *                     recomputing r and Mr) when |Aw| < 1e-13.
+ * The stop test uses the recursively updated residual, absolute tol.  maxit bounds the
+ * iterations, which the reference leaves unbounded. */
+static void ma_apply(orc_op *A, orc_op *M, const double *x, double *tmp, double *y)
+{
+    A->apply(A, x, tmp);
+    if (M) M->apply(M, tmp, y); else memcpy(y, tmp, sizeof(double) * A->n);
+}
+
+int orc_gmres2(orc_op *A, orc_op *M, const double *b, double *x, double tol, long maxit, orc_result *out)
+{
+    const long n = A->n;
+    double *r = vnew(n), *Mr = vnew(n), *t = vnew(n), *q = vnew(n), *qn = vnew(n), *w = vnew(n), *aw = vnew(n);
+    long cap = 16;
+    double **V = (double **)calloc(cap, sizeof(double *)), **AV = (double **)calloc(cap, sizeof(double *));
+    long it = 0;
+    int restarts = 0;
+    for (;;) {
+        A->apply(A, x, t);
+        for (long e = 0; e < n; e++) r[e] = b[e] - t[e];
+        if (M) M->apply(M, r, Mr); else memcpy(Mr, r, sizeof(double) * n);
+        /* Arnoldi constructor */
+        long nv = 0;
+        double s = 1. / nrm(n, Mr);
+        for (long e = 0; e < n; e++) w[e] = Mr[e] * s;                  /* w0 */
+        ma_apply(A, M, w, t, q);                                          /* q1 = Aw0 */
+        const double n1 = nrm(n, q);
+        V[0] = vnew(n); AV[0] = vnew(n);
+        for (long e = 0; e < n; e++) { V[0][e] = w[e] * (1. / n1); AV[0][e] = q[e] * (1. / n1); }
+        nv = 1;
+        s = 1. / n1;
+        for (long e = 0; e < n; e++) w[e] = q[e] * s;
+        ma_apply(A, M, w, t, qn);                                         /* q2 */
+        memcpy(q, qn, sizeof(double) * n);
+        int restart = 0;
+        while (nrm(n, r) > tol) {
+            if (it >= maxit) break;
+            const double *v = V[nv - 1], *av = AV[nv - 1];
+            const double tt = dot(n, Mr, av) / dot(n, av, av);
+            for (long e = 0; e < n; e++) x[e] = x[e] + v[e] * tt;
+            A->apply(A, v, t);
+            for (long e = 0; e < n; e++) r[e] = r[e] - t[e] * tt;
+            for (long e = 0; e < n; e++) Mr[e] = Mr[e] - av[e] * tt;
+            it++;
+            /* arnoldiStep */
+            s = 1. / nrm(n, q);
+            for (long e = 0; e < n; e++) w[e] = q[e] * s;
+            ma_apply(A, M, w, t, qn);
+            memcpy(aw, qn, sizeof(double) * n);
+            for (long i = 0; i < nv; i++) {
+                const double p = dot(n, aw, AV[i]);
+                for (long e = 0; e < n; e++) w[e] = w[e] - V[i][e] * p;
+                for (long e = 0; e < n; e++) aw[e] = aw[e] - AV[i][e] * p;
+            }
+            const double na = nrm(n, aw);
+            if (nv == cap) {
+                cap *= 2;
+                V = (double **)realloc(V, sizeof(double *) * cap);
+                AV = (double **)realloc(AV, sizeof(double *) * cap);
+            }
+            V[nv] = vnew(n); AV[nv] = vnew(n);
+            for (long e = 0; e < n; e++) { V[nv][e] = w[e] * (1. / na); AV[nv][e] = aw[e] * (1. / na); }
+            nv++;
+            memcpy(q, qn, sizeof(double) * n);
+            if (na < 1e-13) { restart = 1; break; }
+        }
+        for (long i = 0; i < nv; i++) { free(V[i]); free(AV[i]); }
+        if (!restart || it >= maxit) break;
+        restarts++;
+    }
+    A->apply(A, x, t);
+    for (long e = 0; e < n; e++) t[e] = b[e] - t[e];
+    out->iterations = (int)it;
+    out->restarts = restarts;
+    out->resnorm = nrm(n, t);
+    free(V); free(AV); free(r); free(Mr); free(t); free(q); free(qn); free(w); free(aw);
+    return 0;
+}
+
 static void make_ops(orc_op *A, csr_ctx *ac, long n, const long *rowptr, const int *cols, const double *vals)
 {
     ac->n = n; ac->rowptr = rowptr; ac->cols = cols; ac->vals = vals;
@@ -1381,6 +1493,7 @@ int orc_solve(int method, long n, const long *rowptr, const int *cols, const dou
     case 0: rc = Mp ? orc_pcg(&A, Mp, b, x, tol, maxit, &r, hist) : orc_cg(&A, b, x, tol, maxit, &r, hist); break;
     case 1: rc = orc_bicgstab(&A, Mp, b, x, tol, &r); break;
     case 2: rc = orc_gmres(&A, Mp, b, x, tol, restart, (int)maxit, &r); break;
+    case 3: rc = orc_gmres2(&A, precond == 1 ? NULL : Mp, b, x, tol, maxit, &r); break;
     default: return -1;
     }
     out->iterations = r.iterations; out->fell_back = r.fell_back; out->restarts = r.restarts; out->resnorm = r.resnorm;

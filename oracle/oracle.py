@@ -88,6 +88,7 @@ def lib():
         L.orc_mg_prol_csr.argtypes = [C.c_void_p, C.c_int, lp, ip, dp]
         L.orc_mg_apply.argtypes = [C.c_void_p, dp, dp]
         L.orc_mg_vcycle.argtypes = [C.c_void_p, dp, dp]
+        L.orc_mg_wcycle.argtypes = [C.c_void_p, dp, dp]
         L.orc_mg_smooth.argtypes = [C.c_void_p, C.c_int, dp, dp]
         L.orc_mg_set_interval.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
         L.orc_solve.argtypes = [C.c_int, C.c_long, lp, ip, dp, C.c_int, C.c_void_p, lp, ip, dp,
@@ -323,7 +324,7 @@ def prolongation(cells_coarse, k, start, end):
     return _take_csr(ptr, ncoarse)
 
 
-METHOD = {"cg": 0, "bicgstab": 1, "gmres": 2}
+METHOD = {"cg": 0, "bicgstab": 1, "gmres": 2, "gmres2": 3}
 
 
 def solve(method, A: CSR, b, x0, tol, precond=None, maxit=10 ** 9, restart=150, hist=None):
@@ -426,6 +427,11 @@ class MG:
     def vcycle(self, x, b):
         x = np.ascontiguousarray(x, dtype=np.float64).copy()
         lib().orc_mg_vcycle(self.ptr, _d(x), _d(np.ascontiguousarray(b, float)))
+        return x
+
+    def wcycle(self, x, b):
+        x = np.ascontiguousarray(x, dtype=np.float64).copy()
+        lib().orc_mg_wcycle(self.ptr, _d(x), _d(np.ascontiguousarray(b, float)))
         return x
 
     def smooth(self, l, b, x):

@@ -178,6 +178,41 @@ def test_krylov_against_oracle(jf, oracle, method):
     assert np.linalg.norm(rhs - ref.matvec(x)) <= 1.01 * tol
 
 
+def test_gmres2_against_oracle(jf, oracle):
+    """GMRES2.solve(A, b) (M/linalg/GMRES2.java:34-104) on the C1 system: the same solution as
+    the oracle's restatement (dot products summed in another order)"""
+    s, ref, rhs = _c1_system(jf, oracle, (24, 20))
+    tol = 1e-10 * np.linalg.norm(rhs)
+    g = jf.GMRES2(tol)
+    x = g.solve(s.getSystemMatrix(), s.getRhs()).numpy()
+    x0 = np.zeros(ref.n)
+    x0[0] = 1
+    xr, itr, resr, _ = oracle.solve("gmres2", ref, rhs, x0, tol)
+    assert np.linalg.norm(x - xr) <= 1e-8 * np.linalg.norm(xr)
+    assert np.linalg.norm(rhs - ref.matvec(x)) <= 1.01 * tol
+    assert abs(g.info.iterations - itr) <= max(2, itr // 10), (g.info.iterations, itr)
+
+
+def test_gmres2_mg_preconditioned_against_oracle(jf, oracle):
+    """GMRES2.solve(A, M, b) with the multigrid preconditioner (MGPreconditionerInterface.mvMul),
+    as StokesMG2 / LaplaceMG use it"""
+    G, O, spaces = _mg_pair(jf, oracle, [2, 2], 2, 3, jf.JacobiSmoother(2, 0.6), cycles=1, fmg=False)
+    A = G.getFinestSystem()
+    n = O.level_n(2)
+    rng = np.random.default_rng(9)
+    b = rng.standard_normal(n)
+    tol = 1e-9 * np.linalg.norm(b)
+    g = jf.GMRES2(tol)
+    x = g.solve(A, G, jf.DenseVector(b)).numpy()
+    ref = O.level_matrix(2)
+    x0 = np.zeros(n)
+    x0[0] = 1
+    xr, itr, resr, _ = oracle.solve("gmres2", ref, b, x0, tol, precond=O)
+    assert np.linalg.norm(x - xr) <= 1e-7 * np.linalg.norm(xr)
+    assert abs(g.info.iterations - itr) <= 2, (g.info.iterations, itr)
+    assert itr < 30
+
+
 @pytest.mark.parametrize("mat", [[[0, 1, 2], [1, 0, 3], [2, 3, 0]], [[3, 1, 7], [2, 0, 3], [2, 3, 0]],
                                  [[3, 1, 7], [2, 4, 3], [2, 3, 7]]])
 def test_small_known_answers(jf, mat):
@@ -190,6 +225,8 @@ def test_small_known_answers(jf, mat):
     for solve in ("solveBiCGStab", "solveGMRES"):
         x = getattr(jf.IterativeSolver(), solve)(A, b, 1e-12).numpy()
         assert np.allclose(x, exact, atol=1e-9), solve
+    x = jf.GMRES2(1e-12).solve(A, b).numpy()   # GMResTest.testSymm / testNonSymm (GMRES2)
+    assert np.allclose(x, exact, atol=1e-9)
 
 
 # ---------------------------------------------------------------- multigrid
@@ -230,6 +267,24 @@ def test_mg_level_matrices_and_vcycle(jf, oracle, which):
     # transfer weights are exact dyadic numbers here and evaluated Lagrange
     # values in the oracle; summation orders differ: 1e-11 relative
     assert np.linalg.norm(got - exp) <= 1e-11 * np.linalg.norm(exp)
+
+
+@pytest.mark.parametrize("which,precision", [("jacobi", 0), ("chebyshev", 0), ("chebyshev", 1)])
+def test_mg_wcycle(jf, oracle, which, precision):
+    """wCycle -> mgStepW (MGPreconditionerInterface.java:200-305) against the oracle's
+    restatement, from a nonzero start iterate; it must differ from the V-cycle"""
+    sm = {"jacobi": jf.JacobiSmoother(2, 0.6), "chebyshev": jf.ChebyshevSmoother(4, 2)}[which]
+    G, O, spaces = _mg_pair(jf, oracle, [2, 3], 2, 3, sm, cycles=1, fmg=False, precision=precision)
+    n = O.level_n(2)
+    rng = np.random.default_rng(7)
+    b = rng.standard_normal(n)
+    x = 0.1 * rng.standard_normal(n)
+    got = G.wCycle(jf.DenseVector(x), jf.DenseVector(b)).numpy()
+    exp = O.wcycle(x, b)
+    tol = 1e-11 if precision == 0 else 1e-5   # FP32 hierarchy: single-precision levels
+    assert np.linalg.norm(got - exp) <= tol * np.linalg.norm(exp)
+    v = O.vcycle(x, b)
+    assert np.linalg.norm(v - exp) > 1e-6 * np.linalg.norm(exp)
 
 
 @pytest.mark.parametrize("base,k", [([5], 2), ([2, 3], 1), ([2, 3], 2), ([2, 2], 3), ([2, 3, 2], 1), ([2, 2, 3], 2)])

@@ -300,6 +300,45 @@ def test_gmres_known_answers(oracle, mat):
     assert np.allclose(x, np.linalg.solve(mat, b), atol=1e-9)
 
 
+@pytest.mark.parametrize("mat", [SYMM, NONSYMM, GOOD])
+@pytest.mark.parametrize("precond", [None, "identity"])
+def test_gmres2_known_answers(oracle, mat, precond):
+    """GMResTest.testSymm / testNonSymm / testPSymm / testPNonSymm / testGoodPNonSymm
+    (Core/src/test/java/linalg/GMResTest.java:20-40, 69-125): GMRES2(1e-12), start e_0"""
+    A = csr_of(mat)
+    b = np.array([3.0, 4.0, 5.0])
+    x, it, res, fb = oracle.solve("gmres2", A, b, np.array([1.0, 0, 0]), 1e-12, precond=precond)
+    assert np.allclose(np.array(mat) @ x, b, atol=1e-10)
+    assert np.allclose(x, np.linalg.solve(mat, b), atol=1e-9)
+
+
+def test_gmres2_identity_and_large(oracle):
+    """GMResTest.testIdentity / testPIdentity (:11-18, 60-67) and testLarge / testPLarge
+    (:42-58, 94-110) with a seeded matrix of the same construction"""
+    n = 15
+    b = np.arange(1.0, n + 1)
+    e0 = np.zeros(n)
+    e0[0] = 1
+    for pc in (None, "identity"):
+        x, it, res, fb = oracle.solve("gmres2", csr_of(np.eye(n)), b, e0, 1e-12, precond=pc)
+        assert np.allclose(x, b, atol=1e-12)
+    import scipy.sparse as sp
+    n = 2000
+    rng = np.random.default_rng(4)
+    rows = np.concatenate([np.arange(10 * n) % n, rng.integers(0, n, 10 * n)])
+    cols = np.concatenate([np.arange(10 * n) % n, rng.integers(0, n, 10 * n)])
+    vals = np.concatenate([np.full(10 * n, 0.2), rng.random(10 * n)])
+    M = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    M.sum_duplicates()
+    M.sort_indices()
+    A = oracle.CSR(n, M.indptr.astype(np.int64), M.indices.astype(np.int32), M.data.copy())
+    b = np.full(n, 10.0)
+    e0 = np.zeros(n)
+    e0[0] = 1
+    x, it, res, fb = oracle.solve("gmres2", A, b, e0, 1e-10)
+    assert np.linalg.norm(M @ x - b) <= 1.01e-10
+
+
 @pytest.mark.parametrize("n", [5, 15])
 def test_identity_solves(oracle, n):
     A = csr_of(np.eye(n))
@@ -394,6 +433,48 @@ def test_mg_vcycle_contracts(oracle, sm):
     for _ in range(5):
         x = G.vcycle(x, b)
     assert np.linalg.norm(b - A @ x) < 1e-3 * r0
+
+
+def test_mg_wcycle_contracts_more_than_vcycle(oracle):
+    """mgStepW's second coarse correction: one W-cycle leaves a smaller residual than one
+    V-cycle from the same start (only slightly: the smoothing, shared by both, leaves the
+    high frequencies that dominate the residual); on two levels, where the coarse V-cycle
+    is the exact coarse solve, the W-cycle equals pre-smoothing, two exact two-grid
+    corrections (restriction P^T, prolongation P, zero boundary values), post-smoothing,
+    composed here from scipy and the oracle's prolongation matrix"""
+    O = oracle
+    nlev = 4
+    G = O.MG([2, 2], 2, nlev, [0, 0], [1, 1], [O.GRAD_GRAD], [1.0], smoother=O.SM_JACOBI, runs=2, omega=0.6,
+             cycles=1, fmg=False)
+    A = G.level_matrix(nlev - 1).to_scipy()
+    rng = np.random.default_rng(1)
+    b = rng.standard_normal(A.shape[0])
+    b[O.boundary_nodes([16, 16], 2)] = 0
+    x0 = np.zeros_like(b)
+    rv = np.linalg.norm(b - A @ G.vcycle(x0, b))
+    rw = np.linalg.norm(b - A @ G.wcycle(x0, b))
+    assert rw < rv, (rw, rv)
+    # on two levels the coarse V-cycle is the exact coarse solve, so the W-cycle equals
+    # pre-smoothing, two exact two-grid corrections, post-smoothing
+    G2 = O.MG([4, 4], 2, 2, [0, 0], [1, 1], [O.GRAD_GRAD], [1.0], smoother=O.SM_JACOBI, runs=2, omega=0.6,
+              cycles=1, fmg=False)
+    A2 = G2.level_matrix(1).to_scipy()
+    Ac = G2.level_matrix(0).to_scipy().toarray()
+    P = O.prolongation([4, 4], 2, [0, 0], [1, 1]).to_scipy()
+    bnd_f = O.boundary_nodes([8, 8], 2)
+    bnd_c = O.boundary_nodes([4, 4], 2)
+    b2 = rng.standard_normal(A2.shape[0])
+    b2[bnd_f] = 0
+    x = G2.smooth(1, b2, np.zeros_like(b2))
+    for _ in range(2):
+        rc = P.T @ (b2 - A2 @ x)
+        rc[bnd_c] = 0
+        e = P @ np.linalg.solve(Ac, rc)
+        e[bnd_f] = 0
+        x = x + e
+    x = G2.smooth(1, b2, x)
+    got = G2.wcycle(np.zeros_like(b2), b2)
+    assert np.linalg.norm(got - x) <= 1e-12 * np.linalg.norm(x)
 
 
 @pytest.mark.parametrize("cells,k", [([3, 2], 1), ([3, 2], 2), ([2, 2, 2], 1), ([2, 1, 2], 2)])
