@@ -114,8 +114,11 @@ int jfem_vec_axpby(double a, const jfem_vec *x, double b, jfem_vec *y);
  *      (M/tensorproduct/TPCellIntegral.java:111-159) into a CSR whose pattern is
  *      the key set of SparseMatrix.getCoordinateEntryList()
  *      (M/linalg/SparseMatrix.java:402-418).  kinds/weights: nint integrals
- *      summed per entry; cell_coeff (nullable, ncells host values) multiplies
- *      every integral's weight on that cell. ---------------------------------- */
+ *      summed per entry (0 GRAD_GRAD, 1 VALUE_VALUE, 2 H1, 3 GRAD_VALUE,
+ *      4 VALUE_GRAD); weights packed in kind order, one value per scalar-weighted
+ *      integral and dim values (a constant vector) per GRAD_VALUE / VALUE_GRAD,
+ *      which need a Gauss rule; cell_coeff (nullable, ncells host values)
+ *      multiplies every integral's weight on that cell. ----------------------- */
 int jfem_assemble(jfem_space *s, int nint, const int *kinds, const double *weights, const double *cell_coeff,
                   int quad_rule, jfem_mat **out);
 /* TPRightHandSideIntegral VALUE (M/tensorproduct/TPRightHandSideIntegral.java:22-31)

@@ -64,7 +64,10 @@ class TPCellIntegral:
     GRAD_GRAD = "GradGrad"
     VALUE_VALUE = "ValueValue"
     H1 = "H1"
-    _KIND = {GRAD_GRAD: 0, VALUE_VALUE: 1, H1: 2}
+    GRAD_VALUE = "GradValue"
+    VALUE_GRAD = "ValueGrad"
+    _KIND = {GRAD_GRAD: 0, VALUE_VALUE: 1, H1: 2, GRAD_VALUE: 3, VALUE_GRAD: 4}
+    _VECTOR = (GRAD_VALUE, VALUE_GRAD)
 
     def __init__(self, *args, quadratureRule1D: int = QuadratureRule1D.Gauss5):
         # TPCellIntegral(name) | TPCellIntegral(weight, name) | (weight, name, rule)
@@ -78,9 +81,15 @@ class TPCellIntegral:
             raise TypeError("TPCellIntegral(name) or TPCellIntegral(weight, name[, rule])")
         if name not in self._KIND:
             raise N.UnsupportedOperationException(3, "unknown integral name")
-        if not isinstance(weight, (int, float)):
+        if name in self._VECTOR:
+            # TPCellIntegral ctor (:50-55): a Vector weight (constant here)
+            if isinstance(weight, (int, float)) or np.ndim(weight) != 1:
+                raise N.IllegalArgumentException(1, f"{name} needs a Vector weight")
+            self.weight = [float(w) for w in weight]
+        elif not isinstance(weight, (int, float)):
             raise N.IllegalArgumentException(1, f"{name} needs a Double weight")
-        self.weight = float(weight)
+        else:
+            self.weight = float(weight)
         self.name = name
         self.quadratureRule1D = quadratureRule1D
 
@@ -347,7 +356,15 @@ class ContinuousTPFESpace:
         if len(rules) != 1:
             raise N.UnsupportedOperationException(3, "all integrals must share one quadrature rule")
         kinds = i32([ci.kind for ci in cellIntegrals])
-        w = f64([ci.weight for ci in cellIntegrals])
+        packed = []
+        for ci in cellIntegrals:
+            if isinstance(ci.weight, list):
+                if len(ci.weight) != self.dim:
+                    raise N.IllegalArgumentException(1, f"{ci.name} weight must have {self.dim} components")
+                packed += ci.weight
+            else:
+                packed.append(ci.weight)
+        w = f64(packed)
         cc = _cell_array(cell_coeff, self, "cell_coeff")
         h = C.c_void_p()
         check(N.lib().jfem_assemble(self.h, len(kinds), iptr(kinds), dptr(w), dptr(cc) if cc is not None else None,
@@ -480,6 +497,8 @@ class MGPreconditioner:
         opts.fmg = 1 if fmg else 0
         opts.precision = precision
         opts.galerkin = 1 if galerkin else 0   # coarse systems P^T A P (AMGFromMatrix) instead of re-assembly
+        if any(isinstance(ci.weight, list) for ci in cellIntegrals):
+            raise N.UnsupportedOperationException(3, "multigrid hierarchies take scalar-weighted integrals")
         kinds = i32([ci.kind for ci in cellIntegrals])
         w = f64([ci.weight for ci in cellIntegrals])
         arr = (C.c_void_p * len(spaces))(*[s.h for s in spaces])

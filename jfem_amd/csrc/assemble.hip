@@ -145,6 +145,8 @@ struct ElemParams {
     int d, k, Q, nloc, nint;
     int kinds[4];
     double wconst[4];
+    double wvec[4][MAXD];   // constant vector weight of GRAD_VALUE / VALUE_GRAD
+    int full;               // 1: every (iv, iu) pair is listed (non-symmetric integrals), no mirrored write
     int ncls[MAXD];
     long ntuples;
 };
@@ -156,7 +158,8 @@ struct ElemParams {
 // written to E[iv][iu] and E[iu][iv].  Swapping the two functions leaves every
 // operation of the sum unchanged (products commute exactly in IEEE arithmetic
 // and the quadrature order is the same), so the table is symmetric bit for
-// bit.  DD, KK, QQ > 0 fix dimension, degree and points per axis, so the
+// bit.  With a non-symmetric integral (GRAD_VALUE, VALUE_GRAD) every pair is
+// listed (P.full) and only E[iv][iu] is written.  DD, KK, QQ > 0 fix dimension, degree and points per axis, so the
 // quadrature loop unrolls and the 1D tables stay in registers.
 template <int DD, int KK, int QQ>
 __global__ __launch_bounds__(256) void k_element_tables(ElemParams P, const Tab1D* __restrict__ tabs0,
@@ -167,7 +170,7 @@ __global__ __launch_bounds__(256) void k_element_tables(ElemParams P, const Tab1
 #pragma clang fp contract(off)
     const int d = DD ? DD : P.d, k = KK ? KK : P.k, Q = QQ ? QQ : P.Q;
     const long nent = (long)P.nloc * P.nloc;
-    const long npair = (long)P.nloc * (P.nloc + 1) / 2;
+    const long npair = P.full ? (long)P.nloc * P.nloc : (long)P.nloc * (P.nloc + 1) / 2;
     const long total = npair * P.ntuples * P.nint;
     for (long id = blockIdx.x * (long)blockDim.x + threadIdx.x; id < total; id += (long)gridDim.x * blockDim.x) {
         const int pr = pairs[id % npair];
@@ -206,6 +209,20 @@ __global__ __launch_bounds__(256) void k_element_tables(ElemParams P, const Tab1
             }
             if (kind == 1) {
                 val = vu * vv * wc;
+            } else if (kind >= 3) {
+                // GRAD_VALUE: (grad u . w) v; VALUE_GRAD: (grad v . w) u (TPCellIntegral.java:142-157)
+                const int* lg = kind == 3 ? lu : lv;
+                double inner = 0;
+#pragma unroll
+                for (int i = 0; i < MAXD; i++) {
+                    if (i >= d) break;
+                    double g = 1;
+#pragma unroll
+                    for (int j = 0; j < MAXD; j++)
+                        if (j < d) g *= i == j ? tb[j]->der[lg[j]][c[j]] : tb[j]->val[lg[j]][c[j]];
+                    inner += g * P.wvec[ii][i];
+                }
+                val = inner * (kind == 3 ? vv : vu);
             } else {
                 double inner = 0;
 #pragma unroll
@@ -229,7 +246,7 @@ __global__ __launch_bounds__(256) void k_element_tables(ElemParams P, const Tab1
         }
         const long base = ((long)ii * P.ntuples + t) * nent;
         E[base + (long)iv * P.nloc + iu] = ret;
-        E[base + (long)iu * P.nloc + iv] = ret;
+        if (!P.full) E[base + (long)iu * P.nloc + iv] = ret;
     }
 }
 
@@ -1004,8 +1021,17 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     jfem_ctx* c = s->ctx;
     const Grid& g = s->grid;
     JFEM_CHECK(nint >= 1 && nint <= 4, JFEM_ERR_ARG, "1..4 integrals supported");
-    for (int i = 0; i < nint; i++)
-        JFEM_CHECK(kinds[i] >= 0 && kinds[i] <= 2, JFEM_ERR_UNSUPPORTED, "unknown integral name");
+    bool full = false;
+    for (int i = 0; i < nint; i++) {
+        JFEM_CHECK(kinds[i] >= 0 && kinds[i] <= 4, JFEM_ERR_UNSUPPORTED, "unknown integral name");
+        if (kinds[i] >= 3) {
+            full = true;
+            // these integrands use value(x) / gradient(x) of the whole shape function (the first support
+            // cell containing x), which differs from the integrated cell only at points on its faces
+            JFEM_CHECK(rule >= 1 && rule <= 5, JFEM_ERR_UNSUPPORTED,
+                       "GRAD_VALUE / VALUE_GRAD need a Gauss rule (Gauss-Lobatto points lie on cell faces)");
+        }
+    }
     ClassTables C = build_classes(g, rule);
     DevTables D;
     upload_classes(c, g, C, D);
@@ -1014,16 +1040,26 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     EP.d = g.d; EP.k = g.k; EP.Q = C.Q; EP.nloc = g.nloc(); EP.nint = nint;
     EP.ntuples = 1;
     for (int a = 0; a < g.d; a++) { EP.ncls[a] = C.ncls[a]; EP.ntuples *= C.ncls[a]; }
-    for (int i = 0; i < nint; i++) { EP.kinds[i] = kinds[i]; EP.wconst[i] = weights[i]; }
+    // weights are packed: one value per scalar-weighted integral, dim values per vector-weighted one
+    EP.full = full ? 1 : 0;
+    for (int i = 0, o = 0; i < nint; i++) {
+        EP.kinds[i] = kinds[i];
+        if (kinds[i] >= 3) {
+            for (int a = 0; a < g.d; a++) EP.wvec[i][a] = weights[o + a];
+            o += g.d;
+        } else {
+            EP.wconst[i] = weights[o++];
+        }
+    }
     const long Esize = (long)nint * EP.ntuples * EP.nloc * EP.nloc;
     JFEM_CHECK(Esize < (1l << 30), JFEM_ERR_UNSUPPORTED, "too many distinct cell geometries");
     double* E = nullptr;
     JFEM_HIP(hipMalloc(&E, sizeof(double) * Esize));
     {
-        // node pairs iv <= iu, packed iv << 16 | iu
+        // node pairs iv <= iu (all pairs for non-symmetric integrals), packed iv << 16 | iu
         std::vector<int> pairs;
         for (int v = 0; v < EP.nloc; v++)
-            for (int u = v; u < EP.nloc; u++) pairs.push_back(v << 16 | u);
+            for (int u = full ? 0 : v; u < EP.nloc; u++) pairs.push_back(v << 16 | u);
         int* pairs_d = nullptr;
         JFEM_HIP(hipMalloc(&pairs_d, sizeof(int) * pairs.size()));
         JFEM_HIP(hipMemcpyAsync(pairs_d, pairs.data(), sizeof(int) * pairs.size(), hipMemcpyHostToDevice, c->stream));

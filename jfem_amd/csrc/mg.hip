@@ -1458,6 +1458,9 @@ int jfem_mg_create(jfem_space* const* spaces, int nlev, int nint, const int* kin
 {
     API_BEGIN
     JFEM_CHECK(out && kinds && weights, JFEM_ERR_ARG, "null argument");
+    for (int i = 0; i < nint; i++)
+        JFEM_CHECK(kinds[i] >= 0 && kinds[i] <= 2, JFEM_ERR_UNSUPPORTED,
+                   "multigrid hierarchies take GRAD_GRAD, VALUE_VALUE and H1 integrals");
     *out = create_mg(spaces, nlev, nint, kinds, weights, cell_coeff, quad_rule, opts);
     API_END
 }

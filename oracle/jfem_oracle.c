@@ -429,8 +429,29 @@ static int setup_grid(const orc_asm_desc *D)
 }
 static void teardown_grid(void) { free(g_s1d); free(g_e1d); g_s1d = NULL; g_e1d = NULL; }
 
+/* constant vector weight for GRAD_VALUE / VALUE_GRAD: ctx holds d values (scaled per cell) */
+static void const_vfun(const double *x, int d, double *out, void *ctx)
+{
+    (void)x;
+    const double *w = (const double *)ctx;
+    for (int i = 0; i < d; i++) out[i] = w[i];
+}
+
+orc_csr *orc_assemble_w(int d, int k, const int *cells, const double *start, const double *end, int nint,
+                        const int *kinds, const double *wconst, const double *wvec, const double *cell_coeff,
+                        int rule);
+
 orc_csr *orc_assemble(int d, int k, const int *cells, const double *start, const double *end,
                       int nint, const int *kinds, const double *wconst, const double *cell_coeff, int rule)
+{
+    return orc_assemble_w(d, k, cells, start, end, nint, kinds, wconst, NULL, cell_coeff, rule);
+}
+
+/* as orc_assemble; wvec (nint x d, nullable) holds the constant vector weight of each
+ * GRAD_VALUE / VALUE_GRAD integral, multiplied by the cell coefficient on each cell */
+orc_csr *orc_assemble_w(int d, int k, const int *cells, const double *start, const double *end, int nint,
+                        const int *kinds, const double *wconst, const double *wvec, const double *cell_coeff,
+                        int rule)
 {
     orc_asm_desc D;
     memset(&D, 0, sizeof(D));
@@ -461,7 +482,14 @@ orc_csr *orc_assemble(int d, int k, const int *cells, const double *start, const
                 double integral = 0;
                 for (int i = 0; i < nint; i++) {
                     const double w = cell_coeff ? cell_coeff[c] * wconst[i] : wconst[i];
-                    integral += integral_tabs(&K, &T, kinds[i], lu, lv, w, NULL, NULL, NULL);
+                    if (kinds[i] == ORC_GRAD_VALUE || kinds[i] == ORC_VALUE_GRAD) {
+                        double wv[ORC_MAXD];
+                        for (int a = 0; a < d; a++)
+                            wv[a] = cell_coeff ? cell_coeff[c] * wvec[i * d + a] : wvec[i * d + a];
+                        integral += integral_tabs(&K, &T, kinds[i], lu, lv, 0.0, NULL, const_vfun, wv);
+                    } else {
+                        integral += integral_tabs(&K, &T, kinds[i], lu, lv, w, NULL, NULL, NULL);
+                    }
                 }
                 if (integral != 0) {
                     coo[m].row = cd[c * nloc + iv];

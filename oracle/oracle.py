@@ -62,6 +62,8 @@ def lib():
                                         C.c_double, _SFUN, _VFUN, C.c_void_p]
         L.orc_assemble.restype = C.POINTER(_CSR)
         L.orc_assemble.argtypes = [C.c_int, C.c_int, ip, dp, dp, C.c_int, ip, dp, dp, C.c_int]
+        L.orc_assemble_w.restype = C.POINTER(_CSR)
+        L.orc_assemble_w.argtypes = [C.c_int, C.c_int, ip, dp, dp, C.c_int, ip, dp, dp, dp, C.c_int]
         L.orc_csr_free.argtypes = [C.POINTER(_CSR)]
         L.orc_element_matrix.argtypes = [C.c_int, C.c_int, ip, dp, dp, C.c_long, C.c_int, ip, dp, C.c_int, dp]
         L.orc_assemble_rhs_sines.argtypes = [C.c_int, C.c_int, ip, dp, dp, C.c_int, C.c_int, ip, dp, dp]
@@ -200,6 +202,22 @@ def _take_csr(ptr, ncols=None):
 
 
 def assemble(cells, k, start, end, kinds, weights, cell_coeff=None, rule=5):
+    """weights: one float per integral, or a sequence of dim floats for GRAD_VALUE / VALUE_GRAD"""
+    if any(kk in (GRAD_VALUE, VALUE_GRAD) for kk in kinds):
+        d = len(cells)
+        wc = np.array([0.0 if np.ndim(w) else float(w) for w in weights])
+        wv = np.zeros(len(kinds) * d)
+        for i, w in enumerate(weights):
+            if np.ndim(w):
+                wv[i * d:(i + 1) * d] = w
+        cells_ = np.ascontiguousarray(cells, dtype=np.int32)
+        kinds_ = np.ascontiguousarray(kinds, dtype=np.int32)
+        cc = None if cell_coeff is None else np.ascontiguousarray(cell_coeff, dtype=np.float64)
+        ptr = lib().orc_assemble_w(d, k, _i(cells_), _d(np.asarray(start, float)), _d(np.asarray(end, float)),
+                                   len(kinds_), _i(kinds_), _d(wc), _d(wv), _d(cc) if cc is not None else None, rule)
+        if not ptr:
+            raise ValueError("oracle assembly failed")
+        return _take_csr(ptr)
     cells = np.ascontiguousarray(cells, dtype=np.int32)
     d = len(cells)
     s = np.ascontiguousarray(start, dtype=np.float64)

@@ -60,6 +60,33 @@ def test_assembly_bit_exact(jf, oracle, cells, k, kinds, weights):
     assert np.array_equal(vals, ref.vals), np.max(np.abs(vals - ref.vals))
 
 
+@pytest.mark.parametrize("cells,k", [([3, 2], 1), ([4, 3], 2), ([2, 1, 2], 1), ([2, 2, 2], 2)])
+def test_assembly_grad_value_bit_exact(jf, oracle, cells, k):
+    """GRAD_VALUE / VALUE_GRAD with a constant vector weight (TPCellIntegral.java:142-157),
+    alone and summed with GRAD_GRAD: pattern and values bit-exact (the same operation order)"""
+    s, start, end = _space(jf, cells, k)
+    d = len(cells)
+    w = [0.3, -1.2, 0.7][:d]
+    TP = jf.TPCellIntegral
+    for ints, kinds, weights in [([TP(w, TP.GRAD_VALUE)], [3], [w]), ([TP(w, TP.VALUE_GRAD)], [4], [w]),
+                                 ([TP(0.5, TP.GRAD_GRAD), TP(w, TP.VALUE_GRAD)], [0, 4], [0.5, w])]:
+        A = s.assemble_matrix(ints)
+        rp, cols, vals = A.to_csr()
+        ref = oracle.assemble(cells, k, start, end, kinds, weights)
+        assert np.array_equal(rp, ref.rowptr) and np.array_equal(cols, ref.cols), kinds
+        assert np.array_equal(vals, ref.vals), (kinds, np.max(np.abs(vals - ref.vals)))
+    rng = np.random.default_rng(3)
+    coeff = np.exp(rng.standard_normal(int(np.prod(cells))))
+    A = s.assemble_matrix([TP(w, TP.GRAD_VALUE)], cell_coeff=coeff)
+    rp, cols, vals = A.to_csr()
+    ref = oracle.assemble(cells, k, start, end, [3], [w], cell_coeff=coeff)
+    assert np.array_equal(rp, ref.rowptr) and np.array_equal(cols, ref.cols)
+    scale = np.repeat(np.maximum.reduceat(np.abs(ref.vals), ref.rowptr[:-1]), np.diff(ref.rowptr))
+    assert np.max(np.abs(vals - ref.vals) / np.maximum(scale, 1e-300)) < 1e-14
+    with pytest.raises(jf.UnsupportedOperationException):   # Gauss-Lobatto points on faces
+        s.assemble_matrix([TP(w, TP.GRAD_VALUE, 15)])
+
+
 def test_assembly_with_cell_coefficient(jf, oracle):
     cells, k = [6, 5, 4], 2
     s, start, end = _space(jf, cells, k)

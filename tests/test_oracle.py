@@ -435,6 +435,27 @@ def test_mg_vcycle_contracts(oracle, sm):
     assert np.linalg.norm(b - A @ x) < 1e-3 * r0
 
 
+@pytest.mark.parametrize("cells,k", [([3, 2], 1), ([2, 3], 2), ([2, 1, 2], 1)])
+def test_assembly_grad_value_invariants(oracle, cells, k):
+    """GRAD_VALUE (grad u . w) v and VALUE_GRAD (grad v . w) u with a constant vector w: for
+    the interpolant of a linear f(x) = a . x (exact in Qk), A_GV f = (a . w) M 1 (the
+    VALUE_VALUE matrix times ones), A_GV 1 = 0, and VALUE_GRAD is the transpose of GRAD_VALUE"""
+    O = oracle
+    d = len(cells)
+    start, end = [0.0, -1.0, 0.5][:d], [1.0, 0.5, 2.0][:d]
+    w = [0.3, -1.2, 0.7][:d]
+    a = np.array([1.5, -0.4, 2.0][:d])
+    GV = O.assemble(cells, k, start, end, [O.GRAD_VALUE], [w]).to_scipy()
+    VG = O.assemble(cells, k, start, end, [O.VALUE_GRAD], [w]).to_scipy()
+    M = O.assemble(cells, k, start, end, [O.VALUE_VALUE], [1.0]).to_scipy()
+    x = O.node_coords(cells, k, start, end)
+    f = x @ a
+    one = np.ones(GV.shape[0])
+    assert np.allclose(GV @ f, (a @ w) * (M @ one), rtol=0, atol=1e-12)
+    assert np.allclose(GV @ one, 0, atol=1e-13)
+    assert abs(GV - VG.T).max() <= 1e-15
+
+
 def test_mg_wcycle_contracts_more_than_vcycle(oracle):
     """mgStepW's second coarse correction: one W-cycle leaves a smaller residual than one
     V-cycle from the same start (only slightly: the smoothing, shared by both, leaves the
