@@ -166,6 +166,10 @@ int jfem_mat_export_csr(const jfem_mat *A, long *rowptr, int *colidx, double *va
 int jfem_apply_dirichlet(jfem_mat *A, jfem_vec *rhs, long nb, const int *nodes, const double *values);
 /* VectorMultiplyable.mvMul (SparseMatrix.mvMul, M/linalg/SparseMatrix.java:479-489) */
 int jfem_spmv(jfem_mat *A, const jfem_vec *x, jfem_vec *y);
+/* VectorMultiplyable.tvMul (SparseMatrix.tvMul, M/linalg/SparseMatrix.java:491-501): y = A^T x.
+ * The transpose is built on the first call (stable sort by column) and kept until A's values
+ * change (jfem_apply_dirichlet drops it). */
+int jfem_spmv_t(jfem_mat *A, const jfem_vec *x, jfem_vec *y);
 /* y = A x with FP32 matrix values and FP32 vectors (device pointers to float) */
 int jfem_spmv_f32(jfem_mat *A, const float *x, float *y);
 /* repeat SpMV `reps` times on the context stream and return the mean kernel time (ms) */

@@ -100,6 +100,7 @@ _SIGS = {
     "jfem_mat_info": (C.c_int, [_vp, _lp, _lp, _lp]),
     "jfem_mat_export_csr": (C.c_int, [_vp, _lp, _ip, _dp]),
     "jfem_apply_dirichlet": (C.c_int, [_vp, _vp, C.c_long, _ip, _dp]),
+    "jfem_spmv_t": (C.c_int, [_vp, _vp, _vp]),
     "jfem_spmv": (C.c_int, [_vp, _vp, _vp]),
     "jfem_spmv_f32": (C.c_int, [_vp, _fp, _fp]),
     "jfem_spmv_bench": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _dp]),

@@ -258,6 +258,12 @@ class SparseMatrix:
         check(N.lib().jfem_spmv(self.h, v.h, y.h))
         return y
 
+    def tvMul(self, v: DenseVector, out: Optional[DenseVector] = None) -> DenseVector:
+        """SparseMatrix.tvMul (M/linalg/SparseMatrix.java:491-501): A^T v."""
+        y = out if out is not None else DenseVector(self.getCols(), self.ctx)
+        check(N.lib().jfem_spmv_t(self.h, v.h, y.h))
+        return y
+
     def spmv_time_ms(self, x: DenseVector, y: DenseVector, reps: int = 20, fp32: bool = False) -> float:
         ms = C.c_double()
         check(N.lib().jfem_spmv_bench(self.h, x.h, y.h, reps, 1 if fp32 else 0, C.byref(ms)))

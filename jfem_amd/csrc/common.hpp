@@ -185,6 +185,7 @@ struct jfem_mat {
     StBlk* blk32 = nullptr;  // the same for FP32 values (twice the entries per block)
     long nrblk32 = 0;
     double* part = nullptr;  // device: one partial sum per stream block
+    jfem_mat* T = nullptr;   // cached transpose for tvMul (dropped when values change)
 };
 
 namespace jfem {
@@ -258,6 +259,9 @@ void mat_build_blocks(jfem_mat* A);
 void reduce_array_dev(jfem_ctx* c, const double* partials, long n, double* dst);
 void mat_diag(jfem_mat* A, double* out);
 void mat_free(jfem_mat* A);
+// the transpose of A (built on first use, cached until A's values change)
+jfem_mat* mat_transpose(jfem_mat* A);
+void mat_drop_transpose(jfem_mat* A);
 void apply_dirichlet_mask(jfem_mat* A, double* rhs, const unsigned char* fixed_dev, const double* g_dev);
 void space_boundary_mask(jfem_space* s, unsigned char* mask_dev);
 jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const double* weights, const double* coeff_host,

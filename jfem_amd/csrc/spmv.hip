@@ -11,6 +11,8 @@
 #include "common.hpp"
 #include "stream.hpp"
 
+#include <hipcub/hipcub.hpp>
+
 #include <cstring>
 
 namespace jfem {
@@ -384,6 +386,7 @@ static void insert_fixed_diagonals(jfem_mat* A, const unsigned char* fixed_dev)
 void apply_dirichlet_mask(jfem_mat* A, double* rhs, const unsigned char* fixed_dev, const double* g_dev)
 {
     jfem_ctx* c = A->ctx;
+    mat_drop_transpose(A);   // the values change
     insert_fixed_diagonals(A, fixed_dev);
     int* flag = nullptr;
     JFEM_HIP(hipMalloc(&flag, sizeof(int)));
@@ -404,9 +407,94 @@ void apply_dirichlet_mask(jfem_mat* A, double* rhs, const unsigned char* fixed_d
     }
 }
 
+// ---- transpose (SparseMatrix.tvMul, M/linalg/SparseMatrix.java:491-501) ----
+__global__ void k_entry_rows(long n, const int* __restrict__ rowptr, int* __restrict__ rows)
+{
+    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x)
+        for (int p = rowptr[r]; p < rowptr[r + 1]; p++) rows[p] = (int)r;
+}
+
+__global__ void k_iota(long m, int* __restrict__ v)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < m; i += (long)gridDim.x * blockDim.x) v[i] = (int)i;
+}
+
+__global__ void k_col_counts(long m, const int* __restrict__ cols, int* __restrict__ counts)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < m; i += (long)gridDim.x * blockDim.x)
+        atomicAdd(&counts[cols[i]], 1);
+}
+
+__global__ void k_gather_transpose(long m, const int* __restrict__ perm, const int* __restrict__ rows,
+                                   const double* __restrict__ vals, int* __restrict__ tcols, double* __restrict__ tvals)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < m; i += (long)gridDim.x * blockDim.x) {
+        const int e = perm[i];
+        tcols[i] = rows[e];
+        tvals[i] = vals[e];
+    }
+}
+
+void mat_drop_transpose(jfem_mat* A)
+{
+    if (A && A->T) {
+        mat_free(A->T);
+        A->T = nullptr;
+    }
+}
+
+// entries sorted by column with a stable radix sort (equal columns keep their row order),
+// so every row of the transpose lists its columns ascending and the result is deterministic
+jfem_mat* mat_transpose(jfem_mat* A)
+{
+    if (A->T) return A->T;
+    jfem_ctx* c = A->ctx;
+    hipStream_t st = c->stream;
+    const long m = A->nnz;
+    auto* T = new jfem_mat();
+    T->ctx = c;
+    T->rows = A->cols;
+    T->cols = A->rows;
+    T->nnz = m;
+    try {
+        DevBuf<int> rows(m), keys_out(m), idx(m), perm(m), counts(T->rows);
+        JFEM_HIP(hipMalloc(&T->rowptr, sizeof(int) * (T->rows + 1)));
+        JFEM_HIP(hipMalloc(&T->colidx, sizeof(int) * (m + NNZ_PAD)));
+        JFEM_HIP(hipMalloc(&T->vals, sizeof(double) * (m + NNZ_PAD)));
+        JFEM_HIP(hipMemsetAsync(T->colidx, 0, sizeof(int) * (m + NNZ_PAD), st));
+        JFEM_HIP(hipMemsetAsync(T->vals, 0, sizeof(double) * (m + NNZ_PAD), st));
+        JFEM_HIP(hipMemsetAsync(counts, 0, sizeof(int) * (T->rows > 0 ? T->rows : 1), st));
+        if (A->rows) k_entry_rows<<<grid_for(A->rows, 256, 16384), 256, 0, st>>>(A->rows, A->rowptr, rows);
+        if (m) {
+            k_iota<<<grid_for(m, 256, 16384), 256, 0, st>>>(m, idx);
+            k_col_counts<<<grid_for(m, 256, 16384), 256, 0, st>>>(m, A->colidx, counts);
+            int bits = 1;
+            while (bits < 31 && (1l << bits) < T->rows) bits++;
+            size_t tmp_bytes = 0;
+            JFEM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, A->colidx, keys_out.p, idx.p, perm.p,
+                                                        (int)m, 0, bits, st));
+            DevBuf<char> tmp(tmp_bytes);
+            JFEM_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, A->colidx, keys_out.p, idx.p, perm.p,
+                                                        (int)m, 0, bits, st));
+            k_gather_transpose<<<grid_for(m, 256, 16384), 256, 0, st>>>(m, perm, rows, A->vals, T->colidx, T->vals);
+        }
+        JFEM_HIP(hipGetLastError());
+        const long tot = scan_counts(c, counts, T->rows, T->rowptr);
+        JFEM_CHECK(tot == m, JFEM_ERR_STATE, "transpose lost entries");
+        T->lanes = choose_lanes(T->rows > 0 ? (double)m / T->rows : 1.0);
+        mat_build_blocks(T);
+    } catch (...) {
+        mat_free(T);
+        throw;
+    }
+    A->T = T;
+    return T;
+}
+
 void mat_free(jfem_mat* A)
 {
     if (!A) return;
+    mat_drop_transpose(A);
     (void)hipStreamSynchronize(A->ctx->stream);
     (void)hipFree(A->rowptr);
     (void)hipFree(A->colidx);
@@ -513,6 +601,18 @@ int jfem_apply_dirichlet(jfem_mat* A, jfem_vec* rhs, long nb, const int* nodes, 
     // the host arrays must outlive the copies
     JFEM_HIP(hipStreamSynchronize(A->ctx->stream));
     apply_dirichlet_mask(A, rhs ? rhs->d : nullptr, dm, values ? dg.p : nullptr);
+    API_END
+}
+
+int jfem_spmv_t(jfem_mat* A, const jfem_vec* x, jfem_vec* y)
+{
+    API_BEGIN
+    JFEM_CHECK(A && x && y, JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(x->n == A->rows && y->n == A->cols, JFEM_ERR_ARG, "Incompatible sizes");
+    JFEM_CHECK(A->nnz < (1l << 31) - 1, JFEM_ERR_UNSUPPORTED, "2^31 or more entries");
+    spmv(mat_transpose(A), x->d, y->d);
+    JFEM_HIP(hipGetLastError());
+    JFEM_HIP(hipStreamSynchronize(A->ctx->stream));
     API_END
 }
 

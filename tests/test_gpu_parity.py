@@ -87,6 +87,35 @@ def test_assembly_grad_value_bit_exact(jf, oracle, cells, k):
         s.assemble_matrix([TP(w, TP.GRAD_VALUE, 15)])
 
 
+def test_tvmul_transpose(jf, oracle):
+    """SparseMatrix.tvMul (M/linalg/SparseMatrix.java:491-501) on a non-symmetric GRAD_VALUE
+    matrix: A^T x against scipy, equal to the VALUE_GRAD matrix's mvMul up to rounding; after
+    Dirichlet elimination the cached transpose follows the new values"""
+    cells, k = [5, 4], 2
+    s, start, end = _space(jf, cells, k)
+    TP = jf.TPCellIntegral
+    w = [0.7, -0.4]
+    A = s.assemble_matrix([TP(w, TP.GRAD_VALUE)])
+    B = s.assemble_matrix([TP(w, TP.VALUE_GRAD)])
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(A.getRows())
+    rp, cols, vals = A.to_csr()
+    import scipy.sparse as sp
+    S = sp.csr_matrix((vals, cols, rp), shape=(A.getRows(), A.getCols()))
+    got = A.tvMul(jf.DenseVector(x)).numpy()
+    assert np.linalg.norm(got - S.T @ x) <= 1e-14 * np.linalg.norm(S.T @ x)
+    assert np.linalg.norm(got - B.mvMul(jf.DenseVector(x)).numpy()) <= 1e-13 * np.linalg.norm(got)
+    assert np.array_equal(got, A.tvMul(jf.DenseVector(x)).numpy())   # cached, deterministic
+    nodes = s.boundary_nodes()
+    rhs = jf.DenseVector(A.getRows())
+    check = jf._native.check
+    check(jf.lib().jfem_apply_dirichlet(A.h, rhs.h, len(nodes), jf._native.iptr(nodes), None))
+    rp, cols, vals = A.to_csr()
+    S = sp.csr_matrix((vals, cols, rp), shape=(A.getRows(), A.getCols()))
+    got = A.tvMul(jf.DenseVector(x)).numpy()
+    assert np.linalg.norm(got - S.T @ x) <= 1e-14 * np.linalg.norm(S.T @ x)
+
+
 def test_assembly_with_cell_coefficient(jf, oracle):
     cells, k = [6, 5, 4], 2
     s, start, end = _space(jf, cells, k)
