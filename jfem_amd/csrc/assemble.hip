@@ -1207,7 +1207,7 @@ void assemble_rhs_sines(jfem_space* s, int nterms, const int* kw, const double* 
     double qp[8], qw[8];
     quad_rule(rule, qp, qw);
     // per-axis tables sin(kw * pi * x_q) (cos where the cosine flag is set); the amplitude is folded into axis 0
-    std::vector<double*> dsin(g.d, nullptr);
+    DevBuf<double> dsin[MAXD];
     RhsParams P;
     std::memset(&P, 0, sizeof(P));
     P.g = to_dev(g);
@@ -1225,8 +1225,9 @@ void assemble_rhs_sines(jfem_space* s, int nterms, const int* kw, const double* 
                     if (a == 0) v = amp[m] * v;
                     t[((size_t)m * g.n[a] + j) * C.Q + q] = v;
                 }
-        JFEM_HIP(hipMalloc(&dsin[a], sizeof(double) * t.size()));
+        dsin[a].alloc(t.size());
         JFEM_HIP(hipMemcpyAsync(dsin[a], t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, c->stream));
+        JFEM_HIP(hipStreamSynchronize(c->stream));   // t leaves scope
         P.sinq[a] = dsin[a];
         P.cls[a] = D.cls[a];
         P.ncls[a] = C.ncls[a];
@@ -1257,7 +1258,6 @@ void assemble_rhs_sines(jfem_space* s, int nterms, const int* kw, const double* 
         JFEM_HIP(hipGetLastError());
     }
     JFEM_HIP(hipStreamSynchronize(c->stream));
-    for (auto p : dsin) (void)hipFree(p);
 }
 
 void space_boundary_mask(jfem_space* s, unsigned char* mask_dev)

@@ -276,7 +276,7 @@ static void solve_bicgstab(jfem_mat* A, jfem_mg* M, const double* b, double* x, 
             // the reference restarts from its default start iterate: e_0 (GMRES) or 0 (PGMRES)
             vec_fill(c, x, 0.0, n);
             if (!M) {
-                const double one = 1.0;
+                static const double one = 1.0;   // outlives the asynchronous copy
                 JFEM_HIP(hipMemcpyAsync(x, &one, sizeof(double), hipMemcpyHostToDevice, c->stream));
             }
             solve_gmres(A, M, b, x, tol, 150, 1000, info);
@@ -676,6 +676,26 @@ struct StokesPrec {
     StokesPrec(jfem_mat* A_, jfem_mg* M_, const double* ps_, long np_, const jfem_stokes_opts* o)
         : A(A_), M(M_), np(np_), ps(ps_)
     {
+        try {
+            init(o);
+        } catch (...) {
+            release();
+            throw;
+        }
+    }
+    void release()
+    {
+        mat_free(B);
+        B = nullptr;
+        if (ev_in) (void)hipEventDestroy(ev_in);
+        ev_in = nullptr;
+        for (auto& e : ev_done) {
+            if (e) (void)hipEventDestroy(e);
+            e = nullptr;
+        }
+    }
+    void init(const jfem_stokes_opts* o)
+    {
         nv = mg_size(M);
         const long n = A->rows;
         JFEM_CHECK(nv > 0 && np > 0 && (n - np) % nv == 0, JFEM_ERR_ARG, "system size is not dim * velocity + pressure");
@@ -716,7 +736,7 @@ struct StokesPrec {
         JFEM_HIP(hipMemcpyAsync(&base, A->rowptr + first, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         JFEM_HIP(hipStreamSynchronize(c->stream));
         B = new jfem_mat();
-        try {
+        {
             B->ctx = c;
             B->rows = np;
             B->cols = n;
@@ -730,9 +750,6 @@ struct StokesPrec {
             JFEM_HIP(hipMemcpyAsync(B->vals, A->vals + base, sizeof(double) * B->nnz, hipMemcpyDeviceToDevice, c->stream));
             B->lanes = choose_lanes((double)B->nnz / np);
             mat_build_blocks(B);
-        } catch (...) {
-            mat_free(B);
-            throw;
         }
         if (o && o->pressure_mass && o->mass_its > 0) {
             Mp = o->pressure_mass;
@@ -758,12 +775,7 @@ struct StokesPrec {
             JFEM_HIP(hipGetLastError());
         }
     }
-    ~StokesPrec()
-    {
-        mat_free(B);
-        if (ev_in) (void)hipEventDestroy(ev_in);
-        for (auto e : ev_done) if (e) (void)hipEventDestroy(e);
-    }
+    ~StokesPrec() { release(); }
     // z = P^-1 r
     void apply(const double* r, double* z)
     {
