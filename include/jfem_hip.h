@@ -246,12 +246,17 @@ typedef struct {
     double cheb_safety;/* lmax = safety * (power-iteration estimate) */
     int power_its;     /* power iterations for the estimate */
     int cycles;        /* V-cycles per application (reference: 3) */
-    int fmg;           /* 1: reference mvMul (copy BC, full-V correction, cycles V) */
+    int fmg;           /* 1: reference mvMul (copy BC, full-V correction, cycles V);
+                        * 2: the same from v itself instead of its boundary values */
     int precision;     /* JFEM_FP64 or JFEM_FP32 */
     int galerkin;      /* 1: coarse systems A_{l-1} = P^T A_l P from the finest assembled one
                         * (AMGFromMatrix, M/multigrid/MGPreconditionerInterface.java:387-410)
                         * instead of re-assembly per level; the transfers keep their zero
                         * boundary conditions */
+    int raw_transfers; /* 1: MGSpace (M/multigrid/MGSpace.java:25-206): restriction and
+                        * prolongation by the prolongation matrix without zero boundary values.
+                        * With fmg = 2 and cycles = 1 the application is MGPReconditioner2.mvMul
+                        * (M/multigrid/MGPReconditioner2.java:38-76): v + fullVCycleSolver(v - A v) */
 } jfem_mg_opts;
 
 /* levels: nlev spaces, coarsest first, each with twice the cells of the

@@ -4,7 +4,7 @@ Dirichlet elimination, SpMV, CG/BiCGStab/GMRES and geometric multigrid, all in
 libjfem_hip.so behind the C ABI of include/jfem_hip.h."""
 from ._native import (IllegalArgumentException, IllegalStateException, JfemError, NativeLibraryError,
                       UnsupportedOperationException, lib)
-from .api import (FP32, FP64, GMRES2, ChebyshevSmoother, ContinuousTPFESpace, Context, DenseVector, IterativeSolver,
+from .api import (FP32, FP64, GMRES2, MGSpacePreconditioner, ChebyshevSmoother, ContinuousTPFESpace, Context, DenseVector, IterativeSolver,
                   JacobiSmoother, MGPreconditioner, QuadratureRule1D, RichardsonSmoother, ScalarFESpaceFunction,
                   SinusoidSource, SmootherNet, SparseMatrix, TPCellIntegral, TPRightHandSideIntegral, context,
                   normL2Difference)
@@ -15,7 +15,7 @@ from .mixed import (MixedCellIntegral, MixedRightHandSideIntegral, MixedTPCellIn
 
 __all__ = [
     "IllegalArgumentException", "IllegalStateException", "JfemError", "NativeLibraryError",
-    "UnsupportedOperationException", "lib", "FP32", "FP64", "GMRES2", "ChebyshevSmoother", "ContinuousTPFESpace", "Context",
+    "UnsupportedOperationException", "lib", "FP32", "FP64", "GMRES2", "MGSpacePreconditioner", "ChebyshevSmoother", "ContinuousTPFESpace", "Context",
     "DenseVector", "IterativeSolver", "JacobiSmoother", "MGPreconditioner", "QuadratureRule1D",
     "RichardsonSmoother", "ScalarFESpaceFunction", "SinusoidSource", "SmootherNet", "SparseMatrix", "TPCellIntegral",
     "TPRightHandSideIntegral", "context", "normL2Difference", "MixedCellIntegral", "MixedRightHandSideIntegral",

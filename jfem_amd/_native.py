@@ -56,7 +56,8 @@ class StokesOpts(C.Structure):
 class MgOpts(C.Structure):
     _fields_ = [("smoother", C.c_int), ("runs", C.c_int), ("omega", C.c_double), ("degree", C.c_int),
                 ("cheb_ratio", C.c_double), ("cheb_safety", C.c_double), ("power_its", C.c_int),
-                ("cycles", C.c_int), ("fmg", C.c_int), ("precision", C.c_int), ("galerkin", C.c_int)]
+                ("cycles", C.c_int), ("fmg", C.c_int), ("precision", C.c_int), ("galerkin", C.c_int),
+                ("raw_transfers", C.c_int)]
 
 
 _vp = C.c_void_p

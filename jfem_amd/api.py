@@ -488,7 +488,8 @@ class MGPreconditioner:
     M/multigrid/MGPreconditionerInterface.java:352-379).  Levels: coarsest first."""
 
     def __init__(self, spaces: Sequence[ContinuousTPFESpace], cellIntegrals: Sequence[TPCellIntegral], smoother,
-                 cycles: int = 3, fmg: bool = True, precision: int = FP64, cell_coeff=None, galerkin: bool = False):
+                 cycles: int = 3, fmg=True, precision: int = FP64, cell_coeff=None, galerkin: bool = False,
+                 raw_transfers: bool = False):
         self.ctx = spaces[0].ctx
         self.spaces = list(spaces)
         opts = N.MgOpts()
@@ -500,7 +501,8 @@ class MGPreconditioner:
         opts.cheb_safety = getattr(smoother, "safety", 1.1)
         opts.power_its = getattr(smoother, "power_its", 30)
         opts.cycles = cycles
-        opts.fmg = 1 if fmg else 0
+        opts.fmg = 2 if fmg == 2 else (1 if fmg else 0)
+        opts.raw_transfers = 1 if raw_transfers else 0
         opts.precision = precision
         opts.galerkin = 1 if galerkin else 0   # coarse systems P^T A P (AMGFromMatrix) instead of re-assembly
         if any(isinstance(ci.weight, list) for ci in cellIntegrals):
@@ -616,6 +618,14 @@ class MGPreconditioner:
 # --------------------------------------------------------------------------
 # iterative solvers
 # --------------------------------------------------------------------------
+def MGSpacePreconditioner(spaces, cellIntegrals, smoother, precision: int = FP64, cell_coeff=None):
+    """MGSpace + MGPReconditioner2 (M/multigrid/MGSpace.java:25-206, MGPReconditioner2.java:38-76):
+    the older multigrid API, transfers by the raw prolongation matrix and the application
+    v + fullVCycleSolver(v - A v)"""
+    return MGPreconditioner(spaces, cellIntegrals, smoother, cycles=1, fmg=2, precision=precision,
+                            cell_coeff=cell_coeff, raw_transfers=True)
+
+
 class IterativeSolver:
     """IterativeSolver (M/linalg/IterativeSolver.java:11-402): stateful, keeps
     `iterations` and `lastSolution` (next start iterate) like the reference."""

@@ -511,7 +511,7 @@ static void restrict_to(jfem_mg* G, int lf, const T* xf, T* xc)
     // transfers apply none (AMGFromMatrix's applyZeroBoundaryConditions is empty)
     const MgLevel& F = G->lev[lf];
     const MgLevel& C = G->lev[lf - 1];
-    tp_transfer<T>(G->ctx->stream, F, C, true, xf, xc, G->opt.galerkin ? nullptr : C.bnd, 0);
+    tp_transfer<T>(G->ctx->stream, F, C, true, xf, xc, G->opt.galerkin || G->opt.raw_transfers ? nullptr : C.bnd, 0);
 }
 
 template <typename T>
@@ -519,7 +519,8 @@ static void prolong_to(jfem_mg* G, int lf, const T* xc, T* xf, bool add, bool ze
 {
     const MgLevel& F = G->lev[lf];
     const MgLevel& C = G->lev[lf - 1];
-    tp_transfer<T>(G->ctx->stream, F, C, false, xc, xf, zero_bc && !G->opt.galerkin ? F.bnd : nullptr, add ? 1 : 0);
+    tp_transfer<T>(G->ctx->stream, F, C, false, xc, xf,
+                   zero_bc && !G->opt.galerkin && !G->opt.raw_transfers ? F.bnd : nullptr, add ? 1 : 0);
 }
 
 template <typename T>
@@ -600,7 +601,10 @@ static void apply_t(jfem_mg* G, const double* rin, double* zout)
     }
     // MGPreconditionerInterface.mvMul (:352-379): initial = copyBoundaryValues(v),
     // defect = v - A initial, full-V correction, cycles V-cycles, initial + iterate
-    k_copy_bnd<<<g, 256, 0, st>>>(n, F.bnd, rin, G->init);
+    if (G->opt.fmg == 2)   // MGPReconditioner2.mvMul: v + fullVCycleSolver(v - A v)
+        vec_copy(G->ctx, G->init, rin, n);
+    else
+        k_copy_bnd<<<g, 256, 0, st>>>(n, F.bnd, rin, G->init);
     spmv(F.A, G->init, G->tmp);
     vec_copy(G->ctx, G->defect, rin, n);
     vec_axpby(G->ctx, n, -1.0, G->tmp, 1.0, G->defect);

@@ -1070,7 +1070,9 @@ typedef struct {
     orc_level *lev;     /* 0 = coarsest */
     double *coarse_dense;  /* dense copy of A_0 */
     int cycles;         /* V-cycles after the FMG correction (reference: 3) */
-    int fmg;            /* 1 = reference mvMul (copy BC, FMG, cycles V); 0 = cycles V from zero */
+    int fmg;            /* 1 = reference mvMul (copy BC, FMG, cycles V); 0 = cycles V from zero;
+                           2 = MGPReconditioner2.mvMul: v + FMG solve of v - A v */
+    int raw;            /* 1 = MGSpace transfers: the prolongation matrix without zero boundary values */
 } orc_mg;
 
 static void spmv_l(const orc_level *L, const double *x, double *y) { orc_spmv(L->n, L->rowptr, L->cols, L->vals, x, y); }
@@ -1122,13 +1124,13 @@ static void prolong(const orc_mg *G, int l, const double *xc, double *xf, int ze
 {
     const orc_level *C = &G->lev[l], *F = &G->lev[l + 1];
     orc_spmv(C->pn, C->prow, C->pcol, C->pval, xc, xf);
-    if (zero_bc) for (long i = 0; i < F->n; i++) if (F->bnd[i]) xf[i] = 0;
+    if (zero_bc && !G->raw) for (long i = 0; i < F->n; i++) if (F->bnd[i]) xf[i] = 0;
 }
 static void restrict_(const orc_mg *G, int l, const double *xf, double *xc)
 {
     const orc_level *C = &G->lev[l];
     orc_spmv_t(C->pn, C->n, C->prow, C->pcol, C->pval, xf, xc);
-    for (long i = 0; i < C->n; i++) if (C->bnd[i]) xc[i] = 0;
+    if (!G->raw) for (long i = 0; i < C->n; i++) if (C->bnd[i]) xc[i] = 0;
 }
 
 static void coarse_solve(const orc_mg *G, const double *b, double *x)
@@ -1159,6 +1161,9 @@ static void mg_step_v(const orc_mg *G, int l, double *x, const double *b)
 }
 
 void orc_mg_vcycle(const orc_mg *G, double *x, const double *b) { mg_step_v(G, G->nlev - 1, x, b); }
+
+/* MGSpace (M/multigrid/MGSpace.java:25-206): transfers by the raw prolongation matrix */
+void orc_mg_set_raw(orc_mg *G, int raw) { G->raw = raw; }
 
 /* mgStepW (MGPreconditionerInterface.java:200-291): pre-smooth, then twice
  * { restrict the defect, V-cycle (mgStepV) on the next coarser level from zero,
@@ -1197,7 +1202,10 @@ void orc_mg_apply(const orc_mg *G, const double *v, double *out)
         return;
     }
     double *init = vnew(n), *t = vnew(n), *defect = vnew(n);
-    for (long i = 0; i < n; i++) if (F->bnd[i]) init[i] = v[i];          /* copyBoundaryValues */
+    if (G->fmg == 2)   /* MGPReconditioner2.mvMul (M/multigrid/MGPReconditioner2.java:38-76): v + fullVCycleSolver(v - A v) */
+        memcpy(init, v, sizeof(double) * n);
+    else
+        for (long i = 0; i < n; i++) if (F->bnd[i]) init[i] = v[i];      /* copyBoundaryValues */
     spmv_l(F, init, t);
     for (long i = 0; i < n; i++) defect[i] = v[i] - t[i];
     /* fullVCycleCorrection (:307-326) */

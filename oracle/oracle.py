@@ -91,6 +91,7 @@ def lib():
         L.orc_mg_apply.argtypes = [C.c_void_p, dp, dp]
         L.orc_mg_vcycle.argtypes = [C.c_void_p, dp, dp]
         L.orc_mg_wcycle.argtypes = [C.c_void_p, dp, dp]
+        L.orc_mg_set_raw.argtypes = [C.c_void_p, C.c_int]
         L.orc_mg_smooth.argtypes = [C.c_void_p, C.c_int, dp, dp]
         L.orc_mg_set_interval.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
         L.orc_solve.argtypes = [C.c_int, C.c_long, lp, ip, dp, C.c_int, C.c_void_p, lp, ip, dp,
@@ -378,7 +379,8 @@ class MG:
         self.ptr = lib().orc_mg_create(len(bc), k, _i(bc), nlev, _d(np.asarray(start, float)),
                                        _d(np.asarray(end, float)), len(kinds), _i(kinds), _d(w), rule,
                                        smoother, runs, omega, degree, _d(lo) if lo is not None else None,
-                                       _d(hi) if hi is not None else None, cycles, 1 if fmg else 0)
+                                       _d(hi) if hi is not None else None, cycles,
+                                       2 if (fmg == 2 and fmg is not True) else (1 if fmg else 0))
 
     def __del__(self):
         if getattr(self, "ptr", None):
@@ -446,6 +448,10 @@ class MG:
         x = np.ascontiguousarray(x, dtype=np.float64).copy()
         lib().orc_mg_vcycle(self.ptr, _d(x), _d(np.ascontiguousarray(b, float)))
         return x
+
+    def set_raw(self, raw=True):
+        """MGSpace transfers: the prolongation matrix without zero boundary values"""
+        lib().orc_mg_set_raw(self.ptr, 1 if raw else 0)
 
     def wcycle(self, x, b):
         x = np.ascontiguousarray(x, dtype=np.float64).copy()

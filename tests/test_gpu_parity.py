@@ -362,6 +362,28 @@ def test_mg_transfers_match_prolongation_matrix(jf, oracle, base, k):
         assert np.linalg.norm(got_r - exp_r) <= 1e-14 * np.linalg.norm(exp_r) * (1 + k)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_mgspace_preconditioner_apply(jf, oracle, precision):
+    """MGSpace + MGPReconditioner2.mvMul (M/multigrid/MGSpace.java:124-206,
+    MGPReconditioner2.java:38-76): raw prolongation matrices, v + fullVCycleSolver(v - A v)"""
+    d, base, k, nlev = 2, [2, 2], 1, 4
+    spaces = []
+    for l in range(nlev):
+        s, _, _ = _space(jf, [c << l for c in base], k, [0.0] * d, [1.0] * d)
+        spaces.append(s)
+    G = jf.MGSpacePreconditioner(spaces, [jf.TPCellIntegral("GradGrad")], jf.RichardsonSmoother(0.1, 8),
+                                 precision=precision)
+    O = oracle.MG(base, k, nlev, [0.0] * d, [1.0] * d, [0], [1.0], smoother=oracle.SM_RICHARDSON, runs=8,
+                  omega=0.1, cycles=1, fmg=2)
+    O.set_raw(True)
+    n = O.level_n(nlev - 1)
+    v = np.random.default_rng(4).standard_normal(n)
+    got = G.mvMul(jf.DenseVector(v)).numpy()
+    exp = O.apply(v)
+    tol = 1e-11 if precision == 0 else 2e-5
+    assert np.linalg.norm(got - exp) <= tol * np.linalg.norm(exp)
+
+
 def test_mg_reference_apply_fmg(jf, oracle):
     """MGPreconditionerInterface.mvMul: copy BC, full-V correction, 3 V-cycles."""
     G, O, spaces = _mg_pair(jf, oracle, [2, 2], 1, 4, jf.RichardsonSmoother(0.1, 8), cycles=3, fmg=True)

@@ -417,6 +417,32 @@ def test_mg_preconditioned_gmres_laplace(oracle):
         assert res <= 1e-8
 
 
+def test_mgspace_preconditioned_gmres_laplace(oracle):
+    """LaplaceMG.testMG as written (Core/src/test/java/examples/LaplaceMG.java:23-101): MGSpace
+    with Richardson(0.1, 8) smoothers, preconditioner MGPReconditioner2 (v + fullVCycleSolver(v - A v),
+    raw prolongation matrices), solvePGMRES to 1e-8 in fewer than 10 iterations; the application
+    is linear (it must be, for GMRES) and differs from the MGPreconditionerSpace one"""
+    O = oracle
+    for refinements in (2, 3):
+        nlev = refinements + 1
+        G = O.MG([4, 4], 1, nlev, [0, 0], [1, 1], [O.GRAD_GRAD], [1.0], smoother=O.SM_RICHARDSON, runs=8,
+                 omega=0.1, cycles=1, fmg=2)
+        G.set_raw(True)
+        cells = [4 << refinements] * 2
+        A = G.level_matrix(nlev - 1)
+        rhs = O.rhs_sines(cells, 1, [0, 0], [1, 1], [[1, 1]], [1.0])
+        rhs[O.boundary_nodes(cells, 1)] = 0
+        x, it, res, _ = O.solve("gmres", A, rhs, np.zeros(A.n), 1e-8, precond=G)
+        assert it < 10
+        assert res <= 1e-8
+    rng = np.random.default_rng(0)
+    u, v = rng.standard_normal(A.n), rng.standard_normal(A.n)
+    assert np.allclose(G.apply(2 * u - 3 * v), 2 * G.apply(u) - 3 * G.apply(v), atol=1e-9)
+    H = O.MG([4, 4], 1, nlev, [0, 0], [1, 1], [O.GRAD_GRAD], [1.0], smoother=O.SM_RICHARDSON, runs=8,
+             omega=0.1, cycles=1, fmg=True)
+    assert np.linalg.norm(G.apply(u) - H.apply(u)) > 1e-6 * np.linalg.norm(H.apply(u))
+
+
 @pytest.mark.parametrize("sm", ["jacobi", "chebyshev"])
 def test_mg_vcycle_contracts(oracle, sm):
     O = oracle
