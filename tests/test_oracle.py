@@ -400,9 +400,11 @@ def test_galerkin_identity(oracle):
 
 
 def test_mg_preconditioned_gmres_laplace(oracle):
-    """LaplaceMG.testMG (Core/src/test/java/examples/LaplaceMG.java:23-101):
-    Q1 on [0,1]^2, Richardson(0.1, 8) smoothing, MG-preconditioned GMRES to
-    1e-8 in fewer than 10 iterations."""
+    """The LaplaceMG.testMG bound (Core/src/test/java/examples/LaplaceMG.java:23-101: Q1 on
+    [0,1]^2, Richardson(0.1, 8) smoothing, MG-preconditioned GMRES to 1e-8 in fewer than 10
+    iterations) with the MGPreconditionerSpace application (copy boundary values, full-V
+    correction, 3 V-cycles); the test as written (MGSpace) is
+    test_mgspace_preconditioned_gmres_laplace."""
     O = oracle
     for refinements in (2, 3):
         nlev = refinements + 1
