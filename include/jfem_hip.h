@@ -196,6 +196,13 @@ int jfem_solve_bicgstab(jfem_mat *A, jfem_mg *M, const jfem_vec *b, jfem_vec *x,
  * classical Gram-Schmidt, Givens; restart = ITERATIONS_BEFORE_RESTART (150) */
 int jfem_solve_gmres(jfem_mat *A, jfem_mg *M, const jfem_vec *b, jfem_vec *x, double tol, int restart,
                      int max_cycles, jfem_solve_info *info);
+/* Smoother.smooth(operator, rhs, iterate) (M/multigrid/Smoother.java:15) on any square matrix,
+ * x in/out: JFEM_SM_JACOBI runs x { x += (D^-1 (b - A x)) omega }, |D_ii| < 1e-10 taken as 1
+ * (JacobiSmoother.java:12-50); JFEM_SM_RICHARDSON runs x { r = b - A x; stop when ||r|| < 1e-10;
+ * x += r omega } (RichardsonSmoother.java:13-50); JFEM_SM_CHEBYSHEV runs x a Chebyshev-Jacobi
+ * polynomial of the given degree on [lmin, lmax] of D^-1 A (new). */
+int jfem_smooth(jfem_mat *A, int kind, int runs, double omega, int degree, double lmin, double lmax,
+                const jfem_vec *b, jfem_vec *x);
 /* GMRES2.solve (M/linalg/GMRES2.java:34-104, Arnoldi M/linalg/Arnoldi.java:13-82): minimal
  * residual steps along M A-orthonormalised directions of the power sequence of M A, stop when
  * the recursively updated ||r||_2 <= tol (absolute), restart on a vanishing direction; M

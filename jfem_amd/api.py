@@ -442,12 +442,21 @@ def normL2Difference(f1: ScalarFESpaceFunction, f2: ScalarFESpaceFunction, point
 # --------------------------------------------------------------------------
 # multigrid
 # --------------------------------------------------------------------------
+def _smooth(kind, operator, rhs, iterate, runs, omega, degree=0, lmin=0.0, lmax=0.0):
+    check(N.lib().jfem_smooth(operator.h, kind, runs, omega, degree, lmin, lmax, rhs.h, iterate.h))
+    return iterate
+
+
 class RichardsonSmoother:
     """RichardsonSmoother(omega, runs) (M/multigrid/RichardsonSmoother.java:13-50)."""
     kind = 0
 
     def __init__(self, omega: float, runs: int):
         self.omega, self.runs = omega, runs
+
+    def smooth(self, operator: "SparseMatrix", rhs: "DenseVector", iterate: "DenseVector") -> "DenseVector":
+        """Smoother.smooth on any matrix (iterate updated in place and returned)"""
+        return _smooth(self.kind, operator, rhs, iterate, self.runs, self.omega)
 
 
 class JacobiSmoother:
@@ -457,6 +466,10 @@ class JacobiSmoother:
     def __init__(self, runs: int, omega: float):
         self.runs, self.omega = runs, omega
 
+    def smooth(self, operator: "SparseMatrix", rhs: "DenseVector", iterate: "DenseVector") -> "DenseVector":
+        """Smoother.smooth on any matrix (iterate updated in place and returned)"""
+        return _smooth(self.kind, operator, rhs, iterate, self.runs, self.omega)
+
 
 class ChebyshevSmoother:
     """Chebyshev-Jacobi smoother of the given degree, applied `runs` times."""
@@ -465,6 +478,13 @@ class ChebyshevSmoother:
     def __init__(self, degree: int = 4, runs: int = 2, ratio: float = 20.0, safety: float = 1.1, power_its: int = 30):
         self.degree, self.runs, self.ratio, self.safety, self.power_its = degree, runs, ratio, safety, power_its
         self.omega = 1.0
+
+    def smooth(self, operator: "SparseMatrix", rhs: "DenseVector", iterate: "DenseVector",
+               lmax: float = None) -> "DenseVector":
+        """Chebyshev-Jacobi on [lmax / ratio, lmax] of D^-1 A (lmax given by the caller)"""
+        if lmax is None:
+            raise N.IllegalArgumentException(1, "Chebyshev smoothing needs the upper bound lmax of D^-1 A")
+        return _smooth(self.kind, operator, rhs, iterate, self.runs, 1.0, self.degree, lmax / self.ratio, lmax)
 
 
 FP64, FP32 = 0, 1

@@ -496,6 +496,78 @@ static void solve_gmres(jfem_mat* A, jfem_mg* M, const double* b, double* x, dou
 }
 
 
+// ------------------------------------------------------------------- smoothers on any matrix
+// 1 / D_ii with |D_ii| < 1e-10 taken as 1 (JacobiSmoother: getDiagonalMatrix, invert)
+__global__ void k_dinv_of(long n, const double* __restrict__ d, double* __restrict__ dinv)
+{
+#pragma clang fp contract(off)
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double v = std::fabs(d[i]) < 1e-10 ? 1.0 : d[i];
+        dinv[i] = 1.0 / v;
+    }
+}
+
+// x = x + (dinv r) omega (Jacobi) or x = x + r omega (Richardson, dinv null)
+__global__ void k_relax(long n, double omega, const double* __restrict__ dinv, const double* __restrict__ r,
+                        double* __restrict__ x)
+{
+#pragma clang fp contract(off)
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        x[i] = x[i] + (dinv ? dinv[i] * r[i] : r[i]) * omega;
+}
+
+// Chebyshev step: d = (dinv r) / theta (first) or d = c1 d + c2 (dinv r); x = x + d
+__global__ void k_cheb_apply(long n, int first, double theta, double c1, double c2, const double* __restrict__ dinv,
+                             const double* __restrict__ r, double* __restrict__ d, double* __restrict__ x)
+{
+#pragma clang fp contract(off)
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double v = first ? (dinv[i] * r[i]) / theta : c1 * d[i] + c2 * (dinv[i] * r[i]);
+        d[i] = v;
+        x[i] = x[i] + v;
+    }
+}
+
+static void smooth_any(jfem_mat* A, int kind, int runs, double omega, int degree, double lmin, double lmax,
+                       const double* b, double* x)
+{
+    jfem_ctx* c = A->ctx;
+    const long n = A->rows;
+    const int nb = kgrid(n);
+    DevBuf<double> t(n), r(n), dinv, dd;
+    if (kind != JFEM_SM_RICHARDSON) {
+        dinv.alloc(n);
+        mat_diag(A, t);
+        k_dinv_of<<<nb, KB, 0, c->stream>>>(n, t, dinv);
+    }
+    if (kind == JFEM_SM_CHEBYSHEV) dd.alloc(n);
+    const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
+    for (int it = 0; it < runs; it++) {
+        if (kind == JFEM_SM_CHEBYSHEV) {
+            double rho_old = 1.0 / sigma;
+            for (int kd = 0; kd < degree; kd++) {
+                spmv(A, x, t);
+                k_sub<<<nb, KB, 0, c->stream>>>(n, b, t, r);
+                double c1 = 0, c2 = 0;
+                if (kd > 0) {
+                    const double rho = 1.0 / (2.0 * sigma - rho_old);
+                    c1 = rho * rho_old;
+                    c2 = 2.0 * rho / delta;
+                    rho_old = rho;
+                }
+                k_cheb_apply<<<nb, KB, 0, c->stream>>>(n, kd == 0, theta, c1, c2, dinv, r, dd, x);
+            }
+            continue;
+        }
+        spmv(A, x, t);
+        k_sub<<<nb, KB, 0, c->stream>>>(n, b, t, r);
+        if (kind == JFEM_SM_RICHARDSON && std::sqrt(vec_dot(c, n, r, r)) < 1e-10) break;   // RichardsonSmoother :36-41
+        k_relax<<<nb, KB, 0, c->stream>>>(n, omega, kind == JFEM_SM_JACOBI ? dinv.p : nullptr, r, x);
+    }
+    JFEM_HIP(hipGetLastError());
+    JFEM_HIP(hipStreamSynchronize(c->stream));
+}
+
 // ------------------------------------------------------------------- GMRES2
 // GMRES2.solveInternal (M/linalg/GMRES2.java:59-104) with Arnoldi (M/linalg/Arnoldi.java:13-82):
 // minimal-residual steps x += t v along MA-orthonormalised directions (MA = M(A .)) of the
@@ -976,6 +1048,21 @@ int jfem_solve_gmres(jfem_mat* A, jfem_mg* M, const jfem_vec* b, jfem_vec* x, do
     jfem_solve_info tmp;
     jfem_solve_info* inf = info ? info : &tmp;
     timed(A->ctx, inf, [&] { solve_gmres(A, M, b->d, x->d, tol, restart, max_cycles, inf); });
+    API_END
+}
+
+int jfem_smooth(jfem_mat* A, int kind, int runs, double omega, int degree, double lmin, double lmax,
+                const jfem_vec* b, jfem_vec* x)
+{
+    API_BEGIN
+    JFEM_CHECK(A && b && x, JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(A->rows == A->cols && b->n == A->rows && x->n == A->rows, JFEM_ERR_ARG, "Incompatible sizes");
+    JFEM_CHECK(kind == JFEM_SM_RICHARDSON || kind == JFEM_SM_JACOBI || kind == JFEM_SM_CHEBYSHEV, JFEM_ERR_UNSUPPORTED,
+               "unknown smoother");
+    JFEM_CHECK(runs >= 0, JFEM_ERR_ARG, "negative run count");
+    if (kind == JFEM_SM_CHEBYSHEV)
+        JFEM_CHECK(degree >= 1 && lmin > 0 && lmax > lmin, JFEM_ERR_ARG, "Chebyshev needs degree >= 1 and 0 < lmin < lmax");
+    smooth_any(A, kind, runs, omega, degree, lmin, lmax, b->d, x->d);
     API_END
 }
 

@@ -325,6 +325,28 @@ def test_mg_level_matrices_and_vcycle(jf, oracle, which):
     assert np.linalg.norm(got - exp) <= 1e-11 * np.linalg.norm(exp)
 
 
+@pytest.mark.parametrize("which", ["jacobi", "chebyshev", "richardson"])
+def test_smoother_on_any_matrix(jf, oracle, which):
+    """Smoother.smooth(operator, rhs, iterate) (M/multigrid/Smoother.java:15) through jfem_smooth,
+    against the oracle's smoothing of the same multigrid level"""
+    sm = {"jacobi": jf.JacobiSmoother(3, 0.6), "chebyshev": jf.ChebyshevSmoother(4, 2),
+          "richardson": jf.RichardsonSmoother(0.1, 5)}[which]
+    G, O, spaces = _mg_pair(jf, oracle, [2, 3], 2, 3, sm, cycles=1, fmg=False)
+    rng = np.random.default_rng(12)
+    for l in (1, 2):
+        A = G.getSystem(l)
+        n = O.level_n(l)
+        b = rng.standard_normal(n)
+        x = 0.1 * rng.standard_normal(n)
+        xg = jf.DenseVector(x)
+        if which == "chebyshev":
+            sm.smooth(A, jf.DenseVector(b), xg, lmax=G.lmax()[l])
+        else:
+            sm.smooth(A, jf.DenseVector(b), xg)
+        exp = O.smooth(l, b, x)
+        assert np.linalg.norm(xg.numpy() - exp) <= 1e-13 * np.linalg.norm(exp), (which, l)
+
+
 @pytest.mark.parametrize("which,precision", [("jacobi", 0), ("chebyshev", 0), ("chebyshev", 1)])
 def test_mg_wcycle(jf, oracle, which, precision):
     """wCycle -> mgStepW (MGPreconditionerInterface.java:200-305) against the oracle's
