@@ -339,7 +339,7 @@ def bench_c5(jf, args):
     if not args.no_learned:
         t0 = time.perf_counter()
         Gt, _ = c5_velocity_mg(jf, 128, nu, jf.FP64, cheb)
-        net = jf.SmootherNet(2, 4, seed=7)
+        net = jf.SmootherNet(2, args.c5_cin, seed=7)
         Gt.set_learned(net, np.zeros(128 * 128), runs=1, fallback_ratio=0.5)
         h = Gt.train(rhs_per_field=400, cycles_per_rhs=8, lr=1e-4, seed=1, mode=5)
         jf.context().sync()
@@ -348,7 +348,7 @@ def bench_c5(jf, args):
         log(f"learned smoother trained on 128^2 in {t_train:.1f}s, two-grid energy ratio "
             f"{np.mean(h[:8]):.3f} -> {np.mean(h[-8:]):.4f}")
         G32.set_learned(net, np.zeros(cells * cells), runs=1, fallback_ratio=0.5)
-        net2 = jf.SmootherNet(2, 4, ctx=ctx2)   # the same weights for the second component's hierarchy
+        net2 = jf.SmootherNet(2, args.c5_cin, ctx=ctx2)   # the same weights for the second component's hierarchy
         net2.set_params(net.get_params())
         G32b.set_learned(net2, np.zeros(cells * cells), runs=1, fallback_ratio=0.5)
         res = []
@@ -509,6 +509,8 @@ def main():
     ap.add_argument("--pairs", type=int, default=64, help="(residual, error) pairs per training field")
     ap.add_argument("--eval-fields", type=int, default=8)
     ap.add_argument("--galerkin", action="store_true", help="C4: Galerkin coarse systems on every hierarchy")
+    ap.add_argument("--c5-cin", type=int, default=4, choices=[2, 4],
+                    help="C5: network input channels (2: residual and coefficient; 4: plus node-parity channels)")
     ap.add_argument("--mass-its", type=int, default=3,
                     help="C5: Chebyshev steps on the pressure mass matrix in the Schur complement (0 = lumped)")
     ap.add_argument("--spmv-only", action="store_true",
