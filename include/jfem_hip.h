@@ -142,6 +142,8 @@ int jfem_space_lex_rank(jfem_space *s, int *out);
                                            (M/mixed/MixedCellIntegral.java:58-68, M/tensorproduct/TPVectorCellIntegral.java:80-88) */
 #define JFEM_MIXED_DIV_VALUE 1          /* MixedTPCellIntegral DIV_VALUE (M/mixed/MixedTPCellIntegral.java:37-59),
                                            placed at (velocity, pressure) and (pressure, velocity) */
+#define JFEM_MIXED_VALUE_GRAD 2         /* MixedTPCellIntegral VALUE_GRAD (M/mixed/MixedTPCellIntegral.java:50-57),
+                                           velocity value . pressure gradient, placed like DIV_VALUE */
 /* TaylorHoodSpace.writeCellIntegralsToMatrix (M/basic/MatrixFESpace.java:22-42 with
  * MixedCellIntegral.evaluateCellIntegral, M/mixed/MixedCellIntegral.java:93-124): at most one
  * integral of each kind, constant weights; pattern = COO key set as for jfem_assemble */

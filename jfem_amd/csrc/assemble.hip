@@ -1476,9 +1476,13 @@ __global__ __launch_bounds__(256) void k_th_rows(ThParams P, long nrows, int* __
 // :116-120, fastGradientInCell product order, ContinuousTPShapeFunction.java:173-198),
 // value = product of the 1D values (fastValueInCell :154-171), then times each
 // axis' quadrature weight, summed over the points with the last axis fastest.
+// VALUE_GRAD (:50-57) is velocity value . pressure gradient * weight; the velocity value vector
+// has the component function's value at entry c and exact zeros elsewhere, so the inner product
+// (DenseVector.inner, M/linalg/DenseVector.java:104-112) is value * gradient component c exactly.
+// With both integrals the element value is their sum, as writeCellIntegralsToMatrix adds them.
 #pragma clang fp contract(off)
 static void div_value_table(int d, int kv, int kp, int Q, const Tab1D* const* tv, const Tab1D* const* tp, double w,
-                            double* out)
+                            double* out, bool value_grad = false)
 {
     const int nlv = (int)std::lround(std::pow(kv + 1, d)), nlp = (int)std::lround(std::pow(kp + 1, d));
     int nq = 1;
@@ -1494,12 +1498,22 @@ static void div_value_table(int d, int kv, int kp, int Q, const Tab1D* const* tv
                     int qa[MAXD] = {0, 0, 0};
                     int qq = q;
                     for (int a = d - 1; a >= 0; a--) { qa[a] = qq % Q; qq /= Q; }
-                    double div = 1;
-                    for (int j = 0; j < d; j++)
-                        div *= j == c ? tv[j]->der[lv[j]][qa[j]] : tv[j]->val[lv[j]][qa[j]];
-                    double pv = 1;
-                    for (int a = 0; a < d; a++) pv *= tp[a]->val[lp[a]][qa[a]];
-                    double val = div * pv * w;
+                    double val;
+                    if (!value_grad) {
+                        double div = 1;
+                        for (int j = 0; j < d; j++)
+                            div *= j == c ? tv[j]->der[lv[j]][qa[j]] : tv[j]->val[lv[j]][qa[j]];
+                        double pv = 1;
+                        for (int a = 0; a < d; a++) pv *= tp[a]->val[lp[a]][qa[a]];
+                        val = div * pv * w;
+                    } else {
+                        double vv = 1;
+                        for (int a = 0; a < d; a++) vv *= tv[a]->val[lv[a]][qa[a]];
+                        double g = 1;
+                        for (int j = 0; j < d; j++)
+                            g *= j == c ? tp[j]->der[lp[j]][qa[j]] : tp[j]->val[lp[j]][qa[j]];
+                        val = vv * g * w;
+                    }
                     for (int a = 0; a < d; a++) val *= tv[a]->w[qa[a]];
                     ret += val;
                 }
@@ -1508,7 +1522,8 @@ static void div_value_table(int d, int kv, int kp, int Q, const Tab1D* const* tv
 }
 #pragma clang fp contract(on)
 
-jfem_mat* th_assemble(jfem_space* V, jfem_space* Pp, bool has_vv, double vv_w, bool has_div, double div_w, int rule)
+jfem_mat* th_assemble(jfem_space* V, jfem_space* Pp, bool has_vv, double vv_w, bool has_div, double div_w, int rule,
+                      bool has_vg, double vg_w)
 {
     jfem_ctx* c = V->ctx;
     const Grid& gv = V->grid;
@@ -1520,7 +1535,7 @@ jfem_mat* th_assemble(jfem_space* V, jfem_space* Pp, bool has_vv, double vv_w, b
     for (int a = 0; a < d; a++)
         JFEM_CHECK(gv.n[a] == gp.n[a] && gv.start[a] == gp.start[a] && gv.end[a] == gp.end[a], JFEM_ERR_ARG,
                    "velocity and pressure spaces on different grids");
-    JFEM_CHECK(has_vv || has_div, JFEM_ERR_ARG, "no cell integrals");
+    JFEM_CHECK(has_vv || has_div || has_vg, JFEM_ERR_ARG, "no cell integrals");
     double qp[8], qw[8];
     const int Q = quad_rule(rule, qp, qw);
     JFEM_CHECK(Q > 0, JFEM_ERR_ARG, "unknown quadrature rule");
@@ -1585,16 +1600,22 @@ jfem_mat* th_assemble(jfem_space* V, jfem_space* Pp, bool has_vv, double vv_w, b
         JFEM_HIP(hipGetLastError());
         JFEM_HIP(hipStreamSynchronize(c->stream));
     }
-    if (has_div) {
+    if (has_div || has_vg) {
         const long per = (long)d * nlv * nlp;
-        std::vector<double> eb((size_t)ntuples * per);
+        std::vector<double> eb((size_t)ntuples * per), eg(has_div && has_vg ? per : 0);
         for (long t = 0; t < ntuples; t++) {
             int cl[MAXD] = {0, 0, 0};
             long r = t;
             for (int a = d - 1; a >= 0; a--) { cl[a] = (int)(r % CV.ncls[a]); r /= CV.ncls[a]; }
             const Tab1D* tv[MAXD] = {&CV.tabs[0][cl[0]], &CV.tabs[1][cl[1]], d > 2 ? &CV.tabs[2][cl[2]] : nullptr};
             const Tab1D* tq[MAXD] = {&tabp[0][cl[0]], &tabp[1][cl[1]], d > 2 ? &tabp[2][cl[2]] : nullptr};
-            div_value_table(d, gv.k, gp.k, Q, tv, tq, div_w, eb.data() + t * per);
+            double* et = eb.data() + t * per;
+            if (has_div) div_value_table(d, gv.k, gp.k, Q, tv, tq, div_w, et);
+            if (has_vg && !has_div) div_value_table(d, gv.k, gp.k, Q, tv, tq, vg_w, et, true);
+            if (has_vg && has_div) {
+                div_value_table(d, gv.k, gp.k, Q, tv, tq, vg_w, eg.data(), true);
+                for (long i = 0; i < per; i++) et[i] += eg[i];
+            }
         }
         EB.alloc(eb.size());
         // on the context stream: a blocking pageable copy pays a one-time runtime set-up (12 ms)
@@ -1613,7 +1634,7 @@ jfem_mat* th_assemble(jfem_space* V, jfem_space* Pp, bool has_vv, double vv_w, b
         P.cls[a] = D.cls[a];
     }
     P.Evv = has_vv ? Evv.p : nullptr;
-    P.EB = has_div ? EB.p : nullptr;
+    P.EB = (has_div || has_vg) ? EB.p : nullptr;
     const long n = d * P.nv + P.np;
     JFEM_CHECK(n < (1l << 31), JFEM_ERR_UNSUPPORTED, "2^31 or more unknowns");
     auto* A = new jfem_mat();
@@ -1858,8 +1879,8 @@ int jfem_th_assemble(jfem_space* velocity, jfem_space* pressure, int nint, const
     API_BEGIN
     JFEM_CHECK(velocity && pressure && out && (nint == 0 || (kinds && weights)), JFEM_ERR_ARG, "null argument");
     JFEM_CHECK(velocity->ctx == pressure->ctx, JFEM_ERR_ARG, "spaces on different contexts");
-    bool has_vv = false, has_div = false;
-    double wv = 0, wd = 0;
+    bool has_vv = false, has_div = false, has_vg = false;
+    double wv = 0, wd = 0, wg = 0;
     for (int i = 0; i < nint; i++) {
         if (kinds[i] == JFEM_MIXED_VELOCITY_GRAD_GRAD) {
             JFEM_CHECK(!has_vv, JFEM_ERR_UNSUPPORTED, "one velocity GRAD_GRAD integral at most");
@@ -1869,11 +1890,15 @@ int jfem_th_assemble(jfem_space* velocity, jfem_space* pressure, int nint, const
             JFEM_CHECK(!has_div, JFEM_ERR_UNSUPPORTED, "one DIV_VALUE integral at most");
             has_div = true;
             wd = weights[i];
+        } else if (kinds[i] == JFEM_MIXED_VALUE_GRAD) {
+            JFEM_CHECK(!has_vg, JFEM_ERR_UNSUPPORTED, "one VALUE_GRAD integral at most");
+            has_vg = true;
+            wg = weights[i];
         } else {
             JFEM_CHECK(false, JFEM_ERR_UNSUPPORTED, "Integral type not supported");
         }
     }
-    *out = th_assemble(velocity, pressure, has_vv, wv, has_div, wd, quad_rule);
+    *out = th_assemble(velocity, pressure, has_vv, wv, has_div, wd, quad_rule, has_vg, wg);
     API_END
 }
 

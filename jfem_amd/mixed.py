@@ -80,10 +80,10 @@ class MixedCellIntegral:
 
 
 class MixedTPCellIntegral:
-    """MixedTPCellIntegral DIV_VALUE (M/mixed/MixedTPCellIntegral.java:17-59): weight * div(u) * p,
-    at (velocity, pressure) and (pressure, velocity)."""
+    """MixedTPCellIntegral (M/mixed/MixedTPCellIntegral.java:17-59): DIV_VALUE weight * div(u) * p or
+    VALUE_GRAD weight * u . grad(p), at (velocity, pressure) and (pressure, velocity)."""
     DIV_VALUE = "DivValue"
-    kind = 1
+    VALUE_GRAD = "ValueGrad"
 
     def __init__(self, *args, quadratureRule1D: int = QuadratureRule1D.Gauss5):
         if len(args) == 1:
@@ -92,9 +92,10 @@ class MixedTPCellIntegral:
             weight, name = args
         else:
             raise TypeError("MixedTPCellIntegral(name) or MixedTPCellIntegral(weight, name)")
-        if name != self.DIV_VALUE:
+        if name not in (self.DIV_VALUE, self.VALUE_GRAD):
             raise N.UnsupportedOperationException(3, "Integral type not supported")
         self.weight, self.name, self.quadratureRule1D = float(weight), name, quadratureRule1D
+        self.kind = 1 if name == self.DIV_VALUE else 2
 
 
 class TPVectorRightHandSideIntegral:

@@ -125,7 +125,50 @@ def div_value_table(s, e, kv, kp, weight=1.0, rule=5):
     return out
 
 
-def th_assemble(cells, kp, start, end, nu=None, div_weight=None, rule=5):
+def value_grad_table(s, e, kv, kp, weight=1.0, rule=5):
+    """VALUE_GRAD (MixedTPCellIntegral.java:50-57): (velocity value . pressure gradient) * weight;
+    for component c the velocity value vector has one nonzero entry, so the inner product is the
+    velocity function's value times the pressure gradient's component c (the other terms are
+    exact zeros).  Table [c][iv][ip] like div_value_table."""
+    d = len(s)
+    qp, qw = O.quad(rule)
+    Q = len(qp)
+    pts, wts = [], []
+    for a in range(d):
+        ln = e[a] - s[a]
+        pts.append([qp[q] * ln + s[a] for q in range(Q)])
+        wts.append([qw[q] * ln for q in range(Q)])
+    lv, lp = _local_indices(kv, d), _local_indices(kp, d)
+
+    def val1(a, l, k, x):
+        return O.lagrange_value((x - s[a]) / (e[a] - s[a]), l, k)
+
+    def der1(a, l, k, x):
+        ln = e[a] - s[a]
+        return O.lagrange_deriv((x - s[a]) / ln, l, k) / ln
+
+    out = np.zeros((d, len(lv), len(lp)))
+    for c in range(d):
+        for iv, mv in enumerate(lv):
+            for ip, mp in enumerate(lp):
+                ret = 0.0
+                for qa in itertools.product(range(Q), repeat=d):
+                    x = [pts[a][qa[a]] for a in range(d)]
+                    vv = 1.0
+                    for a in range(d):
+                        vv *= val1(a, mv[a], kv, x[a])
+                    g = 1.0
+                    for j in range(d):
+                        g *= der1(j, mp[j], kp, x[j]) if j == c else val1(j, mp[j], kp, x[j])
+                    v = vv * g * weight
+                    for a in range(d):
+                        v *= wts[a][qa[a]]
+                    ret += v
+                out[c, iv, ip] = ret
+    return out
+
+
+def th_assemble(cells, kp, start, end, nu=None, div_weight=None, rule=5, vg_weight=None):
     """the mixed matrix as a dict {(row, col): value} in the mixed numbering (pattern = keys)"""
     cells = [int(c) for c in cells]
     d = len(cells)
@@ -146,6 +189,9 @@ def th_assemble(cells, kp, start, end, nu=None, div_weight=None, rule=5):
         if nu is not None:
             K = O.element_matrix(cells, kv, start, end, ci, [O.GRAD_GRAD], [nu], rule)
         B = div_value_table(s, e, kv, kp, div_weight, rule) if div_weight is not None else None
+        if vg_weight is not None:   # the integrals' sum in list order: (0 + DIV_VALUE) + VALUE_GRAD
+            G = value_grad_table(s, e, kv, kp, vg_weight, rule)
+            B = G if B is None else B + G
         for (tv, cv, iv, gv) in loc:          # test function (row)
             for (tu, cu, iu, gu) in loc:      # trial function (column)
                 val = 0.0

@@ -21,13 +21,15 @@ def _th(jf, cells, kp, start, end):
     return s
 
 
-def _integrals(jf, nu, wd, rule=5):
+def _integrals(jf, nu, wd, rule=5, wg=None):
     out = []
     if nu is not None:
         out.append(jf.MixedCellIntegral.fromVelocityIntegral(
             jf.TPVectorCellIntegral(nu, jf.TPVectorCellIntegral.GRAD_GRAD, quadratureRule1D=rule)))
     if wd is not None:
         out.append(jf.MixedTPCellIntegral(wd, jf.MixedTPCellIntegral.DIV_VALUE, quadratureRule1D=rule))
+    if wg is not None:
+        out.append(jf.MixedTPCellIntegral(wg, jf.MixedTPCellIntegral.VALUE_GRAD, quadratureRule1D=rule))
     return out
 
 
@@ -51,6 +53,19 @@ def test_th_assembly_matches_oracle(jf, oracle, cells, kp, nu, wd):
     A = s.writeCellIntegralsToMatrix(_integrals(jf, nu, wd))
     E = MO.th_assemble(cells, kp, start, end, nu, wd)
     assert A.getRows() == s.ndofs()
+    _compare(A, E, s.ndofs())
+
+
+@pytest.mark.parametrize("cells,kp,nu,wd,wg", [((3, 4), 1, None, None, 1.0), ((3, 2), 1, 0.7, -1.0, 0.5),
+                                               ((2, 2), 2, None, 1.0, -2.0), ((1, 2, 2), 1, 1.5, None, -1.0)])
+def test_th_assembly_value_grad(jf, oracle, cells, kp, nu, wd, wg):
+    """MixedTPCellIntegral VALUE_GRAD alone and summed with DIV_VALUE in one element value"""
+    d = len(cells)
+    start, end = [0.5, -1.0, 0.0][:d], [2.0, 1.0, 0.75][:d]
+    print(f"th assembly value_grad cells={cells} kp={kp} nu={nu} wd={wd} wg={wg}", flush=True)
+    s = _th(jf, cells, kp, start, end)
+    A = s.writeCellIntegralsToMatrix(_integrals(jf, nu, wd, wg=wg))
+    E = MO.th_assemble(cells, kp, start, end, nu, wd, vg_weight=wg)
     _compare(A, E, s.ndofs())
 
 
@@ -91,11 +106,12 @@ def test_th_argument_errors(jf):
     assert rc == 1
     # unknown integral kind, two integrals of one kind
     with pytest.raises(jf.UnsupportedOperationException):
-        jf.MixedTPCellIntegral(1.0, "ValueGrad")
-    bad = np.array([0, 0], dtype=np.int32)
-    rc = jf.lib().jfem_th_assemble(s.velocity.h, s.pressure.h, 2, jf._native.iptr(bad), jf._native.dptr(w), 5,
-                                   jf.api.C.byref(h))
-    assert rc == 3
+        jf.MixedTPCellIntegral(1.0, "GradValue")
+    for bad in ([0, 0], [2, 2], [0, 7]):
+        bad = np.array(bad, dtype=np.int32)
+        rc = jf.lib().jfem_th_assemble(s.velocity.h, s.pressure.h, 2, jf._native.iptr(bad), jf._native.dptr(w), 5,
+                                       jf.api.C.byref(h))
+        assert rc == 3
     A = s.writeCellIntegralsToMatrix(ints)
     rhs = jf.DenseVector(s.ndofs())
     # a body force with the wrong number of components

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from oracle import mixed_oracle as MO
+from oracle import oracle as O
 
 # MixedCellIntegralTest.testMixedIntegral (Core/src/test/java/mixed/MixedCellIntegralTest.java:72-96):
 # cell [3,5]x[0,4], Q1 velocity function i (component i // 4) against the first Q1
@@ -21,6 +22,48 @@ def test_div_value_golden(oracle):
     # the same values exactly: (int L_a' L_0 dx)(int L_b L_0 dy) on a 2 x 4 cell
     exact = [-2 / 3, 2 / 3, -1 / 3, 1 / 3, -1 / 3, -1 / 6, 1 / 3, 1 / 6]
     assert np.allclose(got, exact, rtol=0, atol=1e-14)
+
+
+VALUES_VG = [-0.666, -0.666, -0.333, -0.333, -0.333, -0.1666, -0.333, -0.1666]
+
+
+def test_value_grad_golden(oracle):
+    """MixedCellIntegralTest.testMixedIntegral's VALUE_GRAD vector (:84, :92), to 1e-2 as there and
+    exactly: (int L_a L_0' dx)(int L_b L_0 dy) and its y counterpart on the 2 x 4 cell"""
+    G = MO.value_grad_table([3.0, 0.0], [5.0, 4.0], 1, 1, 1.0)
+    got = [G[i // 4, i % 4, 0] for i in range(8)]
+    assert np.allclose(got, VALUES_VG, atol=1e-2)
+    exact = [-2 / 3, -2 / 3, -1 / 3, -1 / 3, -1 / 3, -1 / 6, -1 / 3, -1 / 6]
+    assert np.allclose(got, exact, rtol=0, atol=1e-14)
+    # integration by parts on the cell: VALUE_GRAD + DIV_VALUE = boundary term, which for the
+    # pressure function 0 and velocity function 3 (nodes (5,4), pressure node (3,0)) vanishes
+    B = MO.div_value_table([3.0, 0.0], [5.0, 4.0], 1, 1, 1.0)
+    assert abs(G[0, 3, 0] + B[0, 3, 0]) < 1e-14
+
+
+def test_value_grad_is_minus_div_value_for_interior_velocity(oracle):
+    """integration by parts over the domain: for a velocity function vanishing on the boundary,
+    the assembled VALUE_GRAD row equals minus the DIV_VALUE row (boundary rows do not)"""
+    cells, kp = (3, 2), 1
+    start, end = [0.0, -1.0], [1.5, 1.0]
+    G = MO.th_assemble(cells, kp, start, end, vg_weight=1.0)
+    B = MO.th_assemble(cells, kp, start, end, div_weight=1.0)
+    vel, _ = MO.th_numbering(cells, kp, start, end)
+    xv = O.node_coords(cells, kp + 1, start, end)
+    inner = {int(vel[c, i]) for c in range(2) for i in range(len(xv))
+             if all(start[a] + 1e-12 < xv[i][a] < end[a] - 1e-12 for a in range(2))}
+    nv = 2 * len(xv)
+    checked = broken = 0
+    for r in range(nv):
+        row_g = {c: v for (rr, c), v in G.items() if rr == r and c >= nv}
+        row_b = {c: v for (rr, c), v in B.items() if rr == r and c >= nv}
+        diff = max(abs(row_g.get(c, 0.0) + row_b.get(c, 0.0)) for c in set(row_g) | set(row_b))
+        if r in inner:
+            assert diff < 1e-13, (r, diff)
+            checked += 1
+        elif diff > 1e-3:
+            broken += 1
+    assert checked > 0 and broken > 0
 
 
 def test_div_value_golden_detects_swapped_component(oracle):
