@@ -69,3 +69,19 @@ def test_cell_arrays_are_checked_against_the_grid():
     assert api._cell_array(np.ones((3, 4)), _Space(), "c").shape == (12,)
     with pytest.raises(N.IllegalArgumentException):
         api._cell_array(np.ones(16), _Space(), "c")
+
+
+def test_mixed_integral_kinds_match_header():
+    """the Python mirror passes the JFEM_MIXED_* kinds the header declares; other names are refused"""
+    import re
+
+    from jfem_amd import mixed as M, _native as N
+    hdr = open(os.path.join(os.path.dirname(__file__), "..", "include", "jfem_hip.h")).read()
+    kinds = {m.group(1): int(m.group(2)) for m in re.finditer(r"#define JFEM_MIXED_(\w+)\s+(\d+)", hdr)}
+    assert kinds == {"VELOCITY_GRAD_GRAD": 0, "DIV_VALUE": 1, "VALUE_GRAD": 2}
+    vi = M.TPVectorCellIntegral(1.0, M.TPVectorCellIntegral.GRAD_GRAD)
+    assert M.MixedCellIntegral.fromVelocityIntegral(vi).kind == kinds["VELOCITY_GRAD_GRAD"]
+    assert M.MixedTPCellIntegral(M.MixedTPCellIntegral.DIV_VALUE).kind == kinds["DIV_VALUE"]
+    assert M.MixedTPCellIntegral(-2.0, M.MixedTPCellIntegral.VALUE_GRAD).kind == kinds["VALUE_GRAD"]
+    with pytest.raises(N.UnsupportedOperationException):
+        M.MixedTPCellIntegral("GradValue")
