@@ -108,6 +108,9 @@ int jfem_vec_fill(jfem_vec *v, double value);
 int jfem_vec_dot(const jfem_vec *x, const jfem_vec *y, double *out);
 /* y = a*x + b*y */
 int jfem_vec_axpby(double a, const jfem_vec *x, double b, jfem_vec *y);
+/* dst = src, a device-to-device copy on the context stream (new DenseVector(Vector),
+ * M/linalg/DenseVector.java:29-36); sizes must match */
+int jfem_vec_copy(const jfem_vec *src, jfem_vec *dst);
 
 /* ---- assembly: MatrixFESpace.evaluateCellIntegrals / writeCellIntegralsToMatrix
  *      (M/basic/MatrixFESpace.java:22-42,115-139) with TPCellIntegral

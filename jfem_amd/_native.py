@@ -90,6 +90,7 @@ _SIGS = {
     "jfem_vec_fill": (C.c_int, [_vp, C.c_double]),
     "jfem_vec_dot": (C.c_int, [_vp, _vp, _dp]),
     "jfem_vec_axpby": (C.c_int, [C.c_double, _vp, C.c_double, _vp]),
+    "jfem_vec_copy": (C.c_int, [_vp, _vp]),
     "jfem_assemble": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp, C.c_int, C.POINTER(_vp)]),
     "jfem_assemble_rhs_sines": (C.c_int, [_vp, C.c_int, _ip, _dp, C.c_int, _vp]),
     "jfem_assemble_rhs_trig": (C.c_int, [_vp, C.c_int, _ip, _ip, _dp, C.c_int, _vp]),

@@ -186,8 +186,13 @@ class DenseVector:
 
     def copy(self) -> "DenseVector":
         v = DenseVector(self.n, self.ctx)
-        check(N.lib().jfem_vec_axpby(1.0, self.h, 0.0, v.h))
+        v.copy_from(self)
         return v
+
+    def copy_from(self, other: "DenseVector"):
+        """self = other (device copy)"""
+        check(N.lib().jfem_vec_copy(other.h, self.h))
+        return self
 
 
 class SparseMatrix:
@@ -203,10 +208,16 @@ class SparseMatrix:
     def from_csr(cls, rowptr, cols, vals, shape=None, ctx: Optional[Context] = None):
         ctx = ctx or context()
         rp = np.ascontiguousarray(rowptr, dtype=np.int64)
+        if rp.ndim != 1 or rp.shape[0] < 1:
+            raise N.IllegalArgumentException(1, "row pointer must be a non-empty 1-D array")
         n = rp.shape[0] - 1
         ncols = n if shape is None else shape[1]
+        cols, vals = i32(cols), f64(vals)
+        # the library reads rp[-1] entries of each array: refuse shorter ones here
+        if cols.shape[0] < rp[-1] or vals.shape[0] < rp[-1]:
+            raise N.IllegalArgumentException(1, f"CSR arrays hold fewer than rowptr[-1] = {rp[-1]} entries")
         h = C.c_void_p()
-        check(N.lib().jfem_mat_from_csr(ctx.h, n, ncols, lptr(rp), iptr(i32(cols)), dptr(f64(vals)), C.byref(h)))
+        check(N.lib().jfem_mat_from_csr(ctx.h, n, ncols, lptr(rp), iptr(cols), dptr(vals), C.byref(h)))
         return cls(h, ctx)
 
     def __del__(self):
@@ -652,13 +663,25 @@ class IterativeSolver:
 
     def __init__(self, showProgress: bool = False):
         self.iterations = 0
-        self.lastSolution: Optional[np.ndarray] = None
+        self._last: Optional[DenseVector] = None    # device copy of the last solution
         self.info = N.SolveInfo()
         self.residual_history: Optional[np.ndarray] = None
 
+    @property
+    def lastSolution(self) -> Optional[np.ndarray]:
+        """the last solution (IterativeSolver.lastSolution, :17,98), the next start iterate;
+        kept on the device and downloaded only when read here"""
+        return None if self._last is None else self._last.numpy()
+
+    @lastSolution.setter
+    def lastSolution(self, values):
+        self._last = None if values is None else DenseVector(np.asarray(values, float))
+
     def _start(self, n, ctx, default_e0=True):
-        if self.lastSolution is not None:
-            return DenseVector(self.lastSolution, ctx)
+        if self._last is not None:
+            if self._last.n != n:
+                raise N.IllegalArgumentException(1, "Incompatible sizes")
+            return self._last.copy()
         x0 = np.zeros(n)
         if default_e0 and n:
             x0[0] = 1.0
@@ -666,7 +689,9 @@ class IterativeSolver:
 
     def _finish(self, x: DenseVector):
         self.iterations = self.info.iterations
-        self.lastSolution = x.numpy()
+        if self._last is None or self._last.n != x.n or self._last.ctx is not x.ctx:
+            self._last = DenseVector(x.n, x.ctx)
+        self._last.copy_from(x)
         return x
 
     def solveCG(self, operator: SparseMatrix, rhs: DenseVector, tol: float, maxit: int = 2 ** 31 - 1,

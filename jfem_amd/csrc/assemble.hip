@@ -1024,13 +1024,16 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     bool full = false;
     for (int i = 0; i < nint; i++) {
         JFEM_CHECK(kinds[i] >= 0 && kinds[i] <= 4, JFEM_ERR_UNSUPPORTED, "unknown integral name");
-        if (kinds[i] >= 3) {
-            full = true;
-            // these integrands use value(x) / gradient(x) of the whole shape function (the first support
-            // cell containing x), which differs from the integrated cell only at points on its faces
-            JFEM_CHECK(rule >= 1 && rule <= 5, JFEM_ERR_UNSUPPORTED,
-                       "GRAD_VALUE / VALUE_GRAD need a Gauss rule (Gauss-Lobatto points lie on cell faces)");
-        }
+        if (kinds[i] >= 3) full = true;
+        // every kind but GRAD_GRAD evaluates value(x) / gradient(x) of the whole shape function, i.e.
+        // in the first of its support cells containing x (M/basic/ScalarShapeFunction.java:26-39), not
+        // in the integrated cell (TPCellIntegral.java:124-157).  The two differ only at points on the
+        // cell's faces: there the neighbour's normal derivative differs, and its values differ by
+        // rounding, which decides whether a lumped (exactly zero) entry enters the pattern.
+        // Gauss-Lobatto points lie on the faces, so those kinds need a Gauss rule.
+        JFEM_CHECK(kinds[i] == JFEM_GRAD_GRAD || (rule >= 1 && rule <= 5), JFEM_ERR_UNSUPPORTED,
+                   "VALUE_VALUE / H1 / GRAD_VALUE / VALUE_GRAD need a Gauss rule (Gauss-Lobatto points lie on "
+                   "cell faces, where the reference evaluates the first support cell containing the point)");
     }
     ClassTables C = build_classes(g, rule);
     DevTables D;
@@ -1201,6 +1204,10 @@ void assemble_rhs_sines(jfem_space* s, int nterms, const int* kw, const double* 
     jfem_ctx* c = s->ctx;
     const Grid& g = s->grid;
     JFEM_CHECK(nterms >= 1, JFEM_ERR_ARG, "no source terms");
+    // TPRightHandSideIntegral VALUE evaluates shapeFunction.value(x) (TPRightHandSideIntegral.java:26-29),
+    // the first support cell containing x: per-cell tables model it at interior (Gauss) points only
+    JFEM_CHECK(rule >= 1 && rule <= 5, JFEM_ERR_UNSUPPORTED,
+               "right-hand side integrals need a Gauss rule (Gauss-Lobatto points lie on cell faces)");
     ClassTables C = build_classes(g, rule);
     DevTables D;
     upload_classes(c, g, C, D);

@@ -148,6 +148,10 @@ struct jfem_ctx {
     double* red_result = nullptr;     // RED_SLOTS doubles (device)
     double* red_host = nullptr;       // pinned mirror of red_result
     double* scal = nullptr;           // device scalars used by fused solver kernels
+    // work vectors of the CG / FCG / BiCGStab solvers: grown on demand and kept until the
+    // context is destroyed, so repeated solves allocate nothing
+    double* work = nullptr;
+    size_t work_n = 0;                // doubles held by work
     int cu_count = 256;
 };
 
@@ -227,6 +231,15 @@ struct DevBuf {
     DevBuf& operator=(const DevBuf&) = delete;
     operator T*() const { return p; }
 };
+
+// a non-owning view of device memory with DevBuf's interface (.p, conversion to T*)
+template <typename T>
+struct DevPtr {
+    T* p;
+    operator T*() const { return p; }
+};
+// the context's solver workspace with room for n doubles (reallocated only to grow)
+double* ctx_work(jfem_ctx* c, size_t n);
 
 double vec_dot(jfem_ctx* c, long n, const double* x, const double* y);
 // deterministic two-stage reduction of partials already written into ctx->red_partials

@@ -184,6 +184,24 @@ double vec_dot(jfem_ctx* c, long n, const double* x, const double* y)
 using namespace jfem;
 
 
+namespace jfem {
+
+double* ctx_work(jfem_ctx* c, size_t n)
+{
+    if (n > c->work_n) {
+        // the old buffer may still be read by work queued on the stream
+        JFEM_HIP(hipStreamSynchronize(c->stream));
+        (void)hipFree(c->work);
+        c->work = nullptr;
+        c->work_n = 0;
+        JFEM_HIP(hipMalloc(&c->work, sizeof(double) * n));
+        c->work_n = n;
+    }
+    return c->work;
+}
+
+}  // namespace jfem
+
 extern "C" {
 
 const char* jfem_last_error(void) { return g_last_error.c_str(); }
@@ -219,6 +237,7 @@ int jfem_ctx_destroy(jfem_ctx* c)
     (void)hipFree(c->red_partials);
     (void)hipFree(c->red_result);
     (void)hipFree(c->scal);
+    (void)hipFree(c->work);
     (void)hipHostFree(c->red_host);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -378,6 +397,16 @@ int jfem_vec_axpby(double a, const jfem_vec* x, double b, jfem_vec* y)
     JFEM_CHECK(x && y, JFEM_ERR_ARG, "null argument");
     JFEM_CHECK(x->n == y->n, JFEM_ERR_ARG, "Vectors have different size");
     vec_axpby(x->ctx, x->n, a, x->d, b, y->d);
+    API_END
+}
+
+int jfem_vec_copy(const jfem_vec* src, jfem_vec* dst)
+{
+    API_BEGIN
+    JFEM_CHECK(src && dst, JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(src->n == dst->n, JFEM_ERR_ARG, "Vectors have different size");
+    if (src->n > 0 && src->d != dst->d)
+        JFEM_HIP(hipMemcpyAsync(dst->d, src->d, sizeof(double) * src->n, hipMemcpyDeviceToDevice, src->ctx->stream));
     API_END
 }
 

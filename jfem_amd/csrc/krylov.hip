@@ -128,7 +128,8 @@ static void solve_cg(jfem_mat* A, jfem_mg* M, const double* b, double* x, double
     jfem_ctx* c = A->ctx;
     const long n = A->rows;
     double* sc = c->scal;
-    DevBuf<double> r(n), z(n), p(n), q(n);
+    double* W = ctx_work(c, 4 * (size_t)n);
+    const DevPtr<double> r{W}, z{W + n}, p{W + 2 * n}, q{W + 3 * n};
     // r = b - A x
     spmv(A, x, q.p);
     k_sub<<<kgrid(n), KB, 0, c->stream>>>(n, b, q.p, r.p);
@@ -179,7 +180,8 @@ static void solve_fcg(jfem_mat* A, jfem_mg* M, const double* b, double* x, doubl
     jfem_ctx* c = A->ctx;
     const long n = A->rows;
     double* sc = c->scal;
-    DevBuf<double> r(n), z(n), p(n), q(n);
+    double* W = ctx_work(c, 4 * (size_t)n);
+    const DevPtr<double> r{W}, z{W + n}, p{W + 2 * n}, q{W + 3 * n};
     spmv(A, x, q.p);
     k_sub<<<kgrid(n), KB, 0, c->stream>>>(n, b, q.p, r.p);
     mg_apply(M, r.p, z.p);
@@ -257,7 +259,9 @@ static void solve_bicgstab(jfem_mat* A, jfem_mg* M, const double* b, double* x, 
 {
     jfem_ctx* c = A->ctx;
     const long n = A->rows;
-    DevBuf<double> v(n), s(n), t(n), r0(n), r(n), p(n), y(n), z(n), pt(n), tmp(n);
+    double* W = ctx_work(c, 10 * (size_t)n);
+    const DevPtr<double> v{W}, s{W + n}, t{W + 2 * n}, r0{W + 3 * n}, r{W + 4 * n}, p{W + 5 * n}, y{W + 6 * n},
+        z{W + 7 * n}, pt{W + 8 * n}, tmp{W + 9 * n};
     vec_fill(c, v.p, 0, n);
     vec_fill(c, p.p, 0, n);
     spmv(A, x, tmp.p);
@@ -595,7 +599,6 @@ static void solve_gmres2(jfem_mat* A, jfem_mg* M, const double* b, double* x, do
     DevBuf<double> r(n), Mr(n), t(n), q(n), qn(n), w(n), aw(n), V((size_t)cap * n), AV((size_t)cap * n);
     const int nb = kgrid(n);
     long it = 0;
-    int restarts = 0;
     for (;;) {
         spmv(A, x, t);
         k_sub<<<nb, KB, 0, c->stream>>>(n, b, t, r);
@@ -639,7 +642,6 @@ static void solve_gmres2(jfem_mat* A, jfem_mg* M, const double* b, double* x, do
         }
         JFEM_HIP(hipGetLastError());
         if (!restart || it >= maxit) break;
-        restarts++;
     }
     info->iterations = (int)it;
     info->resnorm = true_resnorm(A, b, x, t);

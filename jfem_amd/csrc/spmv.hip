@@ -518,8 +518,12 @@ int jfem_mat_from_csr(jfem_ctx* ctx, long rows, long cols, const long* rowptr, c
 {
     API_BEGIN
     JFEM_CHECK(ctx && rowptr && out && rows >= 0 && cols >= 0, JFEM_ERR_ARG, "null argument");
+    // a CSR row pointer starts at 0 and never decreases; then every entry belongs to a row and
+    // the kernels' reads stay inside colidx / vals
+    JFEM_CHECK(rowptr[0] == 0, JFEM_ERR_ARG, "row pointer does not start at 0");
     const long nnz = rowptr[rows];
     JFEM_CHECK(nnz >= 0 && nnz < (1l << 31) - 1, JFEM_ERR_UNSUPPORTED, "2^31 or more entries");
+    JFEM_CHECK(nnz == 0 || (colidx && vals), JFEM_ERR_ARG, "null argument");
     std::vector<int> rp(rows + 1);
     for (long i = 0; i <= rows; i++) {
         JFEM_CHECK(i == 0 || rowptr[i] >= rowptr[i - 1], JFEM_ERR_ARG, "row pointer not monotone");
@@ -609,6 +613,8 @@ int jfem_spmv_t(jfem_mat* A, const jfem_vec* x, jfem_vec* y)
     API_BEGIN
     JFEM_CHECK(A && x && y, JFEM_ERR_ARG, "null argument");
     JFEM_CHECK(x->n == A->rows && y->n == A->cols, JFEM_ERR_ARG, "Incompatible sizes");
+    // the product reads x from every workgroup while others write y: in place it is wrong
+    JFEM_CHECK(x->d != y->d || A->rows == 0 || A->cols == 0, JFEM_ERR_ARG, "tvMul output aliases its input");
     JFEM_CHECK(A->nnz < (1l << 31) - 1, JFEM_ERR_UNSUPPORTED, "2^31 or more entries");
     spmv(mat_transpose(A), x->d, y->d);
     JFEM_HIP(hipGetLastError());
@@ -631,6 +637,7 @@ int jfem_spmv_f32(jfem_mat* A, const float* x, float* y)
 {
     API_BEGIN
     JFEM_CHECK(A && x && y, JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK((const void*)x != (const void*)y || A->rows == 0, JFEM_ERR_ARG, "SpMV output aliases its input");
     mat_make_f32(A);
     spmv32(A, x, y);
     JFEM_HIP(hipStreamSynchronize(A->ctx->stream));

@@ -66,28 +66,39 @@ def splitmix_normals(seed: int, count: int) -> np.ndarray:
     return (np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2))[:count]
 
 
-def gaussian_random_field(seed: int, cells, length: float = 0.1, contrast: float = 1e3):
-    """log-coefficient g per cell (reference cell order: last axis fastest) of a
-    zero-mean Gaussian random field with squared-exponential covariance
-    exp(-|x-y|^2 / (2 length^2)) on the cell centres of the unit box, sampled
-    exactly through the separable kernel: g = (S_0 (x) S_1 (x) ...) xi with S_a the
-    symmetric square root of the 1D covariance matrix (eigendecomposition), then
-    rescaled so that max g - min g = ln(contrast).  a = exp(g)."""
+def se_covariance_1d(n: int, length: float) -> np.ndarray:
+    """squared-exponential covariance exp(-(x-y)^2 / (2 length^2)) between the n cell
+    centres (i + 1/2) / n of the unit interval"""
+    x = (np.arange(n) + 0.5) / n
+    return np.exp(-((x[:, None] - x[None, :]) ** 2) / (2 * length ** 2))
+
+
+def gaussian_field_raw(seed: int, cells, length: float = 0.1) -> np.ndarray:
+    """an exact sample of the zero-mean Gaussian field with covariance
+    K_0 (x) K_1 (x) ... (K_a = se_covariance_1d(cells[a], length)) over the cell centres
+    of the unit box, flattened in the reference's cell order (last axis fastest):
+    g = (S_0 (x) S_1 (x) ...) xi with S_a the symmetric square root of K_a
+    (eigendecomposition) and xi the SplitMix64 normals of `seed`."""
     cells = tuple(int(c) for c in cells)
-    roots = []
-    for n in cells:
-        x = (np.arange(n) + 0.5) / n
-        K = np.exp(-((x[:, None] - x[None, :]) ** 2) / (2 * length ** 2))
-        lam, V = np.linalg.eigh(K)
-        roots.append((V * np.sqrt(np.maximum(lam, 0.0))) @ V.T)
     g = splitmix_normals(seed, int(np.prod(cells))).reshape(cells)
-    for a, S in enumerate(roots):
+    for a, n in enumerate(cells):
+        lam, V = np.linalg.eigh(se_covariance_1d(n, length))
+        S = (V * np.sqrt(np.maximum(lam, 0.0))) @ V.T
         g = np.moveaxis(np.tensordot(S, g, axes=([1], [a])), 0, a)
+    return g.reshape(-1)
+
+
+def gaussian_random_field(seed: int, cells, length: float = 0.1, contrast: float = 1e3):
+    """log-coefficient g per cell (reference cell order: last axis fastest): the sample
+    gaussian_field_raw(seed, cells, length), centred and rescaled so that
+    max g - min g = ln(contrast).  The coefficient a = exp(g) then has
+    max a / min a = contrast exactly."""
+    g = gaussian_field_raw(seed, cells, length)
     g = g - g.mean()
     span = g.max() - g.min()
     if span > 0:
         g = g * (np.log(contrast) / span)
-    return g.reshape(-1)
+    return g
 
 
 # the five workloads: name -> (dim, degree, cells, start, end, seed, nterms)

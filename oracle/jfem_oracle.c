@@ -454,6 +454,12 @@ orc_csr *orc_assemble_w(int d, int k, const int *cells, const double *start, con
                         int rule)
 {
     orc_asm_desc D;
+    /* Every kind but GRAD_GRAD evaluates value(x) / gradient(x), i.e. in the first support cell
+     * containing x (ScalarShapeFunction.java:26-39; TPCellIntegral.java:124-157); this restatement
+     * evaluates in the integrated cell, which is the same at interior points only.  Gauss-Lobatto
+     * points lie on cell faces, so those kinds are refused with them. */
+    for (int i = 0; i < nint; i++)
+        if (kinds[i] != ORC_GRAD_GRAD && !(rule >= 1 && rule <= 5)) return NULL;
     memset(&D, 0, sizeof(D));
     D.d = d; D.k = k; D.rule = rule; D.nint = nint; D.kinds = kinds; D.wconst = wconst; D.cell_coeff = cell_coeff;
     long nc = 1;
@@ -543,6 +549,9 @@ int orc_assemble_rhs(int d, int k, const int *cells, const double *start, const 
                      orc_sfun f, void *ctx, double *rhs)
 {
     orc_asm_desc D;
+    /* VALUE evaluates shapeFunction.value(x) (TPRightHandSideIntegral.java:26-29): in-cell only at
+     * interior points, so Gauss-Lobatto rules are refused as in orc_assemble_w */
+    if (!(rule >= 1 && rule <= 5)) return -1;
     memset(&D, 0, sizeof(D));
     D.d = d; D.k = k;
     long nc = 1;
@@ -1077,6 +1086,13 @@ typedef struct {
 
 static void spmv_l(const orc_level *L, const double *x, double *y) { orc_spmv(L->n, L->rowptr, L->cols, L->vals, x, y); }
 
+/* Test hook: plants a known mistake in smooth() so that the closed-form smoother tests can show
+ * they catch it.  0 = none (the default, never set outside those tests); 1 = Chebyshev c1 without
+ * the factor rho_old; 2 = Chebyshev c2 without the factor 2; 3 = interval centre taken as the
+ * half-width; 4 = Jacobi without omega; 5 = Richardson without its residual-norm exit. */
+static int g_smooth_plant = 0;
+void orc_set_smooth_plant(int plant) { g_smooth_plant = plant; }
+
 static void smooth(const orc_level *L, const double *b, double *x)
 {
     const long n = L->n;
@@ -1086,17 +1102,19 @@ static void smooth(const orc_level *L, const double *b, double *x)
         for (int it = 0; it < S->runs; it++) {
             spmv_l(L, x, t);
             for (long i = 0; i < n; i++) r[i] = b[i] - t[i];
-            if (nrm(n, r) < 1e-10) break;
+            if (nrm(n, r) < 1e-10 && g_smooth_plant != 5) break;
             for (long i = 0; i < n; i++) x[i] = x[i] + r[i] * S->omega;
         }
     } else if (S->kind == ORC_SM_JACOBI) {
         for (int it = 0; it < S->runs; it++) {
             spmv_l(L, x, t);
             for (long i = 0; i < n; i++) r[i] = b[i] - t[i];
-            for (long i = 0; i < n; i++) x[i] = x[i] + (L->dinv[i] * r[i]) * S->omega;
+            const double om = g_smooth_plant == 4 ? 1.0 : S->omega;
+            for (long i = 0; i < n; i++) x[i] = x[i] + (L->dinv[i] * r[i]) * om;
         }
     } else {
-        const double theta = 0.5 * (S->lmax + S->lmin), delta = 0.5 * (S->lmax - S->lmin);
+        const double delta = 0.5 * (S->lmax - S->lmin);
+        const double theta = g_smooth_plant == 3 ? delta : 0.5 * (S->lmax + S->lmin);
         const double sigma = theta / delta;
         double *dd = vnew(n);
         for (int app = 0; app < S->runs; app++) {
@@ -1108,7 +1126,8 @@ static void smooth(const orc_level *L, const double *b, double *x)
                     for (long i = 0; i < n; i++) dd[i] = (L->dinv[i] * r[i]) / theta;
                 } else {
                     const double rho = 1.0 / (2.0 * sigma - rho_old);
-                    const double c1 = rho * rho_old, c2 = 2.0 * rho / delta;
+                    const double c1 = g_smooth_plant == 1 ? rho : rho * rho_old;
+                    const double c2 = (g_smooth_plant == 2 ? 1.0 : 2.0) * rho / delta;
                     for (long i = 0; i < n; i++) dd[i] = c1 * dd[i] + c2 * (L->dinv[i] * r[i]);
                     rho_old = rho;
                 }

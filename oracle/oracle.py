@@ -93,6 +93,7 @@ def lib():
         L.orc_mg_wcycle.argtypes = [C.c_void_p, dp, dp]
         L.orc_mg_set_raw.argtypes = [C.c_void_p, C.c_int]
         L.orc_mg_smooth.argtypes = [C.c_void_p, C.c_int, dp, dp]
+        L.orc_set_smooth_plant.argtypes = [C.c_int]
         L.orc_mg_set_interval.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
         L.orc_solve.argtypes = [C.c_int, C.c_long, lp, ip, dp, C.c_int, C.c_void_p, lp, ip, dp,
                                 dp, dp, C.c_double, C.c_long, C.c_int, C.POINTER(_SolveOut), dp]
@@ -252,8 +253,10 @@ def rhs_sines(cells, k, start, end, kw, amp, rule=5):
     rhs = np.zeros(n)
     kw = np.ascontiguousarray(kw, dtype=np.int32)
     amp = np.ascontiguousarray(amp, dtype=np.float64)
-    lib().orc_assemble_rhs_sines(d, k, _i(cells), _d(np.asarray(start, float)), _d(np.asarray(end, float)),
-                                 rule, len(amp), _i(kw), _d(amp), _d(rhs))
+    rc = lib().orc_assemble_rhs_sines(d, k, _i(cells), _d(np.asarray(start, float)), _d(np.asarray(end, float)),
+                                      rule, len(amp), _i(kw), _d(amp), _d(rhs))
+    if rc < 0:
+        raise ValueError("oracle right-hand side failed (Gauss-Lobatto rules are refused)")
     return rhs
 
 
@@ -462,6 +465,23 @@ class MG:
         x = np.ascontiguousarray(x, dtype=np.float64).copy()
         lib().orc_mg_smooth(self.ptr, l, _d(np.ascontiguousarray(b, float)), _d(x))
         return x
+
+
+class planted_smoother:
+    """Context manager for the closed-form smoother tests: plants a known mistake in the oracle's
+    smoother (see orc_set_smooth_plant) and removes it on exit."""
+    CHEB_C1, CHEB_C2, CHEB_CENTRE, JACOBI_OMEGA, RICHARDSON_EXIT = 1, 2, 3, 4, 5
+
+    def __init__(self, plant):
+        self.plant = plant
+
+    def __enter__(self):
+        lib().orc_set_smooth_plant(self.plant)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_smooth_plant(0)
+        return False
 
 
 class SplitMix64:

@@ -85,3 +85,32 @@ def test_mixed_integral_kinds_match_header():
     assert M.MixedTPCellIntegral(-2.0, M.MixedTPCellIntegral.VALUE_GRAD).kind == kinds["VALUE_GRAD"]
     with pytest.raises(N.UnsupportedOperationException):
         M.MixedTPCellIntegral("GradValue")
+
+
+def test_csr_import_argument_errors_without_device():
+    """jfem_mat_from_csr refuses a row pointer that does not start at 0 or is not monotone with
+    JFEM_ERR_ARG before it touches the context or the device (the context pointer here is a
+    dummy that is never dereferenced); the Python mirror refuses CSR arrays shorter than
+    rowptr[-1] before calling the library"""
+    import numpy as np
+    from jfem_amd import api, _native as N
+    L = N.lib()
+    ctx_storage = ctypes.create_string_buffer(64)
+    dummy_ctx = ctypes.c_void_p(ctypes.addressof(ctx_storage))
+    cols = np.zeros(4, dtype=np.int32)
+    vals = np.ones(4)
+    out = ctypes.c_void_p()
+    for rp in ([1, 2, 4], [-1, 2, 4], [0, 3, 2]):
+        rp = np.array(rp, dtype=np.int64)
+        rc = L.jfem_mat_from_csr(dummy_ctx, 2, 2, N.lptr(rp), N.iptr(cols), N.dptr(vals), ctypes.byref(out))
+        assert rc == 1, (rp, rc)      # JFEM_ERR_ARG
+        assert not out.value
+    assert b"row pointer" in L.jfem_last_error()
+
+    class _Ctx:
+        h = dummy_ctx
+
+    with pytest.raises(N.IllegalArgumentException):
+        api.SparseMatrix.from_csr([0, 2, 5], [0, 1, 0, 1], [1.0, 2.0, 3.0, 4.0, 5.0], ctx=_Ctx())
+    with pytest.raises(N.IllegalArgumentException):
+        api.SparseMatrix.from_csr([0, 2, 4], [0, 1, 0, 1], [1.0, 2.0, 3.0], ctx=_Ctx())

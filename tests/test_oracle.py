@@ -551,3 +551,127 @@ def test_fe_value_reproduces_qk_polynomials(oracle, cells, k):
     assert np.abs(got - u(pts)).max() <= 1e-12 * np.abs(u(pts)).max()
     outside = np.array([[end[a] + 0.5 for a in range(d)]])
     assert O.fe_value(cells, k, start, end, u(xyz), outside)[0] == 0.0
+
+
+# ------------------------------------------- smoothers, pinned by their closed forms
+# The 1D Q1 Laplacian on n uniform cells, after Dirichlet elimination
+# (AcceptsMatrixBoundaryValues.java:18-44), is A = (1/h) tridiag(-1, 2, -1) on the
+# interior nodes and the identity on the two boundary nodes.  D^-1 A restricted to the
+# interior is tridiag(-1/2, 1, -1/2) with eigenpairs lambda_j = 1 - cos(j pi / n),
+# v_j(i) = sin(j pi i / n), j = 1..n-1 (zero on the boundary nodes), and A's own
+# eigenvalues are mu_j = 2 n lambda_j.  With b = 0 every smoother maps v_j to a known
+# multiple of v_j:
+#   Jacobi (JacobiSmoother.java:28-50), `runs` sweeps:          (1 - omega lambda)^runs
+#   Richardson (RichardsonSmoother.java:30-48), `runs` sweeps:  (1 - omega mu)^runs
+#   Chebyshev-Jacobi on [lmin, lmax] (Saad, Iterative Methods for Sparse Linear Systems,
+#   2nd ed., Alg. 12.1), degree m, `runs` applications:
+#       (T_m((theta - lambda) / delta) / T_m(theta / delta))^runs,
+#       theta = (lmax + lmin) / 2, delta = (lmax - lmin) / 2.
+def _laplace_1d(O, n, smoother, **kw):
+    return O.MG([n], 1, 1, [0.0], [1.0], [O.GRAD_GRAD], [1.0], smoother=smoother, cycles=1, fmg=False, **kw)
+
+
+def _cheb_factor(lam, lmin, lmax, deg):
+    theta, delta = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
+    c = [0] * deg + [1]
+    return (np.polynomial.chebyshev.chebval((theta - lam) / delta, c)
+            / np.polynomial.chebyshev.chebval(theta / delta, c))
+
+
+def _smoother_deviation(O, G, n, factor):
+    """max over the eigenvectors v_j of |smooth(0, v_j) - factor(lambda_j) v_j| / |v_j|"""
+    i = np.arange(n + 1)
+    dev = 0.0
+    for j in range(1, n):
+        lam = 1 - np.cos(j * np.pi / n)
+        v = np.sin(j * np.pi * i / n)
+        out = G.smooth(0, np.zeros(n + 1), v)
+        dev = max(dev, np.linalg.norm(out - factor(lam) * v) / np.linalg.norm(v))
+    return dev
+
+
+def test_laplace_1d_spectrum(oracle):
+    """the premise of the closed-form smoother tests: the oracle's level matrix and Jacobi
+    diagonal are the ones stated above"""
+    O = oracle
+    n = 16
+    G = _laplace_1d(O, n, O.SM_JACOBI, runs=1, omega=1.0)
+    A = G.level_matrix(0).to_scipy().toarray()
+    T = n * (2 * np.eye(n - 1) - np.eye(n - 1, k=1) - np.eye(n - 1, k=-1))
+    assert np.allclose(A[1:-1, 1:-1], T, rtol=0, atol=1e-12 * n)
+    assert A[0, 0] == 1 and A[n, n] == 1 and not A[0, 1:].any() and not A[1:, 0].any()
+    dinv = G.level_dinv(0)
+    assert np.allclose(dinv[1:-1], 1 / (2 * n), rtol=1e-14) and dinv[0] == 1 and dinv[n] == 1
+
+
+@pytest.mark.parametrize("deg,runs,lmin,lmax", [(4, 1, 0.05, 2.0), (4, 2, 2.0 / 30, 2.0), (3, 2, 0.3, 1.8),
+                                                (1, 3, 0.5, 2.0), (6, 1, 0.1, 2.2)])
+def test_chebyshev_smoother_closed_form(oracle, deg, runs, lmin, lmax):
+    """the oracle's Chebyshev-Jacobi smoother damps each eigenvector of D^-1 A by the scaled
+    Chebyshev polynomial, to 1e-12; each planted mistake in the recurrence or the interval
+    (c1 without rho_old, c2 without the factor 2, the centre taken as the half-width) breaks it"""
+    O = oracle
+    n = 24
+    G = _laplace_1d(O, n, O.SM_CHEBYSHEV, runs=runs, degree=deg, lmin=[lmin], lmax=[lmax])
+    f = lambda lam: _cheb_factor(lam, lmin, lmax, deg) ** runs
+    assert _smoother_deviation(O, G, n, f) < 1e-12
+    plants = [O.planted_smoother.CHEB_CENTRE] if deg == 1 else \
+        [O.planted_smoother.CHEB_C1, O.planted_smoother.CHEB_C2, O.planted_smoother.CHEB_CENTRE]
+    for plant in plants:     # degree 1 never reaches c1, c2
+        with O.planted_smoother(plant):
+            assert _smoother_deviation(O, G, n, f) > 1e-4, plant
+
+
+@pytest.mark.parametrize("omega,runs", [(0.6, 2), (2.0 / 3.0, 3), (1.0, 1)])
+def test_jacobi_smoother_closed_form(oracle, omega, runs):
+    """(1 - omega lambda)^runs per eigenvector to 1e-12; dropping omega breaks it (omega != 1)"""
+    O = oracle
+    n = 24
+    G = _laplace_1d(O, n, O.SM_JACOBI, runs=runs, omega=omega)
+    f = lambda lam: (1 - omega * lam) ** runs
+    assert _smoother_deviation(O, G, n, f) < 1e-12
+    if omega != 1.0:
+        with O.planted_smoother(O.planted_smoother.JACOBI_OMEGA):
+            assert _smoother_deviation(O, G, n, f) > 1e-4
+
+
+def test_richardson_smoother_closed_form_and_exit(oracle):
+    """(1 - omega mu)^runs per eigenvector of A; the sweep stops once ||b - A x|| < 1e-10
+    (RichardsonSmoother.java:38-40): with a residual of norm 1e-12 x stays unchanged bit for bit,
+    and without that exit (planted) it moves"""
+    O = oracle
+    n, omega, runs = 12, 0.01, 3
+    G = _laplace_1d(O, n, O.SM_RICHARDSON, runs=runs, omega=omega)
+    f = lambda lam: (1 - omega * 2 * n * lam) ** runs
+    assert _smoother_deviation(O, G, n, f) < 1e-12
+    A = G.level_matrix(0).to_scipy()
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(n + 1)
+    d = rng.standard_normal(n + 1)
+    b = A @ x + 1e-12 * d / np.linalg.norm(d)
+    assert np.linalg.norm(b - A @ x) < 1e-10
+    assert np.array_equal(G.smooth(0, b, x), x)
+    with O.planted_smoother(O.planted_smoother.RICHARDSON_EXIT):
+        assert not np.array_equal(G.smooth(0, b, x), x)
+
+
+def test_face_quadrature_refused(oracle):
+    """every integral kind but GRAD_GRAD evaluates the shape functions through value(x) /
+    gradient(x), i.e. in the first support cell containing x (ScalarShapeFunction.java:26-39,
+    TPCellIntegral.java:124-157, TPRightHandSideIntegral.java:26-29); Gauss-Lobatto points lie on
+    cell faces, where that cell can be the neighbour, so the restatement (which integrates in the
+    cell itself) refuses those kinds with Lobatto rules and keeps GRAD_GRAD (gradientInCell)"""
+    O = oracle
+    cells, k = [3, 2], 2
+    for rule in (12, 13, 14, 15, 16):
+        A = O.assemble(cells, k, [0, 0], [1, 1], [O.GRAD_GRAD], [1.0], rule=rule)
+        assert A.n == 35
+        for kind in (O.VALUE_VALUE, O.H1):
+            with pytest.raises(ValueError):
+                O.assemble(cells, k, [0, 0], [1, 1], [kind], [1.0], rule=rule)
+        with pytest.raises(ValueError):
+            O.assemble(cells, k, [0, 0], [1, 1], [O.GRAD_GRAD, O.VALUE_VALUE], [1.0, 1.0], rule=rule)
+        with pytest.raises(ValueError):
+            O.rhs_sines(cells, k, [0, 0], [1, 1], [[1, 2]], [1.0], rule=rule)
+    for rule in (1, 2, 3, 4, 5):
+        O.assemble(cells, k, [0, 0], [1, 1], [O.VALUE_VALUE, O.H1], [1.0, 2.0], rule=rule)
