@@ -542,3 +542,76 @@ def test_mg_galerkin_coarse_systems(jf, oracle, base, k):
     A = sp.csr_matrix((vals, cols, rp), shape=(n, n))
     assert np.linalg.norm(b - A @ x) <= 1.01e-8 * np.linalg.norm(b)
     assert it.iterations < 40
+
+
+# ------------------------------------------------- BiCGStab's breakdown fallback
+def test_bicgstab_breakdown_falls_back_to_gmres(jf, oracle):
+    """solveBiCGStab returns solveGMRES(operator, rhs, tol) once |rhoLast| < tol
+    (IterativeSolver.java:293-294; GMRES then starts from its default e_0).  With the C1 system
+    scaled to ||b|| = 1e-4, x0 = 0 and tol = 1e-7 (absolute), rhoLast = r0.r0 = 1e-8 < tol at the
+    second iteration: the GPU falls back as the oracle does and agrees with it to 1e-8"""
+    s, ref, rhs = _c1_system(jf, oracle, (24, 20))
+    b = rhs * (1e-4 / np.linalg.norm(rhs))
+    tol = 1e-7
+    it = jf.IterativeSolver()
+    x = it.solveBiCGStab(s.getSystemMatrix(), jf.DenseVector(b), tol, x0=jf.DenseVector(np.zeros(ref.n))).numpy()
+    assert it.info.fell_back == 1
+    xr, itr, resr, fb = oracle.solve("bicgstab", ref, b, np.zeros(ref.n), tol)
+    assert fb == 1
+    assert np.linalg.norm(x - xr) <= 1e-8 * np.linalg.norm(xr)
+    assert np.linalg.norm(b - ref.matvec(x)) <= 1.01 * tol
+    # the unscaled system at a relative tolerance: the fallback decision (|r_k . r_0| < tol
+    # near convergence) is the oracle's, and so is the solution
+    tol2 = 1e-10 * np.linalg.norm(rhs)
+    it2 = jf.IterativeSolver()
+    x2 = it2.solveBiCGStab(s.getSystemMatrix(), jf.DenseVector(rhs), tol2,
+                           x0=jf.DenseVector(np.zeros(ref.n))).numpy()
+    xr2, _, _, fb2 = oracle.solve("bicgstab", ref, rhs, np.zeros(ref.n), tol2)
+    assert it2.info.fell_back == fb2
+    assert np.linalg.norm(x2 - xr2) <= 1e-8 * np.linalg.norm(xr2)
+
+
+def test_pbicgstab_breakdown_falls_back_to_pgmres(jf, oracle):
+    """solvePBiCGStab returns solvePGMRES(operator, preconditioner, rhs, tol) once |rhoLast| < tol
+    (IterativeSolver.java:369-370; PGMRES starts from 0), with the multigrid preconditioner"""
+    G, O, spaces = _mg_pair(jf, oracle, [2, 2], 2, 3, jf.JacobiSmoother(2, 0.6), cycles=1, fmg=False)
+    A = G.getFinestSystem()
+    n = O.level_n(2)
+    rng = np.random.default_rng(21)
+    b = rng.standard_normal(n)
+    b[oracle.boundary_nodes([8, 8], 2)] = 0
+    b *= 1e-4 / np.linalg.norm(b)
+    tol = 1e-7
+    it = jf.IterativeSolver()
+    x = it.solvePBiCGStab(A, G, jf.DenseVector(b), tol, x0=jf.DenseVector(np.zeros(n))).numpy()
+    assert it.info.fell_back == 1
+    xr, itr, resr, fb = oracle.solve("bicgstab", O.level_matrix(2), b, np.zeros(n), tol, precond=O)
+    assert fb == 1
+    assert np.linalg.norm(x - xr) <= 1e-8 * np.linalg.norm(xr)
+
+
+# ------------------------------------------------- quadrature on cell faces
+def test_face_quadrature_refused(jf, oracle):
+    """GRAD_GRAD integrates gradientInCell(x, cell) and accepts Gauss-Lobatto rules (bit-exact
+    with the oracle); VALUE_VALUE, H1 and the right-hand side evaluate value(x) / gradient(x) in
+    the first support cell containing x (ScalarShapeFunction.java:26-39), which on a face can be
+    the neighbour: with Lobatto rules they raise UnsupportedOperationException, as the oracle
+    refuses them"""
+    cells, k = [4, 3], 2
+    s, start, end = _space(jf, cells, k)
+    TP = jf.TPCellIntegral
+    for rule in (12, 13, 14):
+        A = s.assemble_matrix([TP(1.0, TP.GRAD_GRAD, rule)])
+        rp, cols, vals = A.to_csr()
+        ref = oracle.assemble(cells, k, start, end, [0], [1.0], rule=rule)
+        assert np.array_equal(rp, ref.rowptr) and np.array_equal(cols, ref.cols)
+        assert np.array_equal(vals, ref.vals)
+        for name in (TP.VALUE_VALUE, TP.H1):
+            with pytest.raises(jf.UnsupportedOperationException):
+                s.assemble_matrix([TP(1.0, name, rule)])
+        with pytest.raises(jf.UnsupportedOperationException):
+            s.assemble_matrix([TP(1.0, TP.GRAD_GRAD, rule), TP(1.0, TP.VALUE_VALUE, rule)])
+        s.initializeRhs()
+        src = jf.SinusoidSource(np.array([[1, 2]], dtype=np.int32), np.array([1.0]))
+        with pytest.raises(jf.UnsupportedOperationException):
+            s.writeCellIntegralsToRhs([jf.TPRightHandSideIntegral(src, "Value", rule)], s.getRhs())
