@@ -208,6 +208,17 @@ int jfem_solve_gmres(jfem_mat *A, jfem_mg *M, const jfem_vec *b, jfem_vec *x, do
  * polynomial of the given degree on [lmin, lmax] of D^-1 A (new). */
 int jfem_smooth(jfem_mat *A, int kind, int runs, double omega, int degree, double lmin, double lmax,
                 const jfem_vec *b, jfem_vec *x);
+/* jfem_smooth without any device allocation.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, kernels and results as jfem_smooth; its four work vectors
+ * (A x, the residual, D^-1, the Chebyshev direction) live in a static device array of
+ * 4 * JFEM_NOALLOC_MAX_ROWS doubles (32 MiB) that each GPU the library runs on holds until the
+ * process ends.  A with more than JFEM_NOALLOC_MAX_ROWS rows is refused (JFEM_ERR_ARG).  All
+ * calls on one GPU share that array: calls that overlap on one GPU (other threads, other
+ * contexts' streams) race. */
+#define JFEM_NOALLOC_MAX_ROWS 1048576
+int jfem_smooth_noalloc(jfem_mat *A, int kind, int runs, double omega, int degree, double lmin, double lmax,
+                        const jfem_vec *b, jfem_vec *x);
 /* GMRES2.solve (M/linalg/GMRES2.java:34-104, Arnoldi M/linalg/Arnoldi.java:13-82): minimal
  * residual steps along M A-orthonormalised directions of the power sequence of M A, stop when
  * the recursively updated ||r||_2 <= tol (absolute), restart on a vanishing direction; M

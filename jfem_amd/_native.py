@@ -110,6 +110,8 @@ _SIGS = {
     "jfem_solve_bicgstab": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_long, C.POINTER(SolveInfo)]),
     "jfem_solve_gmres": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, C.POINTER(SolveInfo)]),
     "jfem_smooth": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, _vp, _vp]),
+    "jfem_smooth_noalloc": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, _vp,
+                                      _vp]),
     "jfem_solve_gmres2": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_long, C.POINTER(SolveInfo)]),
     "jfem_solve_fcg": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_long, C.POINTER(SolveInfo), _dp]),
     "jfem_solve_stokes_fgmres": (C.c_int, [_vp, _vp, _vp, C.POINTER(StokesOpts), _vp, _vp, C.c_double,

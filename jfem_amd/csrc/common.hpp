@@ -152,6 +152,7 @@ struct jfem_ctx {
     // context is destroyed, so repeated solves allocate nothing
     double* work = nullptr;
     size_t work_n = 0;                // doubles held by work
+    double* noalloc = nullptr;        // this GPU's copy of the no-allocation scratch array
     int cu_count = 256;
 };
 
@@ -240,6 +241,9 @@ struct DevPtr {
 };
 // the context's solver workspace with room for n doubles (reallocated only to grow)
 double* ctx_work(jfem_ctx* c, size_t n);
+// device address of the static scratch array of the no-allocation entry points on the context's
+// GPU (4 * JFEM_NOALLOC_MAX_ROWS doubles), resolved once per device by jfem_ctx_create
+double* noalloc_scratch(const jfem_ctx* c);
 
 double vec_dot(jfem_ctx* c, long n, const double* x, const double* y);
 // deterministic two-stage reduction of partials already written into ctx->red_partials

@@ -186,6 +186,38 @@ using namespace jfem;
 
 namespace jfem {
 
+// Scratch of the no-allocation entry points (jfem_smooth_noalloc): a file-scope device array
+// that every GPU the library runs on holds until the process ends.  In host code its name is
+// a host-side stand-in, not a device address; the address of each GPU's copy comes from
+// hipGetSymbolAddress with that GPU current (scratch_address) and only that address is used.
+static __device__ __align__(256) double scratch[4 * (size_t)JFEM_NOALLOC_MAX_ROWS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// once per device, with `device` current
+double* scratch_address(int device)
+{
+    JFEM_CHECK(device >= 0 && device < kMaxDevices, JFEM_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void* ptr = nullptr;
+        JFEM_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        JFEM_CHECK(ptr != nullptr, JFEM_ERR_DEVICE, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_dev[device] = static_cast<double*>(ptr);
+    }
+    return g_scratch_dev[device];
+}
+}  // namespace
+
+double* noalloc_scratch(const jfem_ctx* c)
+{
+    JFEM_CHECK(c->noalloc, JFEM_ERR_STATE, "no-allocation scratch not resolved for this context");
+    return c->noalloc;
+}
+
 double* ctx_work(jfem_ctx* c, size_t n)
 {
     if (n > c->work_n) {
@@ -224,6 +256,7 @@ int jfem_ctx_create(int device, jfem_ctx** out)
     hipDeviceProp_t prop;
     JFEM_HIP(hipGetDeviceProperties(&prop, device));
     c->cu_count = prop.multiProcessorCount;
+    c->noalloc = scratch_address(device);   // device is current (hipSetDevice above)
     *out = c;
     API_END
 }

@@ -532,19 +532,19 @@ __global__ void k_cheb_apply(long n, int first, double theta, double c1, double 
     }
 }
 
+// work: 4 n doubles (A x, the residual, D^-1, the Chebyshev direction); every element a sweep
+// reads has been written earlier in the same call, so the contents on entry do not matter
 static void smooth_any(jfem_mat* A, int kind, int runs, double omega, int degree, double lmin, double lmax,
-                       const double* b, double* x)
+                       const double* b, double* x, double* work)
 {
     jfem_ctx* c = A->ctx;
     const long n = A->rows;
     const int nb = kgrid(n);
-    DevBuf<double> t(n), r(n), dinv, dd;
+    const DevPtr<double> t{work}, r{work + n}, dinv{work + 2 * n}, dd{work + 3 * n};
     if (kind != JFEM_SM_RICHARDSON) {
-        dinv.alloc(n);
         mat_diag(A, t);
         k_dinv_of<<<nb, KB, 0, c->stream>>>(n, t, dinv);
     }
-    if (kind == JFEM_SM_CHEBYSHEV) dd.alloc(n);
     const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
     for (int it = 0; it < runs; it++) {
         if (kind == JFEM_SM_CHEBYSHEV) {
@@ -1064,7 +1064,28 @@ int jfem_smooth(jfem_mat* A, int kind, int runs, double omega, int degree, doubl
     JFEM_CHECK(runs >= 0, JFEM_ERR_ARG, "negative run count");
     if (kind == JFEM_SM_CHEBYSHEV)
         JFEM_CHECK(degree >= 1 && lmin > 0 && lmax > lmin, JFEM_ERR_ARG, "Chebyshev needs degree >= 1 and 0 < lmin < lmax");
-    smooth_any(A, kind, runs, omega, degree, lmin, lmax, b->d, x->d);
+    DevBuf<double> work(4 * (size_t)A->rows);
+    smooth_any(A, kind, runs, omega, degree, lmin, lmax, b->d, x->d, work);
+    API_END
+}
+
+int jfem_smooth_noalloc(jfem_mat* A, int kind, int runs, double omega, int degree, double lmin, double lmax,
+                        const jfem_vec* b, jfem_vec* x)
+{
+    API_BEGIN
+    JFEM_CHECK(A && b && x, JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(A->rows == A->cols && b->n == A->rows && x->n == A->rows, JFEM_ERR_ARG, "Incompatible sizes");
+    JFEM_CHECK(A->rows <= JFEM_NOALLOC_MAX_ROWS, JFEM_ERR_ARG,
+               "jfem_smooth_noalloc: more than JFEM_NOALLOC_MAX_ROWS rows (its scratch holds 4 vectors of that length)");
+    JFEM_CHECK(kind == JFEM_SM_RICHARDSON || kind == JFEM_SM_JACOBI || kind == JFEM_SM_CHEBYSHEV, JFEM_ERR_UNSUPPORTED,
+               "unknown smoother");
+    JFEM_CHECK(runs >= 0, JFEM_ERR_ARG, "negative run count");
+    if (kind == JFEM_SM_CHEBYSHEV)
+        JFEM_CHECK(degree >= 1 && lmin > 0 && lmax > lmin, JFEM_ERR_ARG, "Chebyshev needs degree >= 1 and 0 < lmin < lmax");
+    // the device address of this GPU's copy of the static scratch array, resolved when the
+    // context was created on it
+    double* work = noalloc_scratch(A->ctx);
+    smooth_any(A, kind, runs, omega, degree, lmin, lmax, b->d, x->d, work);
     API_END
 }
 
