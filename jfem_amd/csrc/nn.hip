@@ -554,9 +554,6 @@ __global__ __launch_bounds__(MidTile<D>::NW * 64) void k_last_lds(Lat L, const u
 #ifndef JFEM_NN_NT
 #define JFEM_NN_NT 4
 #endif
-#ifndef JFEM_NN_NOLDS
-#define JFEM_NN_NOLDS 0
-#endif
 constexpr int MID_NT = JFEM_NN_NT;   // pixel tiles per wavefront in the 32 -> 32 layers
 
 // LDS bytes one workgroup may allocate on the current device (160 KiB on gfx950)
@@ -578,7 +575,6 @@ template <int MODE>
 static void launch_mid(hipStream_t st, const Lat& L, int taps, const unsigned short* in, const unsigned short* W,
                        const float* bias, const unsigned short* mask, unsigned short* out)
 {
-#if !JFEM_NN_NOLDS
     if ((L.d == 3 || L.d == 2) && lds_per_block() >= (long)sizeof(bf16x8) * 4 * MidTile<3>::cs<4>()) {
         const int d = L.d;
         const int ty = d == 3 ? MidTile<3>::TY : MidTile<2>::TY, tz = d == 3 ? MidTile<3>::TZ : MidTile<2>::TZ;
@@ -593,7 +589,6 @@ static void launch_mid(hipStream_t st, const Lat& L, int taps, const unsigned sh
         JFEM_HIP(hipGetLastError());
         return;
     }
-#endif
     const long ngroups = (L.np + 32 * MID_NT - 1) / (32 * MID_NT);
     const int grid = (int)std::min<long>((ngroups + 3) / 4, 1024);
     const size_t lds = sizeof(bf16x8) * taps * 2 * 64;
@@ -833,38 +828,6 @@ __global__ void k_refresh_bf16(int taps, const float* __restrict__ W, unsigned s
 __global__ void k_to_bf16(int n, const float* __restrict__ src, unsigned short* __restrict__ dst)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = f2bf(src[i]);
-}
-
-// last layer with packed BF16 dot products (v_dot2_f32_bf16, FP32 accumulation):
-// out[p] = b3 + sum_t sum_c W3[t][c] act[nb(p, t)][c], W3 in BF16
-template <int TAPS>
-__global__ __launch_bounds__(256) void k_last_dot(Lat L, const unsigned short* __restrict__ act,
-                                                  const unsigned short* __restrict__ W3h,
-                                                  const float* __restrict__ b3, float* __restrict__ out)
-{
-    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < L.np; p += (long)gridDim.x * blockDim.x) {
-        int ip[3];
-        lat_xyz(L, p, ip);
-        float acc = b3[0];
-        for (int t = 0; t < TAPS; t++) {
-            const long q = lat_nb(L, p, ip, t, 1);
-            if (q < 0) continue;
-#pragma unroll
-            for (int g = 0; g < 4; g++) {
-                const bfv8 a = *reinterpret_cast<const bfv8*>(act + q * NN_H + 8 * g);
-                const bfv8 w = *reinterpret_cast<const bfv8*>(W3h + t * NN_H + 8 * g);
-                acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_shufflevector(a, a, 0, 1),
-                                                      __builtin_shufflevector(w, w, 0, 1), acc, false);
-                acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_shufflevector(a, a, 2, 3),
-                                                      __builtin_shufflevector(w, w, 2, 3), acc, false);
-                acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_shufflevector(a, a, 4, 5),
-                                                      __builtin_shufflevector(w, w, 4, 5), acc, false);
-                acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_shufflevector(a, a, 6, 7),
-                                                      __builtin_shufflevector(w, w, 6, 7), acc, false);
-            }
-        }
-        out[p] = acc;
-    }
 }
 
 // ------------------------------------------------------------- host side

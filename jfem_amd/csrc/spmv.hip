@@ -87,22 +87,18 @@ static void launch_spmv(jfem_mat* A, const VT* vals, const XT* x, XT* y)
     JFEM_HIP(hipGetLastError());
 }
 
-#ifdef JFEM_NO_STREAM
-constexpr bool kUseStream = false;   // tuning builds only: the vector kernel
-#else
-constexpr bool kUseStream = true;
-#endif
+
 
 void spmv(jfem_mat* A, const double* x, double* y)
 {
-    if (kUseStream && A->blk) launch_stream<double, double>(A, A->vals, x, EpiStore<double>{y}, nullptr);
+    if (A->blk) launch_stream<double, double>(A, A->vals, x, EpiStore<double>{y}, nullptr);
     else launch_spmv<double, double>(A, A->vals, x, y);
 }
 
 void spmv32(jfem_mat* A, const float* x, float* y)
 {
     JFEM_CHECK(A->vals32, JFEM_ERR_STATE, "matrix has no FP32 values");
-    if (kUseStream && A->blk) launch_stream<float, float>(A, A->vals32, x, EpiStore<float>{y}, nullptr);
+    if (A->blk) launch_stream<float, float>(A, A->vals32, x, EpiStore<float>{y}, nullptr);
     else launch_spmv<float, float>(A, A->vals32, x, y);
 }
 
