@@ -280,6 +280,14 @@ typedef struct {
                         * prolongation by the prolongation matrix without zero boundary values.
                         * With fmg = 2 and cycles = 1 the application is MGPReconditioner2.mvMul
                         * (M/multigrid/MGPReconditioner2.java:38-76): v + fullVCycleSolver(v - A v) */
+    int matrix_free;   /* 1: smoothing and residuals on re-assembled levels (every level, or the
+                        * finest of a Galerkin hierarchy) apply the level operator matrix-free: the
+                        * cell element matrices of GRAD_GRAD / VALUE_VALUE / H1 by 1D sum
+                        * factorisation, summed per node, on 2D / 3D grids of degree 1 or 2 (others
+                        * keep the CSR product).  The level matrices are still assembled
+                        * (jfem_mg_level_matrix); the products agree with them to rounding.  New;
+                        * the reference's smoothers call getSystem(level).mvMul
+                        * (M/multigrid/MGPreconditionerInterface.java:125-198). */
 } jfem_mg_opts;
 
 /* levels: nlev spaces, coarsest first, each with twice the cells of the
@@ -292,6 +300,11 @@ int jfem_mg_create(jfem_space *const *spaces, int nlev, int nint, const int *kin
 int jfem_mg_destroy(jfem_mg *mg);
 /* the finest level's (Dirichlet-eliminated) system matrix, owned by the hierarchy */
 int jfem_mg_level_matrix(jfem_mg *mg, int level, jfem_mat **out);
+/* getSystem(level).mvMul(x) (MGPreconditionerInterface.java:158): y = A_level x with the operator
+ * the hierarchy's cycles use (matrix-free where jfem_mg_level_matrix_free reports 1), FP64 */
+int jfem_mg_level_apply(jfem_mg *mg, int level, const jfem_vec *x, jfem_vec *y);
+/* whether the hierarchy applies level `level` matrix-free (out: 1) or by its CSR matrix (0); new */
+int jfem_mg_level_matrix_free(jfem_mg *mg, int level, int *out);
 /* MGPreconditionerInterface.mvMul: z = M r */
 int jfem_mg_apply(jfem_mg *mg, const jfem_vec *r, jfem_vec *z);
 /* one V-cycle mgStepV(finest, x, b) updating x in place */

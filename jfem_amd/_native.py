@@ -57,7 +57,7 @@ class MgOpts(C.Structure):
     _fields_ = [("smoother", C.c_int), ("runs", C.c_int), ("omega", C.c_double), ("degree", C.c_int),
                 ("cheb_ratio", C.c_double), ("cheb_safety", C.c_double), ("power_its", C.c_int),
                 ("cycles", C.c_int), ("fmg", C.c_int), ("precision", C.c_int), ("galerkin", C.c_int),
-                ("raw_transfers", C.c_int)]
+                ("raw_transfers", C.c_int), ("matrix_free", C.c_int)]
 
 
 _vp = C.c_void_p
@@ -120,6 +120,8 @@ _SIGS = {
                                  C.POINTER(_vp)]),
     "jfem_mg_destroy": (C.c_int, [_vp]),
     "jfem_mg_level_matrix": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
+    "jfem_mg_level_apply": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+    "jfem_mg_level_matrix_free": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
     "jfem_mg_apply": (C.c_int, [_vp, _vp, _vp]),
     "jfem_mg_vcycle": (C.c_int, [_vp, _vp, _vp]),
     "jfem_mg_wcycle": (C.c_int, [_vp, _vp, _vp]),

@@ -520,7 +520,7 @@ class MGPreconditioner:
 
     def __init__(self, spaces: Sequence[ContinuousTPFESpace], cellIntegrals: Sequence[TPCellIntegral], smoother,
                  cycles: int = 3, fmg=True, precision: int = FP64, cell_coeff=None, galerkin: bool = False,
-                 raw_transfers: bool = False):
+                 raw_transfers: bool = False, matrix_free: bool = True):
         self.ctx = spaces[0].ctx
         self.spaces = list(spaces)
         opts = N.MgOpts()
@@ -536,6 +536,7 @@ class MGPreconditioner:
         opts.raw_transfers = 1 if raw_transfers else 0
         opts.precision = precision
         opts.galerkin = 1 if galerkin else 0   # coarse systems P^T A P (AMGFromMatrix) instead of re-assembly
+        opts.matrix_free = 1 if matrix_free else 0   # level operators by sum factorisation in the cycles
         if any(isinstance(ci.weight, list) for ci in cellIntegrals):
             raise N.UnsupportedOperationException(3, "multigrid hierarchies take scalar-weighted integrals")
         kinds = i32([ci.kind for ci in cellIntegrals])
@@ -558,6 +559,17 @@ class MGPreconditioner:
         h = C.c_void_p()
         check(N.lib().jfem_mg_level_matrix(self.h, level, C.byref(h)))
         return SparseMatrix(h, self.ctx, owned=False, keepalive=self)
+
+    def applySystem(self, level: int, x: DenseVector, out: Optional[DenseVector] = None) -> DenseVector:
+        """getSystem(level).mvMul(x) with the operator the cycles use (matrix-free where enabled)"""
+        y = out if out is not None else DenseVector(x.n, self.ctx)
+        check(N.lib().jfem_mg_level_apply(self.h, level, x.h, y.h))
+        return y
+
+    def isMatrixFree(self, level: int) -> bool:
+        v = C.c_int()
+        check(N.lib().jfem_mg_level_matrix_free(self.h, level, C.byref(v)))
+        return bool(v.value)
 
     def getFinestSystem(self) -> SparseMatrix:
         return self.getSystem(self.nlev - 1)

@@ -987,6 +987,33 @@ static ClassTables build_classes(const Grid& g, int rule)
     return C;
 }
 
+// 1D element matrices of the first cell of axis a, from the same quadrature table the assembly
+// uses: K1[i][j] = sum_q w_q phi_i'(q) phi_j'(q), M1[i][j] = sum_q w_q phi_i(q) phi_j(q), (k+1) x (k+1)
+// row-major.  Tensor-product Gauss quadrature factorises over the axes, so a cell's GRAD_GRAD
+// matrix is sum_a K1_a (x) M1_b (x) ... and its VALUE_VALUE matrix the product of the M1_a (the
+// cells of an axis differ in length by rounding only).
+void element_1d(const Grid& g, int a, int rule, double* K1, double* M1)
+{
+    double qp[8], qw[8];
+    const int Q = quad_rule(rule, qp, qw);
+    JFEM_CHECK(Q > 0, JFEM_ERR_ARG, "unknown quadrature rule");
+    JFEM_CHECK(g.k >= 1 && g.k < MAXK1, JFEM_ERR_UNSUPPORTED, "polynomial degree must be 1..4");
+    std::vector<double> s1, e1;
+    cells_1d(g.start[a], g.end[a], g.n[a], s1, e1);
+    const Tab1D t = make_tab(s1[0], e1[0], g.k, qp, qw, Q);
+    const int N = g.k + 1;
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) {
+            double kk = 0, mm = 0;
+            for (int q = 0; q < Q; q++) {
+                kk += t.w[q] * t.der[i][q] * t.der[j][q];
+                mm += t.w[q] * t.val[i][q] * t.val[j][q];
+            }
+            K1[i * N + j] = kk;
+            M1[i * N + j] = mm;
+        }
+}
+
 struct DevTables {
     Tab1D* tabs[MAXD] = {nullptr, nullptr, nullptr};
     int* cls[MAXD] = {nullptr, nullptr, nullptr};

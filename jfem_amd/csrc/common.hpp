@@ -283,6 +283,8 @@ void apply_dirichlet_mask(jfem_mat* A, double* rhs, const unsigned char* fixed_d
 void space_boundary_mask(jfem_space* s, unsigned char* mask_dev);
 jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const double* weights, const double* coeff_host,
                           int rule);
+// 1D stiffness and mass matrices ((k+1) x (k+1), row-major) of the first cell of axis a
+void element_1d(const Grid& g, int a, int rule, double* K1, double* M1);
 // multigrid preconditioner application z = M r (FP64 vectors on the context stream)
 void mg_apply(jfem_mg* mg, const double* r, double* z);
 long mg_size(const jfem_mg* mg);

@@ -15,6 +15,7 @@
 // Every level operation is templated on the working precision (FP64 or FP32).
 #include "common.hpp"
 #include "nn.hpp"
+#include "stencil.hpp"
 #include "stream.hpp"
 
 #include <chrono>
@@ -38,6 +39,12 @@ struct MgLevel {
     double* tp_w[2][jfem::MAXD] = {};         // weights
     void* tp_buf = nullptr;             // owns the tables above
     void *tw0 = nullptr, *tw1 = nullptr;
+    // matrix-free operator (smoothing and residuals without reading A): parameters and the
+    // level's cell coefficients (nullptr: constant 1)
+    bool mf = false;
+    jfem::MfParams mfp;
+    double* coef = nullptr;
+    float* coef32 = nullptr;
 };
 
 // learned smoother on the finest level (a network applied to the normalised
@@ -422,10 +429,23 @@ template <typename T> static const T* dinv_of(const MgLevel& L);
 template <> const double* dinv_of<double>(const MgLevel& L) { return L.dinv; }
 template <> const float* dinv_of<float>(const MgLevel& L) { return L.dinv32; }
 
+template <typename T> static const T* coef_of(const MgLevel& L);
+template <> const double* coef_of<double>(const MgLevel& L) { return L.coef; }
+template <> const float* coef_of<float>(const MgLevel& L) { return L.coef32; }
+
+// (A_l x)_row handed to a row epilogue: matrix-free on levels that have the operator,
+// otherwise the CSR-stream product with the level matrix
+template <typename T, typename Epi>
+static void level_apply(jfem_mg* G, const MgLevel& L, const T* x, const Epi& epi)
+{
+    if (L.mf && launch_mf<T>(G->ctx->stream, L.mfp, x, coef_of<T>(L), epi)) return;
+    launch_stream<T, T>(L.A, vals_of<T>(L), x, epi, nullptr);
+}
+
 template <typename T>
 static void level_residual(jfem_mg* G, MgLevel& L, const T* b, const T* x, T* r)
 {
-    launch_stream<T, T>(L.A, vals_of<T>(L), x, EpiResidual<T>{b, r}, nullptr);
+    level_apply<T>(G, L, x, EpiResidual<T>{b, r});
 }
 
 template <typename T>
@@ -462,7 +482,7 @@ static void smooth(jfem_mg* G, MgLevel& L, const T* b, T*& x, T*& xalt, bool x_z
                 k_jacobi0<T><<<grid_for(n, MB, 16384), MB, 0, st>>>(n, dinv_of<T>(L), b, xalt, (T)o.omega);
                 JFEM_HIP(hipGetLastError());
             } else {
-                launch_stream<T, T>(L.A, vals_of<T>(L), x, EpiJacobi<T>{b, x, dinv_of<T>(L), xalt, (T)o.omega}, nullptr);
+                level_apply<T>(G, L, x, EpiJacobi<T>{b, x, dinv_of<T>(L), xalt, (T)o.omega});
             }
             std::swap(x, xalt);
         }
@@ -495,7 +515,7 @@ static void smooth(jfem_mg* G, MgLevel& L, const T* b, T*& x, T*& xalt, bool x_z
                     JFEM_HIP(hipGetLastError());
                 } else {
                     EpiCheby<T> e{b, x, dinv_of<T>(L), xalt, dv, (T)theta, (T)c1, (T)c2, kd == 0 ? 1 : 0};
-                    launch_stream<T, T>(L.A, vals_of<T>(L), x, e, nullptr);
+                    level_apply<T>(G, L, x, e);
                 }
                 std::swap(x, xalt);
                 zero = false;
@@ -694,7 +714,7 @@ static double power_lmax(jfem_mg* G, MgLevel& L, int its)
     vec_axpby(c, n, 0.0, v, 1.0 / nv, v);
     double lam = 0;
     for (int i = 0; i < its; i++) {
-        launch_stream<double, double>(L.A, L.A->vals, v, EpiDinv{L.dinv, w}, nullptr);
+        level_apply<double>(G, L, v, EpiDinv{L.dinv, w});
         const double nw = std::sqrt(vec_dot(c, n, w, w));
         lam = nw;
         if (nw == 0) break;
@@ -712,6 +732,7 @@ static void free_mg(jfem_mg* G)
         (void)hipFree(L.bnd); (void)hipFree(L.dinv); (void)hipFree(L.dinv32);
         (void)hipFree(L.x); (void)hipFree(L.b); (void)hipFree(L.r); (void)hipFree(L.t); (void)hipFree(L.dv);
         (void)hipFree(L.tp_buf); (void)hipFree(L.tw0); (void)hipFree(L.tw1);
+        (void)hipFree(L.coef); (void)hipFree(L.coef32);
     }
     (void)hipFree(G->coarse_inv);
     (void)hipFree(G->init); (void)hipFree(G->defect); (void)hipFree(G->tmp);
@@ -840,6 +861,29 @@ static jfem_mg* create_mg(jfem_space* const* spaces, int nlev, int nint, const i
                 JFEM_HIP(hipMemsetAsync(*p, 0, sizeof(double) * n, c->stream));
             }
             if (l > 0) build_transfer(L, G->lev[l - 1]);
+            // matrix-free level operator: re-assembled systems (every level, or the finest of a
+            // Galerkin hierarchy) of GRAD_GRAD / VALUE_VALUE / H1 on 2D / 3D Q1 / Q2 grids
+            bool mf_ok = opts->matrix_free && mf_supported(L.grid) && (!opts->galerkin || l == nlev - 1);
+            for (int i = 0; i < nint; i++) mf_ok = mf_ok && kinds[i] >= JFEM_GRAD_GRAD && kinds[i] <= JFEM_H1;
+            if (mf_ok) {
+                L.mf = true;
+                std::memset(&L.mfp, 0, sizeof(L.mfp));
+                L.mfp.g = to_dev(L.grid);
+                mf_tiles(L.grid, L.mfp.ntile);
+                for (int a = 0; a < L.grid.d; a++) element_1d(L.grid, a, rule, L.mfp.K1[a], L.mfp.M1[a]);
+                for (int i = 0; i < nint; i++) {
+                    if (kinds[i] == JFEM_GRAD_GRAD || kinds[i] == JFEM_H1) L.mfp.wgg += weights[i];
+                    if (kinds[i] == JFEM_VALUE_VALUE || kinds[i] == JFEM_H1) L.mfp.wvv += weights[i];
+                }
+                if (coeff) {
+                    const long nc = L.grid.ncells();
+                    std::vector<float> c32(coef[l].begin(), coef[l].end());
+                    JFEM_HIP(hipMalloc(&L.coef, sizeof(double) * nc));
+                    JFEM_HIP(hipMalloc(&L.coef32, sizeof(float) * nc));
+                    JFEM_HIP(hipMemcpy(L.coef, coef[l].data(), sizeof(double) * nc, hipMemcpyHostToDevice));
+                    JFEM_HIP(hipMemcpy(L.coef32, c32.data(), sizeof(float) * nc, hipMemcpyHostToDevice));
+                }
+            }
         }
         // AMGFromMatrix: coarse systems from the finest one, top down
         if (opts->galerkin)
@@ -1145,7 +1189,7 @@ static void learned_smooth(jfem_mg* G, MgLevel& L, const T* b, T* x, T* u, bool 
             continue;
         }
         k_unpack_out<T><<<gr, 256, 0, st>>>(S.perm, n, S.nn->out, L.bnd, u);
-        launch_stream<T, T>(L.A, vals_of<T>(L), u, EpiStore<T>{w}, nullptr);
+        level_apply<T>(G, L, u, EpiStore<T>{w});
         const int blocks = grid_for(n, 256, RED_BLOCKS);
         k_dot2<T><<<blocks, 256, 0, st>>>(n, r, w, u, S.mr == 2 ? 1 : 0, c->red_partials);
         reduce_partials_dev(c, 2, blocks, S.rr + 2);
@@ -1362,14 +1406,14 @@ static int train_mg(jfem_mg* G, const jfem_train_opts* o, double* loss_host)
                 k_rand_rhs<<<gr, 256, 0, st>>>(n, o->seed * 0x9E3779B97F4A7C15ull + 0x51ull * (step + 1), F.bnd, S.gd);
                 spmv(F.A, S.gd, r);
             } else {
-                launch_stream<double, double>(F.A, F.A->vals, S.xs, EpiResidual<double>{S.bs, r}, nullptr);
+                level_apply<double>(G, F, S.xs, EpiResidual<double>{S.bs, r});
             }
             sumsq_dev<double>(c, n, r, F.dinv, S.rr);
             k_gather_in<double><<<gr, 256, 0, st>>>(S.perm, n, r, F.dinv, S.rr, S.stat, S.nn->cin,
                                                     S.nn->in);
             nn_forward(S.nn, S.lat);
             k_train_unpack<<<gr, 256, 0, st>>>(S.perm, n, S.nn->out, r, S.rr, F.bnd, S.u, S.rh);
-            launch_stream<double, double>(F.A, F.A->vals, S.u, EpiResidual<double>{S.rh, S.z}, nullptr);   // z = rh - A u
+            level_apply<double>(G, F, S.u, EpiResidual<double>{S.rh, S.z});   // z = rh - A u
             const int blocks = grid_for(n, 256, RED_BLOCKS);
             if (twogrid) {
                 // two-grid loss: energy of e2 = T (e - u), T = I - P A_c^-1 P^T A the coarse-grid
@@ -1377,7 +1421,7 @@ static int train_mg(jfem_mg* G, const jfem_train_opts* o, double* loss_host)
                 // projector the gradient is -2 A e2 = -2 y, y = z - A w, w = P A_c^-1 P^T z
                 k_scaled_error<<<gr, 256, 0, st>>>(n, S.xt, captured ? S.capx : S.xs, S.rr, S.eh);
                 coarse_correction(G, S.z, S.w);
-                launch_stream<double, double>(F.A, F.A->vals, S.w, EpiResidual<double>{S.z, S.y}, nullptr);
+                level_apply<double>(G, F, S.w, EpiResidual<double>{S.z, S.y});
                 k_twogrid_dots<<<blocks, 256, 0, st>>>(n, S.eh, S.u, S.w, S.y, S.rh, c->red_partials);
                 reduce_partials_dev(c, 2, blocks, S.loss2 + 2 * step);
                 k_train_gout<<<gr, 256, 0, st>>>(S.perm, n, S.y, F.bnd, S.nn->gout);
@@ -1482,6 +1526,28 @@ int jfem_mg_level_matrix(jfem_mg* mg, int level, jfem_mat** out)
     JFEM_CHECK(mg && out, JFEM_ERR_ARG, "null argument");
     JFEM_CHECK(level >= 0 && level < (int)mg->lev.size(), JFEM_ERR_ARG, "no such level");
     *out = mg->lev[level].A;
+    API_END
+}
+
+int jfem_mg_level_apply(jfem_mg* mg, int level, const jfem_vec* x, jfem_vec* y)
+{
+    API_BEGIN
+    JFEM_CHECK(mg && x && y, JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(level >= 0 && level < (int)mg->lev.size(), JFEM_ERR_ARG, "no such level");
+    const MgLevel& L = mg->lev[level];
+    JFEM_CHECK(x->n == L.A->rows && y->n == L.A->rows, JFEM_ERR_ARG, "Incompatible sizes");
+    JFEM_CHECK(x->d != y->d || L.A->rows == 0, JFEM_ERR_ARG, "output aliases the input");
+    level_apply<double>(mg, L, x->d, EpiStore<double>{y->d});
+    JFEM_HIP(hipStreamSynchronize(mg->ctx->stream));
+    API_END
+}
+
+int jfem_mg_level_matrix_free(jfem_mg* mg, int level, int* out)
+{
+    API_BEGIN
+    JFEM_CHECK(mg && out, JFEM_ERR_ARG, "null argument");
+    JFEM_CHECK(level >= 0 && level < (int)mg->lev.size(), JFEM_ERR_ARG, "no such level");
+    *out = mg->lev[level].mf ? 1 : 0;
     API_END
 }
 

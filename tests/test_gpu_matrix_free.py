@@ -1,0 +1,108 @@
+"""The matrix-free level operator of the multigrid hierarchy (jfem_mg_opts.matrix_free; sum
+factorisation of the cell element matrices, no matrix read) against the assembled, Dirichlet-
+eliminated CSR level matrices of the same hierarchy (getSystem(level), which the parity tests
+show bit-exact with the oracle's assembly), and the V-cycle built on it against the CSR one and
+the oracle.  Grids with cell counts that are not multiples of the tile edge exercise partial
+tiles on every axis."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+
+
+def _hier(jf, base, k, nlev, kinds_w, coeff_seed=None, matrix_free=True, precision=0, smoother=None,
+          galerkin=False):
+    d = len(base)
+    spaces = []
+    for l in range(nlev):
+        s = jf.ContinuousTPFESpace([0.0] * d, [1.0, 0.75, 1.5][:d], [c << l for c in base])
+        s.assembleCells()
+        s.assembleFunctions(k)
+        spaces.append(s)
+    ints = [jf.TPCellIntegral(w, name) for name, w in kinds_w]
+    coeff = None
+    if coeff_seed is not None:
+        ncell = int(np.prod([c << (nlev - 1) for c in base]))
+        coeff = np.exp(np.random.default_rng(coeff_seed).standard_normal(ncell))
+    sm = smoother or jf.ChebyshevSmoother(4, 2)
+    return jf.MGPreconditioner(spaces, ints, sm, cycles=1, fmg=False, precision=precision, cell_coeff=coeff,
+                               matrix_free=matrix_free, galerkin=galerkin), spaces
+
+
+def _csr(A):
+    rp, cols, vals = A.to_csr()
+    n = len(rp) - 1
+    return sp.csr_matrix((vals, cols, rp), shape=(n, n))
+
+
+# (coarsest cells, degree, integrals, coefficient seed, levels); the coarsest level stays below
+# 800 DOFs (its dense inverse is formed on the host in O(n^3))
+CASES = [
+    ([5, 3, 4], 2, [("GradGrad", 1.0)], None, 2),            # 3D Q2, partial tiles
+    ([3, 5, 3], 2, [("GradGrad", 1.0)], 3, 3),               # 3D Q2 12 x 20 x 12, per-cell coefficient
+    ([3, 5, 4], 2, [("GradGrad", 0.7), ("ValueValue", 2.5)], 4, 2),
+    ([7, 5, 9], 1, [("GradGrad", 1.0), ("ValueValue", 5.0)], 5, 2),   # 3D Q1
+    ([9, 14], 2, [("H1", 1.5)], 6, 2),                       # 2D Q2, H1
+    ([33, 21], 1, [("GradGrad", 1.0), ("ValueValue", 5.0)], None, 2),  # 2D Q1
+    ([20, 18], 1, [("GradGrad", 1.0)], 7, 3),                # 2D Q1 80 x 72
+]
+
+
+@pytest.mark.parametrize("base,k,kinds_w,seed,nlev", CASES)
+def test_matrix_free_operator_matches_level_matrices(jf, base, k, kinds_w, seed, nlev):
+    """y = A_l x on every level, matrix-free against the CSR level matrix: within 1e-13 of
+    (|A_l| |x|) row by row (the element matrices are formed in another order than the
+    quadrature sums of the assembly), boundary rows exactly x"""
+    G, spaces = _hier(jf, base, k, nlev, kinds_w, seed)
+    rng = np.random.default_rng(11)
+    for l in range(nlev):
+        assert G.isMatrixFree(l)
+        A = _csr(G.getSystem(l))
+        x = rng.standard_normal(A.shape[0])
+        y = G.applySystem(l, jf.DenseVector(x)).numpy()
+        bound = abs(A) @ np.abs(x)
+        assert np.all(np.abs(y - A @ x) <= 1e-13 * bound + 1e-300), (l, np.max(np.abs(y - A @ x) / bound))
+        bnd = spaces[l].boundary_nodes()
+        assert np.array_equal(y[bnd], x[bnd])
+
+
+def test_matrix_free_fp32_and_galerkin_levels(jf):
+    """FP32 hierarchy: the matrix-free V-cycle agrees with the CSR FP32 one to FP32 rounding;
+    a Galerkin hierarchy keeps CSR below the finest level (P^T A P is no cell assembly)"""
+    base, k = [3, 3, 3], 2
+    G1, spaces = _hier(jf, base, k, 3, [("GradGrad", 1.0)], 9, True, 1)
+    G0, _ = _hier(jf, base, k, 3, [("GradGrad", 1.0)], 9, False, 1)
+    n = spaces[-1].ndofs()
+    rng = np.random.default_rng(2)
+    b = rng.standard_normal(n)
+    b[spaces[-1].boundary_nodes()] = 0
+    x1 = G1.mvMul(jf.DenseVector(b)).numpy()
+    x0 = G0.mvMul(jf.DenseVector(b)).numpy()
+    assert np.linalg.norm(x1 - x0) <= 2e-5 * np.linalg.norm(x0)
+    Gg, _ = _hier(jf, base, k, 3, [("GradGrad", 1.0)], 9, True, 0, galerkin=True)
+    assert [Gg.isMatrixFree(l) for l in range(3)] == [False, False, True]
+    assert not any(G0.isMatrixFree(l) for l in range(3))
+
+
+@pytest.mark.parametrize("which", ["jacobi", "chebyshev", "richardson"])
+def test_matrix_free_vcycle_matches_oracle(jf, oracle, which):
+    """mgStepV with matrix-free smoothing and residuals against the oracle's V-cycle
+    (MGPreconditionerInterface.java:125-198), 3D Q2 and 2D Q1, as the CSR V-cycle is tested"""
+    sm = {"jacobi": jf.JacobiSmoother(2, 0.6), "chebyshev": jf.ChebyshevSmoother(4, 2),
+          "richardson": jf.RichardsonSmoother(0.1, 4)}[which]
+    for base, k in (([2, 3, 2], 2), ([5, 3], 1)):
+        d = len(base)
+        G, spaces = _hier(jf, base, k, 3, [("GradGrad", 1.0)], None, True, 0, sm)
+        lmax = G.lmax()
+        O = oracle.MG(base, k, 3, [0.0] * d, [1.0, 0.75, 1.5][:d], [0], [1.0], smoother=sm.kind, runs=sm.runs,
+                      omega=getattr(sm, "omega", 1.0), degree=getattr(sm, "degree", 0),
+                      lmin=lmax / sm.ratio if which == "chebyshev" else None,
+                      lmax=lmax if which == "chebyshev" else None, cycles=1, fmg=False)
+        n = O.level_n(2)
+        rng = np.random.default_rng(5)
+        b = rng.standard_normal(n)
+        x = 0.1 * rng.standard_normal(n)
+        got = G.vCycle(jf.DenseVector(x), jf.DenseVector(b)).numpy()
+        exp = O.vcycle(x, b)
+        assert np.linalg.norm(got - exp) <= 1e-11 * np.linalg.norm(exp), (base, which)

@@ -286,7 +286,7 @@ def test_small_known_answers(jf, mat):
 
 
 # ---------------------------------------------------------------- multigrid
-def _mg_pair(jf, oracle, base, k, nlev, smoother, cycles, fmg, precision=0):
+def _mg_pair(jf, oracle, base, k, nlev, smoother, cycles, fmg, precision=0, matrix_free=True):
     d = len(base)
     start, end = [0.0] * d, [1.0] * d
     spaces = []
@@ -294,7 +294,8 @@ def _mg_pair(jf, oracle, base, k, nlev, smoother, cycles, fmg, precision=0):
         s, _, _ = _space(jf, [c << l for c in base], k, start, end)
         spaces.append(s)
     ints = [jf.TPCellIntegral("GradGrad")]
-    G = jf.MGPreconditioner(spaces, ints, smoother, cycles=cycles, fmg=fmg, precision=precision)
+    G = jf.MGPreconditioner(spaces, ints, smoother, cycles=cycles, fmg=fmg, precision=precision,
+                            matrix_free=matrix_free)
     kind = smoother.kind
     lmax = G.lmax()
     degree = getattr(smoother, "degree", 0)
@@ -305,11 +306,13 @@ def _mg_pair(jf, oracle, base, k, nlev, smoother, cycles, fmg, precision=0):
     return G, O, spaces
 
 
+@pytest.mark.parametrize("matrix_free", [True, False])
 @pytest.mark.parametrize("which", ["jacobi", "chebyshev", "richardson"])
-def test_mg_level_matrices_and_vcycle(jf, oracle, which):
+def test_mg_level_matrices_and_vcycle(jf, oracle, which, matrix_free):
     sm = {"jacobi": jf.JacobiSmoother(2, 0.6), "chebyshev": jf.ChebyshevSmoother(4, 2),
           "richardson": jf.RichardsonSmoother(0.1, 4)}[which]
-    G, O, spaces = _mg_pair(jf, oracle, [2, 3], 2, 3, sm, cycles=1, fmg=False)
+    G, O, spaces = _mg_pair(jf, oracle, [2, 3], 2, 3, sm, cycles=1, fmg=False, matrix_free=matrix_free)
+    assert all(G.isMatrixFree(l) == matrix_free for l in range(3))
     for l in range(3):
         rp, cols, vals = G.getSystem(l).to_csr()
         ref = O.level_matrix(l)
