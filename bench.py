@@ -554,6 +554,9 @@ def main():
                     help="C5: network input channels (2: residual and coefficient; 4: plus node-parity channels)")
     ap.add_argument("--mass-its", type=int, default=3,
                     help="C5: Chebyshev steps on the pressure mass matrix in the Schur complement (0 = lumped)")
+    ap.add_argument("--learned-sides", type=int, default=3, choices=[1, 2, 3],
+                    help="C2: V-cycle smoothing calls that use the network (1 pre, 2 post, 3 both); the "
+                         "other side keeps Chebyshev")
     ap.add_argument("--spmv-only", action="store_true",
                     help="assemble the finest matrix and time only the SpMV and the copy calibration (for PMC runs)")
     args = ap.parse_args()
@@ -698,6 +701,23 @@ def main():
             _, G32, _ = build_problem(jf, base, nlev, 2, coarse_sm, jf.FP32, 2, 16)
             A32 = G32.getFinestSystem()
             G32.set_learned(net, np.zeros(ncell), runs=1, fallback_ratio=0.5)
+            # the network on one side of the V-cycle (1 = pre-smoothing, 2 = post-smoothing) or both
+            # (3); the other side keeps the hierarchy's Chebyshev smoother
+            for sides in (1, 2):
+                if sides == args.learned_sides:
+                    continue
+                G32.set_learned_sides(sides)
+                ts_ = []
+                for s in range(2):
+                    G32.set_learned_active(True)
+                    its_ = jf.IterativeSolver()
+                    its_.solveFCG(A32, G32, b, tol)
+                    ts_.append(its_.info.seconds)
+                name = "pre" if sides == 1 else "post"
+                extra[f"learned_{name}_only_fcg_s"] = float(np.median(ts_))
+                extra[f"learned_{name}_only_fcg_iterations"] = its_.iterations
+                log(f"learned on the {name}-smoothing side only: {its_.iterations} its, {np.median(ts_):.3f}s")
+            G32.set_learned_sides(args.learned_sides)
 
             def solve_learned():
                 G32.set_learned_active(True)   # re-arms the smoother and clears the fallback flag

@@ -321,3 +321,45 @@ def test_stokes_reference_solution_convergence(jf):
     # Q2 velocity: the pointwise error falls like h^3
     rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
     assert rates[-1] > 2.5, rates
+
+
+def test_stokes_learned_velocity_block(jf):
+    """The Taylor-Hood preconditioner with the learned smoother on the finest level of the FP32
+    velocity multigrid (network 2 + dim input channels, trained with the two-grid loss on the
+    scalar velocity hierarchy of the same grid): FGMRES meets the tolerance on the true residual,
+    the velocity matches scipy's direct solve and the pressure does modulo a constant, and the
+    network never falls back to Chebyshev.  The iteration counts of the learned and the
+    Chebyshev velocity blocks are printed."""
+    cells, nu = (32, 32), 1.0
+    print("stokes learned velocity block", flush=True)
+    src = jf.VectorTrigSource([jf.TrigSource([[1, 2]], [3.0]), jf.TrigSource([[2, 1]], [-2.0], [[1, 0]])])
+    s, A, rhs = _stokes_system(jf, cells, nu, src, _lid)
+    ncell = cells[0] * cells[1]
+    mgt, _ = _velocity_mg(jf, cells, nu, 0)
+    net = jf.SmootherNet(2, 4, seed=7)
+    mgt.set_learned(net, np.zeros(ncell), runs=1, fallback_ratio=0.5)
+    h = mgt.train(rhs_per_field=120, cycles_per_rhs=8, lr=1e-4, seed=1, mode=5)
+    assert np.all(np.isfinite(h)) and np.mean(h[-16:]) < 0.5 * np.mean(h[:8]), (np.mean(h[:8]), np.mean(h[-16:]))
+    mg, _ = _velocity_mg(jf, cells, nu, 1)
+    mg.set_learned(net, np.zeros(ncell), runs=1, fallback_ratio=0.5)
+    rp, cols, vals = A.to_csr()
+    M = sp.csr_matrix((vals, cols, rp), shape=(s.ndofs(), s.ndofs()))
+    b = rhs.numpy()
+    xd = spla.spsolve(M.tocsc(), b)
+    nv2 = s.getVelocitySize()
+    its = {}
+    for learned in (True, False):
+        mg.set_learned_active(learned)
+        prec = jf.StokesBlockPreconditioner(s, mg, nu, mass_its=3)
+        x = jf.DenseVector(s.ndofs())
+        info, _ = jf.solveStokesFGMRES(A, prec, rhs, x, 1e-12, restart=50, maxit=400)
+        xs = x.numpy()
+        assert np.linalg.norm(M @ xs - b) <= 1.01e-12 * np.linalg.norm(b), learned
+        assert np.linalg.norm(xs[:nv2] - xd[:nv2]) <= 1e-8 * np.linalg.norm(xd[:nv2]), learned
+        dp = xs[nv2 + 1:] - xd[nv2 + 1:]
+        assert np.linalg.norm(dp - dp.mean()) <= 1e-8 * np.linalg.norm(xd[nv2:]), learned
+        assert info.iterations < 120, (learned, info.iterations)
+        its[learned] = info.iterations
+        if learned:
+            assert not mg.learned_status()[1], "the learned smoother fell back"
+    print(f"FGMRES iterations to 1e-12: learned velocity block {its[True]}, Chebyshev {its[False]}")
