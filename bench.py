@@ -221,12 +221,13 @@ def bench_c4(jf, args):
         field = gaussian_random_field(4000 + f, (cells, cells), 0.1, 1e3)
         G, spaces = c4_hierarchy(jf, field, jf.FP64, galerkin=args.galerkin)
         G.set_learned(net, field, runs=1, fallback_ratio=0.5)
-        h = G.train(rhs_per_field=max(1, args.pairs // 8), cycles_per_rhs=8, lr=3e-4, seed=f + 1, mode=5)
+        h = G.train(rhs_per_field=max(1, args.pairs // 10), cycles_per_rhs=10, lr=3e-4, seed=f + 1, mode=5)
         losses.append(float(np.mean(h[-8:])))
         log(f"train field {f}: two-grid energy ratio {np.mean(h[:8]):.4f} -> {np.mean(h[-8:]):.4f}")
         del G
     t_train = time.perf_counter() - t0
-    res = {"pcg64": [], "fcg32_learned": [], "vcyc_cheb": [], "vcyc_learned": [], "err": [], "fallbacks": 0}
+    res = {"pcg64": [], "fcg32_learned": [], "fcg32_cheb": [], "vcyc_cheb": [], "vcyc_learned": [], "err": [],
+           "fallbacks": 0}
     kw, amp = sinusoid_terms(4, 8, 2)
     for f in range(args.eval_fields):
         field = gaussian_random_field(5000 + f, (cells, cells), 0.1, 1e3)
@@ -255,6 +256,10 @@ def bench_c4(jf, args):
         x32 = it2.solveFCG(G32.getFinestSystem(), G32, b, tol)
         res["fcg32_learned"].append((it2.info.seconds, it2.iterations))
         res["fallbacks"] += int(G32.learned_status()[1])
+        G32.set_learned_active(False)      # the same FP32 hierarchy with Chebyshev on every level
+        it3 = jf.IterativeSolver()
+        it3.solveFCG(G32.getFinestSystem(), G32, b, tol)
+        res["fcg32_cheb"].append((it3.info.seconds, it3.iterations))
         res["err"].append(float(np.linalg.norm(x32.numpy() - x64.numpy()) / np.linalg.norm(x64.numpy())))
         log(f"eval field {f}: pcg64 {res['pcg64'][-1]}, fcg32+learned {res['fcg32_learned'][-1]}, "
             f"V cheb {res['vcyc_cheb'][-1]}, V learned {res['vcyc_learned'][-1]}")
@@ -268,10 +273,12 @@ def bench_c4(jf, args):
         "dtype": "f64 outer / f32 multigrid / bf16 network", "data": "synthetic",
         "config": {"workload": "2D -div(a grad u)=f, Q1 1024^2 cells, a=exp(GRF) contrast 1e3"
                                + ("; Galerkin coarse systems" if args.galerkin else ""),
-                   "train_fields": args.train_fields, "pairs_per_field": max(1, args.pairs // 8) * 8,
+                   "train_fields": args.train_fields, "pairs_per_field": max(1, args.pairs // 10) * 10,
                    "eval_fields": args.eval_fields},
         "fp64_mgpcg_s": med("pcg64", 0), "fp64_mgpcg_iterations": med("pcg64", 1),
         "learned_fcg32_iterations": med("fcg32_learned", 1),
+        "fp32_cheb_fcg_s": med("fcg32_cheb", 0), "fp32_cheb_fcg_iterations": med("fcg32_cheb", 1),
+        "learned_vs_best_plain_fp32": med("fcg32_cheb", 0) / value,
         "speedup_vs_fp64_mgpcg": med("pcg64", 0) / value,
         "vcycles_chebyshev": med("vcyc_cheb", 1), "vcycles_learned": med("vcyc_learned", 1),
         "vcycle_solve_s_chebyshev": med("vcyc_cheb", 0), "vcycle_solve_s_learned": med("vcyc_learned", 0),
@@ -546,9 +553,9 @@ def main():
                          "log-normal coefficient (contrast 1e4), FP32 MG in FP64 FCG vs FP64 MG-PCG; c4: 2D Q1 "
                          "1024^2 log-normal coefficient, learned smoother vs Chebyshev; c5: 2D Taylor-Hood Stokes "
                          "256^2, block-preconditioned FGMRES")
-    ap.add_argument("--train-fields", type=int, default=16)
-    ap.add_argument("--pairs", type=int, default=64, help="(residual, error) pairs per training field")
-    ap.add_argument("--eval-fields", type=int, default=8)
+    ap.add_argument("--train-fields", type=int, default=200, help="C4: training coefficient fields (seeds 4000+)")
+    ap.add_argument("--pairs", type=int, default=100, help="C4: (residual, error) pairs per training field")
+    ap.add_argument("--eval-fields", type=int, default=50, help="C4: held-out evaluation fields (seeds 5000+)")
     ap.add_argument("--galerkin", action="store_true", help="C4: Galerkin coarse systems on every hierarchy")
     ap.add_argument("--c5-cin", type=int, default=4, choices=[2, 4],
                     help="C5: network input channels (2: residual and coefficient; 4: plus node-parity channels)")

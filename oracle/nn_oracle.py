@@ -94,37 +94,54 @@ def conv_wgrad(g, x, d):
     return np.stack([gf.T @ _shift(x, off).reshape(-1, x.shape[-1]) for off in offs])
 
 
-def forward(params, shape, inp, cin):
+def bf16(a):
+    """round to bfloat16 (8 significant bits, round to nearest, ties to even; unit roundoff 2^-8),
+    returned as float64: the error model of the device's BF16 storage"""
+    f = np.ascontiguousarray(a, np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64).reshape(np.shape(a))
+
+
+def forward(params, shape, inp, cin, emulate_bf16=False):
+    """FP64 forward; emulate_bf16 rounds to BF16 where the device stores BF16: the 32->32 and last
+    layer weights and the activations after each ReLU (the first layer runs in split BF16, close
+    to FP32, and is kept exact)"""
     d = len(shape)
     W0, b0, W1, b1, W2, b2, W3, b3 = split(params, d, cin)
+    r = bf16 if emulate_bf16 else (lambda v: v)
+    W1, W2, W3 = r(W1), r(W2), r(W3)
     x = np.asarray(inp, np.float64).reshape(tuple(reversed(shape)) + (cin,))
     z0 = conv(x, W0, b0, d)
-    a0 = np.maximum(z0, 0)
+    a0 = r(np.maximum(z0, 0))
     z1 = conv(a0, W1, b1, d)
-    a1 = np.maximum(z1, 0)
+    a1 = r(np.maximum(z1, 0))
     z2 = conv(a1, W2, b2, d)
-    a2 = np.maximum(z2, 0)
+    a2 = r(np.maximum(z2, 0))
     out = conv(a2, W3, b3, d)[..., 0]
     cache = dict(x=x, a0=a0, a1=a1, a2=a2, W=(W0, W1, W2, W3), d=d)
     return out.reshape(-1), cache
 
 
-def gradient(params, shape, inp, gout, cin):
-    """gradient of sum(gout * forward(inp)) w.r.t. the parameters (flat, C-ABI order)"""
-    out, c = forward(params, shape, inp, cin)
+def gradient(params, shape, inp, gout, cin, emulate_bf16=False):
+    """gradient of sum(gout * forward(inp)) w.r.t. the parameters (flat, C-ABI order);
+    emulate_bf16 also rounds the output gradient, the back-propagated gradients of each layer
+    and the input of the first weight gradient to BF16, as the device stores them"""
+    out, c = forward(params, shape, inp, cin, emulate_bf16)
+    r = bf16 if emulate_bf16 else (lambda v: v)
     d = c["d"]
     W0, W1, W2, W3 = c["W"]
     g = np.asarray(gout, np.float64).reshape(c["a2"].shape[:-1] + (1,))
-    dW3 = conv_wgrad(g, c["a2"], d)
+    dW3 = conv_wgrad(r(g), c["a2"], d)
     db3 = g.sum(axis=tuple(range(g.ndim - 1)))
-    g2 = conv_transpose(g, W3, d) * (c["a2"] > 0)
+    g2 = r(conv_transpose(g, W3, d) * (c["a2"] > 0))
     dW2 = conv_wgrad(g2, c["a1"], d)
     db2 = g2.sum(axis=tuple(range(g2.ndim - 1)))
-    g1 = conv_transpose(g2, W2, d) * (c["a1"] > 0)
+    g1 = r(conv_transpose(g2, W2, d) * (c["a1"] > 0))
     dW1 = conv_wgrad(g1, c["a0"], d)
     db1 = g1.sum(axis=tuple(range(g1.ndim - 1)))
-    g0 = conv_transpose(g1, W1, d) * (c["a0"] > 0)
-    dW0 = conv_wgrad(g0, c["x"], d)
+    g0 = r(conv_transpose(g1, W1, d) * (c["a0"] > 0))
+    dW0 = conv_wgrad(g0, r(c["x"]), d)
     db0 = g0.sum(axis=tuple(range(g0.ndim - 1)))
     return np.concatenate([a.reshape(-1) for a in (dW0, db0, dW1, db1, dW2, db2, dW3, db3)])
 

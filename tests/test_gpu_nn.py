@@ -1,7 +1,11 @@
 """The learned-smoother network on the GPU (BF16 MFMA, FP32 accumulation)
-against the FP64 oracle: forward and parameter gradients.  Tolerances reflect
-BF16 weights and activations (8-bit mantissa): relative L2 error <= 2e-2 for
-the output and <= 4e-2 for the gradient."""
+against the FP64 oracle: forward and parameter gradients.  The tolerances come
+from an error model: the same FP64 oracle with BF16 rounding (unit roundoff
+2^-8) where the device stores BF16 (weights of the 32->32 and last layers,
+activations, back-propagated gradients).  Device and model errors are sums of
+many independent roundings of the same size, so the device error of every
+output and gradient block stays within 3x the model's (plus 1e-3); a wrong
+tap, channel or term gives errors of order one."""
 import os
 
 import numpy as np
@@ -26,23 +30,20 @@ def test_forward_and_gradient_vs_oracle(jf, d, shape, cin):
     inp = rng.standard_normal((np_, cin))
     out = net.forward(shape, inp)
     ref, _ = NO.forward(p, shape, inp, cin)
-    assert rel(out, ref) <= 2e-2, rel(out, ref)
+    model, _ = NO.forward(p, shape, inp, cin, emulate_bf16=True)
+    assert rel(out, ref) <= 3 * rel(model, ref) + 1e-3, (rel(out, ref), rel(model, ref))
     gout = rng.standard_normal(np_)
     g = net.gradient(shape, inp, gout)
     gr = NO.gradient(p, shape, inp, gout, cin)
-    assert rel(g, gr) <= 4e-2, rel(g, gr)
-    # per-layer check so that a wrong layer cannot hide behind the others (a
-    # transposed tap, a wrong channel or a dropped term gives errors of order
-    # one).  The device rounds activations and back-propagated gradients to
-    # BF16 (8 significant bits) at every layer, so a weight gradient, a sum of
-    # such products over all pixels, agrees to a few percent (8e-2 bound);
-    # bias gradients cancel more heavily (1.2e-1)
+    gm = NO.gradient(p, shape, inp, gout, cin, emulate_bf16=True)
+    assert rel(g, gr) <= 3 * rel(gm, gr) + 1e-3, (rel(g, gr), rel(gm, gr))
+    # per block (each layer's weights and bias), so that a wrong layer cannot hide behind the others
     sizes = [3 ** d * 32 * cin, 32, 3 ** d * 1024, 32, 3 ** d * 1024, 32, 3 ** d * 32, 1]
     o = 0
     for i, s in enumerate(sizes):
         if np.linalg.norm(gr[o:o + s]) > 1e-6:
-            bound = 1.2e-1 if i % 2 else 8e-2
-            assert rel(g[o:o + s], gr[o:o + s]) <= bound, (o, s, rel(g[o:o + s], gr[o:o + s]))
+            e, em = rel(g[o:o + s], gr[o:o + s]), rel(gm[o:o + s], gr[o:o + s])
+            assert e <= 3 * em + 1e-3, (i, e, em)
         o += s
 
 
@@ -52,8 +53,11 @@ def test_torch_fixture_on_gpu(jf, d):
     shape, cin = tuple(int(s) for s in f["shape"]), int(f["cin"])
     net = jf.SmootherNet(d, cin)
     net.set_params(f["params"])
-    assert rel(net.forward(shape, f["inp"]), f["out"]) <= 2e-2
-    assert rel(net.gradient(shape, f["inp"], f["gout"]), f["grad"]) <= 4e-2
+    p = np.asarray(f["params"], np.float64)
+    model, _ = NO.forward(p, shape, f["inp"], cin, emulate_bf16=True)
+    gm = NO.gradient(p, shape, f["inp"], f["gout"], cin, emulate_bf16=True)
+    assert rel(net.forward(shape, f["inp"]), f["out"]) <= 3 * rel(model, f["out"]) + 1e-3
+    assert rel(net.gradient(shape, f["inp"], f["gout"]), f["grad"]) <= 3 * rel(gm, f["grad"]) + 1e-3
 
 
 def test_checkpoint_roundtrip_and_adam(jf, tmp_path):

@@ -48,3 +48,35 @@ def test_tap_order_matters():
     p[:T * 32 * cin] = W0[perm].reshape(-1)
     out, _ = NO.forward(p, shape, f["inp"], cin)
     assert np.max(np.abs(out - f["out"])) > 1e-3
+
+
+def test_bf16_rounding_model():
+    """the BF16 rounding of the error model: 8 significant bits, nearest, ties to even, exact
+    on representable values, relative error at most the unit roundoff 2^-8"""
+    from oracle.nn_oracle import bf16
+    assert bf16(1.0 + 2 ** -8) == 1.0                  # halfway, even neighbour below
+    assert bf16(1.0 + 3 * 2 ** -8) == 1.0 + 2 ** -6    # halfway, even neighbour above
+    assert bf16(1.0 + 2 ** -8 + 2 ** -12) == 1.0 + 2 ** -7
+    assert bf16(-3.0) == -3.0 and bf16(0.0) == 0.0
+    x = np.random.default_rng(1).standard_normal(10000) * 100
+    assert np.max(np.abs(bf16(x) - x) / np.abs(x)) <= 2 ** -8
+    assert np.array_equal(bf16(bf16(x)), bf16(x))
+
+
+def test_bf16_emulation_changes_results_at_bf16_scale():
+    """the emulated forward and gradient differ from the exact FP64 ones at the BF16 scale
+    (up to 2^-8 relative per rounding, compounded over the layers): far above FP64 rounding,
+    far below the order-one errors of a wrong tap, channel or term"""
+    from oracle import nn_oracle as NO
+    d, shape, cin = 2, (9, 7), 2
+    p = NO.init_params(d, cin, 3)
+    rng = np.random.default_rng(4)
+    inp = rng.standard_normal((63, cin))
+    gout = rng.standard_normal(63)
+    o, _ = NO.forward(p, shape, inp, cin)
+    oe, _ = NO.forward(p, shape, inp, cin, emulate_bf16=True)
+    g = NO.gradient(p, shape, inp, gout, cin)
+    ge = NO.gradient(p, shape, inp, gout, cin, emulate_bf16=True)
+    r1 = np.linalg.norm(oe - o) / np.linalg.norm(o)
+    r2 = np.linalg.norm(ge - g) / np.linalg.norm(g)
+    assert 1e-5 < r1 < 0.2 and 1e-5 < r2 < 0.2, (r1, r2)
