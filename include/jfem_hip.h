@@ -284,10 +284,11 @@ typedef struct {
                         * finest of a Galerkin hierarchy) apply the level operator matrix-free: the
                         * cell element matrices of GRAD_GRAD / VALUE_VALUE / H1 by 1D sum
                         * factorisation, summed per node, on 2D / 3D grids of degree 1 or 2 (others
-                        * keep the CSR product).  The level matrices are still assembled
-                        * (jfem_mg_level_matrix); the products agree with them to rounding.  New;
-                        * the reference's smoothers call getSystem(level).mvMul
-                        * (M/multigrid/MGPreconditionerInterface.java:125-198). */
+                        * keep the CSR product); 2: the same on 3D grids only (2D rows hold 9 or 25
+                        * entries, where the CSR product is as cheap); 0: CSR everywhere.  The level
+                        * matrices are still assembled (jfem_mg_level_matrix); the products agree
+                        * with them to rounding.  New; the reference's smoothers call
+                        * getSystem(level).mvMul (M/multigrid/MGPreconditionerInterface.java:125-198). */
 } jfem_mg_opts;
 
 /* levels: nlev spaces, coarsest first, each with twice the cells of the

@@ -520,7 +520,7 @@ class MGPreconditioner:
 
     def __init__(self, spaces: Sequence[ContinuousTPFESpace], cellIntegrals: Sequence[TPCellIntegral], smoother,
                  cycles: int = 3, fmg=True, precision: int = FP64, cell_coeff=None, galerkin: bool = False,
-                 raw_transfers: bool = False, matrix_free: bool = True):
+                 raw_transfers: bool = False, matrix_free="auto"):
         self.ctx = spaces[0].ctx
         self.spaces = list(spaces)
         opts = N.MgOpts()
@@ -536,7 +536,9 @@ class MGPreconditioner:
         opts.raw_transfers = 1 if raw_transfers else 0
         opts.precision = precision
         opts.galerkin = 1 if galerkin else 0   # coarse systems P^T A P (AMGFromMatrix) instead of re-assembly
-        opts.matrix_free = 1 if matrix_free else 0   # level operators by sum factorisation in the cycles
+        # level operators by sum factorisation in the cycles: True everywhere it applies, "auto" on
+        # 3D grids only, False never
+        opts.matrix_free = 2 if matrix_free == "auto" else (1 if matrix_free else 0)
         if any(isinstance(ci.weight, list) for ci in cellIntegrals):
             raise N.UnsupportedOperationException(3, "multigrid hierarchies take scalar-weighted integrals")
         kinds = i32([ci.kind for ci in cellIntegrals])

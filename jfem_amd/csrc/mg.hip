@@ -863,7 +863,10 @@ static jfem_mg* create_mg(jfem_space* const* spaces, int nlev, int nint, const i
             if (l > 0) build_transfer(L, G->lev[l - 1]);
             // matrix-free level operator: re-assembled systems (every level, or the finest of a
             // Galerkin hierarchy) of GRAD_GRAD / VALUE_VALUE / H1 on 2D / 3D Q1 / Q2 grids
-            bool mf_ok = opts->matrix_free && mf_supported(L.grid) && (!opts->galerkin || l == nlev - 1);
+            // matrix_free 2 (auto): 3D grids only, where a CSR row holds 27 (Q1) or 125 (Q2) entries;
+            // on 2D grids (9 or 25 per row) the CSR product measured faster
+            const bool want = opts->matrix_free == 1 || (opts->matrix_free == 2 && L.grid.d == 3);
+            bool mf_ok = want && mf_supported(L.grid) && (!opts->galerkin || l == nlev - 1);
             for (int i = 0; i < nint; i++) mf_ok = mf_ok && kinds[i] >= JFEM_GRAD_GRAD && kinds[i] <= JFEM_H1;
             if (mf_ok) {
                 L.mf = true;

@@ -675,3 +675,17 @@ def test_face_quadrature_refused(oracle):
             O.rhs_sines(cells, k, [0, 0], [1, 1], [[1, 2]], [1.0], rule=rule)
     for rule in (1, 2, 3, 4, 5):
         O.assemble(cells, k, [0, 0], [1, 1], [O.VALUE_VALUE, O.H1], [1.0, 2.0], rule=rule)
+
+
+def test_power_iteration_estimate_converges_to_lambda_max(oracle):
+    """the oracle's power iteration on D^-1 A (the Chebyshev interval's upper end before the
+    safety factor) converges to the largest eigenvalue of D^-1 A: within 1e-6 after 400
+    iterations, and from below-or-near after 30 (the hierarchy's default)"""
+    O = oracle
+    G = O.MG([2, 2], 2, 3, [0, 0], [1, 1], [O.GRAD_GRAD], [1.0], smoother=O.SM_CHEBYSHEV, runs=1, degree=2,
+             lmin=[0.0] * 3, lmax=[0.0] * 3, cycles=1, fmg=False)
+    for l in (1, 2):
+        A = G.level_matrix(l).to_scipy().toarray()
+        lam = np.max(np.abs(np.linalg.eigvals(G.level_dinv(l)[:, None] * A)))
+        assert abs(G.estimate_lmax(l, its=400) - lam) <= 1e-6 * lam
+        assert abs(G.estimate_lmax(l, its=30) - lam) <= 0.05 * lam
