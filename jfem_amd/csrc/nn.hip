@@ -20,6 +20,7 @@ namespace jfem {
 typedef __bf16 bfv8 __attribute__((ext_vector_type(8)));   // the same bits as bf16 for the dot-product builtins
 typedef __bf16 bfv2 __attribute__((ext_vector_type(2)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // lattice coordinates of pixel p (32-bit arithmetic: np < 2^31 is checked in
 // make_lat); unused axes have extent 1 and coordinate 0
@@ -315,82 +316,225 @@ __device__ inline void store_fragment(const f32x16& acc, long p, int kh, const f
     }
 }
 
-template <int MODE, int D>
-__global__ __launch_bounds__(MidTile<D>::NW * 64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mid_lds(Lat L, const unsigned short* __restrict__ in,
-                                                 const unsigned short* __restrict__ Wh, const float* __restrict__ bias,
-                                                 const unsigned short* __restrict__ mask, unsigned short* __restrict__ out)
+// 32 -> 32 layer with the weights resident in registers (2D and 3D lattices).
+// v_mfma_f32_16x16x32_bf16: A = 16 output channels (half h) x 32 input
+// channels of one tap, B = 32 input channels x 16 pixels along axis 0.  A
+// workgroup of eight waves (two per SIMD) owns a tile of 16 x 8 x 4 pixels
+// (2D: 16 x 32), 32 rows of 16 pixels; wave w computes output half h = w % 2
+// of the 8 rows of block w / 2 (3D: 4 x 2 rows along axes 1, 2; 2D: 8 rows
+// along axis 1).  The wave reads each halo row of its block once per axis-0
+// shift (3D: 6 x 4 rows x 3 shifts = 72 fragments) and multiplies it into
+// every output row it reaches (up to 9 taps): 216 MFMAs per tile against 8 x
+// 27 fragment reads without that reuse.  The half's 27 A fragments (27 x 16 B
+// per lane) stay in registers for the whole persistent loop, one workgroup per
+// CU, and the fragment reads run one group ahead of the MFMAs.  The halo tiles
+// are double-buffered in LDS and filled by buffer loads into LDS (16 B per
+// lane, no registers in flight): the next tile's loads are issued before the
+// current tile's MFMAs.  Out-of-lattice halo pixels read zeros.  The tiles
+// of XCD x (workgroups b with b % 8 = x: the hardware dispatches workgroups
+// round-robin over the eight XCDs) are a contiguous range, so neighbouring
+// tiles share their halos in one L2.
+template <int D>
+struct Conv32 {
+    static constexpr int NW = 8;
+    static constexpr int TX = 16, TY = D == 3 ? 8 : 32, TZ = D == 3 ? 4 : 1;
+    static constexpr int WY = D == 3 ? 4 : 8, WZ = D == 3 ? 2 : 1;   // a wave's rows along axes 1, 2
+    static constexpr int NR = WY * WZ;            // 8
+    static constexpr int HX = TX + 2, HY = TY + 2, HZ = D == 3 ? TZ + 2 : 1;
+    static constexpr int HP = HX * HY * HZ;       // halo pixels: 1080 (3D), 612 (2D)
+    static constexpr int NG = (HP + 63) / 64;     // 64-slot groups per chunk
+    static constexpr int CS = NG * 64;            // chunk stride in 16-byte slots
+    static constexpr int BUF = 4 * CS;            // slots per halo buffer (3D: 69,632 bytes)
+    static constexpr int TAPS = D == 3 ? 27 : 9;
+    static constexpr int RZ = D == 3 ? WZ + 2 : 1, RY = WY + 2;   // halo rows of a wave's block
+    static constexpr int NB = 3 * RZ * RY;        // fragment reads per tile and wave
+    static constexpr int KPW = (NG + 1) / 2;      // halo loads per wave and tile (at most)
+    static constexpr int FG = D == 3 ? 4 : 6;     // fragments per read group
+};
+
+// buffer_load_dwordx4 ... lds (gfx950): lane l's 16 bytes at byte offset voff of the buffer
+// [base, base + nbytes) land at lds + 16 l (lds is wave-uniform); offsets at or past nbytes
+// read zeros.  The host pass does not know the builtins and never calls this
+__device__ inline void lds_dma16(const void* base, unsigned nbytes, unsigned voff, bf16x8* lds)
 {
-    using T = MidTile<D>;
-    constexpr int TAPS = D == 3 ? 27 : 9, CS = T::template cs<4>(), NR = T::NR;
-    constexpr int WD = 3;                // taps of weight fragments in flight ahead of their use
-    extern __shared__ bf16x8 tile[];     // chunk-major halo tile, 4 chunks of 8 channels
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int j = lane & 31, kh = lane >> 5;
-    // A fragment of tap t and K step kb: W[t][j][16 kb + 8 kh .. + 7] = wl[128 t + 2 kb]
-    const bf16x8* wl = reinterpret_cast<const bf16x8*>(Wh) + j * 4 + kh;
-    const int nx = (L.n[0] + 31) / 32, ny = (L.n[1] + T::TY - 1) / T::TY, nz = (L.n[2] + T::TZ - 1) / T::TZ;
-    const long ntile = (long)nx * ny * nz;
-    for (long tb = blockIdx.x; tb < ntile; tb += gridDim.x) {
-        const int bx = (int)(tb % nx), by = (int)((tb / nx) % ny), bz = (int)(tb / ((long)nx * ny));
-        const int x0 = bx * 32, y0 = by * T::TY, z0 = bz * T::TZ;
-        // the weight loads are the same for every tile; an opaque zero offset keeps
-        // the compiler from hoisting all of them out of the tile loop (which spills)
-        int wz = 0;
-        asm volatile("" : "+s"(wz));
-        const bf16x8* wt = wl + wz;
-        bf16x8 aw[WD + 1][2];
+#if defined(__HIP_DEVICE_COMPILE__)
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)nbytes, 0x00020000);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)lds, 16, (int)voff, 0, 0, 0);
+#endif
+}
+
+// the MFMAs of one tile from halo buffer hb into acc[row] (the wave's half).  The fragments
+// are read in groups of FG, one group ahead of the MFMAs that use them
+template <int MODE, int D>
+__device__ inline void conv32_tile(const bf16x8* hb, const bf16x8 (&aw)[Conv32<D>::TAPS], int wy0, int wz0,
+                                   f32x4 (&acc)[Conv32<D>::NR])
+{
+    using T = Conv32<D>;
+    constexpr int FG = T::FG, NGRP = T::NB / FG;
+    static_assert(T::NB % FG == 0, "whole fragment groups");
+    const int lane = threadIdx.x & 63;
+    const bf16x8* base = hb + (lane >> 4) * T::CS + (wz0 * T::HY + wy0) * T::HX + (lane & 15);
+    // fragment i: axis-0 shift dx = i / (RZ RY), halo row (rz, ry) of the wave's block
+    auto slot = [](int i) {
+        const int dx = i / (T::RZ * T::RY), rz = (i / T::RY) % T::RZ, ry = i % T::RY;
+        return (rz * T::HY + ry) * T::HX + dx;
+    };
 #pragma unroll
-        for (int t = 0; t < WD; t++)
+    for (int u = 0; u < T::NR; u++) acc[u] = f32x4{0.f, 0.f, 0.f, 0.f};
+    bf16x8 bs[2][FG];
 #pragma unroll
-            for (int kb = 0; kb < 2; kb++) aw[t][kb] = wt[128 * t + 2 * kb];
-        stage_halo<D, 4>(L, in, x0, y0, z0, tile);
-        __syncthreads();
-        const bf16x8* bl[NR];
+    for (int f = 0; f < FG; f++) bs[0][f] = base[slot(f)];
 #pragma unroll
-        for (int u = 0; u < NR; u++) bl[u] = tile + kh * CS + T::row_base(wave * NR + u) + j;
-        f32x16 acc[NR];
+    for (int gr = 0; gr < NGRP; gr++) {
+        if (gr + 1 < NGRP)
 #pragma unroll
-        for (int u = 0; u < NR; u++)
+            for (int f = 0; f < FG; f++) bs[(gr + 1) & 1][f] = base[slot((gr + 1) * FG + f)];
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int r = 0; r < 16; r++) acc[u][r] = 0.f;
-        bf16x8 bc[2][NR];
+        for (int f = 0; f < FG; f++) {
+            const int i = gr * FG + f;
+            const int dx = i / (T::RZ * T::RY), rz = (i / T::RY) % T::RZ, ry = i % T::RY;
 #pragma unroll
-        for (int kb = 0; kb < 2; kb++)
+            for (int dz = 0; dz < (D == 3 ? 3 : 1); dz++) {
 #pragma unroll
-            for (int u = 0; u < NR; u++) bc[kb][u] = bl[u][T::template tap<MODE == 1>(0) + 2 * kb * CS];
-#pragma unroll
-        for (int t = 0; t < TAPS; t++) {
-            // keep each tap's loads in its own step (unbounded hoisting would spill)
-            __builtin_amdgcn_sched_barrier(0);
-            if (t + WD < TAPS)
-#pragma unroll
-                for (int kb = 0; kb < 2; kb++) aw[(t + WD) % (WD + 1)][kb] = wt[128 * (t + WD) + 2 * kb];
-            bf16x8 bn[2][NR];
-            if (t + 1 < TAPS)
-#pragma unroll
-                for (int kb = 0; kb < 2; kb++)
-#pragma unroll
-                    for (int u = 0; u < NR; u++) bn[kb][u] = bl[u][T::template tap<MODE == 1>(t + 1) + 2 * kb * CS];
-#pragma unroll
-            for (int kb = 0; kb < 2; kb++)
-#pragma unroll
-                for (int u = 0; u < NR; u++)
-                    acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aw[t % (WD + 1)][kb], bc[kb][u], acc[u], 0, 0, 0);
-            if (t + 1 < TAPS)
-#pragma unroll
-                for (int kb = 0; kb < 2; kb++)
-#pragma unroll
-                    for (int u = 0; u < NR; u++) bc[kb][u] = bn[kb][u];
+                for (int dy = 0; dy < 3; dy++) {
+                    const int oz = rz - dz, oy = ry - dy;
+                    if (oz >= 0 && oz < T::WZ && oy >= 0 && oy < T::WY) {
+                        // forward: tap (dx, dy, dz) reads the input at offset (dx-1, dy-1, dz-1);
+                        // the input gradient reads the same position with the mirrored tap's weights
+                        const int t = (dz * 3 + dy) * 3 + dx, tw = MODE == 1 ? T::TAPS - 1 - t : t;
+                        const int u = oz * T::WY + oy;
+                        acc[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bfv8, aw[tw]),
+                                                                         __builtin_bit_cast(bfv8, bs[gr & 1][f]),
+                                                                         acc[u], 0, 0, 0);
+                    }
+                }
+            }
         }
-#pragma unroll
-        for (int u = 0; u < NR; u++) {
-            const int row = wave * NR + u;
-            const int gx = x0 + j, gy = y0 + row % T::TY, gz = z0 + row / T::TY;
-            if (gx >= L.n[0] || gy >= L.n[1] || gz >= L.n[2]) continue;
-            store_fragment<MODE>(acc[u], (long)gx + (long)L.n[0] * (gy + (long)L.n[1] * gz), kh, bias, mask, out);
-        }
-        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
     }
+}
+
+// the destination of stores for pixels outside the lattice: every tile issues the same
+// number of stores per wave, so that the wait for the next tile's halo can leave them in flight
+__device__ unsigned long long kTrash[64];
+
+template <int MODE, int D>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv32(
+    Lat L, const unsigned short* __restrict__ in, const unsigned short* __restrict__ W, const float* __restrict__ bias,
+    const unsigned short* __restrict__ mask, unsigned short* __restrict__ out)
+{
+    using T = Conv32<D>;
+    // two halo buffers as separate objects, so that the loads into one are not waited for by
+    // the reads of the other
+    __shared__ bf16x8 buf0[T::BUF];
+    __shared__ bf16x8 buf1[T::BUF];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j = lane & 15, q = lane >> 4, h = wave & 1, blk = wave >> 1;
+    const int wy0 = D == 3 ? (blk & 1) * T::WY : blk * T::WY, wz0 = D == 3 ? (blk >> 1) * T::WZ : 0;
+    const int c0 = 16 * h + 4 * q;      // this lane's four output channels
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    bf16x8 aw[T::TAPS];
+#pragma unroll
+    for (int t = 0; t < T::TAPS; t++) aw[t] = *reinterpret_cast<const bf16x8*>(W + ((t * NN_H + 16 * h + j) * NN_H + 8 * q));
+    float bv[4] = {0.f, 0.f, 0.f, 0.f};
+    if (MODE == 0)
+#pragma unroll
+        for (int r = 0; r < 4; r++) bv[r] = bias[c0 + r];
+    // halo loads: wave w fills chunk w % 4 (channels 8 (w%4) .. +7) of the halo pixels 64 k + lane
+    // for the groups k = 2 i + w / 4; rel[i] is that pixel's byte offset from the halo origin,
+    // hc[i] its halo coordinates (8 bits each; all ones past the halo)
+    const int ch = wave & 3, kpar = wave >> 2;
+    const unsigned nbytes = (unsigned)L.np * (NN_H * 2);   // below 2^31: checked at the launch
+    int rel[T::KPW];
+    unsigned hc[T::KPW];
+#pragma unroll
+    for (int i = 0; i < T::KPW; i++) {
+        const int hp = 64 * (2 * i + kpar) + lane;
+        const int hx = hp % T::HX, hy = (hp / T::HX) % T::HY, hz = hp / (T::HX * T::HY);
+        rel[i] = (hx + nx * (hy + ny * hz)) * (NN_H * 2);
+        hc[i] = hp < T::HP ? (unsigned)hx | (unsigned)hy << 8 | (unsigned)hz << 16 : 0xffffffu;
+    }
+    const int ntx = (nx + T::TX - 1) / T::TX, nty = (ny + T::TY - 1) / T::TY, ntz = (nz + T::TZ - 1) / T::TZ;
+    const int ntile = ntx * nty * ntz;
+    // issue the halo loads of the tile at (x0, y0, z0) into buf (chunk-major: 16-byte chunk c of
+    // halo pixel hp at slot c * CS + hp); pixels outside the lattice or past the halo read at
+    // offset nbytes (zeros); interior tiles need no checks
+    auto stage = [&](int x0, int y0, int z0, bf16x8* buf) {
+        const bool interior = x0 >= 1 && x0 + T::TX + 1 <= nx && y0 >= 1 && y0 + T::TY + 1 <= ny &&
+                              (D == 2 || (z0 >= 1 && z0 + T::TZ + 1 <= nz));
+        const int org = ((x0 - 1) + nx * ((y0 - 1) + ny * (D == 3 ? z0 - 1 : 0))) * (NN_H * 2) + ch * 16;
+#pragma unroll
+        for (int i = 0; i < T::KPW; i++) {
+            const int k = 2 * i + kpar;
+            if (T::NG % 2 != 0 && k >= T::NG) break;   // wave-uniform
+            bool ok = hc[i] != 0xffffffu;
+            if (!interior) {
+                const int gx = x0 - 1 + (int)(hc[i] & 0xff), gy = y0 - 1 + (int)((hc[i] >> 8) & 0xff);
+                const int gz = D == 3 ? z0 - 1 + (int)(hc[i] >> 16) : 0;
+                ok = ok && (unsigned)gx < (unsigned)nx && (unsigned)gy < (unsigned)ny && (unsigned)gz < (unsigned)nz;
+            }
+            lds_dma16(in, nbytes, ok ? (unsigned)(org + rel[i]) : nbytes, buf + ch * T::CS + k * 64);
+        }
+    };
+    int tb, tend, tstep;
+    if ((gridDim.x & 7) == 0) {
+        const int x = blockIdx.x & 7;
+        tb = (int)((long)x * ntile / 8) + (int)(blockIdx.x >> 3);
+        tend = (int)((long)(x + 1) * ntile / 8);
+        tstep = gridDim.x >> 3;
+    } else {
+        tb = blockIdx.x;
+        tend = ntile;
+        tstep = gridDim.x;
+    }
+    if (tb < tend) stage((tb % ntx) * T::TX, (tb / ntx % nty) * T::TY, tb / (ntx * nty) * T::TZ, buf0);
+    const long sy = (long)nx * NN_H, sz = (long)nx * ny * NN_H;   // row strides in elements
+    for (int it = 0; tb < tend; tb += tstep, it++) {
+        const bf16x8* cur = (it & 1) ? buf1 : buf0;
+        bf16x8* next = (it & 1) ? buf0 : buf1;
+        // this tile's halo loads (issued one iteration earlier, before the previous tile's NR
+        // stores) have landed in every wave; every wave is done with the buffer the next loads fill
+        if (it == 0) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        else asm volatile("s_waitcnt vmcnt(%0)" ::"n"(T::NR) : "memory");
+        __syncthreads();
+        const int tn = tb + tstep;
+        if (tn < tend) stage((tn % ntx) * T::TX, (tn / ntx % nty) * T::TY, tn / (ntx * nty) * T::TZ, next);
+        f32x4 acc[T::NR];
+        conv32_tile<MODE, D>(cur, aw, wy0, wz0, acc);
+        const int x0 = (tb % ntx) * T::TX, y0 = (tb / ntx % nty) * T::TY, z0 = tb / (ntx * nty) * T::TZ;
+        const bool xin = x0 + j < nx;
+        const long p0 = (long)(x0 + j) + (long)nx * ((y0 + wy0) + (long)ny * (z0 + wz0));
+#pragma unroll
+        for (int u = 0; u < T::NR; u++) {
+            const int oy = u % T::WY, oz = u / T::WY;
+            const bool ok = xin && y0 + wy0 + oy < ny && z0 + wz0 + oz < nz;
+            const long e = p0 * NN_H + oy * sy + oz * sz + c0;   // element offset of the lane's 4 channels
+            float v[4];
+            unsigned long long mk = 0;
+            if (MODE == 1 && ok) mk = *reinterpret_cast<const unsigned long long*>(mask + e);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                float x = acc[u][r];
+                if (MODE == 0) {
+                    x += bv[r];
+                    x = x > 0.f ? x : 0.f;
+                } else {
+                    // bf16 bits m encode a value > 0 iff 0 < m <= 0x7f80 (sign clear, not zero, not NaN)
+                    const unsigned int m = (unsigned int)(mk >> (16 * r)) & 0xffffu;
+                    x = m - 1u < 0x7f80u ? x : 0.f;
+                }
+                v[r] = x;
+            }
+            uint2 pk;
+            pk.x = pack_bf16x2(v[0], v[1]);
+            pk.y = pack_bf16x2(v[2], v[3]);
+            uint2* dst = ok ? reinterpret_cast<uint2*>(out + e) : reinterpret_cast<uint2*>(kTrash + lane);
+            *dst = pk;
+        }
+    }
+    // no LDS load may still be in flight when the workgroup's LDS is released
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
 // split-BF16 copy of the input: inb[p][0..7] = hi(in[p][c]), inb[p][8..15] =
@@ -551,10 +695,7 @@ __global__ __launch_bounds__(MidTile<D>::NW * 64) void k_last_lds(Lat L, const u
     }
 }
 
-#ifndef JFEM_NN_NT
-#define JFEM_NN_NT 4
-#endif
-constexpr int MID_NT = JFEM_NN_NT;   // pixel tiles per wavefront in the 32 -> 32 layers
+constexpr int MID_NT = 4;   // pixel tiles per wavefront in the 32 -> 32 layers (1D lattices)
 
 // LDS bytes one workgroup may allocate on the current device (160 KiB on gfx950)
 static long lds_per_block()
@@ -575,17 +716,16 @@ template <int MODE>
 static void launch_mid(hipStream_t st, const Lat& L, int taps, const unsigned short* in, const unsigned short* W,
                        const float* bias, const unsigned short* mask, unsigned short* out)
 {
-    if ((L.d == 3 || L.d == 2) && lds_per_block() >= (long)sizeof(bf16x8) * 4 * MidTile<3>::cs<4>()) {
+    if ((L.d == 3 || L.d == 2) && L.np * NN_H * 2 < (1L << 31) &&
+        lds_per_block() >= (long)sizeof(bf16x8) * 2 * Conv32<3>::BUF) {
+        // persistent: one workgroup of eight waves per CU (256 CUs), a multiple of 8 for the XCD
+        // tile ranges
         const int d = L.d;
-        const int ty = d == 3 ? MidTile<3>::TY : MidTile<2>::TY, tz = d == 3 ? MidTile<3>::TZ : MidTile<2>::TZ;
-        const long ntile = (long)((L.n[0] + 31) / 32) * ((L.n[1] + ty - 1) / ty) * ((L.n[2] + tz - 1) / tz);
-        const int grid = (int)std::min<long>(ntile, 4096);
-        if (d == 3)
-            k_mid_lds<MODE, 3><<<grid, MidTile<3>::NW * 64, sizeof(bf16x8) * 4 * MidTile<3>::cs<4>(), st>>>(L, in, W, bias,
-                                                                                                      mask, out);
-        else
-            k_mid_lds<MODE, 2><<<grid, MidTile<2>::NW * 64, sizeof(bf16x8) * 4 * MidTile<2>::cs<4>(), st>>>(L, in, W, bias,
-                                                                                                      mask, out);
+        const long tx = (L.n[0] + 15) / 16;
+        const long nt = d == 3 ? tx * ((L.n[1] + 7) / 8) * ((L.n[2] + 3) / 4) : tx * ((L.n[1] + 31) / 32);
+        const int grid = (int)(nt >= 256 ? 256 : (nt >= 8 ? nt & ~7L : nt));
+        if (d == 3) k_conv32<MODE, 3><<<grid, Conv32<3>::NW * 64, 0, st>>>(L, in, W, bias, mask, out);
+        else k_conv32<MODE, 2><<<grid, Conv32<2>::NW * 64, 0, st>>>(L, in, W, bias, mask, out);
         JFEM_HIP(hipGetLastError());
         return;
     }
