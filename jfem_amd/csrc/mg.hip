@@ -988,26 +988,23 @@ __global__ void k_gather_in(const int* __restrict__ perm, long n, const T* __res
     }
 }
 
-// the network input of k_gather_in already split for the LDS-tiled first
-// layer: one thread per DOF, reading r, D^-1 and the static channels (kept in
-// DOF order for this) contiguously and storing the pixel's 32 bytes (one
-// sector) at its lattice position; the values are those of k_gather_in
-// followed by the split (the same float products, rounded once)
+// the network input of k_gather_in as BF16 for the MFMA first layer: one thread per DOF,
+// reading r, D^-1 and the static channels (kept in DOF order for this) contiguously and
+// storing the pixel's 16 bytes at its lattice position; the values are those of k_gather_in
+// rounded to BF16
 template <typename T>
-__global__ void k_gather_split(const int* __restrict__ perm, long n, const T* __restrict__ r,
-                               const T* __restrict__ dinv, const double* __restrict__ rr,
-                               const float* __restrict__ stat_dof, int cin, unsigned short* __restrict__ inb)
+__global__ void k_gather_bf16(const int* __restrict__ perm, long n, const T* __restrict__ r,
+                              const T* __restrict__ dinv, const double* __restrict__ rr,
+                              const float* __restrict__ stat_dof, int cin, unsigned short* __restrict__ inb)
 {
     const double scale = rr[0] > 0 ? sqrt((double)n / rr[0]) : 0.0;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         const float v = (float)((double)dinv[i] * (double)r[i] * scale);
-        bf16x8 hi, lo;
-        split_bf16(v, hi, lo, 0);
+        bf16x8 b;
+        b[0] = (short)f2bf(v);
 #pragma unroll
-        for (int c = 1; c < 8; c++) split_bf16(c < cin ? v * stat_dof[i * (cin - 1) + c - 1] : 0.f, hi, lo, c);
-        const long p = perm[i];
-        *reinterpret_cast<bf16x8*>(inb + p * 16) = hi;
-        *reinterpret_cast<bf16x8*>(inb + p * 16 + 8) = lo;
+        for (int c = 1; c < 8; c++) b[c] = (short)f2bf(c < cin ? v * stat_dof[i * (cin - 1) + c - 1] : 0.f);
+        *reinterpret_cast<bf16x8*>(inb + (long)perm[i] * 8) = b;
     }
 }
 
@@ -1177,9 +1174,9 @@ static void learned_smooth(jfem_mg* G, MgLevel& L, const T* b, T* x, T* u, bool 
             level_residual<T>(G, L, b, x, r);
         }
         sumsq_dev<T>(c, n, r, dinv_of<T>(L), S.rr);
-        if (nn_split_input(S.nn, S.lat)) {
-            k_gather_split<T><<<gr, 256, 0, st>>>(S.perm, n, r, dinv_of<T>(L), S.rr, S.stat_dof, S.nn->cin,
-                                                  S.nn->inb);
+        if (nn_bf16_input(S.nn, S.lat)) {
+            k_gather_bf16<T><<<gr, 256, 0, st>>>(S.perm, n, r, dinv_of<T>(L), S.rr, S.stat_dof, S.nn->cin,
+                                                 S.nn->inb);
             nn_forward(S.nn, S.lat, true);
         } else {
             k_gather_in<T><<<gr, 256, 0, st>>>(S.perm, n, r, dinv_of<T>(L), S.rr, S.stat, S.nn->cin,

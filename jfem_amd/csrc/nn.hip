@@ -205,76 +205,6 @@ __global__ __launch_bounds__(256) void k_mid(Lat L, int taps_rt, const unsigned 
     }
 }
 
-// 32 -> 32 layer with the activations of a block's tile staged in LDS (2D and
-// 3D lattices).  A block owns a tile of 32 x TY x TZ pixels (x fastest) and
-// loads it with a one-pixel halo (zeros outside the lattice) into LDS once;
-// every tap then reads its shifted B fragments from LDS instead of L2.  Wave w
-// computes the 32-pixel rows w*NR .. w*NR+NR-1 of the tile, so one weight
-// fragment (from global memory, the same for all waves) feeds NR MFMAs.
-template <int D>
-struct MidTile {
-    static constexpr int NW = 4;                  // wavefronts per block
-    static constexpr int TY = D == 3 ? 4 : 16;    // rows along axis 1
-    static constexpr int TZ = D == 3 ? 4 : 1;     // rows along axis 2
-    static constexpr int NR = TY * TZ / NW;       // rows per wavefront
-    static constexpr int HX = 34, HY = TY + 2, HZ = D == 3 ? TZ + 2 : 1;
-    static constexpr int HP = HX * HY * HZ;       // halo pixels (3D: 1224 = 78 KB of LDS at 32 channels, 2D: 612)
-    // The halo tile is chunk-major: 16-byte chunk c (8 channels) of halo pixel
-    // hp sits at slot c * CS + hp, CS = HP.  16 lanes reading one chunk of 16
-    // consecutive pixels touch the 64 LDS banks once, and a tap is a constant
-    // slot offset from the lane's base slot.
-    template <int CH>
-    static constexpr int cs()
-    {
-        return HP;
-    }
-    // slot offset of tap t (x fastest, offsets -1..1) from the halo pixel of
-    // offset (-1, -1, -1); the input gradient (mirror) uses the negated offsets
-    template <bool MIRROR>
-    static constexpr int tap(int t)
-    {
-        const int a0 = t % 3, a1 = (t / 3) % 3, a2 = D == 3 ? t / 9 : 0;
-        return MIRROR ? ((D == 3 ? 2 - a2 : 0) * HY + 2 - a1) * HX + 2 - a0 : (a2 * HY + a1) * HX + a0;
-    }
-    // halo pixel of row `row`'s first pixel at offset (-1, -1, -1)
-    static __device__ int row_base(int row) { return ((row / TY) * HY + row % TY) * HX; }
-};
-
-template <int D, int CH>
-__device__ inline void stage_halo(const Lat& L, const unsigned short* __restrict__ src, int x0, int y0, int z0,
-                                  bf16x8* tile)
-{
-    // all loads of a thread are issued before any LDS store, so that they are in flight together.
-    // One wave-wide step covers PPW = 64 / CH consecutive halo pixels (contiguous in global
-    // memory); lane i takes chunk i / PPW of pixel i % PPW, so each 16-lane group stores
-    // 16 consecutive slots of one chunk (no LDS bank conflicts)
-    using T = MidTile<D>;
-    constexpr int NT = T::NW * 64, NF = T::HP * CH, PER = (NF + NT - 1) / NT, CS = T::template cs<CH>();
-    constexpr int PPW = 64 / CH;
-    bf16x8 v[PER];
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
-        const int f = threadIdx.x + k * NT;
-        const int hp = (f / 64) * PPW + f % PPW, ch = (f % 64) / PPW;
-        const int hx = hp % T::HX, hy = (hp / T::HX) % T::HY, hz = hp / (T::HX * T::HY);
-        const int gx = x0 - 1 + hx, gy = D >= 2 ? y0 - 1 + hy : 0, gz = D == 3 ? z0 - 1 + hz : 0;
-        if (hp < T::HP && (unsigned)gx < (unsigned)L.n[0] && (unsigned)gy < (unsigned)L.n[1] &&
-            (unsigned)gz < (unsigned)L.n[2])
-            v[k] = *reinterpret_cast<const bf16x8*>(src + ((long)gx + (long)L.n[0] * (gy + (long)L.n[1] * gz)) *
-                                                              (CH * 8) + ch * 8);
-        else {
-#pragma unroll
-            for (int e = 0; e < 8; e++) v[k][e] = 0;
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
-        const int f = threadIdx.x + k * NT;
-        const int hp = (f / 64) * PPW + f % PPW, ch = (f % 64) / PPW;
-        if (hp < T::HP) tile[ch * CS + hp] = v[k];
-    }
-}
-
 // two floats -> packed bf16 pair (round to nearest even, v_cvt_pk_bf16_f32)
 __device__ inline unsigned int pack_bf16x2(float a, float b)
 {
@@ -282,38 +212,6 @@ __device__ inline unsigned int pack_bf16x2(float a, float b)
     v[0] = (__bf16)a;
     v[1] = (__bf16)b;
     return __builtin_bit_cast(unsigned int, v);
-}
-
-// D[c][p] fragment of row u -> NHWC bf16: lane (j, kh) holds channels
-// c = 8 g + 4 kh + e (g, e = 0..3) of pixel p; MODE 0 adds the bias and applies
-// ReLU, MODE 1 keeps the values where the forward activation is positive
-template <int MODE>
-__device__ inline void store_fragment(const f32x16& acc, long p, int kh, const float* __restrict__ bias,
-                                      const unsigned short* __restrict__ mask, unsigned short* __restrict__ out)
-{
-#pragma unroll
-    for (int g = 0; g < 4; g++) {
-        float v[4];
-        unsigned long long mk = 0;
-        if (MODE == 1) mk = *reinterpret_cast<const unsigned long long*>(mask + p * NN_H + 8 * g + 4 * kh);
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            float x = acc[4 * g + e];
-            if (MODE == 0) {
-                x += bias[8 * g + 4 * kh + e];
-                x = x > 0.f ? x : 0.f;
-            } else {
-                // bf16 bits m encode a value > 0 iff 0 < m <= 0x7f80 (sign clear, not zero, not NaN)
-                const unsigned int m = (unsigned int)(mk >> (16 * e)) & 0xffffu;
-                x = m - 1u < 0x7f80u ? x : 0.f;
-            }
-            v[e] = x;
-        }
-        uint2 pk;
-        pk.x = pack_bf16x2(v[0], v[1]);
-        pk.y = pack_bf16x2(v[2], v[3]);
-        *reinterpret_cast<uint2*>(out + p * NN_H + 8 * g + 4 * kh) = pk;
-    }
 }
 
 // 32 -> 32 layer with the weights resident in registers (2D and 3D lattices).
@@ -537,161 +435,271 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
-// split-BF16 copy of the input: inb[p][0..7] = hi(in[p][c]), inb[p][8..15] =
-// lo(in[p][c]) = bf16(in - hi) for c < cin, zeros past cin
-__global__ void k_in_split(long np, int cin, const float* __restrict__ in, unsigned short* __restrict__ inb)
+// first layer (cin <= 8 input channels -> 32, 2D and 3D lattices) on
+// v_mfma_f32_16x16x32_bf16 with BF16 input and weights: one K step of 32
+// covers four taps x 8 channels (zero weights past cin and past the last tap),
+// so 27 taps take 7 steps per output half.  The input is 16 bytes per pixel
+// (8 BF16 channels).  A workgroup of four waves owns a tile of 16 x 8 x 4
+// pixels (2D: 16 x 32) and wave w its 8 rows of block w (3D: 4 x 2 rows
+// along axes 1, 2; 2D: rows 8w .. 8w+7), both output halves; the halo tile
+// (16 bytes per pixel) is double-buffered in LDS and filled by buffer loads
+// into LDS.  Lane (j, q) of a B fragment reads pixel j's channels at tap
+// 4m + q, a per-lane offset into the halo tile.
+template <int D>
+struct ConvIn {
+    static constexpr int NW = 4;
+    static constexpr int TX = 16, TY = D == 3 ? 8 : 32, TZ = D == 3 ? 4 : 1;
+    static constexpr int WY = D == 3 ? 4 : 8, WZ = D == 3 ? 2 : 1;
+    static constexpr int NR = WY * WZ;            // 8 rows of 16 pixels per wave
+    static constexpr int HX = TX + 2, HY = TY + 2, HZ = D == 3 ? TZ + 2 : 1;
+    static constexpr int HP = HX * HY * HZ;
+    static constexpr int NG = (HP + 63) / 64;
+    static constexpr int BUF = NG * 64;           // 16-byte slots per halo buffer
+    static constexpr int TAPS = D == 3 ? 27 : 9, NK = (TAPS + 3) / 4;
+};
+
+template <int D>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv_in(
+    Lat L, int cin, const unsigned short* __restrict__ in, const float* __restrict__ W0, const float* __restrict__ b0,
+    unsigned short* __restrict__ out)
+{
+    using T = ConvIn<D>;
+    __shared__ bf16x8 buf0[T::BUF];
+    __shared__ bf16x8 buf1[T::BUF];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j = lane & 15, q = lane >> 4;
+    const int wy0 = D == 3 ? (wave & 1) * T::WY : wave * T::WY, wz0 = D == 3 ? (wave >> 1) * T::WZ : 0;
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    // A fragments: K step m, half h, lane (i, q): W0[t][16 h + i][0..7] for tap t = 4 m + q
+    bf16x8 aw[T::NK][2];
+    int toff[T::NK];   // this lane's halo-slot offset of tap 4 m + q (the last tap past the end)
+#pragma unroll
+    for (int m = 0; m < T::NK; m++) {
+        const int t = 4 * m + q, tt = t < T::TAPS ? t : T::TAPS - 1;
+        toff[m] = ((D == 3 ? tt / 9 : 0) * T::HY + (tt / 3) % 3) * T::HX + tt % 3;
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int c = 0; c < 8; c++)
+                aw[m][h][c] = (short)f2bf(t < T::TAPS && c < cin ? W0[(t * NN_H + 16 * h + j) * cin + c] : 0.f);
+    }
+    float bv[2][4];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) bv[h][r] = b0[16 * h + 4 * q + r];
+    const unsigned nbytes = (unsigned)L.np * 16;   // below 2^31: checked at the launch
+    const int ntx = (nx + T::TX - 1) / T::TX, nty = (ny + T::TY - 1) / T::TY, ntz = (nz + T::TZ - 1) / T::TZ;
+    const int ntile = ntx * nty * ntz;
+    // halo loads of the tile at (x0, y0, z0): wave w issues groups k = w, w + 4, ...; pixels
+    // outside the lattice or past the halo read at offset nbytes (zeros)
+    auto stage = [&](int x0, int y0, int z0, bf16x8* buf) {
+#pragma unroll
+        for (int i = 0; i < (T::NG + T::NW - 1) / T::NW; i++) {
+            const int k = wave + T::NW * i;
+            if (k >= T::NG) break;   // wave-uniform
+            const int hp = 64 * k + lane;
+            const int hx = hp % T::HX, hy = (hp / T::HX) % T::HY, hz = hp / (T::HX * T::HY);
+            const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = D == 3 ? z0 - 1 + hz : 0;
+            const bool ok = hp < T::HP && (unsigned)gx < (unsigned)nx && (unsigned)gy < (unsigned)ny &&
+                            (unsigned)gz < (unsigned)nz;
+            const unsigned off = ((unsigned)gx + (unsigned)nx * ((unsigned)gy + (unsigned)ny * (unsigned)gz)) * 16u;
+            lds_dma16(in, nbytes, ok ? off : nbytes, buf + k * 64);
+        }
+    };
+    int tb, tend, tstep;
+    if ((gridDim.x & 7) == 0) {
+        const int x = blockIdx.x & 7;
+        tb = (int)((long)x * ntile / 8) + (int)(blockIdx.x >> 3);
+        tend = (int)((long)(x + 1) * ntile / 8);
+        tstep = gridDim.x >> 3;
+    } else {
+        tb = blockIdx.x;
+        tend = ntile;
+        tstep = gridDim.x;
+    }
+    if (tb < tend) stage((tb % ntx) * T::TX, (tb / ntx % nty) * T::TY, tb / (ntx * nty) * T::TZ, buf0);
+    const long sy = (long)nx * NN_H, sz = (long)nx * ny * NN_H;
+    for (int it = 0; tb < tend; tb += tstep, it++) {
+        const bf16x8* cur = (it & 1) ? buf1 : buf0;
+        bf16x8* next = (it & 1) ? buf0 : buf1;
+        // this tile's halo loads (issued before the previous tile's 2 NR stores) have landed in
+        // every wave; every wave is done with the buffer the next loads fill
+        if (it == 0) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        else asm volatile("s_waitcnt vmcnt(%0)" ::"n"(2 * T::NR) : "memory");
+        __syncthreads();
+        const int tn = tb + tstep;
+        if (tn < tend) stage((tn % ntx) * T::TX, (tn / ntx % nty) * T::TY, tn / (ntx * nty) * T::TZ, next);
+        f32x4 acc[T::NR][2];
+        const bf16x8* base = cur + (wz0 * T::HY + wy0) * T::HX + j;
+        bf16x8 bf[2][T::NK];
+#pragma unroll
+        for (int m = 0; m < T::NK; m++) bf[0][m] = base[toff[m]];
+#pragma unroll
+        for (int u = 0; u < T::NR; u++) {
+            const int ro = ((u / T::WY) * T::HY + u % T::WY) * T::HX;   // the row's halo-slot offset
+            if (u + 1 < T::NR) {
+                const int rn = (((u + 1) / T::WY) * T::HY + (u + 1) % T::WY) * T::HX;
+#pragma unroll
+                for (int m = 0; m < T::NK; m++) bf[(u + 1) & 1][m] = base[rn + toff[m]];
+            }
+            (void)ro;
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                acc[u][h] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int m = 0; m < T::NK; m++)
+                    acc[u][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bfv8, aw[m][h]),
+                                                                        __builtin_bit_cast(bfv8, bf[u & 1][m]),
+                                                                        acc[u][h], 0, 0, 0);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        const int x0 = (tb % ntx) * T::TX, y0 = (tb / ntx % nty) * T::TY, z0 = tb / (ntx * nty) * T::TZ;
+        const bool xin = x0 + j < nx;
+        const long p0 = (long)(x0 + j) + (long)nx * ((y0 + wy0) + (long)ny * (z0 + wz0));
+#pragma unroll
+        for (int u = 0; u < T::NR; u++) {
+            const int oy = u % T::WY, oz = u / T::WY;
+            const bool ok = xin && y0 + wy0 + oy < ny && z0 + wz0 + oz < nz;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const long e = p0 * NN_H + oy * sy + oz * sz + 16 * h + 4 * q;
+                float v[4];
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const float x = acc[u][h][r] + bv[h][r];
+                    v[r] = x > 0.f ? x : 0.f;
+                }
+                uint2 pk;
+                pk.x = pack_bf16x2(v[0], v[1]);
+                pk.y = pack_bf16x2(v[2], v[3]);
+                uint2* dst = ok ? reinterpret_cast<uint2*>(out + e) : reinterpret_cast<uint2*>(kTrash + lane);
+                *dst = pk;
+            }
+        }
+    }
+    // no LDS load may still be in flight when the workgroup's LDS is released
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+// 16 bytes at byte offset voff of the buffer [base, base + nbytes), zeros at or past nbytes
+__device__ inline bf16x8 load16_or_zero(const void* base, unsigned nbytes, unsigned voff)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)nbytes, 0x00020000);
+    return __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, 0, 0));
+#else
+    return bf16x8{};
+#endif
+}
+
+// last layer (32 -> 1, 2D and 3D lattices) as two small products: for every halo pixel q of a
+// tile, c[q][t] = W3[t] . act[q] for all taps t at once on v_mfma_f32_16x16x32_bf16 (A = 16
+// halo pixels x 32 channels straight from HBM, B = 32 channels x 16 taps), into LDS; then
+// out[p] = b3 + sum_t c[p + off(t)][t] for each output pixel, the taps summed in a fixed order.
+// The tile is 14 pixels wide, so that its halo rows are exactly 16 pixels; halo pixels outside
+// the lattice read zeros.  Four waves, two workgroups per CU.
+template <int D>
+struct ConvOut {
+    static constexpr int NW = 4;
+    static constexpr int TX = 14, TY = D == 3 ? 4 : 16, TZ = D == 3 ? 4 : 1;
+    static constexpr int HX = 16, HY = TY + 2, HZ = D == 3 ? TZ + 2 : 1;
+    static constexpr int NROW = HY * HZ;          // halo rows of 16 pixels: 36 (3D), 18 (2D)
+    static constexpr int TAPS = D == 3 ? 27 : 9, NTB = (TAPS + 15) / 16;
+    static constexpr int NOUT = TX * TY * TZ;     // 224 output pixels per tile
+};
+
+template <int D>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv_out(
+    Lat L, const unsigned short* __restrict__ act, const unsigned short* __restrict__ W3h,
+    const float* __restrict__ b3, float* __restrict__ out)
+{
+    using T = ConvOut<D>;
+    __shared__ float ct[T::NROW * 16 * T::TAPS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = lane & 15, kq = lane >> 4;
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    const unsigned nbytes = (unsigned)L.np * (NN_H * 2);   // below 2^31: checked at the launch
+    // B fragments: taps 16 b + i, channels 8 kq .. 8 kq + 7 (zero past the last tap)
+    bf16x8 bw[T::NTB];
+#pragma unroll
+    for (int b = 0; b < T::NTB; b++) {
+        const int t = 16 * b + i;
+        bw[b] = t < T::TAPS ? *reinterpret_cast<const bf16x8*>(W3h + t * NN_H + 8 * kq) : bf16x8{};
+    }
+    const float bias = b3[0];
+    const int ntx = (nx + T::TX - 1) / T::TX, nty = (ny + T::TY - 1) / T::TY, ntz = (nz + T::TZ - 1) / T::TZ;
+    const int ntile = ntx * nty * ntz;
+    int tb, tend, tstep;
+    if ((gridDim.x & 7) == 0) {
+        const int x = blockIdx.x & 7;
+        tb = (int)((long)x * ntile / 8) + (int)(blockIdx.x >> 3);
+        tend = (int)((long)(x + 1) * ntile / 8);
+        tstep = gridDim.x >> 3;
+    } else {
+        tb = blockIdx.x;
+        tend = ntile;
+        tstep = gridDim.x;
+    }
+    for (; tb < tend; tb += tstep) {
+        const int x0 = (tb % ntx) * T::TX, y0 = (tb / ntx % nty) * T::TY, z0 = tb / (ntx * nty) * T::TZ;
+        // phase 1: wave w takes halo rows w, w + 4, ...; all of its row loads first
+        constexpr int RPW = (T::NROW + T::NW - 1) / T::NW;
+        bf16x8 a[RPW];
+        const int gx = x0 - 1 + i;
+#pragma unroll
+        for (int k = 0; k < RPW; k++) {
+            const int row = wave + T::NW * k;
+            const int gy = y0 - 1 + row % T::HY, gz = D == 3 ? z0 - 1 + row / T::HY : 0;
+            const bool ok = row < T::NROW && (unsigned)gx < (unsigned)nx && (unsigned)gy < (unsigned)ny &&
+                            (unsigned)gz < (unsigned)nz;
+            const unsigned off = ((unsigned)gx + (unsigned)nx * ((unsigned)gy + (unsigned)ny * (unsigned)gz)) * (NN_H * 2) +
+                                 16u * kq;
+            a[k] = load16_or_zero(act, nbytes, ok ? off : nbytes);
+        }
+#pragma unroll
+        for (int k = 0; k < RPW; k++) {
+            const int row = wave + T::NW * k;
+            if (row >= T::NROW) break;   // wave-uniform
+#pragma unroll
+            for (int b = 0; b < T::NTB; b++) {
+                f32x4 d = {0.f, 0.f, 0.f, 0.f};
+                d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bfv8, a[k]), __builtin_bit_cast(bfv8, bw[b]),
+                                                            d, 0, 0, 0);
+                // lane (tap column i, kq) holds c[halo pixel 4 kq + r of the row][tap 16 b + i]
+                const int t = 16 * b + i;
+                if (t < T::TAPS)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) ct[(row * 16 + 4 * kq + r) * T::TAPS + t] = d[r];
+            }
+        }
+        __syncthreads();
+        // phase 2: one output pixel per thread
+        if (threadIdx.x < T::NOUT) {
+            const int ox = threadIdx.x % T::TX, oy = (threadIdx.x / T::TX) % T::TY, oz = threadIdx.x / (T::TX * T::TY);
+            float sum = bias;
+#pragma unroll
+            for (int t = 0; t < T::TAPS; t++) {
+                const int dx = t % 3, dy = (t / 3) % 3, dz = D == 3 ? t / 9 : 0;
+                sum += ct[(((oz + dz) * T::HY + oy + dy) * 16 + ox + dx) * T::TAPS + t];
+            }
+            const int px = x0 + ox, py = y0 + oy, pz = z0 + oz;
+            if (px < nx && py < ny && pz < nz) out[(long)px + (long)nx * (py + (long)ny * pz)] = sum;
+        }
+        __syncthreads();
+    }
+}
+
+// BF16 copy of the input: inb[p][c] = bf16(in[p][c]) for c < cin, zeros up to 8 channels
+__global__ void k_in_bf16(long np, int cin, const float* __restrict__ in, unsigned short* __restrict__ inb)
 {
     for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
-        bf16x8 hi, lo;
-#pragma unroll
-        for (int c = 0; c < 8; c++) split_bf16(c < cin ? in[p * cin + c] : 0.f, hi, lo, c);
-        *reinterpret_cast<bf16x8*>(inb + p * 16) = hi;
-        *reinterpret_cast<bf16x8*>(inb + p * 16 + 8) = lo;
-    }
-}
-
-// first layer on MFMA (2D/3D, cin <= 8) in split-BF16, two taps per 16-wide K
-// step: lanes 0-31 feed tap t0 = 2p and lanes 32-63 tap t1 = 2p + 1 of pair p
-// (zero weights past the last tap).  Input x = hi + lo and weights
-// w = w_hi + w_lo in BF16 parts; three MFMAs per pair accumulate
-// w_hi*hi + w_hi*lo + w_lo*hi, i.e. the product up to the w_lo*lo term
-// (below 2^-16 of it), close to FP32 accuracy.  The input tile is staged in
-// LDS, chunk 0 the high parts, chunk 1 the low parts.
-template <int D>
-__global__ __launch_bounds__(MidTile<D>::NW * 64) void k_first_lds(Lat L, int cin, const unsigned short* __restrict__ inb,
-                                                                  const float* __restrict__ W0,
-                                                                  const float* __restrict__ b0,
-                                                                  unsigned short* __restrict__ out)
-{
-    using T = MidTile<D>;
-    constexpr int TAPS = D == 3 ? 27 : 9, NP = (TAPS + 1) / 2, CS = T::template cs<2>(), NR = T::NR;
-    extern __shared__ bf16x8 sm[];
-    bf16x8* wf = sm;                     // NP x {w_hi, w_lo} x 64 lanes: A fragments of the tap pairs
-    bf16x8* tile = sm + NP * 2 * 64;     // chunk-major halo tile
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int j = lane & 31, kh = lane >> 5;
-    for (int f = threadIdx.x; f < NP * 2 * 64; f += T::NW * 64) {
-        const int p = f / 128, part = (f / 64) & 1, l = f & 63;
-        const int t = 2 * p + (l >> 5), co = l & 31;
         bf16x8 v;
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const float w = e < cin && t < TAPS ? W0[(t * NN_H + co) * cin + e] : 0.f;
-            const unsigned short h = f2bf(w);
-            v[e] = (short)(part == 0 ? h : f2bf(w - bf2f(h)));
-        }
-        wf[f] = v;
-    }
-    const int nx = (L.n[0] + 31) / 32, ny = (L.n[1] + T::TY - 1) / T::TY, nz = (L.n[2] + T::TZ - 1) / T::TZ;
-    const long ntile = (long)nx * ny * nz;
-    for (long tb = blockIdx.x; tb < ntile; tb += gridDim.x) {
-        const int bx = (int)(tb % nx), by = (int)((tb / nx) % ny), bz = (int)(tb / ((long)nx * ny));
-        const int x0 = bx * 32, y0 = by * T::TY, z0 = bz * T::TZ;
-        stage_halo<D, 2>(L, inb, x0, y0, z0, tile);
-        __syncthreads();
-        const bf16x8* bl[NR];
-#pragma unroll
-        for (int u = 0; u < NR; u++) bl[u] = tile + T::row_base(wave * NR + u) + j;
-        f32x16 acc[NR];
-#pragma unroll
-        for (int u = 0; u < NR; u++)
-#pragma unroll
-            for (int r = 0; r < 16; r++) acc[u][r] = 0.f;
-        // halo slot offset of this lane's tap in pair p (the last pair's second tap: any slot, zero weights)
-        auto toff = [&](int p) {
-            const int t0 = 2 * p, t1 = 2 * p + 1 < TAPS ? 2 * p + 1 : 2 * p;
-            return kh ? T::template tap<false>(t1) : T::template tap<false>(t0);
-        };
-        bf16x8 ahi = wf[lane], alo = wf[64 + lane], bh[NR], bo[NR];
-        {
-            const int o = toff(0);
-#pragma unroll
-            for (int u = 0; u < NR; u++) { bh[u] = bl[u][o]; bo[u] = bl[u][o + CS]; }
-        }
-#pragma unroll
-        for (int p = 0; p < NP; p++) {
-            bf16x8 ahn, aln, bhn[NR], bon[NR];
-            if (p + 1 < NP) {
-                ahn = wf[(p + 1) * 128 + lane];
-                aln = wf[(p + 1) * 128 + 64 + lane];
-                const int o = toff(p + 1);
-#pragma unroll
-                for (int u = 0; u < NR; u++) { bhn[u] = bl[u][o]; bon[u] = bl[u][o + CS]; }
-            }
-#pragma unroll
-            for (int u = 0; u < NR; u++) {
-                acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bh[u], acc[u], 0, 0, 0);
-                acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bo[u], acc[u], 0, 0, 0);
-                acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bh[u], acc[u], 0, 0, 0);
-            }
-            if (p + 1 < NP) {
-                ahi = ahn;
-                alo = aln;
-#pragma unroll
-                for (int u = 0; u < NR; u++) { bh[u] = bhn[u]; bo[u] = bon[u]; }
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < NR; u++) {
-            const int row = wave * NR + u;
-            const int gx = x0 + j, gy = y0 + row % T::TY, gz = z0 + row / T::TY;
-            if (gx >= L.n[0] || gy >= L.n[1] || gz >= L.n[2]) continue;
-            store_fragment<0>(acc[u], (long)gx + (long)L.n[0] * (gy + (long)L.n[1] * gz), kh, b0, nullptr, out);
-        }
-        __syncthreads();
-    }
-}
-
-// last layer with the activation tile staged in LDS (2D/3D): thread q owns
-// the tile pixels q and q + NW*64 (two rows apart by NW*64 / 32), so every
-// weight pair fetched feeds two dot products; BF16 pairs, FP32 accumulation
-template <int D>
-__global__ __launch_bounds__(MidTile<D>::NW * 64) void k_last_lds(Lat L, const unsigned short* __restrict__ act,
-                                                                 const unsigned short* __restrict__ W3h,
-                                                                 const float* __restrict__ b3, float* __restrict__ out)
-{
-    using T = MidTile<D>;
-    constexpr int TAPS = D == 3 ? 27 : 9, CS = T::template cs<4>(), NT = T::NW * 64;
-    static_assert(32 * T::TY * T::TZ == 2 * NT, "two pixels per thread");
-    extern __shared__ bf16x8 tile[];     // chunk-major halo tile, 4 chunks of 8 channels
-    bf16x8* wl = tile + 4 * CS;          // the weights, TAPS x 4 chunks
-    for (int f = threadIdx.x; f < TAPS * 4; f += NT) wl[f] = reinterpret_cast<const bf16x8*>(W3h)[f];
-    const int nx = (L.n[0] + 31) / 32, ny = (L.n[1] + T::TY - 1) / T::TY, nz = (L.n[2] + T::TZ - 1) / T::TZ;
-    const long ntile = (long)nx * ny * nz;
-    const int xx = threadIdx.x % 32, r0 = threadIdx.x / 32, r1 = r0 + NT / 32;
-    for (long tb = blockIdx.x; tb < ntile; tb += gridDim.x) {
-        const int bx = (int)(tb % nx), by = (int)((tb / nx) % ny), bz = (int)(tb / ((long)nx * ny));
-        const int x0 = bx * 32, y0 = by * T::TY, z0 = bz * T::TZ;
-        stage_halo<D, 4>(L, act, x0, y0, z0, tile);
-        __syncthreads();
-        const bf16x8* p0 = tile + T::row_base(r0) + xx;
-        const bf16x8* p1 = tile + T::row_base(r1) + xx;
-        float acc0 = b3[0], acc1 = acc0;
-        // the tap loop stays rolled (fully unrolled, the compiler issues all tap loads at once and spills);
-        // a tap is a uniform slot offset
-#pragma unroll 1
-        for (int t = 0; t < TAPS; t++) {
-            const int to = T::template tap<false>(t);
-#pragma unroll
-            for (int g = 0; g < 4; g++) {
-                const bfv8 a0 = __builtin_bit_cast(bfv8, p0[to + g * CS]);
-                const bfv8 a1 = __builtin_bit_cast(bfv8, p1[to + g * CS]);
-                const bfv8 w = __builtin_bit_cast(bfv8, wl[4 * t + g]);
-#pragma unroll
-                for (int e = 0; e < 8; e += 2) {
-                    const bfv2 we = {w[e], w[e + 1]};
-                    acc0 = __builtin_amdgcn_fdot2_f32_bf16(bfv2{a0[e], a0[e + 1]}, we, acc0, false);
-                    acc1 = __builtin_amdgcn_fdot2_f32_bf16(bfv2{a1[e], a1[e + 1]}, we, acc1, false);
-                }
-            }
-        }
-        const int gx = x0 + xx;
-        if (gx < L.n[0]) {
-            const int gy0 = y0 + r0 % T::TY, gz0 = z0 + r0 / T::TY, gy1 = y0 + r1 % T::TY, gz1 = z0 + r1 / T::TY;
-            if (gy0 < L.n[1] && gz0 < L.n[2]) out[(long)gx + (long)L.n[0] * (gy0 + (long)L.n[1] * gz0)] = acc0;
-            if (gy1 < L.n[1] && gz1 < L.n[2]) out[(long)gx + (long)L.n[0] * (gy1 + (long)L.n[1] * gz1)] = acc1;
-        }
-        __syncthreads();
+        for (int c = 0; c < 8; c++) v[c] = (short)f2bf(c < cin ? in[p * cin + c] : 0.f);
+        *reinterpret_cast<bf16x8*>(inb + p * 8) = v;
     }
 }
 
@@ -1016,7 +1024,7 @@ void nn_reserve(jfem_nn* nn, long np)
     JFEM_HIP(hipMalloc(&nn->g1, hb));
     JFEM_HIP(hipMalloc(&nn->g2, hb));
     JFEM_HIP(hipMalloc(&nn->in, sizeof(float) * nn->cin * (size_t)np));
-    JFEM_HIP(hipMalloc(&nn->inb, sizeof(unsigned short) * 16 * (size_t)np));
+    JFEM_HIP(hipMalloc(&nn->inb, sizeof(unsigned short) * 8 * (size_t)np));
     JFEM_HIP(hipMalloc(&nn->out, sizeof(float) * (size_t)np));
     JFEM_HIP(hipMalloc(&nn->gout, sizeof(float) * (size_t)np));
     nn->cap = np;
@@ -1025,9 +1033,10 @@ void nn_reserve(jfem_nn* nn, long np)
 static int nn_grid(long np) { return grid_for(np, 256, 8192); }
 
 // forward on nn->in (np x cin FP32, lattice order) into nn->out (np FP32)
-bool nn_split_input(const jfem_nn* nn, const Lat& L)
+bool nn_bf16_input(const jfem_nn* nn, const Lat& L)
 {
-    return (L.d == 2 || L.d == 3) && nn->cin <= 8 && lds_per_block() >= (long)sizeof(bf16x8) * 4 * MidTile<3>::cs<4>();
+    return (L.d == 2 || L.d == 3) && nn->cin <= 8 && L.np * NN_H * 2 < (1L << 31) &&
+           lds_per_block() >= (long)sizeof(bf16x8) * 2 * Conv32<3>::BUF;
 }
 
 void nn_forward(jfem_nn* nn, const Lat& L, bool inb_ready)
@@ -1037,31 +1046,28 @@ void nn_forward(jfem_nn* nn, const Lat& L, bool inb_ready)
     hipStream_t st = nn->ctx->stream;
     const int g = nn_grid(L.np);
     const float *W0 = nn->w + o[0], *b0 = nn->w + o[1];
-    if (inb_ready && !nn_split_input(nn, L))
-        throw Error{JFEM_ERR_STATE, "nn_forward: split-BF16 input on a lattice without the LDS-tiled layers"};
-    if (nn_split_input(nn, L)) {
-        // 2D / 3D lattices: first and last layers with LDS tiles, first layer on MFMA
-        if (!inb_ready) k_in_split<<<g, 256, 0, st>>>(L.np, nn->cin, nn->in, nn->inb);
+    if (inb_ready && !nn_bf16_input(nn, L))
+        throw Error{JFEM_ERR_STATE, "nn_forward: BF16 input on a lattice without the LDS-tiled layers"};
+    if (nn_bf16_input(nn, L)) {
+        // 2D / 3D lattices: every layer on MFMA, the first from the BF16 input
+        if (!inb_ready) k_in_bf16<<<g, 256, 0, st>>>(L.np, nn->cin, nn->in, nn->inb);
+        const long tx = (L.n[0] + 15) / 16;
+        const long nt = L.d == 3 ? tx * ((L.n[1] + 7) / 8) * ((L.n[2] + 3) / 4) : tx * ((L.n[1] + 31) / 32);
+        // persistent: two workgroups per CU, a multiple of 8 for the XCD tile ranges
+        const int gi = (int)(nt >= 512 ? 512 : (nt >= 8 ? nt & ~7L : nt));
+        const long tox = (L.n[0] + 13) / 14;
+        const long nto = L.d == 3 ? tox * ((L.n[1] + 3) / 4) * ((L.n[2] + 3) / 4) : tox * ((L.n[1] + 15) / 16);
+        const int go = (int)(nto >= 512 ? 512 : (nto >= 8 ? nto & ~7L : nto));
         if (L.d == 3) {
-            using T = MidTile<3>;
-            const long nt = (long)((L.n[0] + 31) / 32) * ((L.n[1] + T::TY - 1) / T::TY) * ((L.n[2] + T::TZ - 1) / T::TZ);
-            const int grid = (int)std::min<long>(nt, 4096);
-            k_first_lds<3><<<grid, T::NW * 64, sizeof(bf16x8) * (14 * 128 + 2 * T::cs<2>()), st>>>(L, nn->cin, nn->inb, W0,
-                                                                                                 b0, nn->act0);
+            k_conv_in<3><<<gi, ConvIn<3>::NW * 64, 0, st>>>(L, nn->cin, nn->inb, W0, b0, nn->act0);
             launch_mid<0>(st, L, nn->taps, nn->act0, nn->wh1, nn->w + o[3], nullptr, nn->act1);
             launch_mid<0>(st, L, nn->taps, nn->act1, nn->wh2, nn->w + o[5], nullptr, nn->act2);
-            k_last_lds<3><<<grid, T::NW * 64, sizeof(bf16x8) * (4 * T::cs<4>() + 4 * 27), st>>>(L, nn->act2, nn->wh3, nn->w + o[7],
-                                                                               nn->out);
+            k_conv_out<3><<<go, ConvOut<3>::NW * 64, 0, st>>>(L, nn->act2, nn->wh3, nn->w + o[7], nn->out);
         } else {
-            using T = MidTile<2>;
-            const long nt = (long)((L.n[0] + 31) / 32) * ((L.n[1] + T::TY - 1) / T::TY);
-            const int grid = (int)std::min<long>(nt, 4096);
-            k_first_lds<2><<<grid, T::NW * 64, sizeof(bf16x8) * (5 * 128 + 2 * T::cs<2>()), st>>>(L, nn->cin, nn->inb, W0,
-                                                                                                b0, nn->act0);
+            k_conv_in<2><<<gi, ConvIn<2>::NW * 64, 0, st>>>(L, nn->cin, nn->inb, W0, b0, nn->act0);
             launch_mid<0>(st, L, nn->taps, nn->act0, nn->wh1, nn->w + o[3], nullptr, nn->act1);
             launch_mid<0>(st, L, nn->taps, nn->act1, nn->wh2, nn->w + o[5], nullptr, nn->act2);
-            k_last_lds<2><<<grid, T::NW * 64, sizeof(bf16x8) * (4 * T::cs<4>() + 4 * 9), st>>>(L, nn->act2, nn->wh3, nn->w + o[7],
-                                                                               nn->out);
+            k_conv_out<2><<<go, ConvOut<2>::NW * 64, 0, st>>>(L, nn->act2, nn->wh3, nn->w + o[7], nn->out);
         }
         JFEM_HIP(hipGetLastError());
         return;

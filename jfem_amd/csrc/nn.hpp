@@ -36,15 +36,6 @@ __host__ __device__ inline float bf2f(unsigned short h)
     return f;
 }
 
-// split-BF16 pair of an FP32 value: hi = bf16(v), lo = bf16(v - hi); hi + lo
-// carries about 16 mantissa bits
-__device__ inline void split_bf16(float v, bf16x8& hi, bf16x8& lo, int c)
-{
-    const unsigned short h = f2bf(v);
-    hi[c] = (short)h;
-    lo[c] = (short)f2bf(v - bf2f(h));
-}
-
 }  // namespace jfem
 
 struct jfem_nn {
@@ -61,7 +52,7 @@ struct jfem_nn {
     unsigned short *act0 = nullptr, *act1 = nullptr, *act2 = nullptr;   // hidden activations (post-ReLU), NHWC bf16
     unsigned short *g0 = nullptr, *g1 = nullptr, *g2 = nullptr;         // gradients w.r.t. hidden pre-activations
     float* in = nullptr;     // np x cin input (lattice order)
-    unsigned short* inb = nullptr;   // np x 16 split-bf16 copy of the input (8 high parts, 8 low parts)
+    unsigned short* inb = nullptr;   // np x 8 BF16 copy of the input (zeros past cin)
     float* out = nullptr;    // np output
     float* gout = nullptr;   // dLoss/dout
     float* part = nullptr;   // per-block weight-gradient partials
@@ -71,10 +62,10 @@ struct jfem_nn {
 namespace jfem {
 void nn_reserve(jfem_nn* nn, long np);
 // forward on nn->in, or, when inb_ready, on nn->inb already written by the
-// caller (only valid where nn_split_input(nn, L) holds)
+// caller (only valid where nn_bf16_input(nn, L) holds)
 void nn_forward(jfem_nn* nn, const Lat& L, bool inb_ready = false);
-// true where the forward reads its input as the split-BF16 copy nn->inb
-bool nn_split_input(const jfem_nn* nn, const Lat& L);
+// true where the forward reads its input as the BF16 copy nn->inb
+bool nn_bf16_input(const jfem_nn* nn, const Lat& L);
 void nn_backward(jfem_nn* nn, const Lat& L, int add);
 void nn_adam_step(jfem_nn* nn, float lr, float grad_scale);
 void nn_set_bias_free(jfem_nn* nn, int on);

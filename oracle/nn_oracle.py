@@ -104,14 +104,13 @@ def bf16(a):
 
 
 def forward(params, shape, inp, cin, emulate_bf16=False):
-    """FP64 forward; emulate_bf16 rounds to BF16 where the device stores BF16: the 32->32 and last
-    layer weights and the activations after each ReLU (the first layer runs in split BF16, close
-    to FP32, and is kept exact)"""
+    """FP64 forward; emulate_bf16 rounds to BF16 where the device stores BF16: the input, the weights
+    of every layer and the activations after each ReLU (biases and accumulation stay exact)"""
     d = len(shape)
     W0, b0, W1, b1, W2, b2, W3, b3 = split(params, d, cin)
     r = bf16 if emulate_bf16 else (lambda v: v)
-    W1, W2, W3 = r(W1), r(W2), r(W3)
-    x = np.asarray(inp, np.float64).reshape(tuple(reversed(shape)) + (cin,))
+    W0, W1, W2, W3 = r(W0), r(W1), r(W2), r(W3)
+    x = r(np.asarray(inp, np.float64).reshape(tuple(reversed(shape)) + (cin,)))
     z0 = conv(x, W0, b0, d)
     a0 = r(np.maximum(z0, 0))
     z1 = conv(a0, W1, b1, d)
