@@ -1008,6 +1008,46 @@ __global__ void k_gather_bf16(const int* __restrict__ perm, long n, const T* __r
     }
 }
 
+// the network input of one plain learned step, from the residual epilogue of the level
+// operator: v = D^-1 (b - A x) and v times each static channel, as BF16 at the node's lattice
+// position.  The attached network has no biases, so N(c v) = c N(v) for c > 0 and the input
+// needs no normalisation: x += N(v) is the step x += rms(v) N(v / rms(v)).
+__device__ inline void netin_store(float v, const float* __restrict__ stat, long row, int cin, int pix,
+                                   unsigned short* __restrict__ inb)
+{
+    bf16x8 h;
+    h[0] = (short)f2bf(v);
+#pragma unroll
+    for (int c = 1; c < 8; c++) h[c] = (short)f2bf(c < cin ? v * stat[row * (cin - 1) + c - 1] : 0.f);
+    *reinterpret_cast<bf16x8*>(inb + (long)pix * 8) = h;
+}
+
+template <typename T>
+struct EpiNetIn {
+    static constexpr bool reduce = false;
+    const T *b, *dinv;
+    const float* stat;   // static channels in DOF order, cin - 1 per node
+    const int* perm;
+    unsigned short* inb;
+    int cin;
+    struct Pre { T b, dinv; int pix; };
+    __device__ Pre load(long row) const { return {b[row], dinv[row], perm[row]}; }
+    __device__ double operator()(long row, T ax, const Pre& p) const
+    {
+        netin_store((float)(p.dinv * (p.b - ax)), stat, row, cin, p.pix, inb);
+        return 0.0;
+    }
+};
+
+// the same input for x = 0 (r = b)
+template <typename T>
+__global__ void k_netin_b(long n, const T* __restrict__ b, const T* __restrict__ dinv, const float* __restrict__ stat,
+                          const int* __restrict__ perm, int cin, unsigned short* __restrict__ inb)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        netin_store((float)(dinv[i] * b[i]), stat, i, cin, perm[i], inb);
+}
+
 // static channels in DOF order: out[i][c] = stat[perm[i]][c]
 __global__ void k_stat_to_dof(long n, int ns, const int* __restrict__ perm, const float* __restrict__ stat,
                               float* __restrict__ out)
@@ -1025,6 +1065,17 @@ __global__ void k_scatter_add(const int* __restrict__ perm, long n, const float*
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         if (bnd[i]) continue;
         x[i] = (T)((double)x[i] + (double)out[perm[i]] * inv);
+    }
+}
+
+// x[i] += out[perm[i]] on free nodes (the unnormalised plain step)
+template <typename T>
+__global__ void k_add_out(const int* __restrict__ perm, long n, const float* __restrict__ out,
+                          const unsigned char* __restrict__ bnd, T* __restrict__ x)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        if (bnd[i]) continue;
+        x[i] = (T)((double)x[i] + (double)out[perm[i]]);
     }
 }
 
@@ -1167,6 +1218,20 @@ static void learned_smooth(jfem_mg* G, MgLevel& L, const T* b, T* x, T* u, bool 
     T* r = (T*)L.r;
     T* w = (T*)L.dv;
     for (int it = 0; it < S.runs; it++) {
+        if (!S.mr && nn_bf16_input(S.nn, S.lat)) {
+            // plain step on the MFMA network: the residual epilogue writes the network input
+            // (DOF order -> lattice order); x += N(v) on the free nodes gathers the output back
+            if (x_zero && it == 0) {
+                k_fill_t<T><<<gr, 256, 0, st>>>(n, (T)0, x);
+                k_netin_b<T><<<gr, 256, 0, st>>>(n, b, dinv_of<T>(L), S.stat_dof, S.perm, S.nn->cin, S.nn->inb);
+            } else {
+                level_apply<T>(G, L, x, EpiNetIn<T>{b, dinv_of<T>(L), S.stat_dof, S.perm, S.nn->inb, S.nn->cin});
+            }
+            nn_forward(S.nn, S.lat, true);
+            k_add_out<T><<<gr, 256, 0, st>>>(S.perm, n, S.nn->out, L.bnd, x);
+            JFEM_HIP(hipGetLastError());
+            continue;
+        }
         if (x_zero && it == 0) {
             k_fill_t<T><<<gr, 256, 0, st>>>(n, (T)0, x);
             JFEM_HIP(hipMemcpyAsync(r, b, sizeof(T) * n, hipMemcpyDeviceToDevice, st));

@@ -1058,17 +1058,12 @@ void nn_forward(jfem_nn* nn, const Lat& L, bool inb_ready)
         const long tox = (L.n[0] + 13) / 14;
         const long nto = L.d == 3 ? tox * ((L.n[1] + 3) / 4) * ((L.n[2] + 3) / 4) : tox * ((L.n[1] + 15) / 16);
         const int go = (int)(nto >= 512 ? 512 : (nto >= 8 ? nto & ~7L : nto));
-        if (L.d == 3) {
-            k_conv_in<3><<<gi, ConvIn<3>::NW * 64, 0, st>>>(L, nn->cin, nn->inb, W0, b0, nn->act0);
-            launch_mid<0>(st, L, nn->taps, nn->act0, nn->wh1, nn->w + o[3], nullptr, nn->act1);
-            launch_mid<0>(st, L, nn->taps, nn->act1, nn->wh2, nn->w + o[5], nullptr, nn->act2);
-            k_conv_out<3><<<go, ConvOut<3>::NW * 64, 0, st>>>(L, nn->act2, nn->wh3, nn->w + o[7], nn->out);
-        } else {
-            k_conv_in<2><<<gi, ConvIn<2>::NW * 64, 0, st>>>(L, nn->cin, nn->inb, W0, b0, nn->act0);
-            launch_mid<0>(st, L, nn->taps, nn->act0, nn->wh1, nn->w + o[3], nullptr, nn->act1);
-            launch_mid<0>(st, L, nn->taps, nn->act1, nn->wh2, nn->w + o[5], nullptr, nn->act2);
-            k_conv_out<2><<<go, ConvOut<2>::NW * 64, 0, st>>>(L, nn->act2, nn->wh3, nn->w + o[7], nn->out);
-        }
+        if (L.d == 3) k_conv_in<3><<<gi, ConvIn<3>::NW * 64, 0, st>>>(L, nn->cin, nn->inb, W0, b0, nn->act0);
+        else k_conv_in<2><<<gi, ConvIn<2>::NW * 64, 0, st>>>(L, nn->cin, nn->inb, W0, b0, nn->act0);
+        launch_mid<0>(st, L, nn->taps, nn->act0, nn->wh1, nn->w + o[3], nullptr, nn->act1);
+        launch_mid<0>(st, L, nn->taps, nn->act1, nn->wh2, nn->w + o[5], nullptr, nn->act2);
+        if (L.d == 3) k_conv_out<3><<<go, ConvOut<3>::NW * 64, 0, st>>>(L, nn->act2, nn->wh3, nn->w + o[7], nn->out);
+        else k_conv_out<2><<<go, ConvOut<2>::NW * 64, 0, st>>>(L, nn->act2, nn->wh3, nn->w + o[7], nn->out);
         JFEM_HIP(hipGetLastError());
         return;
     }
