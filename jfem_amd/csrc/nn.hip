@@ -228,7 +228,8 @@ __device__ inline unsigned int pack_bf16x2(float a, float b)
 // CU, and the fragment reads run one group ahead of the MFMAs.  The halo tiles
 // are double-buffered in LDS and filled by buffer loads into LDS (16 B per
 // lane, no registers in flight): the next tile's loads are issued before the
-// current tile's MFMAs.  Out-of-lattice halo pixels read zeros.  The tiles
+// current tile's MFMAs, between its fragment groups.  Out-of-lattice halo pixels read
+// zeros.  The tiles
 // of XCD x (workgroups b with b % 8 = x: the hardware dispatches workgroups
 // round-robin over the eight XCDs) are a contiguous range, so neighbouring
 // tiles share their halos in one L2.
@@ -262,14 +263,15 @@ __device__ inline void lds_dma16(const void* base, unsigned nbytes, unsigned vof
 }
 
 // the MFMAs of one tile from halo buffer hb into acc[row] (the wave's half).  The fragments
-// are read in groups of FG, one group ahead of the MFMAs that use them
-template <int MODE, int D>
+// are read in groups of FG, one group ahead of the MFMAs that use them; stage(i) issues the
+// next tile's i-th halo load of this wave between the groups
+template <int MODE, int D, class Stage>
 __device__ inline void conv32_tile(const bf16x8* hb, const bf16x8 (&aw)[Conv32<D>::TAPS], int wy0, int wz0,
-                                   f32x4 (&acc)[Conv32<D>::NR])
+                                   f32x4 (&acc)[Conv32<D>::NR], Stage&& stage)
 {
     using T = Conv32<D>;
     constexpr int FG = T::FG, NGRP = T::NB / FG;
-    static_assert(T::NB % FG == 0, "whole fragment groups");
+    static_assert(T::NB % FG == 0 && NGRP >= 2, "whole fragment groups");
     const int lane = threadIdx.x & 63;
     const bf16x8* base = hb + (lane >> 4) * T::CS + (wz0 * T::HY + wy0) * T::HX + (lane & 15);
     // fragment i: axis-0 shift dx = i / (RZ RY), halo row (rz, ry) of the wave's block
@@ -287,6 +289,7 @@ __device__ inline void conv32_tile(const bf16x8* hb, const bf16x8 (&aw)[Conv32<D
         if (gr + 1 < NGRP)
 #pragma unroll
             for (int f = 0; f < FG; f++) bs[(gr + 1) & 1][f] = base[slot((gr + 1) * FG + f)];
+        if (gr < T::KPW) stage(gr);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int f = 0; f < FG; f++) {
@@ -311,6 +314,8 @@ __device__ inline void conv32_tile(const bf16x8* hb, const bf16x8 (&aw)[Conv32<D
         }
         __builtin_amdgcn_sched_barrier(0);
     }
+#pragma unroll
+    for (int i = NGRP; i < T::KPW; i++) stage(i);
 }
 
 // the destination of stores for pixels outside the lattice: every tile issues the same
@@ -357,23 +362,25 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const int ntile = ntx * nty * ntz;
     // issue the halo loads of the tile at (x0, y0, z0) into buf (chunk-major: 16-byte chunk c of
     // halo pixel hp at slot c * CS + hp); pixels outside the lattice or past the halo read at
-    // offset nbytes (zeros); interior tiles need no checks
-    auto stage = [&](int x0, int y0, int z0, bf16x8* buf) {
+    // offset nbytes (zeros)
+    // halo load i of this wave for the tile at (x0, y0, z0) into buf; interior tiles need no checks
+    auto stage_i = [&](int i, int x0, int y0, int z0, bf16x8* buf) {
+        const int k = 2 * i + kpar;
+        if (T::NG % 2 != 0 && k >= T::NG) return;   // wave-uniform
         const bool interior = x0 >= 1 && x0 + T::TX + 1 <= nx && y0 >= 1 && y0 + T::TY + 1 <= ny &&
                               (D == 2 || (z0 >= 1 && z0 + T::TZ + 1 <= nz));
         const int org = ((x0 - 1) + nx * ((y0 - 1) + ny * (D == 3 ? z0 - 1 : 0))) * (NN_H * 2) + ch * 16;
-#pragma unroll
-        for (int i = 0; i < T::KPW; i++) {
-            const int k = 2 * i + kpar;
-            if (T::NG % 2 != 0 && k >= T::NG) break;   // wave-uniform
-            bool ok = hc[i] != 0xffffffu;
-            if (!interior) {
-                const int gx = x0 - 1 + (int)(hc[i] & 0xff), gy = y0 - 1 + (int)((hc[i] >> 8) & 0xff);
-                const int gz = D == 3 ? z0 - 1 + (int)(hc[i] >> 16) : 0;
-                ok = ok && (unsigned)gx < (unsigned)nx && (unsigned)gy < (unsigned)ny && (unsigned)gz < (unsigned)nz;
-            }
-            lds_dma16(in, nbytes, ok ? (unsigned)(org + rel[i]) : nbytes, buf + ch * T::CS + k * 64);
+        bool ok = hc[i] != 0xffffffu;
+        if (!interior) {
+            const int gx = x0 - 1 + (int)(hc[i] & 0xff), gy = y0 - 1 + (int)((hc[i] >> 8) & 0xff);
+            const int gz = D == 3 ? z0 - 1 + (int)(hc[i] >> 16) : 0;
+            ok = ok && (unsigned)gx < (unsigned)nx && (unsigned)gy < (unsigned)ny && (unsigned)gz < (unsigned)nz;
         }
+        lds_dma16(in, nbytes, ok ? (unsigned)(org + rel[i]) : nbytes, buf + ch * T::CS + k * 64);
+    };
+    auto stage = [&](int x0, int y0, int z0, bf16x8* buf) {
+#pragma unroll
+        for (int i = 0; i < T::KPW; i++) stage_i(i, x0, y0, z0, buf);
     };
     int tb, tend, tstep;
     if ((gridDim.x & 7) == 0) {
@@ -397,9 +404,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         else asm volatile("s_waitcnt vmcnt(%0)" ::"n"(T::NR) : "memory");
         __syncthreads();
         const int tn = tb + tstep;
-        if (tn < tend) stage((tn % ntx) * T::TX, (tn / ntx % nty) * T::TY, tn / (ntx * nty) * T::TZ, next);
+        const bool more = tn < tend;
+        const int nx0 = (tn % ntx) * T::TX, ny0 = (tn / ntx % nty) * T::TY, nz0 = tn / (ntx * nty) * T::TZ;
         f32x4 acc[T::NR];
-        conv32_tile<MODE, D>(cur, aw, wy0, wz0, acc);
+        // the next tile's halo loads go out between the MFMA groups
+        conv32_tile<MODE, D>(cur, aw, wy0, wz0, acc, [&](int i) {
+            if (more) stage_i(i, nx0, ny0, nz0, next);
+        });
         const int x0 = (tb % ntx) * T::TX, y0 = (tb / ntx % nty) * T::TY, z0 = tb / (ntx * nty) * T::TZ;
         const bool xin = x0 + j < nx;
         const long p0 = (long)(x0 + j) + (long)nx * ((y0 + wy0) + (long)ny * (z0 + wz0));

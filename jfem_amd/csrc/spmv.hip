@@ -226,31 +226,55 @@ void mat_build_blocks(jfem_mat* A)
 // fixed rows: zeroed, diagonal set to 0 + (1 - 0) = 1, rhs = g.
 // free rows: rhs_j += -(A_ji) * g_i for fixed columns i in ascending order,
 // then those entries are zeroed (entries stay in the pattern).
-__global__ void k_dirichlet(long n, const int* __restrict__ rowptr, const int* __restrict__ cols,
-                            double* __restrict__ vals, const unsigned char* __restrict__ fixed,
-                            const double* __restrict__ g, double* __restrict__ rhs, int* __restrict__ missing_diag)
+// Dirichlet elimination, 16 lanes per row (coalesced over the row's entries).  A fixed row
+// becomes the identity row and its right-hand side the boundary value; in a free row every
+// column of a fixed node is zeroed.  With boundary values g, a free row that has such columns
+// moves -A_rc g_c to its right-hand side in ascending column order, on one lane, as the
+// reference's loop does (AcceptsMatrixBoundaryValues.java:18-44); rows without fixed columns
+// are left alone
+__global__ __launch_bounds__(256) void k_dirichlet(long n, const int* __restrict__ rowptr, const int* __restrict__ cols,
+                                                   double* __restrict__ vals, const unsigned char* __restrict__ fixed,
+                                                   const double* __restrict__ g, double* __restrict__ rhs,
+                                                   int* __restrict__ missing_diag)
 {
 #pragma clang fp contract(off)
-    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) {
+    const int sub = threadIdx.x & 15;
+    const bool move = rhs && g;
+    for (long r = (blockIdx.x * 256L + threadIdx.x) >> 4; r < n; r += (gridDim.x * 256L) >> 4) {
         const int s = rowptr[r], e = rowptr[r + 1];
+        int found = 0;
         if (fixed[r]) {
-            bool diag = false;
-            for (int p = s; p < e; p++) {
-                if (cols[p] == r) { vals[p] = 1.0; diag = true; }
-                else vals[p] = 0;
+            for (int p = s + sub; p < e; p += 16) {
+                const bool dg = cols[p] == r;
+                vals[p] = dg ? 1.0 : 0.0;
+                found |= dg;
             }
-            if (!diag) *missing_diag = 1;
-            if (rhs) rhs[r] = g ? g[r] : 0.0;
+            for (int o = 8; o > 0; o >>= 1) found |= __shfl_xor(found, o, 16);
+            if (sub == 0) {
+                if (!found) *missing_diag = 1;
+                if (rhs) rhs[r] = g ? g[r] : 0.0;
+            }
         } else {
-            double acc = rhs ? rhs[r] : 0.0;
-            for (int p = s; p < e; p++) {
-                const int c = cols[p];
-                if (fixed[c]) {
-                    if (g) acc += -vals[p] * g[c];
-                    vals[p] = 0;
+            for (int p = s + sub; p < e; p += 16) {
+                if (fixed[cols[p]]) {
+                    found = 1;
+                    if (!move) vals[p] = 0;
                 }
             }
-            if (rhs && g) rhs[r] = acc;
+            if (move) {
+                for (int o = 8; o > 0; o >>= 1) found |= __shfl_xor(found, o, 16);
+                if (found && sub == 0) {
+                    double acc = rhs[r];
+                    for (int p = s; p < e; p++) {
+                        const int c = cols[p];
+                        if (fixed[c]) {
+                            acc += -vals[p] * g[c];
+                            vals[p] = 0;
+                        }
+                    }
+                    rhs[r] = acc;
+                }
+            }
         }
     }
 }
@@ -260,13 +284,17 @@ __global__ void k_to_f32(long n, const double* __restrict__ a, float* __restrict
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = (float)a[i];
 }
 
-__global__ void k_diag(long n, const int* __restrict__ rowptr, const int* __restrict__ cols,
-                       const double* __restrict__ vals, double* __restrict__ diag)
+// diagonal of a canonical CSR matrix (at most one entry per row on the diagonal), 16 lanes per row
+__global__ __launch_bounds__(256) void k_diag(long n, const int* __restrict__ rowptr, const int* __restrict__ cols,
+                                              const double* __restrict__ vals, double* __restrict__ diag)
 {
-    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) {
+    const int sub = threadIdx.x & 15;
+    for (long r = (blockIdx.x * 256L + threadIdx.x) >> 4; r < n; r += (gridDim.x * 256L) >> 4) {
         double d = 0;
-        for (int p = rowptr[r]; p < rowptr[r + 1]; p++) if (cols[p] == r) d += vals[p];
-        diag[r] = d;
+        for (int p = rowptr[r] + sub; p < rowptr[r + 1]; p += 16)
+            if (cols[p] == r) d = vals[p];
+        for (int o = 8; o > 0; o >>= 1) d += __shfl_xor(d, o, 16);
+        if (sub == 0) diag[r] = d;
     }
 }
 
@@ -280,7 +308,7 @@ void mat_make_f32(jfem_mat* A)
 
 void mat_diag(jfem_mat* A, double* out)
 {
-    k_diag<<<grid_for(A->rows, 256, 16384), 256, 0, A->ctx->stream>>>(A->rows, A->rowptr, A->colidx, A->vals, out);
+    k_diag<<<grid_for(A->rows * 16, 256, 16384), 256, 0, A->ctx->stream>>>(A->rows, A->rowptr, A->colidx, A->vals, out);
     JFEM_HIP(hipGetLastError());
 }
 
@@ -387,7 +415,7 @@ void apply_dirichlet_mask(jfem_mat* A, double* rhs, const unsigned char* fixed_d
     int* flag = nullptr;
     JFEM_HIP(hipMalloc(&flag, sizeof(int)));
     JFEM_HIP(hipMemsetAsync(flag, 0, sizeof(int), c->stream));
-    k_dirichlet<<<grid_for(A->rows, 256, 16384), 256, 0, c->stream>>>(A->rows, A->rowptr, A->colidx, A->vals,
+    k_dirichlet<<<grid_for(A->rows * 16, 256, 16384), 256, 0, c->stream>>>(A->rows, A->rowptr, A->colidx, A->vals,
                                                                         fixed_dev, g_dev, rhs, flag);
     int h = 0;
     hipError_t e = hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
