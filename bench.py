@@ -185,6 +185,24 @@ def spmv_only(jf, cells):
                       "copy_gbs": gbs, "copy_ms": cms, "n": n, "nnz": nnz}), flush=True)
 
 
+def network_only(jf, cells, reps=5):
+    """the learned smoother's network alone on the finest node lattice of `cells`^3 Q2 cells
+    (2 cells + 1 nodes per axis), with the C2 network's shape (3D, 5 input channels) and random
+    weights and input: `reps` forward passes through jfem_nn_forward_host, for kernel traces"""
+    n = 2 * cells + 1
+    net = jf.SmootherNet(3, 5, seed=3)
+    rng = np.random.default_rng(0)
+    net.set_params((rng.standard_normal(net.num_params()) * 0.05).astype(np.float32))
+    x = rng.standard_normal((n * n * n, 5)).astype(np.float32)
+    times = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        y = net.forward([n, n, n], x.reshape(-1))
+        times.append(time.perf_counter() - t)
+    print(json.dumps({"network_forward_s_host_copies_included": times, "nodes": n ** 3,
+                      "output_abs_sum": float(np.abs(y).sum())}), flush=True)
+
+
 def c4_hierarchy(jf, field, precision, cells=1024, base=16, galerkin=False):
     nlev = 1
     c = cells
@@ -564,6 +582,8 @@ def main():
     ap.add_argument("--learned-sides", type=int, default=3, choices=[1, 2, 3],
                     help="C2: V-cycle smoothing calls that use the network (1 pre, 2 post, 3 both); the "
                          "other side keeps Chebyshev")
+    ap.add_argument("--network-only", action="store_true",
+                    help="time the learned smoother's network forward alone on the finest node lattice")
     ap.add_argument("--spmv-only", action="store_true",
                     help="assemble the finest matrix and time only the SpMV and the copy calibration (for PMC runs)")
     args = ap.parse_args()
@@ -574,6 +594,9 @@ def main():
 
     import jfem_amd as jf
 
+    if args.network_only:
+        network_only(jf, args.cells)
+        return
     if args.spmv_only:
         spmv_only(jf, args.cells)
         return
