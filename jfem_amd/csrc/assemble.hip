@@ -193,6 +193,83 @@ __global__ __launch_bounds__(256) void k_element_tables(ElemParams P, const Tab1
         // quadrature points of the cell, a constant with fixed d and Q (1: the loop is not unrolled)
         constexpr int QN = QQ && DD ? (DD == 3 ? QQ * QQ * QQ : DD == 2 ? QQ * QQ : QQ) : 1;
         double ret = 0;
+        if constexpr (DD > 0 && KK > 0 && QQ > 0) {
+            // fixed shape: the quadrature sum below is the generic one, operation for operation, with
+            // the 1D values, derivatives and weights of axes 1.. held in registers and those of axis
+            // 0 loaded once per axis-0 point (the points are visited in the same order, last axis
+            // fastest)
+            constexpr int R = DD > 1 ? QN / QQ : 1;   // points of axes 1.. per axis-0 point
+            double Vu[MAXD][QQ], Vv[MAXD][QQ], Du[MAXD][QQ], Dv[MAXD][QQ], Wq[MAXD][QQ];
+#pragma unroll
+            for (int a = 1; a < DD; a++)
+#pragma unroll
+                for (int c = 0; c < QQ; c++) {
+                    Vu[a][c] = tb[a]->val[lu[a]][c];
+                    Vv[a][c] = tb[a]->val[lv[a]][c];
+                    Du[a][c] = tb[a]->der[lu[a]][c];
+                    Dv[a][c] = tb[a]->der[lv[a]][c];
+                    Wq[a][c] = tb[a]->w[c];
+                }
+            const double* wv = P.wvec[ii];
+#pragma unroll 1
+            for (int c0 = 0; c0 < QQ; c0++) {
+                const double vu0 = tb[0]->val[lu[0]][c0], vv0 = tb[0]->val[lv[0]][c0];
+                const double du0 = tb[0]->der[lu[0]][c0], dv0 = tb[0]->der[lv[0]][c0], w0 = tb[0]->w[c0];
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    // this point's 1D factors per axis
+                    double pu[DD], pv[DD], qu[DD], qv[DD], pw[DD];
+                    pu[0] = vu0; pv[0] = vv0; qu[0] = du0; qv[0] = dv0; pw[0] = w0;
+                    int rr = r;
+#pragma unroll
+                    for (int a = DD - 1; a >= 1; a--) {
+                        const int c = rr % QQ;
+                        rr /= QQ;
+                        pu[a] = Vu[a][c]; pv[a] = Vv[a][c]; qu[a] = Du[a][c]; qv[a] = Dv[a][c]; pw[a] = Wq[a][c];
+                    }
+                    double val;
+                    double vu = 1, vv = 1;
+                    if (kind != 0) {
+#pragma unroll
+                        for (int a = 0; a < DD; a++) { vu *= pu[a]; vv *= pv[a]; }
+                    }
+                    if (kind == 1) {
+                        val = vu * vv * wc;
+                    } else if (kind >= 3) {
+                        double inner = 0;
+#pragma unroll
+                        for (int i = 0; i < DD; i++) {
+                            double gg = 1;
+#pragma unroll
+                            for (int j = 0; j < DD; j++)
+                                gg *= i == j ? (kind == 3 ? qu[j] : qv[j]) : (kind == 3 ? pu[j] : pv[j]);
+                            inner += gg * wv[i];
+                        }
+                        val = inner * (kind == 3 ? vv : vu);
+                    } else {
+                        double inner = 0;
+#pragma unroll
+                        for (int i = 0; i < DD; i++) {
+                            double g1 = 1, g2 = 1;
+#pragma unroll
+                            for (int j = 0; j < DD; j++) {
+                                if (i == j) { g1 *= qu[j]; g2 *= qv[j]; }
+                                else { g1 *= pu[j]; g2 *= pv[j]; }
+                            }
+                            inner += g1 * g2;
+                        }
+                        val = kind == 0 ? inner * wc : (vu * vv + inner) * wc;
+                    }
+#pragma unroll
+                    for (int a = 0; a < DD; a++) val *= pw[a];
+                    ret += val;
+                }
+            }
+            const long base = ((long)ii * P.ntuples + t) * nent;
+            E[base + (long)iv * P.nloc + iu] = ret;
+            if (!P.full) E[base + (long)iu * P.nloc + iv] = ret;
+            continue;
+        }
 #pragma unroll QN
         for (int q = 0; q < (QN > 1 ? QN : nq); q++) {
             int c[MAXD] = {0, 0, 0};
@@ -589,6 +666,248 @@ __global__ __launch_bounds__(256) void k_assemble_rows(RowParams P, long nrows, 
         }
         if (!FILL && active && lane == 0) counts[row] = written;
         if (FILL && active && lane == 0 && written != rowptr[row + 1] - base) atomicOr(bad, 1);
+    }
+}
+
+// ---- cell-major row assembly (d <= 3, k <= 2) ----
+// Per axis a row's node p lies in one or two cells lo..hi, and its candidate columns are the
+// lattice points k*lo .. k*hi + k, at offsets o = q - k*lo.  The order of the candidates (the
+// DOF order of their owner cells, which k_assemble_rows decodes per candidate) depends only on
+// the row's type per axis: (lo > 0, hi > lo).  The host tabulates, for each combination of
+// types, candidate t -> offsets (fwd, 4 bits per axis) and offsets -> t (inv).  A row then runs
+// over its own cells in lexicographic order (last axis fastest); lane l takes local node l of
+// the cell and adds the cell's integral to its candidate's sum in LDS.  The cells containing
+// both nodes of an entry are exactly the row's cells that contain the column node, visited in
+// the same lexicographic order as entry_value, so every entry is summed with the same
+// operations in the same order (bit-identical).
+constexpr int CA_MAXW = 125;   // (2k + 1)^d candidates for d = 3, k = 2
+
+template <int DD, int KK>
+struct CellAsm {
+    static constexpr int K1 = KK + 1, W1 = 2 * KK + 1;
+    static constexpr int NLOC = DD == 3 ? K1 * K1 * K1 : DD == 2 ? K1 * K1 : K1;
+    static constexpr int WMAX = DD == 3 ? W1 * W1 * W1 : DD == 2 ? W1 * W1 : W1;
+    static constexpr int NCOMBO = 1 << (2 * DD);
+};
+
+// one axis of a row's node p_a: the cells lo, lo + 1 (nc of them) containing it, its type bits,
+// and per cell j the cell's contributions to the class-tuple index, the cell index and the
+// local index of p
+struct AxInfo {
+    int lo, nc, tau;
+    unsigned T[2], C[2];
+    int IV[2];
+};
+
+template <int KK>
+__device__ inline AxInfo axis_info(int p, int n, int ncls, const int* __restrict__ cls, unsigned ctup, unsigned ccell,
+                                   int civ)
+{
+    AxInfo x;
+    int l, h;
+    if (p % KK == 0) { l = p / KK - 1; h = p / KK; } else { l = h = p / KK; }
+    l = max(l, 0);
+    h = min(h, n - 1);
+    x.lo = l;
+    x.nc = h - l + 1;
+    x.tau = (l > 0 ? 2 : 0) | (h > l ? 1 : 0);
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int c = min(l + j, h);   // j = 1 with one cell: a valid index, never used
+        x.T[j] = (ncls == 1 ? 0u : (unsigned)cls[c]) * ctup;
+        x.C[j] = (unsigned)c * ccell;
+        x.IV[j] = (p - KK * c) * civ;
+    }
+    return x;
+}
+
+// One wavefront per segment of seg_len nodes of a lattice line (every axis but the last fixed;
+// nseg segments per line); G lanes per row, 64 / G consecutive nodes at a time.  Per row the loads go out first (row bounds,
+// candidate offsets, and per cell the candidate index of the lane's node, the coefficient and
+// the element-table entries); the sums then run over the cells in order.  A candidate's sum
+// starts at -0.0: x + (-0.0) = x for every x, and a sum of nonzero terms is never -0.0 in
+// round-to-nearest, so "present" is "the sum is not -0.0" and needs no flag.
+template <int DD, int KK, int G, bool FILL, bool LDS>
+__global__ __launch_bounds__(256) void k_asm_cells(RowParams P, long nunits, int nseg, int seg_len,
+                                                  int* __restrict__ counts,
+                                                  const int* __restrict__ rowptr, int* __restrict__ cols,
+                                                  double* __restrict__ vals, int* __restrict__ bad,
+                                                  const unsigned short* __restrict__ fwd,
+                                                  const unsigned char* __restrict__ inv)
+{
+#pragma clang fp contract(off)
+    using CA = CellAsm<DD, KK>;
+    constexpr int NLOC = CA::NLOC, WMAX = CA::WMAX, K1 = CA::K1, W1 = CA::W1;
+    constexpr int R = 64 / G;                         // rows per wavefront at a time
+    constexpr int CHUNKS = (WMAX + G - 1) / G;
+    constexpr int NH = 1 << DD;                       // at most two cells per axis contain a node
+    constexpr int LAST = DD - 1;
+    static_assert(NLOC <= G && G <= 64 && WMAX <= CA_MAXW, "one lane per local node");
+    __shared__ double acc_all[(256 / G) * WMAX];
+    extern __shared__ double lds_tab[];
+    const double* E = P.E;
+    const int* cls[MAXD] = {P.cls[0], P.cls[1], P.cls[2]};
+    if constexpr (LDS) {
+        const long esize = (long)P.nint * P.ntuples * NLOC * NLOC;
+        for (long i = threadIdx.x; i < esize; i += blockDim.x) lds_tab[i] = P.E[i];
+        int* lc = reinterpret_cast<int*>(lds_tab + esize);
+        int off = 0;
+#pragma unroll
+        for (int a = 0; a < DD; a++) {
+            for (int i = threadIdx.x; i < P.g.n[a]; i += blockDim.x) lc[off + i] = P.cls[a][i];
+            cls[a] = lc + off;
+            off += P.g.n[a];
+        }
+        E = lds_tab;
+        __syncthreads();
+    }
+    const int lane = threadIdx.x % G;
+    const int sub = (threadIdx.x & 63) / G;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double* acc = acc_all + (threadIdx.x / G) * WMAX;
+    const long unit = blockIdx.x * 4L + wave;
+    if (unit >= nunits) return;   // wave-uniform
+    const long line = unit / nseg;
+    const int seg0 = (int)(unit - line * nseg) * seg_len;
+    const int wlane = threadIdx.x & 63;
+    const int gbase = wlane - lane;
+    // strides of the class-tuple index (first axis most significant), the cell index and the
+    // local node index (axis 0 fastest)
+    unsigned ctup[MAXD] = {1, 1, 1}, ccell[MAXD] = {1, 1, 1};
+    int civ[MAXD] = {1, 1, 1};
+    {
+        unsigned st = 1, sc = 1;
+#pragma unroll
+        for (int a = DD - 1; a >= 0; a--) {
+            ctup[a] = st; st *= (unsigned)P.ncls[a];
+            ccell[a] = sc; sc *= (unsigned)P.g.n[a];
+        }
+        int m = 1;
+#pragma unroll
+        for (int a = 0; a < DD; a++) { civ[a] = m; m *= K1; }
+    }
+    long tails[MAXD] = {1, 1, 1};
+    lattice_tails<DD, KK>(P.g, tails);
+    int p[MAXD] = {0, 0, 0};
+    AxInfo ax[MAXD];
+    {
+        long r = line;
+#pragma unroll
+        for (int a = LAST - 1; a >= 0; a--) {
+            const int Na = KK * P.g.n[a] + 1;
+            p[a] = (int)(r % Na);
+            r /= Na;
+        }
+#pragma unroll
+        for (int a = 0; a < LAST; a++) ax[a] = axis_info<KK>(p[a], P.g.n[a], P.ncls[a], cls[a], ctup[a], ccell[a], civ[a]);
+    }
+    // this lane's local node (axis 0 fastest) as an offset in the candidate box
+    int lofs = 0;
+    {
+        int r = lane, wm = 1;
+#pragma unroll
+        for (int a = 0; a < DD; a++) { lofs += (r % K1) * wm; r /= K1; wm *= W1; }
+    }
+    const bool node_lane = lane < NLOC;
+    const unsigned nent = (unsigned)(NLOC * NLOC);
+    const int NL = min(KK * P.g.n[LAST] + 1, seg0 + seg_len);
+    const int iters = (NL - seg0 + R - 1) / R;
+    for (int it = 0; it < iters; it++) {
+        const int pl = seg0 + it * R + sub;
+        const bool active = pl < NL;
+        p[LAST] = active ? pl : 0;
+        ax[LAST] = axis_info<KK>(p[LAST], P.g.n[LAST], P.ncls[LAST], cls[LAST], ctup[LAST], ccell[LAST], civ[LAST]);
+        int combo = 0, total = 1;
+#pragma unroll
+        for (int a = 0; a < DD; a++) {
+            combo |= ax[a].tau << (2 * a);
+            total *= KK * ax[a].nc + 1;
+        }
+        const long row = dof_of_lat_tails<DD, KK>(P.g, tails, p);
+        const int base = (FILL && active) ? rowptr[row] : 0;
+        const int lim = (FILL && active) ? rowptr[row + 1] : 0;
+        unsigned fo[CHUNKS];
+#pragma unroll
+        for (int ch = 0; ch < CHUNKS; ch++) {
+            const int t = ch * G + lane;
+            fo[ch] = (active && t < total) ? fwd[combo * WMAX + t] : 0u;
+        }
+        bool hv[NH];
+        int ht[NH];
+        unsigned hidx[NH];
+        double cf[NH];
+#pragma unroll
+        for (int h = 0; h < NH; h++) {
+            bool v = active;
+            unsigned tup = 0, cell = 0;
+            int iv = 0, hofs = 0, wm = 1;
+#pragma unroll
+            for (int a = 0; a < DD; a++) {
+                const int j = (h >> (DD - 1 - a)) & 1;   // lexicographic: last axis fastest
+                v = v && j < ax[a].nc;
+                tup += ax[a].T[j];
+                cell += ax[a].C[j];
+                iv += ax[a].IV[j];
+                hofs += j * KK * wm;
+                wm *= W1;
+            }
+            hv[h] = v;
+            // element-table index < 2^30 and cell index < 2^31 (checked on the host): 32-bit
+            hidx[h] = tup * nent + (unsigned)(iv * NLOC + lane);
+            ht[h] = (v && node_lane) ? inv[combo * WMAX + hofs + lofs] : 0;
+            cf[h] = (P.coeff && v) ? P.coeff[cell] : 0.0;
+        }
+        double integ[NH];
+#pragma unroll
+        for (int h = 0; h < NH; h++) integ[h] = 0;
+        for (int i = 0; i < P.nint; i++) {
+            const unsigned off = (unsigned)i * (unsigned)P.ntuples * nent;
+            double ev[NH];
+#pragma unroll
+            for (int h = 0; h < NH; h++) ev[h] = (hv[h] && node_lane) ? E[off + hidx[h]] : 0.0;
+#pragma unroll
+            for (int h = 0; h < NH; h++) integ[h] += P.coeff ? cf[h] * ev[h] : ev[h];
+        }
+        if (active)
+            for (int t = lane; t < WMAX; t += G) acc[t] = -0.0;
+#pragma unroll
+        for (int h = 0; h < NH; h++)
+            if (hv[h] && node_lane && integ[h] != 0) acc[ht[h]] = integ[h] + acc[ht[h]];
+        // candidates in column order: ballot-compacted writes (G lanes per row)
+        int written = 0;
+#pragma unroll
+        for (int ch = 0; ch < CHUNKS; ch++) {
+            const int t = ch * G + lane;
+            bool present = false;
+            double v = 0;
+            int colv = 0;
+            if (active && t < total) {
+                int q[MAXD] = {0, 0, 0};
+#pragma unroll
+                for (int a = 0; a < DD; a++) q[a] = KK * ax[a].lo + (int)((fo[ch] >> (4 * a)) & 15u);
+                v = acc[t];
+                present = __double_as_longlong(v) != (long long)0x8000000000000000ull;
+                if (FILL || row == 0) colv = (int)dof_of_lat_tails<DD, KK>(P.g, tails, q);
+                if (row == 0 && colv == 0 && !present) {   // dummy (0,0) entry of a new matrix
+                    present = true;
+                    v = 0.0;
+                }
+            }
+            const unsigned long long bal = __ballot(present);
+            const unsigned long long gmask = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << gbase);
+            const unsigned long long mine = bal & gmask;
+            if (FILL && present) {
+                const unsigned long long below = mine & ((1ull << wlane) - 1ull);
+                const int pos = base + written + __popcll(below);
+                if (pos < lim) {   // a row never writes past its reserved range
+                    cols[pos] = colv;
+                    vals[pos] = v;
+                }
+            }
+            written += __popcll(mine);
+        }
+        if (!FILL && active && lane == 0) counts[row] = written;
+        if (FILL && active && lane == 0 && written != lim - base) atomicOr(bad, 1);
     }
 }
 
@@ -1042,6 +1361,62 @@ int choose_lanes(double mean_row)
     return 64;
 }
 
+// Candidate tables of k_asm_cells for dimension d and degree k: for each combination of per-axis
+// row types (type = 2 [lo > 0] + [hi > lo], two bits per axis), fwd[t] = the offsets of candidate
+// t (4 bits per axis) and inv[offsets] = t.  The candidate order is the one decode_candidate
+// produces: per axis the segments of the owner cells lo - 1 (the point k*lo, when lo > 0), lo and
+// hi, the segment of axis 0 most significant, then inside the owner box axis 0 fastest.
+static void cell_asm_tables(int d, int k, std::vector<unsigned short>& fwd, std::vector<unsigned char>& inv)
+{
+    const int W1 = 2 * k + 1;
+    int wmax = 1;
+    for (int a = 0; a < d; a++) wmax *= W1;
+    const int ncombo = 1 << (2 * d);
+    fwd.assign((size_t)ncombo * wmax, 0);
+    inv.assign((size_t)ncombo * wmax, 255);
+    for (int combo = 0; combo < ncombo; combo++) {
+        int nseg[MAXD], first[MAXD][3], cnt[MAXD][3], W[MAXD];
+        for (int a = 0; a < d; a++) {
+            const int tau = (combo >> (2 * a)) & 3;
+            const bool lo_pos = tau >> 1, hi_gt = tau & 1;
+            if (lo_pos) {
+                const int f[3] = {0, 1, k + 1}, c[3] = {1, k, k};
+                nseg[a] = hi_gt ? 3 : 2;
+                for (int s = 0; s < 3; s++) { first[a][s] = f[s]; cnt[a][s] = c[s]; }
+            } else {
+                const int f[3] = {0, k + 1, 0}, c[3] = {k + 1, k, 0};
+                nseg[a] = hi_gt ? 2 : 1;
+                for (int s = 0; s < 3; s++) { first[a][s] = f[s]; cnt[a][s] = c[s]; }
+            }
+            W[a] = 0;
+            for (int s = 0; s < nseg[a]; s++) W[a] += cnt[a][s];
+        }
+        int tailW[MAXD], tot = 1;
+        for (int a = d - 1; a >= 0; a--) { tailW[a] = tot; tot *= W[a]; }
+        for (int t = 0; t < tot; t++) {
+            int rem = t, M = 1, cs[MAXD], fs[MAXD];
+            for (int a = 0; a < d; a++) {
+                int s = 0;
+                while (s < nseg[a] - 1 && rem >= M * cnt[a][s] * tailW[a]) { rem -= M * cnt[a][s] * tailW[a]; s++; }
+                cs[a] = cnt[a][s];
+                fs[a] = first[a][s];
+                M *= cs[a];
+            }
+            unsigned packed = 0;
+            int oidx = 0, wm = 1;
+            for (int a = 0; a < d; a++) {
+                const int o = fs[a] + rem % cs[a];
+                rem /= cs[a];
+                packed |= (unsigned)o << (4 * a);
+                oidx += o * wm;
+                wm *= W1;
+            }
+            fwd[(size_t)combo * wmax + t] = (unsigned short)packed;
+            inv[(size_t)combo * wmax + oidx] = (unsigned char)t;
+        }
+    }
+}
+
 jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const double* weights, const double* coeff_host,
                           int rule)
 {
@@ -1144,21 +1519,64 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
         JFEM_HIP(hipGetLastError());
     };
     auto run2 = [&](auto kern_lds, auto kern_glob) { use_lds ? run(kern_lds) : run(kern_glob); };
+    // k <= 2: the cell-major kernel, on a grid of co-resident blocks (each block stages the
+    // candidate tables once); element tables and cell classes in LDS when they fit in 16 KiB
+    const bool cellwise = g.k <= 2;
+    std::vector<unsigned short> fwd_h;
+    std::vector<unsigned char> inv_h;
+    DevBuf<unsigned short> fwd_d;
+    DevBuf<unsigned char> inv_d;
+    if (cellwise) {
+        cell_asm_tables(g.d, g.k, fwd_h, inv_h);
+        fwd_d.alloc(fwd_h.size());
+        inv_d.alloc(inv_h.size());
+        JFEM_HIP(hipMemcpyAsync(fwd_d, fwd_h.data(), sizeof(unsigned short) * fwd_h.size(), hipMemcpyHostToDevice,
+                                c->stream));
+        JFEM_HIP(hipMemcpyAsync(inv_d, inv_h.data(), inv_h.size(), hipMemcpyHostToDevice, c->stream));
+    }
+    const bool cell_lds = lds_bytes <= 16 * 1024;
+    auto run_cells = [&](auto kern, int GG) {
+        const size_t dyn = cell_lds ? lds_bytes : 0;
+        long nlines = 1;
+        for (int a = 0; a + 1 < g.d; a++) nlines *= (long)g.k * g.n[a] + 1;
+        // lines split into segments (a multiple of the rows per wavefront) until there are
+        // about 32 wavefronts per CU
+        const int rpw = 64 / GG;
+        const int nl = g.k * g.n[g.d - 1] + 1;
+        const long want = (long)c->cu_count * 32;
+        int nseg = (int)std::min<long>((want + nlines - 1) / nlines, (nl + rpw - 1) / rpw);
+        nseg = std::max(nseg, 1);
+        int seg_len = ((nl + nseg - 1) / nseg + rpw - 1) / rpw * rpw;
+        nseg = (nl + seg_len - 1) / seg_len;
+        const long nunits = nlines * nseg;
+        const long nb = (nunits + 3) / 4;
+        JFEM_CHECK(nb < (1l << 31), JFEM_ERR_UNSUPPORTED, "too many lattice lines");
+        kern<<<(int)nb, 256, dyn, c->stream>>>(RP, nunits, nseg, seg_len, counts, A->rowptr, A->colidx, A->vals,
+                                               bad, fwd_d, inv_d);
+        JFEM_HIP(hipGetLastError());
+    };
+    auto run_cells2 = [&](auto kern_lds, auto kern_glob, int GG) {
+        cell_lds ? run_cells(kern_lds, GG) : run_cells(kern_glob, GG);
+    };
 #define JFEM_ASM_CASE(DD, KK, GG)                                                                   \
     fill ? run2(k_assemble_rows<DD, KK, GG, true, true>, k_assemble_rows<DD, KK, GG, true, false>)  \
          : run2(k_assemble_rows<DD, KK, GG, false, true>, k_assemble_rows<DD, KK, GG, false, false>)
-    // G follows from (d, k): (2k+1)^d candidates per row
+#define JFEM_CELL_CASE(DD, KK, GG)                                                                            \
+    fill ? run_cells2(k_asm_cells<DD, KK, GG, true, true>, k_asm_cells<DD, KK, GG, true, false>, GG)          \
+         : run_cells2(k_asm_cells<DD, KK, GG, false, true>, k_asm_cells<DD, KK, GG, false, false>, GG)
+    // k_assemble_rows: G follows from (d, k), (2k+1)^d candidates per row; k_asm_cells: the
+    // smallest power of two >= (k+1)^d lanes per row
     auto launch = [&](bool fill) {
         const int dk = g.d * 10 + g.k;
         switch (dk) {
-        case 11: JFEM_ASM_CASE(1, 1, 16); break;
-        case 12: JFEM_ASM_CASE(1, 2, 16); break;
+        case 11: JFEM_CELL_CASE(1, 1, 4); break;
+        case 12: JFEM_CELL_CASE(1, 2, 4); break;
         case 13: JFEM_ASM_CASE(1, 3, 16); break;
-        case 21: JFEM_ASM_CASE(2, 1, 16); break;
-        case 22: JFEM_ASM_CASE(2, 2, 32); break;
+        case 21: JFEM_CELL_CASE(2, 1, 4); break;
+        case 22: JFEM_CELL_CASE(2, 2, 16); break;
         case 23: JFEM_ASM_CASE(2, 3, 64); break;
-        case 31: JFEM_ASM_CASE(3, 1, 32); break;
-        case 32: JFEM_ASM_CASE(3, 2, 64); break;
+        case 31: JFEM_CELL_CASE(3, 1, 8); break;
+        case 32: JFEM_CELL_CASE(3, 2, 64); break;
         case 33: JFEM_ASM_CASE(3, 3, 64); break;
         default:
             if (G == 16) JFEM_ASM_CASE(0, 0, 16);
@@ -1167,6 +1585,7 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
         }
     };
 #undef JFEM_ASM_CASE
+#undef JFEM_CELL_CASE
     // With one integral, no exactly-zero element entry, and coefficients whose
     // products with the entries cannot round to zero, every cell contribution
     // is nonzero, so every structural candidate is in the pattern and the row

@@ -45,6 +45,9 @@ ASM_CASES = [
     ([4, 3], 3, [0, 1], [1.0, 1.0]),
     ([3, 3], 2, [2], [1.5]),                     # H1
     ([9], 2, [1], [1.0]),
+    ([12, 10, 9], 2, [0], [1.0]),                # 3D Q2 with interior rows of every type, 9-12 cell classes
+    ([10, 9, 8], 1, [0, 1], [1.0, 2.0]),
+    ([40, 33], 2, [2], [0.5]),
 ]
 
 
