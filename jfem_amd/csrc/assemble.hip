@@ -721,19 +721,33 @@ __device__ inline AxInfo axis_info(int p, int n, int ncls, const int* __restrict
     return x;
 }
 
+// class of an axis of a row whose cells lo, lo + 1 both have index >= 2 (an "inner" row): 0 a
+// vertex between two cells, 1 a node inside a cell, 2 the last vertex of the axis
+template <int KK>
+__device__ inline int axis_class(const AxInfo& x, int p)
+{
+    return x.tau == 3 ? 0 : (p - KK * x.lo < KK ? 1 : 2);
+}
+
 // One wavefront per segment of seg_len nodes of a lattice line (every axis but the last fixed;
-// nseg segments per line); G lanes per row, 64 / G consecutive nodes at a time.  Per row the loads go out first (row bounds,
-// candidate offsets, and per cell the candidate index of the lane's node, the coefficient and
-// the element-table entries); the sums then run over the cells in order.  A candidate's sum
-// starts at -0.0: x + (-0.0) = x for every x, and a sum of nonzero terms is never -0.0 in
-// round-to-nearest, so "present" is "the sum is not -0.0" and needs no flag.
-template <int DD, int KK, int G, bool FILL, bool LDS>
+// nseg segments per line); G lanes per row, 64 / G consecutive nodes at a time.  The fixed
+// axes' parts of the row's cells are computed once per line.  Per row the loads go out first
+// (row bounds, candidate offsets, and per cell of the row the candidate index of the lane's
+// node, the coefficient and the element-table entry); the sums then run over the cells in
+// order.  A candidate's sum starts at -0.0: x + (-0.0) = x for every x, and a sum of nonzero
+// terms is never -0.0 in round-to-nearest, so "present" is "the sum is not -0.0".
+// Columns: on an inner row (every axis' cells have index >= 2, so every candidate's owner cell
+// has index >= 1) col - row depends only on the row's axis classes and the candidate (the DOF
+// is sum_a k^a (q_a tail_a + (tail_a - 1)(1 - l_a)) there, l_a the node's index in its owner
+// cell), tabulated on the host in delta; other rows evaluate the closed form per candidate.
+template <int DD, int KK, int G, bool FILL, bool LDS, bool COEF>
 __global__ __launch_bounds__(256) void k_asm_cells(RowParams P, long nunits, int nseg, int seg_len,
                                                   int* __restrict__ counts,
                                                   const int* __restrict__ rowptr, int* __restrict__ cols,
                                                   double* __restrict__ vals, int* __restrict__ bad,
                                                   const unsigned short* __restrict__ fwd,
-                                                  const unsigned char* __restrict__ inv)
+                                                  const unsigned char* __restrict__ inv,
+                                                  const int* __restrict__ delta, int inner_ok)
 {
 #pragma clang fp contract(off)
     using CA = CellAsm<DD, KK>;
@@ -741,16 +755,22 @@ __global__ __launch_bounds__(256) void k_asm_cells(RowParams P, long nunits, int
     constexpr int R = 64 / G;                         // rows per wavefront at a time
     constexpr int CHUNKS = (WMAX + G - 1) / G;
     constexpr int NH = 1 << DD;                       // at most two cells per axis contain a node
+    constexpr int NF = NH / 2;                        // cell slots of the fixed axes
     constexpr int LAST = DD - 1;
-    static_assert(NLOC <= G && G <= 64 && WMAX <= CA_MAXW, "one lane per local node");
-    __shared__ double acc_all[(256 / G) * WMAX];
+    constexpr int WLAST = DD == 3 ? W1 * W1 : DD == 2 ? W1 : 1;   // W1^LAST
+    constexpr int C3LAST = DD == 3 ? 9 : DD == 2 ? 3 : 1;         // 3^LAST
+    // per row group: the candidates' sums and one slot (STRIDE - 1) for the lanes that hold no
+    // node of a cell, so that every lane can take part in the sums' stores
+    constexpr int STRIDE = (WMAX + 1 + G - 1) / G * G;
+    static_assert(NLOC <= G && G <= 64 && WMAX <= CA_MAXW && CHUNKS * G <= STRIDE, "one lane per local node");
+    __shared__ double acc_all[(256 / G) * STRIDE];
     extern __shared__ double lds_tab[];
     const double* E = P.E;
     const int* cls[MAXD] = {P.cls[0], P.cls[1], P.cls[2]};
     if constexpr (LDS) {
         const long esize = (long)P.nint * P.ntuples * NLOC * NLOC;
         for (long i = threadIdx.x; i < esize; i += blockDim.x) lds_tab[i] = P.E[i];
-        int* lc = reinterpret_cast<int*>(lds_tab + esize);
+        int* lc = reinterpret_cast<int*>(lds_tab + esize + 64);   // after E's padding
         int off = 0;
 #pragma unroll
         for (int a = 0; a < DD; a++) {
@@ -764,7 +784,7 @@ __global__ __launch_bounds__(256) void k_asm_cells(RowParams P, long nunits, int
     const int lane = threadIdx.x % G;
     const int sub = (threadIdx.x & 63) / G;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    double* acc = acc_all + (threadIdx.x / G) * WMAX;
+    double* acc = acc_all + (threadIdx.x / G) * STRIDE;
     const long unit = blockIdx.x * 4L + wave;
     if (unit >= nunits) return;   // wave-uniform
     const long line = unit / nseg;
@@ -801,6 +821,44 @@ __global__ __launch_bounds__(256) void k_asm_cells(RowParams P, long nunits, int
 #pragma unroll
         for (int a = 0; a < LAST; a++) ax[a] = axis_info<KK>(p[a], P.g.n[a], P.ncls[a], cls[a], ctup[a], ccell[a], civ[a]);
     }
+    // the fixed axes' part of each cell slot hf (their cell bits, first axis most significant)
+    unsigned TF[NF], CFi[NF];
+    int IVF[NF], HF[NF];
+    bool VF[NF];
+#pragma unroll
+    for (int hf = 0; hf < NF; hf++) {
+        unsigned tt = 0, cc = 0;
+        int iv = 0, ho = 0, wm = 1;
+        bool v = true;
+#pragma unroll
+        for (int a = 0; a < LAST; a++) {
+            const int j = (hf >> (LAST - 1 - a)) & 1;
+            v = v && j < ax[a].nc;
+            tt += ax[a].T[j];
+            cc += ax[a].C[j];
+            iv += ax[a].IV[j];
+            ho += j * KK * wm;
+            wm *= W1;
+        }
+        TF[hf] = tt; CFi[hf] = cc; IVF[hf] = iv; HF[hf] = ho; VF[hf] = v;
+    }
+    int combo_f = 0, total_f = 1, class_f = 0, cw = 1;
+    bool inner_f = inner_ok != 0;
+#pragma unroll
+    for (int a = 0; a < LAST; a++) {
+        combo_f |= ax[a].tau << (2 * a);
+        total_f *= KK * ax[a].nc + 1;
+        inner_f = inner_f && ax[a].lo >= 2;
+        class_f += axis_class<KK>(ax[a], p[a]) * cw;
+        cw *= 3;
+    }
+    // the row's DOF along the line: dof(p) = dof(p with p_last = 0) + mprod * p_last, mprod the
+    // product over the fixed axes of the owner-cell extents (k, or k + 1 in cell 0)
+    p[LAST] = 0;
+    const long rbase = dof_of_lat_tails<DD, KK>(P.g, tails, p);
+    int rstep = 1;
+#pragma unroll
+    for (int a = 0; a < LAST; a++) rstep *= (p[a] == 0 ? 0 : (p[a] - 1) / KK) > 0 ? KK : KK + 1;
     // this lane's local node (axis 0 fastest) as an offset in the candidate box
     int lofs = 0;
     {
@@ -810,69 +868,71 @@ __global__ __launch_bounds__(256) void k_asm_cells(RowParams P, long nunits, int
     }
     const bool node_lane = lane < NLOC;
     const unsigned nent = (unsigned)(NLOC * NLOC);
+    const unsigned ioff = (unsigned)P.ntuples * nent;   // between the integrals' tables
     const int NL = min(KK * P.g.n[LAST] + 1, seg0 + seg_len);
     const int iters = (NL - seg0 + R - 1) / R;
     for (int it = 0; it < iters; it++) {
         const int pl = seg0 + it * R + sub;
         const bool active = pl < NL;
-        p[LAST] = active ? pl : 0;
-        ax[LAST] = axis_info<KK>(p[LAST], P.g.n[LAST], P.ncls[LAST], cls[LAST], ctup[LAST], ccell[LAST], civ[LAST]);
-        int combo = 0, total = 1;
-#pragma unroll
-        for (int a = 0; a < DD; a++) {
-            combo |= ax[a].tau << (2 * a);
-            total *= KK * ax[a].nc + 1;
+        p[LAST] = active ? pl : seg0;
+        const AxInfo xl = axis_info<KK>(p[LAST], P.g.n[LAST], P.ncls[LAST], cls[LAST], ctup[LAST], ccell[LAST],
+                                        civ[LAST]);
+        const int combo = combo_f | (xl.tau << (2 * LAST));
+        const int total = total_f * (KK * xl.nc + 1);
+        const long row = rbase + (long)rstep * p[LAST];
+        const bool inner = inner_f && xl.lo >= 2;
+        const int dcls = class_f + axis_class<KK>(xl, p[LAST]) * C3LAST;
+        int base = 0, lim = 0;
+        if (FILL) {
+            base = rowptr[row];
+            lim = rowptr[row + 1];
+            if (!active) base = lim = 0;
         }
-        const long row = dof_of_lat_tails<DD, KK>(P.g, tails, p);
-        const int base = (FILL && active) ? rowptr[row] : 0;
-        const int lim = (FILL && active) ? rowptr[row + 1] : 0;
         unsigned fo[CHUNKS];
+        int dl[CHUNKS];
 #pragma unroll
         for (int ch = 0; ch < CHUNKS; ch++) {
-            const int t = ch * G + lane;
-            fo[ch] = (active && t < total) ? fwd[combo * WMAX + t] : 0u;
+            fo[ch] = fwd[combo * WMAX + ch * G + lane];
+            dl[ch] = delta[(inner ? dcls : 0) * WMAX + ch * G + lane];
         }
+        // the row's cells: slot h = (fixed-axes slot hf, last-axis bit j), lexicographic order.
+        // With one row per wavefront a slot's validity is uniform: skipped slots cost a branch.
+        // All lanes load (lanes past the cell's nodes read padding) and all take part in the
+        // sums; those lanes add into the spare slot.
         bool hv[NH];
         int ht[NH];
-        unsigned hidx[NH];
-        double cf[NH];
+        double ev[NH], cf[NH];
 #pragma unroll
         for (int h = 0; h < NH; h++) {
-            bool v = active;
-            unsigned tup = 0, cell = 0;
-            int iv = 0, hofs = 0, wm = 1;
-#pragma unroll
-            for (int a = 0; a < DD; a++) {
-                const int j = (h >> (DD - 1 - a)) & 1;   // lexicographic: last axis fastest
-                v = v && j < ax[a].nc;
-                tup += ax[a].T[j];
-                cell += ax[a].C[j];
-                iv += ax[a].IV[j];
-                hofs += j * KK * wm;
-                wm *= W1;
+            const int hf = h >> 1, j = h & 1;
+            hv[h] = active && VF[hf] && j < xl.nc;
+            if (hv[h]) {
+                const unsigned tup = TF[hf] + xl.T[j];
+                const int iv = IVF[hf] + xl.IV[j];
+                cf[h] = COEF ? P.coeff[CFi[hf] + xl.C[j]] : 1.0;
+                ht[h] = inv[combo * WMAX + HF[hf] + j * KK * WLAST + lofs];
+                // element-table index < 2^30 (checked on the host): 32-bit
+                ev[h] = E[tup * nent + (unsigned)(iv * NLOC + lane)];
             }
-            hv[h] = v;
-            // element-table index < 2^30 and cell index < 2^31 (checked on the host): 32-bit
-            hidx[h] = tup * nent + (unsigned)(iv * NLOC + lane);
-            ht[h] = (v && node_lane) ? inv[combo * WMAX + hofs + lofs] : 0;
-            cf[h] = (P.coeff && v) ? P.coeff[cell] : 0.0;
         }
-        double integ[NH];
+        for (int t = lane; t < STRIDE; t += G) acc[t] = -0.0;
 #pragma unroll
-        for (int h = 0; h < NH; h++) integ[h] = 0;
-        for (int i = 0; i < P.nint; i++) {
-            const unsigned off = (unsigned)i * (unsigned)P.ntuples * nent;
-            double ev[NH];
-#pragma unroll
-            for (int h = 0; h < NH; h++) ev[h] = (hv[h] && node_lane) ? E[off + hidx[h]] : 0.0;
-#pragma unroll
-            for (int h = 0; h < NH; h++) integ[h] += P.coeff ? cf[h] * ev[h] : ev[h];
+        for (int h = 0; h < NH; h++) {
+            if (hv[h]) {
+                double x = 0;
+                x += COEF ? cf[h] * ev[h] : ev[h];
+                for (int i = 1; i < P.nint; i++) {   // further integrals: E of integral i at i * ioff
+                    const int hf = h >> 1, j = h & 1;
+                    const unsigned tup = TF[hf] + xl.T[j];
+                    const int iv = IVF[hf] + xl.IV[j];
+                    const double e = E[(unsigned)i * ioff + tup * nent + (unsigned)(iv * NLOC + lane)];
+                    x += COEF ? cf[h] * e : e;
+                }
+                const int t = node_lane ? ht[h] : STRIDE - 1;
+                const double old = acc[t];
+                acc[t] = x != 0 ? x + old : old;
+            }
         }
-        if (active)
-            for (int t = lane; t < WMAX; t += G) acc[t] = -0.0;
-#pragma unroll
-        for (int h = 0; h < NH; h++)
-            if (hv[h] && node_lane && integ[h] != 0) acc[ht[h]] = integ[h] + acc[ht[h]];
         // candidates in column order: ballot-compacted writes (G lanes per row)
         int written = 0;
 #pragma unroll
@@ -881,13 +941,20 @@ __global__ __launch_bounds__(256) void k_asm_cells(RowParams P, long nunits, int
             bool present = false;
             double v = 0;
             int colv = 0;
+            v = acc[t];
             if (active && t < total) {
-                int q[MAXD] = {0, 0, 0};
-#pragma unroll
-                for (int a = 0; a < DD; a++) q[a] = KK * ax[a].lo + (int)((fo[ch] >> (4 * a)) & 15u);
-                v = acc[t];
                 present = __double_as_longlong(v) != (long long)0x8000000000000000ull;
-                if (FILL || row == 0) colv = (int)dof_of_lat_tails<DD, KK>(P.g, tails, q);
+                if (FILL || row == 0) {
+                    if (inner) {
+                        colv = (int)row + dl[ch];
+                    } else {
+                        int q[MAXD] = {0, 0, 0};
+                        q[LAST] = KK * xl.lo + (int)((fo[ch] >> (4 * LAST)) & 15u);
+#pragma unroll
+                        for (int a = 0; a < LAST; a++) q[a] = KK * ax[a].lo + (int)((fo[ch] >> (4 * a)) & 15u);
+                        colv = (int)dof_of_lat_tails<DD, KK>(P.g, tails, q);
+                    }
+                }
                 if (row == 0 && colv == 0 && !present) {   // dummy (0,0) entry of a new matrix
                     present = true;
                     v = 0.0;
@@ -1417,6 +1484,44 @@ static void cell_asm_tables(int d, int k, std::vector<unsigned short>& fwd, std:
     }
 }
 
+// Column offsets of k_asm_cells' inner rows: delta[cls * wmax + t] = dof(q_t) - dof(p) for a
+// representative row p of each class combination (per axis 0: a vertex between two cells, 1: a
+// node inside a cell, 2: the last vertex; their cells lo, lo + 1 with index >= 2).  Returns
+// false (no inner rows) when an axis has fewer than 4 cells.
+static bool cell_asm_delta(const Grid& g, const std::vector<unsigned short>& fwd, std::vector<int>& delta)
+{
+    const int d = g.d, k = g.k, W1 = 2 * k + 1;
+    int wmax = 1, ncls = 1;
+    for (int a = 0; a < d; a++) { wmax *= W1; ncls *= 3; }
+    delta.assign((size_t)ncls * wmax + 64, 0);
+    for (int a = 0; a < d; a++)
+        if (g.n[a] < 4) return false;
+    const GridDev gd = to_dev(g);
+    for (int cl = 0; cl < ncls; cl++) {
+        int p[MAXD] = {0, 0, 0}, lo[MAXD] = {0, 0, 0}, combo = 0, total = 1, r = cl;
+        bool ok = true;
+        for (int a = 0; a < d; a++) {
+            const int ca = r % 3;
+            r /= 3;
+            int nc;
+            if (ca == 0) { p[a] = 3 * k; lo[a] = 2; nc = 2; }
+            else if (ca == 1) { p[a] = 2 * k + 1; lo[a] = 2; nc = 1; ok = ok && k >= 2; }
+            else { p[a] = k * g.n[a]; lo[a] = g.n[a] - 1; nc = 1; }
+            combo |= (nc == 2 ? 3 : 2) << (2 * a);
+            total *= k * nc + 1;
+        }
+        if (!ok) continue;   // no nodes inside a cell for k = 1
+        const long row = dof_of_lattice(gd, p);
+        for (int t = 0; t < total; t++) {
+            const unsigned f = fwd[(size_t)combo * wmax + t];
+            int q[MAXD] = {0, 0, 0};
+            for (int a = 0; a < d; a++) q[a] = k * lo[a] + (int)((f >> (4 * a)) & 15u);
+            delta[(size_t)cl * wmax + t] = (int)(dof_of_lattice(gd, q) - row);
+        }
+    }
+    return true;
+}
+
 jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const double* weights, const double* coeff_host,
                           int rule)
 {
@@ -1459,7 +1564,9 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     const long Esize = (long)nint * EP.ntuples * EP.nloc * EP.nloc;
     JFEM_CHECK(Esize < (1l << 30), JFEM_ERR_UNSUPPORTED, "too many distinct cell geometries");
     double* E = nullptr;
-    JFEM_HIP(hipMalloc(&E, sizeof(double) * Esize));
+    // 64 doubles of padding: k_asm_cells reads whole wavefronts past a table row's end
+    JFEM_HIP(hipMalloc(&E, sizeof(double) * (Esize + 64)));
+    JFEM_HIP(hipMemsetAsync(E + Esize, 0, sizeof(double) * 64, c->stream));
     {
         // node pairs iv <= iu (all pairs for non-symmetric integrals), packed iv << 16 | iu
         std::vector<int> pairs;
@@ -1524,19 +1631,28 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     const bool cellwise = g.k <= 2;
     std::vector<unsigned short> fwd_h;
     std::vector<unsigned char> inv_h;
+    std::vector<int> delta_h;
     DevBuf<unsigned short> fwd_d;
     DevBuf<unsigned char> inv_d;
+    DevBuf<int> delta_d;
+    int inner_ok = 0;
     if (cellwise) {
         cell_asm_tables(g.d, g.k, fwd_h, inv_h);
+        fwd_h.resize(fwd_h.size() + 64, 0);   // whole wavefronts read past the last combination's entries
+        inner_ok = cell_asm_delta(g, fwd_h, delta_h) ? 1 : 0;
         fwd_d.alloc(fwd_h.size());
         inv_d.alloc(inv_h.size());
+        delta_d.alloc(delta_h.size());
         JFEM_HIP(hipMemcpyAsync(fwd_d, fwd_h.data(), sizeof(unsigned short) * fwd_h.size(), hipMemcpyHostToDevice,
                                 c->stream));
         JFEM_HIP(hipMemcpyAsync(inv_d, inv_h.data(), inv_h.size(), hipMemcpyHostToDevice, c->stream));
+        JFEM_HIP(hipMemcpyAsync(delta_d, delta_h.data(), sizeof(int) * delta_h.size(), hipMemcpyHostToDevice,
+                                c->stream));
     }
-    const bool cell_lds = lds_bytes <= 16 * 1024;
+    const size_t cell_lds_bytes = lds_bytes + sizeof(double) * 64;   // the same padding as E's
+    const bool cell_lds = cell_lds_bytes <= 16 * 1024;
     auto run_cells = [&](auto kern, int GG) {
-        const size_t dyn = cell_lds ? lds_bytes : 0;
+        const size_t dyn = cell_lds ? cell_lds_bytes : 0;
         long nlines = 1;
         for (int a = 0; a + 1 < g.d; a++) nlines *= (long)g.k * g.n[a] + 1;
         // lines split into segments (a multiple of the rows per wavefront) until there are
@@ -1552,18 +1668,21 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
         const long nb = (nunits + 3) / 4;
         JFEM_CHECK(nb < (1l << 31), JFEM_ERR_UNSUPPORTED, "too many lattice lines");
         kern<<<(int)nb, 256, dyn, c->stream>>>(RP, nunits, nseg, seg_len, counts, A->rowptr, A->colidx, A->vals,
-                                               bad, fwd_d, inv_d);
+                                               bad, fwd_d, inv_d, delta_d, inner_ok);
         JFEM_HIP(hipGetLastError());
     };
-    auto run_cells2 = [&](auto kern_lds, auto kern_glob, int GG) {
-        cell_lds ? run_cells(kern_lds, GG) : run_cells(kern_glob, GG);
+    auto run_cells4 = [&](auto kern_lds_c, auto kern_lds, auto kern_glob_c, auto kern_glob, int GG) {
+        if (cell_lds) coeff ? run_cells(kern_lds_c, GG) : run_cells(kern_lds, GG);
+        else coeff ? run_cells(kern_glob_c, GG) : run_cells(kern_glob, GG);
     };
 #define JFEM_ASM_CASE(DD, KK, GG)                                                                   \
     fill ? run2(k_assemble_rows<DD, KK, GG, true, true>, k_assemble_rows<DD, KK, GG, true, false>)  \
          : run2(k_assemble_rows<DD, KK, GG, false, true>, k_assemble_rows<DD, KK, GG, false, false>)
-#define JFEM_CELL_CASE(DD, KK, GG)                                                                            \
-    fill ? run_cells2(k_asm_cells<DD, KK, GG, true, true>, k_asm_cells<DD, KK, GG, true, false>, GG)          \
-         : run_cells2(k_asm_cells<DD, KK, GG, false, true>, k_asm_cells<DD, KK, GG, false, false>, GG)
+#define JFEM_CELL_CASE(DD, KK, GG)                                                                           \
+    fill ? run_cells4(k_asm_cells<DD, KK, GG, true, true, true>, k_asm_cells<DD, KK, GG, true, true, false>,     \
+                      k_asm_cells<DD, KK, GG, true, false, true>, k_asm_cells<DD, KK, GG, true, false, false>, GG) \
+         : run_cells4(k_asm_cells<DD, KK, GG, false, true, true>, k_asm_cells<DD, KK, GG, false, true, false>,   \
+                      k_asm_cells<DD, KK, GG, false, false, true>, k_asm_cells<DD, KK, GG, false, false, false>, GG)
     // k_assemble_rows: G follows from (d, k), (2k+1)^d candidates per row; k_asm_cells: the
     // smallest power of two >= (k+1)^d lanes per row
     auto launch = [&](bool fill) {
