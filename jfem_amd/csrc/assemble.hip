@@ -690,48 +690,28 @@ struct CellAsm {
     static constexpr int NCOMBO = 1 << (2 * DD);
 };
 
-// one axis of a row's node p_a: the cells lo, lo + 1 (nc of them) containing it, its type bits,
-// and per cell j the cell's contributions to the class-tuple index, the cell index and the
-// local index of p
-struct AxInfo {
-    int lo, nc, tau;
-    unsigned T[2], C[2];
-    int IV[2];
+// What a row needs of one axis a at its lattice coordinate p (tabulated on the host by
+// cell_asm_axes, 32 bytes, so one scalar load brings it): the first cell lo containing p; info =
+// nc | tau << 2 | class << 4 with nc the cells containing p (1 or 2), tau the type bits
+// 2 [lo > 0] + [hi > lo], class 0 a vertex between two cells, 1 a node inside a cell, 2 the last
+// vertex (when lo >= 2), 3 (lo < 2); and per cell j < nc its part of the element-table offset
+// (cell class * stride of the class-tuple index * (k+1)^2d + local index of p * (k+1)^d) and of
+// the cell index.
+struct __align__(32) AxEntry {
+    int lo, info;
+    unsigned L[2], C[2];
+    int pad[2];
 };
 
-template <int KK>
-__device__ inline AxInfo axis_info(int p, int n, int ncls, const int* __restrict__ cls, unsigned ctup, unsigned ccell,
-                                   int civ)
-{
-    AxInfo x;
-    int l, h;
-    if (p % KK == 0) { l = p / KK - 1; h = p / KK; } else { l = h = p / KK; }
-    l = max(l, 0);
-    h = min(h, n - 1);
-    x.lo = l;
-    x.nc = h - l + 1;
-    x.tau = (l > 0 ? 2 : 0) | (h > l ? 1 : 0);
-#pragma unroll
-    for (int j = 0; j < 2; j++) {
-        const int c = min(l + j, h);   // j = 1 with one cell: a valid index, never used
-        x.T[j] = (ncls == 1 ? 0u : (unsigned)cls[c]) * ctup;
-        x.C[j] = (unsigned)c * ccell;
-        x.IV[j] = (p - KK * c) * civ;
-    }
-    return x;
-}
-
-// class of an axis of a row whose cells lo, lo + 1 both have index >= 2 (an "inner" row): 0 a
-// vertex between two cells, 1 a node inside a cell, 2 the last vertex of the axis
-template <int KK>
-__device__ inline int axis_class(const AxInfo& x, int p)
-{
-    return x.tau == 3 ? 0 : (p - KK * x.lo < KK ? 1 : 2);
-}
+struct AxOffsets {
+    int o[MAXD];
+};
 
 // One wavefront per segment of seg_len nodes of a lattice line (every axis but the last fixed;
-// nseg segments per line); G lanes per row, 64 / G consecutive nodes at a time.  The fixed
-// axes' parts of the row's cells are computed once per line.  Per row the loads go out first
+// nseg segments per line; line_order lists the lines grouped by the classes of their fixed axes'
+// cells, so that the lines in flight together read few element tables, which then stay in L2);
+// G lanes per row, 64 / G consecutive nodes at a time.  The fixed
+// axes' parts of the row's cells are summed once per line.  Per row the loads go out first
 // (row bounds, candidate offsets, and per cell of the row the candidate index of the lane's
 // node, the coefficient and the element-table entry); the sums then run over the cells in
 // order.  A candidate's sum starts at -0.0: x + (-0.0) = x for every x, and a sum of nonzero
@@ -747,11 +727,12 @@ __global__ __launch_bounds__(256) void k_asm_cells(RowParams P, long nunits, int
                                                   double* __restrict__ vals, int* __restrict__ bad,
                                                   const unsigned short* __restrict__ fwd,
                                                   const unsigned char* __restrict__ inv,
-                                                  const int* __restrict__ delta, int inner_ok)
+                                                  const int* __restrict__ delta, const AxEntry* __restrict__ axtab,
+                                                  AxOffsets aoff, const int* __restrict__ line_order, int inner_ok)
 {
 #pragma clang fp contract(off)
     using CA = CellAsm<DD, KK>;
-    constexpr int NLOC = CA::NLOC, WMAX = CA::WMAX, K1 = CA::K1, W1 = CA::W1;
+    constexpr int NLOC = CA::NLOC, WMAX = CA::WMAX, W1 = CA::W1, K1 = CA::K1;
     constexpr int R = 64 / G;                         // rows per wavefront at a time
     constexpr int CHUNKS = (WMAX + G - 1) / G;
     constexpr int NH = 1 << DD;                       // at most two cells per axis contain a node
@@ -763,21 +744,12 @@ __global__ __launch_bounds__(256) void k_asm_cells(RowParams P, long nunits, int
     // node of a cell, so that every lane can take part in the sums' stores
     constexpr int STRIDE = (WMAX + 1 + G - 1) / G * G;
     static_assert(NLOC <= G && G <= 64 && WMAX <= CA_MAXW && CHUNKS * G <= STRIDE, "one lane per local node");
-    __shared__ double acc_all[(256 / G) * STRIDE];
+    __shared__ double acc_all[2 * (256 / G) * STRIDE];   // two rows in flight per row group
     extern __shared__ double lds_tab[];
     const double* E = P.E;
-    const int* cls[MAXD] = {P.cls[0], P.cls[1], P.cls[2]};
     if constexpr (LDS) {
         const long esize = (long)P.nint * P.ntuples * NLOC * NLOC;
         for (long i = threadIdx.x; i < esize; i += blockDim.x) lds_tab[i] = P.E[i];
-        int* lc = reinterpret_cast<int*>(lds_tab + esize + 64);   // after E's padding
-        int off = 0;
-#pragma unroll
-        for (int a = 0; a < DD; a++) {
-            for (int i = threadIdx.x; i < P.g.n[a]; i += blockDim.x) lc[off + i] = P.cls[a][i];
-            cls[a] = lc + off;
-            off += P.g.n[a];
-        }
         E = lds_tab;
         __syncthreads();
     }
@@ -787,29 +759,15 @@ __global__ __launch_bounds__(256) void k_asm_cells(RowParams P, long nunits, int
     double* acc = acc_all + (threadIdx.x / G) * STRIDE;
     const long unit = blockIdx.x * 4L + wave;
     if (unit >= nunits) return;   // wave-uniform
-    const long line = unit / nseg;
-    const int seg0 = (int)(unit - line * nseg) * seg_len;
+    const long li = unit / nseg;
+    const long line = line_order[li];
+    const int seg0 = (int)(unit - li * nseg) * seg_len;
     const int wlane = threadIdx.x & 63;
     const int gbase = wlane - lane;
-    // strides of the class-tuple index (first axis most significant), the cell index and the
-    // local node index (axis 0 fastest)
-    unsigned ctup[MAXD] = {1, 1, 1}, ccell[MAXD] = {1, 1, 1};
-    int civ[MAXD] = {1, 1, 1};
-    {
-        unsigned st = 1, sc = 1;
-#pragma unroll
-        for (int a = DD - 1; a >= 0; a--) {
-            ctup[a] = st; st *= (unsigned)P.ncls[a];
-            ccell[a] = sc; sc *= (unsigned)P.g.n[a];
-        }
-        int m = 1;
-#pragma unroll
-        for (int a = 0; a < DD; a++) { civ[a] = m; m *= K1; }
-    }
     long tails[MAXD] = {1, 1, 1};
     lattice_tails<DD, KK>(P.g, tails);
     int p[MAXD] = {0, 0, 0};
-    AxInfo ax[MAXD];
+    AxEntry ax[MAXD];
     {
         long r = line;
 #pragma unroll
@@ -819,37 +777,39 @@ __global__ __launch_bounds__(256) void k_asm_cells(RowParams P, long nunits, int
             r /= Na;
         }
 #pragma unroll
-        for (int a = 0; a < LAST; a++) ax[a] = axis_info<KK>(p[a], P.g.n[a], P.ncls[a], cls[a], ctup[a], ccell[a], civ[a]);
+        for (int a = 0; a < LAST; a++) ax[a] = axtab[aoff.o[a] + p[a]];
     }
-    // the fixed axes' part of each cell slot hf (their cell bits, first axis most significant)
-    unsigned TF[NF], CFi[NF];
-    int IVF[NF], HF[NF];
-    bool VF[NF];
+    // the fixed axes' part of each cell slot hf (their cell bits, first axis most significant);
+    // vmask: bits 2 hf and 2 hf + 1 set for the slots whose cells exist
+    unsigned LF[NF], CF[NF];
+    int HF[NF];
+    int vmask = 0;
 #pragma unroll
     for (int hf = 0; hf < NF; hf++) {
-        unsigned tt = 0, cc = 0;
-        int iv = 0, ho = 0, wm = 1;
+        unsigned ll = 0, cc = 0;
+        int ho = 0, wm = 1;
         bool v = true;
 #pragma unroll
         for (int a = 0; a < LAST; a++) {
             const int j = (hf >> (LAST - 1 - a)) & 1;
-            v = v && j < ax[a].nc;
-            tt += ax[a].T[j];
+            v = v && j < (ax[a].info & 3);
+            ll += ax[a].L[j];
             cc += ax[a].C[j];
-            iv += ax[a].IV[j];
             ho += j * KK * wm;
             wm *= W1;
         }
-        TF[hf] = tt; CFi[hf] = cc; IVF[hf] = iv; HF[hf] = ho; VF[hf] = v;
+        LF[hf] = ll; CF[hf] = cc; HF[hf] = ho;
+        if (v) vmask |= 3 << (2 * hf);
     }
     int combo_f = 0, total_f = 1, class_f = 0, cw = 1;
     bool inner_f = inner_ok != 0;
 #pragma unroll
     for (int a = 0; a < LAST; a++) {
-        combo_f |= ax[a].tau << (2 * a);
-        total_f *= KK * ax[a].nc + 1;
-        inner_f = inner_f && ax[a].lo >= 2;
-        class_f += axis_class<KK>(ax[a], p[a]) * cw;
+        combo_f |= ((ax[a].info >> 2) & 3) << (2 * a);
+        total_f *= KK * (ax[a].info & 3) + 1;
+        const int cl = ax[a].info >> 4;
+        inner_f = inner_f && cl < 3;
+        class_f += cl * cw;
         cw *= 3;
     }
     // the row's DOF along the line: dof(p) = dof(p with p_last = 0) + mprod * p_last, mprod the
@@ -867,114 +827,187 @@ __global__ __launch_bounds__(256) void k_asm_cells(RowParams P, long nunits, int
         for (int a = 0; a < DD; a++) { lofs += (r % K1) * wm; r /= K1; wm *= W1; }
     }
     const bool node_lane = lane < NLOC;
-    const unsigned nent = (unsigned)(NLOC * NLOC);
-    const unsigned ioff = (unsigned)P.ntuples * nent;   // between the integrals' tables
+    const unsigned ioff = (unsigned)P.ntuples * (unsigned)(NLOC * NLOC);   // between the integrals' tables
     const int NL = min(KK * P.g.n[LAST] + 1, seg0 + seg_len);
     const int iters = (NL - seg0 + R - 1) / R;
-    for (int it = 0; it < iters; it++) {
-        const int pl = seg0 + it * R + sub;
-        const bool active = pl < NL;
-        p[LAST] = active ? pl : seg0;
-        const AxInfo xl = axis_info<KK>(p[LAST], P.g.n[LAST], P.ncls[LAST], cls[LAST], ctup[LAST], ccell[LAST],
-                                        civ[LAST]);
-        const int combo = combo_f | (xl.tau << (2 * LAST));
-        const int total = total_f * (KK * xl.nc + 1);
-        const long row = rbase + (long)rstep * p[LAST];
-        const bool inner = inner_f && xl.lo >= 2;
-        const int dcls = class_f + axis_class<KK>(xl, p[LAST]) * C3LAST;
-        int base = 0, lim = 0;
-        if (FILL) {
-            base = rowptr[row];
-            lim = rowptr[row + 1];
-            if (!active) base = lim = 0;
-        }
+    // A row's inputs.  Two rows of the line are in flight at a time: both rows' loads go out
+    // before either is used, their sums interleave, then both rows are written.
+    struct RowIn {
+        bool active, inner;
+        int M, combo, total, lo, base, lim;
+        long row;
+        unsigned L[2];
         unsigned fo[CHUNKS];
         int dl[CHUNKS];
-#pragma unroll
-        for (int ch = 0; ch < CHUNKS; ch++) {
-            fo[ch] = fwd[combo * WMAX + ch * G + lane];
-            dl[ch] = delta[(inner ? dcls : 0) * WMAX + ch * G + lane];
-        }
-        // the row's cells: slot h = (fixed-axes slot hf, last-axis bit j), lexicographic order.
-        // With one row per wavefront a slot's validity is uniform: skipped slots cost a branch.
-        // All lanes load (lanes past the cell's nodes read padding) and all take part in the
-        // sums; those lanes add into the spare slot.
-        bool hv[NH];
         int ht[NH];
         double ev[NH], cf[NH];
+    };
+    auto fetch = [&](int it, RowIn& r) {
+        const int pl = seg0 + it * R + sub;
+        r.active = pl < NL;
+        const int pla = r.active ? pl : seg0;
+        const AxEntry xl = axtab[aoff.o[LAST] + pla];
+        const int ncl = xl.info & 3, cll = xl.info >> 4;
+        r.combo = combo_f | (((xl.info >> 2) & 3) << (2 * LAST));
+        r.total = total_f * (KK * ncl + 1);
+        r.row = rbase + (long)rstep * pla;
+        r.inner = inner_f && cll < 3;
+        r.lo = xl.lo;
+        r.L[0] = xl.L[0];
+        r.L[1] = xl.L[1];
+        const int dcls = r.inner ? class_f + cll * C3LAST : 0;
+        r.M = r.active ? (vmask & (ncl == 2 ? 0xff : 0x55)) : 0;   // the row's cell slots
+        r.base = r.lim = 0;
+        if (FILL) {
+            r.base = rowptr[r.row];
+            r.lim = rowptr[r.row + 1];
+            if (!r.active) r.base = r.lim = 0;
+        }
+#pragma unroll
+        for (int ch = 0; ch < CHUNKS; ch++) {
+            r.fo[ch] = fwd[r.combo * WMAX + ch * G + lane];
+            r.dl[ch] = delta[dcls * WMAX + ch * G + lane];
+        }
+        // the row's cells: slot h = (fixed-axes slot hf, last-axis bit j), lexicographic order.
+        // With one row per wavefront the slot mask is uniform: skipped slots cost a branch.
+        // All lanes load (lanes past the cell's nodes read padding) and all take part in the
+        // sums; those lanes add into the spare slot.
 #pragma unroll
         for (int h = 0; h < NH; h++) {
             const int hf = h >> 1, j = h & 1;
-            hv[h] = active && VF[hf] && j < xl.nc;
-            if (hv[h]) {
-                const unsigned tup = TF[hf] + xl.T[j];
-                const int iv = IVF[hf] + xl.IV[j];
-                cf[h] = COEF ? P.coeff[CFi[hf] + xl.C[j]] : 1.0;
-                ht[h] = inv[combo * WMAX + HF[hf] + j * KK * WLAST + lofs];
+            if ((r.M >> h) & 1) {
+                r.cf[h] = COEF ? P.coeff[CF[hf] + xl.C[j]] : 1.0;
+                r.ht[h] = inv[r.combo * WMAX + HF[hf] + j * KK * WLAST + lofs];
                 // element-table index < 2^30 (checked on the host): 32-bit
-                ev[h] = E[tup * nent + (unsigned)(iv * NLOC + lane)];
+                r.ev[h] = E[LF[hf] + xl.L[j] + (unsigned)lane];
             }
         }
-        for (int t = lane; t < STRIDE; t += G) acc[t] = -0.0;
-#pragma unroll
-        for (int h = 0; h < NH; h++) {
-            if (hv[h]) {
-                double x = 0;
-                x += COEF ? cf[h] * ev[h] : ev[h];
-                for (int i = 1; i < P.nint; i++) {   // further integrals: E of integral i at i * ioff
-                    const int hf = h >> 1, j = h & 1;
-                    const unsigned tup = TF[hf] + xl.T[j];
-                    const int iv = IVF[hf] + xl.IV[j];
-                    const double e = E[(unsigned)i * ioff + tup * nent + (unsigned)(iv * NLOC + lane)];
-                    x += COEF ? cf[h] * e : e;
-                }
-                const int t = node_lane ? ht[h] : STRIDE - 1;
-                const double old = acc[t];
-                acc[t] = x != 0 ? x + old : old;
+    };
+    // the sums of one row over its cells in their order into acc
+    auto sum_row = [&](const RowIn& r, double* acc, int h) {
+        const int hf = h >> 1, j = h & 1;
+        if ((r.M >> h) & 1) {
+            double x = 0;
+            x += COEF ? r.cf[h] * r.ev[h] : r.ev[h];
+            for (int i = 1; i < P.nint; i++) {   // further integrals: E of integral i at i * ioff
+                const double e = E[(unsigned)i * ioff + LF[hf] + r.L[j] + (unsigned)lane];
+                x += COEF ? r.cf[h] * e : e;
             }
+            const int t = node_lane ? r.ht[h] : STRIDE - 1;
+            const double old = acc[t];
+            acc[t] = x != 0 ? x + old : old;
         }
-        // candidates in column order: ballot-compacted writes (G lanes per row)
+    };
+    // candidates in column order: ballot-compacted writes (G lanes per row).  Per-lane
+    // conditions become selects; the one branch per chunk is around the stores.
+    auto write_row = [&](const RowIn& r, const double* acc) {
         int written = 0;
 #pragma unroll
         for (int ch = 0; ch < CHUNKS; ch++) {
             const int t = ch * G + lane;
-            bool present = false;
-            double v = 0;
+            double v = acc[t];
+            bool present = r.active && t < r.total &&
+                           __double_as_longlong(v) != (long long)0x8000000000000000ull;
             int colv = 0;
-            v = acc[t];
-            if (active && t < total) {
-                present = __double_as_longlong(v) != (long long)0x8000000000000000ull;
-                if (FILL || row == 0) {
-                    if (inner) {
-                        colv = (int)row + dl[ch];
-                    } else {
-                        int q[MAXD] = {0, 0, 0};
-                        q[LAST] = KK * xl.lo + (int)((fo[ch] >> (4 * LAST)) & 15u);
+            if (FILL || r.row == 0) {
+                if (r.inner) {
+                    colv = (int)r.row + r.dl[ch];
+                } else {
+                    int q[MAXD] = {0, 0, 0};
+                    q[LAST] = KK * r.lo + (int)((r.fo[ch] >> (4 * LAST)) & 15u);
 #pragma unroll
-                        for (int a = 0; a < LAST; a++) q[a] = KK * ax[a].lo + (int)((fo[ch] >> (4 * a)) & 15u);
-                        colv = (int)dof_of_lat_tails<DD, KK>(P.g, tails, q);
-                    }
+                    for (int a = 0; a < LAST; a++) q[a] = KK * ax[a].lo + (int)((r.fo[ch] >> (4 * a)) & 15u);
+                    colv = (int)dof_of_lat_tails<DD, KK>(P.g, tails, q);
                 }
-                if (row == 0 && colv == 0 && !present) {   // dummy (0,0) entry of a new matrix
-                    present = true;
-                    v = 0.0;
-                }
+            }
+            if (r.row == 0) {   // the dummy (0,0) entry of a new matrix
+                const bool dummy = r.active && t < r.total && colv == 0 && !present;
+                present = present || dummy;
+                v = dummy ? 0.0 : v;
             }
             const unsigned long long bal = __ballot(present);
             const unsigned long long gmask = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << gbase);
             const unsigned long long mine = bal & gmask;
-            if (FILL && present) {
+            if (FILL) {
                 const unsigned long long below = mine & ((1ull << wlane) - 1ull);
-                const int pos = base + written + __popcll(below);
-                if (pos < lim) {   // a row never writes past its reserved range
+                const int pos = r.base + written + __popcll(below);
+                if (present && pos < r.lim) {   // a row never writes past its reserved range
                     cols[pos] = colv;
                     vals[pos] = v;
                 }
             }
             written += __popcll(mine);
         }
-        if (!FILL && active && lane == 0) counts[row] = written;
-        if (FILL && active && lane == 0 && written != lim - base) atomicOr(bad, 1);
+        if (!FILL && r.active && lane == 0) counts[r.row] = written;
+        if (FILL && r.active && lane == 0 && written != r.lim - r.base) atomicOr(bad, 1);
+    };
+    double* acc1 = acc_all + (256 / G) * STRIDE + (threadIdx.x / G) * STRIDE;
+    for (int it = 0; it < iters; it += 2) {
+        const bool two = it + 1 < iters;
+        RowIn ra, rb;
+        fetch(it, ra);
+        fetch(two ? it + 1 : it, rb);
+        if (!two) rb.M = 0, rb.active = false;
+#pragma unroll
+        for (int k2 = 0; k2 < STRIDE / G; k2++) {
+            acc[k2 * G + lane] = -0.0;
+            acc1[k2 * G + lane] = -0.0;
+        }
+#pragma unroll
+        for (int h = 0; h < NH; h++) {
+            sum_row(ra, acc, h);
+            sum_row(rb, acc1, h);
+        }
+        write_row(ra, acc);
+        write_row(rb, acc1);
+    }
+}
+
+// the per-axis node table of k_asm_cells (AxEntry) for every axis, axes one after another
+static void cell_asm_axes(const Grid& g, const std::vector<int>* cls, const int* ncls, std::vector<AxEntry>& tab,
+                          AxOffsets& off)
+{
+    const int d = g.d, k = g.k;
+    int nloc = 1;
+    for (int a = 0; a < d; a++) nloc *= k + 1;
+    const unsigned nent = (unsigned)(nloc * nloc);
+    unsigned ctup[MAXD], civ[MAXD];
+    {
+        unsigned st = 1;
+        for (int a = d - 1; a >= 0; a--) { ctup[a] = st; st *= (unsigned)ncls[a]; }
+        unsigned m = 1;
+        for (int a = 0; a < d; a++) { civ[a] = m; m *= (unsigned)(k + 1); }
+    }
+    unsigned ccell[MAXD];
+    {
+        unsigned sc = 1;
+        for (int a = d - 1; a >= 0; a--) { ccell[a] = sc; sc *= (unsigned)g.n[a]; }
+    }
+    tab.clear();
+    for (int a = 0; a < MAXD; a++) off.o[a] = 0;
+    for (int a = 0; a < d; a++) {
+        off.o[a] = (int)tab.size();
+        const int N = k * g.n[a] + 1;
+        for (int p = 0; p < N; p++) {
+            int l, h;
+            if (p % k == 0) { l = p / k - 1; h = p / k; } else { l = h = p / k; }
+            l = std::max(l, 0);
+            h = std::min(h, g.n[a] - 1);
+            const int nc = h - l + 1;
+            const int tau = (l > 0 ? 2 : 0) | (h > l ? 1 : 0);
+            const int cl = l < 2 ? 3 : (tau == 3 ? 0 : (p - k * l < k ? 1 : 2));
+            AxEntry e;
+            std::memset(&e, 0, sizeof(e));
+            e.lo = l;
+            e.info = nc | tau << 2 | cl << 4;
+            for (int j = 0; j < 2; j++) {
+                const int c = std::min(l + j, h);   // j = 1 with one cell: a valid index, never used
+                const unsigned cv = ncls[a] == 1 ? 0u : (unsigned)cls[a][c];
+                e.L[j] = cv * ctup[a] * nent + (unsigned)(p - k * c) * civ[a] * (unsigned)nloc;
+                e.C[j] = (unsigned)c * ccell[a];
+            }
+            tab.push_back(e);
+        }
     }
 }
 
@@ -1564,16 +1597,17 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     const long Esize = (long)nint * EP.ntuples * EP.nloc * EP.nloc;
     JFEM_CHECK(Esize < (1l << 30), JFEM_ERR_UNSUPPORTED, "too many distinct cell geometries");
     double* E = nullptr;
+    DevBuf<int> pairs_d;
     // 64 doubles of padding: k_asm_cells reads whole wavefronts past a table row's end
     JFEM_HIP(hipMalloc(&E, sizeof(double) * (Esize + 64)));
     JFEM_HIP(hipMemsetAsync(E + Esize, 0, sizeof(double) * 64, c->stream));
+    // node pairs iv <= iu (all pairs for non-symmetric integrals), packed iv << 16 | iu; the
+    // host vectors of asynchronous uploads live until the function's last synchronisation
+    std::vector<int> pairs;
     {
-        // node pairs iv <= iu (all pairs for non-symmetric integrals), packed iv << 16 | iu
-        std::vector<int> pairs;
         for (int v = 0; v < EP.nloc; v++)
             for (int u = full ? 0 : v; u < EP.nloc; u++) pairs.push_back(v << 16 | u);
-        int* pairs_d = nullptr;
-        JFEM_HIP(hipMalloc(&pairs_d, sizeof(int) * pairs.size()));
+        pairs_d.alloc(pairs.size());
         JFEM_HIP(hipMemcpyAsync(pairs_d, pairs.data(), sizeof(int) * pairs.size(), hipMemcpyHostToDevice, c->stream));
         const long work = (long)pairs.size() * EP.ntuples * nint;
         const int eg = grid_for(work, 256, 16384);
@@ -1586,8 +1620,17 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
         default: k_element_tables<0, 0, 0><<<eg, 256, 0, c->stream>>>(EP, D.tabs[0], D.tabs[1], D.tabs[2], pairs_d, E); break;
         }
         JFEM_HIP(hipGetLastError());
-        JFEM_HIP(hipStreamSynchronize(c->stream));
-        (void)hipFree(pairs_d);
+    }
+    // the smallest |entry| of the element tables, read back after the host work below
+    DevBuf<unsigned long long> emin_d;
+    unsigned long long emin_bits = 0;
+    const bool one_integral = nint == 1;
+    if (one_integral) {
+        emin_d.alloc(1);
+        JFEM_HIP(hipMemsetAsync(emin_d, 0xff, sizeof(unsigned long long), c->stream));
+        k_absmin<<<grid_for(Esize, 256, 2048), 256, 0, c->stream>>>(Esize, E, emin_d);
+        JFEM_HIP(hipGetLastError());
+        JFEM_HIP(hipMemcpyAsync(&emin_bits, emin_d, sizeof(emin_bits), hipMemcpyDeviceToHost, c->stream));
     }
 
     double* coeff = nullptr;
@@ -1635,8 +1678,50 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     DevBuf<unsigned short> fwd_d;
     DevBuf<unsigned char> inv_d;
     DevBuf<int> delta_d;
+    std::vector<AxEntry> axtab_h;
+    std::vector<int> order_h;
+    DevBuf<int> order_d;
+    AxOffsets aoff;
+    DevBuf<AxEntry> axtab_d;
     int inner_ok = 0;
     if (cellwise) {
+        cell_asm_axes(g, C.cls, C.ncls, axtab_h, aoff);
+        // lines (lattice points of the fixed axes, last fixed axis fastest) ordered by the
+        // classes of the cells containing them, axis by axis: each fixed axis' points sorted by
+        // their cells' classes, the lines their product in that order
+        {
+            std::vector<int> sorted[MAXD];
+            long nl = 1;
+            for (int a = 0; a + 1 < g.d; a++) {
+                const int Na = g.k * g.n[a] + 1;
+                nl *= Na;
+                std::vector<std::pair<long, int>> keyed((size_t)Na);
+                for (int pa = 0; pa < Na; pa++) {
+                    const AxEntry& e = axtab_h[(size_t)aoff.o[a] + pa];
+                    const long c0 = C.ncls[a] == 1 ? 0 : C.cls[a][e.lo];
+                    const long c1 = (e.info & 3) == 2 && C.ncls[a] > 1 ? C.cls[a][e.lo + 1] : c0;
+                    keyed[(size_t)pa] = {c0 * (C.ncls[a] + 1) + c1, pa};
+                }
+                std::stable_sort(keyed.begin(), keyed.end(),
+                                 [](const std::pair<long, int>& x, const std::pair<long, int>& y) { return x.first < y.first; });
+                for (auto& kv : keyed) sorted[a].push_back(kv.second);
+            }
+            order_h.assign((size_t)nl, 0);
+            if (g.d == 3) {
+                const int N1 = g.k * g.n[1] + 1;
+                size_t i = 0;
+                for (int p0 : sorted[0])
+                    for (int p1 : sorted[1]) order_h[i++] = p0 * N1 + p1;
+            } else if (g.d == 2) {
+                order_h = sorted[0];
+            }
+            order_d.alloc(order_h.size());
+            JFEM_HIP(hipMemcpyAsync(order_d, order_h.data(), sizeof(int) * order_h.size(), hipMemcpyHostToDevice,
+                                    c->stream));
+        }
+        axtab_d.alloc(axtab_h.size());
+        JFEM_HIP(hipMemcpyAsync(axtab_d, axtab_h.data(), sizeof(AxEntry) * axtab_h.size(), hipMemcpyHostToDevice,
+                                c->stream));
         cell_asm_tables(g.d, g.k, fwd_h, inv_h);
         fwd_h.resize(fwd_h.size() + 64, 0);   // whole wavefronts read past the last combination's entries
         inner_ok = cell_asm_delta(g, fwd_h, delta_h) ? 1 : 0;
@@ -1649,7 +1734,7 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
         JFEM_HIP(hipMemcpyAsync(delta_d, delta_h.data(), sizeof(int) * delta_h.size(), hipMemcpyHostToDevice,
                                 c->stream));
     }
-    const size_t cell_lds_bytes = lds_bytes + sizeof(double) * 64;   // the same padding as E's
+    const size_t cell_lds_bytes = sizeof(double) * (Esize + 64);   // E with the same padding
     const bool cell_lds = cell_lds_bytes <= 16 * 1024;
     auto run_cells = [&](auto kern, int GG) {
         const size_t dyn = cell_lds ? cell_lds_bytes : 0;
@@ -1668,7 +1753,7 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
         const long nb = (nunits + 3) / 4;
         JFEM_CHECK(nb < (1l << 31), JFEM_ERR_UNSUPPORTED, "too many lattice lines");
         kern<<<(int)nb, 256, dyn, c->stream>>>(RP, nunits, nseg, seg_len, counts, A->rowptr, A->colidx, A->vals,
-                                               bad, fwd_d, inv_d, delta_d, inner_ok);
+                                               bad, fwd_d, inv_d, delta_d, axtab_d, aoff, order_d, inner_ok);
         JFEM_HIP(hipGetLastError());
     };
     auto run_cells4 = [&](auto kern_lds_c, auto kern_lds, auto kern_glob_c, auto kern_glob, int GG) {
@@ -1709,19 +1794,28 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     // products with the entries cannot round to zero, every cell contribution
     // is nonzero, so every structural candidate is in the pattern and the row
     // lengths have a closed form; otherwise the count pass evaluates them.
-    bool structural = nint == 1;
+    // the entries of the structural pattern bound nnz (the pattern is a subset of it): cols and
+    // vals are allocated for it now, while the element tables are computed
+    long nnz_bound = 1;
+    for (int a = 0; a < g.d; a++) {
+        long sa = 0;
+        for (int pa = 0; pa <= g.k * g.n[a]; pa++) {
+            int l, h;
+            if (pa % g.k == 0) { l = pa / g.k - 1; h = pa / g.k; } else { l = h = pa / g.k; }
+            l = std::max(l, 0);
+            h = std::min(h, g.n[a] - 1);
+            sa += (long)g.k * (h - l + 1) + 1;
+        }
+        nnz_bound *= sa;
+    }
+    JFEM_CHECK(nnz_bound < (1l << 31) - 1, JFEM_ERR_UNSUPPORTED, "matrix has 2^31 or more entries (int32 CSR indices)");
+    JFEM_HIP(hipMalloc(&A->colidx, sizeof(int) * (nnz_bound + NNZ_PAD)));
+    JFEM_HIP(hipMalloc(&A->vals, sizeof(double) * (nnz_bound + NNZ_PAD)));
+    JFEM_HIP(hipStreamSynchronize(c->stream));
+    bool structural = one_integral;
     if (structural) {
-        unsigned long long* mn = nullptr;
-        JFEM_HIP(hipMalloc(&mn, sizeof(unsigned long long)));
-        JFEM_HIP(hipMemsetAsync(mn, 0xff, sizeof(unsigned long long), c->stream));
-        k_absmin<<<grid_for(Esize, 256, 2048), 256, 0, c->stream>>>(Esize, E, mn);
-        JFEM_HIP(hipGetLastError());
-        unsigned long long bits = 0;
-        JFEM_HIP(hipMemcpyAsync(&bits, mn, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
-        JFEM_HIP(hipStreamSynchronize(c->stream));
-        (void)hipFree(mn);
         double emin;
-        std::memcpy(&emin, &bits, sizeof(emin));
+        std::memcpy(&emin, &emin_bits, sizeof(emin));
         double cmin = 1.0;
         if (coeff_host)
             for (long i = 0; i < g.ncells(); i++) cmin = std::min(cmin, std::fabs(coeff_host[i]));
@@ -1744,8 +1838,7 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     }
     const long nnz = scan_counts(c, counts, n, A->rowptr);
     A->nnz = nnz;
-    JFEM_HIP(hipMalloc(&A->colidx, sizeof(int) * (nnz + NNZ_PAD)));
-    JFEM_HIP(hipMalloc(&A->vals, sizeof(double) * (nnz + NNZ_PAD)));
+    JFEM_CHECK(nnz <= nnz_bound, JFEM_ERR_STATE, "more entries than the structural pattern");
     launch(true);
     int bad_h = 0;
     JFEM_HIP(hipMemcpyAsync(&bad_h, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
