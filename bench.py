@@ -185,6 +185,31 @@ def spmv_only(jf, cells):
                       "copy_gbs": gbs, "copy_ms": cms, "n": n, "nnz": nnz}), flush=True)
 
 
+def assembly_only(jf, cells, reps=2):
+    """jfem_assemble of the 3D Q2 GRAD_GRAD matrix on cells^3, `reps` times with a constant and
+    `reps` times with a log-normal per-cell coefficient (host wall time per call, device-resident
+    result), then the copy calibration kernel; for kernel traces and PMC runs"""
+    s = jf.ContinuousTPFESpace([0.0] * 3, [1.0] * 3, [cells] * 3)
+    s.assembleCells()
+    s.assembleFunctions(2)
+    coeff = np.exp(np.random.default_rng(0).standard_normal(cells ** 3))
+    ctx = jf.context()
+    out = {}
+    for name, cc in (("constant", None), ("cell_coeff", coeff)):
+        times = []
+        for _ in range(reps):
+            ctx.sync()
+            t = time.perf_counter()
+            A = s.assemble_matrix([jf.TPCellIntegral("GradGrad")], cell_coeff=cc)
+            ctx.sync()
+            times.append(time.perf_counter() - t)
+            n, _, nnz = A.info()
+            del A
+        out[f"assembly_{name}_s"] = times
+    gbs, cms = copy_calibration(jf, reps=3)
+    print(json.dumps({**out, "n": n, "nnz": nnz, "copy_gbs": gbs}), flush=True)
+
+
 def network_only(jf, cells, reps=5):
     """the learned smoother's network alone on the finest node lattice of `cells`^3 Q2 cells
     (2 cells + 1 nodes per axis), with the C2 network's shape (3D, 5 input channels) and random
@@ -584,6 +609,9 @@ def main():
                          "other side keeps Chebyshev")
     ap.add_argument("--network-only", action="store_true",
                     help="time the learned smoother's network forward alone on the finest node lattice")
+    ap.add_argument("--assembly-only", action="store_true",
+                    help="assemble the finest 3D Q2 matrix with a constant and a per-cell coefficient (for kernel "
+                         "traces and PMC runs)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="assemble the finest matrix and time only the SpMV and the copy calibration (for PMC runs)")
     args = ap.parse_args()
@@ -599,6 +627,9 @@ def main():
         return
     if args.spmv_only:
         spmv_only(jf, args.cells)
+        return
+    if args.assembly_only:
+        assembly_only(jf, args.cells)
         return
     if args.workload == "c4":
         bench_c4(jf, args)
