@@ -48,6 +48,13 @@ ASM_CASES = [
     ([12, 10, 9], 2, [0], [1.0]),                # 3D Q2 with interior rows of every type, 9-12 cell classes
     ([10, 9, 8], 1, [0, 1], [1.0, 2.0]),
     ([40, 33], 2, [2], [0.5]),
+    # one or two cells on an axis (row types with lo = hi = 0), fewer than four cells (no inner rows)
+    ([1, 1], 1, [0], [1.0]),
+    ([1, 2, 1], 2, [0, 1], [1.0, 0.5]),
+    ([2, 1], 2, [0], [1.0]),
+    ([3, 3, 3], 1, [0], [1.0]),
+    ([5], 1, [0, 1], [1.0, 3.0]),
+    ([4, 1, 6], 2, [0], [2.0]),
 ]
 
 
@@ -119,9 +126,11 @@ def test_tvmul_transpose(jf, oracle):
     assert np.linalg.norm(got - S.T @ x) <= 1e-14 * np.linalg.norm(S.T @ x)
 
 
-def test_assembly_with_cell_coefficient(jf, oracle):
-    cells, k = [6, 5, 4], 2
-    s, start, end = _space(jf, cells, k)
+@pytest.mark.parametrize("cells,k,unit", [([6, 5, 4], 2, False), ([8, 8, 8], 2, True), ([16, 12], 1, True)])
+def test_assembly_with_cell_coefficient(jf, oracle, cells, k, unit):
+    """per-cell coefficient; on the unit box with 2^m cells per axis every cell of an axis has
+    the same 1D tables (one class), so the element tables are staged in LDS"""
+    s, start, end = _space(jf, cells, k, *(([0.0] * len(cells), [1.0] * len(cells)) if unit else ()))
     rng = np.random.default_rng(7)
     coeff = np.exp(rng.standard_normal(int(np.prod(cells))))
     A = s.assemble_matrix([jf.TPCellIntegral("GradGrad")], cell_coeff=coeff)
