@@ -45,6 +45,7 @@ struct MgLevel {
     jfem::MfParams mfp;
     double* coef = nullptr;
     float* coef32 = nullptr;
+    unsigned short* mf_cand = nullptr;   // the inner-tile table of mfp (device)
 };
 
 // learned smoother on the finest level (a network applied to the normalised
@@ -733,6 +734,7 @@ static void free_mg(jfem_mg* G)
         (void)hipFree(L.x); (void)hipFree(L.b); (void)hipFree(L.r); (void)hipFree(L.t); (void)hipFree(L.dv);
         (void)hipFree(L.tp_buf); (void)hipFree(L.tw0); (void)hipFree(L.tw1);
         (void)hipFree(L.coef); (void)hipFree(L.coef32);
+        (void)hipFree(L.mf_cand);
     }
     (void)hipFree(G->coarse_inv);
     (void)hipFree(G->init); (void)hipFree(G->defect); (void)hipFree(G->tmp);
@@ -874,6 +876,16 @@ static jfem_mg* create_mg(jfem_space* const* spaces, int nlev, int nint, const i
                 L.mfp.g = to_dev(L.grid);
                 mf_tiles(L.grid, L.mfp.ntile);
                 for (int a = 0; a < L.grid.d; a++) element_1d(L.grid, a, rule, L.mfp.K1[a], L.mfp.M1[a]);
+                {
+                    std::vector<unsigned short> cd;
+                    mf_tables(L.grid, cd);
+                    if (!cd.empty()) {
+                        JFEM_HIP(hipMalloc(&L.mf_cand, sizeof(unsigned short) * cd.size()));
+                        JFEM_HIP(hipMemcpy(L.mf_cand, cd.data(), sizeof(unsigned short) * cd.size(),
+                                           hipMemcpyHostToDevice));
+                        L.mfp.cand = L.mf_cand;
+                    }
+                }
                 for (int i = 0; i < nint; i++) {
                     if (kinds[i] == JFEM_GRAD_GRAD || kinds[i] == JFEM_H1) L.mfp.wgg += weights[i];
                     if (kinds[i] == JFEM_VALUE_VALUE || kinds[i] == JFEM_H1) L.mfp.wvv += weights[i];

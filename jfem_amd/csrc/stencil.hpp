@@ -27,6 +27,8 @@
 
 #include "common.hpp"
 
+#include <vector>
+
 namespace jfem {
 
 // per-level parameters
@@ -36,6 +38,11 @@ struct MfParams {
     double K1[MAXD][9];      // 1D stiffness of one cell per axis, (k+1) x (k+1) row-major
     double M1[MAXD][9];      // 1D mass
     double wgg, wvv;         // weights of the gradient and value terms
+    // table of the inner tiles (tiles whose node box starts at cell 2 or later and ends before the
+    // last lattice point on every axis: full tiles without boundary nodes), built by mf_tables:
+    // cand[8 q + c] = the offset in the tile's cell results of candidate cell c of node-box
+    // position q (0xffff: none), the same for every inner tile
+    const unsigned short* cand;
 };
 
 // cells per axis of a tile (the computed box has one more per axis) and threads per
@@ -207,9 +214,13 @@ __global__ __launch_bounds__((MfShape<D, K>::BS), (sizeof(T) == 8 ? 5 : 7)) void
     // DOF(org + q) = g_0(q_0) + g_1(q_1) + g_2(q_2), with per-axis tables built once per tile;
     // other tiles use the closed form per node
     __shared__ int gtab[NB0 + NB1 + NB2];
-    bool sep = true;
+    bool sep = true, inner = P.cand != nullptr;
 #pragma unroll
-    for (int a = 0; a < D; a++) sep = sep && clo[a] >= 2;
+    for (int a = 0; a < D; a++) {
+        sep = sep && clo[a] >= 2;
+        inner = inner && clo[a] >= 2 && clo[a] + TC[a] + 1 < n[a];
+    }
+    // inner tiles: the candidate cells of the owned nodes from cand (no per-node index arithmetic)
     if (sep) {
         const int base[3] = {K + 1, K + 1, D == 3 ? K + 1 : 0};
         const int b0 = mf_dof<D, K>(n, base);
@@ -233,6 +244,7 @@ __global__ __launch_bounds__((MfShape<D, K>::BS), (sizeof(T) == 8 ? 5 : 7)) void
     int row[NOWN];
     bool own[NOWN];
     typename Epi::Pre pre[NOWN];
+    uint4 candw[NOWN];   // inner tiles: the owned node's candidate offsets, loaded early
 #pragma unroll
     for (int u = 0; u < NOWN; u++) {
         const int i = tid + u * BS;
@@ -242,6 +254,9 @@ __global__ __launch_bounds__((MfShape<D, K>::BS), (sizeof(T) == 8 ? 5 : 7)) void
         if (own[u]) {
             row[u] = dof_of(olo[0] + o0 - org[0], olo[1] + o1 - org[1], olo[2] + o2 - org[2]);
             pre[u] = epi.load(row[u]);
+            if (inner)
+                candw[u] = *reinterpret_cast<const uint4*>(
+                    P.cand + 8 * (((o0 + 1) * NB1 + o1 + 1) * NB2 + (D == 3 ? o2 + 1 : 0)));
         }
     }
     // 1. node values (boundary nodes as 0: their columns are eliminated) and cell coefficients;
@@ -253,6 +268,10 @@ __global__ __launch_bounds__((MfShape<D, K>::BS), (sizeof(T) == 8 ? 5 : 7)) void
         const int i = tid + u * BS;
         const int q2 = i % NB2, q1 = (i / NB2) % NB1, q0 = i / (NB1 * NB2);
         xv[u] = 0;
+        if (inner) {   // a full box without boundary nodes
+            if (i < NNB) xv[u] = x[dof_of(q0, q1, q2)];
+            continue;
+        }
         if (i < NNB && q0 < nb[0] && q1 < nb[1] && q2 < nb[2]) {
             const int p[3] = {org[0] + q0, org[1] + q1, org[2] + q2};
             bool bnd = false;
@@ -301,6 +320,22 @@ __global__ __launch_bounds__((MfShape<D, K>::BS), (sizeof(T) == 8 ? 5 : 7)) void
         if (!own[u]) continue;
         const int i = tid + u * BS;
         const int o2 = i % OB2, o1 = (i / OB2) % OB1, o0 = i / (OB1 * OB2);
+        if (inner) {
+            // the candidate cells of node-box position o + 1, ascending, as in the general path
+            const unsigned wv[4] = {candw[u].x, candw[u].y, candw[u].z, candw[u].w};
+            T v[1 << D];
+#pragma unroll
+            for (int c = 0; c < (1 << D); c++) {
+                const unsigned off = (wv[c >> 1] >> (16 * (c & 1))) & 0xffffu;
+                const T w = cr[off != 0xffffu ? off : 0];
+                v[c] = off != 0xffffu ? w : (T)0;
+            }
+            T acc = 0;
+#pragma unroll
+            for (int c = 0; c < (1 << D); c++) acc += v[c];
+            epi(row[u], acc, pre[u]);
+            continue;
+        }
         const int p[3] = {olo[0] + o0, olo[1] + o1, olo[2] + o2};
         bool bnd = false;
 #pragma unroll
@@ -372,5 +407,55 @@ inline void mf_tiles(const Grid& g, int* ntile)
 }
 
 inline bool mf_supported(const Grid& g) { return (g.d == 2 || g.d == 3) && (g.k == 1 || g.k == 2); }
+
+// the inner-tile table cand of k_mf for grid g (see MfParams), 8 entries per node-box position;
+// empty when the grid has no inner tile
+template <int D, int K>
+inline void mf_tables_dk(const Grid& g, std::vector<unsigned short>& cand)
+{
+    using S = MfShape<D, K>;
+    constexpr int N = K + 1, NL = D == 3 ? N * N * N : N * N;
+    const int TC[3] = {S::T0, S::T1, D == 3 ? S::T2 : 1};
+    const int CC[3] = {S::T0 + 1, S::T1 + 1, D == 3 ? S::T2 + 1 : 1};
+    const int NB[3] = {CC[0] * K + 1, CC[1] * K + 1, D == 3 ? CC[2] * K + 1 : 1};
+    cand.clear();
+    for (int a = 0; a < D; a++)
+        if (2 + TC[a] + 1 >= g.n[a]) return;   // no tile with clo >= 2 and clo + T + 1 < n
+    const int nnb = NB[0] * NB[1] * NB[2];
+    cand.assign((size_t)8 * nnb, 0xffff);
+    for (int q0 = 0; q0 < NB[0]; q0++)
+        for (int q1 = 0; q1 < NB[1]; q1++)
+            for (int q2 = 0; q2 < NB[2]; q2++) {
+                const int i = (q0 * NB[1] + q1) * NB[2] + q2;
+                // candidate cells of node q: per axis lo = q/K - [q mod K == 0], hi = q/K, the 2^d
+                // bit combinations in ascending order (as in k_mf's general path)
+                const int q[3] = {q0, q1, q2};
+                int elo[3] = {0, 0, 0}, ehi[3] = {0, 0, 0};
+                for (int a = 0; a < D; a++) {
+                    ehi[a] = q[a] / K;
+                    elo[a] = q[a] % K == 0 ? ehi[a] - 1 : ehi[a];
+                }
+                for (int c = 0; c < (1 << D); c++) {
+                    const int e0 = elo[0] + (c >> (D - 1) & 1), e1 = elo[1] + (c >> (D - 2) & 1);
+                    const int e2 = D == 3 ? elo[2] + (c & 1) : 0;
+                    const bool in = e0 >= 0 && e1 >= 0 && e2 >= 0 && e0 <= ehi[0] && e1 <= ehi[1] &&
+                                    (D == 2 || e2 <= ehi[2]) && e0 < CC[0] && e1 < CC[1] && e2 < CC[2];
+                    if (!in) continue;
+                    const int l = (q[0] - e0 * K) + N * (q[1] - e1 * K) + (D == 3 ? N * N * (q[2] - e2 * K) : 0);
+                    cand[(size_t)8 * i + c] = (unsigned short)(((e0 * CC[1] + e1) * CC[2] + e2) * NL + l);
+                }
+            }
+}
+
+inline void mf_tables(const Grid& g, std::vector<unsigned short>& cand)
+{
+    switch (g.d * 10 + g.k) {
+    case 32: mf_tables_dk<3, 2>(g, cand); break;
+    case 31: mf_tables_dk<3, 1>(g, cand); break;
+    case 22: mf_tables_dk<2, 2>(g, cand); break;
+    case 21: mf_tables_dk<2, 1>(g, cand); break;
+    default: cand.clear();
+    }
+}
 
 }  // namespace jfem
