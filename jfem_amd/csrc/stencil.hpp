@@ -244,7 +244,10 @@ __global__ __launch_bounds__((MfShape<D, K>::BS), (sizeof(T) == 8 ? 5 : 7)) void
     int row[NOWN];
     bool own[NOWN];
     typename Epi::Pre pre[NOWN];
-    uint4 candw[NOWN];   // inner tiles: the owned node's candidate offsets, loaded early
+    // inner tiles: the owned node's candidate offsets, loaded with the row in FP64 (in FP32 the
+    // registers of the seven workgroups per CU do not hold them; loaded where they are used)
+    constexpr bool CAND_EARLY = sizeof(T) == 8;
+    uint4 candw[NOWN];
 #pragma unroll
     for (int u = 0; u < NOWN; u++) {
         const int i = tid + u * BS;
@@ -254,7 +257,7 @@ __global__ __launch_bounds__((MfShape<D, K>::BS), (sizeof(T) == 8 ? 5 : 7)) void
         if (own[u]) {
             row[u] = dof_of(olo[0] + o0 - org[0], olo[1] + o1 - org[1], olo[2] + o2 - org[2]);
             pre[u] = epi.load(row[u]);
-            if (inner)
+            if (inner && CAND_EARLY)
                 candw[u] = *reinterpret_cast<const uint4*>(
                     P.cand + 8 * (((o0 + 1) * NB1 + o1 + 1) * NB2 + (D == 3 ? o2 + 1 : 0)));
         }
@@ -322,7 +325,10 @@ __global__ __launch_bounds__((MfShape<D, K>::BS), (sizeof(T) == 8 ? 5 : 7)) void
         const int o2 = i % OB2, o1 = (i / OB2) % OB1, o0 = i / (OB1 * OB2);
         if (inner) {
             // the candidate cells of node-box position o + 1, ascending, as in the general path
-            const unsigned wv[4] = {candw[u].x, candw[u].y, candw[u].z, candw[u].w};
+            const uint4 cw = CAND_EARLY ? candw[u]
+                                        : *reinterpret_cast<const uint4*>(
+                                              P.cand + 8 * (((o0 + 1) * NB1 + o1 + 1) * NB2 + (D == 3 ? o2 + 1 : 0)));
+            const unsigned wv[4] = {cw.x, cw.y, cw.z, cw.w};
             T v[1 << D];
 #pragma unroll
             for (int c = 0; c < (1 << D); c++) {
