@@ -1104,34 +1104,54 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_sums(long* __restrict__ sums, 
     if (threadIdx.x == 0) *grand = carry;
 }
 
-__global__ void k_scan_finish(const long* __restrict__ tmp, const long* __restrict__ sums, long n, int* __restrict__ rowptr)
+// rowptr[i] = tile-local prefix + tile offset; rowptr[n] = the grand total (only read where it
+// is below 2^31)
+__global__ void k_scan_finish(const long* __restrict__ tmp, const long* __restrict__ sums, const long* __restrict__ grand,
+                              long n, int* __restrict__ rowptr)
 {
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        rowptr[i] = (int)(tmp[i] + sums[i / SCAN_TILE]);
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i <= n; i += (long)gridDim.x * blockDim.x)
+        rowptr[i] = i < n ? (int)(tmp[i] + sums[i / SCAN_TILE]) : (int)*grand;
+}
+
+size_t scan_work_bytes(long n)
+{
+    const long tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    return sizeof(long) * ((n > 0 ? n : 1) + tiles + 2);
+}
+
+long scan_counts(jfem_ctx* c, const int* counts, long n, int* rowptr, long* work, long known_total)
+{
+    const long tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    long* tmp = work;
+    long* sums = work + (n > 0 ? n : 1);
+    if (n > 0) k_scan_tiles<<<tiles, SCAN_BS, 0, c->stream>>>(counts, n, tmp, sums);
+    k_scan_sums<<<1, SCAN_BS, 0, c->stream>>>(sums, tiles, sums + tiles + 1);
+    long total = known_total;
+    if (known_total < 0) {
+        JFEM_HIP(hipMemcpyAsync(c->pin, sums + tiles + 1, sizeof(long), hipMemcpyDeviceToHost, c->stream));
+        JFEM_HIP(hipStreamSynchronize(c->stream));
+        total = c->pin[0];
+    }
+    JFEM_CHECK(total >= 0 && total < (1l << 31) - 1, JFEM_ERR_UNSUPPORTED,
+               "matrix has 2^31 or more entries (int32 CSR indices)");
+    k_scan_finish<<<grid_for(n + 1, 256, 8192), 256, 0, c->stream>>>(tmp, sums, sums + tiles + 1, n, rowptr);
+    JFEM_HIP(hipGetLastError());
+    return total;
 }
 
 // rowptr[0..n] from counts[0..n-1]; returns total (host)
 long scan_counts(jfem_ctx* c, const int* counts, long n, int* rowptr)
 {
-    const long tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-    long *tmp = nullptr, *sums = nullptr;
-    JFEM_HIP(hipMalloc(&tmp, sizeof(long) * (n > 0 ? n : 1)));
-    JFEM_HIP(hipMalloc(&sums, sizeof(long) * (tiles + 2)));
-    if (n > 0) k_scan_tiles<<<tiles, SCAN_BS, 0, c->stream>>>(counts, n, tmp, sums);
-    k_scan_sums<<<1, SCAN_BS, 0, c->stream>>>(sums, tiles, sums + tiles + 1);
-    long total = 0;
-    JFEM_HIP(hipMemcpyAsync(&total, sums + tiles + 1, sizeof(long), hipMemcpyDeviceToHost, c->stream));
-    JFEM_HIP(hipStreamSynchronize(c->stream));
-    if (total < 0 || total >= (1l << 31) - 1) {
-        (void)hipFree(tmp); (void)hipFree(sums);
-        throw Error{JFEM_ERR_UNSUPPORTED, "matrix has 2^31 or more entries (int32 CSR indices)"};
+    long* work = nullptr;
+    JFEM_HIP(hipMalloc(&work, scan_work_bytes(n)));
+    long total = -1;
+    try {
+        total = scan_counts(c, counts, n, rowptr, work, -1);
+    } catch (...) {
+        (void)hipFree(work);   // waits for the queued scan kernels
+        throw;
     }
-    if (n > 0) k_scan_finish<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(tmp, sums, n, rowptr);
-    const int tot = (int)total;
-    JFEM_HIP(hipMemcpyAsync(rowptr + n, &tot, sizeof(int), hipMemcpyHostToDevice, c->stream));
-    JFEM_HIP(hipStreamSynchronize(c->stream));
-    (void)hipFree(tmp);
-    (void)hipFree(sums);
+    (void)hipFree(work);
     return total;
 }
 
@@ -1596,18 +1616,30 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     }
     const long Esize = (long)nint * EP.ntuples * EP.nloc * EP.nloc;
     JFEM_CHECK(Esize < (1l << 30), JFEM_ERR_UNSUPPORTED, "too many distinct cell geometries");
-    double* E = nullptr;
-    DevBuf<int> pairs_d;
-    // 64 doubles of padding: k_asm_cells reads whole wavefronts past a table row's end
-    JFEM_HIP(hipMalloc(&E, sizeof(double) * (Esize + 64)));
-    JFEM_HIP(hipMemsetAsync(E + Esize, 0, sizeof(double) * 64, c->stream));
+    const long n = g.ndofs();
     // node pairs iv <= iu (all pairs for non-symmetric integrals), packed iv << 16 | iu; the
     // host vectors of asynchronous uploads live until the function's last synchronisation
     std::vector<int> pairs;
+    for (int v = 0; v < EP.nloc; v++)
+        for (int u = full ? 0 : v; u < EP.nloc; u++) pairs.push_back(v << 16 | u);
+    // every temporary in the context's set-up scratch (no allocation or free per call once it
+    // has grown to this grid); 64 doubles of padding after E: k_asm_cells reads whole
+    // wavefronts past a table row's end
+    ScratchPlan plan;
+    const size_t o_E = plan.add(sizeof(double) * (Esize + 64)), o_pairs = plan.add(sizeof(int) * pairs.size());
+    const size_t o_emin = plan.add(sizeof(unsigned long long)), o_bad = plan.add(sizeof(int));
+    const size_t o_coeff = plan.add(coeff_host ? sizeof(double) * g.ncells() : 0);
+    const size_t o_counts = plan.add(sizeof(int) * (n > 0 ? n : 1)), o_scan = plan.add(scan_work_bytes(n));
+    unsigned char* ws = static_cast<unsigned char*>(ctx_scratch(c, SCRATCH_ASM, plan.total));
+    double* E = reinterpret_cast<double*>(ws + o_E);
+    int* pairs_d = reinterpret_cast<int*>(ws + o_pairs);
+    unsigned long long* emin_d = reinterpret_cast<unsigned long long*>(ws + o_emin);
+    int* bad = reinterpret_cast<int*>(ws + o_bad);
+    double* coeff = coeff_host ? reinterpret_cast<double*>(ws + o_coeff) : nullptr;
+    int* counts = reinterpret_cast<int*>(ws + o_counts);
+    long* scan_ws = reinterpret_cast<long*>(ws + o_scan);
+    JFEM_HIP(hipMemsetAsync(E + Esize, 0, sizeof(double) * 64, c->stream));
     {
-        for (int v = 0; v < EP.nloc; v++)
-            for (int u = full ? 0 : v; u < EP.nloc; u++) pairs.push_back(v << 16 | u);
-        pairs_d.alloc(pairs.size());
         JFEM_HIP(hipMemcpyAsync(pairs_d, pairs.data(), sizeof(int) * pairs.size(), hipMemcpyHostToDevice, c->stream));
         const long work = (long)pairs.size() * EP.ntuples * nint;
         const int eg = grid_for(work, 256, 16384);
@@ -1621,43 +1653,31 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
         }
         JFEM_HIP(hipGetLastError());
     }
-    // the smallest |entry| of the element tables, read back after the host work below
-    DevBuf<unsigned long long> emin_d;
-    unsigned long long emin_bits = 0;
+    // the smallest |entry| of the element tables, read back (into pinned memory) after the host
+    // work below
     const bool one_integral = nint == 1;
     if (one_integral) {
-        emin_d.alloc(1);
         JFEM_HIP(hipMemsetAsync(emin_d, 0xff, sizeof(unsigned long long), c->stream));
         k_absmin<<<grid_for(Esize, 256, 2048), 256, 0, c->stream>>>(Esize, E, emin_d);
         JFEM_HIP(hipGetLastError());
-        JFEM_HIP(hipMemcpyAsync(&emin_bits, emin_d, sizeof(emin_bits), hipMemcpyDeviceToHost, c->stream));
     }
-
-    double* coeff = nullptr;
-    if (coeff_host) {
-        JFEM_HIP(hipMalloc(&coeff, sizeof(double) * g.ncells()));
+    if (coeff_host)
         JFEM_HIP(hipMemcpyAsync(coeff, coeff_host, sizeof(double) * g.ncells(), hipMemcpyHostToDevice, c->stream));
-    }
     RowParams RP;
     std::memset(&RP, 0, sizeof(RP));
     RP.g = to_dev(g);
     RP.nloc = EP.nloc; RP.nint = nint; RP.ntuples = EP.ntuples; RP.E = E; RP.coeff = coeff;
     for (int a = 0; a < MAXD; a++) { RP.cls[a] = D.cls[a]; RP.ncls[a] = a < g.d ? C.ncls[a] : 1; }
 
-    const long n = g.ndofs();
     auto* A = new jfem_mat();
     A->ctx = c;
     A->rows = A->cols = n;
-    int* counts = nullptr;
-    JFEM_HIP(hipMalloc(&counts, sizeof(int) * n));
     JFEM_HIP(hipMalloc(&A->rowptr, sizeof(int) * (n + 1)));
     int maxW = 1;
     for (int a = 0; a < g.d; a++) maxW *= 2 * g.k + 1;
     const int G = maxW <= 16 ? 16 : (maxW <= 32 ? 32 : 64);
     const int groups_per_block = 256 / G;
     const int blocks = grid_for(n, groups_per_block, 65536);
-    int* bad = nullptr;
-    JFEM_HIP(hipMalloc(&bad, sizeof(int)));
     JFEM_HIP(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
     // tables in LDS when they fit in 48 KiB (any uniform or few-class grid)
     long cls_ints = 0;
@@ -1675,14 +1695,14 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     std::vector<unsigned short> fwd_h;
     std::vector<unsigned char> inv_h;
     std::vector<int> delta_h;
-    DevBuf<unsigned short> fwd_d;
-    DevBuf<unsigned char> inv_d;
-    DevBuf<int> delta_d;
+    unsigned short* fwd_d = nullptr;
+    unsigned char* inv_d = nullptr;
+    int* delta_d = nullptr;
     std::vector<AxEntry> axtab_h;
     std::vector<int> order_h;
-    DevBuf<int> order_d;
+    int* order_d = nullptr;
     AxOffsets aoff;
-    DevBuf<AxEntry> axtab_d;
+    AxEntry* axtab_d = nullptr;
     int inner_ok = 0;
     if (cellwise) {
         cell_asm_axes(g, C.cls, C.ncls, axtab_h, aoff);
@@ -1715,19 +1735,24 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
             } else if (g.d == 2) {
                 order_h = sorted[0];
             }
-            order_d.alloc(order_h.size());
-            JFEM_HIP(hipMemcpyAsync(order_d, order_h.data(), sizeof(int) * order_h.size(), hipMemcpyHostToDevice,
-                                    c->stream));
         }
-        axtab_d.alloc(axtab_h.size());
-        JFEM_HIP(hipMemcpyAsync(axtab_d, axtab_h.data(), sizeof(AxEntry) * axtab_h.size(), hipMemcpyHostToDevice,
-                                c->stream));
         cell_asm_tables(g.d, g.k, fwd_h, inv_h);
         fwd_h.resize(fwd_h.size() + 64, 0);   // whole wavefronts read past the last combination's entries
         inner_ok = cell_asm_delta(g, fwd_h, delta_h) ? 1 : 0;
-        fwd_d.alloc(fwd_h.size());
-        inv_d.alloc(inv_h.size());
-        delta_d.alloc(delta_h.size());
+        ScratchPlan cp;
+        const size_t o_order = cp.add(sizeof(int) * order_h.size()), o_ax = cp.add(sizeof(AxEntry) * axtab_h.size());
+        const size_t o_fwd = cp.add(sizeof(unsigned short) * fwd_h.size()), o_inv = cp.add(inv_h.size());
+        const size_t o_delta = cp.add(sizeof(int) * (delta_h.size() > 0 ? delta_h.size() : 1));
+        unsigned char* cs = static_cast<unsigned char*>(ctx_scratch(c, SCRATCH_CELLS, cp.total));
+        order_d = reinterpret_cast<int*>(cs + o_order);
+        axtab_d = reinterpret_cast<AxEntry*>(cs + o_ax);
+        fwd_d = reinterpret_cast<unsigned short*>(cs + o_fwd);
+        inv_d = cs + o_inv;
+        delta_d = reinterpret_cast<int*>(cs + o_delta);
+        JFEM_HIP(hipMemcpyAsync(order_d, order_h.data(), sizeof(int) * order_h.size(), hipMemcpyHostToDevice,
+                                c->stream));
+        JFEM_HIP(hipMemcpyAsync(axtab_d, axtab_h.data(), sizeof(AxEntry) * axtab_h.size(), hipMemcpyHostToDevice,
+                                c->stream));
         JFEM_HIP(hipMemcpyAsync(fwd_d, fwd_h.data(), sizeof(unsigned short) * fwd_h.size(), hipMemcpyHostToDevice,
                                 c->stream));
         JFEM_HIP(hipMemcpyAsync(inv_d, inv_h.data(), inv_h.size(), hipMemcpyHostToDevice, c->stream));
@@ -1811,11 +1836,12 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     JFEM_CHECK(nnz_bound < (1l << 31) - 1, JFEM_ERR_UNSUPPORTED, "matrix has 2^31 or more entries (int32 CSR indices)");
     JFEM_HIP(hipMalloc(&A->colidx, sizeof(int) * (nnz_bound + NNZ_PAD)));
     JFEM_HIP(hipMalloc(&A->vals, sizeof(double) * (nnz_bound + NNZ_PAD)));
+    if (one_integral) JFEM_HIP(hipMemcpyAsync(c->pin, emin_d, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
     JFEM_HIP(hipStreamSynchronize(c->stream));
     bool structural = one_integral;
     if (structural) {
         double emin;
-        std::memcpy(&emin, &emin_bits, sizeof(emin));
+        std::memcpy(&emin, c->pin, sizeof(emin));
         double cmin = 1.0;
         if (coeff_host)
             for (long i = 0; i < g.ncells(); i++) cmin = std::min(cmin, std::fabs(coeff_host[i]));
@@ -1836,23 +1862,29 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     } else {
         launch(false);
     }
-    const long nnz = scan_counts(c, counts, n, A->rowptr);
+    // with the structural pattern the entry count is nnz_bound: nothing is read back before the
+    // fill; the scan's total comes back with the fill's status and is checked then
+    const long tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    const long nnz = scan_counts(c, counts, n, A->rowptr, scan_ws, structural ? nnz_bound : -1);
     A->nnz = nnz;
     JFEM_CHECK(nnz <= nnz_bound, JFEM_ERR_STATE, "more entries than the structural pattern");
+    // the row blocks of the SpMV need only the row pointer: their count passes go before the
+    // fill, their counts come back with its status
+    const BlockSplit bs = mat_blocks_count(A);
     launch(true);
-    int bad_h = 0;
-    JFEM_HIP(hipMemcpyAsync(&bad_h, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    long* tail = c->pin + PIN_LONGS - 2;   // clear of the block counts
+    JFEM_HIP(hipMemcpyAsync(tail, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    JFEM_HIP(hipMemcpyAsync(tail + 1, scan_ws + (n > 0 ? n : 1) + tiles + 1, sizeof(long), hipMemcpyDeviceToHost,
+                            c->stream));
     JFEM_HIP(hipStreamSynchronize(c->stream));
-    (void)hipFree(bad);
-    (void)hipFree(counts);
-    (void)hipFree(E);
-    (void)hipFree(coeff);
-    if (bad_h) {
+    const int bad_h = *reinterpret_cast<const int*>(tail);
+    const long total_h = tail[1];
+    if (bad_h || total_h != nnz) {
         mat_free(A);
         JFEM_CHECK(false, JFEM_ERR_STATE, "assembled rows differ from their reserved lengths");
     }
     A->lanes = choose_lanes(n > 0 ? (double)nnz / n : 1.0);
-    mat_build_blocks(A);
+    mat_blocks_finish(A, bs);
     return A;
 }
 

@@ -154,6 +154,12 @@ struct jfem_ctx {
     size_t work_n = 0;                // doubles held by work
     double* noalloc = nullptr;        // this GPU's copy of the no-allocation scratch array
     int cu_count = 256;
+    // set-up scratch: device temporaries of assembly and matrix set-up in a few independent
+    // slots, grown on demand and kept (hipFree of a temporary costs a device-wide wait), and a
+    // pinned host area for small read-backs
+    unsigned char* scratch[3] = {nullptr, nullptr, nullptr};
+    size_t scratch_n[3] = {0, 0, 0};
+    long* pin = nullptr;              // PIN_LONGS values (pinned host memory)
 };
 
 struct jfem_space {
@@ -241,6 +247,21 @@ struct DevPtr {
 };
 // the context's solver workspace with room for n doubles (reallocated only to grow)
 double* ctx_work(jfem_ctx* c, size_t n);
+// set-up scratch slot `slot` (0: assembly, 1: assembly's cell tables, 2: CSR-stream block
+// split) with room for `bytes`; growing it waits for the context's stream first
+constexpr int SCRATCH_ASM = 0, SCRATCH_CELLS = 1, SCRATCH_SPLIT = 2;
+void* ctx_scratch(jfem_ctx* c, int slot, size_t bytes);
+constexpr size_t PIN_LONGS = 65536;
+// 256-byte aligned offsets of consecutive arrays in one scratch slot
+struct ScratchPlan {
+    size_t total = 0;
+    size_t add(size_t bytes)
+    {
+        const size_t o = total;
+        total += (bytes + 255) / 256 * 256;
+        return o;
+    }
+};
 // device address of the static scratch array of the no-allocation entry points on the context's
 // GPU (4 * JFEM_NOALLOC_MAX_ROWS doubles), resolved once per device by jfem_ctx_create
 double* noalloc_scratch(const jfem_ctx* c);
@@ -260,6 +281,10 @@ void spmv_dot_dev(jfem_mat* A, const double* x, double* y, double* dst);
 int choose_lanes(double mean_row);
 // rowptr[0..n] (device) from per-row counts (device); returns the total
 long scan_counts(jfem_ctx* c, const int* counts, long n, int* rowptr);
+// the same with a caller's workspace of scan_work_bytes(n) bytes; with known_total >= 0 (the
+// caller knows the sum) nothing is read back and nothing waits
+size_t scan_work_bytes(long n);
+long scan_counts(jfem_ctx* c, const int* counts, long n, int* rowptr, long* work, long known_total);
 // lexicographic node rank (last axis fastest) of every DOF of a scalar space, into a device array
 void space_lex_rank(jfem_space* s, int* out_dev);
 // finest grid of a multigrid hierarchy
@@ -272,6 +297,16 @@ hipStream_t mg_stream(const jfem_mg* mg);
 void reduce_partials_dev(jfem_ctx* c, int slots, int blocks, double* dst_dev);
 void mat_make_f32(jfem_mat* A);
 void mat_build_blocks(jfem_mat* A);
+// the same in two phases around a wait on the context's stream that the caller makes anyway:
+// mat_blocks_count queues the count passes and their read-back (into ctx->pin[0, 3 nseg),
+// clear of its last 16 values), mat_blocks_finish completes the split after that wait
+struct BlockSplit {
+    long nseg = 0;
+    bool pinned = false;
+    size_t o_cnt = 0, o_off = 0;
+};
+BlockSplit mat_blocks_count(jfem_mat* A);
+void mat_blocks_finish(jfem_mat* A, const BlockSplit& bs);
 // sum of n partials (device) into dst (device), deterministic
 void reduce_array_dev(jfem_ctx* c, const double* partials, long n, double* dst);
 void mat_diag(jfem_mat* A, double* out);

@@ -232,6 +232,21 @@ double* ctx_work(jfem_ctx* c, size_t n)
     return c->work;
 }
 
+void* ctx_scratch(jfem_ctx* c, int slot, size_t bytes)
+{
+    JFEM_CHECK(slot >= 0 && slot < 3, JFEM_ERR_ARG, "scratch slot out of range");
+    if (bytes > c->scratch_n[slot]) {
+        // the old buffer may still be read by work queued on the stream
+        JFEM_HIP(hipStreamSynchronize(c->stream));
+        (void)hipFree(c->scratch[slot]);
+        c->scratch[slot] = nullptr;
+        c->scratch_n[slot] = 0;
+        JFEM_HIP(hipMalloc(&c->scratch[slot], bytes));
+        c->scratch_n[slot] = bytes;
+    }
+    return c->scratch[slot];
+}
+
 }  // namespace jfem
 
 extern "C" {
@@ -253,6 +268,7 @@ int jfem_ctx_create(int device, jfem_ctx** out)
     JFEM_HIP(hipMalloc(&c->red_result, sizeof(double) * RED_SLOTS));
     JFEM_HIP(hipMalloc(&c->scal, sizeof(double) * 64));
     JFEM_HIP(hipHostMalloc(&c->red_host, sizeof(double) * RED_SLOTS));
+    JFEM_HIP(hipHostMalloc(&c->pin, sizeof(long) * PIN_LONGS));
     hipDeviceProp_t prop;
     JFEM_HIP(hipGetDeviceProperties(&prop, device));
     c->cu_count = prop.multiProcessorCount;
@@ -271,7 +287,9 @@ int jfem_ctx_destroy(jfem_ctx* c)
     (void)hipFree(c->red_result);
     (void)hipFree(c->scal);
     (void)hipFree(c->work);
+    for (unsigned char* p : c->scratch) (void)hipFree(p);
     (void)hipHostFree(c->red_host);
+    (void)hipHostFree(c->pin);
     (void)hipStreamDestroy(c->stream);
     delete c;
     API_END

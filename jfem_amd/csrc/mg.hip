@@ -56,6 +56,8 @@ struct LearnedState {
     float* stat = nullptr;       // np x (cin - 1) static input channels
     int* perm = nullptr;         // lattice index of every finest DOF
     float* stat_dof = nullptr;   // the static channels in DOF order
+    float* slog = nullptr;       // the log-coefficient channel in lattice order (nullptr: zero)
+    int* ip = nullptr;           // DOF of every lattice pixel, -1 on Dirichlet nodes
     int runs = 1;                // learned steps per smoothing call
     int sides = 3;               // smoothing calls that use the network: 1 pre-smoothing, 2 post-smoothing, 3 both
     int active = 0;
@@ -739,7 +741,7 @@ static void free_mg(jfem_mg* G)
     (void)hipFree(G->coarse_inv);
     (void)hipFree(G->init); (void)hipFree(G->defect); (void)hipFree(G->tmp);
     LearnedState& S = G->ls;
-    for (void* p : {(void*)S.stat, (void*)S.stat_dof, (void*)S.perm, (void*)S.rr, (void*)S.u, (void*)S.z, (void*)S.gd, (void*)S.rh, (void*)S.xs,
+    for (void* p : {(void*)S.stat, (void*)S.stat_dof, (void*)S.slog, (void*)S.ip, (void*)S.perm, (void*)S.rr, (void*)S.u, (void*)S.z, (void*)S.gd, (void*)S.rh, (void*)S.xs,
                     (void*)S.bs, (void*)S.loss, (void*)S.capbuf, (void*)S.capx, (void*)S.xt, (void*)S.eh,
                     (void*)S.w, (void*)S.y, (void*)S.loss2})
         (void)hipFree(p);
@@ -1021,43 +1023,38 @@ __global__ void k_gather_bf16(const int* __restrict__ perm, long n, const T* __r
 }
 
 // the network input of one plain learned step, from the residual epilogue of the level
-// operator: v = D^-1 (b - A x) and v times each static channel, as BF16 at the node's lattice
-// position.  The attached network has no biases, so N(c v) = c N(v) for c > 0 and the input
-// needs no normalisation: x += N(v) is the step x += rms(v) N(v / rms(v)).
-__device__ inline void netin_store(float v, const float* __restrict__ stat, long row, int cin, int pix,
-                                   unsigned short* __restrict__ inb)
-{
-    bf16x8 h;
-    h[0] = (short)f2bf(v);
-#pragma unroll
-    for (int c = 1; c < 8; c++) h[c] = (short)f2bf(c < cin ? v * stat[row * (cin - 1) + c - 1] : 0.f);
-    *reinterpret_cast<bf16x8*>(inb + (long)pix * 8) = h;
-}
-
+// operator: v = D^-1 (b - A x) in FP32 at the node's lattice position (the first layer forms
+// the static channels from it).  The attached network has no biases, so N(c v) = c N(v) for
+// c > 0 and the input needs no normalisation: x += N(v) is the step x += rms(v) N(v / rms(v)).
 template <typename T>
 struct EpiNetIn {
     static constexpr bool reduce = false;
     const T *b, *dinv;
-    const float* stat;   // static channels in DOF order, cin - 1 per node
     const int* perm;
-    unsigned short* inb;
-    int cin;
+    float* vin;
     struct Pre { T b, dinv; int pix; };
     __device__ Pre load(long row) const { return {b[row], dinv[row], perm[row]}; }
     __device__ double operator()(long row, T ax, const Pre& p) const
     {
-        netin_store((float)(p.dinv * (p.b - ax)), stat, row, cin, p.pix, inb);
+        vin[p.pix] = (float)(p.dinv * (p.b - ax));
         return 0.0;
     }
 };
 
 // the same input for x = 0 (r = b)
 template <typename T>
-__global__ void k_netin_b(long n, const T* __restrict__ b, const T* __restrict__ dinv, const float* __restrict__ stat,
-                          const int* __restrict__ perm, int cin, unsigned short* __restrict__ inb)
+__global__ void k_netin_b(long n, const T* __restrict__ b, const T* __restrict__ dinv, const int* __restrict__ perm,
+                          float* __restrict__ vin)
 {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        netin_store((float)(dinv[i] * b[i]), stat, i, cin, perm[i], inb);
+        vin[perm[i]] = (float)(dinv[i] * b[i]);
+}
+
+// ip[perm[i]] = i on free DOFs, -1 on Dirichlet DOFs
+__global__ void k_pixel_dof(const int* __restrict__ perm, long n, const unsigned char* __restrict__ bnd, int* __restrict__ ip)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        ip[perm[i]] = bnd[i] ? -1 : (int)i;
 }
 
 // static channels in DOF order: out[i][c] = stat[perm[i]][c]
@@ -1077,17 +1074,6 @@ __global__ void k_scatter_add(const int* __restrict__ perm, long n, const float*
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         if (bnd[i]) continue;
         x[i] = (T)((double)x[i] + (double)out[perm[i]] * inv);
-    }
-}
-
-// x[i] += out[perm[i]] on free nodes (the unnormalised plain step)
-template <typename T>
-__global__ void k_add_out(const int* __restrict__ perm, long n, const float* __restrict__ out,
-                          const unsigned char* __restrict__ bnd, T* __restrict__ x)
-{
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        if (bnd[i]) continue;
-        x[i] = (T)((double)x[i] + (double)out[perm[i]]);
     }
 }
 
@@ -1231,17 +1217,16 @@ static void learned_smooth(jfem_mg* G, MgLevel& L, const T* b, T* x, T* u, bool 
     T* w = (T*)L.dv;
     for (int it = 0; it < S.runs; it++) {
         if (!S.mr && nn_bf16_input(S.nn, S.lat)) {
-            // plain step on the MFMA network: the residual epilogue writes the network input
-            // (DOF order -> lattice order); x += N(v) on the free nodes gathers the output back
+            // plain step on the MFMA network: the residual epilogue writes v at the nodes'
+            // lattice positions; the last layer adds N(v) into x on the free nodes
             if (x_zero && it == 0) {
                 k_fill_t<T><<<gr, 256, 0, st>>>(n, (T)0, x);
-                k_netin_b<T><<<gr, 256, 0, st>>>(n, b, dinv_of<T>(L), S.stat_dof, S.perm, S.nn->cin, S.nn->inb);
+                k_netin_b<T><<<gr, 256, 0, st>>>(n, b, dinv_of<T>(L), S.perm, S.nn->in);
             } else {
-                level_apply<T>(G, L, x, EpiNetIn<T>{b, dinv_of<T>(L), S.stat_dof, S.perm, S.nn->inb, S.nn->cin});
+                level_apply<T>(G, L, x, EpiNetIn<T>{b, dinv_of<T>(L), S.perm, S.nn->in});
             }
-            nn_forward(S.nn, S.lat, true);
-            k_add_out<T><<<gr, 256, 0, st>>>(S.perm, n, S.nn->out, L.bnd, x);
             JFEM_HIP(hipGetLastError());
+            nn_forward_plain(S.nn, S.lat, S.slog, S.ip, x, sizeof(T) == 8);
             continue;
         }
         if (x_zero && it == 0) {
@@ -1401,6 +1386,19 @@ static void set_learned(jfem_mg* G, jfem_nn* nn, const double* cell_logc, int ru
     JFEM_HIP(hipMalloc(&S.stat_dof, sizeof(float) * stat.size()));
     k_stat_to_dof<<<grid_for(np, 256, 16384), 256, 0, G->ctx->stream>>>(np, ns, S.perm, S.stat, S.stat_dof);
     JFEM_HIP(hipGetLastError());
+    (void)hipFree(S.ip);
+    S.ip = nullptr;
+    JFEM_HIP(hipMalloc(&S.ip, sizeof(int) * np));
+    k_pixel_dof<<<grid_for(np, 256, 16384), 256, 0, G->ctx->stream>>>(S.perm, np, F.bnd, S.ip);
+    JFEM_HIP(hipGetLastError());
+    (void)hipFree(S.slog);
+    S.slog = nullptr;
+    if (cell_logc) {
+        std::vector<float> sl((size_t)np);
+        for (long pix = 0; pix < np; pix++) sl[(size_t)pix] = stat[(size_t)pix * ns];
+        JFEM_HIP(hipMalloc(&S.slog, sizeof(float) * np));
+        JFEM_HIP(hipMemcpy(S.slog, sl.data(), sizeof(float) * np, hipMemcpyHostToDevice));
+    }
     if (!S.rr) JFEM_HIP(hipMalloc(&S.rr, sizeof(double) * 8));
     nn_reserve(nn, np);
     nn_set_bias_free(nn, 1);   // N(0) = 0 and N(c r) = c N(r): a consistent smoother

@@ -469,21 +469,14 @@ struct ConvIn {
     static constexpr int TAPS = D == 3 ? 27 : 9, NK = (TAPS + 3) / 4;
 };
 
+// A fragments of the first layer (K step m, half h, lane (i, q): W0[t][16 h + i][0..7] for
+// tap t = 4 m + q) and this lane's halo-slot offset of tap 4 m + q (the last tap past the end)
 template <int D>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv_in(
-    Lat L, int cin, const unsigned short* __restrict__ in, const float* __restrict__ W0, const float* __restrict__ b0,
-    unsigned short* __restrict__ out)
+__device__ inline void conv_in_setup(int cin, const float* __restrict__ W0, bf16x8 (&aw)[ConvIn<D>::NK][2],
+                                     int (&toff)[ConvIn<D>::NK])
 {
     using T = ConvIn<D>;
-    __shared__ bf16x8 buf0[T::BUF];
-    __shared__ bf16x8 buf1[T::BUF];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int j = lane & 15, q = lane >> 4;
-    const int wy0 = D == 3 ? (wave & 1) * T::WY : wave * T::WY, wz0 = D == 3 ? (wave >> 1) * T::WZ : 0;
-    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
-    // A fragments: K step m, half h, lane (i, q): W0[t][16 h + i][0..7] for tap t = 4 m + q
-    bf16x8 aw[T::NK][2];
-    int toff[T::NK];   // this lane's halo-slot offset of tap 4 m + q (the last tap past the end)
+    const int lane = threadIdx.x & 63, j = lane & 15, q = lane >> 4;
 #pragma unroll
     for (int m = 0; m < T::NK; m++) {
         const int t = 4 * m + q, tt = t < T::TAPS ? t : T::TAPS - 1;
@@ -494,11 +487,105 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             for (int c = 0; c < 8; c++)
                 aw[m][h][c] = (short)f2bf(t < T::TAPS && c < cin ? W0[(t * NN_H + 16 * h + j) * cin + c] : 0.f);
     }
-    float bv[2][4];
+}
+
+// the first layer's MFMAs of this wave's 8 rows (both output halves) from the channel tile cur
+template <int D>
+__device__ inline void conv_in_mfma(const bf16x8* cur, const bf16x8 (&aw)[ConvIn<D>::NK][2],
+                                    const int (&toff)[ConvIn<D>::NK], int wy0, int wz0, f32x4 (&acc)[ConvIn<D>::NR][2])
+{
+    using T = ConvIn<D>;
+    const int j = threadIdx.x & 15;
+    const bf16x8* base = cur + (wz0 * T::HY + wy0) * T::HX + j;
+    bf16x8 bf[2][T::NK];
 #pragma unroll
-    for (int h = 0; h < 2; h++)
+    for (int m = 0; m < T::NK; m++) bf[0][m] = base[toff[m]];
 #pragma unroll
-        for (int r = 0; r < 4; r++) bv[h][r] = b0[16 * h + 4 * q + r];
+    for (int u = 0; u < T::NR; u++) {
+        if (u + 1 < T::NR) {
+            const int rn = (((u + 1) / T::WY) * T::HY + (u + 1) % T::WY) * T::HX;   // the next row's halo-slot offset
+#pragma unroll
+            for (int m = 0; m < T::NK; m++) bf[(u + 1) & 1][m] = base[rn + toff[m]];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            acc[u][h] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int m = 0; m < T::NK; m++)
+                acc[u][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bfv8, aw[m][h]),
+                                                                    __builtin_bit_cast(bfv8, bf[u & 1][m]),
+                                                                    acc[u][h], 0, 0, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// bias, ReLU and the BF16 stores of the tile at (x0, y0, z0): 2 NR stores per wave, pixels
+// outside the lattice into kTrash
+template <int D>
+__device__ inline void conv_in_store(const Lat& L, const f32x4 (&acc)[ConvIn<D>::NR][2], const float* __restrict__ b0,
+                                     int wy0, int wz0, int x0, int y0, int z0, unsigned short* __restrict__ out)
+{
+    using T = ConvIn<D>;
+    const int lane = threadIdx.x & 63, j = lane & 15, q = lane >> 4;
+    const float4 bq0 = *reinterpret_cast<const float4*>(b0 + 4 * q), bq1 = *reinterpret_cast<const float4*>(b0 + 16 + 4 * q);
+    const float bv[2][4] = {{bq0.x, bq0.y, bq0.z, bq0.w}, {bq1.x, bq1.y, bq1.z, bq1.w}};
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    const long sy = (long)nx * NN_H, sz = (long)nx * ny * NN_H;
+    const bool xin = x0 + j < nx;
+    const long p0 = (long)(x0 + j) + (long)nx * ((y0 + wy0) + (long)ny * (z0 + wz0));
+#pragma unroll
+    for (int u = 0; u < T::NR; u++) {
+        const int oy = u % T::WY, oz = u / T::WY;
+        const bool ok = xin && y0 + wy0 + oy < ny && z0 + wz0 + oz < nz;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const long e = p0 * NN_H + oy * sy + oz * sz + 16 * h + 4 * q;
+            float v[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const float x = acc[u][h][r] + bv[h][r];
+                v[r] = x > 0.f ? x : 0.f;
+            }
+            uint2 pk;
+            pk.x = pack_bf16x2(v[0], v[1]);
+            pk.y = pack_bf16x2(v[2], v[3]);
+            uint2* dst = ok ? reinterpret_cast<uint2*>(out + e) : reinterpret_cast<uint2*>(kTrash + lane);
+            *dst = pk;
+        }
+    }
+}
+
+// the tile range of this workgroup: tiles of XCD x (workgroups b with b % 8 = x) contiguous
+__device__ inline void xcd_tiles(int ntile, int& tb, int& tend, int& tstep)
+{
+    if ((gridDim.x & 7) == 0) {
+        const int x = blockIdx.x & 7;
+        tb = (int)((long)x * ntile / 8) + (int)(blockIdx.x >> 3);
+        tend = (int)((long)(x + 1) * ntile / 8);
+        tstep = gridDim.x >> 3;
+    } else {
+        tb = blockIdx.x;
+        tend = ntile;
+        tstep = gridDim.x;
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv_in(
+    Lat L, int cin, const unsigned short* __restrict__ in, const float* __restrict__ W0, const float* __restrict__ b0,
+    unsigned short* __restrict__ out)
+{
+    using T = ConvIn<D>;
+    __shared__ bf16x8 buf0[T::BUF];
+    __shared__ bf16x8 buf1[T::BUF];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wy0 = D == 3 ? (wave & 1) * T::WY : wave * T::WY, wz0 = D == 3 ? (wave >> 1) * T::WZ : 0;
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    bf16x8 aw[T::NK][2];
+    int toff[T::NK];
+    conv_in_setup<D>(cin, W0, aw, toff);
     const unsigned nbytes = (unsigned)L.np * 16;   // below 2^31: checked at the launch
     const int ntx = (nx + T::TX - 1) / T::TX, nty = (ny + T::TY - 1) / T::TY, ntz = (nz + T::TZ - 1) / T::TZ;
     const int ntile = ntx * nty * ntz;
@@ -519,18 +606,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         }
     };
     int tb, tend, tstep;
-    if ((gridDim.x & 7) == 0) {
-        const int x = blockIdx.x & 7;
-        tb = (int)((long)x * ntile / 8) + (int)(blockIdx.x >> 3);
-        tend = (int)((long)(x + 1) * ntile / 8);
-        tstep = gridDim.x >> 3;
-    } else {
-        tb = blockIdx.x;
-        tend = ntile;
-        tstep = gridDim.x;
-    }
+    xcd_tiles(ntile, tb, tend, tstep);
     if (tb < tend) stage((tb % ntx) * T::TX, (tb / ntx % nty) * T::TY, tb / (ntx * nty) * T::TZ, buf0);
-    const long sy = (long)nx * NN_H, sz = (long)nx * ny * NN_H;
     for (int it = 0; tb < tend; tb += tstep, it++) {
         const bf16x8* cur = (it & 1) ? buf1 : buf0;
         bf16x8* next = (it & 1) ? buf0 : buf1;
@@ -542,56 +619,129 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         const int tn = tb + tstep;
         if (tn < tend) stage((tn % ntx) * T::TX, (tn / ntx % nty) * T::TY, tn / (ntx * nty) * T::TZ, next);
         f32x4 acc[T::NR][2];
-        const bf16x8* base = cur + (wz0 * T::HY + wy0) * T::HX + j;
-        bf16x8 bf[2][T::NK];
-#pragma unroll
-        for (int m = 0; m < T::NK; m++) bf[0][m] = base[toff[m]];
-#pragma unroll
-        for (int u = 0; u < T::NR; u++) {
-            const int ro = ((u / T::WY) * T::HY + u % T::WY) * T::HX;   // the row's halo-slot offset
-            if (u + 1 < T::NR) {
-                const int rn = (((u + 1) / T::WY) * T::HY + (u + 1) % T::WY) * T::HX;
-#pragma unroll
-                for (int m = 0; m < T::NK; m++) bf[(u + 1) & 1][m] = base[rn + toff[m]];
-            }
-            (void)ro;
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                acc[u][h] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int m = 0; m < T::NK; m++)
-                    acc[u][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bfv8, aw[m][h]),
-                                                                        __builtin_bit_cast(bfv8, bf[u & 1][m]),
-                                                                        acc[u][h], 0, 0, 0);
-            }
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        const int x0 = (tb % ntx) * T::TX, y0 = (tb / ntx % nty) * T::TY, z0 = tb / (ntx * nty) * T::TZ;
-        const bool xin = x0 + j < nx;
-        const long p0 = (long)(x0 + j) + (long)nx * ((y0 + wy0) + (long)ny * (z0 + wz0));
-#pragma unroll
-        for (int u = 0; u < T::NR; u++) {
-            const int oy = u % T::WY, oz = u / T::WY;
-            const bool ok = xin && y0 + wy0 + oy < ny && z0 + wz0 + oz < nz;
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const long e = p0 * NN_H + oy * sy + oz * sz + 16 * h + 4 * q;
-                float v[4];
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const float x = acc[u][h][r] + bv[h][r];
-                    v[r] = x > 0.f ? x : 0.f;
-                }
-                uint2 pk;
-                pk.x = pack_bf16x2(v[0], v[1]);
-                pk.y = pack_bf16x2(v[2], v[3]);
-                uint2* dst = ok ? reinterpret_cast<uint2*>(out + e) : reinterpret_cast<uint2*>(kTrash + lane);
-                *dst = pk;
-            }
-        }
+        conv_in_mfma<D>(cur, aw, toff, wy0, wz0, acc);
+        conv_in_store<D>(L, acc, b0, wy0, wz0, (tb % ntx) * T::TX, (tb / ntx % nty) * T::TY, tb / (ntx * nty) * T::TZ,
+                         out);
     }
     // no LDS load may still be in flight when the workgroup's LDS is released
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+// buffer_load_dword ... lds: lane l's 4 bytes at byte offset voff of the buffer [base, base +
+// nbytes) land at lds + 4 l (lds is wave-uniform); offsets at or past nbytes read zeros
+__device__ inline void lds_dma4(const void* base, unsigned nbytes, unsigned voff, float* lds)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)nbytes, 0x00020000);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)lds, 4, (int)voff, 0, 0, 0);
+#endif
+}
+
+// First layer of the plain learned step, from the scaled residual v alone (np FP32 values in
+// lattice order) instead of a stored 8-channel input: the channels are v, v times the static
+// log-coefficient channel slog (lattice order; nullptr: zero) and, for cin = 2 + d, v times
+// the pixel's lattice parity along each axis, each rounded to BF16 -- bit for bit the values
+// of the input k_conv_in reads.  Each wave stages the halo groups k = w, w + 4, ... of the
+// next tile (v and slog, 4 bytes per pixel, by buffer loads into LDS) before this tile's
+// MFMAs and converts the same groups into the next tile's 16-byte channel tile after its
+// stores, so one barrier per tile orders the two channel tiles; the MFMAs and stores are
+// those of k_conv_in.
+template <int D>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv_in_v(
+    Lat L, int cin, const float* __restrict__ vin, const float* __restrict__ slog, const float* __restrict__ W0,
+    const float* __restrict__ b0, unsigned short* __restrict__ out)
+{
+    using T = ConvIn<D>;
+    static_assert(T::TX % 2 == 0 && T::TY % 2 == 0 && (D == 2 || T::TZ % 2 == 0), "even tile origins");
+    __shared__ bf16x8 ch0[T::BUF];
+    __shared__ bf16x8 ch1[T::BUF];
+    __shared__ float vb[T::BUF];
+    __shared__ float lb[T::BUF];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wy0 = D == 3 ? (wave & 1) * T::WY : wave * T::WY, wz0 = D == 3 ? (wave >> 1) * T::WZ : 0;
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    bf16x8 aw[T::NK][2];
+    int toff[T::NK];
+    conv_in_setup<D>(cin, W0, aw, toff);
+    const unsigned nbytes = (unsigned)L.np * 4;   // below 2^31: checked at the launch
+    const int ntx = (nx + T::TX - 1) / T::TX, nty = (ny + T::TY - 1) / T::TY, ntz = (nz + T::TZ - 1) / T::TZ;
+    const int ntile = ntx * nty * ntz;
+    constexpr int KW = (T::NG + T::NW - 1) / T::NW;   // halo groups per wave (at most)
+    // halo coordinates of this lane's pixel in group k = wave + NW i (8 bits each; all ones
+    // past the halo)
+    unsigned hc[KW];
+#pragma unroll
+    for (int i = 0; i < KW; i++) {
+        const int hp = 64 * (wave + T::NW * i) + lane;
+        const int hx = hp % T::HX, hy = (hp / T::HX) % T::HY, hz = hp / (T::HX * T::HY);
+        hc[i] = hp < T::HP ? (unsigned)hx | (unsigned)hy << 8 | (unsigned)hz << 16 : 0xffffffu;
+    }
+    // v (and slog) of this wave's halo groups of the tile at (x0, y0, z0); pixels outside the
+    // lattice or past the halo read zeros
+    auto stage = [&](int x0, int y0, int z0) {
+#pragma unroll
+        for (int i = 0; i < KW; i++) {
+            const int k = wave + T::NW * i;
+            if (k >= T::NG) break;   // wave-uniform
+            const int gx = x0 - 1 + (int)(hc[i] & 0xff), gy = y0 - 1 + (int)((hc[i] >> 8) & 0xff);
+            const int gz = D == 3 ? z0 - 1 + (int)(hc[i] >> 16) : 0;
+            const bool ok = hc[i] != 0xffffffu && (unsigned)gx < (unsigned)nx && (unsigned)gy < (unsigned)ny &&
+                            (unsigned)gz < (unsigned)nz;
+            const unsigned off = ((unsigned)gx + (unsigned)nx * ((unsigned)gy + (unsigned)ny * (unsigned)gz)) * 4u;
+            lds_dma4(vin, nbytes, ok ? off : nbytes, vb + 64 * k);
+            if (slog) lds_dma4(slog, nbytes, ok ? off : nbytes, lb + 64 * k);
+        }
+    };
+    // the same groups as BF16 input channels into the channel tile ch
+    auto convert = [&](int x0, int y0, int z0, bf16x8* ch) {
+#pragma unroll
+        for (int i = 0; i < KW; i++) {
+            const int k = wave + T::NW * i;
+            if (k >= T::NG) break;   // wave-uniform
+            const int hp = 64 * k + lane;
+            const float v = vb[hp], sl = slog ? lb[hp] : 0.f;
+            const float vz = v * 0.f;   // v times a zero parity channel
+            // the tile origin is even on every axis, so pixel g = origin - 1 + h has parity 1
+            // along an axis iff its halo coordinate h is even there
+            const unsigned ev = ~hc[i];
+            bf16x8 h;
+            h[0] = (short)f2bf(v);
+            h[1] = (short)f2bf(cin > 1 ? v * sl : 0.f);
+            h[2] = (short)f2bf(cin > 2 ? ((ev & 1u) ? v : vz) : 0.f);
+            h[3] = (short)f2bf(cin > 3 ? ((ev & 0x100u) ? v : vz) : 0.f);
+            h[4] = (short)f2bf(D == 3 && cin > 4 ? ((ev & 0x10000u) ? v : vz) : 0.f);
+            h[5] = h[6] = h[7] = 0;
+            if (hp < T::HP) ch[hp] = h;
+        }
+    };
+    int tb, tend, tstep;
+    xcd_tiles(ntile, tb, tend, tstep);
+    if (tb < tend) {
+        const int x0 = (tb % ntx) * T::TX, y0 = (tb / ntx % nty) * T::TY, z0 = tb / (ntx * nty) * T::TZ;
+        stage(x0, y0, z0);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        convert(x0, y0, z0, ch0);
+    }
+    for (int it = 0; tb < tend; tb += tstep, it++) {
+        const bf16x8* cur = (it & 1) ? ch1 : ch0;
+        bf16x8* next = (it & 1) ? ch0 : ch1;
+        // every wave has converted its groups of this tile and is done with the other tile
+        __syncthreads();
+        const int tn = tb + tstep;
+        const bool more = tn < tend;
+        const int nx0 = (tn % ntx) * T::TX, ny0 = (tn / ntx % nty) * T::TY, nz0 = tn / (ntx * nty) * T::TZ;
+        if (more) stage(nx0, ny0, nz0);
+        f32x4 acc[T::NR][2];
+        conv_in_mfma<D>(cur, aw, toff, wy0, wz0, acc);
+        conv_in_store<D>(L, acc, b0, wy0, wz0, (tb % ntx) * T::TX, (tb / ntx % nty) * T::TY, tb / (ntx * nty) * T::TZ,
+                         out);
+        if (more) {
+            // this wave's v (and slog) groups of the next tile have landed; the tile's 2 NR
+            // stores, issued after them, may still be in flight
+            asm volatile("s_waitcnt vmcnt(%0)" ::"n"(2 * T::NR) : "memory");
+            convert(nx0, ny0, nz0, next);
+        }
+    }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
@@ -622,10 +772,14 @@ struct ConvOut {
     static constexpr int NOUT = TX * TY * TZ;     // 224 output pixels per tile
 };
 
-template <int D>
+// MODE 0 writes out[p]; MODE 1 (FP32 x) and 2 (FP64 x) add the output into x at the DOF of
+// pixel p instead, x[ip[p]] = x + out in FP64 rounded to x's type, where ip[p] >= 0 (the
+// pixel's DOF is free; ip = -1 on Dirichlet nodes): the plain learned step without storing
+// its output
+template <int D, int MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv_out(
     Lat L, const unsigned short* __restrict__ act, const unsigned short* __restrict__ W3h,
-    const float* __restrict__ b3, float* __restrict__ out)
+    const float* __restrict__ b3, float* __restrict__ out, void* __restrict__ x, const int* __restrict__ ip)
 {
     using T = ConvOut<D>;
     __shared__ float ct[T::NROW * 16 * T::TAPS];
@@ -644,18 +798,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const int ntx = (nx + T::TX - 1) / T::TX, nty = (ny + T::TY - 1) / T::TY, ntz = (nz + T::TZ - 1) / T::TZ;
     const int ntile = ntx * nty * ntz;
     int tb, tend, tstep;
-    if ((gridDim.x & 7) == 0) {
-        const int x = blockIdx.x & 7;
-        tb = (int)((long)x * ntile / 8) + (int)(blockIdx.x >> 3);
-        tend = (int)((long)(x + 1) * ntile / 8);
-        tstep = gridDim.x >> 3;
-    } else {
-        tb = blockIdx.x;
-        tend = ntile;
-        tstep = gridDim.x;
-    }
+    xcd_tiles(ntile, tb, tend, tstep);
     for (; tb < tend; tb += tstep) {
         const int x0 = (tb % ntx) * T::TX, y0 = (tb / ntx % nty) * T::TY, z0 = tb / (ntx * nty) * T::TZ;
+        // this thread's output pixel of phase 2; in the add modes its DOF is loaded with the halo
+        // rows and x at that DOF after the MFMAs (a load issued before the halo rows would hold up
+        // their wait), so that both latencies overlap the MFMAs, the barrier and the tap sums
+        const int ox = threadIdx.x % T::TX, oy = (threadIdx.x / T::TX) % T::TY, oz = threadIdx.x / (T::TX * T::TY);
+        const int px = x0 + ox, py = y0 + oy, pz = z0 + oz;
+        const bool mine = threadIdx.x < T::NOUT && px < nx && py < ny && pz < nz;
+        const long p = (long)px + (long)nx * (py + (long)ny * pz);
+        int di = -1;
+        if (MODE != 0 && mine) di = ip[p];
         // phase 1: wave w takes halo rows w, w + 4, ...; all of its row loads first
         constexpr int RPW = (T::NROW + T::NW - 1) / T::NW;
         bf16x8 a[RPW];
@@ -686,18 +840,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                     for (int r = 0; r < 4; r++) ct[(row * 16 + 4 * kq + r) * T::TAPS + t] = d[r];
             }
         }
+        double xo = 0.0;
+        if (MODE == 1 && di >= 0) xo = (double)static_cast<const float*>(x)[di];
+        if (MODE == 2 && di >= 0) xo = static_cast<const double*>(x)[di];
         __syncthreads();
         // phase 2: one output pixel per thread
-        if (threadIdx.x < T::NOUT) {
-            const int ox = threadIdx.x % T::TX, oy = (threadIdx.x / T::TX) % T::TY, oz = threadIdx.x / (T::TX * T::TY);
+        if (mine) {
             float sum = bias;
 #pragma unroll
             for (int t = 0; t < T::TAPS; t++) {
                 const int dx = t % 3, dy = (t / 3) % 3, dz = D == 3 ? t / 9 : 0;
                 sum += ct[(((oz + dz) * T::HY + oy + dy) * 16 + ox + dx) * T::TAPS + t];
             }
-            const int px = x0 + ox, py = y0 + oy, pz = z0 + oz;
-            if (px < nx && py < ny && pz < nz) out[(long)px + (long)nx * (py + (long)ny * pz)] = sum;
+            if (MODE == 0) out[p] = sum;
+            else if (MODE == 1 && di >= 0) static_cast<float*>(x)[di] = (float)(xo + (double)sum);
+            else if (MODE == 2 && di >= 0) static_cast<double*>(x)[di] = xo + (double)sum;
         }
         __syncthreads();
     }
@@ -1050,6 +1207,44 @@ bool nn_bf16_input(const jfem_nn* nn, const Lat& L)
            lds_per_block() >= (long)sizeof(bf16x8) * 2 * Conv32<3>::BUF;
 }
 
+// persistent grids of the first and last layers: two workgroups per CU, a multiple of 8 for
+// the XCD tile ranges
+static int grid_in(const Lat& L)
+{
+    const long tx = (L.n[0] + 15) / 16;
+    const long nt = L.d == 3 ? tx * ((L.n[1] + 7) / 8) * ((L.n[2] + 3) / 4) : tx * ((L.n[1] + 31) / 32);
+    return (int)(nt >= 512 ? 512 : (nt >= 8 ? nt & ~7L : nt));
+}
+static int grid_out(const Lat& L)
+{
+    const long tox = (L.n[0] + 13) / 14;
+    const long nto = L.d == 3 ? tox * ((L.n[1] + 3) / 4) * ((L.n[2] + 3) / 4) : tox * ((L.n[1] + 15) / 16);
+    return (int)(nto >= 512 ? 512 : (nto >= 8 ? nto & ~7L : nto));
+}
+
+void nn_forward_plain(jfem_nn* nn, const Lat& L, const float* slog, const int* ip, void* x, bool x_double)
+{
+    if (!nn_bf16_input(nn, L))
+        throw Error{JFEM_ERR_STATE, "nn_forward_plain: lattice without the LDS-tiled layers"};
+    if (L.d == 3 && nn->cin != 2 && nn->cin != 5) throw Error{JFEM_ERR_STATE, "nn_forward_plain: 2 or 2 + d input channels"};
+    if (L.d == 2 && nn->cin != 2 && nn->cin != 4) throw Error{JFEM_ERR_STATE, "nn_forward_plain: 2 or 2 + d input channels"};
+    long o[9];
+    nn_offsets(nn, o);
+    hipStream_t st = nn->ctx->stream;
+    const float *W0 = nn->w + o[0], *b0 = nn->w + o[1];
+    const int gi = grid_in(L), go = grid_out(L);
+    if (L.d == 3) k_conv_in_v<3><<<gi, ConvIn<3>::NW * 64, 0, st>>>(L, nn->cin, nn->in, slog, W0, b0, nn->act0);
+    else k_conv_in_v<2><<<gi, ConvIn<2>::NW * 64, 0, st>>>(L, nn->cin, nn->in, slog, W0, b0, nn->act0);
+    launch_mid<0>(st, L, nn->taps, nn->act0, nn->wh1, nn->w + o[3], nullptr, nn->act1);
+    launch_mid<0>(st, L, nn->taps, nn->act1, nn->wh2, nn->w + o[5], nullptr, nn->act2);
+    const float* b3 = nn->w + o[7];
+    if (L.d == 3 && x_double) k_conv_out<3, 2><<<go, ConvOut<3>::NW * 64, 0, st>>>(L, nn->act2, nn->wh3, b3, nullptr, x, ip);
+    else if (L.d == 3) k_conv_out<3, 1><<<go, ConvOut<3>::NW * 64, 0, st>>>(L, nn->act2, nn->wh3, b3, nullptr, x, ip);
+    else if (x_double) k_conv_out<2, 2><<<go, ConvOut<2>::NW * 64, 0, st>>>(L, nn->act2, nn->wh3, b3, nullptr, x, ip);
+    else k_conv_out<2, 1><<<go, ConvOut<2>::NW * 64, 0, st>>>(L, nn->act2, nn->wh3, b3, nullptr, x, ip);
+    JFEM_HIP(hipGetLastError());
+}
+
 void nn_forward(jfem_nn* nn, const Lat& L, bool inb_ready)
 {
     long o[9];
@@ -1062,19 +1257,13 @@ void nn_forward(jfem_nn* nn, const Lat& L, bool inb_ready)
     if (nn_bf16_input(nn, L)) {
         // 2D / 3D lattices: every layer on MFMA, the first from the BF16 input
         if (!inb_ready) k_in_bf16<<<g, 256, 0, st>>>(L.np, nn->cin, nn->in, nn->inb);
-        const long tx = (L.n[0] + 15) / 16;
-        const long nt = L.d == 3 ? tx * ((L.n[1] + 7) / 8) * ((L.n[2] + 3) / 4) : tx * ((L.n[1] + 31) / 32);
-        // persistent: two workgroups per CU, a multiple of 8 for the XCD tile ranges
-        const int gi = (int)(nt >= 512 ? 512 : (nt >= 8 ? nt & ~7L : nt));
-        const long tox = (L.n[0] + 13) / 14;
-        const long nto = L.d == 3 ? tox * ((L.n[1] + 3) / 4) * ((L.n[2] + 3) / 4) : tox * ((L.n[1] + 15) / 16);
-        const int go = (int)(nto >= 512 ? 512 : (nto >= 8 ? nto & ~7L : nto));
+        const int gi = grid_in(L), go = grid_out(L);
         if (L.d == 3) k_conv_in<3><<<gi, ConvIn<3>::NW * 64, 0, st>>>(L, nn->cin, nn->inb, W0, b0, nn->act0);
         else k_conv_in<2><<<gi, ConvIn<2>::NW * 64, 0, st>>>(L, nn->cin, nn->inb, W0, b0, nn->act0);
         launch_mid<0>(st, L, nn->taps, nn->act0, nn->wh1, nn->w + o[3], nullptr, nn->act1);
         launch_mid<0>(st, L, nn->taps, nn->act1, nn->wh2, nn->w + o[5], nullptr, nn->act2);
-        if (L.d == 3) k_conv_out<3><<<go, ConvOut<3>::NW * 64, 0, st>>>(L, nn->act2, nn->wh3, nn->w + o[7], nn->out);
-        else k_conv_out<2><<<go, ConvOut<2>::NW * 64, 0, st>>>(L, nn->act2, nn->wh3, nn->w + o[7], nn->out);
+        if (L.d == 3) k_conv_out<3, 0><<<go, ConvOut<3>::NW * 64, 0, st>>>(L, nn->act2, nn->wh3, nn->w + o[7], nn->out, nullptr, nullptr);
+        else k_conv_out<2, 0><<<go, ConvOut<2>::NW * 64, 0, st>>>(L, nn->act2, nn->wh3, nn->w + o[7], nn->out, nullptr, nullptr);
         JFEM_HIP(hipGetLastError());
         return;
     }

@@ -51,7 +51,7 @@ struct jfem_nn {
     long cap = 0;
     unsigned short *act0 = nullptr, *act1 = nullptr, *act2 = nullptr;   // hidden activations (post-ReLU), NHWC bf16
     unsigned short *g0 = nullptr, *g1 = nullptr, *g2 = nullptr;         // gradients w.r.t. hidden pre-activations
-    float* in = nullptr;     // np x cin input (lattice order)
+    float* in = nullptr;     // np x cin input (lattice order; v of nn_forward_plain in the first np values)
     unsigned short* inb = nullptr;   // np x 8 BF16 copy of the input (zeros past cin)
     float* out = nullptr;    // np output
     float* gout = nullptr;   // dLoss/dout
@@ -66,6 +66,13 @@ void nn_reserve(jfem_nn* nn, long np);
 void nn_forward(jfem_nn* nn, const Lat& L, bool inb_ready = false);
 // true where the forward reads its input as the BF16 copy nn->inb
 bool nn_bf16_input(const jfem_nn* nn, const Lat& L);
+// the plain learned step x += N(v) on a lattice where nn_bf16_input holds, for a network with
+// 2 or 2 + d input channels: v (np FP32 values, lattice order) in the first np values of
+// nn->in, the input channels (v, v slog, v times the lattice parity along each axis) formed
+// while the first layer stages its halo (slog: lattice order, nullptr for zeros), and the
+// output added into x (FP32, or FP64 where x_double) at the DOF ip[p] of every pixel p with
+// ip[p] >= 0
+void nn_forward_plain(jfem_nn* nn, const Lat& L, const float* slog, const int* ip, void* x, bool x_double);
 void nn_backward(jfem_nn* nn, const Lat& L, int add);
 void nn_adam_step(jfem_nn* nn, float lr, float grad_scale);
 void nn_set_bias_free(jfem_nn* nn, int on);

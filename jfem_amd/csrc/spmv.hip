@@ -175,51 +175,96 @@ __global__ __launch_bounds__(64) void k_split_blocks(const int* __restrict__ row
     if (!out) seg_count[sg] = cnt;
 }
 
-static StBlk* split_blocks(jfem_mat* A, long cap, int vec, long* nblk)
-{
-    jfem_ctx* c = A->ctx;
-    const long n = A->rows;
-    const long nseg = (n + ST_SEG_ROWS - 1) / ST_SEG_ROWS;
-    StBlk* d = nullptr;
-    if (nseg == 0) {
-        JFEM_HIP(hipMalloc(&d, sizeof(StBlk)));
-        *nblk = 0;
-        return d;
-    }
-    int* cnt_d = nullptr;
-    long* off_d = nullptr;
-    JFEM_HIP(hipMalloc(&cnt_d, sizeof(int) * nseg));
-    JFEM_HIP(hipMalloc(&off_d, sizeof(long) * nseg));
-    const int nb = (int)nseg;
-    k_split_blocks<<<nb, 64, 0, c->stream>>>(A->rowptr, n, cap, vec, nseg, cnt_d, nullptr, nullptr);
-    JFEM_HIP(hipGetLastError());
-    std::vector<int> cnt(nseg);
-    std::vector<long> off(nseg);
-    JFEM_HIP(hipMemcpyAsync(cnt.data(), cnt_d, sizeof(int) * nseg, hipMemcpyDeviceToHost, c->stream));
-    JFEM_HIP(hipStreamSynchronize(c->stream));
-    long tot = 0;
-    for (long i = 0; i < nseg; i++) { off[i] = tot; tot += cnt[i]; }
-    JFEM_HIP(hipMemcpyAsync(off_d, off.data(), sizeof(long) * nseg, hipMemcpyHostToDevice, c->stream));
-    JFEM_HIP(hipMalloc(&d, sizeof(StBlk) * tot));
-    k_split_blocks<<<nb, 64, 0, c->stream>>>(A->rowptr, n, cap, vec, nseg, nullptr, off_d, d);
-    JFEM_HIP(hipGetLastError());
-    JFEM_HIP(hipStreamSynchronize(c->stream));
-    (void)hipFree(cnt_d);
-    (void)hipFree(off_d);
-    *nblk = tot;
-    return d;
-}
+// The CSR-stream row blocks of A for FP64 values (cap ST_CAP) and FP32 values (ST_CAP32) in two
+// phases: mat_blocks_count queues both count passes and the read-back of the per-segment
+// counts into pinned memory; after the caller's next wait on the stream mat_blocks_finish
+// turns them into offsets and queues both write passes.  Counts and offsets live in the
+// context's set-up scratch; matrices with more segments than the pinned area holds take a
+// pageable read-back in mat_blocks_finish instead.
+constexpr size_t PIN_SPLIT_LONGS = PIN_LONGS - 16;   // the last 16 pinned values are the callers'
 
-void mat_build_blocks(jfem_mat* A)
+static void blocks_reset(jfem_mat* A)
 {
     (void)hipFree(A->blk);
     (void)hipFree(A->blk32);
     (void)hipFree(A->part);
     A->blk = A->blk32 = nullptr;
     A->part = nullptr;
-    A->blk = split_blocks(A, ST_CAP, st_vec<double>(), &A->nrblk);
-    A->blk32 = split_blocks(A, ST_CAP32, st_vec<float>(), &A->nrblk32);
-    JFEM_HIP(hipMalloc(&A->part, sizeof(double) * (A->nrblk > 0 ? A->nrblk : 1)));
+    A->nrblk = A->nrblk32 = 0;
+}
+
+BlockSplit mat_blocks_count(jfem_mat* A)
+{
+    jfem_ctx* c = A->ctx;
+    BlockSplit bs;
+    bs.nseg = (A->rows + ST_SEG_ROWS - 1) / ST_SEG_ROWS;
+    if (bs.nseg == 0) return bs;
+    ScratchPlan plan;
+    bs.o_cnt = plan.add(sizeof(int) * 2 * bs.nseg);
+    bs.o_off = plan.add(sizeof(long) * 2 * bs.nseg);
+    unsigned char* ws = static_cast<unsigned char*>(ctx_scratch(c, SCRATCH_SPLIT, plan.total));
+    int* cnt_d = reinterpret_cast<int*>(ws + bs.o_cnt);
+    const long caps[2] = {ST_CAP, ST_CAP32};
+    const int vecs[2] = {st_vec<double>(), st_vec<float>()};
+    for (int v = 0; v < 2; v++)
+        k_split_blocks<<<(int)bs.nseg, 64, 0, c->stream>>>(A->rowptr, A->rows, caps[v], vecs[v], bs.nseg,
+                                                           cnt_d + v * bs.nseg, nullptr, nullptr);
+    JFEM_HIP(hipGetLastError());
+    // counts (2 nseg ints) and offsets (2 nseg longs) in pinned memory where they fit
+    bs.pinned = 3 * (size_t)bs.nseg <= PIN_SPLIT_LONGS;
+    if (bs.pinned) JFEM_HIP(hipMemcpyAsync(c->pin, cnt_d, sizeof(int) * 2 * bs.nseg, hipMemcpyDeviceToHost, c->stream));
+    return bs;
+}
+
+void mat_blocks_finish(jfem_mat* A, const BlockSplit& bs)
+{
+    jfem_ctx* c = A->ctx;
+    blocks_reset(A);
+    const long nseg = bs.nseg;
+    if (nseg == 0) {
+        JFEM_HIP(hipMalloc(&A->blk, sizeof(StBlk)));
+        JFEM_HIP(hipMalloc(&A->blk32, sizeof(StBlk)));
+        JFEM_HIP(hipMalloc(&A->part, sizeof(double)));
+        return;
+    }
+    unsigned char* ws = c->scratch[SCRATCH_SPLIT];
+    const int* cnt_d = reinterpret_cast<const int*>(ws + bs.o_cnt);
+    long* off_d = reinterpret_cast<long*>(ws + bs.o_off);
+    std::vector<int> cnt_v(bs.pinned ? 0 : 2 * nseg);
+    std::vector<long> off_v(bs.pinned ? 0 : 2 * nseg);
+    const int* cnt = bs.pinned ? reinterpret_cast<const int*>(c->pin) : cnt_v.data();
+    long* off = bs.pinned ? c->pin + nseg : off_v.data();
+    if (!bs.pinned) {
+        JFEM_HIP(hipMemcpyAsync(cnt_v.data(), cnt_d, sizeof(int) * 2 * nseg, hipMemcpyDeviceToHost, c->stream));
+        JFEM_HIP(hipStreamSynchronize(c->stream));
+    }
+    long tot[2] = {0, 0};
+    for (int v = 0; v < 2; v++)
+        for (long i = 0; i < nseg; i++) {
+            off[v * nseg + i] = tot[v];
+            tot[v] += cnt[v * nseg + i];
+        }
+    JFEM_HIP(hipMemcpyAsync(off_d, off, sizeof(long) * 2 * nseg, hipMemcpyHostToDevice, c->stream));
+    JFEM_HIP(hipMalloc(&A->blk, sizeof(StBlk) * (tot[0] > 0 ? tot[0] : 1)));
+    JFEM_HIP(hipMalloc(&A->blk32, sizeof(StBlk) * (tot[1] > 0 ? tot[1] : 1)));
+    JFEM_HIP(hipMalloc(&A->part, sizeof(double) * (tot[0] > 0 ? tot[0] : 1)));
+    k_split_blocks<<<(int)nseg, 64, 0, c->stream>>>(A->rowptr, A->rows, ST_CAP, st_vec<double>(), nseg, nullptr, off_d,
+                                                    A->blk);
+    k_split_blocks<<<(int)nseg, 64, 0, c->stream>>>(A->rowptr, A->rows, ST_CAP32, st_vec<float>(), nseg, nullptr,
+                                                    off_d + nseg, A->blk32);
+    JFEM_HIP(hipGetLastError());
+    A->nrblk = tot[0];
+    A->nrblk32 = tot[1];
+    // the pinned area is next written by the host only after a wait on the stream; a pageable
+    // source must outlive its copy
+    if (!bs.pinned) JFEM_HIP(hipStreamSynchronize(c->stream));
+}
+
+void mat_build_blocks(jfem_mat* A)
+{
+    const BlockSplit bs = mat_blocks_count(A);
+    if (bs.pinned) JFEM_HIP(hipStreamSynchronize(A->ctx->stream));
+    mat_blocks_finish(A, bs);
 }
 
 // ---- Dirichlet elimination, one thread group per row ----

@@ -153,9 +153,11 @@ def _learned_step_oracle(NO, params, cin, A, dinv, bnd, perm, shape, stat, x, b)
     return x
 
 
-@pytest.mark.parametrize("base,k,precision,sides", [([4, 5], 1, 0, 3), ([2, 2, 3], 2, 0, 3), ([2, 2, 3], 2, 1, 3),
-                                                   ([4, 5], 1, 0, 1), ([4, 5], 1, 0, 2)])
-def test_learned_vcycle_matches_oracle(jf, oracle, base, k, precision, sides):
+@pytest.mark.parametrize("base,k,precision,sides,with_logc", [([4, 5], 1, 0, 3, True), ([2, 2, 3], 2, 0, 3, True),
+                                                             ([2, 2, 3], 2, 1, 3, True), ([4, 5], 1, 0, 1, True),
+                                                             ([4, 5], 1, 0, 2, True), ([2, 2, 3], 2, 1, 3, False),
+                                                             ([3, 2], 2, 1, 3, False)])
+def test_learned_vcycle_matches_oracle(jf, oracle, base, k, precision, sides, with_logc):
     """A two-level V-cycle with the learned smoother on the finest level against
     its FP64 restatement: learned step, coarse-grid correction (the oracle's
     V-cycle with no smoothing steps), learned step.  The DOF -> lattice map comes
@@ -165,7 +167,8 @@ def test_learned_vcycle_matches_oracle(jf, oracle, base, k, precision, sides):
     contribute to the cycle, as in the network tests; in an FP64 or an FP32
     hierarchy.  With the network on one side only (sides 1:
     pre-smoothing, 2: post-smoothing) the other side is the oracle's
-    Chebyshev smoother on the device's eigenvalue interval."""
+    Chebyshev smoother on the device's eigenvalue interval.  Without a coefficient field
+    the log-coefficient channel is zero (the constant-coefficient Poisson workload)."""
     from oracle import nn_oracle as NO
     d = len(base)
     start, end = [0.0] * d, [1.0] * d
@@ -182,8 +185,8 @@ def test_learned_vcycle_matches_oracle(jf, oracle, base, k, precision, sides):
     net = jf.SmootherNet(d, cin, seed=3)
     ncell = int(np.prod(cells))
     rng = np.random.default_rng(17)
-    logc = rng.standard_normal(ncell)
-    G.set_learned(net, logc, runs=1, fallback_ratio=10.0)
+    logc = rng.standard_normal(ncell) if with_logc else np.zeros(ncell)
+    G.set_learned(net, logc if with_logc else None, runs=1, fallback_ratio=10.0)
     G.set_learned_sides(sides)
     params = net.get_params().astype(np.float64)   # biases zeroed by set_learned
     O = oracle.MG(base, k, 2, start, end, [0], [1.0], smoother=oracle.SM_RICHARDSON, runs=0, omega=0.1)
