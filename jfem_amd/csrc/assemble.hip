@@ -1024,7 +1024,14 @@ __global__ void k_absmin(long n, const double* __restrict__ e, unsigned long lon
         const unsigned long long t = __shfl_xor(m, o, 64);
         m = t < m ? t : m;
     }
-    if ((threadIdx.x & 63) == 0) atomicMin(out, m);
+    // one atomic per workgroup (atomics on one address serialise)
+    __shared__ unsigned long long wm[4];
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = wm[w] < m ? wm[w] : m;
+        atomicMin(out, m);
+    }
 }
 
 // row lengths when every structural candidate is present: prod_a W_a
@@ -1119,22 +1126,23 @@ size_t scan_work_bytes(long n)
     return sizeof(long) * ((n > 0 ? n : 1) + tiles + 2);
 }
 
-long scan_counts(jfem_ctx* c, const int* counts, long n, int* rowptr, long* work, long known_total)
+long scan_counts(jfem_ctx* c, const int* counts, long n, int* rowptr, long* work, long known_total, hipStream_t st)
 {
+    if (!st) st = c->stream;
     const long tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
     long* tmp = work;
     long* sums = work + (n > 0 ? n : 1);
-    if (n > 0) k_scan_tiles<<<tiles, SCAN_BS, 0, c->stream>>>(counts, n, tmp, sums);
-    k_scan_sums<<<1, SCAN_BS, 0, c->stream>>>(sums, tiles, sums + tiles + 1);
+    if (n > 0) k_scan_tiles<<<tiles, SCAN_BS, 0, st>>>(counts, n, tmp, sums);
+    k_scan_sums<<<1, SCAN_BS, 0, st>>>(sums, tiles, sums + tiles + 1);
     long total = known_total;
     if (known_total < 0) {
-        JFEM_HIP(hipMemcpyAsync(c->pin, sums + tiles + 1, sizeof(long), hipMemcpyDeviceToHost, c->stream));
-        JFEM_HIP(hipStreamSynchronize(c->stream));
+        JFEM_HIP(hipMemcpyAsync(c->pin, sums + tiles + 1, sizeof(long), hipMemcpyDeviceToHost, st));
+        JFEM_HIP(hipStreamSynchronize(st));
         total = c->pin[0];
     }
     JFEM_CHECK(total >= 0 && total < (1l << 31) - 1, JFEM_ERR_UNSUPPORTED,
                "matrix has 2^31 or more entries (int32 CSR indices)");
-    k_scan_finish<<<grid_for(n + 1, 256, 8192), 256, 0, c->stream>>>(tmp, sums, sums + tiles + 1, n, rowptr);
+    k_scan_finish<<<grid_for(n + 1, 256, 8192), 256, 0, st>>>(tmp, sums, sums + tiles + 1, n, rowptr);
     JFEM_HIP(hipGetLastError());
     return total;
 }
@@ -1658,7 +1666,7 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     const bool one_integral = nint == 1;
     if (one_integral) {
         JFEM_HIP(hipMemsetAsync(emin_d, 0xff, sizeof(unsigned long long), c->stream));
-        k_absmin<<<grid_for(Esize, 256, 2048), 256, 0, c->stream>>>(Esize, E, emin_d);
+        k_absmin<<<grid_for(Esize, 256, 256), 256, 0, c->stream>>>(Esize, E, emin_d);
         JFEM_HIP(hipGetLastError());
     }
     if (coeff_host)
@@ -1704,61 +1712,65 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     AxOffsets aoff;
     AxEntry* axtab_d = nullptr;
     int inner_ok = 0;
-    if (cellwise) {
-        cell_asm_axes(g, C.cls, C.ncls, axtab_h, aoff);
-        // lines (lattice points of the fixed axes, last fixed axis fastest) ordered by the
-        // classes of the cells containing them, axis by axis: each fixed axis' points sorted by
-        // their cells' classes, the lines their product in that order
-        {
-            std::vector<int> sorted[MAXD];
-            long nl = 1;
-            for (int a = 0; a + 1 < g.d; a++) {
-                const int Na = g.k * g.n[a] + 1;
-                nl *= Na;
-                std::vector<std::pair<long, int>> keyed((size_t)Na);
-                for (int pa = 0; pa < Na; pa++) {
-                    const AxEntry& e = axtab_h[(size_t)aoff.o[a] + pa];
-                    const long c0 = C.ncls[a] == 1 ? 0 : C.cls[a][e.lo];
-                    const long c1 = (e.info & 3) == 2 && C.ncls[a] > 1 ? C.cls[a][e.lo + 1] : c0;
-                    keyed[(size_t)pa] = {c0 * (C.ncls[a] + 1) + c1, pa};
+    // the cell-major kernel's tables (host work and uploads), made once cols and vals exist:
+    // the element tables and the pattern run meanwhile
+    auto prepare_cells = [&]() {
+        if (cellwise) {
+            cell_asm_axes(g, C.cls, C.ncls, axtab_h, aoff);
+            // lines (lattice points of the fixed axes, last fixed axis fastest) ordered by the
+            // classes of the cells containing them, axis by axis: each fixed axis' points sorted by
+            // their cells' classes, the lines their product in that order
+            {
+                std::vector<int> sorted[MAXD];
+                long nl = 1;
+                for (int a = 0; a + 1 < g.d; a++) {
+                    const int Na = g.k * g.n[a] + 1;
+                    nl *= Na;
+                    std::vector<std::pair<long, int>> keyed((size_t)Na);
+                    for (int pa = 0; pa < Na; pa++) {
+                        const AxEntry& e = axtab_h[(size_t)aoff.o[a] + pa];
+                        const long c0 = C.ncls[a] == 1 ? 0 : C.cls[a][e.lo];
+                        const long c1 = (e.info & 3) == 2 && C.ncls[a] > 1 ? C.cls[a][e.lo + 1] : c0;
+                        keyed[(size_t)pa] = {c0 * (C.ncls[a] + 1) + c1, pa};
+                    }
+                    std::stable_sort(keyed.begin(), keyed.end(),
+                                     [](const std::pair<long, int>& x, const std::pair<long, int>& y) { return x.first < y.first; });
+                    for (auto& kv : keyed) sorted[a].push_back(kv.second);
                 }
-                std::stable_sort(keyed.begin(), keyed.end(),
-                                 [](const std::pair<long, int>& x, const std::pair<long, int>& y) { return x.first < y.first; });
-                for (auto& kv : keyed) sorted[a].push_back(kv.second);
+                order_h.assign((size_t)nl, 0);
+                if (g.d == 3) {
+                    const int N1 = g.k * g.n[1] + 1;
+                    size_t i = 0;
+                    for (int p0 : sorted[0])
+                        for (int p1 : sorted[1]) order_h[i++] = p0 * N1 + p1;
+                } else if (g.d == 2) {
+                    order_h = sorted[0];
+                }
             }
-            order_h.assign((size_t)nl, 0);
-            if (g.d == 3) {
-                const int N1 = g.k * g.n[1] + 1;
-                size_t i = 0;
-                for (int p0 : sorted[0])
-                    for (int p1 : sorted[1]) order_h[i++] = p0 * N1 + p1;
-            } else if (g.d == 2) {
-                order_h = sorted[0];
-            }
+            cell_asm_tables(g.d, g.k, fwd_h, inv_h);
+            fwd_h.resize(fwd_h.size() + 64, 0);   // whole wavefronts read past the last combination's entries
+            inner_ok = cell_asm_delta(g, fwd_h, delta_h) ? 1 : 0;
+            ScratchPlan cp;
+            const size_t o_order = cp.add(sizeof(int) * order_h.size()), o_ax = cp.add(sizeof(AxEntry) * axtab_h.size());
+            const size_t o_fwd = cp.add(sizeof(unsigned short) * fwd_h.size()), o_inv = cp.add(inv_h.size());
+            const size_t o_delta = cp.add(sizeof(int) * (delta_h.size() > 0 ? delta_h.size() : 1));
+            unsigned char* cs = static_cast<unsigned char*>(ctx_scratch(c, SCRATCH_CELLS, cp.total));
+            order_d = reinterpret_cast<int*>(cs + o_order);
+            axtab_d = reinterpret_cast<AxEntry*>(cs + o_ax);
+            fwd_d = reinterpret_cast<unsigned short*>(cs + o_fwd);
+            inv_d = cs + o_inv;
+            delta_d = reinterpret_cast<int*>(cs + o_delta);
+            JFEM_HIP(hipMemcpyAsync(order_d, order_h.data(), sizeof(int) * order_h.size(), hipMemcpyHostToDevice,
+                                    c->stream));
+            JFEM_HIP(hipMemcpyAsync(axtab_d, axtab_h.data(), sizeof(AxEntry) * axtab_h.size(), hipMemcpyHostToDevice,
+                                    c->stream));
+            JFEM_HIP(hipMemcpyAsync(fwd_d, fwd_h.data(), sizeof(unsigned short) * fwd_h.size(), hipMemcpyHostToDevice,
+                                    c->stream));
+            JFEM_HIP(hipMemcpyAsync(inv_d, inv_h.data(), inv_h.size(), hipMemcpyHostToDevice, c->stream));
+            JFEM_HIP(hipMemcpyAsync(delta_d, delta_h.data(), sizeof(int) * delta_h.size(), hipMemcpyHostToDevice,
+                                    c->stream));
         }
-        cell_asm_tables(g.d, g.k, fwd_h, inv_h);
-        fwd_h.resize(fwd_h.size() + 64, 0);   // whole wavefronts read past the last combination's entries
-        inner_ok = cell_asm_delta(g, fwd_h, delta_h) ? 1 : 0;
-        ScratchPlan cp;
-        const size_t o_order = cp.add(sizeof(int) * order_h.size()), o_ax = cp.add(sizeof(AxEntry) * axtab_h.size());
-        const size_t o_fwd = cp.add(sizeof(unsigned short) * fwd_h.size()), o_inv = cp.add(inv_h.size());
-        const size_t o_delta = cp.add(sizeof(int) * (delta_h.size() > 0 ? delta_h.size() : 1));
-        unsigned char* cs = static_cast<unsigned char*>(ctx_scratch(c, SCRATCH_CELLS, cp.total));
-        order_d = reinterpret_cast<int*>(cs + o_order);
-        axtab_d = reinterpret_cast<AxEntry*>(cs + o_ax);
-        fwd_d = reinterpret_cast<unsigned short*>(cs + o_fwd);
-        inv_d = cs + o_inv;
-        delta_d = reinterpret_cast<int*>(cs + o_delta);
-        JFEM_HIP(hipMemcpyAsync(order_d, order_h.data(), sizeof(int) * order_h.size(), hipMemcpyHostToDevice,
-                                c->stream));
-        JFEM_HIP(hipMemcpyAsync(axtab_d, axtab_h.data(), sizeof(AxEntry) * axtab_h.size(), hipMemcpyHostToDevice,
-                                c->stream));
-        JFEM_HIP(hipMemcpyAsync(fwd_d, fwd_h.data(), sizeof(unsigned short) * fwd_h.size(), hipMemcpyHostToDevice,
-                                c->stream));
-        JFEM_HIP(hipMemcpyAsync(inv_d, inv_h.data(), inv_h.size(), hipMemcpyHostToDevice, c->stream));
-        JFEM_HIP(hipMemcpyAsync(delta_d, delta_h.data(), sizeof(int) * delta_h.size(), hipMemcpyHostToDevice,
-                                c->stream));
-    }
+    };
     const size_t cell_lds_bytes = sizeof(double) * (Esize + 64);   // E with the same padding
     const bool cell_lds = cell_lds_bytes <= 16 * 1024;
     auto run_cells = [&](auto kern, int GG) {
@@ -1815,12 +1827,15 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
     };
 #undef JFEM_ASM_CASE
 #undef JFEM_CELL_CASE
-    // With one integral, no exactly-zero element entry, and coefficients whose
-    // products with the entries cannot round to zero, every cell contribution
-    // is nonzero, so every structural candidate is in the pattern and the row
-    // lengths have a closed form; otherwise the count pass evaluates them.
-    // the entries of the structural pattern bound nnz (the pattern is a subset of it): cols and
-    // vals are allocated for it now, while the element tables are computed
+    // With one integral, no exactly-zero element entry, and coefficients whose products with the
+    // entries cannot round to zero, every cell contribution is nonzero, so every structural
+    // candidate is in the pattern and the row lengths have a closed form; otherwise the count
+    // pass evaluates them.  The element tables' smallest entry is known only on the device, so
+    // with one integral and nonzero coefficients the structural pattern is assumed: its row
+    // lengths, row pointer, SpMV blocks and the fill are queued at once, and the whole assembly
+    // waits once.  Should the tables hold an exact zero, the count pass and the fill run again.
+    // The entries of the structural pattern bound nnz (the pattern is a subset of it): cols and
+    // vals are allocated for it while the device works.
     long nnz_bound = 1;
     for (int a = 0; a < g.d; a++) {
         long sa = 0;
@@ -1834,57 +1849,78 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
         nnz_bound *= sa;
     }
     JFEM_CHECK(nnz_bound < (1l << 31) - 1, JFEM_ERR_UNSUPPORTED, "matrix has 2^31 or more entries (int32 CSR indices)");
+    double cmin = 1.0;
+    if (coeff_host)
+        for (long i = 0; i < g.ncells(); i++) cmin = std::min(cmin, std::fabs(coeff_host[i]));
+    const long tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    long* tail = c->pin + PIN_LONGS - 2;   // the fill's status and the scan's total
+    // row lengths (closed form or count pass), row pointer, SpMV blocks; the fill is launched by
+    // the caller once cols and vals exist
+    // (the closed form reads neither the element tables nor anything else queued on the
+    // context's stream, so it runs on the auxiliary stream beside the element tables; the
+    // context's stream waits for it before the fill)
+    auto pattern = [&](bool structural) {
+        hipStream_t ps = structural ? c->aux : c->stream;
+        if (structural) {
+            const int cb = grid_for(n, 256, 65536);
+            switch (g.d * 10 + g.k) {
+            case 11: k_count_structural<1, 1><<<cb, 256, 0, ps>>>(RP.g, n, counts); break;
+            case 12: k_count_structural<1, 2><<<cb, 256, 0, ps>>>(RP.g, n, counts); break;
+            case 21: k_count_structural<2, 1><<<cb, 256, 0, ps>>>(RP.g, n, counts); break;
+            case 22: k_count_structural<2, 2><<<cb, 256, 0, ps>>>(RP.g, n, counts); break;
+            case 31: k_count_structural<3, 1><<<cb, 256, 0, ps>>>(RP.g, n, counts); break;
+            case 32: k_count_structural<3, 2><<<cb, 256, 0, ps>>>(RP.g, n, counts); break;
+            default: k_count_structural<0, 0><<<cb, 256, 0, ps>>>(RP.g, n, counts); break;
+            }
+            JFEM_HIP(hipGetLastError());
+        } else {
+            launch(false);
+        }
+        // with the structural pattern the entry count is nnz_bound and nothing is read back; the
+        // scan's total comes back with the fill's status and is checked then
+        A->nnz = scan_counts(c, counts, n, A->rowptr, scan_ws, structural ? nnz_bound : -1, ps);
+        JFEM_CHECK(A->nnz <= nnz_bound, JFEM_ERR_STATE, "more entries than the structural pattern");
+        mat_blocks_queue(A, ps);
+        if (structural) {
+            JFEM_HIP(hipEventRecord(c->aux_done, c->aux));
+            JFEM_HIP(hipStreamWaitEvent(c->stream, c->aux_done, 0));
+        }
+    };
+    auto fill_and_status = [&]() {
+        launch(true);
+        JFEM_HIP(hipMemcpyAsync(tail, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        JFEM_HIP(hipMemcpyAsync(tail + 1, scan_ws + (n > 0 ? n : 1) + tiles + 1, sizeof(long), hipMemcpyDeviceToHost,
+                                c->stream));
+    };
+    const bool assume = one_integral && cmin > 0;
+    if (assume) pattern(true);
     JFEM_HIP(hipMalloc(&A->colidx, sizeof(int) * (nnz_bound + NNZ_PAD)));
     JFEM_HIP(hipMalloc(&A->vals, sizeof(double) * (nnz_bound + NNZ_PAD)));
+    prepare_cells();
+    if (assume) fill_and_status();
     if (one_integral) JFEM_HIP(hipMemcpyAsync(c->pin, emin_d, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
     JFEM_HIP(hipStreamSynchronize(c->stream));
     bool structural = one_integral;
     if (structural) {
         double emin;
         std::memcpy(&emin, c->pin, sizeof(emin));
-        double cmin = 1.0;
-        if (coeff_host)
-            for (long i = 0; i < g.ncells(); i++) cmin = std::min(cmin, std::fabs(coeff_host[i]));
         structural = emin > 0 && cmin > 0 && emin * cmin > 1e-300;
     }
-    if (structural) {
-        const int cb = grid_for(n, 256, 65536);
-        switch (g.d * 10 + g.k) {
-        case 11: k_count_structural<1, 1><<<cb, 256, 0, c->stream>>>(RP.g, n, counts); break;
-        case 12: k_count_structural<1, 2><<<cb, 256, 0, c->stream>>>(RP.g, n, counts); break;
-        case 21: k_count_structural<2, 1><<<cb, 256, 0, c->stream>>>(RP.g, n, counts); break;
-        case 22: k_count_structural<2, 2><<<cb, 256, 0, c->stream>>>(RP.g, n, counts); break;
-        case 31: k_count_structural<3, 1><<<cb, 256, 0, c->stream>>>(RP.g, n, counts); break;
-        case 32: k_count_structural<3, 2><<<cb, 256, 0, c->stream>>>(RP.g, n, counts); break;
-        default: k_count_structural<0, 0><<<cb, 256, 0, c->stream>>>(RP.g, n, counts); break;
-        }
-        JFEM_HIP(hipGetLastError());
-    } else {
-        launch(false);
+    if (!structural) {
+        // the row lengths need the count pass (and the fill its row pointer)
+        JFEM_HIP(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
+        pattern(false);
+        fill_and_status();
+        JFEM_HIP(hipStreamSynchronize(c->stream));
     }
-    // with the structural pattern the entry count is nnz_bound: nothing is read back before the
-    // fill; the scan's total comes back with the fill's status and is checked then
-    const long tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-    const long nnz = scan_counts(c, counts, n, A->rowptr, scan_ws, structural ? nnz_bound : -1);
-    A->nnz = nnz;
-    JFEM_CHECK(nnz <= nnz_bound, JFEM_ERR_STATE, "more entries than the structural pattern");
-    // the row blocks of the SpMV need only the row pointer: their count passes go before the
-    // fill, their counts come back with its status
-    const BlockSplit bs = mat_blocks_count(A);
-    launch(true);
-    long* tail = c->pin + PIN_LONGS - 2;   // clear of the block counts
-    JFEM_HIP(hipMemcpyAsync(tail, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    JFEM_HIP(hipMemcpyAsync(tail + 1, scan_ws + (n > 0 ? n : 1) + tiles + 1, sizeof(long), hipMemcpyDeviceToHost,
-                            c->stream));
-    JFEM_HIP(hipStreamSynchronize(c->stream));
     const int bad_h = *reinterpret_cast<const int*>(tail);
     const long total_h = tail[1];
-    if (bad_h || total_h != nnz) {
+    if (bad_h || total_h != A->nnz) {
         mat_free(A);
         JFEM_CHECK(false, JFEM_ERR_STATE, "assembled rows differ from their reserved lengths");
     }
-    A->lanes = choose_lanes(n > 0 ? (double)nnz / n : 1.0);
-    mat_blocks_finish(A, bs);
+    A->lanes = choose_lanes(n > 0 ? (double)A->nnz / n : 1.0);
+    mat_blocks_done(A);
     return A;
 }
 

@@ -160,6 +160,10 @@ struct jfem_ctx {
     unsigned char* scratch[3] = {nullptr, nullptr, nullptr};
     size_t scratch_n[3] = {0, 0, 0};
     long* pin = nullptr;              // PIN_LONGS values (pinned host memory)
+    // a second stream (default priority) for set-up work that is independent of what the
+    // context's stream is doing, and the event by which the context's stream waits for it
+    hipStream_t aux = nullptr;
+    hipEvent_t aux_done = nullptr;
 };
 
 struct jfem_space {
@@ -195,6 +199,7 @@ struct jfem_mat {
     long nrblk = 0;
     StBlk* blk32 = nullptr;  // the same for FP32 values (twice the entries per block)
     long nrblk32 = 0;
+    long blk_cap = 0, blk32_cap = 0;   // entries allocated for blk / blk32 (and part)
     double* part = nullptr;  // device: one partial sum per stream block
     jfem_mat* T = nullptr;   // cached transpose for tvMul (dropped when values change)
 };
@@ -284,7 +289,8 @@ long scan_counts(jfem_ctx* c, const int* counts, long n, int* rowptr);
 // the same with a caller's workspace of scan_work_bytes(n) bytes; with known_total >= 0 (the
 // caller knows the sum) nothing is read back and nothing waits
 size_t scan_work_bytes(long n);
-long scan_counts(jfem_ctx* c, const int* counts, long n, int* rowptr, long* work, long known_total);
+long scan_counts(jfem_ctx* c, const int* counts, long n, int* rowptr, long* work, long known_total,
+                 hipStream_t st = nullptr);
 // lexicographic node rank (last axis fastest) of every DOF of a scalar space, into a device array
 void space_lex_rank(jfem_space* s, int* out_dev);
 // finest grid of a multigrid hierarchy
@@ -297,16 +303,11 @@ hipStream_t mg_stream(const jfem_mg* mg);
 void reduce_partials_dev(jfem_ctx* c, int slots, int blocks, double* dst_dev);
 void mat_make_f32(jfem_mat* A);
 void mat_build_blocks(jfem_mat* A);
-// the same in two phases around a wait on the context's stream that the caller makes anyway:
-// mat_blocks_count queues the count passes and their read-back (into ctx->pin[0, 3 nseg),
-// clear of its last 16 values), mat_blocks_finish completes the split after that wait
-struct BlockSplit {
-    long nseg = 0;
-    bool pinned = false;
-    size_t o_cnt = 0, o_off = 0;
-};
-BlockSplit mat_blocks_count(jfem_mat* A);
-void mat_blocks_finish(jfem_mat* A, const BlockSplit& bs);
+// the same without waiting: mat_blocks_queue queues the whole split (block arrays sized by a
+// bound, the totals read back into ctx->pin[PIN_LONGS - 4, PIN_LONGS - 2)); mat_blocks_done
+// takes the totals after the caller's next wait on the context's stream
+void mat_blocks_queue(jfem_mat* A, hipStream_t st = nullptr);
+void mat_blocks_done(jfem_mat* A);
 // sum of n partials (device) into dst (device), deterministic
 void reduce_array_dev(jfem_ctx* c, const double* partials, long n, double* dst);
 void mat_diag(jfem_mat* A, double* out);

@@ -236,8 +236,9 @@ void* ctx_scratch(jfem_ctx* c, int slot, size_t bytes)
 {
     JFEM_CHECK(slot >= 0 && slot < 3, JFEM_ERR_ARG, "scratch slot out of range");
     if (bytes > c->scratch_n[slot]) {
-        // the old buffer may still be read by work queued on the stream
+        // the old buffer may still be read by work queued on either stream
         JFEM_HIP(hipStreamSynchronize(c->stream));
+        JFEM_HIP(hipStreamSynchronize(c->aux));
         (void)hipFree(c->scratch[slot]);
         c->scratch[slot] = nullptr;
         c->scratch_n[slot] = 0;
@@ -264,6 +265,8 @@ int jfem_ctx_create(int device, jfem_ctx** out)
     auto* c = new jfem_ctx();
     c->device = device;
     JFEM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    JFEM_HIP(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+    JFEM_HIP(hipEventCreateWithFlags(&c->aux_done, hipEventDisableTiming));
     JFEM_HIP(hipMalloc(&c->red_partials, sizeof(double) * RED_BLOCKS * RED_SLOTS));
     JFEM_HIP(hipMalloc(&c->red_result, sizeof(double) * RED_SLOTS));
     JFEM_HIP(hipMalloc(&c->scal, sizeof(double) * 64));
@@ -283,6 +286,7 @@ int jfem_ctx_destroy(jfem_ctx* c)
     if (!c) return JFEM_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->aux);
     (void)hipFree(c->red_partials);
     (void)hipFree(c->red_result);
     (void)hipFree(c->scal);
@@ -290,6 +294,8 @@ int jfem_ctx_destroy(jfem_ctx* c)
     for (unsigned char* p : c->scratch) (void)hipFree(p);
     (void)hipHostFree(c->red_host);
     (void)hipHostFree(c->pin);
+    (void)hipEventDestroy(c->aux_done);
+    (void)hipStreamDestroy(c->aux);
     (void)hipStreamDestroy(c->stream);
     delete c;
     API_END

@@ -139,7 +139,7 @@ constexpr long ST_SEG_ROWS = 4096;
 
 __global__ __launch_bounds__(64) void k_split_blocks(const int* __restrict__ rowptr, long n, long cap, int vec, long nseg,
                                                     int* __restrict__ seg_count, const long* __restrict__ seg_off,
-                                                    StBlk* __restrict__ out)
+                                                    StBlk* __restrict__ out, long maxblk)
 {
     // one wavefront per segment: its row pointers are staged in LDS, then lane 0
     // walks the segment; the end of each block is found by binary search (the
@@ -168,103 +168,117 @@ __global__ __launch_bounds__(64) void k_split_blocks(const int* __restrict__ row
             else hi_e = mid;
         }
         const int e = lo_e;
-        if (out) out[pos++] = StBlk{(int)(r0 + r), (int)(r0 + e), rp[r], rp[e]};
+        if (out && pos < maxblk) out[pos] = StBlk{(int)(r0 + r), (int)(r0 + e), rp[r], rp[e]};
+        pos++;
         cnt++;
         r = e;
     }
     if (!out) seg_count[sg] = cnt;
 }
 
-// The CSR-stream row blocks of A for FP64 values (cap ST_CAP) and FP32 values (ST_CAP32) in two
-// phases: mat_blocks_count queues both count passes and the read-back of the per-segment
-// counts into pinned memory; after the caller's next wait on the stream mat_blocks_finish
-// turns them into offsets and queues both write passes.  Counts and offsets live in the
-// context's set-up scratch; matrices with more segments than the pinned area holds take a
-// pageable read-back in mat_blocks_finish instead.
-constexpr size_t PIN_SPLIT_LONGS = PIN_LONGS - 16;   // the last 16 pinned values are the callers'
-
-static void blocks_reset(jfem_mat* A)
+// exclusive prefix sums of the per-segment block counts of both splits (cnt[v nseg + i]) into
+// off, and the two totals into tot; one workgroup
+__global__ __launch_bounds__(256) void k_split_offsets(const int* __restrict__ cnt, long nseg, long* __restrict__ off,
+                                                       long* __restrict__ tot)
 {
+    __shared__ long part[256];
+    for (int v = 0; v < 2; v++) {
+        const long per = (nseg + 255) / 256, b = threadIdx.x * per, e = b + per < nseg ? b + per : nseg;
+        long s = 0;
+        for (long i = b; i < e; i++) s += cnt[v * nseg + i];
+        part[threadIdx.x] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            long run = 0;
+            for (int t = 0; t < 256; t++) {
+                const long x = part[t];
+                part[t] = run;
+                run += x;
+            }
+            tot[v] = run;
+        }
+        __syncthreads();
+        long run = part[threadIdx.x];
+        for (long i = b; i < e; i++) {
+            off[v * nseg + i] = run;
+            run += cnt[v * nseg + i];
+        }
+        __syncthreads();
+    }
+}
+
+// An upper bound on the blocks of a split: a block that ends because the next row does not
+// fit holds, with the first row of the next block, more than cap - vec + 1 entries, so such
+// blocks number fewer than 2 nnz / (cap - vec + 1); the others end at ST_BS rows or at a
+// segment's end.
+static long blocks_bound(long n, long nnz, long cap, int vec, long nseg)
+{
+    return 2 * nnz / (cap - vec + 1) + n / ST_BS + 2 * nseg + 2;
+}
+
+// The CSR-stream row blocks of A for FP64 values (cap ST_CAP) and FP32 values (ST_CAP32),
+// queued without waiting: count passes, offsets on the device, write passes into block arrays
+// of the bound's size; mat_blocks_done takes the totals after the caller's next wait on the
+// stream.  Counts and offsets live in the context's set-up scratch, the totals come back into
+// the pinned area's tail.
+void mat_blocks_queue(jfem_mat* A, hipStream_t st)
+{
+    jfem_ctx* c = A->ctx;
+    if (!st) st = c->stream;
     (void)hipFree(A->blk);
     (void)hipFree(A->blk32);
     (void)hipFree(A->part);
     A->blk = A->blk32 = nullptr;
     A->part = nullptr;
     A->nrblk = A->nrblk32 = 0;
-}
-
-BlockSplit mat_blocks_count(jfem_mat* A)
-{
-    jfem_ctx* c = A->ctx;
-    BlockSplit bs;
-    bs.nseg = (A->rows + ST_SEG_ROWS - 1) / ST_SEG_ROWS;
-    if (bs.nseg == 0) return bs;
-    ScratchPlan plan;
-    bs.o_cnt = plan.add(sizeof(int) * 2 * bs.nseg);
-    bs.o_off = plan.add(sizeof(long) * 2 * bs.nseg);
-    unsigned char* ws = static_cast<unsigned char*>(ctx_scratch(c, SCRATCH_SPLIT, plan.total));
-    int* cnt_d = reinterpret_cast<int*>(ws + bs.o_cnt);
+    const long n = A->rows;
+    const long nseg = (n + ST_SEG_ROWS - 1) / ST_SEG_ROWS;
     const long caps[2] = {ST_CAP, ST_CAP32};
     const int vecs[2] = {st_vec<double>(), st_vec<float>()};
-    for (int v = 0; v < 2; v++)
-        k_split_blocks<<<(int)bs.nseg, 64, 0, c->stream>>>(A->rowptr, A->rows, caps[v], vecs[v], bs.nseg,
-                                                           cnt_d + v * bs.nseg, nullptr, nullptr);
-    JFEM_HIP(hipGetLastError());
-    // counts (2 nseg ints) and offsets (2 nseg longs) in pinned memory where they fit
-    bs.pinned = 3 * (size_t)bs.nseg <= PIN_SPLIT_LONGS;
-    if (bs.pinned) JFEM_HIP(hipMemcpyAsync(c->pin, cnt_d, sizeof(int) * 2 * bs.nseg, hipMemcpyDeviceToHost, c->stream));
-    return bs;
-}
-
-void mat_blocks_finish(jfem_mat* A, const BlockSplit& bs)
-{
-    jfem_ctx* c = A->ctx;
-    blocks_reset(A);
-    const long nseg = bs.nseg;
+    long bound[2];
+    for (int v = 0; v < 2; v++) bound[v] = blocks_bound(n, A->nnz, caps[v], vecs[v], nseg);
+    A->blk_cap = bound[0];
+    A->blk32_cap = bound[1];
+    JFEM_HIP(hipMalloc(&A->blk, sizeof(StBlk) * bound[0]));
+    JFEM_HIP(hipMalloc(&A->blk32, sizeof(StBlk) * bound[1]));
+    JFEM_HIP(hipMalloc(&A->part, sizeof(double) * bound[0]));
+    long* tot = c->pin + PIN_LONGS - 4;   // the pinned tail's block totals
     if (nseg == 0) {
-        JFEM_HIP(hipMalloc(&A->blk, sizeof(StBlk)));
-        JFEM_HIP(hipMalloc(&A->blk32, sizeof(StBlk)));
-        JFEM_HIP(hipMalloc(&A->part, sizeof(double)));
+        tot[0] = tot[1] = 0;
         return;
     }
-    unsigned char* ws = c->scratch[SCRATCH_SPLIT];
-    const int* cnt_d = reinterpret_cast<const int*>(ws + bs.o_cnt);
-    long* off_d = reinterpret_cast<long*>(ws + bs.o_off);
-    std::vector<int> cnt_v(bs.pinned ? 0 : 2 * nseg);
-    std::vector<long> off_v(bs.pinned ? 0 : 2 * nseg);
-    const int* cnt = bs.pinned ? reinterpret_cast<const int*>(c->pin) : cnt_v.data();
-    long* off = bs.pinned ? c->pin + nseg : off_v.data();
-    if (!bs.pinned) {
-        JFEM_HIP(hipMemcpyAsync(cnt_v.data(), cnt_d, sizeof(int) * 2 * nseg, hipMemcpyDeviceToHost, c->stream));
-        JFEM_HIP(hipStreamSynchronize(c->stream));
-    }
-    long tot[2] = {0, 0};
+    ScratchPlan plan;
+    const size_t o_cnt = plan.add(sizeof(int) * 2 * nseg), o_off = plan.add(sizeof(long) * 2 * nseg);
+    const size_t o_tot = plan.add(sizeof(long) * 2);
+    unsigned char* ws = static_cast<unsigned char*>(ctx_scratch(c, SCRATCH_SPLIT, plan.total));
+    int* cnt_d = reinterpret_cast<int*>(ws + o_cnt);
+    long* off_d = reinterpret_cast<long*>(ws + o_off);
+    long* tot_d = reinterpret_cast<long*>(ws + o_tot);
     for (int v = 0; v < 2; v++)
-        for (long i = 0; i < nseg; i++) {
-            off[v * nseg + i] = tot[v];
-            tot[v] += cnt[v * nseg + i];
-        }
-    JFEM_HIP(hipMemcpyAsync(off_d, off, sizeof(long) * 2 * nseg, hipMemcpyHostToDevice, c->stream));
-    JFEM_HIP(hipMalloc(&A->blk, sizeof(StBlk) * (tot[0] > 0 ? tot[0] : 1)));
-    JFEM_HIP(hipMalloc(&A->blk32, sizeof(StBlk) * (tot[1] > 0 ? tot[1] : 1)));
-    JFEM_HIP(hipMalloc(&A->part, sizeof(double) * (tot[0] > 0 ? tot[0] : 1)));
-    k_split_blocks<<<(int)nseg, 64, 0, c->stream>>>(A->rowptr, A->rows, ST_CAP, st_vec<double>(), nseg, nullptr, off_d,
-                                                    A->blk);
-    k_split_blocks<<<(int)nseg, 64, 0, c->stream>>>(A->rowptr, A->rows, ST_CAP32, st_vec<float>(), nseg, nullptr,
-                                                    off_d + nseg, A->blk32);
+        k_split_blocks<<<(int)nseg, 64, 0, st>>>(A->rowptr, n, caps[v], vecs[v], nseg, cnt_d + v * nseg, nullptr,
+                                                        nullptr, 0);
+    k_split_offsets<<<1, 256, 0, st>>>(cnt_d, nseg, off_d, tot_d);
+    k_split_blocks<<<(int)nseg, 64, 0, st>>>(A->rowptr, n, caps[0], vecs[0], nseg, nullptr, off_d, A->blk,
+                                                    bound[0]);
+    k_split_blocks<<<(int)nseg, 64, 0, st>>>(A->rowptr, n, caps[1], vecs[1], nseg, nullptr, off_d + nseg,
+                                                    A->blk32, bound[1]);
     JFEM_HIP(hipGetLastError());
+    JFEM_HIP(hipMemcpyAsync(tot, tot_d, sizeof(long) * 2, hipMemcpyDeviceToHost, st));
+}
+
+void mat_blocks_done(jfem_mat* A)
+{
+    const long* tot = A->ctx->pin + PIN_LONGS - 4;
+    JFEM_CHECK(tot[0] <= A->blk_cap && tot[1] <= A->blk32_cap, JFEM_ERR_STATE, "CSR-stream blocks exceed their bound");
     A->nrblk = tot[0];
     A->nrblk32 = tot[1];
-    // the pinned area is next written by the host only after a wait on the stream; a pageable
-    // source must outlive its copy
-    if (!bs.pinned) JFEM_HIP(hipStreamSynchronize(c->stream));
 }
 
 void mat_build_blocks(jfem_mat* A)
 {
-    const BlockSplit bs = mat_blocks_count(A);
-    if (bs.pinned) JFEM_HIP(hipStreamSynchronize(A->ctx->stream));
-    mat_blocks_finish(A, bs);
+    mat_blocks_queue(A);
+    JFEM_HIP(hipStreamSynchronize(A->ctx->stream));
+    mat_blocks_done(A);
 }
 
 // ---- Dirichlet elimination, one thread group per row ----
