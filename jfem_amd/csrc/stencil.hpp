@@ -161,8 +161,12 @@ __device__ inline void mf_cell_part(const MfParams& P, const T* xs, int s0, int 
 
 // Epi: the row epilogues of stream.hpp (Pre load(row), operator()(row, ax, pre)); coef: one
 // value per cell of the level (reference cell order, last axis fastest), nullptr = 1
+// waves per SIMD the kernel is built for: six in FP64 (four workgroups per CU), seven in FP32
+template <typename T, typename Epi>
+constexpr int mf_waves() { return sizeof(T) == 8 ? 6 : 7; }
+
 template <typename T, int D, int K, typename Epi>
-__global__ __launch_bounds__((MfShape<D, K>::BS), (sizeof(T) == 8 ? 5 : 7)) void k_mf(MfParams P, const T* __restrict__ x,
+__global__ __launch_bounds__((MfShape<D, K>::BS), (mf_waves<T, Epi>())) void k_mf(MfParams P, const T* __restrict__ x,
                                                           const T* __restrict__ coef, Epi epi)
 {
     static_assert(!Epi::reduce, "matrix-free epilogues do not reduce");
@@ -240,7 +244,10 @@ __global__ __launch_bounds__((MfShape<D, K>::BS), (sizeof(T) == 8 ? 5 : 7)) void
         const int pp[3] = {org[0] + q0, org[1] + q1, org[2] + q2};
         return mf_dof<D, K>(n, pp);
     };
-    // 0. the owned rows of this thread and their epilogue operands, issued first
+    // 0. the owned rows of this thread and their epilogue operands, issued first; operands of more
+    // than 16 bytes per row (the FP64 Chebyshev step's four) are loaded in phase 3, where they
+    // are used: held from here they would not fit the registers of six waves per SIMD
+    constexpr bool PRE_EARLY = sizeof(typename Epi::Pre) <= 16;
     int row[NOWN];
     bool own[NOWN];
     typename Epi::Pre pre[NOWN];
@@ -256,7 +263,7 @@ __global__ __launch_bounds__((MfShape<D, K>::BS), (sizeof(T) == 8 ? 5 : 7)) void
         row[u] = 0;
         if (own[u]) {
             row[u] = dof_of(olo[0] + o0 - org[0], olo[1] + o1 - org[1], olo[2] + o2 - org[2]);
-            pre[u] = epi.load(row[u]);
+            if (PRE_EARLY) pre[u] = epi.load(row[u]);
             if (inner && CAND_EARLY)
                 candw[u] = *reinterpret_cast<const uint4*>(
                     P.cand + 8 * (((o0 + 1) * NB1 + o1 + 1) * NB2 + (D == 3 ? o2 + 1 : 0)));
@@ -321,6 +328,7 @@ __global__ __launch_bounds__((MfShape<D, K>::BS), (sizeof(T) == 8 ? 5 : 7)) void
 #pragma unroll
     for (int u = 0; u < NOWN; u++) {
         if (!own[u]) continue;
+        if (!PRE_EARLY) pre[u] = epi.load(row[u]);
         const int i = tid + u * BS;
         const int o2 = i % OB2, o1 = (i / OB2) % OB1, o0 = i / (OB1 * OB2);
         if (inner) {
