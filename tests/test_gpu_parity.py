@@ -605,6 +605,30 @@ def test_pbicgstab_breakdown_falls_back_to_pgmres(jf, oracle):
     assert np.linalg.norm(x - xr) <= 1e-8 * np.linalg.norm(xr)
 
 
+@pytest.mark.parametrize("cells", [[5, 4], [4, 3, 5]])
+def test_assembly_exact_zero_entries(jf, oracle, cells):
+    """Q1 GRAD_GRAD with the two-point Gauss-Lobatto rule: the element matrix couples the
+    diagonally opposite corners of a cell with an exact zero, which the reference leaves out of
+    the pattern (5- / 7-point stencils).  The device first assumes the structural pattern (every
+    candidate present), finds the zero in the element tables, and counts and fills again:
+    bit-exact with the oracle, twice in a row on one space"""
+    import scipy.sparse as sp
+    s, start, end = _space(jf, cells, 1)
+    TP = jf.TPCellIntegral
+    ref = oracle.assemble(cells, 1, start, end, [0], [1.0], rule=12)
+    n = int(np.prod([c + 1 for c in cells]))
+    assert ref.rowptr[-1] < n * 3 ** len(cells)   # entries were left out
+    for _ in range(2):
+        A = s.assemble_matrix([TP(1.0, TP.GRAD_GRAD, 12)])
+        rp, cols, vals = A.to_csr()
+        assert np.array_equal(rp, ref.rowptr) and np.array_equal(cols, ref.cols)
+        assert np.array_equal(vals, ref.vals)
+        x = np.random.default_rng(3).standard_normal(n)
+        y = A.mvMul(jf.DenseVector(x)).numpy()
+        S = sp.csr_matrix((ref.vals, ref.cols, ref.rowptr), shape=(n, n))
+        assert np.allclose(y, S @ x, rtol=1e-14, atol=1e-14 * np.abs(S @ x).max())
+
+
 # ------------------------------------------------- quadrature on cell faces
 def test_face_quadrature_refused(jf, oracle):
     """GRAD_GRAD integrates gradientInCell(x, cell) and accepts Gauss-Lobatto rules (bit-exact
