@@ -749,14 +749,14 @@ def main():
             t0 = time.perf_counter()
             _, Gt, _ = build_problem(jf, base, 4, 2, coarse_sm, jf.FP64, 2, 16)
             net = jf.SmootherNet(3, 5, seed=7)
-            Gt.set_learned(net, np.zeros((base[0] << 3) ** 3), runs=1, fallback_ratio=0.5)
+            # constant coefficient: no log-coefficient channel (it would be zero everywhere)
+            Gt.set_learned(net, None, runs=1, fallback_ratio=0.5)
             h = Gt.train(rhs_per_field=400, cycles_per_rhs=8, lr=1e-4, seed=1, mode=5)
             ctx.sync()
             t_train = time.perf_counter() - t0
             del Gt
             log(f"learned smoother trained on 24^3 in {t_train:.1f}s, two-grid energy ratio "
                 f"{np.mean(h[:8]):.3f} -> {np.mean(h[-8:]):.4f}")
-            ncell = int(np.prod(fine.cells.astype(np.int64)))
             tl, fb = [], False
 
             # the learned hierarchy: the network smooths the finest level; the coarser levels keep
@@ -765,7 +765,7 @@ def main():
             del G32
             _, G32, _ = build_problem(jf, base, nlev, 2, coarse_sm, jf.FP32, 2, 16)
             A32 = G32.getFinestSystem()
-            G32.set_learned(net, np.zeros(ncell), runs=1, fallback_ratio=0.5)
+            G32.set_learned(net, None, runs=1, fallback_ratio=0.5)
             # the network on one side of the V-cycle (1 = pre-smoothing, 2 = post-smoothing) or both
             # (3); the other side keeps the hierarchy's Chebyshev smoother
             for sides in (1, 2):
@@ -829,7 +829,7 @@ def main():
             del G1
             # stationary V-cycle iteration with each smoother (FP64 hierarchy, same network)
             G64 = build_problem(jf, base, nlev, 2, smoother, jf.FP64, 2, 16)[1]
-            G64.set_learned(net, np.zeros(ncell), runs=1, fallback_ratio=0.5)
+            G64.set_learned(net, None, runs=1, fallback_ratio=0.5)
             for name, active in (("chebyshev", False), ("learned", True)):
                 G64.set_learned_active(active)
                 G64.solve(b, jf.DenseVector(n), tol, 100)
