@@ -269,8 +269,8 @@ def bench_c4(jf, args):
         log(f"train field {f}: two-grid energy ratio {np.mean(h[:8]):.4f} -> {np.mean(h[-8:]):.4f}")
         del G
     t_train = time.perf_counter() - t0
-    res = {"pcg64": [], "fcg32_learned": [], "fcg32_cheb": [], "vcyc_cheb": [], "vcyc_learned": [], "err": [],
-           "fallbacks": 0}
+    res = {"pcg64": [], "fcg32_learned": [], "fcg32_learned_both": [], "fcg32_cheb": [], "vcyc_cheb": [],
+           "vcyc_learned": [], "err": [], "fallbacks": 0}
     kw, amp = sinusoid_terms(4, 8, 2)
     for f in range(args.eval_fields):
         field = gaussian_random_field(5000 + f, (cells, cells), 0.1, 1e3)
@@ -295,10 +295,19 @@ def bench_c4(jf, args):
         del G64
         G32, _ = c4_hierarchy(jf, field, jf.FP32, galerkin=args.galerkin)
         G32.set_learned(net, field, runs=1, fallback_ratio=0.5)
+        G32.set_learned_sides(args.learned_sides)
         it2 = jf.IterativeSolver()
         x32 = it2.solveFCG(G32.getFinestSystem(), G32, b, tol)
         res["fcg32_learned"].append((it2.info.seconds, it2.iterations))
         res["fallbacks"] += int(G32.learned_status()[1])
+        if args.learned_sides != 3:
+            # the network on both sides of the V-cycle, for comparison
+            G32.set_learned_sides(3)
+            G32.set_learned_active(True)
+            it2b = jf.IterativeSolver()
+            it2b.solveFCG(G32.getFinestSystem(), G32, b, tol)
+            res["fcg32_learned_both"].append((it2b.info.seconds, it2b.iterations))
+            res["fallbacks"] += int(G32.learned_status()[1])
         G32.set_learned_active(False)      # the same FP32 hierarchy with Chebyshev on every level
         it3 = jf.IterativeSolver()
         it3.solveFCG(G32.getFinestSystem(), G32, b, tol)
@@ -317,9 +326,12 @@ def bench_c4(jf, args):
         "config": {"workload": "2D -div(a grad u)=f, Q1 1024^2 cells, a=exp(GRF) contrast 1e3"
                                + ("; Galerkin coarse systems" if args.galerkin else ""),
                    "train_fields": args.train_fields, "pairs_per_field": max(1, args.pairs // 10) * 10,
-                   "eval_fields": args.eval_fields},
+                   "eval_fields": args.eval_fields,
+                   "learned_sides": {1: "pre-smoothing", 2: "post-smoothing", 3: "both"}[args.learned_sides]},
         "fp64_mgpcg_s": med("pcg64", 0), "fp64_mgpcg_iterations": med("pcg64", 1),
         "learned_fcg32_iterations": med("fcg32_learned", 1),
+        **({"learned_both_sides_fcg32_s": med("fcg32_learned_both", 0),
+            "learned_both_sides_fcg32_iterations": med("fcg32_learned_both", 1)} if res["fcg32_learned_both"] else {}),
         "fp32_cheb_fcg_s": med("fcg32_cheb", 0), "fp32_cheb_fcg_iterations": med("fcg32_cheb", 1),
         "learned_vs_best_plain_fp32": med("fcg32_cheb", 0) / value,
         "speedup_vs_fp64_mgpcg": med("pcg64", 0) / value,
@@ -442,20 +454,31 @@ def bench_c5(jf, args):
         net2 = jf.SmootherNet(2, args.c5_cin, ctx=ctx2)   # the same weights for the second component's hierarchy
         net2.set_params(net.get_params())
         G32b.set_learned(net2, np.zeros(cells * cells), runs=1, fallback_ratio=0.5)
-        res = []
-        for _ in range(args.warmup + args.steps):
-            G32.set_learned_active(True)
-            G32b.set_learned_active(True)
-            xl, r = solve(G32, tol, 1, comps=(G32b,))
-            res.append(r[0])
-        res = res[args.warmup:]
+
+        def solve_learned(sides):
+            for G_ in (G32, G32b):
+                G_.set_learned_sides(sides)
+            out = []
+            for _ in range(args.warmup + args.steps):
+                G32.set_learned_active(True)
+                G32b.set_learned_active(True)
+                x_, r_ = solve(G32, tol, 1, comps=(G32b,))
+                out.append(r_[0])
+            return x_, out[args.warmup:]
+
+        both = solve_learned(3)[1] if args.learned_sides != 3 else None
+        xl, res = solve_learned(args.learned_sides)
         errl = float(np.linalg.norm(xl.numpy()[:nv2] - xref_u) / np.linalg.norm(xref_u))
         value = float(np.median([v[0] for v in res]))
         fell = bool(G32.learned_status()[1] or G32b.learned_status()[1])
         log(f"learned + FP32 multigrid FGMRES: {res[-1][1]} its, {value:.3f}s, fell back {fell}")
-        line_extra = {"learned_fgmres_iterations": res[-1][1], "learned_velocity_rel_l2_vs_fp64": errl,
+        line_extra = {"learned_sides": {1: "pre-smoothing", 2: "post-smoothing", 3: "both"}[args.learned_sides],
+                      "learned_fgmres_iterations": res[-1][1], "learned_velocity_rel_l2_vs_fp64": errl,
                       "learned_fell_back": fell, "train_s": t_train,
                       "final_train_loss": float(np.mean(h[-8:]))}
+        if both:
+            line_extra["learned_both_sides_s"] = float(np.median([v[0] for v in both]))
+            line_extra["learned_both_sides_fgmres_iterations"] = both[-1][1]
     med = lambda r, i: float(np.median([v[i] for v in r]))
     line = {
         "metric": "time-to-1e-8 residual (s): Taylor-Hood Q2-Q1 Stokes FGMRES(50), block preconditioner with "
@@ -604,9 +627,10 @@ def main():
                     help="C5: network input channels (2: residual and coefficient; 4: plus node-parity channels)")
     ap.add_argument("--mass-its", type=int, default=3,
                     help="C5: Chebyshev steps on the pressure mass matrix in the Schur complement (0 = lumped)")
-    ap.add_argument("--learned-sides", type=int, default=2, choices=[1, 2, 3],
-                    help="C2: V-cycle smoothing calls that use the network (1 pre, 2 post: the fastest "
-                         "measured, 3 both); the other side keeps Chebyshev")
+    ap.add_argument("--learned-sides", type=int, default=None, choices=[1, 2, 3],
+                    help="V-cycle smoothing calls on the finest level that use the network (1 pre, 2 post, 3 both); "
+                         "the other side keeps Chebyshev.  Default: the fastest measured per workload (C2 and C4: "
+                         "2, C5: 3)")
     ap.add_argument("--network-only", action="store_true",
                     help="time the learned smoother's network forward alone on the finest node lattice")
     ap.add_argument("--assembly-only", action="store_true",
@@ -615,6 +639,8 @@ def main():
     ap.add_argument("--spmv-only", action="store_true",
                     help="assemble the finest matrix and time only the SpMV and the copy calibration (for PMC runs)")
     args = ap.parse_args()
+    if args.learned_sides is None:
+        args.learned_sides = 3 if args.workload == "c5" else 2
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the solve path runs on a single GPU "
               "(a multi-GPU slab decomposition is designed but not implemented); run with --gpus 1", file=sys.stderr)
