@@ -139,48 +139,54 @@ __host__ __device__ inline double lag1(double t, int l, int k)
     return r;
 }
 
+// dof_of_lattice / lattice_of_dof with D fixed, in 32-bit arithmetic: the levels of a hierarchy
+// have fewer than 2^31 nodes (checked when a level is built), so every partial product and every
+// index fits, and the divisions are 32-bit (a 64-bit division is an emulated sequence)
 template <int D>
 __device__ inline long dof_of_lat(const GridDev& g, const int* p)
 {
-    long tail[D];
-    long t = 1;
+    const unsigned k = (unsigned)g.k;
+    unsigned tail[D];
+    unsigned t = 1;
 #pragma unroll
-    for (int a = D - 1; a >= 0; a--) { tail[a] = t; t *= (long)g.k * g.n[a] + 1; }
-    long pre = 0, mprod = 1, rank = 0;
+    for (int a = D - 1; a >= 0; a--) { tail[a] = t; t *= k * (unsigned)g.n[a] + 1u; }
+    unsigned pre = 0, mprod = 1, rank = 0;
 #pragma unroll
     for (int a = 0; a < D; a++) {
-        const int c = p[a] == 0 ? 0 : (p[a] - 1) / g.k;
-        const int l = p[a] - c * g.k;
-        const long m = c > 0 ? g.k : g.k + 1;
-        const long S = c > 0 ? (long)c * g.k + 1 : 0;
+        const unsigned pa = (unsigned)p[a];
+        const unsigned c = pa == 0 ? 0u : (pa - 1u) / k;
+        const unsigned l = pa - c * k;
+        const unsigned m = c > 0 ? k : k + 1u;
+        const unsigned S = c > 0 ? c * k + 1u : 0u;
         pre += mprod * S * tail[a];
-        rank += (long)(l - (c > 0 ? 1 : 0)) * mprod;
+        rank += (l - (c > 0 ? 1u : 0u)) * mprod;
         mprod *= m;
     }
-    return pre + rank;
+    return (long)(pre + rank);
 }
 
 template <int D>
 __device__ inline void lat_of_dof(const GridDev& g, long dof, int* p)
 {
-    long tail[D];
-    long t = 1;
+    const unsigned k = (unsigned)g.k;
+    unsigned tail[D];
+    unsigned t = 1;
 #pragma unroll
-    for (int a = D - 1; a >= 0; a--) { tail[a] = t; t *= (long)g.k * g.n[a] + 1; }
-    long rem = dof, mprod = 1;
-    int c[D];
+    for (int a = D - 1; a >= 0; a--) { tail[a] = t; t *= k * (unsigned)g.n[a] + 1u; }
+    unsigned rem = (unsigned)dof, mprod = 1;
+    unsigned c[D];
 #pragma unroll
     for (int a = 0; a < D; a++) {
-        const long unit = mprod * tail[a];
-        const long q = rem / unit;
-        c[a] = q < g.k + 1 ? 0 : (int)((q - 1) / g.k);
-        rem -= (c[a] > 0 ? (long)c[a] * g.k + 1 : 0) * unit;
-        mprod *= c[a] > 0 ? g.k : g.k + 1;
+        const unsigned unit = mprod * tail[a];
+        const unsigned q = rem / unit;
+        c[a] = q < k + 1u ? 0u : (q - 1u) / k;
+        rem -= (c[a] > 0 ? c[a] * k + 1u : 0u) * unit;
+        mprod *= c[a] > 0 ? k : k + 1u;
     }
 #pragma unroll
     for (int a = 0; a < D; a++) {
-        const long m = c[a] > 0 ? g.k : g.k + 1;
-        p[a] = c[a] * g.k + (int)(rem % m) + (c[a] > 0 ? 1 : 0);
+        const unsigned m = c[a] > 0 ? k : k + 1u;
+        p[a] = (int)(c[a] * k + rem % m + (c[a] > 0 ? 1u : 0u));
         rem /= m;
     }
 }
@@ -853,6 +859,8 @@ static jfem_mg* create_mg(jfem_space* const* spaces, int nlev, int nint, const i
         for (int l = 0; l < nlev; l++) {
             MgLevel& L = G->lev[l];
             L.grid = spaces[l]->grid;
+            // the transfers' index arithmetic is 32-bit (dof_of_lat, lat_of_dof)
+            JFEM_CHECK(L.grid.ndofs() < (1l << 31), JFEM_ERR_UNSUPPORTED, "a multigrid level has 2^31 or more nodes");
             L.A = assemble_matrix(spaces[l], nint, kinds, weights, coeff ? coef[l].data() : nullptr, rule);
             const long n = L.A->rows;
             JFEM_HIP(hipMalloc(&L.bnd, n));
