@@ -56,7 +56,9 @@ typedef struct jfem_nn jfem_nn;
 const char *jfem_last_error(void);
 
 /* ---- context ------------------------------------------------------------ */
-/* no reference counterpart: owns the HIP stream and reduction scratch */
+/* no reference counterpart: owns the HIP stream (plus an auxiliary stream that assembly uses
+ * internally and joins before it returns), reduction scratch and set-up scratch (kept until
+ * jfem_ctx_destroy) */
 int jfem_ctx_create(int device, jfem_ctx **out);
 int jfem_ctx_destroy(jfem_ctx *ctx);
 int jfem_ctx_sync(jfem_ctx *ctx);
