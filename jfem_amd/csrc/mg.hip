@@ -216,17 +216,19 @@ __global__ __launch_bounds__(MB) void k_tp_pass(Shape3 sin, Shape3 sout, GridDev
             }
             lat_of_dof<D>(gout, t, q);
         } else {
-            q[0] = (int)(t % sout.n[0]);
-            const long r = t / sout.n[0];
+            // lattice arrays keep the last axis fastest, the fast axis of the DOF order: the first
+            // and last passes then gather or scatter DOF-ordered data along short runs
+            q[2] = (int)(t % sout.n[2]);
+            const long r = t / sout.n[2];
             q[1] = (int)(r % sout.n[1]);
-            q[2] = (int)(r / sout.n[1]);
+            q[0] = (int)(r / sout.n[1]);
         }
         int p[3] = {q[0], q[1], q[2]};
         const int row = q[AX];
         double acc = 0;
         for (int e = rp[row]; e < rp[row + 1]; e++) {
             p[AX] = ix[e];
-            const long src = IN_DOF ? dof_of_lat<D>(gin, p) : p[0] + (long)sin.n[0] * (p[1] + (long)sin.n[1] * p[2]);
+            const long src = IN_DOF ? dof_of_lat<D>(gin, p) : p[2] + (long)sin.n[2] * (p[1] + (long)sin.n[1] * p[0]);
             acc += w[e] * (double)in[src];
         }
         if (OUT_DOF && add) acc += (double)out[t];
