@@ -142,10 +142,10 @@ __host__ __device__ inline double lag1(double t, int l, int k)
 // dof_of_lattice / lattice_of_dof with D fixed, in 32-bit arithmetic: the levels of a hierarchy
 // have fewer than 2^31 nodes (checked when a level is built), so every partial product and every
 // index fits, and the divisions are 32-bit (a 64-bit division is an emulated sequence)
-template <int D>
-__device__ inline long dof_of_lat(const GridDev& g, const int* p)
+template <int D, int KK>
+__device__ inline long dof_of_lat_k(const GridDev& g, const int* p)
 {
-    const unsigned k = (unsigned)g.k;
+    const unsigned k = KK > 0 ? (unsigned)KK : (unsigned)g.k;
     unsigned tail[D];
     unsigned t = 1;
 #pragma unroll
@@ -165,10 +165,10 @@ __device__ inline long dof_of_lat(const GridDev& g, const int* p)
     return (long)(pre + rank);
 }
 
-template <int D>
-__device__ inline void lat_of_dof(const GridDev& g, long dof, int* p)
+template <int D, int KK>
+__device__ inline void lat_of_dof_k(const GridDev& g, long dof, int* p)
 {
-    const unsigned k = (unsigned)g.k;
+    const unsigned k = KK > 0 ? (unsigned)KK : (unsigned)g.k;
     unsigned tail[D];
     unsigned t = 1;
 #pragma unroll
@@ -189,6 +189,20 @@ __device__ inline void lat_of_dof(const GridDev& g, long dof, int* p)
         p[a] = (int)(c[a] * k + rem % m + (c[a] > 0 ? 1u : 0u));
         rem /= m;
     }
+}
+
+// the degree-2 grids of the Q2 hierarchies with k fixed at compile time (divisions by 2 and 3)
+template <int D>
+__device__ inline long dof_of_lat(const GridDev& g, const int* p)
+{
+    return g.k == 2 ? dof_of_lat_k<D, 2>(g, p) : dof_of_lat_k<D, 0>(g, p);
+}
+
+template <int D>
+__device__ inline void lat_of_dof(const GridDev& g, long dof, int* p)
+{
+    if (g.k == 2) lat_of_dof_k<D, 2>(g, dof, p);
+    else lat_of_dof_k<D, 0>(g, dof, p);
 }
 
 struct Shape3 {
