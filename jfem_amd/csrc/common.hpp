@@ -202,6 +202,10 @@ struct jfem_mat {
     long blk_cap = 0, blk32_cap = 0;   // entries allocated for blk / blk32 (and part)
     double* part = nullptr;  // device: one partial sum per stream block
     jfem_mat* T = nullptr;   // cached transpose for tvMul (dropped when values change)
+    // the hierarchy whose finest level operator is this matrix (set when the hierarchy is built
+    // with a matrix-free finest level, cleared when the values change): the Krylov solvers
+    // apply the matrix through that operator
+    jfem_mg* op = nullptr;
 };
 
 namespace jfem {
@@ -323,6 +327,9 @@ jfem_mat* assemble_matrix(jfem_space* s, int nint, const int* kinds, const doubl
 void element_1d(const Grid& g, int a, int rule, double* K1, double* M1);
 // multigrid preconditioner application z = M r (FP64 vectors on the context stream)
 void mg_apply(jfem_mg* mg, const double* r, double* z);
+// y = A x through the matrix-free operator of the hierarchy A->op names (the same operator as A
+// to rounding); false when A has none
+bool mg_op_apply(const jfem_mat* A, const double* x, double* y);
 long mg_size(const jfem_mg* mg);
 // learned-smoother fallback rule: called with the residual reduction of a cycle/iteration
 void mg_observe_reduction(jfem_mg* mg, double ratio);

@@ -112,10 +112,30 @@ static void read_scalars(jfem_ctx* c, const double* dev, int m, double* out)
 
 static int kgrid(long n) { return grid_for(n, KB, RED_BLOCKS); }
 
+// A x for the solvers: a matrix tagged with a hierarchy's matrix-free finest operator (the
+// assembled level matrix, the same operator to rounding) is applied through it, without
+// reading its entries; any other matrix through the CSR product
+static void op_mul(jfem_mat* A, const double* x, double* y)
+{
+    if (!mg_op_apply(A, x, y)) spmv(A, x, y);
+}
+
+// y = A x and dst = x.y (device)
+static void op_mul_dot(jfem_mat* A, const double* x, double* y, double* dst)
+{
+    if (!mg_op_apply(A, x, y)) {
+        spmv_dot_dev(A, x, y, dst);
+        return;
+    }
+    const double* xs[1] = {x};
+    const double* ys[1] = {y};
+    dots_dev(A->ctx, A->rows, 1, xs, ys, dst);
+}
+
 static double true_resnorm(jfem_mat* A, const double* b, const double* x, double* tmp)
 {
     jfem_ctx* c = A->ctx;
-    spmv(A, x, tmp);
+    op_mul(A, x, tmp);
     k_sub<<<kgrid(A->rows), KB, 0, c->stream>>>(A->rows, b, tmp, tmp);
     return std::sqrt(vec_dot(c, A->rows, tmp, tmp));
 }
@@ -131,7 +151,7 @@ static void solve_cg(jfem_mat* A, jfem_mg* M, const double* b, double* x, double
     double* W = ctx_work(c, 4 * (size_t)n);
     const DevPtr<double> r{W}, z{W + n}, p{W + 2 * n}, q{W + 3 * n};
     // r = b - A x
-    spmv(A, x, q.p);
+    op_mul(A, x, q.p);
     k_sub<<<kgrid(n), KB, 0, c->stream>>>(n, b, q.p, r.p);
     if (M) mg_apply(M, r.p, z.p);
     vec_copy(c, p.p, M ? z.p : r.p, n);
@@ -147,7 +167,7 @@ static void solve_cg(jfem_mat* A, jfem_mg* M, const double* b, double* x, double
     int it = 0;
     const long cap = std::min(n, maxit);
     for (long iter = 0; iter < cap && std::sqrt(rr) > tol; iter++) {
-        spmv_dot_dev(A, p.p, q.p, sc + 1);                   // q = A p, sc[1] = p.Ap
+        op_mul_dot(A, p.p, q.p, sc + 1);                   // q = A p, sc[1] = p.Ap
         k_cg_update<<<kgrid(n), KB, 0, c->stream>>>(n, sc, 0, 1, p.p, q.p, x, r.p, c->red_partials);
         reduce_partials_dev(c, 1, kgrid(n), sc + 2);         // sc[2] = r.r (new)
         if (M) {
@@ -182,7 +202,7 @@ static void solve_fcg(jfem_mat* A, jfem_mg* M, const double* b, double* x, doubl
     double* sc = c->scal;
     double* W = ctx_work(c, 4 * (size_t)n);
     const DevPtr<double> r{W}, z{W + n}, p{W + 2 * n}, q{W + 3 * n};
-    spmv(A, x, q.p);
+    op_mul(A, x, q.p);
     k_sub<<<kgrid(n), KB, 0, c->stream>>>(n, b, q.p, r.p);
     mg_apply(M, r.p, z.p);
     vec_copy(c, p.p, z.p, n);
@@ -203,7 +223,7 @@ static void solve_fcg(jfem_mat* A, jfem_mg* M, const double* b, double* x, doubl
             const double* ys[1] = {r.p};
             dots_dev(c, n, 1, xs, ys, sc + 0);
         }
-        spmv_dot_dev(A, p.p, q.p, sc + 1);
+        op_mul_dot(A, p.p, q.p, sc + 1);
         k_cg_update<<<kgrid(n), KB, 0, c->stream>>>(n, sc, 0, 1, p.p, q.p, x, r.p, c->red_partials);
         reduce_partials_dev(c, 1, kgrid(n), sc + 2);
         mg_apply(M, r.p, z.p);
@@ -264,7 +284,7 @@ static void solve_bicgstab(jfem_mat* A, jfem_mg* M, const double* b, double* x, 
         z{W + 7 * n}, pt{W + 8 * n}, tmp{W + 9 * n};
     vec_fill(c, v.p, 0, n);
     vec_fill(c, p.p, 0, n);
-    spmv(A, x, tmp.p);
+    op_mul(A, x, tmp.p);
     k_sub<<<kgrid(n), KB, 0, c->stream>>>(n, b, tmp.p, r0.p);
     vec_copy(c, r.p, r0.p, n);
     double alpha = 1, omega = 1, rho = 1, rhoLast = 1, beta = 1;
@@ -289,7 +309,7 @@ static void solve_bicgstab(jfem_mat* A, jfem_mg* M, const double* b, double* x, 
         }
         beta = alpha / omega * rho / rhoLast;
         k_bicg_p<<<g, KB, 0, c->stream>>>(n, beta, omega * beta, r.p, v.p, p.p);
-        if (M) { mg_apply(M, p.p, y.p); spmv(A, y.p, v.p); } else spmv(A, p.p, v.p);
+        if (M) { mg_apply(M, p.p, y.p); op_mul(A, y.p, v.p); } else op_mul(A, p.p, v.p);
         alpha = rho / vec_dot(c, n, v.p, r0.p);
         k_axmy<<<g, KB, 0, c->stream>>>(n, alpha, r.p, v.p, s.p);
         if (std::sqrt(vec_dot(c, n, s.p, s.p)) < tol) {
@@ -298,7 +318,7 @@ static void solve_bicgstab(jfem_mat* A, jfem_mg* M, const double* b, double* x, 
         }
         if (M) {
             mg_apply(M, s.p, z.p);
-            spmv(A, z.p, t.p);
+            op_mul(A, z.p, t.p);
             mg_apply(M, t.p, pt.p);
             double d[2];
             const double* xs[2] = {pt.p, pt.p};
@@ -307,7 +327,7 @@ static void solve_bicgstab(jfem_mat* A, jfem_mg* M, const double* b, double* x, 
             omega = d[0] / d[1];
             k_x2<<<g, KB, 0, c->stream>>>(n, alpha, omega, y.p, z.p, x);
         } else {
-            spmv(A, s.p, t.p);
+            op_mul(A, s.p, t.p);
             double d[2];
             const double* xs[2] = {t.p, t.p};
             const double* ys[2] = {s.p, t.p};
@@ -435,7 +455,7 @@ static void solve_gmres(jfem_mat* A, jfem_mg* M, const double* b, double* x, dou
     try {
         for (;;) {
             // r = M (b - A x)
-            spmv(A, x, t.p);
+            op_mul(A, x, t.p);
             k_sub<<<nb, KB, 0, c->stream>>>(n, b, t.p, r0.p);
             double* r = V;   // v_0 slot holds r, normalised in place
             if (M) mg_apply(M, r0.p, r); else vec_copy(c, r, r0.p, n);
@@ -448,7 +468,7 @@ static void solve_gmres(jfem_mat* A, jfem_mg* M, const double* b, double* x, dou
             int nv = 1, j;
             for (j = 0; j < n; j++) {
                 const double* vj = V + (long)j * ldv;
-                if (M) { spmv(A, vj, tq.p); mg_apply(M, tq.p, q.p); } else spmv(A, vj, q.p);
+                if (M) { op_mul(A, vj, tq.p); mg_apply(M, tq.p, q.p); } else op_mul(A, vj, q.p);
                 k_multidot<<<nb, KB, 0, c->stream>>>(n, nv, V, ldv, q.p, part.p, nb);
                 k_sum_partials<<<nv, KB, 0, c->stream>>>(nv, nb, part.p, fdev.p);
                 k_gs_sub<<<nb, KB, 0, c->stream>>>(n, nv, V, ldv, fdev.p, q.p, w.p);
@@ -550,7 +570,7 @@ static void smooth_any(jfem_mat* A, int kind, int runs, double omega, int degree
         if (kind == JFEM_SM_CHEBYSHEV) {
             double rho_old = 1.0 / sigma;
             for (int kd = 0; kd < degree; kd++) {
-                spmv(A, x, t);
+                op_mul(A, x, t);
                 k_sub<<<nb, KB, 0, c->stream>>>(n, b, t, r);
                 double c1 = 0, c2 = 0;
                 if (kd > 0) {
@@ -563,7 +583,7 @@ static void smooth_any(jfem_mat* A, int kind, int runs, double omega, int degree
             }
             continue;
         }
-        spmv(A, x, t);
+        op_mul(A, x, t);
         k_sub<<<nb, KB, 0, c->stream>>>(n, b, t, r);
         if (kind == JFEM_SM_RICHARDSON && std::sqrt(vec_dot(c, n, r, r)) < 1e-10) break;   // RichardsonSmoother :36-41
         k_relax<<<nb, KB, 0, c->stream>>>(n, omega, kind == JFEM_SM_JACOBI ? dinv.p : nullptr, r, x);
@@ -584,7 +604,7 @@ static void smooth_any(jfem_mat* A, int kind, int runs, double omega, int degree
 // bounds the iterations.
 static void ma_apply(jfem_mat* A, jfem_mg* M, const double* x, double* tmp, double* y)
 {
-    spmv(A, x, M ? tmp : y);
+    op_mul(A, x, M ? tmp : y);
     if (M) mg_apply(M, tmp, y);
 }
 
@@ -600,7 +620,7 @@ static void solve_gmres2(jfem_mat* A, jfem_mg* M, const double* b, double* x, do
     const int nb = kgrid(n);
     long it = 0;
     for (;;) {
-        spmv(A, x, t);
+        op_mul(A, x, t);
         k_sub<<<nb, KB, 0, c->stream>>>(n, b, t, r);
         if (M) mg_apply(M, r, Mr); else vec_copy(c, Mr, r, n);
         // Arnoldi constructor
@@ -620,7 +640,7 @@ static void solve_gmres2(jfem_mat* A, jfem_mg* M, const double* b, double* x, do
             const double* av = AV.p + (size_t)(nv - 1) * n;
             const double tt = vec_dot(c, n, Mr, av) / vec_dot(c, n, av, av);
             vec_axpby(c, n, tt, v, 1.0, x);
-            spmv(A, v, t);
+            op_mul(A, v, t);
             vec_axpby(c, n, -tt, t, 1.0, r);
             vec_axpby(c, n, -tt, av, 1.0, Mr);
             it++;
@@ -921,7 +941,7 @@ static void solve_stokes_fgmres(jfem_mat* A, StokesPrec& P, const double* b, dou
     }
     const double target = rtol * bn;
     for (;;) {
-        spmv(A, x, t);
+        op_mul(A, x, t);
         k_sub<<<nb, KB, 0, c->stream>>>(n, b, t, w);
         JFEM_HIP(hipGetLastError());
         res = std::sqrt(vec_dot(c, n, w, w));
@@ -935,7 +955,7 @@ static void solve_stokes_fgmres(jfem_mat* A, StokesPrec& P, const double* b, dou
             double* vj = V.p + (size_t)j * n;
             double* zj = Z.p + (size_t)j * n;
             P.apply(vj, zj);
-            spmv(A, zj, q);
+            op_mul(A, zj, q);
             // h = V^T q, w = q - V h; h2 = V^T w, q = w - V h2
             k_multidot<<<nb, KB, 0, c->stream>>>(n, j + 1, V.p, n, q, part, nb);
             k_sum_partials<<<j + 1, KB, 0, c->stream>>>(j + 1, nb, part, fdev);

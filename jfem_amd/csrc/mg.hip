@@ -681,6 +681,14 @@ void mg_apply(jfem_mg* G, const double* r, double* z)
 
 long mg_size(const jfem_mg* G) { return G->lev.back().A->rows; }
 
+bool mg_op_apply(const jfem_mat* A, const double* x, double* y)
+{
+    jfem_mg* G = A->op;
+    if (!G || G->lev.back().A != A || !G->lev.back().mf || G->ctx != A->ctx) return false;
+    level_apply<double>(G, G->lev.back(), x, EpiStore<double>{y});
+    return true;
+}
+
 const Grid& mg_finest_grid(const jfem_mg* G) { return G->lev.back().grid; }
 
 const unsigned char* mg_finest_bnd(const jfem_mg* G) { return G->lev.back().bnd; }
@@ -958,6 +966,7 @@ static jfem_mg* create_mg(jfem_space* const* spaces, int nlev, int nint, const i
         JFEM_HIP(hipMalloc(&G->init, sizeof(double) * nf));
         JFEM_HIP(hipMalloc(&G->defect, sizeof(double) * nf));
         JFEM_HIP(hipMalloc(&G->tmp, sizeof(double) * nf));
+        if (G->lev.back().mf) G->lev.back().A->op = G;
         JFEM_HIP(hipStreamSynchronize(c->stream));
     } catch (...) {
         free_mg(G);

@@ -470,6 +470,7 @@ void apply_dirichlet_mask(jfem_mat* A, double* rhs, const unsigned char* fixed_d
 {
     jfem_ctx* c = A->ctx;
     mat_drop_transpose(A);   // the values change
+    A->op = nullptr;
     insert_fixed_diagonals(A, fixed_dev);
     int* flag = nullptr;
     JFEM_HIP(hipMalloc(&flag, sizeof(int)));

@@ -106,3 +106,35 @@ def test_matrix_free_vcycle_matches_oracle(jf, oracle, which):
         got = G.vCycle(jf.DenseVector(x), jf.DenseVector(b)).numpy()
         exp = O.vcycle(x, b)
         assert np.linalg.norm(got - exp) <= 1e-11 * np.linalg.norm(exp), (base, which)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_krylov_products_through_matrix_free_finest_operator(jf, precision):
+    """The finest system of a hierarchy with a matrix-free finest level is applied by the Krylov
+    solvers through that operator (no matrix read).  Solving with it and with a CSR copy of the
+    same matrix (an untagged matrix: CSR products) gives the same iteration count and solutions
+    equal to rounding; they are not bit-identical, which shows the tagged matrix took the
+    matrix-free route.  PCG around the FP64 cycle and flexible CG around the FP32 one."""
+    G, spaces = _hier(jf, [3, 5, 3], 2, 3, [("GradGrad", 1.0)], coeff_seed=3, precision=precision)
+    A = G.getFinestSystem()
+    copy = jf.SparseMatrix.from_csr(*A.to_csr())
+    n = A.info()[0]
+    rng = np.random.default_rng(11)
+    rhs = rng.standard_normal(n)
+    rhs[spaces[-1].boundary_nodes()] = 0
+    b = jf.DenseVector(rhs)
+    tol = 1e-9 * np.linalg.norm(rhs)
+    xs, its = [], []
+    for M in (A, copy):
+        it = jf.IterativeSolver()
+        if precision:
+            x = it.solveFCG(M, G, b, tol).numpy()
+        else:
+            x = it.solvePCG(M, G, b, tol, x0=jf.DenseVector(np.zeros(n))).numpy()
+        xs.append(x)
+        its.append(it.iterations)
+    ref = _csr(A)
+    assert its[0] == its[1]
+    assert np.linalg.norm(rhs - ref @ xs[0]) <= 1.01 * tol
+    assert np.linalg.norm(xs[0] - xs[1]) <= 1e-10 * np.linalg.norm(xs[1])
+    assert not np.array_equal(xs[0], xs[1])
