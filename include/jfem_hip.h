@@ -193,7 +193,11 @@ typedef struct {
 /* solveCG / solvePCG (:48-162): stop when ||r||_2 <= tol (absolute), at most
  * min(n, maxit) iterations; x holds the start iterate on entry (the reference
  * default is e_0) and the solution on return.  M nullable (NULL = CG).
- * res_hist (nullable, maxit entries) receives ||r|| per iteration. */
+ * res_hist (nullable, maxit entries) receives ||r|| per iteration.
+ * This and the other Krylov solvers apply A as operator.mvMul does (:134,141);
+ * when A is the finest system of a hierarchy with a matrix-free finest level
+ * (jfem_mg_level_matrix, values unchanged since) the product runs through that
+ * operator, equal to the CSR product to rounding. */
 int jfem_solve_cg(jfem_mat *A, jfem_mg *M, const jfem_vec *b, jfem_vec *x, double tol, long maxit,
                   jfem_solve_info *info, double *res_hist);
 /* solveBiCGStab / solvePBiCGStab (:234-402) */
