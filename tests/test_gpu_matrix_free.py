@@ -138,3 +138,15 @@ def test_krylov_products_through_matrix_free_finest_operator(jf, precision):
     assert np.linalg.norm(rhs - ref @ xs[0]) <= 1.01 * tol
     assert np.linalg.norm(xs[0] - xs[1]) <= 1e-10 * np.linalg.norm(xs[1])
     assert not np.array_equal(xs[0], xs[1])
+    # re-applying the same Dirichlet nodes leaves the values as they are but counts as a change of
+    # values: the tag goes, and the matrix takes the CSR product (bit-identical to the copy's solve)
+    from jfem_amd._native import check, i32, iptr, lib
+    nodes = i32(spaces[-1].boundary_nodes())
+    check(lib().jfem_apply_dirichlet(A.h, None, len(nodes), iptr(nodes), None))
+    it = jf.IterativeSolver()
+    if precision:
+        x = it.solveFCG(A, G, b, tol).numpy()
+    else:
+        x = it.solvePCG(A, G, b, tol, x0=jf.DenseVector(np.zeros(n))).numpy()
+    assert (_csr(A) != _csr(copy)).nnz == 0
+    assert np.array_equal(x, xs[1])
