@@ -27,6 +27,7 @@
 
 #include "common.hpp"
 
+#include <type_traits>
 #include <vector>
 
 namespace jfem {
@@ -159,6 +160,20 @@ __device__ inline void mf_cell_part(const MfParams& P, const T* xs, int s0, int 
     }
 }
 
+// epilogues whose operand x is the product's own (tile_x): k_mf loads the rest of the row operands
+// (load_nox) and takes x at the row from its node tile instead of a second global read
+template <typename E, typename = void>
+struct EpiTileX : std::false_type {};
+template <typename E>
+struct EpiTileX<E, std::void_t<decltype(E::tile_x)>> : std::bool_constant<E::tile_x> {};
+
+template <typename E>
+__device__ inline typename E::Pre epi_load(const E& e, long row)
+{
+    if constexpr (EpiTileX<E>::value) return e.load_nox(row);
+    else return e.load(row);
+}
+
 // Epi: the row epilogues of stream.hpp (Pre load(row), operator()(row, ax, pre)); coef: one
 // value per cell of the level (reference cell order, last axis fastest), nullptr = 1
 // waves per SIMD the kernel is built for: six in FP64 (four workgroups per CU), seven in FP32
@@ -263,7 +278,7 @@ __global__ __launch_bounds__((MfShape<D, K>::BS), (mf_waves<T, Epi>())) void k_m
         row[u] = 0;
         if (own[u]) {
             row[u] = dof_of(olo[0] + o0 - org[0], olo[1] + o1 - org[1], olo[2] + o2 - org[2]);
-            if (PRE_EARLY) pre[u] = epi.load(row[u]);
+            if (PRE_EARLY) pre[u] = epi_load(epi, row[u]);
             if (inner && CAND_EARLY)
                 candw[u] = *reinterpret_cast<const uint4*>(
                     P.cand + 8 * (((o0 + 1) * NB1 + o1 + 1) * NB2 + (D == 3 ? o2 + 1 : 0)));
@@ -328,7 +343,7 @@ __global__ __launch_bounds__((MfShape<D, K>::BS), (mf_waves<T, Epi>())) void k_m
 #pragma unroll
     for (int u = 0; u < NOWN; u++) {
         if (!own[u]) continue;
-        if (!PRE_EARLY) pre[u] = epi.load(row[u]);
+        if (!PRE_EARLY) pre[u] = epi_load(epi, row[u]);
         const int i = tid + u * BS;
         const int o2 = i % OB2, o1 = (i / OB2) % OB1, o0 = i / (OB1 * OB2);
         if (inner) {
@@ -347,6 +362,8 @@ __global__ __launch_bounds__((MfShape<D, K>::BS), (mf_waves<T, Epi>())) void k_m
             T acc = 0;
 #pragma unroll
             for (int c = 0; c < (1 << D); c++) acc += v[c];
+            if constexpr (EpiTileX<Epi>::value)   // inner tile: no boundary nodes, box position o + 1
+                pre[u].x = xs[((o0 + 1) * NB1 + o1 + 1) * NB2 + (D == 3 ? o2 + 1 : 0)];
             epi(row[u], acc, pre[u]);
             continue;
         }
@@ -357,6 +374,7 @@ __global__ __launch_bounds__((MfShape<D, K>::BS), (mf_waves<T, Epi>())) void k_m
         T ax;
         if (bnd) {
             ax = x[row[u]];   // identity row of the eliminated system
+            if constexpr (EpiTileX<Epi>::value) pre[u].x = ax;   // the tile holds 0 for boundary nodes
         } else {
             // the cells containing p; all lie in the computed box
             int q[3], elo[3] = {0, 0, 0}, ehi[3] = {0, 0, 0};
@@ -385,6 +403,7 @@ __global__ __launch_bounds__((MfShape<D, K>::BS), (mf_waves<T, Epi>())) void k_m
 #pragma unroll
             for (int c = 0; c < (1 << D); c++) acc += v[c];
             ax = acc;
+            if constexpr (EpiTileX<Epi>::value) pre[u].x = xs[(q[0] * NB1 + q[1]) * NB2 + (D == 3 ? q[2] : 0)];
         }
         epi(row[u], ax, pre[u]);
     }

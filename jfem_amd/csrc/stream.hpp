@@ -196,6 +196,9 @@ struct EpiJacobi {
     T omega;
     struct Pre { T b, x, dinv; };
     __device__ Pre load(long row) const { return {b[row], x[row], dinv[row]}; }
+    // x is the product's operand: the matrix-free kernel fills pre.x from its node tile
+    static constexpr bool tile_x = true;
+    __device__ Pre load_nox(long row) const { return {b[row], (T)0, dinv[row]}; }
     __device__ double operator()(long row, T ax, const Pre& p) const
     {
 #pragma clang fp contract(off)
@@ -214,6 +217,8 @@ struct EpiCheby {
     int first;
     struct Pre { T b, x, dinv, d; };
     __device__ Pre load(long row) const { return {b[row], x[row], dinv[row], first ? (T)0 : d[row]}; }
+    static constexpr bool tile_x = true;   // as in EpiJacobi
+    __device__ Pre load_nox(long row) const { return {b[row], (T)0, dinv[row], first ? (T)0 : d[row]}; }
     __device__ double operator()(long row, T ax, const Pre& p) const
     {
 #pragma clang fp contract(off)
