@@ -143,9 +143,9 @@ def cpu_baseline(seconds_budget=5.0):
                 spmv_gbs=bytes_spmv / t_spmv / 1e9)
 
 
-def measured_traffic(cells):
-    """HBM bytes per SpMV launch from the committed rocprofv3 PMC summary (96^3 only)."""
-    path = os.path.join(ROOT, "profiles", "pmc_spmv_fp64_q2_96.md")
+def measured_traffic(cells, name="pmc_spmv_fp64_q2_96.md"):
+    """HBM bytes per launch from a committed rocprofv3 PMC summary (96^3 only)."""
+    path = os.path.join(ROOT, "profiles", name)
     if cells != 96 or not os.path.exists(path):
         return None
     for line in open(path):
@@ -744,6 +744,14 @@ def main():
     spmv32_ms = float(np.median([A.spmv_time_ms(xs, ys, reps=50, fp32=True) for _ in range(3)]))
     spmv32_gbs = (8 * nnz + 4 * (n + 1) + 8 * n) / (spmv32_ms * 1e-3) / 1e9
     log(f"SpMV fp64 {spmv_ms:.3f} ms {spmv_gbs:.0f} GB/s; fp32 {spmv32_ms:.3f} ms {spmv32_gbs:.0f} GB/s")
+    # the solve's dominant launch: the finest level's Chebyshev step (matrix-free), timed the same way;
+    # algorithmic bytes: x, b, D^-1 and d read, the new x and d written, 8 bytes each per row (the
+    # constant coefficient adds none)
+    G.level_step_time_ms(nlev - 1, xs, ys, reps=20)
+    step_ms = float(np.median([G.level_step_time_ms(nlev - 1, xs, ys, reps=50) for _ in range(3)]))
+    step_bytes = 48 * n
+    step_gbs = step_bytes / (step_ms * 1e-3) / 1e9
+    log(f"finest Chebyshev step {step_ms * 1e3:.1f} us {step_gbs:.0f} GB/s")
     copy_gbs, _ = copy_calibration(jf)
     read_gbs, _ = read_calibration(jf)
     log(f"copy calibration {copy_gbs:.0f} GB/s, streaming read {read_gbs:.0f} GB/s")
@@ -901,6 +909,16 @@ def main():
                                         "a warm-up batch",
                      "bytes_per_launch": spmv_bytes, "copy_kernel_gbs": copy_gbs,
                      "read_kernel_gbs": read_gbs, "frac_of_read_kernel": spmv_gbs / read_gbs},
+        "roofline_smoother": {"bound": "hbm", "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": step_gbs / HBM_PEAK_GBS,
+                              "traffic": measured_traffic(args.cells, "pmc_matrix_free_q2_96.md"),
+                              "kernel": "k_mf<double,3,2,EpiCheby<double>> (finest-level matrix-free Chebyshev "
+                                        "step, the FP64 solve's dominant launch)",
+                              "traffic_source": "profiles/pmc_matrix_free_q2_96.md: rocprofv3 FETCH_SIZE x 2.0 + "
+                                                "WRITE_SIZE x 1.0 per launch",
+                              "achieved_timing": "hipEvents on the context stream, median of 3 batches of 50 "
+                                                 "launches after a warm-up batch",
+                              "bytes_per_launch": step_bytes, "ms": step_ms},
         "cpu_baseline": cpu,
     }
     line.update(extra)

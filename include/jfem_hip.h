@@ -312,6 +312,11 @@ int jfem_mg_level_matrix(jfem_mg *mg, int level, jfem_mat **out);
 int jfem_mg_level_apply(jfem_mg *mg, int level, const jfem_vec *x, jfem_vec *y);
 /* whether the hierarchy applies level `level` matrix-free (out: 1) or by its CSR matrix (0); new */
 int jfem_mg_level_matrix_free(jfem_mg *mg, int level, int *out);
+/* repeat one Chebyshev step of level `level` (the smoothers' launch: matrix-free where
+ * jfem_mg_level_matrix_free reports 1, else through the CSR matrix) `reps` times on the context
+ * stream from x into y and return the mean kernel time (ms), as jfem_spmv_bench does for the
+ * SpMV; b and the direction are the level's own work vectors; FP64 hierarchies; new */
+int jfem_mg_level_bench(jfem_mg *mg, int level, const jfem_vec *x, jfem_vec *y, int reps, double *ms);
 /* MGPreconditionerInterface.mvMul: z = M r */
 int jfem_mg_apply(jfem_mg *mg, const jfem_vec *r, jfem_vec *z);
 /* one V-cycle mgStepV(finest, x, b) updating x in place */

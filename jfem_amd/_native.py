@@ -121,6 +121,7 @@ _SIGS = {
     "jfem_mg_destroy": (C.c_int, [_vp]),
     "jfem_mg_level_matrix": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
     "jfem_mg_level_apply": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+    "jfem_mg_level_bench": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, _dp]),
     "jfem_mg_level_matrix_free": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
     "jfem_mg_apply": (C.c_int, [_vp, _vp, _vp]),
     "jfem_mg_vcycle": (C.c_int, [_vp, _vp, _vp]),

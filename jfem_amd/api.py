@@ -573,6 +573,12 @@ class MGPreconditioner:
         check(N.lib().jfem_mg_level_matrix_free(self.h, level, C.byref(v)))
         return bool(v.value)
 
+    def level_step_time_ms(self, level: int, x: DenseVector, y: DenseVector, reps: int = 20) -> float:
+        """mean kernel time (ms) of one Chebyshev step of `level` from x into y (FP64 hierarchies)"""
+        ms = C.c_double()
+        check(N.lib().jfem_mg_level_bench(self.h, level, x.h, y.h, reps, C.byref(ms)))
+        return ms.value
+
     def getFinestSystem(self) -> SparseMatrix:
         return self.getSystem(self.nlev - 1)
 

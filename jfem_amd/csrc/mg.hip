@@ -1650,6 +1650,35 @@ int jfem_mg_level_apply(jfem_mg* mg, int level, const jfem_vec* x, jfem_vec* y)
     API_END
 }
 
+int jfem_mg_level_bench(jfem_mg* mg, int level, const jfem_vec* x, jfem_vec* y, int reps, double* ms)
+{
+    API_BEGIN
+    JFEM_CHECK(mg && x && y && ms && reps > 0, JFEM_ERR_ARG, "bad argument");
+    JFEM_CHECK(level >= 0 && level < (int)mg->lev.size(), JFEM_ERR_ARG, "no such level");
+    MgLevel& L = mg->lev[level];
+    JFEM_CHECK(x->n == L.A->rows && y->n == L.A->rows, JFEM_ERR_ARG, "Incompatible sizes");
+    JFEM_CHECK(x->d != y->d || L.A->rows == 0, JFEM_ERR_ARG, "output aliases the input");
+    JFEM_CHECK(mg->opt.precision == JFEM_FP64, JFEM_ERR_UNSUPPORTED, "the step is timed on FP64 hierarchies");
+    // a non-first Chebyshev step (d = c1 d + c2 D^-1 (b - A x), y = x + d) with the level's own
+    // right-hand side and direction work vectors as b and d
+    const EpiCheby<double> e{(const double*)L.b, x->d, L.dinv, y->d, (double*)L.dv, 1.0, 0.5, 0.5, 0};
+    hipStream_t st = mg->ctx->stream;
+    hipEvent_t e0, e1;
+    JFEM_HIP(hipEventCreate(&e0));
+    JFEM_HIP(hipEventCreate(&e1));
+    level_apply<double>(mg, L, x->d, e);   // warm-up launch
+    JFEM_HIP(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; r++) level_apply<double>(mg, L, x->d, e);
+    JFEM_HIP(hipEventRecord(e1, st));
+    JFEM_HIP(hipEventSynchronize(e1));
+    float f = 0;
+    JFEM_HIP(hipEventElapsedTime(&f, e0, e1));
+    *ms = f / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    API_END
+}
+
 int jfem_mg_level_matrix_free(jfem_mg* mg, int level, int* out)
 {
     API_BEGIN
